@@ -1,0 +1,206 @@
+"""Benchmark: global assembly + Jacobi-PCG (1e-10) on one MI355X, FP64.
+
+Default workload (the configuration the metric is quoted on): P1 continuous Lagrange on the unit
+cube split into 128^3 x 6 Kuhn tetrahedra (2 146 689 DoFs), kappa_e = exp(g_e), g ~ N(0,1) from seed,
+f = 1, zero Dirichlet.  A step = MatrixOperator.assemble() (values + rhs + Dirichlet) followed by
+apply_inverse from x0 = 0 to ||r|| <= 1e-10 ||b||; inputs are resident in HBM before timing.
+
+Prints one JSON line.  Other workloads: --config c1 (Q1 128^2), c3 (Q1-DG SWIPDG 512^2).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dune-gdt_amd"))
+
+METRIC = "DoFs/s assembly+CG @1e-10 rel. residual FP64; batched/adjoint vs plain; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(hgdt, torch, config):
+    F = hgdt.fields
+    if config == "c2":
+        c = 128
+        grid = hgdt.Grid.structured("tet", [0, 0, 0], [1, 1, 1], [c, c, c])
+        kappa = torch.from_numpy(F.lognormal(1, grid.num_elements)).cuda()
+        space = hgdt.ContinuousLagrangeSpace(grid, 1)
+        op = hgdt.make_matrix_operator(space, hgdt.Stencil.element)
+        op.append(hgdt.LocalElementIntegralBilinearForm(hgdt.LocalLaplaceIntegrand(kappa)))
+        workload = "p1_kuhn_tets_128x128x128x6_lognormal_kappa_f1_dirichlet0"
+    elif config == "c1":
+        grid = hgdt.Grid.structured("quad", [0, 0], [1, 1], [128, 128])
+        kappa = 1.0
+        space = hgdt.ContinuousLagrangeSpace(grid, 1)
+        op = hgdt.make_matrix_operator(space, hgdt.Stencil.element)
+        op.append(hgdt.LocalElementIntegralBilinearForm(hgdt.LocalLaplaceIntegrand(kappa)))
+        workload = "q1_quads_128x128_kappa1_f1_dirichlet0"
+    elif config == "c3":
+        n = 512
+        grid = hgdt.Grid.structured("quad", [0, 0], [1, 1], [n, n])
+        kappa = torch.from_numpy(F.channel_field(3, n, n)).cuda()
+        space = hgdt.DiscontinuousLagrangeSpace(grid, 1)
+        op = hgdt.make_matrix_operator(space, hgdt.Stencil.element_and_intersection)
+        op += hgdt.LocalElementIntegralBilinearForm(hgdt.LocalLaplaceIntegrand(kappa))
+        op.append(hgdt.LocalCouplingIntersectionIntegralBilinearForm(
+            hgdt.LocalLaplaceIPDGInnerCoupling(1.0, kappa, kappa) + hgdt.LocalIPDGInnerPenalty(16.0, kappa)), {},
+            hgdt.ApplyOn.InnerIntersectionsOnce())
+        op.append(hgdt.LocalIntersectionIntegralBilinearForm(
+            hgdt.LocalIPDGBoundaryPenalty(16.0, kappa) + hgdt.LocalLaplaceIPDGDirichletCoupling(1.0, kappa)), {},
+            hgdt.ApplyOn.CustomBoundaryIntersections(hgdt.AllDirichletBoundaryInfo()))
+        workload = "q1dg_swipdg_quads_512x512_channels_kappa_1e-4_sigma16_f1"
+    else:
+        raise SystemExit(f"unknown config {config}")
+    rhs = hgdt.make_vector_functional(space)
+    rhs.append(hgdt.LocalElementIntegralFunctional(hgdt.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    if config != "c3":
+        op.append(hgdt.make_dirichlet_constraints(space, hgdt.AllDirichletBoundaryInfo()))
+    return grid, space, op, kappa, workload
+
+
+def cpu_baseline(config, kappa_np, iterations, sample_iters):
+    """The oracle (CPU restatement) on this host: full assembly + a bounded number of PCG iterations;
+    PCG time projected to the GPU's iteration count (same recursion and stopping rule)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if config == "c2":
+        m = O.Mesh.structured(O.TET, [128, 128, 128], [0, 0, 0], [1, 1, 1])
+        s = O.Space(m, True, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(kappa_np)
+        op.append_source(1.0)
+        op.set_dirichlet(True)
+    elif config == "c1":
+        m = O.Mesh.structured(O.QUAD, [128, 128], [0, 0], [1, 1])
+        s = O.Space(m, True, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(1.0)
+        op.append_source(1.0)
+        op.set_dirichlet(True)
+    else:
+        m = O.Mesh.structured(O.QUAD, [512, 512], [0, 0], [1, 1])
+        s = O.Space(m, False, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+        op.append_laplace(kappa_np)
+        op.append_inner_coupling(1.0, kappa_np, kappa_np)
+        op.append_inner_penalty(16.0, kappa_np)
+        op.append_dirichlet_coupling(1.0, kappa_np)
+        op.append_boundary_penalty(16.0, kappa_np)
+        op.append_source(1.0)
+    t0 = time.perf_counter()
+    op.assemble(cores)
+    t_asm = time.perf_counter() - t0
+    rp, cols = op.pattern()
+    vals, b = op.values(), op.rhs()
+    t0 = time.perf_counter()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, vals, b, rtol=1e-10, nthreads=cores, max_iters_run=sample_iters)
+    t_it = (time.perf_counter() - t0) / max(it, 1)
+    t_total = t_asm + t_it * iterations
+    return {"value": s.size / t_total, "unit": "DoFs/s", "cores": cores, "kind": "port",
+            "sample": f"oracle assembly of the full workload ({t_asm:.2f} s, {cores} threads) + {it} PCG "
+                      f"iterations ({t_it * 1e3:.2f} ms each) projected to {iterations} iterations",
+            "t_assembly_s": t_asm, "t_iteration_s": t_it}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    ap.add_argument("--graph", action="store_true", help="replay PCG iteration blocks as a hipGraph")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
+              f"(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import hgdt
+
+    torch.cuda.set_device(0)
+    t_setup = time.perf_counter()
+    grid, space, op, kappa, workload = build_problem(hgdt, torch, args.config)
+    op.assemble()
+    x = op.apply_inverse("rhs")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    log(f"setup {t_setup:.1f} s: {space.num_DoFs} DoFs, {grid.num_elements} elements, nnz {op.nnz}, "
+        f"{op.format_info()}")
+
+    def step(time_spmv):
+        op.assemble()
+        return op.apply_inverse("rhs", use_graph=args.graph and not time_spmv, time_spmv=time_spmv,
+                                check_every=16)
+
+    for w in range(args.warmup):
+        step(False)
+        log(f"warmup {w + 1}/{args.warmup}: {op.last_solve['iterations']} iterations, "
+            f"{op.last_solve['solve_ms']:.2f} ms solve")
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        x = step(time_spmv=True)
+        stats.append(dict(op.last_solve))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for k, st in enumerate(stats):
+        log(f"step {k + 1}: {st['iterations']} iterations, relres {st['relres']:.2e}, solve {st['solve_ms']:.2f} ms, "
+            f"spmv {st['spmv_avg_ms'] * 1e3:.1f} us")
+    ms_per_step = elapsed / args.steps * 1e3
+    n = space.num_DoFs
+    iters = stats[-1]["iterations"]
+    fi = op.format_info()
+    spmv_ms = float(np.mean([s["spmv_avg_ms"] for s in stats]))
+    achieved = fi["spmv_bytes"] / (spmv_ms * 1e-3) / 1e9
+    solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"pmc_spmv_{args.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": n / (ms_per_step * 1e-3),
+        "unit": "DoFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded splitmix64 / Box-Muller fields, structured grid)",
+        "config": {"workload": workload, "dofs": n, "elements": grid.num_elements, "nnz": op.nnz,
+                   "iterations": iters, "relres": stats[-1]["relres"], "format": fi["format"],
+                   "t_cg_ms": solve_ms, "t_assembly_ms": ms_per_step - solve_ms, "setup_s": t_setup},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_pcg_spmv<{fi['format']}>", "bytes_per_launch": fi["spmv_bytes"],
+                     "avg_launch_us": spmv_ms * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        kappa_np = kappa.cpu().numpy() if hasattr(kappa, "cpu") else kappa
+        cb = cpu_baseline(args.config, kappa_np, iters, args.cpu_sample_iters)
+        line["cpu_baseline"] = cb
+        line["config"]["gpu_vs_cpu"] = line["value"] / cb["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

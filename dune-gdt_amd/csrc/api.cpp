@@ -1,0 +1,673 @@
+// api.cpp -- the extern "C" boundary of libhgdt.so (declared in include/hgdt.h).
+//
+// No exception crosses the boundary: every entry point returns an hgdt_status and
+// keeps the message of the last failure in a thread-local string.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "hgdt_internal.hpp"
+
+struct hgdt_grid_s {
+  std::shared_ptr<hgdt::Grid> g;
+};
+struct hgdt_space_s {
+  std::shared_ptr<hgdt::Space> s;
+};
+struct hgdt_operator_s {
+  std::unique_ptr<hgdt::Operator> op;
+};
+
+namespace hgdt {
+
+thread_local std::string g_last_error;
+
+template <class F>
+hgdt_status guarded(F&& f)
+{
+  try {
+    f();
+    g_last_error.clear();
+    return HGDT_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return HGDT_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return HGDT_ERR_OPERATOR;
+  }
+}
+
+static void require(bool cond, hgdt_status s, const char* msg)
+{
+  if (!cond)
+    throw Error(s, msg);
+}
+
+static void require_device()
+{
+  int count = 0;
+  const hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1)
+    HGDT_THROW(HGDT_ERR_DEVICE, "no usable HIP device: libhgdt has no CPU fallback");
+}
+
+static std::shared_ptr<Grid> make_structured(int geom, const int32_t* cells, const double* lo, const double* hi)
+{
+  auto g = std::make_shared<Grid>();
+  g->ref = reference_element(geom);
+  g->structured = true;
+  const int d = g->ref.dim;
+  int64_t n[3] = {1, 1, 1}, nvx[3] = {1, 1, 1};
+  double h[3] = {0, 0, 0};
+  for (int a = 0; a < d; ++a) {
+    require(cells[a] > 0, HGDT_ERR_INVALID_ARGUMENT, "cells must be positive");
+    require(hi[a] > lo[a], HGDT_ERR_INVALID_ARGUMENT, "upper_right must exceed lower_left");
+    n[a] = cells[a], nvx[a] = cells[a] + 1, h[a] = (hi[a] - lo[a]) / cells[a];
+  }
+  g->nv = nvx[0] * nvx[1] * nvx[2];
+  const int per_cell = (geom == HGDT_TRI) ? 2 : (geom == HGDT_TET) ? 6 : 1;
+  const int64_t ncell = n[0] * n[1] * n[2];
+  g->ne = ncell * per_cell;
+  require(g->nv < (int64_t(1) << 31) && g->ne * g->ref.nverts < (int64_t(1) << 40), HGDT_ERR_NOT_SUPPORTED,
+          "grid too large for 32-bit vertex indices");
+  g->coords.resize(g->nv * d);
+  parallel_for(g->nv, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) {
+      const int64_t idx[3] = {v % nvx[0], (v / nvx[0]) % nvx[1], v / (nvx[0] * nvx[1])};
+      for (int a = 0; a < d; ++a)
+        g->coords[v * d + a] = lo[a] + idx[a] * h[a];
+    }
+  });
+  g->conn.resize(g->ne * g->ref.nverts);
+  static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  parallel_for(ncell, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; ++c) {
+      const int64_t i = c % n[0], j = (c / n[0]) % n[1], k = c / (n[0] * n[1]);
+      int32_t corner[8];
+      for (int bit = 0; bit < (1 << d); ++bit)
+        corner[bit] = int32_t(((k + ((bit >> 2) & 1)) * nvx[1] + (j + ((bit >> 1) & 1))) * nvx[0] + i + (bit & 1));
+      int32_t* out = &g->conn[c * per_cell * g->ref.nverts];
+      if (geom == HGDT_QUAD || geom == HGDT_HEX) {
+        for (int bit = 0; bit < (1 << d); ++bit)
+          out[bit] = corner[bit];
+      } else if (geom == HGDT_TRI) {
+        const int32_t t[6] = {corner[0], corner[1], corner[3], corner[0], corner[2], corner[3]};
+        std::copy(t, t + 6, out);
+      } else {
+        for (int p = 0; p < 6; ++p) {
+          const int b1 = 1 << perms[p][0], b2 = b1 | (1 << perms[p][1]);
+          out[4 * p + 0] = corner[0], out[4 * p + 1] = corner[b1], out[4 * p + 2] = corner[b2];
+          out[4 * p + 3] = corner[7];
+        }
+      }
+    }
+  });
+  g->inst_elem = {0, g->ne};
+  g->inst_vert = {0, g->nv};
+  g->finalize();
+  return g;
+}
+
+void Operator::assemble(hipStream_t stream)
+{
+  prepare_assembly();
+  build_scatter_maps();
+  if (dirichlet)
+    build_dirichlet();
+  launch_assembly(*this, stream);
+  fmt.valid = false;
+}
+
+} // namespace hgdt
+
+using namespace hgdt;
+
+extern "C" {
+
+const char* hgdt_last_error(void)
+{
+  return g_last_error.c_str();
+}
+
+int hgdt_version(void)
+{
+  return 100; // 0.1.0
+}
+
+hgdt_status hgdt_device_count(int* count)
+{
+  return guarded([&] {
+    require(count, HGDT_ERR_INVALID_ARGUMENT, "count is NULL");
+    *count = 0;
+    HGDT_HIP(hipGetDeviceCount(count));
+    require(*count > 0, HGDT_ERR_DEVICE, "no HIP device visible");
+  });
+}
+
+hgdt_status hgdt_memcpy(void* dst, const void* src, int64_t bytes, void* stream)
+{
+  return guarded([&] {
+    require(dst && src && bytes >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid copy");
+    HGDT_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDefault, static_cast<hipStream_t>(stream)));
+  });
+}
+
+hgdt_status hgdt_grid_create_structured(int geometry, const int32_t* cells, const double* lower_left,
+                                        const double* upper_right, hgdt_grid* grid)
+{
+  return guarded([&] {
+    require(cells && lower_left && upper_right && grid, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_device();
+    *grid = new hgdt_grid_s{make_structured(geometry, cells, lower_left, upper_right)};
+  });
+}
+
+hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* coords, int64_t num_elements,
+                             const int32_t* connectivity, hgdt_grid* grid)
+{
+  return guarded([&] {
+    require(coords && connectivity && grid, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require(num_vertices > 0 && num_elements > 0, HGDT_ERR_INVALID_ARGUMENT, "empty grid");
+    require(num_vertices < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED, "too many vertices");
+    require_device();
+    auto g = std::make_shared<Grid>();
+    g->ref = reference_element(geometry);
+    g->nv = num_vertices, g->ne = num_elements;
+    g->coords.assign(coords, coords + num_vertices * g->ref.dim);
+    g->conn.assign(connectivity, connectivity + num_elements * g->ref.nverts);
+    for (int32_t v : g->conn)
+      require(v >= 0 && v < num_vertices, HGDT_ERR_INVALID_ARGUMENT, "connectivity references a vertex out of range");
+    g->check_affine();
+    g->inst_elem = {0, g->ne};
+    g->inst_vert = {0, g->nv};
+    g->finalize();
+    *grid = new hgdt_grid_s{g};
+  });
+}
+
+hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_grid* grid)
+{
+  return guarded([&] {
+    require(grids && grid && count > 0, HGDT_ERR_INVALID_ARGUMENT, "need at least one grid");
+    require_device();
+    auto u = std::make_shared<Grid>();
+    u->ref = grids[0]->g->ref;
+    u->structured = false;
+    int64_t nv = 0, ne = 0;
+    for (int i = 0; i < count; ++i) {
+      require(grids[i] && grids[i]->g->ref.geom == u->ref.geom, HGDT_ERR_INVALID_ARGUMENT,
+              "all grids of a union must have the same geometry type");
+      require(grids[i]->g->ninst() == 1, HGDT_ERR_INVALID_ARGUMENT, "grids of a union must not be unions");
+      nv += grids[i]->g->nv, ne += grids[i]->g->ne;
+    }
+    require(nv < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED, "union too large for 32-bit vertex indices");
+    u->nv = nv, u->ne = ne;
+    u->coords.reserve(nv * u->ref.dim);
+    u->conn.reserve(ne * u->ref.nverts);
+    u->inst_elem = {0};
+    u->inst_vert = {0};
+    for (int i = 0; i < count; ++i) {
+      const Grid& g = *grids[i]->g;
+      const int32_t voff = int32_t(u->inst_vert.back());
+      u->coords.insert(u->coords.end(), g.coords.begin(), g.coords.end());
+      for (int32_t v : g.conn)
+        u->conn.push_back(v + voff);
+      u->inst_elem.push_back(u->inst_elem.back() + g.ne);
+      u->inst_vert.push_back(u->inst_vert.back() + g.nv);
+    }
+    u->finalize();
+    *grid = new hgdt_grid_s{u};
+  });
+}
+
+hgdt_status hgdt_grid_destroy(hgdt_grid grid)
+{
+  return guarded([&] { delete grid; });
+}
+
+hgdt_status hgdt_grid_info(hgdt_grid grid, int64_t* info)
+{
+  return guarded([&] {
+    require(grid && info, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    const Grid& g = *grid->g;
+    info[0] = g.ref.dim, info[1] = g.nv, info[2] = g.ne, info[3] = g.ref.nverts, info[4] = g.ref.nfaces;
+    info[5] = g.n_inner, info[6] = g.n_boundary, info[7] = g.ninst(), info[8] = g.ref.geom;
+  });
+}
+
+hgdt_status hgdt_grid_copy_coords(hgdt_grid grid, double* host_out)
+{
+  return guarded([&] {
+    require(grid && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::memcpy(host_out, grid->g->coords.data(), grid->g->coords.size() * sizeof(double));
+  });
+}
+
+hgdt_status hgdt_grid_copy_connectivity(hgdt_grid grid, int32_t* host_out)
+{
+  return guarded([&] {
+    require(grid && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::memcpy(host_out, grid->g->conn.data(), grid->g->conn.size() * sizeof(int32_t));
+  });
+}
+
+hgdt_status hgdt_grid_instance_elements(hgdt_grid grid, int64_t* host_out)
+{
+  return guarded([&] {
+    require(grid && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::copy(grid->g->inst_elem.begin(), grid->g->inst_elem.end(), host_out);
+  });
+}
+
+hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points)
+{
+  return guarded([&] {
+    require(grid && num_points && order >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    const Grid& g = *grid->g;
+    const Quadrature q = element_rule(g.ref, order);
+    *num_points = q.size();
+    if (!host_out)
+      return;
+    const int d = g.ref.dim, nq = q.size();
+    parallel_for(g.ne, [&](int64_t b, int64_t e) {
+      for (int64_t el = b; el < e; ++el) {
+        const int32_t* c = &g.conn[el * g.ref.nverts];
+        double x0[3], J[3][3];
+        for (int a = 0; a < d; ++a)
+          x0[a] = g.coords[int64_t(c[0]) * d + a];
+        for (int k = 0; k < d; ++k) {
+          const int lv = g.ref.simplex ? (k + 1) : (1 << k);
+          for (int a = 0; a < d; ++a)
+            J[a][k] = g.coords[int64_t(c[lv]) * d + a] - x0[a];
+        }
+        for (int p = 0; p < nq; ++p)
+          for (int a = 0; a < d; ++a) {
+            double s = x0[a];
+            for (int k = 0; k < d; ++k)
+              s += J[a][k] * q.x[p * d + k];
+            host_out[(el * nq + p) * d + a] = s;
+          }
+      }
+    });
+  });
+}
+
+hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space)
+{
+  return guarded([&] {
+    require(grid && space, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require(kind == HGDT_CONTINUOUS_LAGRANGE || kind == HGDT_DISCONTINUOUS_LAGRANGE, HGDT_ERR_INVALID_ARGUMENT,
+            "unknown space kind");
+    if (order != 1)
+      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "device spaces are provided for polynomial order 1 (P1/Q1)");
+    auto s = std::make_shared<Space>();
+    s->grid = grid->g;
+    s->kind = kind, s->order = order;
+    const Grid& g = *s->grid;
+    s->local_size = g.ref.nverts;
+    const int ls = s->local_size;
+    s->dofs.resize(g.ne * ls);
+    if (kind == HGDT_CONTINUOUS_LAGRANGE) {
+      // P1/Q1 Lagrange nodes are the element vertices in reference order: global DoF = vertex index
+      s->dofs = g.conn;
+      s->size = g.nv;
+      s->inst_dof = g.inst_vert;
+    } else {
+      // DiscontinuousMapper: element offset + local index (spaces/mapper/discontinuous.hh:104-109)
+      parallel_for(g.ne, [&](int64_t b, int64_t e) {
+        for (int64_t el = b; el < e; ++el)
+          for (int i = 0; i < ls; ++i)
+            s->dofs[el * ls + i] = int32_t(el * ls + i);
+      });
+      s->size = g.ne * ls;
+      require(s->size < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED, "too many DoFs for 32-bit indices");
+      s->inst_dof.clear();
+      for (int64_t e : g.inst_elem)
+        s->inst_dof.push_back(e * ls);
+    }
+    s->d_dofs.upload(s->dofs);
+    *space = new hgdt_space_s{s};
+  });
+}
+
+hgdt_status hgdt_space_destroy(hgdt_space space)
+{
+  return guarded([&] { delete space; });
+}
+
+hgdt_status hgdt_space_size(hgdt_space space, int64_t* size, int32_t* local_size)
+{
+  return guarded([&] {
+    require(space, HGDT_ERR_INVALID_ARGUMENT, "NULL space");
+    if (size)
+      *size = space->s->size;
+    if (local_size)
+      *local_size = space->s->local_size;
+  });
+}
+
+hgdt_status hgdt_space_copy_global_indices(hgdt_space space, int32_t* host_out)
+{
+  return guarded([&] {
+    require(space && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::memcpy(host_out, space->s->dofs.data(), space->s->dofs.size() * sizeof(int32_t));
+  });
+}
+
+hgdt_status hgdt_space_instance_dofs(hgdt_space space, int64_t* host_out)
+{
+  return guarded([&] {
+    require(space && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::copy(space->s->inst_dof.begin(), space->s->inst_dof.end(), host_out);
+  });
+}
+
+hgdt_status hgdt_operator_create(hgdt_space space, int stencil, hgdt_operator* op)
+{
+  return guarded([&] {
+    require(space && op, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require(stencil >= HGDT_STENCIL_ELEMENT && stencil <= HGDT_STENCIL_ELEMENT_AND_INTERSECTION,
+            HGDT_ERR_INVALID_ARGUMENT, "unknown stencil");
+    auto o = std::make_unique<Operator>();
+    o->space = space->s;
+    o->stencil = stencil;
+    o->build_pattern();
+    *op = new hgdt_operator_s{std::move(o)};
+  });
+}
+
+hgdt_status hgdt_operator_destroy(hgdt_operator op)
+{
+  return guarded([&] { delete op; });
+}
+
+hgdt_status hgdt_operator_shape(hgdt_operator op, int64_t* rows, int64_t* cols, int64_t* nnz)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    if (rows)
+      *rows = op->op->n;
+    if (cols)
+      *cols = op->op->n;
+    if (nnz)
+      *nnz = op->op->nnz;
+  });
+}
+
+hgdt_status hgdt_operator_copy_pattern(hgdt_operator op, int64_t* row_ptr, int32_t* cols)
+{
+  return guarded([&] {
+    require(op && row_ptr && cols, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::copy(op->op->row_ptr.begin(), op->op->row_ptr.end(), row_ptr);
+    std::copy(op->op->cols.begin(), op->op->cols.end(), cols);
+  });
+}
+
+static void forms_changed(Operator& o)
+{
+  o.tables_ready = false;
+}
+
+hgdt_status hgdt_operator_append_laplace(hgdt_operator op, const double* field, double value, int32_t over_integrate,
+                                         double scaling)
+{
+  return guarded([&] {
+    require(op && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    op->op->eforms.push_back({EF_LAPLACE, field, value, over_integrate, scaling});
+    forms_changed(*op->op);
+  });
+}
+
+hgdt_status hgdt_operator_append_product(hgdt_operator op, const double* field, double value, int32_t over_integrate,
+                                         double scaling)
+{
+  return guarded([&] {
+    require(op && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    op->op->eforms.push_back({EF_PRODUCT, field, value, over_integrate, scaling});
+    forms_changed(*op->op);
+  });
+}
+
+static void append_face(hgdt_operator op, int kind, double param, const double* kappa, double kv, const double* omega,
+                        double ov, int over_integrate, double scaling)
+{
+  require(op && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+  const Grid& g = *op->op->space->grid;
+  if (g.ref.dim != 2)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
+  FaceFormDesc F{kind, param, kappa, kv, omega, ov, over_integrate, scaling};
+  if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY) {
+    if (op->op->stencil == HGDT_STENCIL_ELEMENT)
+      HGDT_THROW(HGDT_ERR_OPERATOR, "coupling intersection forms need an intersection stencil");
+    op->op->cforms.push_back(F);
+  } else {
+    op->op->bforms.push_back(F);
+  }
+  forms_changed(*op->op);
+}
+
+hgdt_status hgdt_operator_append_inner_coupling(hgdt_operator op, double s, const double* kappa, double kv,
+                                                const double* omega, double ov, int32_t over_integrate, double scaling)
+{
+  return guarded([&] { append_face(op, FF_INNER_COUPLING, s, kappa, kv, omega, ov, over_integrate, scaling); });
+}
+
+hgdt_status hgdt_operator_append_inner_penalty(hgdt_operator op, double penalty, const double* omega, double ov,
+                                               int32_t over_integrate, double scaling)
+{
+  return guarded([&] { append_face(op, FF_INNER_PENALTY, penalty, nullptr, 1., omega, ov, over_integrate, scaling); });
+}
+
+hgdt_status hgdt_operator_append_dirichlet_coupling(hgdt_operator op, double s, const double* kappa, double kv,
+                                                    int32_t over_integrate, double scaling)
+{
+  return guarded([&] { append_face(op, FF_DIRICHLET_COUPLING, s, kappa, kv, nullptr, 1., over_integrate, scaling); });
+}
+
+hgdt_status hgdt_operator_append_boundary_penalty(hgdt_operator op, double penalty, const double* omega, double ov,
+                                                  int32_t over_integrate, double scaling)
+{
+  return guarded(
+      [&] { append_face(op, FF_BOUNDARY_PENALTY, penalty, nullptr, 1., omega, ov, over_integrate, scaling); });
+}
+
+hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double* data, double value, int32_t f_order,
+                                        int32_t over_integrate)
+{
+  return guarded([&] {
+    require(op && f_order >= 0 && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    require(kind >= HGDT_SOURCE_CONSTANT && kind <= HGDT_SOURCE_PER_QUADRATURE_POINT, HGDT_ERR_INVALID_ARGUMENT,
+            "unknown source kind");
+    require(kind == HGDT_SOURCE_CONSTANT || data, HGDT_ERR_INVALID_ARGUMENT, "source data is NULL");
+    op->op->sources.push_back({kind, data, value, f_order, over_integrate});
+    op->op->maps.built = false; // staging layout gains the right-hand side block
+    forms_changed(*op->op);
+  });
+}
+
+hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    op->op->dirichlet = enabled != 0;
+  });
+}
+
+hgdt_status hgdt_operator_dirichlet_dofs(hgdt_operator op, int64_t* count, int32_t* host_out)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    op->op->build_dirichlet();
+    if (count)
+      *count = int64_t(op->op->dirichlet_dofs.size());
+    if (host_out)
+      std::copy(op->op->dirichlet_dofs.begin(), op->op->dirichlet_dofs.end(), host_out);
+  });
+}
+
+hgdt_status hgdt_operator_assemble(hgdt_operator op, void* stream)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    op->op->assemble(static_cast<hipStream_t>(stream));
+  });
+}
+
+hgdt_status hgdt_operator_values_device(hgdt_operator op, double** values)
+{
+  return guarded([&] {
+    require(op && values, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *values = op->op->d_values.ptr;
+  });
+}
+
+hgdt_status hgdt_operator_rhs_device(hgdt_operator op, double** rhs)
+{
+  return guarded([&] {
+    require(op && rhs, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *rhs = op->op->d_rhs.ptr;
+  });
+}
+
+hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out)
+{
+  return guarded([&] {
+    require(op && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    HGDT_HIP(hipDeviceSynchronize());
+    HGDT_HIP(hipMemcpy(host_out, op->op->d_values.ptr, op->op->nnz * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out)
+{
+  return guarded([&] {
+    require(op && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    HGDT_HIP(hipDeviceSynchronize());
+    HGDT_HIP(hipMemcpy(host_out, op->op->d_rhs.ptr, op->op->n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+hgdt_status hgdt_operator_set_format(hgdt_operator op, int format)
+{
+  return guarded([&] {
+    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_DIA, HGDT_ERR_INVALID_ARGUMENT,
+            "unknown format");
+    op->op->requested_format = format;
+    op->op->fmt.structure_ready = false;
+    op->op->fmt.valid = false;
+  });
+}
+
+hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t* spmv_bytes, int32_t* num_diagonals)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    require(op->op->fmt.structure_ready, HGDT_ERR_OPERATOR, "no apply / apply_inverse since the format was chosen");
+    if (format)
+      *format = op->op->fmt.format;
+    if (spmv_bytes)
+      *spmv_bytes = op->op->fmt.bytes;
+    if (num_diagonals)
+      *num_diagonals = op->op->fmt.ndiag;
+  });
+}
+
+hgdt_status hgdt_operator_apply(hgdt_operator op, const double* x, double* y, void* stream)
+{
+  return guarded([&] {
+    require(op && x && y, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    op->op->apply(x, y, static_cast<hipStream_t>(stream));
+  });
+}
+
+hgdt_status hgdt_solver_options_default(hgdt_solver_options* o)
+{
+  return guarded([&] {
+    require(o, HGDT_ERR_INVALID_ARGUMENT, "NULL options");
+    o->rtol = 1e-10;
+    o->max_iter = 100000;
+    o->check_every = 16;
+    o->use_graph = 0;
+    o->time_spmv = 0;
+    o->zero_initial_guess = 1;
+  });
+}
+
+hgdt_status hgdt_operator_apply_inverse(hgdt_operator op, const double* b, double* x, const hgdt_solver_options* options,
+                                        hgdt_solver_stats* stats, void* stream)
+{
+  return guarded([&] {
+    require(op && b && x, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    hgdt_solver_options o;
+    if (options)
+      o = *options;
+    else
+      hgdt_solver_options_default(&o);
+    hgdt_solver_stats st{};
+    try {
+      op->op->apply_inverse(b, x, o, st, static_cast<hipStream_t>(stream));
+    } catch (...) {
+      if (stats)
+        *stats = st;
+      throw;
+    }
+    if (stats)
+      *stats = st;
+  });
+}
+
+hgdt_status hgdt_operator_instance_stats(hgdt_operator op, int64_t* iterations, double* relres)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    if (iterations)
+      std::copy(op->op->last_iters.begin(), op->op->last_iters.end(), iterations);
+    if (relres)
+      std::copy(op->op->last_relres.begin(), op->op->last_relres.end(), relres);
+  });
+}
+
+hgdt_status hgdt_operator_adjoint_log_diffusion_gradient(hgdt_operator op, int32_t form_index, const double* u,
+                                                         const double* lambda, double* grad, void* stream)
+{
+  return guarded([&] {
+    require(op && u && lambda && grad, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    op->op->adjoint_gradient(form_index, u, lambda, grad, static_cast<hipStream_t>(stream));
+  });
+}
+
+hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t block_size, int32_t items_per_thread)
+{
+  return guarded([&] {
+    require(op && kernel >= 0 && kernel < 5, HGDT_ERR_INVALID_ARGUMENT, "unknown kernel id");
+    require(block_size >= 64 && block_size % 64 == 0 && items_per_thread >= 1 && items_per_thread <= 8,
+            HGDT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64, items_per_thread in [1, 8]");
+    const int limit = (kernel <= 2) ? 256 : 1024; // assembly kernels are compiled for at most 256 lanes
+    require(block_size <= limit, HGDT_ERR_INVALID_ARGUMENT, "block_size above the kernel's launch bound");
+    Launch& L = op->op->launch[kernel];
+    if (kernel == 3 && (L.block != block_size || L.items != items_per_thread)) {
+      op->op->fmt.structure_ready = false;
+      op->op->fmt.valid = false;
+    }
+    L.block = block_size;
+    L.items = (kernel <= 2) ? 1 : items_per_thread;
+  });
+}
+
+hgdt_status hgdt_operator_get_launch(hgdt_operator op, int32_t kernel, int32_t* block_size, int32_t* items_per_thread)
+{
+  return guarded([&] {
+    require(op && kernel >= 0 && kernel < 5, HGDT_ERR_INVALID_ARGUMENT, "unknown kernel id");
+    if (block_size)
+      *block_size = op->op->launch[kernel].block;
+    if (items_per_thread)
+      *items_per_thread = op->op->launch[kernel].items;
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,846 @@
+// assembly.hip -- device assembly for libhgdt.so (gfx950, FP64).
+//
+// The reference's grid walk (XT::Grid::Walker driven by MatrixOperator::assemble,
+// dune/gdt/operators/matrix-based.hh:496-500) becomes three launches:
+//   1. k_element_local: one element per lane.  Geometry (J, J^{-1}, |det J|) and
+//      basis data live in registers; the quadrature rules are read from a small
+//      uniform table.  Computes the sum of the appended element forms
+//      (LocalElementIntegralBilinearForm::apply2, local/bilinear-forms/integrals.hh:97-134
+//      with LocalLaplaceIntegrand, local/integrands/laplace.hh:81-102, and
+//      LocalElementProductIntegrand, local/integrands/product.hh:104-130) and the
+//      right-hand side functional (local/functionals/integrals.hh:72-97), and stores
+//      them entry-major (coalesced across lanes) in the staging buffer.
+//   2. k_inner_local / k_boundary_local: one intersection per lane, SWIPDG coupling
+//      and penalty terms (local/integrands/laplace-ipdg.hh:107-186,340-368,
+//      local/integrands/ipdg.hh:113-171,254-282) through the face quadrature loops of
+//      integrals.hh:197-267,338-369.
+//   3. k_gather_values / k_gather_rhs: one matrix slot (resp. DoF) per lane sums its
+//      contributions in the fixed walk order given by the precomputed scatter map and
+//      applies the Dirichlet flag (tools/dirichlet-constraints.hh:153-184).  No atomics:
+//      results are bitwise reproducible.
+
+#include <algorithm>
+
+#include "hgdt_internal.hpp"
+
+namespace hgdt {
+namespace {
+
+constexpr int MAX_EFORMS = 8;
+constexpr int MAX_FFORMS = 8;
+constexpr int MAX_SOURCES = 4;
+
+struct QRule {
+  int offset; // into the point / weight table
+  int npts;
+};
+
+struct ElementParams {
+  int64_t ne;
+  const double* coords;
+  const int32_t* conn;
+  const double* qtab; // points (dim each) then weights, per rule
+  int nforms;
+  int kind[MAX_EFORMS];
+  const double* field[MAX_EFORMS];
+  double value[MAX_EFORMS];
+  double scaling[MAX_EFORMS];
+  QRule rule[MAX_EFORMS];
+  int nsrc;
+  int src_kind[MAX_SOURCES];
+  const double* src_data[MAX_SOURCES];
+  double src_value[MAX_SOURCES];
+  QRule src_rule[MAX_SOURCES];
+  double* staging;
+  int64_t elem_base, rhs_base;
+  int write_matrix;
+};
+
+struct FaceParams {
+  int64_t nfaces;
+  const int32_t* faces; // inner: 4 ints per face, boundary: 2 ints per face
+  const double* coords;
+  const int32_t* conn;
+  const double* qtab; // line rule points then weights
+  int nforms;
+  int kind[MAX_FFORMS];
+  double param[MAX_FFORMS];
+  const double* kappa[MAX_FFORMS];
+  double kappa_value[MAX_FFORMS];
+  const double* omega[MAX_FFORMS];
+  double omega_value[MAX_FFORMS];
+  double scaling[MAX_FFORMS];
+  QRule rule[MAX_FFORMS];
+  double* staging;
+  int64_t base;
+};
+
+template <int GEOM>
+struct Traits;
+template <>
+struct Traits<HGDT_QUAD> {
+  static constexpr int D = 2, NV = 4, ND = 4;
+  static constexpr bool SIMPLEX = false;
+  static constexpr double VOL = 1.;
+};
+template <>
+struct Traits<HGDT_TRI> {
+  static constexpr int D = 2, NV = 3, ND = 3;
+  static constexpr bool SIMPLEX = true;
+  static constexpr double VOL = 0.5;
+};
+template <>
+struct Traits<HGDT_HEX> {
+  static constexpr int D = 3, NV = 8, ND = 8;
+  static constexpr bool SIMPLEX = false;
+  static constexpr double VOL = 1.;
+};
+template <>
+struct Traits<HGDT_TET> {
+  static constexpr int D = 3, NV = 4, ND = 4;
+  static constexpr bool SIMPLEX = true;
+  static constexpr double VOL = 1. / 6.;
+};
+
+// affine element geometry in registers
+template <int D>
+struct Geo {
+  double x0[D];
+  double J[D][D];
+  double Ji[D][D];
+  double det; // |det J|
+};
+
+template <int GEOM>
+__device__ __forceinline__ void load_geometry(const double* __restrict__ coords, const int32_t* __restrict__ conn,
+                                              int64_t e, Geo<Traits<GEOM>::D>& g)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D;
+  int32_t v[D + 1];
+  v[0] = conn[e * T::NV];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    v[k + 1] = conn[e * T::NV + (T::SIMPLEX ? (k + 1) : (1 << k))];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    g.x0[a] = coords[int64_t(v[0]) * D + a];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      g.J[a][k] = coords[int64_t(v[k + 1]) * D + a] - g.x0[a];
+  bool diag = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (a != k && g.J[a][k] != 0.)
+        diag = false;
+  double det;
+  if (diag) {
+    det = 1.;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      det *= g.J[a][a];
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        g.Ji[a][k] = (a == k) ? 1. / g.J[a][a] : 0.;
+    }
+  } else if constexpr (D == 2) {
+    det = g.J[0][0] * g.J[1][1] - g.J[0][1] * g.J[1][0];
+    const double r = 1. / det;
+    g.Ji[0][0] = g.J[1][1] * r, g.Ji[0][1] = -g.J[0][1] * r;
+    g.Ji[1][0] = -g.J[1][0] * r, g.Ji[1][1] = g.J[0][0] * r;
+  } else {
+    const double(&A)[D][D] = g.J;
+    const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    const double r = 1. / det;
+    g.Ji[0][0] = c00 * r, g.Ji[1][0] = c01 * r, g.Ji[2][0] = c02 * r;
+    g.Ji[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * r;
+    g.Ji[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * r;
+    g.Ji[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * r;
+    g.Ji[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * r;
+    g.Ji[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * r;
+    g.Ji[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * r;
+  }
+  g.det = fabs(det);
+}
+
+// P1 / Q1 shape functions on the reference element: values and reference gradients
+template <int GEOM>
+__device__ __forceinline__ void shape(const double* xh, double* phi, double (*dphi)[Traits<GEOM>::D])
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D;
+  if constexpr (T::SIMPLEX) {
+    double l0 = 1.;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      l0 -= xh[a];
+    phi[0] = l0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      phi[a + 1] = xh[a];
+      dphi[0][a] = -1.;
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        dphi[c + 1][a] = (a == c) ? 1. : 0.;
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < T::NV; ++b) {
+      double v = 1.;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        v *= ((b >> a) & 1) ? xh[a] : (1. - xh[a]);
+      phi[b] = v;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double g = ((b >> a) & 1) ? 1. : -1.;
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+          if (c != a)
+            g *= ((b >> c) & 1) ? xh[c] : (1. - xh[c]);
+        dphi[b][a] = g;
+      }
+    }
+  }
+}
+
+// physical gradient g = J^{-T} gh (spaces/basis/default.hh:152-175)
+template <int D>
+__device__ __forceinline__ void to_physical(const Geo<D>& g, const double* gh, double* out)
+{
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double s = 0.;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      s += g.Ji[c][a] * gh[c];
+    out[a] = s;
+  }
+}
+
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_element_local(ElementParams P)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, ND = T::ND;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= P.ne)
+    return;
+  Geo<D> g;
+  load_geometry<GEOM>(P.coords, P.conn, e, g);
+  const double* __restrict__ qt = P.qtab;
+  if (P.write_matrix) {
+    double A[ND][ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+        A[i][j] = 0.;
+    for (int f = 0; f < P.nforms; ++f) {
+      const double c = P.field[f] ? P.field[f][e] : P.value[f];
+      const double s = P.scaling[f];
+      if (P.kind[f] == EF_LAPLACE && T::SIMPLEX) {
+        // P1 gradients are constant: the integrand is constant on the element and any
+        // rule of the order chosen by integrals.hh:115 reduces to one evaluation
+        double phi[ND], dphi[ND][D], G[ND][D];
+        double xc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          xc[a] = 1. / (D + 1);
+        shape<GEOM>(xc, phi, dphi);
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          to_physical<D>(g, dphi[i], G[i]);
+        const double factor = g.det * T::VOL;
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            double dotv = 0.;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+              dotv += (c * G[j][a]) * G[i][a];
+            A[i][j] += s * (dotv * factor);
+          }
+        continue;
+      }
+      const QRule R = P.rule[f];
+      for (int q = 0; q < R.npts; ++q) {
+        double xh[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          xh[a] = qt[R.offset * D + q * D + a];
+        const double w = qt[R.offset * D + R.npts * D + q];
+        double phi[ND], dphi[ND][D];
+        shape<GEOM>(xh, phi, dphi);
+        const double factor = g.det * w;
+        if (P.kind[f] == EF_LAPLACE) {
+          double G[ND][D];
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            to_physical<D>(g, dphi[i], G[i]);
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+              double dotv = 0.;
+#pragma unroll
+              for (int a = 0; a < D; ++a)
+                dotv += (c * G[j][a]) * G[i][a];
+              A[i][j] += s * (dotv * factor);
+            }
+        } else {
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j)
+              A[i][j] += s * (((c * phi[i]) * phi[j]) * factor);
+        }
+      }
+    }
+    double* __restrict__ out = P.staging + P.elem_base;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+        out[int64_t(i * ND + j) * P.ne + e] = A[i][j];
+  }
+  if (P.nsrc > 0) {
+    double b[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      b[i] = 0.;
+    for (int sidx = 0; sidx < P.nsrc; ++sidx) {
+      const QRule R = P.src_rule[sidx];
+      for (int q = 0; q < R.npts; ++q) {
+        double xh[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          xh[a] = qt[R.offset * D + q * D + a];
+        const double w = qt[R.offset * D + R.npts * D + q];
+        double phi[ND], dphi[ND][D];
+        shape<GEOM>(xh, phi, dphi);
+        double fv;
+        if (P.src_kind[sidx] == HGDT_SOURCE_CONSTANT)
+          fv = P.src_value[sidx];
+        else if (P.src_kind[sidx] == HGDT_SOURCE_PER_ELEMENT)
+          fv = P.src_data[sidx][e];
+        else
+          fv = P.src_data[sidx][e * R.npts + q];
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          b[i] += (phi[i] * fv) * g.det * w;
+      }
+    }
+    double* __restrict__ out = P.staging + P.rhs_base;
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      out[int64_t(i) * P.ne + e] = b[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// intersections (2d: quadrilaterals and triangles)
+// ---------------------------------------------------------------------------
+template <int GEOM>
+__device__ __forceinline__ void face_frame(const double* __restrict__ coords, const int32_t* __restrict__ conn,
+                                           int64_t e, int f, double* Pa, double* tau, double* normal, double& len)
+{
+  using T = Traits<GEOM>;
+  constexpr int fa[4][2] = {{0, 2}, {1, 3}, {0, 1}, {2, 3}};
+  constexpr int ft[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  const int la = T::SIMPLEX ? ft[f][0] : fa[f][0];
+  const int lb = T::SIMPLEX ? ft[f][1] : fa[f][1];
+  const int32_t va = conn[e * T::NV + la], vb = conn[e * T::NV + lb];
+  double Pb[2], cen[2] = {0., 0.};
+  for (int a = 0; a < 2; ++a) {
+    Pa[a] = coords[int64_t(va) * 2 + a];
+    Pb[a] = coords[int64_t(vb) * 2 + a];
+  }
+  for (int v = 0; v < T::NV; ++v) {
+    const int32_t vv = conn[e * T::NV + v];
+    cen[0] += coords[int64_t(vv) * 2 + 0] / T::NV;
+    cen[1] += coords[int64_t(vv) * 2 + 1] / T::NV;
+  }
+  tau[0] = Pb[0] - Pa[0], tau[1] = Pb[1] - Pa[1];
+  len = sqrt(tau[0] * tau[0] + tau[1] * tau[1]);
+  double nx = tau[1] / len, ny = -tau[0] / len;
+  if (nx * (Pa[0] - cen[0]) + ny * (Pa[1] - cen[1]) < 0.)
+    nx = -nx, ny = -ny;
+  normal[0] = nx + 0., normal[1] = ny + 0.;
+}
+
+template <int GEOM>
+__device__ __forceinline__ void eval_at(const Geo<2>& g, const double* X, double* phi, double (*G)[2])
+{
+  constexpr int ND = Traits<GEOM>::ND;
+  double xh[2], dx[2] = {X[0] - g.x0[0], X[1] - g.x0[1]};
+  for (int a = 0; a < 2; ++a)
+    xh[a] = g.Ji[a][0] * dx[0] + g.Ji[a][1] * dx[1];
+  double dphi[ND][2];
+  shape<GEOM>(xh, phi, dphi);
+  for (int i = 0; i < ND; ++i)
+    to_physical<2>(g, dphi[i], G[i]);
+}
+
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
+{
+  using T = Traits<GEOM>;
+  constexpr int ND = T::ND, K2 = ND * ND;
+  const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (fi >= P.nfaces)
+    return;
+  const int64_t e = P.faces[4 * fi + 0], o = P.faces[4 * fi + 2];
+  const int f = P.faces[4 * fi + 1];
+  Geo<2> gin, gout;
+  load_geometry<GEOM>(P.coords, P.conn, e, gin);
+  load_geometry<GEOM>(P.coords, P.conn, o, gout);
+  double Pa[2], tau[2], n[2], len;
+  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
+  double B[4][ND][ND];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+        B[k][i][j] = 0.;
+  for (int fm = 0; fm < P.nforms; ++fm) {
+    const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+    const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
+    const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+    const double sc = P.scaling[fm], prm = P.param[fm];
+    const QRule R = P.rule[fm];
+    for (int q = 0; q < R.npts; ++q) {
+      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
+      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
+      double pin[ND], pout[ND], Gin[ND][2], Gout[ND][2];
+      eval_at<GEOM>(gin, X, pin, Gin);
+      eval_at<GEOM>(gout, X, pout, Gout);
+      const double delta_plus = n[0] * (wout * n[0]) + n[1] * (wout * n[1]);
+      const double delta_minus = n[0] * (win * n[0]) + n[1] * (win * n[1]);
+      const double factor = len * wq;
+      if (P.kind[fm] == FF_INNER_COUPLING) {
+        const double wm = delta_plus / (delta_plus + delta_minus);
+        const double wp = delta_minus / (delta_plus + delta_minus);
+        double fin[ND], fout[ND]; // (kappa grad phi) . n
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+          fin[i] = (kin * Gin[i][0]) * n[0] + (kin * Gin[i][1]) * n[1];
+          fout[i] = (kout * Gout[i][0]) * n[0] + (kout * Gout[i][1]) * n[1];
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            const double ii = -1.0 * wm * fin[j] * pin[i] + -1.0 * prm * wm * pin[j] * fin[i];
+            const double io = -1.0 * wp * fout[j] * pin[i] + prm * wm * pout[j] * fin[i];
+            const double oi = wm * fin[j] * pout[i] + -1.0 * prm * wp * pin[j] * fout[i];
+            const double oo = wp * fout[j] * pout[i] + prm * wp * pout[j] * fout[i];
+            B[0][i][j] += sc * (ii * factor);
+            B[1][i][j] += sc * (io * factor);
+            B[2][i][j] += sc * (oi * factor);
+            B[3][i][j] += sc * (oo * factor);
+          }
+      } else {
+        const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus);
+        const double pen = (prm * weight) / len;
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            B[0][i][j] += sc * ((pen * pin[j] * pin[i]) * factor);
+            B[1][i][j] += sc * ((-1.0 * pen * pout[j] * pin[i]) * factor);
+            B[2][i][j] += sc * ((-1.0 * pen * pin[j] * pout[i]) * factor);
+            B[3][i][j] += sc * ((pen * pout[j] * pout[i]) * factor);
+          }
+      }
+    }
+  }
+  double* __restrict__ out = P.staging + P.base;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+#pragma unroll
+      for (int j = 0; j < ND; ++j)
+        out[int64_t(k * K2 + i * ND + j) * P.nfaces + fi] = B[k][i][j];
+}
+
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
+{
+  using T = Traits<GEOM>;
+  constexpr int ND = T::ND;
+  const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (fi >= P.nfaces)
+    return;
+  const int64_t e = P.faces[2 * fi + 0];
+  const int f = P.faces[2 * fi + 1];
+  Geo<2> gin;
+  load_geometry<GEOM>(P.coords, P.conn, e, gin);
+  double Pa[2], tau[2], n[2], len;
+  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
+  double B[ND][ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+      B[i][j] = 0.;
+  for (int fm = 0; fm < P.nforms; ++fm) {
+    const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+    const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double sc = P.scaling[fm], prm = P.param[fm];
+    const QRule R = P.rule[fm];
+    for (int q = 0; q < R.npts; ++q) {
+      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
+      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
+      double pin[ND], Gin[ND][2];
+      eval_at<GEOM>(gin, X, pin, Gin);
+      const double factor = len * wq;
+      if (P.kind[fm] == FF_DIRICHLET_COUPLING) {
+        double fin[ND];
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          fin[i] = (kin * Gin[i][0]) * n[0] + (kin * Gin[i][1]) * n[1];
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int j = 0; j < ND; ++j)
+            B[i][j] += sc * ((-1.0 * fin[j] * pin[i] + -1.0 * prm * pin[j] * fin[i]) * factor);
+      } else {
+        const double nwn = n[0] * (win * n[0]) + n[1] * (win * n[1]);
+        const double pen = (prm * nwn) / len;
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+#pragma unroll
+          for (int j = 0; j < ND; ++j)
+            B[i][j] += sc * ((pen * pin[j] * pin[i]) * factor);
+      }
+    }
+  }
+  double* __restrict__ out = P.staging + P.base;
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+      out[int64_t(i * ND + j) * P.nfaces + fi] = B[i][j];
+}
+
+// fixed-order gather of the staged local contributions into the CSR values
+__global__ void __launch_bounds__(256) k_gather_values(int64_t nnz, const uint32_t* __restrict__ slot_ptr,
+                                                       const uint32_t* __restrict__ contrib,
+                                                       const double* __restrict__ staging,
+                                                       const uint8_t* __restrict__ flag, double* __restrict__ values)
+{
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nnz)
+    return;
+  const uint8_t fl = flag ? flag[s] : 0;
+  double v;
+  if (fl == 1) {
+    v = 0.;
+  } else if (fl == 2) {
+    v = 1.;
+  } else {
+    v = 0.;
+    const uint32_t c1 = slot_ptr[s + 1];
+    for (uint32_t c = slot_ptr[s]; c < c1; ++c)
+      v += staging[contrib[c]];
+  }
+  values[s] = v;
+}
+
+__global__ void __launch_bounds__(256) k_gather_rhs(int64_t n, const uint32_t* __restrict__ ptr,
+                                                    const uint32_t* __restrict__ contrib,
+                                                    const double* __restrict__ staging,
+                                                    const uint8_t* __restrict__ mask, double* __restrict__ rhs)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double v = 0.;
+  if (!(mask && mask[i])) {
+    const uint32_t c1 = ptr[i + 1];
+    for (uint32_t c = ptr[i]; c < c1; ++c)
+      v += staging[contrib[c]];
+  }
+  rhs[i] = v;
+}
+
+// adjoint element kernel: g_e = -kappa_e lambda_e^T Khat_e u_e with Khat_e recomputed element-locally
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const double* __restrict__ coords,
+                                                          const int32_t* __restrict__ conn,
+                                                          const int32_t* __restrict__ dofs,
+                                                          const double* __restrict__ qtab, QRule R,
+                                                          const double* __restrict__ kappa, double kappa_value,
+                                                          const double* __restrict__ u,
+                                                          const double* __restrict__ lambda,
+                                                          double* __restrict__ grad)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, ND = T::ND;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne)
+    return;
+  Geo<D> g;
+  load_geometry<GEOM>(coords, conn, e, g);
+  double ue[ND], le[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const int32_t d = dofs[e * ND + i];
+    ue[i] = u[d], le[i] = lambda[d];
+  }
+  double acc = 0.;
+  if constexpr (T::SIMPLEX) {
+    double phi[ND], dphi[ND][D], G[ND][D], xc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      xc[a] = 1. / (D + 1);
+    shape<GEOM>(xc, phi, dphi);
+    double gu[D], gl[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      gu[a] = 0., gl[a] = 0.;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      to_physical<D>(g, dphi[i], G[i]);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        gu[a] += ue[i] * G[i][a], gl[a] += le[i] * G[i][a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      acc += gl[a] * gu[a];
+    acc *= g.det * T::VOL;
+  } else {
+    for (int q = 0; q < R.npts; ++q) {
+      double xh[D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        xh[a] = qtab[R.offset * D + q * D + a];
+      const double w = qtab[R.offset * D + R.npts * D + q];
+      double phi[ND], dphi[ND][D];
+      shape<GEOM>(xh, phi, dphi);
+      double gu[D], gl[D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        gu[a] = 0., gl[a] = 0.;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        double G[D];
+        to_physical<D>(g, dphi[i], G);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          gu[a] += ue[i] * G[a], gl[a] += le[i] * G[a];
+      }
+      double dotv = 0.;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        dotv += gl[a] * gu[a];
+      acc += dotv * (g.det * w);
+    }
+  }
+  const double k = kappa ? kappa[e] : kappa_value;
+  grad[e] = -k * acc;
+}
+
+inline unsigned grid_for(int64_t n, int block)
+{
+  return unsigned((n + block - 1) / block);
+}
+
+// quadrature table kept per operator: appended rules (element rules use dim coords per point)
+struct QTable {
+  std::vector<double> data;
+  QRule add(const Quadrature& q)
+  {
+    const int dim = q.dim;
+    // offset in units such that points start at offset*dim and weights follow
+    while (data.size() % dim)
+      data.push_back(0.);
+    QRule r;
+    r.offset = int(data.size() / dim);
+    r.npts = q.size();
+    data.insert(data.end(), q.x.begin(), q.x.end());
+    // weights start at offset*dim + npts*dim
+    data.insert(data.end(), q.w.begin(), q.w.end());
+    return r;
+  }
+};
+
+struct TableCache {
+  DeviceArray<double> elem, line;
+};
+
+template <class F>
+void dispatch_geom(int geom, F&& f)
+{
+  switch (geom) {
+    case HGDT_QUAD:
+      f(std::integral_constant<int, HGDT_QUAD>{});
+      break;
+    case HGDT_TRI:
+      f(std::integral_constant<int, HGDT_TRI>{});
+      break;
+    case HGDT_HEX:
+      f(std::integral_constant<int, HGDT_HEX>{});
+      break;
+    case HGDT_TET:
+      f(std::integral_constant<int, HGDT_TET>{});
+      break;
+    default:
+      HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "unknown geometry");
+  }
+}
+
+} // namespace
+
+// quadrature tables of the appended forms: element rules and face (line) rules
+static void build_tables(const Operator& op, QTable& etab, QTable& ltab, ElementParams& P, FaceParams& C,
+                         FaceParams& B)
+{
+  const Grid& g = *op.space->grid;
+  const int k = op.space->order;
+  P.nforms = int(op.eforms.size());
+  for (int f = 0; f < P.nforms; ++f) {
+    const auto& F = op.eforms[f];
+    P.kind[f] = F.kind, P.field[f] = F.field, P.value[f] = F.value, P.scaling[f] = F.scaling;
+    // integrand order: 0 (piecewise constant coefficient) + test + ansatz (laplace.hh:74-79, product.hh:88-100)
+    P.rule[f] = etab.add(element_rule(g.ref, 0 + k + k + F.over_integrate));
+  }
+  P.nsrc = int(op.sources.size());
+  for (int s = 0; s < P.nsrc; ++s) {
+    const auto& S = op.sources[s];
+    P.src_kind[s] = S.kind, P.src_data[s] = S.data, P.src_value[s] = S.value;
+    // product integrand with the source as ansatz: 0 + test order + f order (conversion.hh:90-93)
+    P.src_rule[s] = etab.add(element_rule(g.ref, 0 + k + S.f_order + S.over_integrate));
+  }
+  auto fill_face = [&](const std::vector<FaceFormDesc>& forms, FaceParams& F) {
+    F.nforms = int(forms.size());
+    for (int f = 0; f < F.nforms; ++f) {
+      const auto& D = forms[f];
+      F.kind[f] = D.kind, F.param[f] = D.param, F.kappa[f] = D.kappa, F.kappa_value[f] = D.kappa_value;
+      F.omega[f] = D.omega, F.omega_value[f] = D.omega_value, F.scaling[f] = D.scaling;
+      // coupling / penalty orders: 0 + 0 + test + ansatz (laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
+      Quadrature q = line_rule(0 + k + k + D.over_integrate);
+      F.rule[f] = ltab.add(q);
+    }
+  };
+  fill_face(op.cforms, C);
+  fill_face(op.bforms, B);
+}
+
+void Operator::prepare_assembly()
+{
+  if (eforms.size() > size_t(MAX_EFORMS) || sources.size() > size_t(MAX_SOURCES) || cforms.size() > size_t(MAX_FFORMS)
+      || bforms.size() > size_t(MAX_FFORMS))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many local forms appended");
+  if ((!cforms.empty() || !bforms.empty()) && space->grid->ref.dim != 2)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
+  if (tables_ready)
+    return;
+  QTable etab, ltab;
+  ElementParams P{};
+  FaceParams C{}, B{};
+  build_tables(*this, etab, ltab, P, C, B);
+  d_qtab_elem.upload(etab.data.empty() ? std::vector<double>{0.} : etab.data);
+  d_qtab_line.upload(ltab.data.empty() ? std::vector<double>{0.} : ltab.data);
+  tables_ready = true;
+}
+
+void launch_assembly(Operator& op, hipStream_t stream)
+{
+  const Grid& g = *op.space->grid;
+  const ScatterMaps& M = op.maps;
+  QTable etab, ltab;
+  ElementParams P{};
+  FaceParams C{}, B{};
+  build_tables(op, etab, ltab, P, C, B);
+  const int eb = op.launch[0].block, fb = op.launch[1].block, gb = op.launch[2].block;
+  if (!op.eforms.empty() || !op.sources.empty()) {
+    P.ne = g.ne;
+    P.coords = g.d_coords.ptr;
+    P.conn = g.d_conn.ptr;
+    P.qtab = op.d_qtab_elem.ptr;
+    P.staging = op.d_staging.ptr;
+    P.elem_base = M.elem_base;
+    P.rhs_base = M.staging_size - (op.sources.empty() ? 0 : g.ne * op.space->local_size);
+    P.write_matrix = op.eforms.empty() ? 0 : 1;
+    dispatch_geom(g.ref.geom, [&](auto G) {
+      hipLaunchKernelGGL(k_element_local<decltype(G)::value>, dim3(grid_for(g.ne, eb)), dim3(eb), 0, stream, P);
+    });
+    HGDT_HIP(hipGetLastError());
+  }
+  const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
+  if (!op.cforms.empty() && nin > 0) {
+    C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
+    C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
+    if (g.ref.geom == HGDT_QUAD)
+      hipLaunchKernelGGL(k_inner_local<HGDT_QUAD>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+    else
+      hipLaunchKernelGGL(k_inner_local<HGDT_TRI>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+    HGDT_HIP(hipGetLastError());
+  }
+  if (!op.bforms.empty() && nbd > 0) {
+    B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
+    B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
+    if (g.ref.geom == HGDT_QUAD)
+      hipLaunchKernelGGL(k_boundary_local<HGDT_QUAD>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+    else
+      hipLaunchKernelGGL(k_boundary_local<HGDT_TRI>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+    HGDT_HIP(hipGetLastError());
+  }
+  if (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty()) {
+    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(op.nnz, gb)), dim3(gb), 0, stream, op.nnz, M.d_slot_ptr.ptr,
+                       M.d_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_slot_flag.ptr : nullptr,
+                       op.d_values.ptr);
+    HGDT_HIP(hipGetLastError());
+  }
+  if (!op.sources.empty()) {
+    hipLaunchKernelGGL(k_gather_rhs, dim3(grid_for(op.n, gb)), dim3(gb), 0, stream, op.n, M.d_rhs_ptr.ptr,
+                       M.d_rhs_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_dof_mask.ptr : nullptr,
+                       op.d_rhs.ptr);
+    HGDT_HIP(hipGetLastError());
+  }
+}
+
+void Operator::adjoint_gradient(int form_index, const double* u, const double* lambda, double* grad,
+                                hipStream_t stream)
+{
+  const Grid& g = *space->grid;
+  int seen = -1;
+  const ElementFormDesc* F = nullptr;
+  for (const auto& f : eforms)
+    if (++seen == form_index)
+      F = &f;
+  if (!F)
+    HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "form_index does not name an appended element form");
+  if (F->kind != EF_LAPLACE)
+    HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "the adjoint gradient is defined for a Laplace form");
+  prepare_assembly();
+  QTable etab, ltab;
+  ElementParams P{};
+  FaceParams C{}, B{};
+  build_tables(*this, etab, ltab, P, C, B);
+  const QRule R = P.rule[form_index];
+  const int block = launch[0].block;
+  dispatch_geom(g.ref.geom, [&](auto G) {
+    hipLaunchKernelGGL(k_adjoint_gradient<decltype(G)::value>, dim3(grid_for(g.ne, block)), dim3(block), 0, stream,
+                       g.ne, g.d_coords.ptr, g.d_conn.ptr, space->d_dofs.ptr, d_qtab_elem.ptr, R, F->field, F->value,
+                       u, lambda, grad);
+  });
+  HGDT_HIP(hipGetLastError());
+}
+
+} // namespace hgdt

@@ -1,0 +1,278 @@
+// hgdt_internal.hpp -- host-side data structures of libhgdt.so.
+//
+// Host C++ owns the set-up (grids, DoF maps, sparsity pattern, fixed-order
+// scatter maps); HIP kernels own every floating-point pass over the data
+// (assembly, constraint application, operator application, solves, adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hgdt.h"
+
+namespace hgdt {
+
+// Error carrying an hgdt_status; the C ABI converts it into a return code.
+struct Error : std::runtime_error {
+  hgdt_status status;
+  Error(hgdt_status s, const std::string& msg)
+    : std::runtime_error(msg)
+    , status(s)
+  {}
+};
+
+#define HGDT_THROW(status, msg) throw ::hgdt::Error((status), (msg))
+
+#define HGDT_HIP(call)                                                                                         \
+  do {                                                                                                         \
+    hipError_t hgdt_e_ = (call);                                                                               \
+    if (hgdt_e_ != hipSuccess)                                                                                 \
+      throw ::hgdt::Error(hgdt_e_ == hipErrorOutOfMemory ? HGDT_ERR_OUT_OF_MEMORY : HGDT_ERR_DEVICE,          \
+                          std::string(#call) + ": " + hipGetErrorString(hgdt_e_));                            \
+  } while (0)
+
+// device buffer owned by the library
+template <class T>
+struct DeviceArray {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  ~DeviceArray()
+  {
+    release();
+  }
+  void release()
+  {
+    if (ptr)
+      (void)hipFree(ptr);
+    ptr = nullptr, count = 0;
+  }
+  void allocate(size_t n)
+  {
+    if (n == count && ptr)
+      return;
+    release();
+    if (n == 0)
+      return;
+    HGDT_HIP(hipMalloc(&ptr, n * sizeof(T)));
+    count = n;
+  }
+  void upload(const std::vector<T>& host)
+  {
+    allocate(host.size());
+    if (!host.empty())
+      HGDT_HIP(hipMemcpy(ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void download(T* host) const
+  {
+    if (count)
+      HGDT_HIP(hipMemcpy(host, ptr, count * sizeof(T), hipMemcpyDeviceToHost));
+  }
+};
+
+struct RefElement {
+  int geom = 0, dim = 0, nverts = 0, nfaces = 0;
+  bool simplex = false;
+  double verts[8][3] = {};
+  int face_nverts = 0;
+  int faces[6][4] = {};
+  double volume = 1.; // reference element measure
+};
+RefElement reference_element(int geom);
+
+// Quadrature rule on a reference element or on [0,1]
+struct Quadrature {
+  int dim = 0;
+  std::vector<double> x; // npts * dim
+  std::vector<double> w;
+  int size() const
+  {
+    return int(w.size());
+  }
+};
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
+Quadrature line_rule(int order);
+Quadrature element_rule(const RefElement& ref, int order);
+
+struct Grid {
+  RefElement ref;
+  bool structured = false;
+  int64_t nv = 0, ne = 0;
+  std::vector<double> coords;  // nv * dim
+  std::vector<int32_t> conn;   // ne * nverts
+  std::vector<int32_t> nb;     // ne * nfaces: neighbour element or -1 (boundary)
+  std::vector<int8_t> nbface;  // ne * nfaces: local face index in the neighbour
+  std::vector<int64_t> inst_elem{0}, inst_vert{0};
+  int64_t n_inner = 0, n_boundary = 0;
+  DeviceArray<double> d_coords;
+  DeviceArray<int32_t> d_conn;
+
+  int ninst() const
+  {
+    return int(inst_elem.size()) - 1;
+  }
+  void finalize(); // neighbours, face counts, device upload
+  void check_affine() const;
+};
+
+struct Space {
+  std::shared_ptr<Grid> grid;
+  int kind = HGDT_CONTINUOUS_LAGRANGE, order = 1;
+  int local_size = 0;
+  int64_t size = 0;
+  std::vector<int32_t> dofs;        // ne * local_size
+  std::vector<int64_t> inst_dof{0}; // DoF offsets of the instances
+  DeviceArray<int32_t> d_dofs;
+};
+
+enum ElementFormKind { EF_LAPLACE = 0, EF_PRODUCT = 1 };
+enum FaceFormKind { FF_INNER_COUPLING = 0, FF_INNER_PENALTY = 1, FF_DIRICHLET_COUPLING = 2, FF_BOUNDARY_PENALTY = 3 };
+
+struct ElementFormDesc {
+  int kind;
+  const double* field;
+  double value;
+  int over_integrate;
+  double scaling;
+};
+struct FaceFormDesc {
+  int kind;
+  double param;
+  const double* kappa;
+  double kappa_value;
+  const double* omega;
+  double omega_value;
+  int over_integrate;
+  double scaling;
+};
+struct SourceDesc {
+  int kind;
+  const double* data;
+  double value;
+  int f_order;
+  int over_integrate;
+};
+
+// launch configuration of one tunable kernel
+struct Launch {
+  int block = 256;
+  int items = 1;
+};
+
+// Fixed-order scatter maps: every matrix slot (and right-hand side entry) lists
+// its contributions, as indices into the staging buffer of local matrices, in the
+// order the reference's grid walk would add them.
+struct ScatterMaps {
+  int64_t elem_base = 0, inner_base = 0, bnd_base = 0, staging_size = 0;
+  std::vector<int32_t> inner_in, inner_fin, inner_out, inner_fout; // inner intersections visited once
+  std::vector<int32_t> bnd_e, bnd_f;                               // boundary intersections
+  DeviceArray<uint32_t> d_slot_ptr, d_contrib;
+  DeviceArray<uint32_t> d_rhs_ptr, d_rhs_contrib;
+  DeviceArray<int32_t> d_inner, d_bnd; // packed (e_in, f_in, e_out, f_out) / (e, f)
+  bool built = false;
+  int nforms_built = -1; // which form set the maps were built for (element/face/source presence)
+  int signature = -1;
+};
+
+// storage of the matrix used by apply / apply_inverse
+struct SolverFormat {
+  int format = 0;      // HGDT_FORMAT_SELL or HGDT_FORMAT_DIA
+  bool structure_ready = false; // partition + storage layout built
+  bool valid = false;  // values converted since the last assemble
+  bool diag_positive = false;
+  int64_t bytes = 0;   // algorithmic bytes of one operator application
+  // row-block partition shared by every solver kernel: blocks never straddle instances
+  int rows_per_block = 0;
+  int64_t nblocks = 0;
+  std::vector<int64_t> block_row0;  // nblocks + 1 (row start of every block, last = n)
+  std::vector<int32_t> block_inst;  // nblocks
+  std::vector<int32_t> inst_block0; // ninst + 1
+  DeviceArray<int64_t> d_block_row0;
+  DeviceArray<int32_t> d_block_inst, d_inst_block0;
+  DeviceArray<int64_t> d_inst_row0;
+  // DIA
+  int ndiag = 0;
+  DeviceArray<int64_t> d_dia_offsets; // ninst * ndiag
+  DeviceArray<double> d_dia;          // ndiag * n
+  DeviceArray<int64_t> d_dia_pos;     // nnz: CSR slot -> DIA position
+  // SELL-64 (slices of 64 rows starting at every block start)
+  int64_t nslices = 0;
+  DeviceArray<int64_t> d_slice_ptr;  // nslices + 1
+  DeviceArray<int32_t> d_slice_row0; // nslices
+  DeviceArray<int32_t> d_sell_cols;
+  DeviceArray<double> d_sell_vals;
+  DeviceArray<int64_t> d_sell_pos; // nnz: CSR slot -> SELL position
+  DeviceArray<double> d_dinv;      // Jacobi preconditioner
+};
+
+// device-side PCG state, one record per instance
+struct PcgScalars {
+  double rz, rr, bb, tol2, alpha, beta, pq;
+  int64_t iters;
+  int32_t done; // 0 running, 1 converged, 2 max_iter reached, 3 breakdown
+  int32_t ticket;
+  int32_t ticket2;
+  int32_t pad;
+};
+
+struct SolverCache; // solver.hip: events, pinned buffers, captured graph
+
+struct Operator {
+  std::shared_ptr<Space> space;
+  int stencil = HGDT_STENCIL_ELEMENT;
+  int64_t n = 0, nnz = 0;
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> cols;
+  DeviceArray<int64_t> d_row_ptr;
+  DeviceArray<int32_t> d_cols;
+  DeviceArray<double> d_values, d_rhs, d_staging;
+  std::vector<ElementFormDesc> eforms;
+  std::vector<FaceFormDesc> cforms, bforms;
+  std::vector<SourceDesc> sources;
+  bool dirichlet = false;
+  std::vector<int32_t> dirichlet_dofs;
+  DeviceArray<uint8_t> d_slot_flag; // 0 keep, 1 zero, 2 one
+  DeviceArray<uint8_t> d_dof_mask;
+  bool dirichlet_built = false;
+  ScatterMaps maps;
+  DeviceArray<double> d_qtab_elem, d_qtab_line; // quadrature tables of the appended forms
+  bool tables_ready = false;
+  int requested_format = HGDT_FORMAT_AUTO;
+  SolverFormat fmt;
+  Launch launch[5];
+  // solver workspace
+  DeviceArray<double> d_r, d_p, d_q;
+  DeviceArray<double> d_partials; // 2 per block
+  DeviceArray<PcgScalars> d_scalars;
+  std::shared_ptr<SolverCache> solver_cache;
+  std::vector<int64_t> last_iters;
+  std::vector<double> last_relres;
+
+  void build_pattern();
+  void build_scatter_maps();
+  void build_dirichlet();
+  void prepare_assembly();
+  void assemble(hipStream_t stream);
+  void prepare_solver(hipStream_t stream);
+  void apply(const double* x, double* y, hipStream_t stream);
+  void apply_inverse(const double* b, double* x, const hgdt_solver_options& opt, hgdt_solver_stats& st,
+                     hipStream_t stream);
+  void adjoint_gradient(int form_index, const double* u, const double* lambda, double* grad, hipStream_t stream);
+};
+
+// worker-thread helper for host set-up loops
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body);
+
+// kernel launchers (assembly.hip)
+void launch_assembly(Operator& op, hipStream_t stream);
+
+} // namespace hgdt

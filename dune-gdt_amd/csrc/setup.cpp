@@ -1,0 +1,578 @@
+// setup.cpp -- host-side set-up of libhgdt.so: reference elements, quadrature,
+// grids, DoF maps, sparsity patterns, fixed-order scatter maps, Dirichlet DoFs.
+//
+// Integer/topological work only; every floating-point pass over the problem
+// data runs in the HIP kernels of assembly.hip and solver.hip.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+
+#include "hgdt_internal.hpp"
+
+namespace hgdt {
+
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body)
+{
+  if (n <= 0)
+    return;
+  int nt = int(std::thread::hardware_concurrency());
+  nt = std::max(1, std::min(nt, 16));
+  if (n < 4096 || nt == 1) {
+    body(0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> errs(nt);
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int64_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e)
+      break;
+    pool.emplace_back([&, t, b, e] {
+      try {
+        body(b, e);
+      } catch (...) {
+        errs[t] = std::current_exception();
+      }
+    });
+  }
+  for (auto& th : pool)
+    th.join();
+  for (auto& ep : errs)
+    if (ep)
+      std::rethrow_exception(ep);
+}
+
+// DUNE reference elements: vertex and face numbering as in dune-geometry
+RefElement reference_element(int geom)
+{
+  RefElement r;
+  r.geom = geom;
+  auto setf = [&](std::initializer_list<std::initializer_list<int>> fl) {
+    int f = 0;
+    for (auto& face : fl) {
+      int k = 0;
+      for (int v : face)
+        r.faces[f][k++] = v;
+      r.face_nverts = k;
+      ++f;
+    }
+    r.nfaces = f;
+  };
+  switch (geom) {
+    case HGDT_QUAD:
+      r.dim = 2, r.nverts = 4;
+      for (int b = 0; b < 4; ++b)
+        r.verts[b][0] = b & 1, r.verts[b][1] = (b >> 1) & 1;
+      setf({{0, 2}, {1, 3}, {0, 1}, {2, 3}});
+      r.volume = 1.;
+      break;
+    case HGDT_TRI:
+      r.dim = 2, r.nverts = 3, r.simplex = true;
+      r.verts[1][0] = 1., r.verts[2][1] = 1.;
+      setf({{0, 1}, {0, 2}, {1, 2}});
+      r.volume = 0.5;
+      break;
+    case HGDT_HEX:
+      r.dim = 3, r.nverts = 8;
+      for (int b = 0; b < 8; ++b)
+        r.verts[b][0] = b & 1, r.verts[b][1] = (b >> 1) & 1, r.verts[b][2] = (b >> 2) & 1;
+      setf({{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}});
+      r.volume = 1.;
+      break;
+    case HGDT_TET:
+      r.dim = 3, r.nverts = 4, r.simplex = true;
+      r.verts[1][0] = 1., r.verts[2][1] = 1., r.verts[3][2] = 1.;
+      setf({{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}});
+      r.volume = 1. / 6.;
+      break;
+    default:
+      HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "unknown geometry type " + std::to_string(geom));
+  }
+  return r;
+}
+
+// Gauss-Legendre points on [0,1] (Newton on the three-term recurrence), ascending
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w)
+{
+  x.assign(n, 0.), w.assign(n, 0.);
+  auto legendre = [n](double t, double& pn, double& dpn) {
+    double p0 = 1., p1 = t;
+    if (n == 0) {
+      pn = 1., dpn = 0.;
+      return;
+    }
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2. * k - 1.) * t * p1 - (k - 1.) * p0) / k;
+      p0 = p1, p1 = p2;
+    }
+    pn = p1;
+    const double pnm1 = (n == 1) ? 1. : p0;
+    dpn = n * (t * pn - pnm1) / (t * t - 1.);
+  };
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pn, dpn;
+    for (int it = 0; it < 100; ++it) {
+      legendre(t, pn, dpn);
+      const double dt = pn / dpn;
+      t -= dt;
+      if (std::fabs(dt) < 1e-17)
+        break;
+    }
+    legendre(t, pn, dpn);
+    // t_i descending in i -> store ascending on [0,1]
+    x[n - 1 - i] = 0.5 * (1. + t);
+    w[n - 1 - i] = 1. / ((1. - t * t) * dpn * dpn);
+  }
+  for (int i = 0; i < n / 2; ++i) { // symmetrise against round-off
+    const double xs = 0.5 * ((1. - x[n - 1 - i]) + x[i]);
+    x[i] = xs, x[n - 1 - i] = 1. - xs;
+    const double ws = 0.5 * (w[i] + w[n - 1 - i]);
+    w[i] = ws, w[n - 1 - i] = ws;
+  }
+}
+
+Quadrature line_rule(int order)
+{
+  Quadrature q;
+  q.dim = 1;
+  gauss_legendre01(order / 2 + 1, q.x, q.w);
+  return q;
+}
+
+Quadrature element_rule(const RefElement& ref, int order)
+{
+  Quadrature q;
+  q.dim = ref.dim;
+  const int d = ref.dim;
+  if (!ref.simplex) {
+    std::vector<double> x, w;
+    gauss_legendre01(order / 2 + 1, x, w);
+    const int n = int(x.size());
+    const int total = (d == 2) ? n * n : n * n * n;
+    for (int t = 0; t < total; ++t) {
+      const int idx[3] = {t % n, (t / n) % n, t / (n * n)};
+      double wt = 1.;
+      for (int a = 0; a < d; ++a)
+        q.x.push_back(x[idx[a]]), wt *= w[idx[a]];
+      q.w.push_back(wt);
+    }
+    return q;
+  }
+  if (order <= 1) {
+    for (int a = 0; a < d; ++a)
+      q.x.push_back(1. / (d + 1));
+    q.w.push_back(ref.volume);
+    return q;
+  }
+  if (order == 2) {
+    if (d == 2) {
+      const double a = 1. / 6., b = 2. / 3.;
+      q.x = {a, a, b, a, a, b};
+      q.w = {1. / 6., 1. / 6., 1. / 6.};
+    } else {
+      const double a = (5. + 3. * std::sqrt(5.)) / 20., b = (5. - std::sqrt(5.)) / 20.;
+      q.x = {b, b, b, a, b, b, b, a, b, b, b, a};
+      q.w = {1. / 24., 1. / 24., 1. / 24., 1. / 24.};
+    }
+    return q;
+  }
+  // collapsed (conical product) Gauss rule, exact to the requested order
+  std::vector<double> xu, wu, xv, wv, xw, ww;
+  if (d == 2) {
+    gauss_legendre01((order + 3) / 2, xu, wu);
+    gauss_legendre01((order + 2) / 2, xv, wv);
+    for (size_t i = 0; i < xu.size(); ++i)
+      for (size_t j = 0; j < xv.size(); ++j) {
+        q.x.push_back(xu[i]), q.x.push_back((1. - xu[i]) * xv[j]);
+        q.w.push_back(wu[i] * wv[j] * (1. - xu[i]));
+      }
+  } else {
+    gauss_legendre01((order + 4) / 2, xu, wu);
+    gauss_legendre01((order + 3) / 2, xv, wv);
+    gauss_legendre01((order + 2) / 2, xw, ww);
+    for (size_t i = 0; i < xu.size(); ++i)
+      for (size_t j = 0; j < xv.size(); ++j)
+        for (size_t k = 0; k < xw.size(); ++k) {
+          const double u = xu[i], v = xv[j];
+          q.x.push_back(u), q.x.push_back((1. - u) * v), q.x.push_back((1. - u) * (1. - v) * xw[k]);
+          q.w.push_back(wu[i] * wv[j] * ww[k] * (1. - u) * (1. - u) * (1. - v));
+        }
+  }
+  return q;
+}
+
+void Grid::check_affine() const
+{
+  if (ref.simplex)
+    return;
+  const int d = ref.dim, k = ref.nverts;
+  std::atomic<bool> bad{false};
+  parallel_for(ne, [&](int64_t b, int64_t e) {
+    for (int64_t el = b; el < e && !bad; ++el) {
+      const int32_t* c = &conn[el * k];
+      for (int v = 0; v < k; ++v)
+        for (int a = 0; a < d; ++a) {
+          double expect = coords[int64_t(c[0]) * d + a];
+          for (int r = 0; r < d; ++r)
+            if ((v >> r) & 1)
+              expect += coords[int64_t(c[1 << r]) * d + a] - coords[int64_t(c[0]) * d + a];
+          const double got = coords[int64_t(c[v]) * d + a];
+          if (std::fabs(expect - got) > 1e-12 * (1. + std::fabs(got)))
+            bad = true;
+        }
+    }
+  });
+  if (bad)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "cube elements must be affine (parallelograms / parallelepipeds)");
+}
+
+// Intersections by face matching: faces are bucketed by their smallest vertex,
+// then matched inside each bucket (O(#faces) for bounded vertex valence).
+void Grid::finalize()
+{
+  const int nf = ref.nfaces, fk = ref.face_nverts, k = ref.nverts;
+  const int64_t F = ne * nf;
+  nb.assign(F, -1);
+  nbface.assign(F, -1);
+  std::vector<int64_t> start(nv + 1, 0);
+  auto face_key = [&](int64_t ef, int32_t* key) {
+    const int64_t e = ef / nf;
+    const int f = int(ef % nf);
+    for (int a = 0; a < fk; ++a)
+      key[a] = conn[e * k + ref.faces[f][a]];
+    std::sort(key, key + fk);
+  };
+  for (int64_t ef = 0; ef < F; ++ef) {
+    int32_t key[4];
+    face_key(ef, key);
+    ++start[key[0] + 1];
+  }
+  for (int64_t v = 0; v < nv; ++v)
+    start[v + 1] += start[v];
+  std::vector<int64_t> bucket(F), fill(start.begin(), start.end() - 1);
+  for (int64_t ef = 0; ef < F; ++ef) {
+    int32_t key[4];
+    face_key(ef, key);
+    bucket[fill[key[0]]++] = ef;
+  }
+  std::atomic<bool> nonconforming{false};
+  parallel_for(nv, [&](int64_t b, int64_t e) {
+    std::vector<std::array<int32_t, 4>> keys;
+    for (int64_t v = b; v < e; ++v) {
+      const int64_t s0 = start[v], s1 = start[v + 1];
+      keys.resize(s1 - s0);
+      for (int64_t i = s0; i < s1; ++i) {
+        int32_t key[4] = {0, 0, 0, 0};
+        face_key(bucket[i], key);
+        std::copy(key, key + 4, keys[i - s0].begin());
+      }
+      for (int64_t i = s0; i < s1; ++i)
+        for (int64_t j = i + 1; j < s1; ++j)
+          if (keys[i - s0] == keys[j - s0]) {
+            const int64_t a = bucket[i], c = bucket[j];
+            if (nb[a] != -1 || nb[c] != -1)
+              nonconforming = true;
+            nb[a] = int32_t(c / nf), nbface[a] = int8_t(c % nf);
+            nb[c] = int32_t(a / nf), nbface[c] = int8_t(a % nf);
+          }
+    }
+  });
+  if (nonconforming)
+    HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "grid is not conforming: a face is shared by more than two elements");
+  n_inner = 0, n_boundary = 0;
+  for (int64_t ef = 0; ef < F; ++ef) {
+    if (nb[ef] < 0)
+      ++n_boundary;
+    else if (nb[ef] > ef / nf)
+      ++n_inner;
+  }
+  d_coords.upload(coords);
+  d_conn.upload(conn);
+}
+
+// ----------------------------------------------------------------------------
+// sparsity pattern: make_element_sparsity_pattern / make_element_and_intersection_sparsity_pattern
+// (dune/gdt/tools/sparsity-pattern.hh:33-50, 117-144) computed row by row from the
+// DoF -> element adjacency, rows sorted
+// ----------------------------------------------------------------------------
+struct Adjacency {
+  std::vector<int64_t> ptr;   // n + 1
+  std::vector<int32_t> elem;  // element of each (dof, element) incidence, ascending per dof
+  std::vector<int8_t> local;  // local index of the dof in that element
+};
+
+static Adjacency dof_adjacency(const Space& s)
+{
+  const int64_t ne = s.grid->ne, n = s.size;
+  const int ls = s.local_size;
+  Adjacency A;
+  A.ptr.assign(n + 1, 0);
+  for (int64_t i = 0; i < ne * ls; ++i)
+    ++A.ptr[s.dofs[i] + 1];
+  for (int64_t i = 0; i < n; ++i)
+    A.ptr[i + 1] += A.ptr[i];
+  A.elem.resize(ne * ls), A.local.resize(ne * ls);
+  std::vector<int64_t> fill(A.ptr.begin(), A.ptr.end() - 1);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int i = 0; i < ls; ++i) {
+      const int64_t p = fill[s.dofs[e * ls + i]]++;
+      A.elem[p] = int32_t(e), A.local[p] = int8_t(i);
+    }
+  return A;
+}
+
+void Operator::build_pattern()
+{
+  const Space& s = *space;
+  const Grid& g = *s.grid;
+  const int ls = s.local_size, nf = g.ref.nfaces;
+  n = s.size;
+  const Adjacency A = dof_adjacency(s);
+  row_ptr.assign(n + 1, 0);
+  std::vector<int32_t> counts(n);
+  auto row_columns = [&](int64_t i, std::vector<int32_t>& c) {
+    c.clear();
+    for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+      const int64_t e = A.elem[p];
+      if (stencil != HGDT_STENCIL_INTERSECTION)
+        for (int j = 0; j < ls; ++j)
+          c.push_back(s.dofs[e * ls + j]);
+      if (stencil != HGDT_STENCIL_ELEMENT)
+        for (int f = 0; f < nf; ++f) {
+          const int64_t o = g.nb[e * nf + f];
+          if (o >= 0)
+            for (int j = 0; j < ls; ++j)
+              c.push_back(s.dofs[o * ls + j]);
+        }
+    }
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+  };
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> c;
+    for (int64_t i = b; i < e; ++i) {
+      row_columns(i, c);
+      counts[i] = int32_t(c.size());
+    }
+  });
+  for (int64_t i = 0; i < n; ++i)
+    row_ptr[i + 1] = row_ptr[i] + counts[i];
+  nnz = row_ptr[n];
+  cols.resize(nnz);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<int32_t> c;
+    for (int64_t i = b; i < e; ++i) {
+      row_columns(i, c);
+      std::copy(c.begin(), c.end(), cols.begin() + row_ptr[i]);
+    }
+  });
+  d_row_ptr.upload(row_ptr);
+  d_cols.upload(cols);
+  d_values.allocate(std::max<int64_t>(nnz, 1));
+  HGDT_HIP(hipMemset(d_values.ptr, 0, nnz * sizeof(double)));
+  d_rhs.allocate(std::max<int64_t>(n, 1));
+  HGDT_HIP(hipMemset(d_rhs.ptr, 0, n * sizeof(double)));
+}
+
+// ----------------------------------------------------------------------------
+// fixed-order scatter maps.  The reference walk (XT::Grid::Walker driven by
+// MatrixOperator::assemble, matrix-based.hh:496-500) visits element e, applies
+// the element assemblers (bilinear-form-assemblers.hh:110-128), then the
+// intersections of e in local order: inner ones once, from the element with the
+// smaller index (ApplyOn::InnerIntersectionsOnce), with four block scatters
+// (:262-277), boundary ones with one block (:392-395).  Every matrix slot gets
+// the list of its contributions in exactly that order.
+// ----------------------------------------------------------------------------
+void Operator::build_scatter_maps()
+{
+  const Space& s = *space;
+  const Grid& g = *s.grid;
+  const int ls = s.local_size, nf = g.ref.nfaces;
+  const int64_t ne = g.ne, k2 = int64_t(ls) * ls;
+  ScatterMaps& M = maps;
+  const bool has_e = !eforms.empty(), has_c = !cforms.empty(), has_b = !bforms.empty();
+  const int signature = (has_e ? 1 : 0) | (has_c ? 2 : 0) | (has_b ? 4 : 0) | (!sources.empty() ? 8 : 0);
+  if (M.built && M.signature == signature)
+    return;
+  // intersections in walk order
+  std::vector<int32_t> inner_index(ne * nf, -1), bnd_index(ne * nf, -1);
+  M.inner_in.clear(), M.inner_fin.clear(), M.inner_out.clear(), M.inner_fout.clear();
+  M.bnd_e.clear(), M.bnd_f.clear();
+  for (int64_t e = 0; e < ne; ++e)
+    for (int f = 0; f < nf; ++f) {
+      const int64_t o = g.nb[e * nf + f];
+      if (o > e) {
+        inner_index[e * nf + f] = int32_t(M.inner_in.size());
+        M.inner_in.push_back(int32_t(e)), M.inner_fin.push_back(f);
+        M.inner_out.push_back(int32_t(o)), M.inner_fout.push_back(g.nbface[e * nf + f]);
+      } else if (o < 0) {
+        bnd_index[e * nf + f] = int32_t(M.bnd_e.size());
+        M.bnd_e.push_back(int32_t(e)), M.bnd_f.push_back(f);
+      }
+    }
+  const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
+  M.elem_base = 0;
+  M.inner_base = has_e ? ne * k2 : 0;
+  M.bnd_base = M.inner_base + (has_c ? nin * 4 * k2 : 0);
+  const int64_t rhs_base = M.bnd_base + (has_b ? nbd * k2 : 0);
+  M.staging_size = rhs_base + (sources.empty() ? 0 : ne * ls);
+  if (M.staging_size >= (int64_t(1) << 32))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "local-matrix staging exceeds 2^32 entries");
+
+  const Adjacency A = dof_adjacency(s);
+  struct Entry {
+    int64_t key;
+    uint32_t staging;
+    int32_t col;
+  };
+  auto row_entries = [&](int64_t i, std::vector<Entry>& out) {
+    out.clear();
+    for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+      const int64_t e = A.elem[p];
+      const int li = A.local[p];
+      const int32_t* de = &s.dofs[e * ls];
+      if (has_e)
+        for (int lj = 0; lj < ls; ++lj)
+          out.push_back({e * (nf + 1), uint32_t(M.elem_base + (li * ls + lj) * ne + e), de[lj]});
+      for (int f = 0; f < nf; ++f) {
+        const int64_t o = g.nb[e * nf + f];
+        if (o >= 0 && has_c) {
+          if (e < o) {
+            const int64_t fi = inner_index[e * nf + f];
+            const int32_t* dout = &s.dofs[o * ls];
+            const int64_t key = e * (nf + 1) + 1 + f;
+            for (int lj = 0; lj < ls; ++lj) // in_in
+              out.push_back({key, uint32_t(M.inner_base + (0 * k2 + li * ls + lj) * nin + fi), de[lj]});
+            for (int lj = 0; lj < ls; ++lj) // in_out
+              out.push_back({key, uint32_t(M.inner_base + (1 * k2 + li * ls + lj) * nin + fi), dout[lj]});
+          } else {
+            const int fo = g.nbface[e * nf + f];
+            const int64_t fi = inner_index[o * nf + fo];
+            const int32_t* din = &s.dofs[o * ls];
+            const int64_t key = o * (nf + 1) + 1 + fo;
+            for (int lj = 0; lj < ls; ++lj) // out_in
+              out.push_back({key, uint32_t(M.inner_base + (2 * k2 + li * ls + lj) * nin + fi), din[lj]});
+            for (int lj = 0; lj < ls; ++lj) // out_out
+              out.push_back({key, uint32_t(M.inner_base + (3 * k2 + li * ls + lj) * nin + fi), de[lj]});
+          }
+        } else if (o < 0 && has_b) {
+          const int64_t bi = bnd_index[e * nf + f];
+          for (int lj = 0; lj < ls; ++lj)
+            out.push_back({e * (nf + 1) + 1 + f, uint32_t(M.bnd_base + (li * ls + lj) * nbd + bi), de[lj]});
+        }
+      }
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Entry& a, const Entry& b) { return a.key < b.key; });
+  };
+  // pass 1: contributions per slot
+  std::vector<uint32_t> slot_count(nnz, 0);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<Entry> ent;
+    for (int64_t i = b; i < e; ++i) {
+      row_entries(i, ent);
+      const int32_t* c0 = &cols[row_ptr[i]];
+      const int32_t* c1 = &cols[row_ptr[i + 1]];
+      for (const auto& en : ent) {
+        const int32_t* it = std::lower_bound(c0, c1, en.col);
+        if (it == c1 || *it != en.col)
+          throw Error(HGDT_ERR_OPERATOR, "local entry outside the sparsity pattern (stencil too small)");
+        ++slot_count[row_ptr[i] + (it - c0)];
+      }
+    }
+  });
+  std::vector<uint32_t> slot_ptr(nnz + 1, 0);
+  uint64_t total = 0;
+  for (int64_t t = 0; t < nnz; ++t) {
+    slot_ptr[t] = uint32_t(total);
+    total += slot_count[t];
+  }
+  if (total >= (uint64_t(1) << 32))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "number of scatter contributions exceeds 2^32");
+  slot_ptr[nnz] = uint32_t(total);
+  std::vector<uint32_t> contrib(std::max<uint64_t>(total, 1));
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    std::vector<Entry> ent;
+    std::vector<uint32_t> fillpos;
+    for (int64_t i = b; i < e; ++i) {
+      row_entries(i, ent);
+      const int64_t r0 = row_ptr[i], r1 = row_ptr[i + 1];
+      fillpos.assign(slot_ptr.begin() + r0, slot_ptr.begin() + r1);
+      const int32_t* c0 = &cols[r0];
+      const int32_t* c1 = &cols[r1];
+      for (const auto& en : ent) {
+        const int64_t t = std::lower_bound(c0, c1, en.col) - c0;
+        contrib[fillpos[t]++] = en.staging;
+      }
+    }
+  });
+  slot_count.clear();
+  slot_count.shrink_to_fit();
+  M.d_slot_ptr.upload(slot_ptr);
+  M.d_contrib.upload(contrib);
+  // right-hand side: element functional contributions per DoF in element order
+  if (!sources.empty()) {
+    std::vector<uint32_t> rptr(n + 1), rc(std::max<int64_t>(ne * ls, 1));
+    for (int64_t i = 0; i <= n; ++i)
+      rptr[i] = uint32_t(A.ptr[i]);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
+        rc[p] = uint32_t(rhs_base + int64_t(A.local[p]) * ne + A.elem[p]);
+    M.d_rhs_ptr.upload(rptr);
+    M.d_rhs_contrib.upload(rc);
+  }
+  std::vector<int32_t> inner_packed(std::max<int64_t>(4 * nin, 1)), bnd_packed(std::max<int64_t>(2 * nbd, 1));
+  for (int64_t f = 0; f < nin; ++f) {
+    inner_packed[4 * f + 0] = M.inner_in[f], inner_packed[4 * f + 1] = M.inner_fin[f];
+    inner_packed[4 * f + 2] = M.inner_out[f], inner_packed[4 * f + 3] = M.inner_fout[f];
+  }
+  for (int64_t f = 0; f < nbd; ++f)
+    bnd_packed[2 * f + 0] = M.bnd_e[f], bnd_packed[2 * f + 1] = M.bnd_f[f];
+  M.d_inner.upload(inner_packed);
+  M.d_bnd.upload(bnd_packed);
+  d_staging.allocate(std::max<int64_t>(M.staging_size, 1));
+  M.built = true;
+  M.signature = signature;
+}
+
+// DirichletConstraints::apply_local (tools/dirichlet-constraints.hh:85-110): DoFs
+// attached to boundary intersections (the face and its lower-dimensional
+// subentities); apply(matrix, vector) with ensure_symmetry (:172-177) becomes a
+// per-slot flag (keep / zero / one) applied by the scatter kernel.
+void Operator::build_dirichlet()
+{
+  if (dirichlet_built)
+    return;
+  const Space& s = *space;
+  const Grid& g = *s.grid;
+  const int ls = s.local_size, nf = g.ref.nfaces, fk = g.ref.face_nverts;
+  std::vector<uint8_t> mask(n, 0);
+  if (s.kind == HGDT_CONTINUOUS_LAGRANGE) {
+    // order 1: local DoF i sits at reference vertex i
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int f = 0; f < nf; ++f)
+        if (g.nb[e * nf + f] < 0)
+          for (int a = 0; a < fk; ++a)
+            mask[s.dofs[e * ls + g.ref.faces[f][a]]] = 1;
+  }
+  dirichlet_dofs.clear();
+  for (int64_t i = 0; i < n; ++i)
+    if (mask[i])
+      dirichlet_dofs.push_back(int32_t(i));
+  std::vector<uint8_t> flag(std::max<int64_t>(nnz, 1), 0);
+  parallel_for(n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i)
+      for (int64_t t = row_ptr[i]; t < row_ptr[i + 1]; ++t)
+        if (mask[i] || mask[cols[t]])
+          flag[t] = (cols[t] == i) ? 2 : 1;
+  });
+  d_slot_flag.upload(flag);
+  d_dof_mask.upload(mask);
+  dirichlet_built = true;
+}
+
+} // namespace hgdt

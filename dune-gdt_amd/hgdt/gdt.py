@@ -1,0 +1,706 @@
+"""Python face of libhgdt, mirroring the dune.gdt names on the assembly + solve path.
+
+Reference bindings mirrored (paths relative to the dune-gdt source tree):
+  MatrixOperator(grid, source_space, range_space, ...)   python/dune/gdt/operators/matrix-based.hh:197-351
+  op.append / op += ..., op.assemble(parallel=False)      :261-351
+  op.apply / op.apply_inverse                             python/dune/gdt/operators/interfaces.hh:55-141
+  ContinuousLagrangeSpace / DiscontinuousLagrangeSpace     python/dune/gdt/spaces/h1/continuous-lagrange.hh:57-61
+  LocalLaplaceIntegrand(diffusion)                         python/dune/gdt/local/integrands/laplace.cc:60-83
+  LocalElementIntegralBilinearForm(integrand, over_integrate=0)
+                                                          python/dune/gdt/local/bilinear-forms/element_integrals.cc:83-98
+  LocalIPDGInnerPenalty(penalty, weight)                   python/dune/gdt/local/integrands/ipdg_inner_penalty.cc:62-86
+  LocalLaplaceIPDGInnerCoupling(symmetry_prefactor, diffusion, weight=1)
+                                                          python/dune/gdt/local/integrands/laplace_ipdg_inner_coupling.cc:62-86
+  DirichletConstraints                                     python/dune/gdt/tools/dirichlet-constraints.cc:54-140
+
+Vectors are torch float64 tensors on the current HIP device; numpy arrays are copied there.
+Every computation runs in libhgdt.so on the GPU; there is no host fallback.
+"""
+import ctypes
+import enum
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+__all__ = [
+    "Grid", "make_cube_grid", "ContinuousLagrangeSpace", "DiscontinuousLagrangeSpace", "GridFunction",
+    "QuadraturePointFunction", "LocalLaplaceIntegrand", "LocalElementProductIntegrand", "LocalProductIntegrand",
+    "LocalLaplaceIPDGInnerCoupling", "LocalIPDGInnerPenalty", "LocalLaplaceIPDGDirichletCoupling",
+    "LocalIPDGBoundaryPenalty", "LocalElementIntegralBilinearForm", "LocalCouplingIntersectionIntegralBilinearForm",
+    "LocalIntersectionIntegralBilinearForm", "LocalElementIntegralFunctional", "VectorFunctional",
+    "make_vector_functional", "DirichletConstraints", "make_dirichlet_constraints", "AllDirichletBoundaryInfo",
+    "ApplyOn", "Stencil", "MatrixOperator", "make_matrix_operator", "CsrMatrix",
+]
+
+GEOMETRIES = {"quad": L.QUAD, "cube": L.QUAD, "tri": L.TRI, "simplex": L.TRI, "hex": L.HEX, "tet": L.TET}
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise L.device_error(8, "no HIP device visible to torch: libhgdt has no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_ptr(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dvec(x, n=None, name="vector"):
+    """float64 contiguous device tensor (copies numpy / host tensors)."""
+    if isinstance(x, np.ndarray):
+        x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64))
+    if not isinstance(x, torch.Tensor):
+        raise L.wrong_input_given(1, f"{name} must be a torch tensor or numpy array")
+    if x.dtype != torch.float64:
+        raise L.wrong_input_given(1, f"{name} must be float64 (FP64 path)")
+    if x.device.type != "cuda":
+        x = x.to(_device())
+    x = x.contiguous()
+    if n is not None and x.numel() != n:
+        raise L.shapes_do_not_match(2, f"{name} has {x.numel()} entries, expected {n}")
+    return x
+
+
+class Stencil(enum.IntEnum):
+    """dune/gdt/type_traits.hh:55"""
+    element = L.STENCIL_ELEMENT
+    intersection = L.STENCIL_INTERSECTION
+    element_and_intersection = L.STENCIL_ELEMENT_AND_INTERSECTION
+
+
+class ApplyOn:
+    """Element / intersection filters of XT::Grid::ApplyOn used on this path."""
+
+    class AllElements:
+        pass
+
+    class InnerIntersectionsOnce:
+        pass
+
+    class BoundaryIntersections:
+        pass
+
+    class CustomBoundaryIntersections:
+        def __init__(self, boundary_info=None, boundary_type=None):
+            self.boundary_info = boundary_info
+
+
+class AllDirichletBoundaryInfo:
+    """XT::Grid::AllDirichletBoundaryInfo: every boundary intersection is a Dirichlet one."""
+
+
+# ---------------------------------------------------------------------------
+# grids
+# ---------------------------------------------------------------------------
+class Grid:
+    """Grid (leaf view) held by libhgdt: vertex coordinates + connectivity on the device."""
+
+    def __init__(self, handle, parts=()):
+        self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle
+        self._parts = tuple(parts)
+        info = (ctypes.c_int64 * 9)()
+        L.call("hgdt_grid_info", self._h, info)
+        (self.dimension, self.num_vertices, self.num_elements, self.vertices_per_element, self.faces_per_element,
+         self.num_inner_intersections, self.num_boundary_intersections, self.num_instances,
+         self.geometry) = [int(v) for v in info]
+
+    @classmethod
+    def structured(cls, geometry, lower_left, upper_right, num_elements):
+        geometry = GEOMETRIES.get(geometry, geometry)
+        dim = 2 if geometry in (L.QUAD, L.TRI) else 3
+        ll = np.broadcast_to(np.asarray(lower_left, np.float64), (dim,)).copy()
+        ur = np.broadcast_to(np.asarray(upper_right, np.float64), (dim,)).copy()
+        ne = np.broadcast_to(np.asarray(num_elements, np.int32), (dim,)).copy()
+        h = ctypes.c_void_p()
+        L.call("hgdt_grid_create_structured", geometry, ne.ctypes.data_as(L._pi32), ll.ctypes.data_as(L._pdbl),
+               ur.ctypes.data_as(L._pdbl), ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, geometry, coords, connectivity):
+        geometry = GEOMETRIES.get(geometry, geometry)
+        c = np.ascontiguousarray(coords, np.float64)
+        k = np.ascontiguousarray(connectivity, np.int32)
+        h = ctypes.c_void_p()
+        L.call("hgdt_grid_create", geometry, c.shape[0], c.ctypes.data_as(L._pdbl), k.shape[0],
+               k.ctypes.data_as(L._pi32), ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def union(cls, grids):
+        """Disjoint union of independent grids: the batch of problem instances."""
+        arr = (ctypes.c_void_p * len(grids))(*[g._h.value for g in grids])
+        h = ctypes.c_void_p()
+        L.call("hgdt_grid_create_union", arr, len(grids), ctypes.byref(h))
+        return cls(h, parts=grids)
+
+    def coordinates(self):
+        out = np.empty((self.num_vertices, self.dimension))
+        L.call("hgdt_grid_copy_coords", self._h, out.ctypes.data_as(L._pdbl))
+        return out
+
+    def connectivity(self):
+        out = np.empty((self.num_elements, self.vertices_per_element), np.int32)
+        L.call("hgdt_grid_copy_connectivity", self._h, out.ctypes.data_as(L._pi32))
+        return out
+
+    def instance_elements(self):
+        out = np.empty(self.num_instances + 1, np.int64)
+        L.call("hgdt_grid_instance_elements", self._h, out.ctypes.data_as(L._pi64))
+        return out
+
+    def quadrature_points(self, order):
+        """Physical coordinates of the element quadrature points of the given order: (E, nq, dim)."""
+        nq = ctypes.c_int32()
+        L.call("hgdt_grid_quadrature_points", self._h, order, None, ctypes.byref(nq))
+        out = np.empty((self.num_elements, nq.value, self.dimension))
+        L.call("hgdt_grid_quadrature_points", self._h, order, out.ctypes.data_as(L._pdbl), ctypes.byref(nq))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.hgdt_grid_destroy(h)
+            self._h = None
+
+
+def make_cube_grid(lower_left, upper_right, num_elements, simplex=False, dim=2):
+    """XT::Grid::make_cube_grid: cube cells, or dim! Kuhn simplices per cube when simplex=True."""
+    n = np.atleast_1d(num_elements)
+    if n.size > 1:
+        dim = n.size
+    geometry = (L.TRI if simplex else L.QUAD) if dim == 2 else (L.TET if simplex else L.HEX)
+    return Grid.structured(geometry, lower_left, upper_right, num_elements)
+
+
+# ---------------------------------------------------------------------------
+# spaces
+# ---------------------------------------------------------------------------
+class _LagrangeSpace:
+    _kind = None
+
+    def __init__(self, grid, order=1):
+        self.grid = grid  # keep-alive, as py::keep_alive in the reference bindings
+        h = ctypes.c_void_p()
+        L.call("hgdt_space_create", grid._h, self._kind, order, ctypes.byref(h))
+        self._h = h
+        self.order = order
+        n = ctypes.c_int64()
+        ls = ctypes.c_int32()
+        L.call("hgdt_space_size", h, ctypes.byref(n), ctypes.byref(ls))
+        self.num_DoFs = int(n.value)
+        self.max_local_size = int(ls.value)
+
+    @property
+    def size(self):
+        return self.num_DoFs
+
+    def global_indices(self):
+        out = np.empty((self.grid.num_elements, self.max_local_size), np.int32)
+        L.call("hgdt_space_copy_global_indices", self._h, out.ctypes.data_as(L._pi32))
+        return out
+
+    def instance_dofs(self):
+        out = np.empty(self.grid.num_instances + 1, np.int64)
+        L.call("hgdt_space_instance_dofs", self._h, out.ctypes.data_as(L._pi64))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.hgdt_space_destroy(h)
+            self._h = None
+
+
+class ContinuousLagrangeSpace(_LagrangeSpace):
+    """dune/gdt/spaces/h1/continuous-lagrange.hh:49-183 (order 1 on the device)."""
+    _kind = L.CONTINUOUS_LAGRANGE
+
+
+class DiscontinuousLagrangeSpace(_LagrangeSpace):
+    """dune/gdt/spaces/l2/discontinuous-lagrange.hh:60-183 (order 1 on the device)."""
+    _kind = L.DISCONTINUOUS_LAGRANGE
+
+
+# ---------------------------------------------------------------------------
+# data functions
+# ---------------------------------------------------------------------------
+class GridFunction:
+    """Element-wise constant scalar function (times the identity where a matrix is expected), or a constant."""
+
+    def __init__(self, value):
+        if isinstance(value, GridFunction):
+            self.constant, self.field = value.constant, value.field
+            return
+        if isinstance(value, (int, float)):
+            self.constant, self.field = float(value), None
+        else:
+            self.constant, self.field = 0.0, _dvec(value, name="element field")
+
+    def args(self, grid=None):
+        if self.field is None:
+            return None, self.constant
+        if grid is not None and self.field.numel() != grid.num_elements:
+            raise L.shapes_do_not_match(2, f"field has {self.field.numel()} values, grid has "
+                                           f"{grid.num_elements} elements")
+        return ctypes.c_void_p(self.field.data_ptr()), 0.0
+
+    def order(self):
+        return 0
+
+
+class QuadraturePointFunction:
+    """Values at the element quadrature points of `order` (layout of Grid.quadrature_points)."""
+
+    def __init__(self, grid, values_or_callable, order):
+        self.grid, self.order_total = grid, int(order)
+        if callable(values_or_callable):
+            pts = grid.quadrature_points(order)
+            vals = np.asarray(values_or_callable(pts.reshape(-1, grid.dimension).T), np.float64)
+            self.values = _dvec(vals.reshape(-1))
+        else:
+            self.values = _dvec(values_or_callable)
+
+
+def _gf(v):
+    return v if isinstance(v, GridFunction) else GridFunction(v)
+
+
+# ---------------------------------------------------------------------------
+# integrands (dune/gdt/local/integrands/*.hh)
+# ---------------------------------------------------------------------------
+class _Integrand:
+    kinds = ()
+
+    def __add__(self, other):
+        return _IntegrandSum(self, other)
+
+
+class _IntegrandSum(_Integrand):
+    """LocalQuaternaryIntersectionIntegrandSum / binary sums (local/integrands/combined.hh:168-440)."""
+
+    def __init__(self, a, b):
+        self.parts = (a.parts if isinstance(a, _IntegrandSum) else (a,)) + (
+            b.parts if isinstance(b, _IntegrandSum) else (b,))
+
+
+class LocalLaplaceIntegrand(_Integrand):
+    """(diffusion grad phi_j) . grad psi_i, local/integrands/laplace.hh:27-109"""
+
+    def __init__(self, diffusion=1.0, logging_prefix=""):
+        self.diffusion = _gf(diffusion)
+
+
+class LocalElementProductIntegrand(_Integrand):
+    """(weight psi_i) phi_j, local/integrands/product.hh:43-137"""
+
+    def __init__(self, weight=1.0, logging_prefix=""):
+        self.weight = _gf(weight)
+
+    def with_ansatz(self, function):
+        """LocalBinaryToUnaryElementIntegrand (local/integrands/conversion.hh:42-124)"""
+        return _ProductWithAnsatz(self, function)
+
+
+LocalProductIntegrand = LocalElementProductIntegrand
+
+
+class _ProductWithAnsatz(_Integrand):
+    def __init__(self, product, function):
+        if product.weight.field is not None or product.weight.constant != 1.0:
+            raise L.not_supported(9, "right-hand side functionals use the unit product weight")
+        self.function = function
+
+
+class LocalLaplaceIPDGInnerCoupling(_Integrand):
+    """LocalLaplaceIPDGIntegrands::InnerCoupling, local/integrands/laplace-ipdg.hh:32-199"""
+
+    def __init__(self, symmetry_prefactor, diffusion, weight=1.0, logging_prefix=""):
+        self.symmetry_prefactor = float(symmetry_prefactor)
+        self.diffusion, self.weight = _gf(diffusion), _gf(weight)
+
+
+class LocalIPDGInnerPenalty(_Integrand):
+    """LocalIPDGIntegrands::InnerPenalty, local/integrands/ipdg.hh:44-183 (face diameter as h)"""
+
+    def __init__(self, penalty, weight=1.0, logging_prefix=""):
+        self.penalty, self.weight = float(penalty), _gf(weight)
+
+
+class LocalLaplaceIPDGDirichletCoupling(_Integrand):
+    """LocalLaplaceIPDGIntegrands::DirichletCoupling, local/integrands/laplace-ipdg.hh:213-382"""
+
+    def __init__(self, symmetry_prefactor, diffusion, logging_prefix=""):
+        self.symmetry_prefactor, self.diffusion = float(symmetry_prefactor), _gf(diffusion)
+
+
+class LocalIPDGBoundaryPenalty(_Integrand):
+    """LocalIPDGIntegrands::BoundaryPenalty, local/integrands/ipdg.hh:186-291"""
+
+    def __init__(self, penalty, weight=1.0, logging_prefix=""):
+        self.penalty, self.weight = float(penalty), _gf(weight)
+
+
+# ---------------------------------------------------------------------------
+# local forms (dune/gdt/local/bilinear-forms/integrals.hh, local/functionals/integrals.hh)
+# ---------------------------------------------------------------------------
+class LocalElementIntegralBilinearForm:
+    def __init__(self, integrand, over_integrate=0, logging_prefix=""):
+        self.integrand, self.over_integrate = integrand, int(over_integrate)
+
+
+class LocalCouplingIntersectionIntegralBilinearForm:
+    def __init__(self, integrand, over_integrate=0, logging_prefix=""):
+        self.integrand, self.over_integrate = integrand, int(over_integrate)
+
+
+class LocalIntersectionIntegralBilinearForm:
+    def __init__(self, integrand, over_integrate=0, logging_prefix=""):
+        self.integrand, self.over_integrate = integrand, int(over_integrate)
+
+
+class LocalElementIntegralFunctional:
+    def __init__(self, integrand, over_integrate=0, logging_prefix=""):
+        if not isinstance(integrand, _ProductWithAnsatz):
+            raise L.not_supported(9, "element functionals take LocalProductIntegrand().with_ansatz(f)")
+        self.integrand, self.over_integrate = integrand, int(over_integrate)
+
+
+class VectorFunctional:
+    """VectorBasedFunctional (functionals/vector-based.hh:132-286), assembled in the operator's walk."""
+
+    def __init__(self, space):
+        self.space = space
+        self.local_functionals = []
+        self._op = None
+
+    def append(self, local_functional):
+        self.local_functionals.append(local_functional)
+        return self
+
+    __iadd__ = append
+
+    @property
+    def vector(self):
+        if self._op is None:
+            raise L.operator_error(3, "functional not assembled: append it to a MatrixOperator and assemble")
+        return self._op.rhs
+
+
+def make_vector_functional(space):
+    return VectorFunctional(space)
+
+
+class DirichletConstraints:
+    """DirichletConstraints (tools/dirichlet-constraints.hh:47-211), applied inside the operator's assembly."""
+
+    def __init__(self, space, boundary_info=None):
+        self.space = space
+        self._op = None
+
+    @property
+    def dirichlet_DoFs(self):
+        if self._op is None:
+            raise L.operator_error(3, "constraints not attached: append them to a MatrixOperator")
+        return self._op.dirichlet_dofs()
+
+    def apply(self, *args, **kwargs):
+        """The unit-row/column update (ensure_symmetry) and b_i = 0 are fused into the assembly."""
+        if self._op is None or not self._op._assembled:
+            raise L.operator_error(3, "constraints are applied during MatrixOperator.assemble()")
+
+
+def make_dirichlet_constraints(space, boundary_info=None):
+    return DirichletConstraints(space, boundary_info)
+
+
+class CsrMatrix:
+    """Assembled matrix: sorted CSR pattern (host) + values (device tensor)."""
+
+    def __init__(self, row_ptr, cols, values, shape):
+        self.row_ptr, self.cols, self.values, self.shape = row_ptr, cols, values, shape
+
+    @property
+    def nnz(self):
+        return int(self.values.numel())
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        return sp.csr_matrix((self.values.cpu().numpy(), self.cols, self.row_ptr), shape=self.shape)
+
+    def rows(self):
+        return self.shape[0]
+
+    def cols_count(self):
+        return self.shape[1]
+
+
+# ---------------------------------------------------------------------------
+# MatrixOperator
+# ---------------------------------------------------------------------------
+class MatrixOperator:
+    """MatrixOperator (dune/gdt/operators/matrix-based.hh:245-508) on the GPU."""
+
+    def __init__(self, grid, source_space, range_space=None, stencil=Stencil.element_and_intersection,
+                 logging_prefix=""):
+        range_space = source_space if range_space is None else range_space
+        if range_space is not source_space:
+            raise L.not_supported(9, "test and ansatz space must be the same space")
+        self.grid, self.space = grid, source_space
+        h = ctypes.c_void_p()
+        L.call("hgdt_operator_create", source_space._h, int(stencil), ctypes.byref(h))
+        self._h = h
+        r, c, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.call("hgdt_operator_shape", h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(nnz))
+        self.shape = (int(r.value), int(c.value))
+        self.nnz = int(nnz.value)
+        self._keep = []  # device fields referenced by the appended forms
+        self._eforms = 0
+        self._assembled = False
+        self.last_solve = None
+        self._pattern = None
+
+    # -- append / += (matrix-based.hh:346-408, python bindings :261-346) --
+    def append(self, what, param=None, filter=None):
+        if isinstance(what, LocalElementIntegralBilinearForm):
+            for part in _parts(what.integrand):
+                self._append_element(part, what.over_integrate, _scaling(param))
+        elif isinstance(what, LocalCouplingIntersectionIntegralBilinearForm):
+            if filter is not None and not isinstance(filter, ApplyOn.InnerIntersectionsOnce):
+                raise L.not_supported(9, "coupling forms are applied with ApplyOn.InnerIntersectionsOnce")
+            for part in _parts(what.integrand):
+                self._append_coupling(part, what.over_integrate, _scaling(param))
+        elif isinstance(what, LocalIntersectionIntegralBilinearForm):
+            for part in _parts(what.integrand):
+                self._append_boundary(part, what.over_integrate, _scaling(param))
+        elif isinstance(what, VectorFunctional):
+            if what.space is not self.space:
+                raise L.not_supported(9, "the functional must live on the operator's space")
+            for lf in what.local_functionals:
+                self._append_source(lf)
+            what._op = self
+        elif isinstance(what, DirichletConstraints):
+            L.call("hgdt_operator_set_dirichlet", self._h, 1)
+            what._op = self
+        elif isinstance(what, tuple):
+            return self.append(*what)
+        else:
+            raise L.wrong_input_given(1, f"cannot append {type(what).__name__}")
+        self._assembled = False
+        return self
+
+    def __iadd__(self, what):
+        return self.append(what)
+
+    def _field(self, gf):
+        ptr, val = gf.args(self.grid)
+        if gf.field is not None:
+            self._keep.append(gf.field)
+        return ptr, val
+
+    def _append_element(self, integrand, over, scaling):
+        if isinstance(integrand, LocalLaplaceIntegrand):
+            p, v = self._field(integrand.diffusion)
+            L.call("hgdt_operator_append_laplace", self._h, p, v, over, scaling)
+        elif isinstance(integrand, LocalElementProductIntegrand):
+            p, v = self._field(integrand.weight)
+            L.call("hgdt_operator_append_product", self._h, p, v, over, scaling)
+        else:
+            raise L.not_supported(9, f"{type(integrand).__name__} is not an element integrand of this path")
+        self._eforms += 1
+
+    def _append_coupling(self, integrand, over, scaling):
+        if isinstance(integrand, LocalLaplaceIPDGInnerCoupling):
+            kp, kv = self._field(integrand.diffusion)
+            wp, wv = self._field(integrand.weight)
+            L.call("hgdt_operator_append_inner_coupling", self._h, integrand.symmetry_prefactor, kp, kv, wp, wv,
+                   over, scaling)
+        elif isinstance(integrand, LocalIPDGInnerPenalty):
+            wp, wv = self._field(integrand.weight)
+            L.call("hgdt_operator_append_inner_penalty", self._h, integrand.penalty, wp, wv, over, scaling)
+        else:
+            raise L.not_supported(9, f"{type(integrand).__name__} is not a coupling integrand of this path")
+
+    def _append_boundary(self, integrand, over, scaling):
+        if isinstance(integrand, LocalLaplaceIPDGDirichletCoupling):
+            kp, kv = self._field(integrand.diffusion)
+            L.call("hgdt_operator_append_dirichlet_coupling", self._h, integrand.symmetry_prefactor, kp, kv, over,
+                   scaling)
+        elif isinstance(integrand, LocalIPDGBoundaryPenalty):
+            wp, wv = self._field(integrand.weight)
+            L.call("hgdt_operator_append_boundary_penalty", self._h, integrand.penalty, wp, wv, over, scaling)
+        else:
+            raise L.not_supported(9, f"{type(integrand).__name__} is not a boundary integrand of this path")
+
+    def _append_source(self, lf):
+        f = lf.integrand.function
+        over = lf.over_integrate
+        if isinstance(f, QuadraturePointFunction):
+            f_order = f.order_total - self.space.order - over
+            if f_order < 0:
+                raise L.wrong_input_given(1, "quadrature order lower than the functional's own order")
+            expected = self.grid.num_elements * self.grid.quadrature_points(f.order_total).shape[1]
+            if f.values.numel() != expected:
+                raise L.shapes_do_not_match(2, "quadrature-point values do not match the grid's rule")
+            self._keep.append(f.values)
+            L.call("hgdt_operator_append_source", self._h, L.SOURCE_PER_QUADRATURE_POINT,
+                   ctypes.c_void_p(f.values.data_ptr()), 0.0, f_order, over)
+            return
+        gf = _gf(f)
+        p, v = self._field(gf)
+        kind = L.SOURCE_CONSTANT if p is None else L.SOURCE_PER_ELEMENT
+        L.call("hgdt_operator_append_source", self._h, kind, p, v, 0, over)
+
+    # -- assemble (matrix-based.hh:496-500) --
+    def assemble(self, parallel=False, stream=None):
+        L.call("hgdt_operator_assemble", self._h, _stream_ptr(stream))
+        self._assembled = True
+        return self
+
+    def pattern(self):
+        if self._pattern is None:
+            rp = np.empty(self.shape[0] + 1, np.int64)
+            cols = np.empty(self.nnz, np.int32)
+            L.call("hgdt_operator_copy_pattern", self._h, rp.ctypes.data_as(L._pi64), cols.ctypes.data_as(L._pi32))
+            self._pattern = (rp, cols)
+        return self._pattern
+
+    def values_device_ptr(self):
+        p = ctypes.c_void_p()
+        L.call("hgdt_operator_values_device", self._h, ctypes.byref(p))
+        return p
+
+    @property
+    def matrix(self):
+        rp, cols = self.pattern()
+        vals = torch.empty(self.nnz, dtype=torch.float64, device=_device())
+        L.call("hgdt_memcpy", ctypes.c_void_p(vals.data_ptr()), self.values_device_ptr(), 8 * self.nnz,
+               _stream_ptr())
+        return CsrMatrix(rp, cols, vals, self.shape)
+
+    @property
+    def rhs(self):
+        p = ctypes.c_void_p()
+        L.call("hgdt_operator_rhs_device", self._h, ctypes.byref(p))
+        out = torch.empty(self.shape[0], dtype=torch.float64, device=_device())
+        L.call("hgdt_memcpy", ctypes.c_void_p(out.data_ptr()), p, 8 * self.shape[0], _stream_ptr())
+        return out
+
+    def dirichlet_dofs(self):
+        cnt = ctypes.c_int64()
+        L.call("hgdt_operator_dirichlet_dofs", self._h, ctypes.byref(cnt), None)
+        out = np.empty(cnt.value, np.int32)
+        L.call("hgdt_operator_dirichlet_dofs", self._h, ctypes.byref(cnt), out.ctypes.data_as(L._pi32))
+        return out
+
+    # -- operator interface (operators/interfaces.hh, matrix-based.hh:121-159) --
+    def apply(self, source, range=None, stream=None):
+        x = _dvec(source, self.shape[1], "source")
+        y = torch.empty(self.shape[0], dtype=torch.float64, device=x.device) if range is None else range
+        L.call("hgdt_operator_apply", self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+               _stream_ptr(stream))
+        return y
+
+    def invert_options(self, type=None):
+        if type is None:
+            return ["cg.jacobi"]
+        if type != "cg.jacobi":
+            raise L.operator_error(3, f"unknown inverse type {type!r}")
+        return {"type": "cg.jacobi", "precision": 1e-10, "max_iter": 100000}
+
+    def apply_inverse(self, range, source=None, opts=None, stream=None, check_every=16, use_graph=False,
+                      time_spmv=False):
+        """Jacobi-PCG from x0 = 0 (or from `source` when opts['initial_guess'] is true).
+
+        `range` may be the string "rhs" to solve with the operator's own assembled right-hand side
+        (no copy)."""
+        if isinstance(range, str) and range == "rhs":
+            p = ctypes.c_void_p()
+            L.call("hgdt_operator_rhs_device", self._h, ctypes.byref(p))
+            b_ptr = p
+            b_like = torch.empty(self.shape[0], dtype=torch.float64, device=_device())
+            return self._apply_inverse(b_ptr, b_like if source is None else source, source is not None, opts,
+                                       stream, check_every, use_graph, time_spmv)
+        b = _dvec(range, self.shape[0], "range")
+        return self._apply_inverse(ctypes.c_void_p(b.data_ptr()), torch.empty_like(b) if source is None else source,
+                                   source is not None, opts, stream, check_every, use_graph, time_spmv, keep=b)
+
+    def _apply_inverse(self, b_ptr, x, has_source, opts, stream, check_every, use_graph, time_spmv, keep=None):
+        opts = dict(self.invert_options("cg.jacobi"), **(opts or {}))
+        if opts.get("type", "cg.jacobi") != "cg.jacobi":
+            raise L.operator_error(3, f"unknown inverse type {opts['type']!r}")
+        x = _dvec(x, self.shape[1], "source")
+        o = L.SolverOptions()
+        L.call("hgdt_solver_options_default", ctypes.byref(o))
+        o.rtol = float(opts.get("precision", 1e-10))
+        o.max_iter = int(opts.get("max_iter", 100000))
+        o.check_every = int(check_every)
+        o.use_graph = int(bool(use_graph))
+        o.time_spmv = int(bool(time_spmv))
+        o.zero_initial_guess = 0 if (has_source and opts.get("initial_guess", False)) else 1
+        st = L.SolverStats()
+        status = L.load().hgdt_operator_apply_inverse(self._h, b_ptr, ctypes.c_void_p(x.data_ptr()), ctypes.byref(o),
+                                                      ctypes.byref(st), _stream_ptr(stream))
+        self.last_solve = st.as_dict()
+        L.check(status)
+        return x
+
+    def instance_stats(self):
+        n = self.grid.num_instances
+        it = np.empty(n, np.int64)
+        rr = np.empty(n)
+        L.call("hgdt_operator_instance_stats", self._h, it.ctypes.data_as(L._pi64), rr.ctypes.data_as(L._pdbl))
+        return it, rr
+
+    def adjoint_log_diffusion_gradient(self, u, lam, form_index=0, stream=None):
+        """grad_e = -kappa_e lambda_e^T Khat_e u_e (no reference counterpart)."""
+        u = _dvec(u, self.shape[0], "u")
+        lam = _dvec(lam, self.shape[0], "lambda")
+        g = torch.empty(self.grid.num_elements, dtype=torch.float64, device=u.device)
+        L.call("hgdt_operator_adjoint_log_diffusion_gradient", self._h, form_index, ctypes.c_void_p(u.data_ptr()),
+               ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(g.data_ptr()), _stream_ptr(stream))
+        return g
+
+    # -- storage / launch configuration --
+    def set_format(self, fmt):
+        L.call("hgdt_operator_set_format", self._h, int(fmt))
+
+    def format_info(self):
+        f, b, d = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        L.call("hgdt_operator_format_info", self._h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(d))
+        return {"format": {L.FORMAT_SELL: "sell64", L.FORMAT_DIA: "dia"}[f.value], "spmv_bytes": int(b.value),
+                "num_diagonals": int(d.value)}
+
+    def set_launch(self, kernel, block_size, items_per_thread=1):
+        L.call("hgdt_operator_set_launch", self._h, int(kernel), int(block_size), int(items_per_thread))
+
+    def get_launch(self, kernel):
+        b, i = ctypes.c_int32(), ctypes.c_int32()
+        L.call("hgdt_operator_get_launch", self._h, int(kernel), ctypes.byref(b), ctypes.byref(i))
+        return b.value, i.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.hgdt_operator_destroy(h)
+            self._h = None
+
+
+def _parts(integrand):
+    return integrand.parts if isinstance(integrand, _IntegrandSum) else (integrand,)
+
+
+def _scaling(param):
+    if not param:
+        return 1.0
+    v = param.get("matrixoperator.scaling", 1.0)
+    return float(v[0] if isinstance(v, (list, tuple)) else v)
+
+
+def make_matrix_operator(space, stencil=Stencil.element_and_intersection, logging_prefix=""):
+    """make_matrix_operator<M>(space, stencil), matrix-based.hh:650-658"""
+    return MatrixOperator(space.grid, space, space, stencil=stencil)

@@ -1,0 +1,253 @@
+/*
+ * hgdt.h -- C ABI of the MI355X-native assembly + solve library (libhgdt.so).
+ *
+ * This is the drop-in boundary for dune-gdt's global assembly and
+ * operator-application path.  Each entry point names the reference interface it
+ * replaces (paths relative to the dune-gdt source tree).  No C++ or torch types
+ * cross this boundary: handles are opaque pointers, arrays are plain pointers
+ * plus sizes, device arrays are HIP device pointers, streams are hipStream_t
+ * passed as void* (NULL = the default stream).
+ *
+ * Conventions
+ *   - every function returns an hgdt_status; 0 is success.  The message of the
+ *     last failure on the calling thread is returned by hgdt_last_error().
+ *     Status codes map the reference's exception families
+ *     (dune/gdt/exceptions.hh:24-77).
+ *   - all floating-point data is double (FP64).
+ *   - geometry: HGDT_QUAD / HGDT_TRI (2d), HGDT_HEX / HGDT_TET (3d).  Reference
+ *     cube vertex b has coordinates given by the bits of b (x = bit 0, y = bit 1,
+ *     z = bit 2); reference simplex vertex 0 is the origin and vertex k is e_k.
+ *   - a handle must not be used from two host threads at the same time.
+ *   - results are deterministic: no floating-point atomics anywhere; every sum
+ *     is taken in a fixed order.
+ */
+#ifndef HGDT_H
+#define HGDT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hgdt_status {
+  HGDT_OK = 0,
+  HGDT_ERR_INVALID_ARGUMENT = 1,   /* XT::Common::Exceptions::wrong_input_given */
+  HGDT_ERR_SHAPES_DO_NOT_MATCH = 2, /* XT::Common::Exceptions::shapes_do_not_match */
+  HGDT_ERR_OPERATOR = 3,           /* Exceptions::operator_error */
+  HGDT_ERR_INTEGRAND = 4,          /* Exceptions::integrand_error */
+  HGDT_ERR_SPACE = 5,              /* Exceptions::space_error */
+  HGDT_ERR_MAPPER = 6,             /* Exceptions::mapper_error */
+  HGDT_ERR_SOLVER_FAILED = 7,      /* XT::LA::Exceptions::linear_solver_failed */
+  HGDT_ERR_DEVICE = 8,             /* HIP runtime failure */
+  HGDT_ERR_NOT_SUPPORTED = 9,      /* combination not provided by this library */
+  HGDT_ERR_OUT_OF_MEMORY = 10
+} hgdt_status;
+
+enum { HGDT_QUAD = 0, HGDT_TRI = 1, HGDT_HEX = 2, HGDT_TET = 3 };
+/* space kinds */
+enum { HGDT_CONTINUOUS_LAGRANGE = 0, HGDT_DISCONTINUOUS_LAGRANGE = 1 };
+/* Stencil, dune/gdt/type_traits.hh:55 */
+enum { HGDT_STENCIL_ELEMENT = 0, HGDT_STENCIL_INTERSECTION = 1, HGDT_STENCIL_ELEMENT_AND_INTERSECTION = 2 };
+/* source data kinds for the right-hand side functional */
+enum { HGDT_SOURCE_CONSTANT = 0, HGDT_SOURCE_PER_ELEMENT = 1, HGDT_SOURCE_PER_QUADRATURE_POINT = 2 };
+/* matrix storage used for apply / apply_inverse */
+enum { HGDT_FORMAT_AUTO = 0, HGDT_FORMAT_SELL = 1, HGDT_FORMAT_DIA = 2 };
+
+typedef struct hgdt_grid_s* hgdt_grid;
+typedef struct hgdt_space_s* hgdt_space;
+typedef struct hgdt_operator_s* hgdt_operator;
+
+/* message of the last failure on this thread ("" if none) */
+const char* hgdt_last_error(void);
+/* library version as major*10000 + minor*100 + patch */
+int hgdt_version(void);
+/* HIP device count visible to the library; fails with HGDT_ERR_DEVICE when no GPU is usable */
+hgdt_status hgdt_device_count(int* count);
+
+/* stream-ordered device-to-device (or host<->device) copy of `bytes` bytes; utility for bindings that
+ * hold device buffers the library does not own (no reference counterpart) */
+hgdt_status hgdt_memcpy(void* dst, const void* src, int64_t bytes, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* grids                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* XT::Grid::make_cube_grid<G>(lower_left, upper_right, num_elements) as used in
+ * examples/stationary-heat-equation.cc:87 and dune/gdt/test/integrands/integrands.hh:64-70.
+ * cells[0..dim-1] > 0.  Simplices: every cube split into dim! Kuhn simplices
+ * (v0, v0+e_p0, v0+e_p0+e_p1[, v7]) for the axis permutations p in lexicographic order. */
+hgdt_status hgdt_grid_create_structured(int geometry, const int32_t* cells, const double* lower_left,
+                                        const double* upper_right, hgdt_grid* grid);
+/* grid from explicit host arrays: coords [num_vertices x dim], connectivity
+ * [num_elements x vertices_per_element] in the reference vertex order above.
+ * Cube elements must be affine (parallelograms / parallelepipeds). */
+hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* coords, int64_t num_elements,
+                             const int32_t* connectivity, hgdt_grid* grid);
+/* disjoint union of grids of one geometry type: the batch of independent problem
+ * instances.  Vertices, elements and DoFs are numbered instance by instance. */
+hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_grid* grid);
+hgdt_status hgdt_grid_destroy(hgdt_grid grid);
+/* info[0..8] = dim, num_vertices, num_elements, vertices_per_element, faces_per_element,
+ *              num_inner_intersections, num_boundary_intersections, num_instances, geometry */
+hgdt_status hgdt_grid_info(hgdt_grid grid, int64_t* info);
+hgdt_status hgdt_grid_copy_coords(hgdt_grid grid, double* host_out);
+hgdt_status hgdt_grid_copy_connectivity(hgdt_grid grid, int32_t* host_out);
+/* element offsets of the instances, host_out[num_instances + 1] */
+hgdt_status hgdt_grid_instance_elements(hgdt_grid grid, int64_t* host_out);
+/* physical coordinates of the element quadrature points of the given order,
+ * host_out[num_elements x num_points x dim]; *num_points receives the rule size.
+ * Pass host_out = NULL to query the size only. */
+hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
+
+/* ------------------------------------------------------------------------- */
+/* spaces                                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* ContinuousLagrangeSpace<GV>(grid_view, order), dune/gdt/spaces/h1/continuous-lagrange.hh:68-79
+ * DiscontinuousLagrangeSpace<GV>(grid_view, order), dune/gdt/spaces/l2/discontinuous-lagrange.hh:78-86
+ * order 1 (P1 on simplices, Q1 on cubes). */
+hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space);
+hgdt_status hgdt_space_destroy(hgdt_space space);
+/* mapper().size() and max_local_size(), spaces/mapper/continuous.hh:139-150 */
+hgdt_status hgdt_space_size(hgdt_space space, int64_t* size, int32_t* local_size);
+/* mapper().global_indices(element) for all elements, host_out[num_elements x local_size]
+ * (spaces/mapper/continuous.hh:139-150, spaces/mapper/discontinuous.hh:102-125) */
+hgdt_status hgdt_space_copy_global_indices(hgdt_space space, int32_t* host_out);
+/* DoF offsets of the instances of a union grid, host_out[num_instances + 1] */
+hgdt_status hgdt_space_instance_dofs(hgdt_space space, int64_t* host_out);
+
+/* ------------------------------------------------------------------------- */
+/* matrix operator                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* make_matrix_operator<M>(space, stencil), dune/gdt/operators/matrix-based.hh:650-658: builds the
+ * sparsity pattern (dune/gdt/tools/sparsity-pattern.hh:33-144) and a zero matrix on it
+ * (MatrixOperator ctor, matrix-based.hh:298-311).  Test and ansatz space are the same space. */
+hgdt_status hgdt_operator_create(hgdt_space space, int stencil, hgdt_operator* op);
+hgdt_status hgdt_operator_destroy(hgdt_operator op);
+/* rows, cols, nnz of the pattern */
+hgdt_status hgdt_operator_shape(hgdt_operator op, int64_t* rows, int64_t* cols, int64_t* nnz);
+/* CSR pattern (sorted rows), host arrays row_ptr[rows + 1], cols[nnz] */
+hgdt_status hgdt_operator_copy_pattern(hgdt_operator op, int64_t* row_ptr, int32_t* cols);
+
+/* Coefficient convention of the append calls: `field` is a device pointer to one value per
+ * element (piecewise constant grid function, scalar times the identity) that must stay valid
+ * until the operator is destroyed, or NULL to use the constant `value`.  `scaling` is the
+ * "matrixoperator.scaling" parameter (matrix-based.hh:342-365, bilinear-form-assemblers.hh:64).
+ * `over_integrate` raises the quadrature order (local/bilinear-forms/integrals.hh:115). */
+
+/* append(LocalElementIntegralBilinearForm(LocalLaplaceIntegrand(diffusion), over_integrate)),
+ * matrix-based.hh:346-369, local/integrands/laplace.hh:74-102 */
+hgdt_status hgdt_operator_append_laplace(hgdt_operator op, const double* diffusion_field, double diffusion_value,
+                                         int32_t over_integrate, double scaling);
+/* append(LocalElementIntegralBilinearForm(LocalElementProductIntegrand(weight))),
+ * local/integrands/product.hh:104-130 */
+hgdt_status hgdt_operator_append_product(hgdt_operator op, const double* weight_field, double weight_value,
+                                         int32_t over_integrate, double scaling);
+/* append(LocalCouplingIntersectionIntegralBilinearForm(LocalLaplaceIPDGIntegrands::InnerCoupling(
+ *   symmetry_prefactor, diffusion, weight)), {}, ApplyOn::InnerIntersectionsOnce),
+ * matrix-based.hh:371-393, local/integrands/laplace-ipdg.hh:47-186 */
+hgdt_status hgdt_operator_append_inner_coupling(hgdt_operator op, double symmetry_prefactor,
+                                                const double* diffusion_field, double diffusion_value,
+                                                const double* weight_field, double weight_value,
+                                                int32_t over_integrate, double scaling);
+/* append(LocalCouplingIntersectionIntegralBilinearForm(LocalIPDGIntegrands::InnerPenalty(penalty, weight)),
+ * {}, ApplyOn::InnerIntersectionsOnce); intersection diameter = face diameter
+ * (local/integrands/ipdg.hh:27-38, 59-171) */
+hgdt_status hgdt_operator_append_inner_penalty(hgdt_operator op, double penalty, const double* weight_field,
+                                               double weight_value, int32_t over_integrate, double scaling);
+/* append(LocalIntersectionIntegralBilinearForm(LocalLaplaceIPDGIntegrands::DirichletCoupling(
+ *   symmetry_prefactor, diffusion)), {}, CustomBoundaryIntersections(all Dirichlet)),
+ * matrix-based.hh:395-408, local/integrands/laplace-ipdg.hh:237-368 */
+hgdt_status hgdt_operator_append_dirichlet_coupling(hgdt_operator op, double symmetry_prefactor,
+                                                    const double* diffusion_field, double diffusion_value,
+                                                    int32_t over_integrate, double scaling);
+/* append(LocalIntersectionIntegralBilinearForm(LocalIPDGIntegrands::BoundaryPenalty(penalty, weight)), ...),
+ * local/integrands/ipdg.hh:201-282 */
+hgdt_status hgdt_operator_append_boundary_penalty(hgdt_operator op, double penalty, const double* weight_field,
+                                                  double weight_value, int32_t over_integrate, double scaling);
+/* right-hand side functional assembled in the same walk:
+ * VectorBasedFunctional::append(LocalElementIntegralFunctional(LocalProductIntegrand().with_ansatz(f)))
+ * (functionals/vector-based.hh:132-308, local/functionals/integrals.hh:72-97,
+ *  local/assembler/functional-assemblers.hh:77-86).  kind: HGDT_SOURCE_*; for PER_ELEMENT `data` has one
+ * value per element, for PER_QUADRATURE_POINT one value per point of hgdt_grid_quadrature_points(order)
+ * with order = 1 + f_order + over_integrate.  `data` is a device pointer that must stay valid. */
+hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double* data, double value,
+                                        int32_t f_order, int32_t over_integrate);
+/* DirichletConstraints(space, AllDirichletBoundaryInfo) applied after assembly with
+ * ensure_symmetry (tools/dirichlet-constraints.hh:85-184): unit rows and columns, rhs entry 0 */
+hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled);
+/* number of constrained DoFs and their (sorted) indices, host_out may be NULL */
+hgdt_status hgdt_operator_dirichlet_dofs(hgdt_operator op, int64_t* count, int32_t* host_out);
+
+/* MatrixOperator::assemble(use_tbb) (matrix-based.hh:496-500): one device walk over elements and
+ * intersections, deterministic fixed-order scatter, Dirichlet constraints, right-hand side. Async on `stream`. */
+hgdt_status hgdt_operator_assemble(hgdt_operator op, void* stream);
+/* device pointers of the assembled CSR values [nnz] and right-hand side [rows] (owned by the operator) */
+hgdt_status hgdt_operator_values_device(hgdt_operator op, double** values);
+hgdt_status hgdt_operator_rhs_device(hgdt_operator op, double** rhs);
+/* synchronous copies to host memory */
+hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out);
+hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
+
+/* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: DIA when every
+ * instance has at most 32 distinct diagonals, else SELL-64).  The format is rebuilt from the CSR
+ * values at the next apply / apply_inverse after each assemble. */
+hgdt_status hgdt_operator_set_format(hgdt_operator op, int format);
+/* format actually in use after the last preparation (HGDT_FORMAT_SELL or HGDT_FORMAT_DIA) and the
+ * algorithmic bytes one application of the operator reads and writes */
+hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t* spmv_bytes, int32_t* num_diagonals);
+
+/* ConstMatrixOperator::apply (matrix-based.hh:121-129): y = A x, device vectors of length rows */
+hgdt_status hgdt_operator_apply(hgdt_operator op, const double* x, double* y, void* stream);
+
+typedef struct hgdt_solver_options {
+  double rtol;            /* stop when ||r_k||_2 <= rtol * ||b||_2 (per instance) */
+  int64_t max_iter;       /* per instance */
+  int32_t check_every;    /* iterations between host-side convergence checks (>= 1) */
+  int32_t use_graph;      /* 1: replay the iteration block as a captured hipGraph */
+  int32_t time_spmv;      /* 1: record HIP events around every operator application */
+  int32_t zero_initial_guess; /* 1: start from x = 0, else from the given x */
+} hgdt_solver_options;
+
+typedef struct hgdt_solver_stats {
+  int32_t converged;      /* all instances converged */
+  int32_t breakdown;      /* non-positive curvature or non-positive diagonal met */
+  int64_t iterations;     /* max over instances */
+  double relres;          /* max over instances of ||r_k|| / ||b|| (recursive residual) */
+  double spmv_avg_ms;     /* mean duration of one operator application (time_spmv = 1) */
+  int64_t spmv_count;     /* applications timed */
+  double solve_ms;        /* wall time of the solve measured with HIP events */
+} hgdt_solver_stats;
+
+hgdt_status hgdt_solver_options_default(hgdt_solver_options* options);
+/* ConstMatrixOperator::apply_inverse (matrix-based.hh:148-159) with a Jacobi-preconditioned CG
+ * (x0 = 0 by default); b and x are device vectors of length rows.  For a union grid every instance
+ * is solved with its own scalars and its own stopping test in the same launches.  Blocks the host
+ * until convergence is known.  Fails with HGDT_ERR_SOLVER_FAILED when an instance does not converge. */
+hgdt_status hgdt_operator_apply_inverse(hgdt_operator op, const double* b, double* x,
+                                        const hgdt_solver_options* options, hgdt_solver_stats* stats,
+                                        void* stream);
+/* per-instance iteration counts and relative residuals of the last apply_inverse, host arrays */
+hgdt_status hgdt_operator_instance_stats(hgdt_operator op, int64_t* iterations, double* relres);
+
+/* Adjoint element kernel (no reference counterpart): for the Laplace form appended at position
+ * `form_index` among the element forms, with per-element diffusion kappa_e,
+ *   grad_e = - kappa_e * lambda_e^T Khat_e u_e
+ * where Khat_e is the unit-coefficient local Laplace matrix recomputed element-locally.  This is
+ * dJ/d(log kappa_e) for J(u(kappa)) when lambda solves A lambda = dJ/du.  u, lambda: device vectors
+ * [rows]; grad: device [num_elements]. */
+hgdt_status hgdt_operator_adjoint_log_diffusion_gradient(hgdt_operator op, int32_t form_index, const double* u,
+                                                         const double* lambda, double* grad, void* stream);
+
+/* launch configuration of the tunable kernels (used by the autotuner); kernel ids: 0 element assembly,
+ * 1 intersection assembly, 2 scatter/gather, 3 operator application, 4 vector updates.
+ * block_size multiple of 64 in [64, 1024]; items_per_thread in [1, 8]. */
+hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t block_size, int32_t items_per_thread);
+hgdt_status hgdt_operator_get_launch(hgdt_operator op, int32_t kernel, int32_t* block_size, int32_t* items_per_thread);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HGDT_H */

@@ -1,0 +1,1580 @@
+// oracle.cpp -- CPU restatement of dune-gdt's global assembly and linear-solve path.
+//
+// TEST INFRASTRUCTURE ONLY: this is the checker.  It is loaded by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+// product library.  It follows the reference's loop nests one to one:
+//   * per element: bind basis, quadrature loop accumulating a dense local
+//     matrix (dune/gdt/local/bilinear-forms/integrals.hh:97-134), then
+//     add_to_entry of scaling * local into the sorted CSR row
+//     (dune/gdt/local/assembler/bilinear-form-assemblers.hh:110-128);
+//   * per inner intersection visited once / per boundary intersection: face
+//     quadrature loops (integrals.hh:197-267, 338-369) and the four-block /
+//     one-block scatters (bilinear-form-assemblers.hh:238-278, 380-396);
+//   * right-hand side functional (dune/gdt/local/functionals/integrals.hh:72-97,
+//     dune/gdt/local/assembler/functional-assemblers.hh:77-86);
+//   * Dirichlet constraints (dune/gdt/tools/dirichlet-constraints.hh:85-184);
+//   * sparsity patterns (dune/gdt/tools/sparsity-pattern.hh:33-144);
+//   * Jacobi-preconditioned CG for apply_inverse
+//     (dune/gdt/operators/matrix-based.hh:148-159; the ISTL solver behind it is
+//     not vendored, the PCG recursion below is the textbook one).
+// Reference paths are relative to the dune-gdt source tree.
+//
+// Build-defined conventions (DUNE's come from un-vendored dune-geometry/grid):
+// reference cube vertex b <-> bits of b (x = bit 0, y = bit 1, z = bit 2);
+// reference simplex vertex 0 = origin, vertex k = e_k; structured vertices and
+// elements lexicographic with x fastest; Kuhn split of each cube into d!
+// simplices ordered by the lexicographic order of the axis permutations.
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// ---------------------------------------------------------------------------
+// reference elements
+// ---------------------------------------------------------------------------
+struct RefElement {
+  int geom = 0, dim = 0, nverts = 0, nfaces = 0;
+  bool simplex = false;
+  std::vector<std::array<double, 3>> verts;
+  std::vector<std::vector<int>> faces; // local vertex ids of each face, DUNE face numbering
+};
+
+RefElement make_ref(int geom)
+{
+  RefElement r;
+  r.geom = geom;
+  switch (geom) {
+    case OR_QUAD:
+      r.dim = 2, r.nverts = 4, r.nfaces = 4;
+      r.verts = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
+      r.faces = {{0, 2}, {1, 3}, {0, 1}, {2, 3}};
+      break;
+    case OR_TRI:
+      r.dim = 2, r.nverts = 3, r.nfaces = 3, r.simplex = true;
+      r.verts = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+      r.faces = {{0, 1}, {0, 2}, {1, 2}};
+      break;
+    case OR_HEX:
+      r.dim = 3, r.nverts = 8, r.nfaces = 6;
+      for (int b = 0; b < 8; ++b)
+        r.verts.push_back({double(b & 1), double((b >> 1) & 1), double((b >> 2) & 1)});
+      r.faces = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+      break;
+    case OR_TET:
+      r.dim = 3, r.nverts = 4, r.nfaces = 4, r.simplex = true;
+      r.verts = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      r.faces = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      break;
+    default:
+      throw Error("unknown geometry type");
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// quadrature (dune-geometry QuadratureRules<D,d>::rule(type, order) is not
+// vendored; these rules integrate polynomials of the requested order exactly)
+// ---------------------------------------------------------------------------
+struct Quadrature {
+  int dim = 0;
+  std::vector<std::array<double, 3>> x;
+  std::vector<double> w;
+};
+
+// Gauss-Legendre with n points on [0,1], ascending
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w)
+{
+  x.assign(n, 0.), w.assign(n, 0.);
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1., p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2. * k - 1.) * t * p1 - (k - 1.) * p0) / k;
+        p0 = p1, p1 = p2;
+      }
+      const double pn = (n == 1) ? t : p1, pnm1 = (n == 1) ? 1. : p0;
+      const double dp = n * (t * pn - pnm1) / (t * t - 1.);
+      const double dt = pn / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-17)
+        break;
+    }
+    double p0 = 1., p1 = t;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2. * k - 1.) * t * p1 - (k - 1.) * p0) / k;
+      p0 = p1, p1 = p2;
+    }
+    const double pn = (n == 1) ? t : p1, pnm1 = (n == 1) ? 1. : p0;
+    const double dp = n * (t * pn - pnm1) / (t * t - 1.);
+    x[i] = 0.5 * (1. - t);
+    w[i] = 1. / ((1. - t * t) * dp * dp);
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i)
+    idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return x[a] < x[b]; });
+  std::vector<double> xs(n), ws(n);
+  for (int i = 0; i < n; ++i)
+    xs[i] = x[idx[i]], ws[i] = w[idx[i]];
+  x = xs, w = ws;
+}
+
+Quadrature line_rule(int order)
+{
+  Quadrature q;
+  q.dim = 1;
+  std::vector<double> x, w;
+  gauss_legendre01(order / 2 + 1, x, w);
+  for (size_t i = 0; i < x.size(); ++i)
+    q.x.push_back({x[i], 0, 0}), q.w.push_back(w[i]);
+  return q;
+}
+
+Quadrature cube_rule(int dim, int order)
+{
+  Quadrature q;
+  q.dim = dim;
+  std::vector<double> x, w;
+  gauss_legendre01(order / 2 + 1, x, w);
+  const int n = int(x.size());
+  const int total = (dim == 2) ? n * n : n * n * n;
+  for (int t = 0; t < total; ++t) {
+    const int i = t % n, j = (t / n) % n, k = t / (n * n);
+    if (dim == 2)
+      q.x.push_back({x[i], x[j], 0}), q.w.push_back(w[i] * w[j]);
+    else
+      q.x.push_back({x[i], x[j], x[k]}), q.w.push_back(w[i] * w[j] * w[k]);
+  }
+  return q;
+}
+
+Quadrature simplex_rule(int dim, int order)
+{
+  Quadrature q;
+  q.dim = dim;
+  if (order <= 1) {
+    if (dim == 2)
+      q.x.push_back({1. / 3., 1. / 3., 0}), q.w.push_back(0.5);
+    else
+      q.x.push_back({0.25, 0.25, 0.25}), q.w.push_back(1. / 6.);
+    return q;
+  }
+  if (order == 2) {
+    if (dim == 2) {
+      const double a = 1. / 6., b = 2. / 3.;
+      q.x = {{a, a, 0}, {b, a, 0}, {a, b, 0}};
+      q.w = {1. / 6., 1. / 6., 1. / 6.};
+    } else {
+      const double a = (5. + 3. * std::sqrt(5.)) / 20., b = (5. - std::sqrt(5.)) / 20.;
+      q.x = {{b, b, b}, {a, b, b}, {b, a, b}, {b, b, a}};
+      q.w = {1. / 24., 1. / 24., 1. / 24., 1. / 24.};
+    }
+    return q;
+  }
+  // conical product (collapsed Gauss-Legendre) rule, exact to the given order
+  std::vector<double> xu, wu, xv, wv, xw, ww;
+  if (dim == 2) {
+    gauss_legendre01((order + 3) / 2, xu, wu);
+    gauss_legendre01((order + 2) / 2, xv, wv);
+    for (size_t i = 0; i < xu.size(); ++i)
+      for (size_t j = 0; j < xv.size(); ++j) {
+        q.x.push_back({xu[i], (1. - xu[i]) * xv[j], 0});
+        q.w.push_back(wu[i] * wv[j] * (1. - xu[i]));
+      }
+  } else {
+    gauss_legendre01((order + 4) / 2, xu, wu);
+    gauss_legendre01((order + 3) / 2, xv, wv);
+    gauss_legendre01((order + 2) / 2, xw, ww);
+    for (size_t i = 0; i < xu.size(); ++i)
+      for (size_t j = 0; j < xv.size(); ++j)
+        for (size_t k = 0; k < xw.size(); ++k) {
+          const double u = xu[i], v = xv[j];
+          q.x.push_back({u, (1. - u) * v, (1. - u) * (1. - v) * xw[k]});
+          q.w.push_back(wu[i] * wv[j] * ww[k] * (1. - u) * (1. - u) * (1. - v));
+        }
+  }
+  return q;
+}
+
+Quadrature element_rule(const RefElement& ref, int order)
+{
+  return ref.simplex ? simplex_rule(ref.dim, order) : cube_rule(ref.dim, order);
+}
+
+// ---------------------------------------------------------------------------
+// Lagrange local finite elements (dune/gdt/local/finite-elements/lagrange.hh:65-186
+// wraps dune-localfunctions; order() as in local/finite-elements/wrapper.hh:125-127)
+// ---------------------------------------------------------------------------
+struct LocalFE {
+  const RefElement* ref = nullptr;
+  int order = 1, size = 0;
+  std::vector<std::array<double, 3>> nodes;
+  std::vector<std::vector<int>> node_subentity; // local vertices spanning the node's subentity
+  // simplex order 2: node -> (a, b) vertex pair (a == b for vertices)
+  std::vector<std::array<int, 2>> simplex_pair;
+
+  static double lag1d(int k, int m, double t)
+  {
+    double v = 1.;
+    for (int l = 0; l <= k; ++l)
+      if (l != m)
+        v *= (t - double(l) / k) / (double(m) / k - double(l) / k);
+    return v;
+  }
+  static double dlag1d(int k, int m, double t)
+  {
+    double s = 0.;
+    for (int q = 0; q <= k; ++q) {
+      if (q == m)
+        continue;
+      double v = 1. / (double(m) / k - double(q) / k);
+      for (int l = 0; l <= k; ++l)
+        if (l != m && l != q)
+          v *= (t - double(l) / k) / (double(m) / k - double(l) / k);
+      s += v;
+    }
+    return s;
+  }
+
+  void init(const RefElement* r, int k)
+  {
+    ref = r, order = k;
+    const int d = r->dim;
+    if (!r->simplex) {
+      const int n1 = k + 1;
+      size = (d == 2) ? n1 * n1 : n1 * n1 * n1;
+      for (int t = 0; t < size; ++t) {
+        const int idx[3] = {t % n1, (t / n1) % n1, t / (n1 * n1)};
+        std::array<double, 3> x = {0, 0, 0};
+        for (int a = 0; a < d; ++a)
+          x[a] = double(idx[a]) / k;
+        nodes.push_back(x);
+        std::vector<int> sub;
+        for (int b = 0; b < r->nverts; ++b) {
+          bool ok = true;
+          for (int a = 0; a < d; ++a) {
+            const int bit = (b >> a) & 1;
+            if (idx[a] == 0 && bit != 0)
+              ok = false;
+            if (idx[a] == k && bit != 1)
+              ok = false;
+          }
+          if (ok)
+            sub.push_back(b);
+        }
+        node_subentity.push_back(sub);
+      }
+    } else {
+      if (k < 1 || k > 2)
+        throw Error("simplex Lagrange elements of order 1 and 2 only");
+      for (int v = 0; v < r->nverts; ++v) {
+        nodes.push_back(r->verts[v]);
+        node_subentity.push_back({v});
+        simplex_pair.push_back({v, v});
+      }
+      if (k == 2) {
+        for (int a = 0; a < r->nverts; ++a)
+          for (int b = a + 1; b < r->nverts; ++b) {
+            std::array<double, 3> x;
+            for (int c = 0; c < 3; ++c)
+              x[c] = 0.5 * (r->verts[a][c] + r->verts[b][c]);
+            nodes.push_back(x);
+            node_subentity.push_back({a, b});
+            simplex_pair.push_back({a, b});
+          }
+      }
+      size = int(nodes.size());
+    }
+  }
+
+  // barycentric coordinates and their (constant) reference gradients
+  void bary(const double* x, double* lam, double (*glam)[3]) const
+  {
+    const int d = ref->dim;
+    lam[0] = 1.;
+    for (int a = 0; a < d; ++a)
+      lam[0] -= x[a];
+    for (int a = 0; a < d; ++a)
+      lam[a + 1] = x[a];
+    for (int v = 0; v <= d; ++v)
+      for (int a = 0; a < 3; ++a)
+        glam[v][a] = 0.;
+    for (int a = 0; a < d; ++a)
+      glam[0][a] = -1., glam[a + 1][a] = 1.;
+  }
+
+  void evaluate(const double* x, double* vals) const
+  {
+    const int d = ref->dim;
+    if (!ref->simplex) {
+      const int n1 = order + 1;
+      for (int t = 0; t < size; ++t) {
+        const int idx[3] = {t % n1, (t / n1) % n1, t / (n1 * n1)};
+        double v = 1.;
+        for (int a = 0; a < d; ++a)
+          v *= lag1d(order, idx[a], x[a]);
+        vals[t] = v;
+      }
+    } else {
+      double lam[4], glam[4][3];
+      bary(x, lam, glam);
+      for (int t = 0; t < size; ++t) {
+        const int a = simplex_pair[t][0], b = simplex_pair[t][1];
+        if (order == 1)
+          vals[t] = lam[a];
+        else if (a == b)
+          vals[t] = lam[a] * (2. * lam[a] - 1.);
+        else
+          vals[t] = 4. * lam[a] * lam[b];
+      }
+    }
+  }
+
+  // reference gradients, grads[t*3 + a]
+  void jacobians(const double* x, double* grads) const
+  {
+    const int d = ref->dim;
+    if (!ref->simplex) {
+      const int n1 = order + 1;
+      for (int t = 0; t < size; ++t) {
+        const int idx[3] = {t % n1, (t / n1) % n1, t / (n1 * n1)};
+        for (int a = 0; a < 3; ++a)
+          grads[t * 3 + a] = 0.;
+        for (int a = 0; a < d; ++a) {
+          double v = dlag1d(order, idx[a], x[a]);
+          for (int c = 0; c < d; ++c)
+            if (c != a)
+              v *= lag1d(order, idx[c], x[c]);
+          grads[t * 3 + a] = v;
+        }
+      }
+    } else {
+      double lam[4], glam[4][3];
+      bary(x, lam, glam);
+      for (int t = 0; t < size; ++t) {
+        const int a = simplex_pair[t][0], b = simplex_pair[t][1];
+        for (int c = 0; c < 3; ++c) {
+          if (order == 1)
+            grads[t * 3 + c] = glam[a][c];
+          else if (a == b)
+            grads[t * 3 + c] = (4. * lam[a] - 1.) * glam[a][c];
+          else
+            grads[t * 3 + c] = 4. * (glam[a][c] * lam[b] + lam[a] * glam[b][c]);
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// affine element geometry (integrationElement, jacobianInverseTransposed,
+// spaces/basis/default.hh:152-175 applies J^{-T} to reference gradients)
+// ---------------------------------------------------------------------------
+struct Geometry {
+  int dim = 0;
+  double x0[3] = {0, 0, 0};
+  double J[3][3] = {{0}}, Jinv[3][3] = {{0}};
+  double det = 0., integration_element = 0.;
+
+  void global(const double* xh, double* x) const
+  {
+    for (int a = 0; a < dim; ++a) {
+      double s = x0[a];
+      for (int c = 0; c < dim; ++c)
+        s += J[a][c] * xh[c];
+      x[a] = s;
+    }
+  }
+  void local(const double* x, double* xh) const
+  {
+    double dx[3];
+    for (int a = 0; a < dim; ++a)
+      dx[a] = x[a] - x0[a];
+    for (int a = 0; a < dim; ++a) {
+      double s = 0.;
+      for (int c = 0; c < dim; ++c)
+        s += Jinv[a][c] * dx[c];
+      xh[a] = s;
+    }
+  }
+  // physical gradient g = J^{-T} gh
+  void transform_gradient(const double* gh, double* g) const
+  {
+    for (int a = 0; a < dim; ++a) {
+      double s = 0.;
+      for (int c = 0; c < dim; ++c)
+        s += Jinv[c][a] * gh[c];
+      g[a] = s;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// grid
+// ---------------------------------------------------------------------------
+struct Mesh {
+  RefElement ref;
+  int64_t nv = 0, ne = 0;
+  std::vector<double> coords; // nv * dim
+  std::vector<int32_t> conn;  // ne * nverts
+  std::vector<int32_t> nb;    // ne * nfaces, neighbour element or -1
+  std::vector<int32_t> nbface;
+
+  Geometry geometry(int64_t e) const
+  {
+    Geometry g;
+    const int d = ref.dim;
+    g.dim = d;
+    const int32_t* c = &conn[e * ref.nverts];
+    for (int a = 0; a < d; ++a)
+      g.x0[a] = coords[int64_t(c[0]) * d + a];
+    for (int k = 0; k < d; ++k) {
+      const int lv = ref.simplex ? (k + 1) : (1 << k);
+      for (int a = 0; a < d; ++a)
+        g.J[a][k] = coords[int64_t(c[lv]) * d + a] - g.x0[a];
+    }
+    bool diagonal = true;
+    for (int a = 0; a < d; ++a)
+      for (int k = 0; k < d; ++k)
+        if (a != k && g.J[a][k] != 0.)
+          diagonal = false;
+    if (diagonal) {
+      g.det = 1.;
+      for (int a = 0; a < d; ++a)
+        g.det *= g.J[a][a], g.Jinv[a][a] = 1. / g.J[a][a];
+    } else if (d == 2) {
+      g.det = g.J[0][0] * g.J[1][1] - g.J[0][1] * g.J[1][0];
+      g.Jinv[0][0] = g.J[1][1] / g.det, g.Jinv[0][1] = -g.J[0][1] / g.det;
+      g.Jinv[1][0] = -g.J[1][0] / g.det, g.Jinv[1][1] = g.J[0][0] / g.det;
+    } else {
+      const double(*A)[3] = g.J;
+      g.det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+              + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+      g.Jinv[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / g.det;
+      g.Jinv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / g.det;
+      g.Jinv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / g.det;
+      g.Jinv[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / g.det;
+      g.Jinv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / g.det;
+      g.Jinv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / g.det;
+      g.Jinv[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / g.det;
+      g.Jinv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / g.det;
+      g.Jinv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / g.det;
+    }
+    g.integration_element = std::fabs(g.det);
+    return g;
+  }
+
+  void check_affine() const
+  {
+    if (ref.simplex)
+      return;
+    const int d = ref.dim;
+    for (int64_t e = 0; e < ne; ++e) {
+      const int32_t* c = &conn[e * ref.nverts];
+      for (int b = 0; b < ref.nverts; ++b)
+        for (int a = 0; a < d; ++a) {
+          double expect = coords[int64_t(c[0]) * d + a];
+          for (int k = 0; k < d; ++k)
+            if ((b >> k) & 1)
+              expect += coords[int64_t(c[1 << k]) * d + a] - coords[int64_t(c[0]) * d + a];
+          const double got = coords[int64_t(c[b]) * d + a];
+          if (std::fabs(expect - got) > 1e-12 * (1. + std::fabs(got)))
+            throw Error("oracle supports affine (parallelogram/parallelepiped) cube elements only");
+        }
+    }
+  }
+
+  void build_neighbors()
+  {
+    const int nf = ref.nfaces;
+    nb.assign(ne * nf, -1);
+    nbface.assign(ne * nf, -1);
+    std::map<std::vector<int32_t>, std::pair<int64_t, int>> open;
+    for (int64_t e = 0; e < ne; ++e)
+      for (int f = 0; f < nf; ++f) {
+        std::vector<int32_t> key;
+        for (int lv : ref.faces[f])
+          key.push_back(conn[e * ref.nverts + lv]);
+        std::sort(key.begin(), key.end());
+        auto it = open.find(key);
+        if (it == open.end()) {
+          open.emplace(key, std::make_pair(e, f));
+        } else {
+          const int64_t o = it->second.first;
+          const int of = it->second.second;
+          nb[e * nf + f] = int32_t(o), nbface[e * nf + f] = of;
+          nb[o * nf + of] = int32_t(e), nbface[o * nf + of] = f;
+          open.erase(it);
+        }
+      }
+  }
+};
+
+Mesh* structured_mesh(int geom, const int32_t* cells, const double* lo, const double* hi)
+{
+  auto* m = new Mesh;
+  m->ref = make_ref(geom);
+  const int d = m->ref.dim;
+  int64_t n[3] = {1, 1, 1}, nvx[3] = {1, 1, 1};
+  double h[3] = {0, 0, 0};
+  for (int a = 0; a < d; ++a) {
+    if (cells[a] < 1)
+      throw Error("cells must be positive");
+    n[a] = cells[a], nvx[a] = cells[a] + 1, h[a] = (hi[a] - lo[a]) / cells[a];
+  }
+  m->nv = nvx[0] * nvx[1] * nvx[2];
+  m->coords.resize(m->nv * d);
+  for (int64_t v = 0; v < m->nv; ++v) {
+    const int64_t idx[3] = {v % nvx[0], (v / nvx[0]) % nvx[1], v / (nvx[0] * nvx[1])};
+    for (int a = 0; a < d; ++a)
+      m->coords[v * d + a] = lo[a] + idx[a] * h[a];
+  }
+  const int64_t ncell = n[0] * n[1] * n[2];
+  auto vid = [&](int64_t i, int64_t j, int64_t k) { return int32_t((k * nvx[1] + j) * nvx[0] + i); };
+  for (int64_t c = 0; c < ncell; ++c) {
+    const int64_t i = c % n[0], j = (c / n[0]) % n[1], k = c / (n[0] * n[1]);
+    int32_t corner[8];
+    for (int b = 0; b < (1 << d); ++b)
+      corner[b] = vid(i + (b & 1), j + ((b >> 1) & 1), k + ((b >> 2) & 1));
+    if (geom == OR_QUAD || geom == OR_HEX) {
+      for (int b = 0; b < (1 << d); ++b)
+        m->conn.push_back(corner[b]);
+    } else if (geom == OR_TRI) {
+      // permutations (x,y), (y,x): v0, v0+e_p0, v0+e_p0+e_p1
+      m->conn.insert(m->conn.end(), {corner[0], corner[1], corner[3]});
+      m->conn.insert(m->conn.end(), {corner[0], corner[2], corner[3]});
+    } else {
+      static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+      for (auto& p : perms) {
+        const int b1 = 1 << p[0], b2 = b1 | (1 << p[1]);
+        m->conn.insert(m->conn.end(), {corner[0], corner[b1], corner[b2], corner[7]});
+      }
+    }
+  }
+  m->ne = int64_t(m->conn.size()) / m->ref.nverts;
+  m->build_neighbors();
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// spaces + mappers (spaces/mapper/continuous.hh:139-182,
+// spaces/mapper/discontinuous.hh:102-125)
+// ---------------------------------------------------------------------------
+struct Space {
+  const Mesh* mesh = nullptr;
+  bool continuous = true;
+  int order = 1;
+  LocalFE fe;
+  int64_t size = 0;
+  std::vector<int32_t> indices; // ne * fe.size
+
+  void init(const Mesh* m, bool cont, int k)
+  {
+    mesh = m, continuous = cont, order = k;
+    fe.init(&m->ref, k);
+    const int ls = fe.size;
+    indices.resize(m->ne * ls);
+    if (!cont) {
+      // offset[element] + local index, offsets in element order
+      for (int64_t e = 0; e < m->ne; ++e)
+        for (int i = 0; i < ls; ++i)
+          indices[e * ls + i] = int32_t(e * ls + i);
+      size = m->ne * ls;
+      return;
+    }
+    if (k == 1) {
+      // Q1/P1 nodes coincide with the reference vertices: global DoF = vertex index
+      for (int64_t e = 0; e < m->ne; ++e)
+        for (int i = 0; i < ls; ++i)
+          indices[e * ls + i] = m->conn[e * m->ref.nverts + fe.node_subentity[i][0]];
+      size = m->nv;
+      return;
+    }
+    // higher order: one DoF per (subentity, position); order <= 2 has one node per subentity
+    std::map<std::vector<int32_t>, int32_t> ids;
+    for (int64_t e = 0; e < m->ne; ++e)
+      for (int i = 0; i < ls; ++i) {
+        std::vector<int32_t> key;
+        for (int lv : fe.node_subentity[i])
+          key.push_back(m->conn[e * m->ref.nverts + lv]);
+        std::sort(key.begin(), key.end());
+        if (key.size() > 1 && order > 2)
+          throw Error("continuous spaces of order > 2 not supported");
+        auto it = ids.find(key);
+        if (it == ids.end())
+          it = ids.emplace(key, int32_t(ids.size())).first;
+        indices[e * ls + i] = it->second;
+      }
+    size = int64_t(ids.size());
+  }
+};
+
+// ---------------------------------------------------------------------------
+// coefficient fields (XT GridFunction restricted to element-wise constants)
+// ---------------------------------------------------------------------------
+struct Coefficient {
+  std::vector<double> values; // empty -> constant
+  double constant = 1.;
+  double at(int64_t e) const
+  {
+    return values.empty() ? constant : values[e];
+  }
+  void set(const double* v, double c, int64_t ne)
+  {
+    constant = c;
+    if (v)
+      values.assign(v, v + ne);
+  }
+};
+
+enum ElementFormKind { EF_LAPLACE = 0, EF_PRODUCT = 1 };
+enum FaceFormKind { FF_INNER_COUPLING = 0, FF_INNER_PENALTY = 1, FF_DIRICHLET_COUPLING = 2, FF_BOUNDARY_PENALTY = 3 };
+
+struct ElementForm {
+  int kind;
+  Coefficient coeff;
+  int over_integrate;
+  double scaling;
+};
+struct FaceForm {
+  int kind;
+  double param; // symmetry prefactor or penalty
+  Coefficient kappa, omega;
+  int over_integrate;
+  double scaling;
+};
+struct Source {
+  int kind;
+  Coefficient f;
+  int f_order;
+  int over_integrate;
+};
+
+// bound local basis: values and physical gradients at a reference point
+struct BoundBasis {
+  const LocalFE* fe = nullptr;
+  const Geometry* geo = nullptr;
+  std::vector<double> vals, rgrads, grads; // grads[i*3+a]
+  void bind(const LocalFE* f, const Geometry* g)
+  {
+    fe = f, geo = g;
+    vals.resize(f->size), rgrads.resize(3 * f->size), grads.assign(3 * f->size, 0.);
+  }
+  void evaluate(const double* xh)
+  {
+    fe->evaluate(xh, vals.data());
+  }
+  void jacobians(const double* xh)
+  {
+    fe->jacobians(xh, rgrads.data());
+    for (int i = 0; i < fe->size; ++i)
+      geo->transform_gradient(&rgrads[3 * i], &grads[3 * i]);
+  }
+};
+
+inline double dot(const double* a, const double* b, int d)
+{
+  double s = 0.;
+  for (int i = 0; i < d; ++i)
+    s += a[i] * b[i];
+  return s;
+}
+
+// (kappa I g) . h with the reference's matrix-vector then dot evaluation order
+inline double kgrad_dot(double kappa, const double* g, const double* h, int d)
+{
+  double s = 0.;
+  for (int a = 0; a < d; ++a)
+    s += (kappa * g[a]) * h[a];
+  return s;
+}
+
+struct Operator {
+  Space* space = nullptr;
+  int stencil = 0;
+  int64_t n = 0;
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> cols;
+  std::vector<double> values, rhs;
+  std::vector<ElementForm> eforms;
+  std::vector<FaceForm> cforms, bforms;
+  std::vector<Source> sources;
+  bool dirichlet = false;
+  std::vector<int32_t> dirichlet_dofs;
+
+  // sparsity pattern, tools/sparsity-pattern.hh:33-50 (element) and :117-144 (element+intersection)
+  void build_pattern()
+  {
+    const Mesh& m = *space->mesh;
+    const int ls = space->fe.size, nf = m.ref.nfaces;
+    n = space->size;
+    std::vector<std::set<int32_t>> pattern(n);
+    for (int64_t e = 0; e < m.ne; ++e) {
+      const int32_t* gi = &space->indices[e * ls];
+      if (stencil != OR_STENCIL_INTERSECTION)
+        for (int i = 0; i < ls; ++i)
+          for (int j = 0; j < ls; ++j)
+            pattern[gi[i]].insert(gi[j]);
+      if (stencil != OR_STENCIL_ELEMENT)
+        for (int f = 0; f < nf; ++f) {
+          const int32_t o = m.nb[e * nf + f];
+          if (o < 0)
+            continue;
+          const int32_t* gj = &space->indices[int64_t(o) * ls];
+          for (int i = 0; i < ls; ++i)
+            for (int j = 0; j < ls; ++j)
+              pattern[gi[i]].insert(gj[j]);
+        }
+    }
+    row_ptr.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+      row_ptr[i + 1] = row_ptr[i] + int64_t(pattern[i].size());
+    cols.resize(row_ptr[n]);
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t k = row_ptr[i];
+      for (int32_t c : pattern[i])
+        cols[k++] = c;
+    }
+    values.assign(row_ptr[n], 0.);
+    rhs.assign(n, 0.);
+  }
+
+  // ISTL-style add_to_entry: locate the column inside the sorted row
+  inline void add_to_entry(int32_t i, int32_t j, double v, bool atomic)
+  {
+    const int32_t* b = &cols[row_ptr[i]];
+    const int32_t* e = &cols[row_ptr[i + 1]];
+    const int32_t* it = std::lower_bound(b, e, j);
+    if (it == e || *it != j)
+      throw Error("add_to_entry: entry not in sparsity pattern");
+    double& slot = values[row_ptr[i] + (it - b)];
+    if (atomic) {
+#pragma omp atomic
+      slot += v;
+    } else {
+      slot += v;
+    }
+  }
+
+  // LocalElementIntegralBilinearForm::apply2, local/bilinear-forms/integrals.hh:97-134
+  void element_local(const ElementForm& F, int64_t e, const Geometry& geo, BoundBasis& basis,
+                     std::vector<double>& result) const
+  {
+    const int ls = space->fe.size, d = geo.dim;
+    result.assign(ls * ls, 0.);
+    const int basis_order = space->fe.order;
+    // LocalLaplaceIntegrand::order (local/integrands/laplace.hh:74-79) and
+    // LocalElementProductIntegrand::order (local/integrands/product.hh:88-100): 0 + test + ansatz
+    const int order = 0 + basis_order + basis_order + F.over_integrate;
+    const Quadrature q = element_rule(space->mesh->ref, order);
+    const double c = F.coeff.at(e);
+    std::vector<double> iv(ls * ls);
+    for (size_t qp = 0; qp < q.w.size(); ++qp) {
+      const double* xh = q.x[qp].data();
+      const double factor = geo.integration_element * q.w[qp];
+      if (F.kind == EF_LAPLACE) {
+        // local/integrands/laplace.hh:94-101
+        basis.jacobians(xh);
+        for (int i = 0; i < ls; ++i)
+          for (int j = 0; j < ls; ++j)
+            iv[i * ls + j] = 0. + kgrad_dot(c, &basis.grads[3 * j], &basis.grads[3 * i], d);
+      } else {
+        // local/integrands/product.hh:120-128
+        basis.evaluate(xh);
+        for (int i = 0; i < ls; ++i)
+          for (int j = 0; j < ls; ++j)
+            iv[i * ls + j] = (c * basis.vals[i]) * basis.vals[j];
+      }
+      for (int i = 0; i < ls; ++i)
+        for (int j = 0; j < ls; ++j)
+          result[i * ls + j] += iv[i * ls + j] * factor;
+    }
+  }
+
+  // face quadrature support: inside/outside reference points, unit outer normal, face volume
+  struct FaceGeo {
+    std::vector<std::array<double, 3>> xin, xout;
+    std::vector<double> w;
+    double normal[3] = {0, 0, 0};
+    double integration_element = 0., diameter = 0.;
+  };
+
+  FaceGeo face_geometry(int64_t e, int f, int64_t o, const Geometry& gin, const Geometry* gout, int order) const
+  {
+    const Mesh& m = *space->mesh;
+    if (m.ref.dim != 2)
+      throw Error("oracle face integrals are implemented for 2d grids");
+    FaceGeo fg;
+    const int a = m.ref.faces[f][0], b = m.ref.faces[f][1];
+    double Pa[3], Pb[3];
+    gin.global(m.ref.verts[a].data(), Pa);
+    gin.global(m.ref.verts[b].data(), Pb);
+    const double tx = Pb[0] - Pa[0], ty = Pb[1] - Pa[1];
+    const double len = std::sqrt(tx * tx + ty * ty);
+    fg.integration_element = len, fg.diameter = len;
+    double nx = ty / len, ny = -tx / len;
+    // orient away from the inside element's centroid
+    double cen[2] = {0, 0};
+    for (int v = 0; v < m.ref.nverts; ++v) {
+      double P[3];
+      gin.global(m.ref.verts[v].data(), P);
+      cen[0] += P[0] / m.ref.nverts, cen[1] += P[1] / m.ref.nverts;
+    }
+    if (nx * (Pa[0] - cen[0]) + ny * (Pa[1] - cen[1]) < 0.)
+      nx = -nx, ny = -ny;
+    fg.normal[0] = nx + 0., fg.normal[1] = ny + 0.;
+    const Quadrature q = line_rule(order);
+    for (size_t qp = 0; qp < q.w.size(); ++qp) {
+      const double t = q.x[qp][0];
+      std::array<double, 3> xh = {0, 0, 0};
+      for (int c = 0; c < 2; ++c)
+        xh[c] = (1. - t) * m.ref.verts[a][c] + t * m.ref.verts[b][c];
+      fg.xin.push_back(xh);
+      if (gout) {
+        double X[3], xo[3] = {0, 0, 0};
+        gin.global(xh.data(), X);
+        gout->local(X, xo);
+        fg.xout.push_back({xo[0], xo[1], 0.});
+      }
+      fg.w.push_back(q.w[qp]);
+    }
+    (void)e, (void)o;
+    return fg;
+  }
+
+  // LocalCouplingIntersectionIntegralBilinearForm::apply2 (integrals.hh:197-267) with
+  // InnerCoupling (local/integrands/laplace-ipdg.hh:107-186) or InnerPenalty (local/integrands/ipdg.hh:113-171)
+  void coupling_local(const FaceForm& F, int64_t e, int64_t o, const FaceGeo& fg, BoundBasis& bin,
+                      BoundBasis& bout, std::vector<double> blocks[4]) const
+  {
+    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    for (int k = 0; k < 4; ++k)
+      blocks[k].assign(ls * ls, 0.);
+    std::vector<double> iv[4];
+    for (int k = 0; k < 4; ++k)
+      iv[k].assign(ls * ls, 0.);
+    const double* n = fg.normal;
+    const double kin = F.kappa.at(e), kout = F.kappa.at(o);
+    const double win = F.omega.at(e), wout = F.omega.at(o);
+    for (size_t qp = 0; qp < fg.w.size(); ++qp) {
+      for (int k = 0; k < 4; ++k)
+        std::fill(iv[k].begin(), iv[k].end(), 0.);
+      bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
+      bout.evaluate(fg.xout[qp].data()), bout.jacobians(fg.xout[qp].data());
+      // delta = n . (omega n)
+      double delta_plus = 0., delta_minus = 0.;
+      for (int a = 0; a < d; ++a)
+        delta_plus += n[a] * (wout * n[a]), delta_minus += n[a] * (win * n[a]);
+      if (F.kind == FF_INNER_COUPLING) {
+        const double s = F.param;
+        const double weight_minus = delta_plus / (delta_plus + delta_minus);
+        const double weight_plus = delta_minus / (delta_plus + delta_minus);
+        for (int i = 0; i < ls; ++i) {
+          const double kgt_in = kgrad_dot(kin, &bin.grads[3 * i], n, d);
+          for (int j = 0; j < ls; ++j) {
+            iv[0][i * ls + j] += -1.0 * weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
+            iv[0][i * ls + j] += -1.0 * s * weight_minus * bin.vals[j] * kgt_in;
+          }
+          for (int j = 0; j < ls; ++j) {
+            iv[1][i * ls + j] += -1.0 * weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bin.vals[i];
+            iv[1][i * ls + j] += s * weight_minus * bout.vals[j] * kgt_in;
+          }
+        }
+        for (int i = 0; i < ls; ++i) {
+          const double kgt_out = kgrad_dot(kout, &bout.grads[3 * i], n, d);
+          for (int j = 0; j < ls; ++j) {
+            iv[2][i * ls + j] += weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bout.vals[i];
+            iv[2][i * ls + j] += -1.0 * s * weight_plus * bin.vals[j] * kgt_out;
+          }
+          for (int j = 0; j < ls; ++j) {
+            iv[3][i * ls + j] += weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bout.vals[i];
+            iv[3][i * ls + j] += s * weight_plus * bout.vals[j] * kgt_out;
+          }
+        }
+      } else {
+        const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus); // half harmonic average
+        const double penalty = (F.param * weight) / fg.diameter;
+        for (int i = 0; i < ls; ++i) {
+          for (int j = 0; j < ls; ++j)
+            iv[0][i * ls + j] += penalty * bin.vals[j] * bin.vals[i];
+          for (int j = 0; j < ls; ++j)
+            iv[1][i * ls + j] += -1.0 * penalty * bout.vals[j] * bin.vals[i];
+        }
+        for (int i = 0; i < ls; ++i) {
+          for (int j = 0; j < ls; ++j)
+            iv[2][i * ls + j] += -1.0 * penalty * bin.vals[j] * bout.vals[i];
+          for (int j = 0; j < ls; ++j)
+            iv[3][i * ls + j] += penalty * bout.vals[j] * bout.vals[i];
+        }
+      }
+      for (int k = 0; k < 4; ++k)
+        for (int t = 0; t < ls * ls; ++t)
+          blocks[k][t] += iv[k][t] * fg.integration_element * fg.w[qp];
+    }
+  }
+
+  // LocalIntersectionIntegralBilinearForm::apply2 (integrals.hh:338-369) with
+  // DirichletCoupling (laplace-ipdg.hh:340-368) or BoundaryPenalty (ipdg.hh:254-282)
+  void boundary_local(const FaceForm& F, int64_t e, const FaceGeo& fg, BoundBasis& bin,
+                      std::vector<double>& result) const
+  {
+    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    result.assign(ls * ls, 0.);
+    std::vector<double> iv(ls * ls);
+    const double* n = fg.normal;
+    const double kin = F.kappa.at(e), win = F.omega.at(e);
+    for (size_t qp = 0; qp < fg.w.size(); ++qp) {
+      std::fill(iv.begin(), iv.end(), 0.);
+      bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
+      if (F.kind == FF_DIRICHLET_COUPLING) {
+        const double s = F.param;
+        for (int i = 0; i < ls; ++i) {
+          const double kgt = kgrad_dot(kin, &bin.grads[3 * i], n, d);
+          for (int j = 0; j < ls; ++j) {
+            iv[i * ls + j] += -1.0 * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
+            iv[i * ls + j] += -1.0 * s * bin.vals[j] * kgt;
+          }
+        }
+      } else {
+        double nwn = 0.;
+        for (int a = 0; a < d; ++a)
+          nwn += n[a] * (win * n[a]);
+        const double penalty = (F.param * nwn) / fg.diameter;
+        for (int i = 0; i < ls; ++i)
+          for (int j = 0; j < ls; ++j)
+            iv[i * ls + j] += penalty * bin.vals[j] * bin.vals[i];
+      }
+      for (int t = 0; t < ls * ls; ++t)
+        result[t] += iv[t] * fg.integration_element * fg.w[qp];
+    }
+  }
+
+  static double esv2007_force(const double* x)
+  {
+    return M_PI_2 * M_PI * std::cos(M_PI_2 * x[0]) * std::cos(M_PI_2 * x[1]);
+  }
+
+  // LocalElementIntegralFunctional::apply (local/functionals/integrals.hh:72-97) with
+  // LocalProductIntegrand().with_ansatz(f) (local/integrands/conversion.hh:90-117)
+  void source_local(const Source& S, int64_t e, const Geometry& geo, BoundBasis& basis,
+                    std::vector<double>& result) const
+  {
+    const int ls = space->fe.size;
+    result.assign(ls, 0.);
+    const int order = 0 + space->fe.order + S.f_order + S.over_integrate;
+    const Quadrature q = element_rule(space->mesh->ref, order);
+    for (size_t qp = 0; qp < q.w.size(); ++qp) {
+      const double* xh = q.x[qp].data();
+      basis.evaluate(xh);
+      double f;
+      if (S.kind == 0) {
+        f = S.f.at(e);
+      } else {
+        double X[3];
+        geo.global(xh, X);
+        f = esv2007_force(X);
+      }
+      for (int i = 0; i < ls; ++i)
+        result[i] += ((1. * basis.vals[i]) * f) * geo.integration_element * q.w[qp];
+    }
+  }
+
+  void assemble(int nthreads)
+  {
+    const Mesh& m = *space->mesh;
+    const int ls = space->fe.size, nf = m.ref.nfaces;
+    std::fill(values.begin(), values.end(), 0.);
+    std::fill(rhs.begin(), rhs.end(), 0.);
+    const bool atomic = nthreads > 1;
+    std::string err;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+      // per-thread copies, as walk(use_tbb=true) copies functors per thread
+      BoundBasis basis, bout;
+      std::vector<double> local, lvec, blocks[4];
+#pragma omp for schedule(static)
+      for (int64_t e = 0; e < m.ne; ++e) {
+        try {
+          const Geometry geo = m.geometry(e);
+          basis.bind(&space->fe, &geo);
+          const int32_t* gi = &space->indices[e * ls];
+          for (const auto& F : eforms) {
+            element_local(F, e, geo, basis, local);
+            for (int i = 0; i < ls; ++i)
+              for (int j = 0; j < ls; ++j)
+                add_to_entry(gi[i], gi[j], F.scaling * local[i * ls + j], atomic);
+          }
+          for (const auto& S : sources) {
+            source_local(S, e, geo, basis, lvec);
+            for (int i = 0; i < ls; ++i) {
+              if (atomic) {
+#pragma omp atomic
+                rhs[gi[i]] += lvec[i];
+              } else {
+                rhs[gi[i]] += lvec[i];
+              }
+            }
+          }
+          for (int f = 0; f < nf; ++f) {
+            const int64_t o = m.nb[e * nf + f];
+            if (o >= 0 && e < o && !cforms.empty()) {
+              // ApplyOn::InnerIntersectionsOnce: visited from the element with the smaller index
+              const Geometry gout = m.geometry(o);
+              bout.bind(&space->fe, &gout);
+              const int32_t* gj = &space->indices[o * ls];
+              for (const auto& F : cforms) {
+                const int order = 0 + 0 + space->fe.order + space->fe.order + F.over_integrate;
+                const FaceGeo fg = face_geometry(e, f, o, geo, &gout, order);
+                coupling_local(F, e, o, fg, basis, bout, blocks);
+                // four scatter loops, bilinear-form-assemblers.hh:262-277
+                for (int i = 0; i < ls; ++i) {
+                  for (int j = 0; j < ls; ++j)
+                    add_to_entry(gi[i], gi[j], F.scaling * blocks[0][i * ls + j], atomic);
+                  for (int j = 0; j < ls; ++j)
+                    add_to_entry(gi[i], gj[j], F.scaling * blocks[1][i * ls + j], atomic);
+                }
+                for (int i = 0; i < ls; ++i) {
+                  for (int j = 0; j < ls; ++j)
+                    add_to_entry(gj[i], gi[j], F.scaling * blocks[2][i * ls + j], atomic);
+                  for (int j = 0; j < ls; ++j)
+                    add_to_entry(gj[i], gj[j], F.scaling * blocks[3][i * ls + j], atomic);
+                }
+              }
+            } else if (o < 0 && !bforms.empty()) {
+              for (const auto& F : bforms) {
+                const int order = 0 + space->fe.order + space->fe.order + F.over_integrate;
+                const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, order);
+                boundary_local(F, e, fg, basis, local);
+                for (int i = 0; i < ls; ++i)
+                  for (int j = 0; j < ls; ++j)
+                    add_to_entry(gi[i], gi[j], F.scaling * local[i * ls + j], atomic);
+              }
+            }
+          }
+        } catch (const std::exception& ex) {
+#pragma omp critical
+          err = ex.what();
+        }
+      }
+    }
+    if (!err.empty())
+      throw Error(err);
+    if (dirichlet)
+      apply_dirichlet();
+  }
+
+  // DirichletConstraints::apply_local (tools/dirichlet-constraints.hh:85-110) + apply(matrix, vector) with
+  // ensure_symmetry (:153-177): unit_col then unit_row for every DoF, vector entry cleared
+  void apply_dirichlet()
+  {
+    const Mesh& m = *space->mesh;
+    const int ls = space->fe.size, nf = m.ref.nfaces;
+    std::set<int32_t> dofs;
+    if (space->continuous) {
+      for (int64_t e = 0; e < m.ne; ++e)
+        for (int f = 0; f < nf; ++f) {
+          if (m.nb[e * nf + f] >= 0)
+            continue;
+          const auto& fv = m.ref.faces[f];
+          for (int i = 0; i < ls; ++i) {
+            bool on_face = true;
+            for (int lv : space->fe.node_subentity[i])
+              if (std::find(fv.begin(), fv.end(), lv) == fv.end())
+                on_face = false;
+            if (on_face)
+              dofs.insert(space->indices[e * ls + i]);
+          }
+        }
+    }
+    dirichlet_dofs.assign(dofs.begin(), dofs.end());
+    std::vector<char> mask(n, 0);
+    for (int32_t d : dirichlet_dofs)
+      mask[d] = 1;
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+        const int32_t j = cols[k];
+        if (mask[i] || mask[j])
+          values[k] = (i == j) ? 1. : 0.;
+      }
+    for (int32_t d : dirichlet_dofs)
+      rhs[d] = 0.;
+  }
+};
+
+template <class F>
+int guarded(F&& f)
+{
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+} // namespace
+
+struct OrMesh : Mesh {};
+struct OrSpace : Space {};
+struct OrOperator : Operator {};
+
+extern "C" {
+
+const char* or_last_error(void)
+{
+  return g_error.c_str();
+}
+
+OrMesh* or_mesh_structured(int geom, const int32_t* cells, const double* lo, const double* hi)
+{
+  OrMesh* out = nullptr;
+  guarded([&] {
+    Mesh* m = structured_mesh(geom, cells, lo, hi);
+    out = new OrMesh;
+    static_cast<Mesh&>(*out) = std::move(*m);
+    delete m;
+  });
+  return out;
+}
+
+OrMesh* or_mesh_create(int geom, int64_t nv, const double* coords, int64_t ne, const int32_t* conn)
+{
+  OrMesh* out = nullptr;
+  guarded([&] {
+    auto* m = new OrMesh;
+    m->ref = make_ref(geom);
+    m->nv = nv, m->ne = ne;
+    m->coords.assign(coords, coords + nv * m->ref.dim);
+    m->conn.assign(conn, conn + ne * m->ref.nverts);
+    for (int64_t k = 0; k < ne * m->ref.nverts; ++k)
+      if (m->conn[k] < 0 || m->conn[k] >= nv) {
+        delete m;
+        throw Error("connectivity references a vertex out of range");
+      }
+    m->check_affine();
+    m->build_neighbors();
+    out = m;
+  });
+  return out;
+}
+
+void or_mesh_destroy(OrMesh* m)
+{
+  delete m;
+}
+int64_t or_mesh_num_vertices(const OrMesh* m)
+{
+  return m->nv;
+}
+int64_t or_mesh_num_elements(const OrMesh* m)
+{
+  return m->ne;
+}
+int or_mesh_dim(const OrMesh* m)
+{
+  return m->ref.dim;
+}
+int or_mesh_verts_per_element(const OrMesh* m)
+{
+  return m->ref.nverts;
+}
+void or_mesh_coords(const OrMesh* m, double* out)
+{
+  std::memcpy(out, m->coords.data(), m->coords.size() * sizeof(double));
+}
+void or_mesh_conn(const OrMesh* m, int32_t* out)
+{
+  std::memcpy(out, m->conn.data(), m->conn.size() * sizeof(int32_t));
+}
+int or_mesh_faces_per_element(const OrMesh* m)
+{
+  return m->ref.nfaces;
+}
+void or_mesh_neighbors(const OrMesh* m, int32_t* out)
+{
+  std::memcpy(out, m->nb.data(), m->nb.size() * sizeof(int32_t));
+}
+
+OrSpace* or_space_create(OrMesh* m, int continuous, int order)
+{
+  OrSpace* out = nullptr;
+  guarded([&] {
+    auto* s = new OrSpace;
+    try {
+      s->init(m, continuous != 0, order);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    out = s;
+  });
+  return out;
+}
+void or_space_destroy(OrSpace* s)
+{
+  delete s;
+}
+int64_t or_space_size(const OrSpace* s)
+{
+  return s->size;
+}
+int or_space_local_size(const OrSpace* s)
+{
+  return s->fe.size;
+}
+void or_space_global_indices(const OrSpace* s, int32_t* out)
+{
+  std::memcpy(out, s->indices.data(), s->indices.size() * sizeof(int32_t));
+}
+
+OrOperator* or_operator_create(OrSpace* s, int stencil)
+{
+  OrOperator* out = nullptr;
+  guarded([&] {
+    auto* op = new OrOperator;
+    op->space = s, op->stencil = stencil;
+    op->build_pattern();
+    out = op;
+  });
+  return out;
+}
+void or_operator_destroy(OrOperator* op)
+{
+  delete op;
+}
+int64_t or_operator_nnz(const OrOperator* op)
+{
+  return op->row_ptr.back();
+}
+void or_operator_pattern(const OrOperator* op, int64_t* row_ptr, int32_t* cols)
+{
+  std::memcpy(row_ptr, op->row_ptr.data(), op->row_ptr.size() * sizeof(int64_t));
+  std::memcpy(cols, op->cols.data(), op->cols.size() * sizeof(int32_t));
+}
+
+int or_operator_append_laplace(OrOperator* op, const double* kappa, double kappa_const, int over_integrate,
+                               double scaling)
+{
+  return guarded([&] {
+    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling};
+    F.coeff.set(kappa, kappa_const, op->space->mesh->ne);
+    op->eforms.push_back(F);
+  });
+}
+int or_operator_append_product(OrOperator* op, const double* weight, double weight_const, int over_integrate,
+                               double scaling)
+{
+  return guarded([&] {
+    ElementForm F{EF_PRODUCT, {}, over_integrate, scaling};
+    F.coeff.set(weight, weight_const, op->space->mesh->ne);
+    op->eforms.push_back(F);
+  });
+}
+static int append_face(OrOperator* op, int kind, double param, const double* kappa, double kc, const double* omega,
+                       double oc, int over_integrate, double scaling)
+{
+  return guarded([&] {
+    FaceForm F{kind, param, {}, {}, over_integrate, scaling};
+    F.kappa.set(kappa, kc, op->space->mesh->ne);
+    F.omega.set(omega, oc, op->space->mesh->ne);
+    if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY)
+      op->cforms.push_back(F);
+    else
+      op->bforms.push_back(F);
+  });
+}
+int or_operator_append_inner_coupling(OrOperator* op, double s, const double* kappa, double kc, const double* omega,
+                                      double oc, int over_integrate, double scaling)
+{
+  return append_face(op, FF_INNER_COUPLING, s, kappa, kc, omega, oc, over_integrate, scaling);
+}
+int or_operator_append_inner_penalty(OrOperator* op, double penalty, const double* omega, double oc,
+                                     int over_integrate, double scaling)
+{
+  return append_face(op, FF_INNER_PENALTY, penalty, nullptr, 1., omega, oc, over_integrate, scaling);
+}
+int or_operator_append_dirichlet_coupling(OrOperator* op, double s, const double* kappa, double kc,
+                                          int over_integrate, double scaling)
+{
+  return append_face(op, FF_DIRICHLET_COUPLING, s, kappa, kc, nullptr, 1., over_integrate, scaling);
+}
+int or_operator_append_boundary_penalty(OrOperator* op, double penalty, const double* omega, double oc,
+                                        int over_integrate, double scaling)
+{
+  return append_face(op, FF_BOUNDARY_PENALTY, penalty, nullptr, 1., omega, oc, over_integrate, scaling);
+}
+int or_operator_append_source(OrOperator* op, int kind, const double* f, double f_const, int f_order,
+                              int over_integrate)
+{
+  return guarded([&] {
+    if (kind != 0 && kind != 1)
+      throw Error("unknown source kind");
+    Source S{kind, {}, f_order, over_integrate};
+    S.f.set(f, f_const, op->space->mesh->ne);
+    op->sources.push_back(S);
+  });
+}
+int or_operator_set_dirichlet(OrOperator* op, int enabled)
+{
+  op->dirichlet = enabled != 0;
+  return 0;
+}
+int or_operator_assemble(OrOperator* op, int nthreads)
+{
+  return guarded([&] { op->assemble(nthreads); });
+}
+void or_operator_values(const OrOperator* op, double* out)
+{
+  std::memcpy(out, op->values.data(), op->values.size() * sizeof(double));
+}
+void or_operator_rhs(const OrOperator* op, double* out)
+{
+  std::memcpy(out, op->rhs.data(), op->rhs.size() * sizeof(double));
+}
+int64_t or_operator_num_dirichlet(const OrOperator* op)
+{
+  return int64_t(op->dirichlet_dofs.size());
+}
+void or_operator_dirichlet_dofs(const OrOperator* op, int32_t* out)
+{
+  std::memcpy(out, op->dirichlet_dofs.data(), op->dirichlet_dofs.size() * sizeof(int32_t));
+}
+
+// ConstMatrixOperator::apply = matrix_.mv (operators/matrix-based.hh:121-129): row-wise sum in column order
+void or_spmv(int64_t n, const int64_t* row_ptr, const int32_t* cols, const double* vals, const double* x, double* y,
+             int nthreads)
+{
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.;
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+      s += vals[k] * x[cols[k]];
+    y[i] = s;
+  }
+}
+
+static double par_dot(int64_t n, const double* a, const double* b, int nthreads)
+{
+  double s = 0.;
+  if (nthreads <= 1) {
+    for (int64_t i = 0; i < n; ++i)
+      s += a[i] * b[i];
+    return s;
+  }
+#pragma omp parallel for reduction(+ : s) schedule(static) num_threads(nthreads)
+  for (int64_t i = 0; i < n; ++i)
+    s += a[i] * b[i];
+  return s;
+}
+
+// apply_inverse with a Jacobi-preconditioned conjugate gradient method
+int or_pcg_jacobi(int64_t n, const int64_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
+                  double* x, double rtol, int64_t max_iter, int nthreads, int64_t max_iters_run, int64_t* iters,
+                  double* relres)
+{
+  const int nt = nthreads > 0 ? nthreads : 1;
+  std::vector<double> dinv(n), r(n), z(n), p(n), q(n);
+  for (int64_t i = 0; i < n; ++i) {
+    double d = 0.;
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+      if (cols[k] == i)
+        d = vals[k];
+    if (!(d > 0.)) {
+      g_error = "Jacobi-PCG needs a positive diagonal (matrix not SPD)";
+      return -2;
+    }
+    dinv[i] = 1. / d;
+  }
+  or_spmv(n, row_ptr, cols, vals, x, q.data(), nt);
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = b[i] - q[i];
+    z[i] = dinv[i] * r[i];
+    p[i] = z[i];
+  }
+  const double bb = par_dot(n, b, b, nt);
+  double rr = par_dot(n, r.data(), r.data(), nt);
+  double rz = par_dot(n, r.data(), z.data(), nt);
+  const double tol2 = rtol * rtol * bb;
+  int64_t k = 0;
+  int status = 1;
+  if (rr <= tol2) {
+    status = 0;
+  } else {
+    for (k = 1; k <= max_iter; ++k) {
+      or_spmv(n, row_ptr, cols, vals, p.data(), q.data(), nt);
+      const double pq = par_dot(n, p.data(), q.data(), nt);
+      const double alpha = rz / pq;
+#pragma omp parallel for schedule(static) num_threads(nt)
+      for (int64_t i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      rr = par_dot(n, r.data(), r.data(), nt);
+      if (rr <= tol2) {
+        status = 0;
+        break;
+      }
+      if (max_iters_run > 0 && k >= max_iters_run)
+        break;
+#pragma omp parallel for schedule(static) num_threads(nt)
+      for (int64_t i = 0; i < n; ++i)
+        z[i] = dinv[i] * r[i];
+      const double rz_new = par_dot(n, r.data(), z.data(), nt);
+      const double beta = rz_new / rz;
+      rz = rz_new;
+#pragma omp parallel for schedule(static) num_threads(nt)
+      for (int64_t i = 0; i < n; ++i)
+        p[i] = z[i] + beta * p[i];
+    }
+    if (k > max_iter)
+      k = max_iter;
+  }
+  *iters = k;
+  *relres = bb > 0. ? std::sqrt(rr / bb) : 0.;
+  if (status != 0 && !(max_iters_run > 0 && k >= max_iters_run))
+    g_error = "Jacobi-PCG did not converge";
+  return status;
+}
+
+void or_laplace_evaluate(int d, int r, int rows, int cols, const double* weight, const double* test_grads,
+                         const double* ansatz_grads, double* result)
+{
+  // local/integrands/laplace.hh:98-101
+  for (int ii = 0; ii < rows; ++ii)
+    for (int jj = 0; jj < cols; ++jj) {
+      double acc = 0.;
+      for (int rr = 0; rr < r; ++rr) {
+        const double* ga = &ansatz_grads[(jj * r + rr) * d];
+        const double* gt = &test_grads[(ii * r + rr) * d];
+        double wg[3];
+        for (int a = 0; a < d; ++a) {
+          double s = 0.;
+          for (int c = 0; c < d; ++c)
+            s += weight[a * d + c] * ga[c];
+          wg[a] = s;
+        }
+        acc += dot(wg, gt, d);
+      }
+      result[ii * cols + jj] = acc;
+    }
+}
+
+void or_product_evaluate(int r, int rows, int cols, const double* weight, const double* test_values,
+                         const double* ansatz_values, double* result)
+{
+  // local/integrands/product.hh:126-128: (weight * test) * ansatz
+  for (int ii = 0; ii < rows; ++ii)
+    for (int jj = 0; jj < cols; ++jj) {
+      double wt[8];
+      for (int a = 0; a < r; ++a) {
+        double s = 0.;
+        for (int c = 0; c < r; ++c)
+          s += weight[a * r + c] * test_values[ii * r + c];
+        wt[a] = s;
+      }
+      result[ii * cols + jj] = dot(wt, &ansatz_values[jj * r], r);
+    }
+}
+
+int or_quadrature(int geom, int order, double* points, double* weights, int capacity)
+{
+  int count = -1;
+  guarded([&] {
+    const RefElement ref = make_ref(geom);
+    const Quadrature q = element_rule(ref, order);
+    count = int(q.w.size());
+    if (count > capacity)
+      return;
+    for (int i = 0; i < count; ++i) {
+      for (int a = 0; a < ref.dim; ++a)
+        points[i * ref.dim + a] = q.x[i][a];
+      weights[i] = q.w[i];
+    }
+  });
+  return count;
+}
+
+double or_esv2007_h1_semi_error(const OrSpace* s, const double* u, int quad_order)
+{
+  const Mesh& m = *s->mesh;
+  const int ls = s->fe.size, d = m.ref.dim;
+  const Quadrature q = element_rule(m.ref, quad_order);
+  double total = 0.;
+  BoundBasis basis;
+  for (int64_t e = 0; e < m.ne; ++e) {
+    const Geometry geo = m.geometry(e);
+    basis.bind(&s->fe, &geo);
+    double acc = 0.;
+    for (size_t qp = 0; qp < q.w.size(); ++qp) {
+      basis.jacobians(q.x[qp].data());
+      double X[3];
+      geo.global(q.x[qp].data(), X);
+      double gu[2] = {-M_PI_2 * std::sin(M_PI_2 * X[0]) * std::cos(M_PI_2 * X[1]),
+                      -M_PI_2 * std::cos(M_PI_2 * X[0]) * std::sin(M_PI_2 * X[1])};
+      for (int i = 0; i < ls; ++i)
+        for (int a = 0; a < d; ++a)
+          gu[a] -= u[s->indices[e * ls + i]] * basis.grads[3 * i + a];
+      acc += (gu[0] * gu[0] + gu[1] * gu[1]) * geo.integration_element * q.w[qp];
+    }
+    total += acc;
+  }
+  return std::sqrt(total);
+}
+
+int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const double* u, const double* lambda,
+                                  double* grad)
+{
+  return guarded([&] {
+    Operator op;
+    op.space = const_cast<OrSpace*>(s);
+    const Mesh& m = *s->mesh;
+    const int ls = s->fe.size;
+    ElementForm F{EF_LAPLACE, {}, 0, 1.};
+    F.coeff.constant = 1.;
+#pragma omp parallel
+    {
+      BoundBasis basis;
+      std::vector<double> local;
+#pragma omp for schedule(static)
+      for (int64_t e = 0; e < m.ne; ++e) {
+        const Geometry geo = m.geometry(e);
+        basis.bind(&s->fe, &geo);
+        op.element_local(F, e, geo, basis, local);
+        const int32_t* gi = &s->indices[e * ls];
+        double acc = 0.;
+        for (int i = 0; i < ls; ++i) {
+          double row = 0.;
+          for (int j = 0; j < ls; ++j)
+            row += local[i * ls + j] * u[gi[j]];
+          acc += lambda[gi[i]] * row;
+        }
+        grad[e] = -kappa[e] * acc;
+      }
+    }
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,301 @@
+"""CPU tests that pin the oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+Each test names what pins it: a reference test's numbers (file:line), an exact value the
+mathematics fixes (closed forms, rational brute force in tests/golden/), an invariant, or an
+independent library routine (scipy).
+"""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle_lib as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))
+
+
+def p2_stats(geom, kind):
+    m = O.Mesh.structured(geom, [9, 2], [0, 0], [3, 1])
+    s = O.Space(m, True, 2)
+    op = O.Operator(s, O.STENCIL_ELEMENT)
+    (op.append_laplace if kind == "laplace" else op.append_product)(1.0)
+    op.assemble(4)  # assemble(true) in the reference test: thread-parallel
+    A = op.csr().toarray()
+    return A.min(), A.max(), float((A * A).sum())
+
+
+@pytest.mark.parametrize("geom,kind", [(O.QUAD, "laplace"), (O.TRI, "laplace"), (O.QUAD, "mass"), (O.TRI, "mass")])
+def test_p2_known_answers(geom, kind):
+    """dune/gdt/test/integrands/integrands_laplace.cc:115-135, integrands_product.cc:107-126"""
+    name = f"p2_{kind}_{'cube' if geom == O.QUAD else 'simplex'}"
+    ref = GOLDEN["reference"][name]
+    exact = {k: float(Fraction(*v)) for k, v in GOLDEN["exact"][name].items()}
+    mn, mx, ss = p2_stats(geom, kind)
+    tmin, tmax, tss = ref["tol"]
+    assert abs(mn - ref["min"]) <= tmin
+    assert abs(mx - ref["max"]) <= tmax
+    # exact rational value (tests/golden/make_golden.py): the oracle is within the reference's tolerance of it
+    assert abs(mn - exact["min"]) <= tmin and abs(mx - exact["max"]) <= tmax
+    assert abs(ss - exact["sumsq"]) <= tss
+    if name == "p2_laplace_simplex":
+        # the reference's printed 2481.524691358029 is itself 4.5e-12 above the exact 402007/162;
+        # the oracle is 1.8e-12 below it, so the two differ by 6.3e-12 (> the reference's 5e-12)
+        assert abs(ss - ref["sumsq"]) <= 1e-11
+    else:
+        assert abs(ss - ref["sumsq"]) <= tss
+
+
+def test_pointwise_laplace_integrand():
+    """dune/gdt/test/integrands/integrands_laplace.cc:71-113 with the bases of integrands.hh:72-141"""
+    rng = np.random.default_rng(0)
+    for x0, x1 in rng.random((5, 2)):
+        W = np.array([[x0, x1], [1.0, 2.0]]) * x0 * x1
+        # scalar bases: ansatz {x, x^2 y}, test {y, x y^3}
+        ag = np.array([[[1.0, 0.0]], [[2 * x0 * x1, x0 ** 2]]])
+        tg = np.array([[[0.0, 1.0]], [[x1 ** 3, 3 * x0 * x1 ** 2]]])
+        res = O.laplace_evaluate(W, tg, ag)
+        expected = np.array([[1, 2 * (x0 * x1 + x0 ** 2)],
+                             [x0 * x1 ** 3 + 3 * x0 * x1 ** 2,
+                              3 * x0 ** 2 * x1 ** 4 + 6 * (x0 ** 2 * x1 ** 3 + x0 ** 3 * x1 ** 2)]]) * x0 * x1
+        np.testing.assert_allclose(res, expected, rtol=4 * np.finfo(float).eps, atol=0)
+        # vector bases: ansatz {(x,y), (x^2,y^2)}, test {(1,2), (xy, x+y)}
+        ag = np.array([[[1, 0], [0, 1]], [[2 * x0, 0], [0, 2 * x1]]], float)
+        tg = np.array([[[0, 0], [0, 0]], [[x1, x0], [1, 1]]], float)
+        res = O.laplace_evaluate(W, tg, ag)
+        expected_row1 = np.array([x0 * x1 + x0 + x1 + 2,
+                                  2 * (x0 ** 2 * x1 + x1 ** 2 + x0 ** 2 + 2 * x1)]) * x0 * x1
+        np.testing.assert_allclose(res[0], 0.0, atol=0)
+        np.testing.assert_allclose(res[1], expected_row1, rtol=4 * np.finfo(float).eps, atol=0)
+
+
+def test_pointwise_product_integrand():
+    """dune/gdt/test/integrands/integrands_product.cc:58-105"""
+    rng = np.random.default_rng(1)
+    for x0, x1 in rng.random((5, 2)):
+        # scalar: weight x y, test {y, x y^3}, ansatz {x, x^2 y}
+        res = O.product_evaluate(np.array([[x0 * x1]]), np.array([[x1], [x0 * x1 ** 3]]),
+                                 np.array([[x0], [x0 ** 2 * x1]]))
+        expected = np.array([[(x0 * x1) ** 2, (x0 * x1) ** 3], [x0 ** 3 * x1 ** 4, x0 ** 4 * x1 ** 5]])
+        np.testing.assert_allclose(res, expected, rtol=4 * np.finfo(float).eps)
+        # vector: weight [[x, y], [1, 2]], test {(1,2), (xy, x+y)}, ansatz {(x,y), (x^2,y^2)}
+        W = np.array([[x0, x1], [1.0, 2.0]])
+        tv = np.array([[1.0, 2.0], [x0 * x1, x0 + x1]])
+        av = np.array([[x0, x1], [x0 ** 2, x1 ** 2]])
+        res = O.product_evaluate(W, tv, av)
+        expected = np.array([
+            [x0 ** 2 + 2 * x0 * x1 + 5 * x1, x0 ** 3 + 2 * x0 ** 2 * x1 + 5 * x1 ** 2],
+            [x0 ** 3 * x1 + x0 ** 2 * x1 + 2 * x0 * x1 ** 2 + 2 * x0 * x1 + 2 * x1 ** 2,
+             x0 ** 4 * x1 + x0 ** 3 * x1 + x0 ** 2 * x1 ** 2 + x0 * x1 ** 3 + 2 * x0 * x1 ** 2 + 2 * x1 ** 3]])
+        np.testing.assert_allclose(res, expected, rtol=8 * np.finfo(float).eps)
+
+
+def swipdg_operator(n, sigma_inner=8.0, sigma_dirichlet=14.0, kappa=1.0, omega=1.0, source=None):
+    m = O.Mesh.structured(O.QUAD, [n, n], [-1, -1], [1, 1])
+    s = O.Space(m, False, 1)
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(kappa)
+    op.append_inner_coupling(1.0, kappa, omega)
+    op.append_inner_penalty(sigma_inner, omega)
+    op.append_dirichlet_coupling(1.0, kappa)
+    op.append_boundary_penalty(sigma_dirichlet, omega)
+    if source is None:
+        op.append_source(esv2007=True, f_order=4)
+    else:
+        op.append_source(source)
+    op.assemble(1)
+    return m, s, op
+
+
+def test_esv2007_swipdg_h1_errors():
+    """stationary_heat_equation__ESV2007__table_1.mini:31-33 (cube grids, Q1-DG, sigma 8 / 14,
+    h_F = face volume, force of order 4, ESV2007.hh:51-61,107-113)"""
+    ref = GOLDEN["reference"]["esv2007_q1dg_cube"]
+    for n, h_ref, e_ref in zip((8, 16, 32), ref["h"], ref["h1_semi"]):
+        m, s, op = swipdg_operator(n)
+        rp, cols = op.pattern()
+        x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-12)
+        assert rc == 0
+        err = O.esv2007_h1_semi_error(s, x, 10)
+        assert float(f"{err:.2e}") == e_ref
+        assert float(f"{2 * math.sqrt(2) / n:.2e}") == h_ref
+
+
+def test_swipdg_structure():
+    """SWIPDG matrix: symmetric; A 1 only sees the boundary penalty (consistency terms of a constant
+    vanish, inner jumps of a constant vanish); nnz = 16 E + 32 inner faces (sparsity-pattern.hh:126-140)"""
+    n = 6
+    m, s, op = swipdg_operator(n, source=0.0)
+    A = op.csr()
+    assert abs(A - A.T).max() < 1e-14 * abs(A).max()
+    assert op.nnz == 16 * n * n + 32 * 2 * n * (n - 1)
+    ones = np.ones(A.shape[0])
+    m2, s2, op2 = swipdg_operator(n, source=0.0)
+    # boundary-only operator: Dirichlet coupling + penalty alone
+    mb = O.Mesh.structured(O.QUAD, [n, n], [-1, -1], [1, 1])
+    sb = O.Space(mb, False, 1)
+    ob = O.Operator(sb, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    ob.append_dirichlet_coupling(1.0, 1.0)
+    ob.append_boundary_penalty(14.0, 1.0)
+    ob.assemble(1)
+    np.testing.assert_allclose(A @ ones, ob.csr() @ ones, atol=1e-12)
+
+
+def q1_heat(n, kappa=1.0, f=1.0, dirichlet=True):
+    m = O.Mesh.structured(O.QUAD, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, True, 1)
+    op = O.Operator(s, O.STENCIL_ELEMENT)
+    op.append_laplace(kappa)
+    op.append_source(f)
+    op.set_dirichlet(dirichlet)
+    op.assemble(1)
+    return m, s, op
+
+
+def test_q1_stencil_closed_form():
+    """Q1 Laplace on squares: interior row = 8/3 and eight -1/3; b_i = h^2 for f = 1"""
+    n = 8
+    m, s, op = q1_heat(n, dirichlet=False)
+    A = op.csr()
+    b = op.rhs()
+    i = 4 * (n + 1) + 4
+    row = A.getrow(i).toarray().ravel()
+    nz = np.nonzero(row)[0]
+    assert len(nz) == 9
+    np.testing.assert_allclose(row[i], 8.0 / 3.0, rtol=1e-15)
+    np.testing.assert_allclose(row[nz[nz != i]], -1.0 / 3.0, rtol=1e-14)
+    np.testing.assert_allclose(b[i], 1.0 / n ** 2, rtol=1e-14)
+    np.testing.assert_allclose(A @ np.ones(A.shape[0]), 0.0, atol=1e-14)  # row sums before constraints
+    assert op.nnz == (3 * n + 1) ** 2
+
+
+def test_mass_and_volume():
+    """sum of all mass-matrix entries = |Omega| (Q1 2d, P1 3d Kuhn)"""
+    for geom, cells, dim in ((O.QUAD, [5, 3], 2), (O.TET, [3, 2, 2], 3), (O.TRI, [4, 5], 2)):
+        m = O.Mesh.structured(geom, cells, [0] * dim, [1, 2, 3][:dim])
+        s = O.Space(m, True, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_product(1.0)
+        op.assemble(1)
+        np.testing.assert_allclose(op.values().sum(), np.prod([1, 2, 3][:dim]), rtol=1e-13)
+
+
+def test_p1_tet_closed_form():
+    """P1 tet: A = kappa |T| G G^T (G rows grad lambda_i), M = |T|/20 (1 + delta_ij)"""
+    coords = np.array([[0.1, 0.2, 0.0], [1.3, 0.1, 0.2], [0.2, 0.9, 0.1], [0.3, 0.4, 1.1]])
+    conn = np.array([[0, 1, 2, 3]], np.int32)
+    m = O.Mesh.explicit(O.TET, coords, conn)
+    s = O.Space(m, True, 1)
+    J = (coords[1:] - coords[0]).T
+    vol = abs(np.linalg.det(J)) / 6
+    G = np.vstack([-np.ones(3), np.eye(3)]) @ np.linalg.inv(J)
+    for kind, expected in (("laplace", 2.5 * vol * G @ G.T), ("mass", vol / 20 * (np.ones((4, 4)) + np.eye(4)))):
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        if kind == "laplace":
+            op.append_laplace(2.5)
+        else:
+            op.append_product(1.0)
+        op.assemble(1)
+        np.testing.assert_allclose(op.csr().toarray(), expected, rtol=1e-13, atol=1e-15)
+
+
+def kuhn_nnz(c):
+    return (c + 1) ** 3 + 2 * (3 * c * (c + 1) ** 2 + 3 * c ** 2 * (c + 1) + c ** 3)
+
+
+def test_pattern_counts_and_mesh():
+    """P1 Kuhn stencil: nnz = N + 2 (#edges); conforming: every interior face shared by 2 tets"""
+    for c in (1, 2, 4):
+        m = O.Mesh.structured(O.TET, [c, c, c], [0, 0, 0], [1, 1, 1])
+        s = O.Space(m, True, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        assert op.nnz == kuhn_nnz(c)
+        nb = m.neighbors()
+        boundary = int((nb < 0).sum())
+        assert boundary == 2 * 6 * c * c  # 2 triangles per boundary square
+        # neighbour relation symmetric
+        for e, f in zip(*np.nonzero(nb >= 0)):
+            assert e in nb[nb[e, f]]
+
+
+def test_dirichlet_constraints():
+    """tools/dirichlet-constraints.hh:153-184: unit rows and columns, b_i = 0, symmetry kept"""
+    n = 16
+    m, s, op = q1_heat(n)
+    A = op.csr()
+    b = op.rhs()
+    d = op.dirichlet_dofs()
+    assert len(d) == 4 * n
+    for i in d[:: max(1, len(d) // 10)]:
+        row = A.getrow(i).toarray().ravel()
+        col = A.getcol(i).toarray().ravel()
+        assert row[i] == 1.0 and col[i] == 1.0
+        assert np.count_nonzero(row) == 1 and np.count_nonzero(col) == 1
+        assert b[i] == 0.0
+    assert abs(A - A.T).max() < 1e-15
+
+
+def test_pcg_against_direct_solve():
+    """Jacobi-PCG to 1e-10 vs scipy's direct solve; C1 (128^2, 16 641 DoFs): max u_h vs the
+    continuous max 0.0736713 of -Laplace u = 1 on the unit square"""
+    m, s, op = q1_heat(128)
+    rp, cols = op.pattern()
+    vals, b = op.values(), op.rhs()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, vals, b, rtol=1e-10)
+    assert rc == 0 and rel <= 1e-10
+    A = sp.csr_matrix((vals, cols, rp))
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 2e-10  # true residual
+    assert abs(x.max() - 0.0736713) < 2e-5
+    assert s.size == 16641 and op.nnz == 148225
+
+
+def test_pcg_failure_modes():
+    """host-forced failures: max_iter exhaustion and a non-SPD diagonal are reported, never silent"""
+    m, s, op = q1_heat(16)
+    rp, cols = op.pattern()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-10, max_iter=3)
+    assert rc != 0 and it == 3
+    m, s, op = q1_heat(8, kappa=-1.0, dirichlet=False)
+    rp, cols = op.pattern()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-10)
+    assert rc == -2
+
+
+def test_adjoint_gradient_central_differences():
+    """dJ/dlog kappa_e = -kappa_e lambda_e^T Khat_e u_e vs central differences of J = 1/2 |u - u_obs|^2"""
+    rng = np.random.default_rng(5)
+    m = O.Mesh.structured(O.TET, [3, 3, 3], [0, 0, 0], [1, 1, 1])
+    s = O.Space(m, True, 1)
+    ne = m.ne
+
+    def solve(logk):
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(np.exp(logk))
+        op.append_source(1.0)
+        op.set_dirichlet(True)
+        op.assemble(1)
+        rp, cols = op.pattern()
+        x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-15, max_iter=10000)
+        return x, (rp, cols, op.values())
+
+    g0 = rng.standard_normal(ne)
+    u_obs, _ = solve(rng.standard_normal(ne))
+    u, (rp, cols, vals) = solve(g0)
+    lam, _, _, _ = O.pcg_jacobi(rp, cols, vals, u - u_obs, rtol=1e-15, max_iter=10000)
+    grad = O.adjoint_log_kappa_gradient(s, np.exp(g0), u, lam)
+    h = 1e-6
+    for e in rng.choice(ne, 8, replace=False):
+        gp, gm = g0.copy(), g0.copy()
+        gp[e] += h
+        gm[e] -= h
+        up, _ = solve(gp)
+        um, _ = solve(gm)
+        fd = (0.5 * np.sum((up - u_obs) ** 2) - 0.5 * np.sum((um - u_obs) ** 2)) / (2 * h)
+        assert abs(fd - grad[e]) <= 1e-6 * max(abs(grad).max(), 1e-30)
