@@ -120,6 +120,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=40)
     ap.add_argument("--graph", action="store_true", help="replay PCG iteration blocks as a hipGraph")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (set-up phase)")
+    ap.add_argument("--spmv-sample", type=int, default=8,
+                    help="HIP events around every k-th operator application of the timed steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
@@ -137,9 +140,17 @@ def main():
     op.assemble()
     x = op.apply_inverse("rhs")
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_tune:
+        from hgdt import autotune
+
+        tuned = autotune.tune_solver(op)
+        tuned.pop("timings", None)
+        op.assemble()
+    torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     log(f"setup {t_setup:.1f} s: {space.num_DoFs} DoFs, {grid.num_elements} elements, nnz {op.nnz}, "
-        f"{op.format_info()}")
+        f"{op.format_info()}, launch {tuned}")
 
     def step(time_spmv):
         op.assemble()
@@ -154,7 +165,7 @@ def main():
     stats = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        x = step(time_spmv=True)
+        x = step(time_spmv=args.spmv_sample)
         stats.append(dict(op.last_solve))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -188,7 +199,8 @@ def main():
         "data": "synthetic (seeded splitmix64 / Box-Muller fields, structured grid)",
         "config": {"workload": workload, "dofs": n, "elements": grid.num_elements, "nnz": op.nnz,
                    "iterations": iters, "relres": stats[-1]["relres"], "format": fi["format"],
-                   "t_cg_ms": solve_ms, "t_assembly_ms": ms_per_step - solve_ms, "setup_s": t_setup},
+                   "t_cg_ms": solve_ms, "t_assembly_ms": ms_per_step - solve_ms, "setup_s": t_setup,
+                   "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_pcg_spmv<{fi['format']}>", "bytes_per_launch": fi["spmv_bytes"],

@@ -555,9 +555,11 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out)
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format)
 {
   return guarded([&] {
-    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_DIA, HGDT_ERR_INVALID_ARGUMENT,
+    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_DIA_NONSYMMETRIC, HGDT_ERR_INVALID_ARGUMENT,
             "unknown format");
-    op->op->requested_format = format;
+    op->op->requested_format = (format == HGDT_FORMAT_DIA_NONSYMMETRIC) ? HGDT_FORMAT_DIA : format;
+    op->op->allow_symmetric = (format != HGDT_FORMAT_DIA_NONSYMMETRIC);
+    op->op->fmt.partitions_ready = false;
     op->op->fmt.structure_ready = false;
     op->op->fmt.valid = false;
   });
@@ -569,7 +571,8 @@ hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t
     require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
     require(op->op->fmt.structure_ready, HGDT_ERR_OPERATOR, "no apply / apply_inverse since the format was chosen");
     if (format)
-      *format = op->op->fmt.format;
+      *format = (op->op->fmt.format == HGDT_FORMAT_DIA && !op->op->fmt.symmetric) ? HGDT_FORMAT_DIA_NONSYMMETRIC
+                                                                                   : op->op->fmt.format;
     if (spmv_bytes)
       *spmv_bytes = op->op->fmt.bytes;
     if (num_diagonals)
@@ -645,17 +648,22 @@ hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t b
 {
   return guarded([&] {
     require(op && kernel >= 0 && kernel < 5, HGDT_ERR_INVALID_ARGUMENT, "unknown kernel id");
-    require(block_size >= 64 && block_size % 64 == 0 && items_per_thread >= 1 && items_per_thread <= 8,
-            HGDT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64, items_per_thread in [1, 8]");
+    require(block_size >= 64 && block_size % 64 == 0 && items_per_thread >= 1 && items_per_thread <= 8 &&
+                (items_per_thread & (items_per_thread - 1)) == 0,
+            HGDT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64, items_per_thread 1, 2, 4 or 8");
     const int limit = (kernel <= 2) ? 256 : 1024; // assembly kernels are compiled for at most 256 lanes
     require(block_size <= limit, HGDT_ERR_INVALID_ARGUMENT, "block_size above the kernel's launch bound");
     Launch& L = op->op->launch[kernel];
-    if (kernel == 3 && (L.block != block_size || L.items != items_per_thread)) {
-      op->op->fmt.structure_ready = false;
-      op->op->fmt.valid = false;
+    if ((kernel == 3 || kernel == 4) && (L.block != block_size || L.items != items_per_thread)) {
+      op->op->fmt.partitions_ready = false;
+      if (kernel == 3 && op->op->fmt.format == HGDT_FORMAT_SELL) { // SELL slices follow the A partition
+        op->op->fmt.structure_ready = false;
+        op->op->fmt.valid = false;
+      }
     }
     L.block = block_size;
     L.items = (kernel <= 2) ? 1 : items_per_thread;
+    L.user = true;
   });
 }
 

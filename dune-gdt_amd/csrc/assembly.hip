@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
-          for (int j = 0; j < ND; ++j) {
+          for (int j = i; j < ND; ++j) {
             double dotv = 0.;
 #pragma unroll
             for (int a = 0; a < D; ++a)
@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 #pragma unroll
           for (int i = 0; i < ND; ++i)
 #pragma unroll
-            for (int j = 0; j < ND; ++j) {
+            for (int j = i; j < ND; ++j) {
               double dotv = 0.;
 #pragma unroll
               for (int a = 0; a < D; ++a)
@@ -300,17 +300,21 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 #pragma unroll
           for (int i = 0; i < ND; ++i)
 #pragma unroll
-            for (int j = 0; j < ND; ++j)
+            for (int j = i; j < ND; ++j)
               A[i][j] += s * (((c * phi[i]) * phi[j]) * factor);
         }
       }
     }
+    // the Laplace and product forms are symmetric: only the upper triangle (i <= j) is computed and
+    // stored, entry-major (entry t of every element contiguous, coalesced across lanes); the scatter
+    // reads entry (min, max) for both (i, j) and (j, i), so assembled matrices are symmetric bit for bit
     double* __restrict__ out = P.staging + P.elem_base;
+    int t = 0;
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
-      for (int j = 0; j < ND; ++j)
-        out[int64_t(i * ND + j) * P.ne + e] = A[i][j];
+      for (int j = i; j < ND; ++j, ++t)
+        out[int64_t(t) * P.ne + e] = A[i][j];
   }
   if (P.nsrc > 0) {
     double b[ND];
@@ -466,14 +470,14 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
       }
     }
   }
-  double* __restrict__ out = P.staging + P.base;
+  double* __restrict__ out = P.staging + P.base + fi * (4 * K2);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
       for (int j = 0; j < ND; ++j)
-        out[int64_t(k * K2 + i * ND + j) * P.nfaces + fi] = B[k][i][j];
+        out[k * K2 + i * ND + j] = B[k][i][j];
 }
 
 template <int GEOM>
@@ -528,12 +532,12 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
       }
     }
   }
-  double* __restrict__ out = P.staging + P.base;
+  double* __restrict__ out = P.staging + P.base + fi * (ND * ND);
 #pragma unroll
   for (int i = 0; i < ND; ++i)
 #pragma unroll
     for (int j = 0; j < ND; ++j)
-      out[int64_t(i * ND + j) * P.nfaces + fi] = B[i][j];
+      out[i * ND + j] = B[i][j];
 }
 
 // fixed-order gather of the staged local contributions into the CSR values

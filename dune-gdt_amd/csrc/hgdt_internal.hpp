@@ -166,6 +166,7 @@ struct SourceDesc {
 struct Launch {
   int block = 256;
   int items = 1;
+  bool user = false; // set explicitly (autotuner / caller) rather than by the default heuristic
 };
 
 // Fixed-order scatter maps: every matrix slot (and right-hand side entry) lists
@@ -183,21 +184,31 @@ struct ScatterMaps {
   int signature = -1;
 };
 
+// consecutive rows handed to the blocks of a launch; a block owns block * items rows
+struct RowPartition {
+  int block = 256, items = 1;
+  int64_t nblocks = 0;
+  std::vector<int64_t> row0;        // nblocks + 1 (row start of every block, last = n)
+  std::vector<int32_t> inst;        // nblocks
+  std::vector<int32_t> inst_block0; // ninst + 1
+  DeviceArray<int64_t> d_row0;
+  DeviceArray<int32_t> d_inst, d_inst_block0;
+  void build(const std::vector<int64_t>& inst_dof, int block_size, int items_per_thread);
+};
+
 // storage of the matrix used by apply / apply_inverse
 struct SolverFormat {
   int format = 0;      // HGDT_FORMAT_SELL or HGDT_FORMAT_DIA
-  bool structure_ready = false; // partition + storage layout built
+  bool partitions_ready = false; // row partitions A and V built for the current launch shapes
+  bool structure_ready = false;  // storage layout (DIA offsets / SELL slices) built
   bool valid = false;  // values converted since the last assemble
   bool diag_positive = false;
+  bool symmetric = false; // DIA read through its upper diagonals only (exactly symmetric matrix)
+  int nup = 0;            // diagonals with offset >= 0
   int64_t bytes = 0;   // algorithmic bytes of one operator application
-  // row-block partition shared by every solver kernel: blocks never straddle instances
-  int rows_per_block = 0;
-  int64_t nblocks = 0;
-  std::vector<int64_t> block_row0;  // nblocks + 1 (row start of every block, last = n)
-  std::vector<int32_t> block_inst;  // nblocks
-  std::vector<int32_t> inst_block0; // ninst + 1
-  DeviceArray<int64_t> d_block_row0;
-  DeviceArray<int32_t> d_block_inst, d_inst_block0;
+  // row-block partitions (blocks never straddle instances): A for operator application, V for the
+  // vector kernels of the solver; they are tuned separately
+  RowPartition A, V;
   DeviceArray<int64_t> d_inst_row0;
   // DIA
   int ndiag = 0;
@@ -214,14 +225,13 @@ struct SolverFormat {
   DeviceArray<double> d_dinv;      // Jacobi preconditioner
 };
 
-// device-side PCG state, one record per instance
+// device-side PCG state, one record per instance; two copies indexed by iteration parity so that
+// no kernel reads a record it writes
 struct PcgScalars {
-  double rz, rr, bb, tol2, alpha, beta, pq;
+  double rz, rr, bb, tol2, alpha, beta, pq, pad0;
   int64_t iters;
   int32_t done; // 0 running, 1 converged, 2 max_iter reached, 3 breakdown
-  int32_t ticket;
-  int32_t ticket2;
-  int32_t pad;
+  int32_t pad1;
 };
 
 struct SolverCache; // solver.hip: events, pinned buffers, captured graph
@@ -247,6 +257,7 @@ struct Operator {
   DeviceArray<double> d_qtab_elem, d_qtab_line; // quadrature tables of the appended forms
   bool tables_ready = false;
   int requested_format = HGDT_FORMAT_AUTO;
+  bool allow_symmetric = true;
   SolverFormat fmt;
   Launch launch[5];
   // solver workspace

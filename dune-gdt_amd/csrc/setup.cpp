@@ -416,8 +416,14 @@ void Operator::build_scatter_maps()
       }
     }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
+  // element staging holds the upper triangle of the (symmetric) element matrices, entry-major
+  const int64_t k2s = int64_t(ls) * (ls + 1) / 2;
+  auto tri = [ls](int i, int j) {
+    const int a = std::min(i, j), b = std::max(i, j);
+    return a * ls - a * (a - 1) / 2 + (b - a);
+  };
   M.elem_base = 0;
-  M.inner_base = has_e ? ne * k2 : 0;
+  M.inner_base = has_e ? ne * k2s : 0;
   M.bnd_base = M.inner_base + (has_c ? nin * 4 * k2 : 0);
   const int64_t rhs_base = M.bnd_base + (has_b ? nbd * k2 : 0);
   M.staging_size = rhs_base + (sources.empty() ? 0 : ne * ls);
@@ -438,7 +444,7 @@ void Operator::build_scatter_maps()
       const int32_t* de = &s.dofs[e * ls];
       if (has_e)
         for (int lj = 0; lj < ls; ++lj)
-          out.push_back({e * (nf + 1), uint32_t(M.elem_base + (li * ls + lj) * ne + e), de[lj]});
+          out.push_back({e * (nf + 1), uint32_t(M.elem_base + tri(li, lj) * ne + e), de[lj]});
       for (int f = 0; f < nf; ++f) {
         const int64_t o = g.nb[e * nf + f];
         if (o >= 0 && has_c) {
@@ -447,23 +453,23 @@ void Operator::build_scatter_maps()
             const int32_t* dout = &s.dofs[o * ls];
             const int64_t key = e * (nf + 1) + 1 + f;
             for (int lj = 0; lj < ls; ++lj) // in_in
-              out.push_back({key, uint32_t(M.inner_base + (0 * k2 + li * ls + lj) * nin + fi), de[lj]});
+              out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 0 * k2 + li * ls + lj), de[lj]});
             for (int lj = 0; lj < ls; ++lj) // in_out
-              out.push_back({key, uint32_t(M.inner_base + (1 * k2 + li * ls + lj) * nin + fi), dout[lj]});
+              out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 1 * k2 + li * ls + lj), dout[lj]});
           } else {
             const int fo = g.nbface[e * nf + f];
             const int64_t fi = inner_index[o * nf + fo];
             const int32_t* din = &s.dofs[o * ls];
             const int64_t key = o * (nf + 1) + 1 + fo;
             for (int lj = 0; lj < ls; ++lj) // out_in
-              out.push_back({key, uint32_t(M.inner_base + (2 * k2 + li * ls + lj) * nin + fi), din[lj]});
+              out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 2 * k2 + li * ls + lj), din[lj]});
             for (int lj = 0; lj < ls; ++lj) // out_out
-              out.push_back({key, uint32_t(M.inner_base + (3 * k2 + li * ls + lj) * nin + fi), de[lj]});
+              out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 3 * k2 + li * ls + lj), de[lj]});
           }
         } else if (o < 0 && has_b) {
           const int64_t bi = bnd_index[e * nf + f];
           for (int lj = 0; lj < ls; ++lj)
-            out.push_back({e * (nf + 1) + 1 + f, uint32_t(M.bnd_base + (li * ls + lj) * nbd + bi), de[lj]});
+            out.push_back({e * (nf + 1) + 1 + f, uint32_t(M.bnd_base + bi * k2 + li * ls + lj), de[lj]});
         }
       }
     }

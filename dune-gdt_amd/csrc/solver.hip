@@ -4,28 +4,40 @@
 // and apply_inverse (XT::LA::Solver, matrix-based.hh:148-159).
 //
 // Storage: the assembled CSR matrix is converted once per assembly into
-//   * DIA  -- one value array per diagonal, no column indices, when every problem
-//             instance has <= 32 distinct (col - row) offsets (structured grids), or
-//   * SELL-64 -- sliced ELLPACK with slices of 64 rows (one wavefront), column-major
-//             inside a slice so every load of a wave is one contiguous 512-byte segment.
-// Both are HBM-bandwidth-bound; one lane owns a row, rows are summed in column order.
+//   * DIA  -- one value array per diagonal, no column indices, when every problem instance
+//             has <= 32 distinct (col - row) offsets (structured grids).  When the matrix is
+//             exactly symmetric (checked bit for bit on the device after every assembly) only
+//             the diagonals with offset >= 0 are read: A(i, i-o) = A(i-o, i) is taken from the
+//             upper diagonal at row i-o, which halves the bytes read per application;
+//   * SELL-64 -- sliced ELLPACK with slices of 64 rows (one wavefront), column-major inside a
+//             slice so every load of a wave is one contiguous 512-byte segment.
+// One lane owns a row and sums it in ascending column order (the CSR order), so DIA, symmetric
+// DIA and SELL give the same bits for the same matrix.
 //
 // PCG iteration = three launches, each one pass over its vectors:
 //   k_pcg_spmv   : q = A p and the block partials of p.q
-//   k_pcg_update : x += alpha p, r -= alpha q, z = D^{-1} r (not stored), partials of r.z and r.r
-//   k_pcg_pupdate: p = D^{-1} r + beta p
-// Dot products are reduced deterministically: butterfly inside a wavefront, waves in
-// fixed order inside a block, then the last block of the instance (found with an
-// integer ticket; the only atomic in the library, never on values) sums the block
-// partials in block order and updates the instance's scalars on the device.  Every
-// block belongs to exactly one problem instance, so a batch of independent problems
-// (a union grid) is solved in the same launches with per-instance scalars and stopping
-// tests; blocks of converged instances exit immediately.
+//   k_pcg_update : alpha from the p.q partials; x += alpha p, r -= alpha q, z = D^{-1} r (not stored),
+//                  block partials of r.z and r.r
+//   k_pcg_pupdate: beta and the stopping test from the r.z / r.r partials; p = D^{-1} r + beta p
+// Data cross blocks only at kernel boundaries.  Every block of an instance reduces that
+// instance's block partials itself, in block order, with a fixed butterfly / wave order, so all
+// blocks obtain bitwise identical alpha and beta and the result does not depend on scheduling;
+// no fence or atomic is used.  The per-instance scalars live in two records selected by the
+// parity of the iteration: iteration j reads record j%2 and its last kernel writes record
+// (j+1)%2.  Every block belongs to exactly one problem instance, so a batch of independent
+// problems (a union grid) is solved in the same launches with per-instance scalars and stopping
+// tests; blocks of finished instances exit at once.
+//
+// Launch geometry: a block owns R = block * items consecutive rows, each lane `items` rows
+// strided by the block size (coalesced, all loads of a lane issued before use).  Hardware
+// workgroups are dealt round-robin to the 8 XCDs; logical block L is given to hardware block
+// (L % q) * 8 + L / q (q = ceil(nblocks / 8)) so every XCD streams one contiguous eighth of the
+// rows and the stencil neighbours of its rows stay in its own L2.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
-#include <atomic>
 #include <mutex>
 
 #include "hgdt_internal.hpp"
@@ -34,21 +46,28 @@ namespace hgdt {
 namespace {
 
 constexpr int MAXDIAG = 32;
+enum { FMT_DIA = HGDT_FORMAT_DIA, FMT_SELL = HGDT_FORMAT_SELL, FMT_DIAS = 3 };
 
 struct PcgParams {
   int64_t n;
-  int items;
-  // partition
-  const int64_t* block_row0;
-  const int32_t* block_inst;
-  const int32_t* inst_block0;
+  // A: row partition of the operator application
+  int a_nb;
+  const int64_t* a_row0;
+  const int32_t* a_inst;
+  const int32_t* a_ib0; // first A-block of every instance
+  // V: row partition of the vector kernels
+  int v_nb;
+  const int64_t* v_row0;
+  const int32_t* v_inst;
+  const int32_t* v_ib0;
   const int64_t* inst_row0;
   // DIA
   int ndiag;
-  const int64_t* dia_off;
-  const double* dia;
+  const int64_t* dia_off; // ninst * ndiag, ascending, sentinel n for unused slots
+  const int32_t* dia_k0;  // ninst: index of offset 0
+  const double* dia;      // ndiag * n
   // SELL
-  const int64_t* block_slice0;
+  const int64_t* block_slice0; // per A-block
   const int64_t* slice_ptr;
   const int32_t* sell_cols;
   const double* sell_vals;
@@ -60,12 +79,23 @@ struct PcgParams {
   double* p;
   double* q;
   const double* dinv;
-  double* partials;
-  PcgScalars* S;
+  double* part1; // 1 per A-block: p.q
+  double* part2; // 2 per V-block: r.z, r.r
+  double* part3; // 3 per V-block: b.b, r.r, r.z (initialisation)
+  PcgScalars* S; // 2 * ninst records
+  int ninst;
   double rtol2;
   int64_t max_iter;
   int zero_guess;
 };
+
+__device__ __forceinline__ int logical_block(int nb)
+{
+  const int q = (nb + 7) >> 3;
+  const int hw = blockIdx.x;
+  const int L = (hw & 7) * q + (hw >> 3);
+  return L < nb ? L : -1;
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -75,11 +105,12 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// deterministic block reduction of NV values; result valid in thread 0
+// deterministic block reduction of NV values; the result is returned to every thread
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV])
 {
   __shared__ double lds[NV][16];
+  __shared__ double res[NV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
@@ -96,103 +127,164 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
       double s = 0.;
       for (int w = 0; w < nw; ++w)
         s += lds[k][w];
-      v[k] = s;
+      res[k] = s;
     }
   __syncthreads();
-}
-
-// store this block's partials; returns true in every thread of the instance's last block,
-// which then sees all partials of the instance
-template <int NV>
-__device__ __forceinline__ bool publish_partials(const PcgParams& P, int b, int inst, const double (&v)[NV])
-{
-  __shared__ int last;
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&P.partials[int64_t(b) * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const int nb = P.inst_block0[inst + 1] - P.inst_block0[inst];
-    const int t = __hip_atomic_fetch_add(&P.S[inst].ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == nb - 1);
-  }
+  for (int k = 0; k < NV; ++k)
+    v[k] = res[k];
   __syncthreads();
-  if (last)
-    __threadfence();
-  return last;
 }
 
-// the last block sums the instance's partials in block order (strided per thread, then block_sum)
+// sum of an instance's block partials (written by an earlier kernel) in block order
+// (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
+// partials are issued together, so the prologue costs one memory round trip per U * B partials)
 template <int NV>
-__device__ __forceinline__ void gather_partials(const PcgParams& P, int inst, double (&tot)[NV])
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
 {
-  const int b0 = P.inst_block0[inst], b1 = P.inst_block0[inst + 1];
+  constexpr int U = 8;
+  const int B = blockDim.x;
 #pragma unroll
   for (int k = 0; k < NV; ++k)
     tot[k] = 0.;
-  for (int bb = b0 + threadIdx.x; bb < b1; bb += blockDim.x)
+  for (int base = b0 + threadIdx.x; base < b1; base += U * B) {
+    double v[U][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      tot[k] += __hip_atomic_load(&P.partials[int64_t(bb) * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int u = 0; u < U; ++u) {
+      const int bb = base + u * B;
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        v[u][k] = (bb < b1) ? part[int64_t(bb) * NV + k] : 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        tot[k] += v[u][k];
+  }
   block_sum<NV>(tot);
 }
 
-template <int FMT>
-__device__ __forceinline__ double row_apply(const PcgParams& P, int b, int inst, int64_t r0, int64_t i,
-                                            const double* __restrict__ v)
+// y_i = sum_j A_ij v_j for the ITEMS rows r0 + threadIdx.x + j * blockDim.x of block b
+template <int FMT, int ITEMS>
+__device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, int64_t r0, int64_t r1,
+                                           const double* __restrict__ v, double (&acc)[ITEMS])
 {
-  double acc = 0.;
-  if constexpr (FMT == HGDT_FORMAT_DIA) {
+  const int64_t B = blockDim.x;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j)
+    acc[j] = 0.;
+  if constexpr (FMT == FMT_DIA || FMT == FMT_DIAS) {
     const int64_t lo = P.inst_row0[inst], hi = P.inst_row0[inst + 1];
     const int64_t* __restrict__ off = P.dia_off + int64_t(inst) * P.ndiag;
-    for (int k = 0; k < P.ndiag; ++k) {
-      const int64_t c = i + off[k];
-      if (c >= lo && c < hi)
-        acc = fma(P.dia[int64_t(k) * P.n + i], v[c], acc);
+    const double* __restrict__ dia = P.dia;
+    const int64_t n = P.n;
+    if constexpr (FMT == FMT_DIA) {
+#pragma unroll 3
+      for (int k = 0; k < P.ndiag; ++k) {
+        const int64_t o = off[k];
+        const double* __restrict__ dk = dia + int64_t(k) * n;
+        double a[ITEMS], x[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const int64_t i = r0 + threadIdx.x + j * B;
+          const int64_t c = i + o;
+          const bool ok = (i < r1) && (c >= lo) && (c < hi);
+          a[j] = ok ? dk[i] : 0.;
+          x[j] = ok ? v[c] : 0.;
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          acc[j] = fma(a[j], x[j], acc[j]);
+      }
+    } else {
+      // lower triangle from the upper diagonals (descending offset = ascending column), then upper
+      const int k0 = P.dia_k0[inst];
+#pragma unroll 2
+      for (int k = P.ndiag - 1; k > k0; --k) {
+        const int64_t o = off[k];
+        const double* __restrict__ dk = dia + int64_t(k) * n;
+        double a[ITEMS], x[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const int64_t i = r0 + threadIdx.x + j * B;
+          const int64_t c = i - o;
+          const bool ok = (i < r1) && (c >= lo) && (c < hi);
+          a[j] = ok ? dk[c] : 0.;
+          x[j] = ok ? v[c] : 0.;
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          acc[j] = fma(a[j], x[j], acc[j]);
+      }
+#pragma unroll 2
+      for (int k = k0; k < P.ndiag; ++k) {
+        const int64_t o = off[k];
+        const double* __restrict__ dk = dia + int64_t(k) * n;
+        double a[ITEMS], x[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const int64_t i = r0 + threadIdx.x + j * B;
+          const int64_t c = i + o;
+          const bool ok = (i < r1) && (c >= lo) && (c < hi);
+          a[j] = ok ? dk[i] : 0.;
+          x[j] = ok ? v[c] : 0.;
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+          acc[j] = fma(a[j], x[j], acc[j]);
+      }
     }
   } else {
-    const int64_t local = i - r0;
-    const int64_t sl = P.block_slice0[b] + (local >> 6);
-    const int lane = int(local & 63);
-    const int64_t s0 = P.slice_ptr[sl];
-    const int w = int((P.slice_ptr[sl + 1] - s0) >> 6);
-    const int32_t* __restrict__ cc = P.sell_cols + s0 + lane;
-    const double* __restrict__ vv = P.sell_vals + s0 + lane;
-    for (int k = 0; k < w; ++k)
-      acc = fma(vv[int64_t(k) * 64], v[cc[int64_t(k) * 64]], acc);
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + j * B;
+      if (i < r1) {
+        const int64_t local = i - r0;
+        const int64_t sl = P.block_slice0[b] + (local >> 6);
+        const int lane = int(local & 63);
+        const int64_t s0 = P.slice_ptr[sl];
+        const int w = int((P.slice_ptr[sl + 1] - s0) >> 6);
+        const int32_t* __restrict__ cc = P.sell_cols + s0 + lane;
+        const double* __restrict__ vv = P.sell_vals + s0 + lane;
+        double s = 0.;
+        for (int k = 0; k < w; ++k)
+          s = fma(vv[int64_t(k) * 64], v[cc[int64_t(k) * 64]], s);
+        acc[j] = s;
+      }
+    }
   }
-  return acc;
 }
 
-template <int FMT>
+template <int FMT, int ITEMS>
 __global__ void __launch_bounds__(1024) k_apply(PcgParams P, const double* __restrict__ x, double* __restrict__ y)
 {
-  const int b = blockIdx.x;
-  const int inst = P.block_inst[b];
-  const int64_t r0 = P.block_row0[b], r1 = P.block_row0[b + 1];
-  for (int j = 0; j < P.items; ++j) {
+  const int b = logical_block(P.a_nb);
+  if (b < 0)
+    return;
+  const int inst = P.a_inst[b];
+  const int64_t r0 = P.a_row0[b], r1 = P.a_row0[b + 1];
+  double acc[ITEMS];
+  apply_rows<FMT, ITEMS>(P, b, inst, r0, r1, x, acc);
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     if (i < r1)
-      y[i] = row_apply<FMT>(P, b, inst, r0, i, x);
+      y[i] = acc[j];
   }
 }
 
-__global__ void k_reset_scalars(PcgScalars* S, int ninst)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ninst) {
-    PcgScalars z{};
-    S[i] = z;
-  }
-}
-
+// r = b - A x0, p = z = D^{-1} r, block partials of b.b, r.r, r.z
+template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_pcg_init(PcgParams P)
 {
-  const int b = blockIdx.x;
-  const int inst = P.block_inst[b];
-  const int64_t r0 = P.block_row0[b], r1 = P.block_row0[b + 1];
+  const int b = logical_block(P.v_nb);
+  if (b < 0)
+    return;
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
   double v[3] = {0., 0., 0.};
-  for (int j = 0; j < P.items; ++j) {
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     if (i < r1) {
       const double bi = P.b[i];
@@ -206,108 +298,162 @@ __global__ void __launch_bounds__(1024) k_pcg_init(PcgParams P)
     }
   }
   block_sum<3>(v);
-  if (publish_partials<3>(P, b, inst, v)) {
-    double tot[3];
-    gather_partials<3>(P, inst, tot);
-    if (threadIdx.x == 0) {
-      PcgScalars& S = P.S[inst];
-      S.bb = tot[0], S.rr = tot[1], S.rz = tot[2];
-      S.tol2 = P.rtol2 * tot[0];
-      S.iters = 0;
-      S.alpha = 0., S.beta = 0., S.pq = 0.;
-      S.done = (tot[1] <= S.tol2) ? 1 : 0;
-      S.ticket = 0;
-    }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      P.part3[int64_t(b) * 3 + k] = v[k];
+}
+
+// one block per instance: initial scalars into record 0
+__global__ void __launch_bounds__(256) k_pcg_init_scalars(PcgParams P)
+{
+  const int inst = blockIdx.x;
+  double t[3];
+  reduce_partials<3>(P.part3, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  if (threadIdx.x == 0) {
+    PcgScalars S{};
+    S.bb = t[0], S.rr = t[1], S.rz = t[2];
+    S.tol2 = P.rtol2 * t[0];
+    S.iters = 0;
+    S.done = (t[1] <= S.tol2) ? 1 : 0;
+    P.S[inst] = S;
   }
 }
 
-template <int FMT>
-__global__ void __launch_bounds__(1024) k_pcg_spmv(PcgParams P)
+template <int FMT, int ITEMS>
+__global__ void __launch_bounds__(1024) k_pcg_spmv(PcgParams P, int par)
 {
-  const int b = blockIdx.x;
-  const int inst = P.block_inst[b];
-  if (P.S[inst].done)
+  const int b = logical_block(P.a_nb);
+  if (b < 0)
     return;
-  const int64_t r0 = P.block_row0[b], r1 = P.block_row0[b + 1];
+  const int inst = P.a_inst[b];
+  if (P.S[par * P.ninst + inst].done)
+    return;
+  const int64_t r0 = P.a_row0[b], r1 = P.a_row0[b + 1];
+  double acc[ITEMS];
+  apply_rows<FMT, ITEMS>(P, b, inst, r0, r1, P.p, acc);
   double v[1] = {0.};
-  for (int j = 0; j < P.items; ++j) {
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     if (i < r1) {
-      const double qi = row_apply<FMT>(P, b, inst, r0, i, P.p);
-      P.q[i] = qi;
-      v[0] += P.p[i] * qi;
+      P.q[i] = acc[j];
+      v[0] += P.p[i] * acc[j];
     }
   }
   block_sum<1>(v);
-  if (publish_partials<1>(P, b, inst, v)) {
-    double tot[1];
-    gather_partials<1>(P, inst, tot);
-    if (threadIdx.x == 0) {
-      PcgScalars& S = P.S[inst];
-      S.pq = tot[0];
-      if (!(tot[0] > 0.)) {
-        S.done = 3; // breakdown: A not positive definite on the Krylov space
-      } else {
-        S.alpha = S.rz / tot[0];
-      }
-      S.ticket = 0;
-    }
-  }
+  if (threadIdx.x == 0)
+    P.part1[b] = v[0];
 }
 
-__global__ void __launch_bounds__(1024) k_pcg_update(PcgParams P)
+template <int ITEMS>
+__global__ void __launch_bounds__(1024) k_pcg_update(PcgParams P, int par)
 {
-  const int b = blockIdx.x;
-  const int inst = P.block_inst[b];
-  if (P.S[inst].done)
+  const int b = logical_block(P.v_nb);
+  if (b < 0)
     return;
-  const double alpha = P.S[inst].alpha;
-  const int64_t r0 = P.block_row0[b], r1 = P.block_row0[b + 1];
+  const int inst = P.v_inst[b];
+  PcgScalars* S = P.S + par * P.ninst + inst;
+  if (S->done)
+    return;
+  const bool first = (b == P.v_ib0[inst]);
+  double pq[1];
+  reduce_partials<1>(P.part1, P.a_ib0[inst], P.a_ib0[inst + 1], pq);
+  if (!(pq[0] > 0.)) {
+    // breakdown (A not positive definite on the Krylov space): every block of the instance sees it
+    if (first && threadIdx.x == 0)
+      S->done = 3;
+    return;
+  }
+  const double alpha = S->rz / pq[0];
+  if (first && threadIdx.x == 0)
+    S->alpha = alpha, S->pq = pq[0];
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+  double xv[ITEMS], pv[ITEMS], qv[ITEMS], rv[ITEMS], dv[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+    const bool ok = i < r1;
+    xv[j] = ok ? P.x[i] : 0.;
+    pv[j] = ok ? P.p[i] : 0.;
+    qv[j] = ok ? P.q[i] : 0.;
+    rv[j] = ok ? P.r[i] : 0.;
+    dv[j] = ok ? P.dinv[i] : 0.;
+  }
   double v[2] = {0., 0.};
-  for (int j = 0; j < P.items; ++j) {
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     if (i < r1) {
-      P.x[i] += alpha * P.p[i];
-      const double ri = P.r[i] - alpha * P.q[i];
+      P.x[i] = xv[j] + alpha * pv[j];
+      const double ri = rv[j] - alpha * qv[j];
       P.r[i] = ri;
-      const double zi = P.dinv[i] * ri;
+      const double zi = dv[j] * ri;
       v[0] += ri * zi, v[1] += ri * ri;
     }
   }
   block_sum<2>(v);
-  if (publish_partials<2>(P, b, inst, v)) {
-    double tot[2];
-    gather_partials<2>(P, inst, tot);
-    if (threadIdx.x == 0) {
-      PcgScalars& S = P.S[inst];
-      S.rr = tot[1];
-      const int64_t it = S.iters + 1;
-      S.iters = it;
-      if (tot[1] <= S.tol2) {
-        S.done = 1;
-      } else if (it >= P.max_iter) {
-        S.done = 2;
-      } else {
-        S.beta = tot[0] / S.rz;
-        S.rz = tot[0];
-      }
-      S.ticket = 0;
-    }
-  }
+  if (threadIdx.x == 0)
+    P.part2[int64_t(b) * 2 + 0] = v[0], P.part2[int64_t(b) * 2 + 1] = v[1];
 }
 
-__global__ void __launch_bounds__(1024) k_pcg_pupdate(PcgParams P)
+template <int ITEMS>
+__global__ void __launch_bounds__(1024) k_pcg_pupdate(PcgParams P, int par)
 {
-  const int b = blockIdx.x;
-  const int inst = P.block_inst[b];
-  if (P.S[inst].done)
+  const int b = logical_block(P.v_nb);
+  if (b < 0)
     return;
-  const double beta = P.S[inst].beta;
-  const int64_t r0 = P.block_row0[b], r1 = P.block_row0[b + 1];
-  for (int j = 0; j < P.items; ++j) {
-    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-    if (i < r1)
-      P.p[i] = P.dinv[i] * P.r[i] + beta * P.p[i];
+  const int inst = P.v_inst[b];
+  const int b0 = P.v_ib0[inst], b1 = P.v_ib0[inst + 1];
+  const PcgScalars* cur = P.S + par * P.ninst + inst;
+  PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
+  const bool writer = (b == b0 && threadIdx.x == 0);
+  // scalar fields are read one by one (no per-lane copy of the record)
+  const int done = cur->done;
+  if (done) {
+    if (writer)
+      *nxt = *cur;
+    return;
+  }
+  const double rz = cur->rz, tol2 = cur->tol2;
+  const int64_t it = cur->iters + 1;
+  double t[2];
+  reduce_partials<2>(P.part2, b0, b1, t);
+  int status = 0;
+  double beta = 0.;
+  if (t[1] <= tol2) {
+    status = 1;
+  } else if (it >= P.max_iter) {
+    status = 2;
+  } else {
+    beta = t[0] / rz;
+    const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+    double rv[ITEMS], dv[ITEMS], pv[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      const bool ok = i < r1;
+      rv[j] = ok ? P.r[i] : 0.;
+      dv[j] = ok ? P.dinv[i] : 0.;
+      pv[j] = ok ? P.p[i] : 0.;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      if (i < r1)
+        P.p[i] = dv[j] * rv[j] + beta * pv[j];
+    }
+  }
+  if (writer) {
+    nxt->bb = cur->bb;
+    nxt->tol2 = tol2;
+    nxt->alpha = cur->alpha;
+    nxt->pq = cur->pq;
+    nxt->rr = t[1];
+    nxt->iters = it;
+    nxt->done = status;
+    nxt->rz = status ? rz : t[0];
+    nxt->beta = beta;
   }
 }
 
@@ -319,6 +465,13 @@ __global__ void k_scatter_values(int64_t nnz, const double* __restrict__ vals, c
     dst[pos[s]] = vals[s];
 }
 
+__global__ void k_fill_int(int n, int* __restrict__ dst, int v)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n)
+    dst[s] = v;
+}
+
 __global__ void k_fill(int64_t n, double* __restrict__ dst, double v)
 {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -326,6 +479,7 @@ __global__ void k_fill(int64_t n, double* __restrict__ dst, double v)
     dst[s] = v;
 }
 
+// Jacobi preconditioner from the CSR diagonal; flags a non-positive diagonal
 __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
                          const double* __restrict__ vals, double* __restrict__ dinv, int* bad)
 {
@@ -337,10 +491,65 @@ __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const i
     if (cols[k] == i)
       d = vals[k];
   if (!(d > 0.)) {
-    *bad = 1;
+    bad[0] = 1;
     d = 1.;
   }
   dinv[i] = 1. / d;
+}
+
+// DIA preparation in one pass over the rows: Jacobi diagonal from the offset-0 diagonal (flagging a
+// non-positive entry) and exact symmetry: for every stored negative offset o, A(i, i+o) must equal
+// A(i+o, i), stored on the mirrored diagonal mirror[k] (offset -o) at row i+o
+__global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ off, const int32_t* __restrict__ k0v,
+                              const int32_t* __restrict__ mirror, const int32_t* __restrict__ row_inst,
+                              const int64_t* __restrict__ inst_row0, const double* __restrict__ dia,
+                              double* __restrict__ dinv, int* flags)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int inst = row_inst[i];
+  const int64_t lo = inst_row0[inst], hi = inst_row0[inst + 1];
+  const int64_t* o = off + int64_t(inst) * ndiag;
+  const int32_t* mir = mirror + int64_t(inst) * ndiag;
+  const int k0 = k0v[inst];
+  double d = dia[int64_t(k0) * n + i];
+  if (!(d > 0.)) {
+    flags[0] = 1;
+    d = 1.;
+  }
+  dinv[i] = 1. / d;
+  bool ok = true;
+  for (int k = 0; k < k0; ++k) {
+    const int64_t c = i + o[k];
+    if (c < lo || c >= hi)
+      continue; // outside the instance: padding, never read
+    const int m = mir[k];
+    ok = ok && (m >= 0) && (dia[int64_t(m) * n + c] == dia[int64_t(k) * n + i]);
+  }
+  if (!ok)
+    flags[1] = 1;
+}
+
+template <class F>
+void dispatch_items(int items, F&& f)
+{
+  switch (items) {
+    case 1:
+      f(std::integral_constant<int, 1>{});
+      break;
+    case 2:
+      f(std::integral_constant<int, 2>{});
+      break;
+    case 4:
+      f(std::integral_constant<int, 4>{});
+      break;
+    case 8:
+      f(std::integral_constant<int, 8>{});
+      break;
+    default:
+      HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "items_per_thread must be 1, 2, 4 or 8");
+  }
 }
 
 inline unsigned blocks_for(int64_t n, int block)
@@ -348,16 +557,26 @@ inline unsigned blocks_for(int64_t n, int block)
   return unsigned((n + block - 1) / block);
 }
 
+inline unsigned hw_blocks(int64_t nb)
+{
+  return unsigned(((nb + 7) / 8) * 8);
+}
+
 } // namespace
 
 // per-operator solver workspace that the header does not need to know about
 struct SolverCache {
   DeviceArray<int64_t> block_slice0;
-  DeviceArray<int> bad;
+  DeviceArray<int32_t> dia_k0;
+  DeviceArray<int32_t> dia_mirror;
+  DeviceArray<int32_t> row_inst;
+  DeviceArray<int> flags; // [0] non-positive diagonal, [1] asymmetric
+  int* host_flags = nullptr;
   PcgScalars* host_scalars[2] = {nullptr, nullptr};
   int host_capacity = 0;
   hipEvent_t copied[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  std::vector<hipEvent_t> timing_events;
   hipStream_t own_stream = nullptr;
   hipEvent_t join = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -372,9 +591,13 @@ struct SolverCache {
     for (auto& h : host_scalars)
       if (h)
         (void)hipHostFree(h);
+    if (host_flags)
+      (void)hipHostFree(host_flags);
     for (auto& e : copied)
       if (e)
         (void)hipEventDestroy(e);
+    for (auto e : timing_events)
+      (void)hipEventDestroy(e);
     if (t0)
       (void)hipEventDestroy(t0);
     if (t1)
@@ -399,33 +622,61 @@ SolverCache& cache_of(Operator& op)
   return *op.solver_cache;
 }
 
-// row-block partition and storage structure of the operator (host set-up)
+void RowPartition::build(const std::vector<int64_t>& inst_dof, int block_size, int items_per_thread)
+{
+  block = block_size, items = items_per_thread;
+  const int64_t R = int64_t(block) * items;
+  row0.clear(), inst.clear(), inst_block0.assign(1, 0);
+  const int ninst = int(inst_dof.size()) - 1;
+  for (int in = 0; in < ninst; ++in) {
+    for (int64_t r = inst_dof[in]; r < inst_dof[in + 1]; r += R) {
+      row0.push_back(r);
+      inst.push_back(in);
+    }
+    inst_block0.push_back(int32_t(row0.size()));
+  }
+  row0.push_back(inst_dof.back());
+  nblocks = int64_t(inst.size());
+  if (nblocks == 0)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "operator has no rows");
+  if (nblocks >= (int64_t(1) << 31) / 8)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many row blocks");
+  d_row0.upload(row0);
+  d_inst.upload(inst);
+  d_inst_block0.upload(inst_block0);
+}
+
+// row-block partitions of the operator (host set-up, O(number of blocks))
+static void build_partitions(Operator& op)
+{
+  SolverFormat& F = op.fmt;
+  SolverCache& C = cache_of(op);
+  const Space& s = *op.space;
+  // default launch shapes: one row per lane for the operator application (many blocks keep enough
+  // loads in flight); the vector kernels take ~1024 blocks so that every block's reduction of an
+  // instance's partials stays small
+  if (!op.launch[3].user)
+    op.launch[3].block = 256, op.launch[3].items = 1;
+  if (!op.launch[4].user) {
+    op.launch[4].block = 256;
+    const int64_t want = (op.n + 256 * 1024 - 1) / (256 * 1024);
+    op.launch[4].items = want <= 1 ? 1 : want <= 2 ? 2 : want <= 4 ? 4 : 8;
+  }
+  F.A.build(s.inst_dof, op.launch[3].block, op.launch[3].items);
+  F.V.build(s.inst_dof, op.launch[4].block, op.launch[4].items);
+  F.d_inst_row0.upload(s.inst_dof);
+  F.partitions_ready = true;
+  ++C.version; // captured graphs hold the launch shapes
+}
+
+// storage structure of the operator (host set-up): DIA offsets and CSR -> DIA positions, or the
+// SELL-64 slices of the A partition
 static void build_structure(Operator& op)
 {
   SolverFormat& F = op.fmt;
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
   const int ninst = int(s.inst_dof.size()) - 1;
-  const int block = op.launch[3].block, items = op.launch[3].items;
-  const int64_t R = int64_t(block) * items;
-  F.rows_per_block = int(R);
-  F.block_row0.clear(), F.block_inst.clear(), F.inst_block0.assign(1, 0);
-  for (int in = 0; in < ninst; ++in) {
-    const int64_t a = s.inst_dof[in], e = s.inst_dof[in + 1];
-    for (int64_t r = a; r < e; r += R) {
-      F.block_row0.push_back(r);
-      F.block_inst.push_back(in);
-    }
-    F.inst_block0.push_back(int32_t(F.block_row0.size()));
-  }
-  F.block_row0.push_back(op.n);
-  F.nblocks = int64_t(F.block_inst.size());
-  if (F.nblocks == 0)
-    HGDT_THROW(HGDT_ERR_OPERATOR, "operator has no rows");
-  F.d_block_row0.upload(F.block_row0);
-  F.d_block_inst.upload(F.block_inst);
-  F.d_inst_block0.upload(F.inst_block0);
-  F.d_inst_row0.upload(s.inst_dof);
 
   // DIA detection: distinct (col - row) offsets per instance
   std::vector<std::vector<int64_t>> offs(ninst);
@@ -455,7 +706,7 @@ static void build_structure(Operator& op)
           o.insert(it, d);
       }
     });
-    if (!ok || o.size() > size_t(MAXDIAG))
+    if (!ok || o.size() > size_t(MAXDIAG) || !std::binary_search(o.begin(), o.end(), int64_t(0)))
       dia_ok = false;
   }
   if (op.requested_format == HGDT_FORMAT_DIA && !dia_ok)
@@ -469,32 +720,51 @@ static void build_structure(Operator& op)
       nd = std::max(nd, int(o.size()));
     F.ndiag = nd;
     std::vector<int64_t> table(int64_t(ninst) * nd, op.n); // sentinel offset: never inside an instance
-    for (int in = 0; in < ninst; ++in)
+    std::vector<int32_t> k0(ninst, 0);
+    int nup = 0;
+    for (int in = 0; in < ninst; ++in) {
       for (size_t k = 0; k < offs[in].size(); ++k)
         table[int64_t(in) * nd + k] = offs[in][k];
+      k0[in] = int32_t(std::lower_bound(offs[in].begin(), offs[in].end(), int64_t(0)) - offs[in].begin());
+      nup = std::max(nup, int(offs[in].size()) - k0[in]);
+    }
+    F.nup = nup;
     std::vector<int64_t> pos(std::max<int64_t>(op.nnz, 1));
+    std::vector<int32_t> row_inst(std::max<int64_t>(op.n, 1));
     for (int in = 0; in < ninst; ++in) {
       const auto& o = offs[in];
       parallel_for(s.inst_dof[in + 1] - s.inst_dof[in], [&](int64_t b0, int64_t b1) {
-        for (int64_t i = s.inst_dof[in] + b0; i < s.inst_dof[in] + b1; ++i)
+        for (int64_t i = s.inst_dof[in] + b0; i < s.inst_dof[in] + b1; ++i) {
+          row_inst[i] = in;
           for (int64_t t = op.row_ptr[i]; t < op.row_ptr[i + 1]; ++t) {
             const int64_t k = std::lower_bound(o.begin(), o.end(), int64_t(op.cols[t]) - i) - o.begin();
             pos[t] = k * op.n + i;
           }
+        }
       });
     }
+    // mirror[k]: index of the diagonal with offset -offset[k] (-1 when the pattern is not symmetric)
+    std::vector<int32_t> mirror(int64_t(ninst) * nd, -1);
+    for (int in = 0; in < ninst; ++in)
+      for (size_t k = 0; k < offs[in].size(); ++k) {
+        auto it = std::lower_bound(offs[in].begin(), offs[in].end(), -offs[in][k]);
+        if (it != offs[in].end() && *it == -offs[in][k])
+          mirror[int64_t(in) * nd + k] = int32_t(it - offs[in].begin());
+      }
+    C.dia_mirror.upload(mirror);
     F.d_dia_offsets.upload(table);
+    C.dia_k0.upload(k0);
+    C.row_inst.upload(row_inst);
     F.d_dia_pos.upload(pos);
     F.d_dia.allocate(std::max<int64_t>(int64_t(nd) * op.n, 1));
-    F.bytes = int64_t(nd) * 8 * op.n + 16 * op.n;
   } else {
     F.format = HGDT_FORMAT_SELL;
     std::vector<int64_t> slice_ptr{0}, bslice0;
-    for (int64_t b = 0; b < F.nblocks; ++b) {
+    for (int64_t b = 0; b < F.A.nblocks; ++b) {
       bslice0.push_back(int64_t(slice_ptr.size()) - 1);
-      for (int64_t r = F.block_row0[b]; r < F.block_row0[b + 1]; r += 64) {
+      for (int64_t r = F.A.row0[b]; r < F.A.row0[b + 1]; r += 64) {
         int64_t w = 0;
-        for (int64_t i = r; i < std::min(r + 64, F.block_row0[b + 1]); ++i)
+        for (int64_t i = r; i < std::min(r + 64, F.A.row0[b + 1]); ++i)
           w = std::max(w, op.row_ptr[i + 1] - op.row_ptr[i]);
         slice_ptr.push_back(slice_ptr.back() + 64 * w);
       }
@@ -504,14 +774,14 @@ static void build_structure(Operator& op)
     const int64_t total = slice_ptr.back();
     std::vector<int32_t> scols(std::max<int64_t>(total, 1));
     std::vector<int64_t> pos(std::max<int64_t>(op.nnz, 1));
-    parallel_for(F.nblocks, [&](int64_t b0, int64_t b1) {
+    parallel_for(F.A.nblocks, [&](int64_t b0, int64_t b1) {
       for (int64_t b = b0; b < b1; ++b) {
         int64_t sl = bslice0[b];
-        for (int64_t r = F.block_row0[b]; r < F.block_row0[b + 1]; r += 64, ++sl) {
+        for (int64_t r = F.A.row0[b]; r < F.A.row0[b + 1]; r += 64, ++sl) {
           const int64_t w = (slice_ptr[sl + 1] - slice_ptr[sl]) / 64;
           for (int64_t i = r; i < r + 64; ++i) {
             const int lane = int(i - r);
-            const int64_t len = (i < F.block_row0[b + 1]) ? op.row_ptr[i + 1] - op.row_ptr[i] : 0;
+            const int64_t len = (i < F.A.row0[b + 1]) ? op.row_ptr[i + 1] - op.row_ptr[i] : 0;
             for (int64_t k = 0; k < w; ++k) {
               const int64_t at = slice_ptr[sl] + k * 64 + lane;
               if (k < len) {
@@ -530,10 +800,19 @@ static void build_structure(Operator& op)
     F.d_sell_cols.upload(scols);
     F.d_sell_pos.upload(pos);
     F.d_sell_vals.allocate(std::max<int64_t>(total, 1));
-    F.bytes = 12 * op.nnz + 16 * op.n;
   }
   F.structure_ready = true;
+  F.valid = false;
   ++C.version;
+}
+
+// algorithmic bytes of one operator application in the format in use
+static int64_t apply_bytes(const Operator& op)
+{
+  const SolverFormat& F = op.fmt;
+  if (F.format == HGDT_FORMAT_DIA)
+    return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
+  return 12 * op.nnz + 16 * op.n;
 }
 
 static PcgParams make_params(Operator& op)
@@ -542,46 +821,73 @@ static PcgParams make_params(Operator& op)
   SolverCache& C = cache_of(op);
   PcgParams P{};
   P.n = op.n;
-  P.items = op.launch[3].items;
-  P.block_row0 = F.d_block_row0.ptr;
-  P.block_inst = F.d_block_inst.ptr;
-  P.inst_block0 = F.d_inst_block0.ptr;
+  P.a_nb = int(F.A.nblocks);
+  P.a_row0 = F.A.d_row0.ptr;
+  P.a_inst = F.A.d_inst.ptr;
+  P.a_ib0 = F.A.d_inst_block0.ptr;
+  P.v_nb = int(F.V.nblocks);
+  P.v_row0 = F.V.d_row0.ptr;
+  P.v_inst = F.V.d_inst.ptr;
+  P.v_ib0 = F.V.d_inst_block0.ptr;
   P.inst_row0 = F.d_inst_row0.ptr;
   P.ndiag = F.ndiag;
   P.dia_off = F.d_dia_offsets.ptr;
+  P.dia_k0 = C.dia_k0.ptr;
   P.dia = F.d_dia.ptr;
   P.block_slice0 = C.block_slice0.ptr;
   P.slice_ptr = F.d_slice_ptr.ptr;
   P.sell_cols = F.d_sell_cols.ptr;
   P.sell_vals = F.d_sell_vals.ptr;
   P.dinv = F.d_dinv.ptr;
+  P.ninst = int(op.space->inst_dof.size()) - 1;
   return P;
+}
+
+static int kernel_format(const Operator& op)
+{
+  if (op.fmt.format == HGDT_FORMAT_SELL)
+    return FMT_SELL;
+  return op.fmt.symmetric ? FMT_DIAS : FMT_DIA;
 }
 
 void Operator::prepare_solver(hipStream_t stream)
 {
   SolverFormat& F = fmt;
   SolverCache& C = cache_of(*this);
+  if (!F.partitions_ready)
+    build_partitions(*this);
   if (!F.structure_ready)
     build_structure(*this);
   if (F.valid)
     return;
   const int B = 256;
-  double* dst = (F.format == HGDT_FORMAT_DIA) ? F.d_dia.ptr : F.d_sell_vals.ptr;
-  const int64_t total = (F.format == HGDT_FORMAT_DIA) ? int64_t(F.ndiag) * n : int64_t(F.d_sell_vals.count);
+  const bool dia = F.format == HGDT_FORMAT_DIA;
+  double* dst = dia ? F.d_dia.ptr : F.d_sell_vals.ptr;
+  const int64_t total = dia ? int64_t(F.ndiag) * n : int64_t(F.d_sell_vals.count);
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
   hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
-                     (F.format == HGDT_FORMAT_DIA) ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
+                     dia ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
   F.d_dinv.allocate(std::max<int64_t>(n, 1));
-  C.bad.allocate(1);
-  HGDT_HIP(hipMemsetAsync(C.bad.ptr, 0, sizeof(int), stream));
-  hipLaunchKernelGGL(k_jacobi, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr, d_values.ptr,
-                     F.d_dinv.ptr, C.bad.ptr);
+  C.flags.allocate(2);
+  if (!C.host_flags)
+    HGDT_HIP(hipHostMalloc((void**)&C.host_flags, 2 * sizeof(int)));
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
+  if (dia)
+    hipLaunchKernelGGL(k_dia_prepare, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, F.ndiag, F.d_dia_offsets.ptr,
+                       C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dinv.ptr,
+                       C.flags.ptr);
+  else
+    hipLaunchKernelGGL(k_jacobi, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
+                       d_values.ptr, F.d_dinv.ptr, C.flags.ptr);
   HGDT_HIP(hipGetLastError());
-  int bad = 0;
-  HGDT_HIP(hipMemcpyAsync(&bad, C.bad.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
   HGDT_HIP(hipStreamSynchronize(stream));
-  F.diag_positive = (bad == 0);
+  F.diag_positive = (C.host_flags[0] == 0);
+  const bool sym = dia && C.host_flags[1] == 0 && allow_symmetric;
+  if (sym != F.symmetric)
+    ++C.version; // captured graphs hold the kernel variant
+  F.symmetric = sym;
+  F.bytes = apply_bytes(*this);
   F.valid = true;
 }
 
@@ -589,29 +895,47 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
 {
   prepare_solver(stream);
   PcgParams P = make_params(*this);
-  const int block = launch[3].block;
-  if (fmt.format == HGDT_FORMAT_DIA)
-    hipLaunchKernelGGL(k_apply<HGDT_FORMAT_DIA>, dim3(unsigned(fmt.nblocks)), dim3(block), 0, stream, P, x, y);
-  else
-    hipLaunchKernelGGL(k_apply<HGDT_FORMAT_SELL>, dim3(unsigned(fmt.nblocks)), dim3(block), 0, stream, P, x, y);
+  const int block = fmt.A.block;
+  const unsigned grid = hw_blocks(fmt.A.nblocks);
+  const int kf = kernel_format(*this);
+  dispatch_items(fmt.A.items, [&](auto IT) {
+    constexpr int I = decltype(IT)::value;
+    if (kf == FMT_DIA)
+      hipLaunchKernelGGL((k_apply<FMT_DIA, I>), dim3(grid), dim3(block), 0, stream, P, x, y);
+    else if (kf == FMT_DIAS)
+      hipLaunchKernelGGL((k_apply<FMT_DIAS, I>), dim3(grid), dim3(block), 0, stream, P, x, y);
+    else
+      hipLaunchKernelGGL((k_apply<FMT_SELL, I>), dim3(grid), dim3(block), 0, stream, P, x, y);
+  });
   HGDT_HIP(hipGetLastError());
 }
 
-static void launch_iteration(const Operator& op, const PcgParams& P, hipStream_t stream, hipEvent_t ev0,
+static void launch_iteration(const Operator& op, const PcgParams& P, int par, hipStream_t stream, hipEvent_t ev0,
                              hipEvent_t ev1)
 {
-  const unsigned nb = unsigned(op.fmt.nblocks);
-  const int block = op.launch[3].block;
+  const RowPartition& A = op.fmt.A;
+  const RowPartition& V = op.fmt.V;
+  const int kf = kernel_format(op);
   if (ev0)
     HGDT_HIP(hipEventRecord(ev0, stream));
-  if (op.fmt.format == HGDT_FORMAT_DIA)
-    hipLaunchKernelGGL(k_pcg_spmv<HGDT_FORMAT_DIA>, dim3(nb), dim3(block), 0, stream, P);
-  else
-    hipLaunchKernelGGL(k_pcg_spmv<HGDT_FORMAT_SELL>, dim3(nb), dim3(block), 0, stream, P);
+  dispatch_items(A.items, [&](auto IT) {
+    constexpr int I = decltype(IT)::value;
+    const unsigned grid = hw_blocks(A.nblocks);
+    if (kf == FMT_DIA)
+      hipLaunchKernelGGL((k_pcg_spmv<FMT_DIA, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+    else if (kf == FMT_DIAS)
+      hipLaunchKernelGGL((k_pcg_spmv<FMT_DIAS, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+    else
+      hipLaunchKernelGGL((k_pcg_spmv<FMT_SELL, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+  });
   if (ev1)
     HGDT_HIP(hipEventRecord(ev1, stream));
-  hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(block), 0, stream, P);
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(nb), dim3(block), 0, stream, P);
+  dispatch_items(V.items, [&](auto IT) {
+    constexpr int I = decltype(IT)::value;
+    const unsigned grid = hw_blocks(V.nblocks);
+    hipLaunchKernelGGL((k_pcg_update<I>), dim3(grid), dim3(V.block), 0, stream, P, par);
+    hipLaunchKernelGGL((k_pcg_pupdate<I>), dim3(grid), dim3(V.block), 0, stream, P, par);
+  });
 }
 
 void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_options& opt, hgdt_solver_stats& st,
@@ -627,8 +951,8 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   d_r.allocate(std::max<int64_t>(n, 1));
   d_p.allocate(std::max<int64_t>(n, 1));
   d_q.allocate(std::max<int64_t>(n, 1));
-  d_partials.allocate(std::max<int64_t>(3 * fmt.nblocks, 1));
-  d_scalars.allocate(ninst);
+  d_partials.allocate(std::max<int64_t>(fmt.A.nblocks + 5 * fmt.V.nblocks, 1));
+  d_scalars.allocate(2 * ninst);
   if (C.host_capacity < ninst) {
     for (auto& h : C.host_scalars) {
       if (h)
@@ -658,21 +982,27 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   }
   PcgParams P = make_params(*this);
   P.b = b, P.x = x, P.r = d_r.ptr, P.p = d_p.ptr, P.q = d_q.ptr;
-  P.partials = d_partials.ptr, P.S = d_scalars.ptr;
+  P.part1 = d_partials.ptr;
+  P.part2 = d_partials.ptr + fmt.A.nblocks;
+  P.part3 = d_partials.ptr + fmt.A.nblocks + 2 * fmt.V.nblocks;
+  P.S = d_scalars.ptr;
   P.rtol2 = opt.rtol * opt.rtol, P.max_iter = opt.max_iter, P.zero_guess = opt.zero_initial_guess ? 1 : 0;
-  const int block = launch[3].block;
   HGDT_HIP(hipEventRecord(C.t0, stream));
-  hipLaunchKernelGGL(k_reset_scalars, dim3(blocks_for(ninst, 256)), dim3(256), 0, stream, d_scalars.ptr, ninst);
   if (!opt.zero_initial_guess) {
     apply(x, d_q.ptr, stream);
     P.ax = d_q.ptr;
   }
-  hipLaunchKernelGGL(k_pcg_init, dim3(unsigned(fmt.nblocks)), dim3(block), 0, stream, P);
+  dispatch_items(fmt.V.items, [&](auto IT) {
+    constexpr int I = decltype(IT)::value;
+    hipLaunchKernelGGL((k_pcg_init<I>), dim3(hw_blocks(fmt.V.nblocks)), dim3(fmt.V.block), 0, stream, P);
+  });
+  hipLaunchKernelGGL(k_pcg_init_scalars, dim3(unsigned(ninst)), dim3(256), 0, stream, P);
   HGDT_HIP(hipGetLastError());
   P.ax = nullptr;
 
-  std::vector<hipEvent_t> evs;
-  const int chunk = opt.check_every;
+  // an even number of iterations per chunk keeps the parity of every captured launch fixed and
+  // leaves the current scalars in record 0 after each chunk
+  const int chunk = opt.check_every + (opt.check_every & 1);
   if (graph && (C.graph == nullptr || C.graph_x != x || C.graph_chunk != chunk || C.graph_version != C.version
                 || C.graph_b != b || C.graph_maxit != opt.max_iter)) {
     if (C.graph)
@@ -681,25 +1011,35 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     hipGraph_t g;
     HGDT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int it = 0; it < chunk; ++it)
-      launch_iteration(*this, P, stream, nullptr, nullptr);
+      launch_iteration(*this, P, it & 1, stream, nullptr, nullptr);
     HGDT_HIP(hipStreamEndCapture(stream, &g));
     HGDT_HIP(hipGraphInstantiate(&C.graph, g, nullptr, nullptr, 0));
     HGDT_HIP(hipGraphDestroy(g));
     C.graph_x = x, C.graph_b = b, C.graph_chunk = chunk, C.graph_version = C.version;
     C.graph_maxit = opt.max_iter;
   }
+  size_t nev = 0;
+  int64_t issued = 0; // iterations launched so far
+  std::vector<int64_t> timed_iter;
+  const int64_t stride = opt.time_spmv > 0 ? opt.time_spmv : 0;
   auto launch_chunk = [&]() {
     if (graph) {
       HGDT_HIP(hipGraphLaunch(C.graph, stream));
+      issued += chunk;
     } else {
-      for (int it = 0; it < chunk; ++it) {
+      for (int it = 0; it < chunk; ++it, ++issued) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (opt.time_spmv) {
-          HGDT_HIP(hipEventCreate(&e0));
-          HGDT_HIP(hipEventCreate(&e1));
-          evs.push_back(e0), evs.push_back(e1);
+        if (stride && issued % stride == 0) {
+          timed_iter.push_back(issued);
+          while (C.timing_events.size() < nev + 2) {
+            hipEvent_t e;
+            HGDT_HIP(hipEventCreate(&e));
+            C.timing_events.push_back(e);
+          }
+          e0 = C.timing_events[nev], e1 = C.timing_events[nev + 1];
+          nev += 2;
         }
-        launch_iteration(*this, P, stream, e0, e1);
+        launch_iteration(*this, P, it & 1, stream, e0, e1);
       }
     }
     HGDT_HIP(hipGetLastError());
@@ -763,20 +1103,17 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   float ms = 0.f;
   HGDT_HIP(hipEventElapsedTime(&ms, C.t0, C.t1));
   st.solve_ms = ms;
-  if (opt.time_spmv && !evs.empty()) {
+  if (opt.time_spmv && nev > 0) {
     // only iterations that did work: the early-exit launches after convergence are skipped
     double total = 0.;
     int64_t cnt = 0;
-    const int64_t useful = st.iterations;
-    for (size_t k = 0; k + 1 < evs.size(); k += 2) {
-      if (int64_t(k / 2) < useful) {
+    for (size_t k = 0; k + 1 < nev; k += 2) {
+      if (timed_iter[k / 2] < st.iterations) {
         float t = 0.f;
-        HGDT_HIP(hipEventElapsedTime(&t, evs[k], evs[k + 1]));
+        HGDT_HIP(hipEventElapsedTime(&t, C.timing_events[k], C.timing_events[k + 1]));
         total += t, ++cnt;
       }
     }
-    for (auto e : evs)
-      (void)hipEventDestroy(e);
     st.spmv_avg_ms = cnt ? total / cnt : 0.;
     st.spmv_count = cnt;
   }

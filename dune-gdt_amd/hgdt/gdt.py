@@ -640,7 +640,7 @@ class MatrixOperator:
         o.max_iter = int(opts.get("max_iter", 100000))
         o.check_every = int(check_every)
         o.use_graph = int(bool(use_graph))
-        o.time_spmv = int(bool(time_spmv))
+        o.time_spmv = int(time_spmv)  # True -> every application, k -> every k-th
         o.zero_initial_guess = 0 if (has_source and opts.get("initial_guess", False)) else 1
         st = L.SolverStats()
         status = L.load().hgdt_operator_apply_inverse(self._h, b_ptr, ctypes.c_void_p(x.data_ptr()), ctypes.byref(o),
@@ -672,7 +672,8 @@ class MatrixOperator:
     def format_info(self):
         f, b, d = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
         L.call("hgdt_operator_format_info", self._h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(d))
-        return {"format": {L.FORMAT_SELL: "sell64", L.FORMAT_DIA: "dia"}[f.value], "spmv_bytes": int(b.value),
+        return {"format": {L.FORMAT_SELL: "sell64", L.FORMAT_DIA: "dia_symmetric",
+                           L.FORMAT_DIA_NONSYMMETRIC: "dia"}[f.value], "spmv_bytes": int(b.value),
                 "num_diagonals": int(d.value)}
 
     def set_launch(self, kernel, block_size, items_per_thread=1):

@@ -52,7 +52,7 @@ enum { HGDT_STENCIL_ELEMENT = 0, HGDT_STENCIL_INTERSECTION = 1, HGDT_STENCIL_ELE
 /* source data kinds for the right-hand side functional */
 enum { HGDT_SOURCE_CONSTANT = 0, HGDT_SOURCE_PER_ELEMENT = 1, HGDT_SOURCE_PER_QUADRATURE_POINT = 2 };
 /* matrix storage used for apply / apply_inverse */
-enum { HGDT_FORMAT_AUTO = 0, HGDT_FORMAT_SELL = 1, HGDT_FORMAT_DIA = 2 };
+enum { HGDT_FORMAT_AUTO = 0, HGDT_FORMAT_SELL = 1, HGDT_FORMAT_DIA = 2, HGDT_FORMAT_DIA_NONSYMMETRIC = 3 };
 
 typedef struct hgdt_grid_s* hgdt_grid;
 typedef struct hgdt_space_s* hgdt_space;
@@ -192,11 +192,13 @@ hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out);
 hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
 
 /* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: DIA when every
- * instance has at most 32 distinct diagonals, else SELL-64).  The format is rebuilt from the CSR
+ * instance has at most 32 distinct diagonals, else SELL-64).  DIA reads only the diagonals with
+ * offset >= 0 when the assembled matrix is exactly (bitwise) symmetric;
+ * HGDT_FORMAT_DIA_NONSYMMETRIC always reads every diagonal.  The format is rebuilt from the CSR
  * values at the next apply / apply_inverse after each assemble. */
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format);
-/* format actually in use after the last preparation (HGDT_FORMAT_SELL or HGDT_FORMAT_DIA) and the
- * algorithmic bytes one application of the operator reads and writes */
+/* format actually in use after the last preparation (HGDT_FORMAT_SELL, HGDT_FORMAT_DIA for symmetric
+ * DIA, HGDT_FORMAT_DIA_NONSYMMETRIC) and the algorithmic bytes one application reads and writes */
 hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t* spmv_bytes, int32_t* num_diagonals);
 
 /* ConstMatrixOperator::apply (matrix-based.hh:121-129): y = A x, device vectors of length rows */
@@ -207,7 +209,7 @@ typedef struct hgdt_solver_options {
   int64_t max_iter;       /* per instance */
   int32_t check_every;    /* iterations between host-side convergence checks (>= 1) */
   int32_t use_graph;      /* 1: replay the iteration block as a captured hipGraph */
-  int32_t time_spmv;      /* 1: record HIP events around every operator application */
+  int32_t time_spmv;      /* k > 0: record HIP events around every k-th operator application */
   int32_t zero_initial_guess; /* 1: start from x = 0, else from the given x */
 } hgdt_solver_options;
 

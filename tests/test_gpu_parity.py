@@ -98,12 +98,31 @@ def test_apply_parity(hgdt_mod, kind, cells, seed):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(op.shape[0])
     yo = O.spmv(orp, ocols, oop.values(), x)
-    for fmt in (hgdt_mod._lib.FORMAT_AUTO, hgdt_mod._lib.FORMAT_SELL):
+    L = hgdt_mod._lib
+    ys = {}
+    for fmt in (L.FORMAT_AUTO, L.FORMAT_DIA_NONSYMMETRIC, L.FORMAT_SELL):
+        if fmt == L.FORMAT_DIA_NONSYMMETRIC and kind == "swipdg":
+            continue
         op.set_format(fmt)
         y = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
         err = np.abs(y - yo).max() / np.abs(yo).max()
-        print(f"  {op.format_info()['format']}: max rel err {err:.3e}")
+        name = op.format_info()["format"]
+        ys[name] = y
+        print(f"  {name}: max rel err {err:.3e}")
         assert err <= 1e-13
+    if kind != "swipdg":
+        # continuous Lagrange matrices are assembled exactly symmetric, so the symmetric DIA path is taken
+        # and, summing every row in column order, agrees bit for bit with full DIA and SELL
+        assert "dia_symmetric" in ys
+        assert np.array_equal(ys["dia_symmetric"], ys["dia"]) and np.array_equal(ys["dia"], ys["sell64"])
+
+
+@pytest.mark.parametrize("kind,cells,seed", [CASES[1], CASES[4]])
+def test_assembled_matrix_exactly_symmetric(hgdt_mod, kind, cells, seed):
+    print(f"case symmetry {kind} cells={cells} seed={seed}", flush=True)
+    oop, op = _build(hgdt_mod, kind, cells, seed)
+    A = op.matrix.to_scipy()
+    assert (A != A.T).nnz == 0
 
 
 def test_bitwise_reproducible(hgdt_mod):
@@ -145,3 +164,27 @@ def test_solver_failure_is_reported(hgdt_mod):
     op.assemble()
     with pytest.raises(hgdt_mod.linear_solver_failed):
         op.apply_inverse(op.rhs)
+
+
+def test_launch_shapes_and_autotuner(hgdt_mod):
+    """every launch shape gives the oracle's solution (1e-10 rel. L2); the tuner picks one of its candidates"""
+    print("case launch shapes p1_tets 8^3", flush=True)
+    from hgdt import autotune
+
+    kind, cells, seed = "p1_tets", [8, 8, 8], 4
+    oop, op = _build(hgdt_mod, kind, cells, seed)
+    orp, ocols = oop.pattern()
+    xo, ito, relo, rc = O.pcg_jacobi(orp, ocols, oop.values(), oop.rhs(), rtol=1e-10)
+    L = hgdt_mod._lib
+    for a, v in [((64, 1), (256, 8)), ((512, 2), (1024, 4)), ((128, 1), (128, 1)), ((1024, 1), (64, 2))]:
+        op.set_launch(L.KERNEL_APPLY, *a)
+        op.set_launch(L.KERNEL_VECTOR, *v)
+        x = op.apply_inverse("rhs").cpu().numpy()
+        rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        print(f"  apply {a} vector {v}: iters {op.last_solve['iterations']} rel {rel:.2e}")
+        assert rel <= SOLUTION_TOL and abs(op.last_solve["iterations"] - ito) <= 2
+    choice = autotune.tune_solver(op, iterations=10, repeats=1)
+    assert tuple(choice["apply"]) in autotune.APPLY_CANDIDATES
+    assert tuple(choice["vector"]) in autotune.VECTOR_CANDIDATES
+    x = op.apply_inverse("rhs").cpu().numpy()
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= SOLUTION_TOL
