@@ -65,6 +65,142 @@ def build_problem(hgdt, torch, config):
     return grid, space, op, kappa, workload
 
 
+def batch_problem(hgdt, grid, kappa, f):
+    space = hgdt.ContinuousLagrangeSpace(grid, 1)
+    op = hgdt.make_matrix_operator(space, hgdt.Stencil.element)
+    op.append(hgdt.LocalElementIntegralBilinearForm(hgdt.LocalLaplaceIntegrand(kappa)))
+    rhs = hgdt.make_vector_functional(space)
+    rhs.append(hgdt.LocalElementIntegralFunctional(hgdt.LocalProductIntegrand().with_ansatz(f)))
+    op.append(rhs)
+    op.append(hgdt.make_dirichlet_constraints(space, hgdt.AllDirichletBoundaryInfo()))
+    return space, op
+
+
+def run_batched(hgdt, torch, args):
+    """256 independent Q1 problems on n x n grids, n ~ U{32..512}: one assembly launch set and one
+    graph-replayed PCG for the whole batch vs 256 sequential plain GPU solves."""
+    import numpy as np
+
+    F = hgdt.fields
+    t_setup = time.perf_counter()
+    sizes = F.batch_sizes(4, 256)
+    parts = [hgdt.Grid.structured("quad", [0, 0], [1, 1], [int(n), int(n)]) for n in sizes]
+    union = hgdt.Grid.union(parts)
+    kappa = F.lognormal(41, union.num_elements)
+    f = F.uniform(42, union.num_elements)
+    space, op = batch_problem(hgdt, union, torch.from_numpy(kappa).cuda(), torch.from_numpy(f).cuda())
+    op.assemble()
+    op.apply_inverse("rhs", use_graph=True)
+    tuned = None
+    if not args.no_tune:
+        from hgdt import autotune
+
+        tuned = autotune.tune_solver(op)
+        tuned.pop("timings", None)
+    eoff = union.instance_elements()
+    plain = []
+    for i, n in enumerate(sizes):
+        s1, op1 = batch_problem(hgdt, parts[i], torch.from_numpy(kappa[eoff[i]:eoff[i + 1]]).cuda(),
+                                torch.from_numpy(f[eoff[i]:eoff[i + 1]]).cuda())
+        op1.assemble()
+        op1.apply_inverse("rhs", use_graph=True)
+        plain.append((s1, op1))
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    log(f"batched set-up {t_setup:.1f} s: {len(sizes)} instances, {space.num_DoFs} DoFs, nnz {op.nnz}, "
+        f"launch {tuned}")
+
+    def step():
+        op.assemble()
+        return op.apply_inverse("rhs", use_graph=True, check_every=16)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    its, rels = op.instance_stats()
+    # plain: every instance assembled and solved on its own, one after the other
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s1, op1 in plain:
+        op1.assemble()
+        op1.apply_inverse("rhs", use_graph=True, check_every=16)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    n = space.num_DoFs
+    return {"batched_dofs_per_s": n / t_batch, "plain_dofs_per_s": n / t_plain, "speedup_vs_plain": t_plain / t_batch,
+            "ms_per_batch": t_batch * 1e3, "ms_plain_sequential": t_plain * 1e3, "instances": len(sizes),
+            "dofs": n, "nnz": op.nnz, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
+            "relres_max": float(rels.max()), "launch": tuned, "setup_s": t_setup,
+            "workload": "256_q1_quads_n_U32-512_lognormal_kappa_uniform_f_dirichlet0"}
+
+
+def run_adjoint(hgdt, torch, args):
+    """32 P1 problems on 64^3 x 6 tets: J = 1/2 |u(log kappa) - u_obs|^2, forward CG + adjoint CG +
+    adjoint element kernel through torch.autograd, batched vs the plain per-instance path."""
+    from hgdt.autograd import DiffusionProblem, solve_log_diffusion
+
+    F = hgdt.fields
+    ninst, c = 32, 64
+    t_setup = time.perf_counter()
+    part = hgdt.Grid.structured("tet", [0, 0, 0], [1, 1, 1], [c, c, c])
+    union = hgdt.Grid.union([part] * ninst)
+    E1 = part.num_elements
+    logk = torch.from_numpy(F.normal(51, E1 * ninst)).cuda()
+    logk_obs = torch.from_numpy(F.normal(52, E1 * ninst)).cuda()
+    prob = DiffusionProblem(union, source=1.0, rtol=1e-10)
+    with torch.no_grad():
+        u_obs = solve_log_diffusion(logk_obs, prob).clone()
+    singles = [DiffusionProblem(part, source=1.0, rtol=1e-10) for _ in range(ninst)]
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    n1 = singles[0].space.num_DoFs
+    log(f"adjoint set-up {t_setup:.1f} s: {ninst} x {n1} DoFs")
+
+    def batched():
+        theta = logk.clone().requires_grad_(True)
+        u = solve_log_diffusion(theta, prob)
+        J = 0.5 * torch.sum((u - u_obs) ** 2)
+        J.backward()
+        return theta.grad
+
+    def plain():
+        grads = []
+        for i, sp in enumerate(singles):
+            theta = logk[i * E1:(i + 1) * E1].clone().requires_grad_(True)
+            u = solve_log_diffusion(theta, sp)
+            J = 0.5 * torch.sum((u - u_obs[i * n1:(i + 1) * n1]) ** 2)
+            J.backward()
+            grads.append(theta.grad)
+        return torch.cat(grads)
+
+    for _ in range(args.warmup):
+        batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gb = batched()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gp = plain()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    agree = float((gb - gp).abs().max() / gp.abs().max())
+    n = ninst * n1
+    return {"batched_dofs_per_s": n / t_batch, "plain_dofs_per_s": n / t_plain, "speedup_vs_plain": t_plain / t_batch,
+            "ms_per_gradient_batch": t_batch * 1e3, "ms_plain_sequential": t_plain * 1e3, "instances": ninst,
+            "dofs": n, "forward_iterations": prob.last_forward["iterations"],
+            "adjoint_iterations": prob.last_adjoint["iterations"], "batched_vs_plain_max_rel_diff": agree,
+            "setup_s": t_setup, "workload": "32_p1_kuhn_tets_64x64x64x6_logkappa_N01_J_half_l2_misfit"}
+
+
 def cpu_baseline(config, kappa_np, iterations, sample_iters):
     """The oracle (CPU restatement) on this host: full assembly + a bounded number of PCG iterations;
     PCG time projected to the GPU's iteration count (same recursion and stopping rule)."""
@@ -116,7 +252,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--extras", action="store_true",
+                    help="add the batched (256 uneven Q1) and adjoint (32 x P1 64^3) comparisons to the line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=40)
     ap.add_argument("--graph", action="store_true", help="replay PCG iteration blocks as a hipGraph")
@@ -135,6 +273,15 @@ def main():
     import hgdt
 
     torch.cuda.set_device(0)
+    if args.config in ("c4", "c5"):
+        res = run_batched(hgdt, torch, args) if args.config == "c4" else run_adjoint(hgdt, torch, args)
+        line = {"metric": METRIC, "value": res["batched_dofs_per_s"], "unit": "DoFs/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": res.get("ms_per_batch", res.get("ms_per_gradient_batch")), "higher_is_better": True,
+                "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded fields)",
+                "config": res}
+        print(json.dumps(line), flush=True)
+        return
     t_setup = time.perf_counter()
     grid, space, op, kappa, workload = build_problem(hgdt, torch, args.config)
     op.assemble()
@@ -206,6 +353,9 @@ def main():
                      "kernel": f"k_pcg_spmv<{fi['format']}>", "bytes_per_launch": fi["spmv_bytes"],
                      "avg_launch_us": spmv_ms * 1e3},
     }
+    if args.extras:
+        line["batched"] = run_batched(hgdt, torch, args)
+        line["adjoint"] = run_adjoint(hgdt, torch, args)
     if not args.no_cpu_baseline:
         kappa_np = kappa.cpu().numpy() if hasattr(kappa, "cpu") else kappa
         cb = cpu_baseline(args.config, kappa_np, iters, args.cpu_sample_iters)

@@ -211,16 +211,23 @@ hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_g
     u->conn.reserve(ne * u->ref.nverts);
     u->inst_elem = {0};
     u->inst_vert = {0};
+    u->nb.reserve(ne * u->ref.nfaces);
+    u->nbface.reserve(ne * u->ref.nfaces);
     for (int i = 0; i < count; ++i) {
       const Grid& g = *grids[i]->g;
       const int32_t voff = int32_t(u->inst_vert.back());
+      const int32_t eoff = int32_t(u->inst_elem.back());
       u->coords.insert(u->coords.end(), g.coords.begin(), g.coords.end());
       for (int32_t v : g.conn)
         u->conn.push_back(v + voff);
+      // the parts are disjoint: their intersections carry over with shifted element indices
+      for (int32_t o : g.nb)
+        u->nb.push_back(o < 0 ? o : o + eoff);
+      u->nbface.insert(u->nbface.end(), g.nbface.begin(), g.nbface.end());
       u->inst_elem.push_back(u->inst_elem.back() + g.ne);
       u->inst_vert.push_back(u->inst_vert.back() + g.nv);
     }
-    u->finalize();
+    u->finalize(/*neighbours_known=*/true);
     *grid = new hgdt_grid_s{u};
   });
 }

@@ -120,7 +120,8 @@ struct Grid {
   {
     return int(inst_elem.size()) - 1;
   }
-  void finalize(); // neighbours, face counts, device upload
+  void finalize(bool neighbours_known = false); // neighbours (face matching), face counts, device upload
+  void count_faces_and_upload();
   void check_affine() const;
 };
 

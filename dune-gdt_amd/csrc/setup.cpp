@@ -233,8 +233,12 @@ void Grid::check_affine() const
 
 // Intersections by face matching: faces are bucketed by their smallest vertex,
 // then matched inside each bucket (O(#faces) for bounded vertex valence).
-void Grid::finalize()
+void Grid::finalize(bool neighbours_known)
 {
+  if (neighbours_known) {
+    count_faces_and_upload();
+    return;
+  }
   const int nf = ref.nfaces, fk = ref.face_nverts, k = ref.nverts;
   const int64_t F = ne * nf;
   nb.assign(F, -1);
@@ -284,6 +288,13 @@ void Grid::finalize()
   });
   if (nonconforming)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "grid is not conforming: a face is shared by more than two elements");
+  count_faces_and_upload();
+}
+
+void Grid::count_faces_and_upload()
+{
+  const int nf = ref.nfaces;
+  const int64_t F = ne * nf;
   n_inner = 0, n_boundary = 0;
   for (int64_t ef = 0; ef < F; ++ef) {
     if (nb[ef] < 0)
