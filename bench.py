@@ -91,12 +91,7 @@ def run_batched(hgdt, torch, args):
     space, op = batch_problem(hgdt, union, torch.from_numpy(kappa).cuda(), torch.from_numpy(f).cuda())
     op.assemble()
     op.apply_inverse("rhs", use_graph=True)
-    tuned = None
-    if not args.no_tune:
-        from hgdt import autotune
-
-        tuned = autotune.tune_solver(op)
-        tuned.pop("timings", None)
+    tuned = choose_launch(op, args)
     eoff = union.instance_elements()
     plain = []
     for i, n in enumerate(sizes):
@@ -201,9 +196,28 @@ def run_adjoint(hgdt, torch, args):
             "setup_s": t_setup, "workload": "32_p1_kuhn_tets_64x64x64x6_logkappa_N01_J_half_l2_misfit"}
 
 
+def choose_launch(op, args):
+    """Launch shapes of the solver kernels: the recorded table (tuning only for an unknown signature),
+    a fresh tuning run, or the library defaults."""
+    if args.tune == "off":
+        return None
+    from hgdt import autotune
+
+    if args.tune == "table":
+        entry = autotune.apply_table(op)
+        if entry is not None:
+            return entry
+    choice = autotune.tune_solver(op)
+    choice.pop("timings", None)
+    if args.tune == "run" and args.tune_out:
+        autotune.save_table(op, choice, args.tune_out)
+    return choice
+
+
 def cpu_baseline(config, kappa_np, iterations, sample_iters):
-    """The oracle (CPU restatement) on this host: full assembly + a bounded number of PCG iterations;
-    PCG time projected to the GPU's iteration count (same recursion and stopping rule)."""
+    """The oracle (CPU restatement) on this host: assembly of the full workload + Jacobi-PCG, run to
+    convergence (sample_iters 0) or for a bounded number of iterations whose time is projected to the
+    GPU's iteration count (same recursion and stopping rule)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
@@ -239,12 +253,18 @@ def cpu_baseline(config, kappa_np, iterations, sample_iters):
     vals, b = op.values(), op.rhs()
     t0 = time.perf_counter()
     x, it, rel, rc = O.pcg_jacobi(rp, cols, vals, b, rtol=1e-10, nthreads=cores, max_iters_run=sample_iters)
-    t_it = (time.perf_counter() - t0) / max(it, 1)
-    t_total = t_asm + t_it * iterations
-    return {"value": s.size / t_total, "unit": "DoFs/s", "cores": cores, "kind": "port",
-            "sample": f"oracle assembly of the full workload ({t_asm:.2f} s, {cores} threads) + {it} PCG "
-                      f"iterations ({t_it * 1e3:.2f} ms each) projected to {iterations} iterations",
-            "t_assembly_s": t_asm, "t_iteration_s": t_it}
+    t_pcg = time.perf_counter() - t0
+    t_it = t_pcg / max(it, 1)
+    if sample_iters == 0:
+        t_total = t_asm + t_pcg
+        sample = (f"one full step of the workload: oracle assembly ({t_asm:.2f} s) + Jacobi-PCG to 1e-10 "
+                  f"({it} iterations, {t_pcg:.2f} s, relres {rel:.1e}), {cores} threads")
+    else:
+        t_total = t_asm + t_it * iterations
+        sample = (f"oracle assembly of the full workload ({t_asm:.2f} s, {cores} threads) + {it} PCG "
+                  f"iterations ({t_it * 1e3:.2f} ms each) projected to {iterations} iterations")
+    return {"value": s.size / t_total, "unit": "DoFs/s", "cores": cores, "kind": "port", "sample": sample,
+            "t_assembly_s": t_asm, "t_iteration_s": t_it, "iterations": it}
 
 
 def main():
@@ -256,12 +276,24 @@ def main():
     ap.add_argument("--extras", action="store_true",
                     help="add the batched (256 uneven Q1) and adjoint (32 x P1 64^3) comparisons to the line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    ap.add_argument("--cpu-sample-iters", type=int, default=None,
+                    help="oracle PCG iterations timed for the CPU baseline (0: to convergence; default 0, "
+                         "400 for c3)")
     ap.add_argument("--graph", action="store_true", help="replay PCG iteration blocks as a hipGraph")
-    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (set-up phase)")
+    ap.add_argument("--tune", default="table", choices=["table", "run", "off"],
+                    help="solver launch shapes: recorded table (tuning an unknown workload), fresh tuning run, "
+                         "or library defaults")
+    ap.add_argument("--tune-out", default=None, help="with --tune run: record the choice in this JSON table")
+    ap.add_argument("--no-tune", action="store_true", help="same as --tune off")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="issue one device-to-device copy of this size first (reference dispatch for HBM counters)")
     ap.add_argument("--spmv-sample", type=int, default=8,
                     help="HIP events around every k-th operator application of the timed steps")
     args = ap.parse_args()
+    if args.no_tune:
+        args.tune = "off"
+    if args.cpu_sample_iters is None:
+        args.cpu_sample_iters = 400 if args.config == "c3" else 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
               f"(multi-GPU sharding is design-only)", file=sys.stderr)
@@ -273,6 +305,17 @@ def main():
     import hgdt
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy_mib:
+        import ctypes
+
+        nb = args.calibrate_copy_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        hgdt._lib.call("hgdt_memcpy", ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nb,
+                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        del src, dst
     if args.config in ("c4", "c5"):
         res = run_batched(hgdt, torch, args) if args.config == "c4" else run_adjoint(hgdt, torch, args)
         line = {"metric": METRIC, "value": res["batched_dofs_per_s"], "unit": "DoFs/s", "n_gpus": 1,
@@ -287,13 +330,8 @@ def main():
     op.assemble()
     x = op.apply_inverse("rhs")
     torch.cuda.synchronize()
-    tuned = None
-    if not args.no_tune:
-        from hgdt import autotune
-
-        tuned = autotune.tune_solver(op)
-        tuned.pop("timings", None)
-        op.assemble()
+    tuned = choose_launch(op, args)
+    op.assemble()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     log(f"setup {t_setup:.1f} s: {space.num_DoFs} DoFs, {grid.num_elements} elements, nnz {op.nnz}, "

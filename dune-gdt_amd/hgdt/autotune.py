@@ -21,6 +21,9 @@ ASSEMBLY_CANDIDATES = [64, 128, 256]
 
 _memory = {}
 
+# launch shapes measured on MI355X for the benchmark workloads, keyed by signature()
+TABLE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "launch_table.json")
+
 
 def signature(op):
     fi = op.format_info()
@@ -88,6 +91,32 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
         with open(cache_file, "w") as f:
             json.dump(data, f, indent=1)
     return dict(choice, timings=timings, cached=False)
+
+
+def apply_table(op, table_file=TABLE_FILE):
+    """Set the solver launch shapes recorded for this operator's signature; returns the entry or None."""
+    if not os.path.exists(table_file):
+        return None
+    with open(table_file) as f:
+        entry = json.load(f).get(signature(op))
+    if entry is None:
+        return None
+    op.set_launch(L.KERNEL_APPLY, *entry["apply"])
+    op.set_launch(L.KERNEL_VECTOR, *entry["vector"])
+    return dict(entry, cached=True)
+
+
+def save_table(op, choice, table_file):
+    """Record a tune_solver() choice for this operator's signature in a JSON launch table."""
+    data = {}
+    if os.path.exists(table_file):
+        with open(table_file) as f:
+            data = json.load(f)
+    data[signature(op)] = {"apply": list(choice["apply"]), "vector": list(choice["vector"]),
+                           "us_per_iteration": choice["us_per_iteration"]}
+    os.makedirs(os.path.dirname(os.path.abspath(table_file)), exist_ok=True)
+    with open(table_file, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
 
 
 def tune_assembly(op, repeats=3, verbose=False):
