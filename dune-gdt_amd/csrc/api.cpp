@@ -157,6 +157,37 @@ hgdt_status hgdt_memcpy(void* dst, const void* src, int64_t bytes, void* stream)
   });
 }
 
+hgdt_status hgdt_device_alloc(int64_t bytes, void** ptr)
+{
+  return guarded([&] {
+    require(ptr && bytes >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    require_device();
+    *ptr = nullptr;
+    if (bytes == 0)
+      return;
+    const hipError_t e = hipMalloc(ptr, size_t(bytes));
+    if (e == hipErrorOutOfMemory)
+      HGDT_THROW(HGDT_ERR_OUT_OF_MEMORY, "device allocation failed");
+    HGDT_HIP(e);
+  });
+}
+
+hgdt_status hgdt_device_free(void* ptr)
+{
+  return guarded([&] {
+    if (ptr)
+      HGDT_HIP(hipFree(ptr));
+  });
+}
+
+hgdt_status hgdt_stream_synchronize(void* stream)
+{
+  return guarded([&] {
+    require_device();
+    HGDT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  });
+}
+
 hgdt_status hgdt_grid_create_structured(int geometry, const int32_t* cells, const double* lower_left,
                                         const double* upper_right, hgdt_grid* grid)
 {
@@ -371,6 +402,18 @@ hgdt_status hgdt_space_instance_dofs(hgdt_space space, int64_t* host_out)
   return guarded([&] {
     require(space && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
     std::copy(space->s->inst_dof.begin(), space->s->inst_dof.end(), host_out);
+  });
+}
+
+hgdt_status hgdt_space_error_norms(hgdt_space space, const double* dofs, int order, const double* g_values,
+                                   const double* g_gradients, const double* diffusion_field, double diffusion_value,
+                                   double* host_out, void* stream)
+{
+  return guarded([&] {
+    require(space && dofs && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_device();
+    error_norms(*space->s, dofs, order, g_values, g_gradients, diffusion_field, diffusion_value, host_out,
+                static_cast<hipStream_t>(stream));
   });
 }
 

@@ -659,6 +659,98 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
   grad[e] = -k * acc;
 }
 
+// BilinearForm(e, e).apply2() for e = u_h - g, element-local part: out[e] = int e^2 (product integrand,
+// weight 1) and out[ne + e] = int (kappa grad e) . grad e (Laplace integrand), with g given by its values
+// and physical gradients at the points of rule R (operators/bilinear-form.hh:344-452,
+// local/bilinear-forms/integrals.hh:97-134)
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* __restrict__ coords,
+                                                     const int32_t* __restrict__ conn,
+                                                     const int32_t* __restrict__ dofs,
+                                                     const double* __restrict__ qtab, QRule R,
+                                                     const double* __restrict__ u,
+                                                     const double* __restrict__ gval,
+                                                     const double* __restrict__ ggrad,
+                                                     const double* __restrict__ kappa, double kappa_value,
+                                                     double* __restrict__ out)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, ND = T::ND;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= ne)
+    return;
+  Geo<D> g;
+  load_geometry<GEOM>(coords, conn, e, g);
+  double ue[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    ue[i] = u[dofs[e * ND + i]];
+  const double k = kappa ? kappa[e] : kappa_value;
+  double l2 = 0., h1 = 0.;
+  for (int q = 0; q < R.npts; ++q) {
+    double xh[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      xh[a] = qtab[R.offset * D + q * D + a];
+    const double w = qtab[R.offset * D + R.npts * D + q];
+    double phi[ND], dphi[ND][D];
+    shape<GEOM>(xh, phi, dphi);
+    double ev = 0., eg[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      eg[a] = 0.;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double G[D];
+      to_physical<D>(g, dphi[i], G);
+      ev += ue[i] * phi[i];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        eg[a] += ue[i] * G[a];
+    }
+    const int64_t qi = e * R.npts + q;
+    if (gval)
+      ev -= gval[qi];
+    if (ggrad)
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        eg[a] -= ggrad[qi * D + a];
+    const double f = g.det * w;
+    double kd = 0.;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      kd += (k * eg[a]) * eg[a];
+    l2 += (ev * ev) * f;
+    h1 += kd * f;
+  }
+  out[e] = l2;
+  out[ne + e] = h1;
+}
+
+// fixed-order sums of element values over chunks (one block of 256 per chunk and component);
+// a chunk never straddles two instances
+__global__ void __launch_bounds__(256) k_sum_chunks(const double* __restrict__ vals, int64_t ne,
+                                                    const int64_t* __restrict__ chunk0, double* __restrict__ part,
+                                                    int64_t nchunks)
+{
+  __shared__ double sh[256];
+  const int64_t c = blockIdx.x;
+  const int comp = blockIdx.y;
+  const int64_t e0 = chunk0[c], e1 = chunk0[c + 1];
+  double s = 0.;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256)
+    s += vals[comp * ne + e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    part[comp * nchunks + c] = sh[0];
+}
+
 inline unsigned grid_for(int64_t n, int block)
 {
   return unsigned((n + block - 1) / block);
@@ -845,6 +937,57 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
                        u, lambda, grad);
   });
   HGDT_HIP(hipGetLastError());
+}
+
+void error_norms(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
+                 const double* kappa, double kappa_value, double* host_out, hipStream_t stream)
+{
+  const Grid& g = *s.grid;
+  if (s.kind != HGDT_CONTINUOUS_LAGRANGE && s.kind != HGDT_DISCONTINUOUS_LAGRANGE)
+    HGDT_THROW(HGDT_ERR_SPACE, "unknown space kind");
+  if (order < 0 || order > 20)
+    HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "quadrature order must be in [0, 20]");
+  QTable tab;
+  const QRule R = tab.add(element_rule(g.ref, order));
+  DeviceArray<double> d_tab, d_vals, d_part;
+  DeviceArray<int64_t> d_chunk;
+  d_tab.upload(tab.data);
+  // chunks of at most 65536 elements inside each instance
+  constexpr int64_t CH = 65536;
+  std::vector<int64_t> chunk0; // start of every chunk, closed by ne
+  for (int i = 0; i < g.ninst(); ++i)
+    for (int64_t e = g.inst_elem[i]; e < g.inst_elem[i + 1]; e += CH)
+      chunk0.push_back(e);
+  const int64_t nchunks = int64_t(chunk0.size());
+  chunk0.push_back(g.ne);
+  d_chunk.upload(chunk0);
+  d_vals.allocate(std::max<int64_t>(2 * g.ne, 1));
+  d_part.allocate(std::max<int64_t>(2 * nchunks, 1));
+  if (g.ne > 0) {
+    dispatch_geom(g.ref.geom, [&](auto G) {
+      hipLaunchKernelGGL(k_error_local<decltype(G)::value>, dim3(grid_for(g.ne, 256)), dim3(256), 0, stream, g.ne,
+                         g.d_coords.ptr, g.d_conn.ptr, s.d_dofs.ptr, d_tab.ptr, R, dofs, gval, ggrad, kappa,
+                         kappa_value, d_vals.ptr);
+    });
+    HGDT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_chunks, dim3(unsigned(nchunks), 2), dim3(256), 0, stream, d_vals.ptr, g.ne, d_chunk.ptr,
+                       d_part.ptr, nchunks);
+    HGDT_HIP(hipGetLastError());
+  }
+  std::vector<double> part(2 * nchunks, 0.);
+  if (g.ne > 0) {
+    HGDT_HIP(hipMemcpyAsync(part.data(), d_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HGDT_HIP(hipStreamSynchronize(stream));
+  }
+  // per instance, chunks summed in order on the host (a handful of values)
+  for (int i = 0; i < g.ninst(); ++i)
+    for (int comp = 0; comp < 2; ++comp) {
+      double acc = 0.;
+      for (int64_t c = 0; c < nchunks; ++c)
+        if (chunk0[c] >= g.inst_elem[i] && chunk0[c] < g.inst_elem[i + 1])
+          acc += part[comp * nchunks + c];
+      host_out[2 * i + comp] = acc;
+    }
 }
 
 } // namespace hgdt

@@ -286,5 +286,8 @@ void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body);
 
 // kernel launchers (assembly.hip)
 void launch_assembly(Operator& op, hipStream_t stream);
+// squared L2 and (kappa-weighted) H1-semi norms of u_h - g per instance (host_out[2 * ninst])
+void error_norms(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
+                 const double* kappa, double kappa_value, double* host_out, hipStream_t stream);
 
 } // namespace hgdt

@@ -136,6 +136,9 @@ SIGNATURES = {
     "hgdt_version": (ctypes.c_int, []),
     "hgdt_device_count": (ctypes.c_int, [_pi32]),
     "hgdt_memcpy": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
+    "hgdt_device_alloc": (ctypes.c_int, [_i64, _pvp]),
+    "hgdt_device_free": (ctypes.c_int, [_vp]),
+    "hgdt_stream_synchronize": (ctypes.c_int, [_vp]),
     "hgdt_grid_create_structured": (ctypes.c_int, [ctypes.c_int, _pi32, _pdbl, _pdbl, _pvp]),
     "hgdt_grid_create": (ctypes.c_int, [ctypes.c_int, _i64, _pdbl, _i64, _pi32, _pvp]),
     "hgdt_grid_create_union": (ctypes.c_int, [_pvp, _i32, _pvp]),
@@ -150,6 +153,7 @@ SIGNATURES = {
     "hgdt_space_size": (ctypes.c_int, [_vp, _pi64, _pi32]),
     "hgdt_space_copy_global_indices": (ctypes.c_int, [_vp, _pi32]),
     "hgdt_space_instance_dofs": (ctypes.c_int, [_vp, _pi64]),
+    "hgdt_space_error_norms": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _dbl, _pdbl, _vp]),
     "hgdt_operator_create": (ctypes.c_int, [_vp, ctypes.c_int, _pvp]),
     "hgdt_operator_destroy": (ctypes.c_int, [_vp]),
     "hgdt_operator_shape": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),

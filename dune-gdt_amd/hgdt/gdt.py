@@ -31,7 +31,8 @@ __all__ = [
     "LocalIPDGBoundaryPenalty", "LocalElementIntegralBilinearForm", "LocalCouplingIntersectionIntegralBilinearForm",
     "LocalIntersectionIntegralBilinearForm", "LocalElementIntegralFunctional", "VectorFunctional",
     "make_vector_functional", "DirichletConstraints", "make_dirichlet_constraints", "AllDirichletBoundaryInfo",
-    "ApplyOn", "Stencil", "MatrixOperator", "make_matrix_operator", "CsrMatrix",
+    "ApplyOn", "Stencil", "MatrixOperator", "make_matrix_operator", "CsrMatrix", "GenericFunction",
+    "DiscreteFunction", "make_discrete_function", "BilinearForm", "make_bilinear_form",
 ]
 
 GEOMETRIES = {"quad": L.QUAD, "cube": L.QUAD, "tri": L.TRI, "simplex": L.TRI, "hex": L.HEX, "tet": L.TET}
@@ -263,6 +264,30 @@ class QuadraturePointFunction:
             self.values = _dvec(vals.reshape(-1))
         else:
             self.values = _dvec(values_or_callable)
+
+
+class GenericFunction:
+    """XT::Functions::GenericFunction(order, evaluate, name, parameter_type, jacobian) as used in
+    examples/stationary-heat-equation.cc:68-85: an analytic function of the physical coordinates, evaluated
+    on the host at the element quadrature points.  `evaluate(X)` and `jacobian(X)` receive X of shape
+    (dim, N) and return (N,) values and (dim, N) gradients."""
+
+    def __init__(self, order, evaluate, name="", parameter_type=None, jacobian=None):
+        self.order_value, self.evaluate, self.name, self.jacobian = int(order), evaluate, name, jacobian
+
+    def order(self):
+        return self.order_value
+
+    def values_at(self, grid, order):
+        pts = grid.quadrature_points(order)
+        return np.asarray(self.evaluate(pts.reshape(-1, grid.dimension).T), np.float64).reshape(-1)
+
+    def gradients_at(self, grid, order):
+        if self.jacobian is None:
+            raise L.not_supported(9, f"GenericFunction {self.name!r} has no jacobian")
+        pts = grid.quadrature_points(order)
+        g = np.asarray(self.jacobian(pts.reshape(-1, grid.dimension).T), np.float64).reshape(grid.dimension, -1)
+        return np.ascontiguousarray(g.T).reshape(-1)
 
 
 def _gf(v):
@@ -550,6 +575,13 @@ class MatrixOperator:
             L.call("hgdt_operator_append_source", self._h, L.SOURCE_PER_QUADRATURE_POINT,
                    ctypes.c_void_p(f.values.data_ptr()), 0.0, f_order, over)
             return
+        if isinstance(f, GenericFunction):
+            order_total = self.space.order + f.order() + over
+            vals = _dvec(f.values_at(self.grid, order_total), name="source values")
+            self._keep.append(vals)
+            L.call("hgdt_operator_append_source", self._h, L.SOURCE_PER_QUADRATURE_POINT,
+                   ctypes.c_void_p(vals.data_ptr()), 0.0, f.order(), over)
+            return
         gf = _gf(f)
         p, v = self._field(gf)
         kind = L.SOURCE_CONSTANT if p is None else L.SOURCE_PER_ELEMENT
@@ -705,3 +737,99 @@ def _scaling(param):
 def make_matrix_operator(space, stencil=Stencil.element_and_intersection, logging_prefix=""):
     """make_matrix_operator<M>(space, stencil), matrix-based.hh:650-658"""
     return MatrixOperator(space.grid, space, space, stencil=stencil)
+
+
+# ---------------------------------------------------------------------------
+# discrete functions and bilinear forms on them (error norms)
+# ---------------------------------------------------------------------------
+class DiscreteFunction:
+    """DiscreteFunction(space, vector) (discretefunction/default.hh): a DoF vector on a space."""
+
+    def __init__(self, space, vector=None, name=""):
+        self.space, self.name = space, name
+        self.vector = (torch.zeros(space.num_DoFs, dtype=torch.float64, device=_device()) if vector is None
+                       else _dvec(vector, space.num_DoFs, "DoF vector"))
+
+    def dofs(self):
+        return self.vector
+
+    def order(self):
+        return self.space.order
+
+    def __sub__(self, other):
+        return _Difference(self, other)
+
+
+def make_discrete_function(space, vector=None, name=""):
+    return DiscreteFunction(space, vector, name)
+
+
+class _Difference:
+    def __init__(self, discrete, function):
+        if not isinstance(discrete, DiscreteFunction) or not isinstance(function, (GenericFunction, type(None))):
+            raise L.not_supported(9, "differences are DiscreteFunction - GenericFunction")
+        self.discrete, self.function = discrete, function
+
+    def order(self):
+        return max(self.discrete.order(), self.function.order() if self.function else 0)
+
+
+class BilinearForm:
+    """BilinearForm(grid_view, source, range) += LocalElementIntegralBilinearForm(...); apply2()
+    (operators/bilinear-form.hh:344-452): with source = range = u_h - g and the Laplace or product
+    integrand this is the squared energy (H1-semi) or L2 norm of the error, computed on the device
+    by one element kernel and a fixed-order sum."""
+
+    def __init__(self, grid, source, range=None, logging_prefix=""):
+        range = source if range is None else range
+        if range is not source:
+            raise L.not_supported(9, "apply2 is provided for source == range (norms)")
+        self.grid = grid
+        self.function = source if isinstance(source, _Difference) else _Difference(source, None)
+        self.local_forms = []
+
+    def append(self, local_form):
+        if not isinstance(local_form, LocalElementIntegralBilinearForm) or not isinstance(
+                local_form.integrand, (LocalLaplaceIntegrand, LocalElementProductIntegrand)):
+            raise L.not_supported(9, "bilinear forms of element Laplace / product integrals")
+        self.local_forms.append(local_form)
+        return self
+
+    __iadd__ = append
+
+    def apply2_per_instance(self, stream=None):
+        d = self.function.discrete
+        space, g = d.space, self.function.function
+        ninst = self.grid.num_instances
+        total = np.zeros(ninst)
+        for lf in self.local_forms:
+            it = lf.integrand
+            order = 0 + 2 * self.function.order() + lf.over_integrate
+            gv = gg = None
+            if g is not None:
+                gv = _dvec(g.values_at(self.grid, order), name="function values")
+                if isinstance(it, LocalLaplaceIntegrand):
+                    gg = _dvec(g.gradients_at(self.grid, order), name="function gradients")
+            out = np.empty(2 * ninst)
+            if isinstance(it, LocalLaplaceIntegrand):
+                kp, kv = it.diffusion.args(self.grid)
+                comp, factor = 1, 1.0
+            else:
+                if it.weight.field is not None:
+                    raise L.not_supported(9, "product norms with a constant weight")
+                kp, kv, comp, factor = None, 1.0, 0, it.weight.constant
+            L.call("hgdt_space_error_norms", space._h, ctypes.c_void_p(d.vector.data_ptr()), order,
+                   ctypes.c_void_p(gv.data_ptr()) if gv is not None else None,
+                   ctypes.c_void_p(gg.data_ptr()) if gg is not None else None, kp, kv,
+                   out.ctypes.data_as(L._pdbl), _stream_ptr(stream))
+            total += factor * out[comp::2]
+        return total
+
+    def apply2(self, stream=None):
+        return float(self.apply2_per_instance(stream).sum())
+
+    result = apply2
+
+
+def make_bilinear_form(grid, source, range=None, logging_prefix=""):
+    return BilinearForm(grid, source, range, logging_prefix)

@@ -69,6 +69,12 @@ hgdt_status hgdt_device_count(int* count);
  * hold device buffers the library does not own (no reference counterpart) */
 hgdt_status hgdt_memcpy(void* dst, const void* src, int64_t bytes, void* stream);
 
+/* device memory and stream utilities for bindings that hold no HIP runtime of their own, such as the
+ * C++ classes of hgdt.hh (no reference counterpart: the reference's containers live in host memory) */
+hgdt_status hgdt_device_alloc(int64_t bytes, void** ptr);
+hgdt_status hgdt_device_free(void* ptr);
+hgdt_status hgdt_stream_synchronize(void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* grids                                                                     */
 /* ------------------------------------------------------------------------- */
@@ -116,6 +122,20 @@ hgdt_status hgdt_space_size(hgdt_space space, int64_t* size, int32_t* local_size
 hgdt_status hgdt_space_copy_global_indices(hgdt_space space, int32_t* host_out);
 /* DoF offsets of the instances of a union grid, host_out[num_instances + 1] */
 hgdt_status hgdt_space_instance_dofs(hgdt_space space, int64_t* host_out);
+
+/* Squared error norms of e = u_h - g, the discrete function u_h with DoF vector `dofs` (device, [space size])
+ * minus a function g given at the points of hgdt_grid_quadrature_points(order): values
+ * [num_elements x num_points] and physical gradients [num_elements x num_points x dim] (device arrays;
+ * NULL for a zero value / gradient).  This is
+ *   make_bilinear_form(grid_view, e, e) += LocalElementIntegralBilinearForm(LocalElementProductIntegrand())
+ *   make_bilinear_form(grid_view, e, e) += LocalElementIntegralBilinearForm(LocalLaplaceIntegrand(diffusion))
+ * followed by apply2() (operators/bilinear-form.hh:344-452, as in examples/stationary-heat-equation.cc:120-132),
+ * with the quadrature order given explicitly (integrand order + over_integrate, integrals.hh:115).
+ * host_out[2*i] = ||e||_L2^2 and host_out[2*i+1] = sum_e int (kappa grad e) . grad e for every instance i.
+ * Element values are summed in a fixed order (deterministic).  Blocks until the results are on the host. */
+hgdt_status hgdt_space_error_norms(hgdt_space space, const double* dofs, int order, const double* g_values,
+                                   const double* g_gradients, const double* diffusion_field, double diffusion_value,
+                                   double* host_out, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* matrix operator                                                           */

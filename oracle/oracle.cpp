@@ -1544,6 +1544,71 @@ double or_esv2007_h1_semi_error(const OrSpace* s, const double* u, int quad_orde
   return std::sqrt(total);
 }
 
+// Squared error norms of e = u_h - g: BilinearForm(grid_view, e, e) with
+// LocalElementIntegralBilinearForm(LocalElementProductIntegrand()) and with
+// LocalElementIntegralBilinearForm(LocalLaplaceIntegrand(kappa)), apply2()
+// (operators/bilinear-form.hh:344-452, local/bilinear-forms/integrals.hh:97-134,
+// as used in examples/stationary-heat-equation.cc:120-132).  g is given by its values
+// [ne][npts] and physical gradients [ne][npts][dim] at the points of element_rule(order).
+int or_error_norms(const OrSpace* s, const double* u, int order, const double* gval, const double* ggrad,
+                   const double* kappa, double kappa_const, double* out)
+{
+  return guarded([&] {
+    const Mesh& m = *s->mesh;
+    const int ls = s->fe.size, d = m.ref.dim;
+    const Quadrature q = element_rule(m.ref, order);
+    const int nq = int(q.w.size());
+    double l2 = 0., h1 = 0.;
+    BoundBasis basis;
+    for (int64_t e = 0; e < m.ne; ++e) {
+      const Geometry geo = m.geometry(e);
+      basis.bind(&s->fe, &geo);
+      const double k = kappa ? kappa[e] : kappa_const;
+      double el2 = 0., eh1 = 0.;
+      for (int qp = 0; qp < nq; ++qp) {
+        basis.evaluate(q.x[qp].data());
+        basis.jacobians(q.x[qp].data());
+        double ev = 0., eg[3] = {0., 0., 0.};
+        for (int i = 0; i < ls; ++i) {
+          const double ui = u[s->indices[e * ls + i]];
+          ev += ui * basis.vals[i];
+          for (int a = 0; a < d; ++a)
+            eg[a] += ui * basis.grads[3 * i + a];
+        }
+        if (gval)
+          ev -= gval[e * nq + qp];
+        if (ggrad)
+          for (int a = 0; a < d; ++a)
+            eg[a] -= ggrad[(e * nq + qp) * d + a];
+        const double f = geo.integration_element * q.w[qp];
+        el2 += (ev * ev) * f;
+        eh1 += kgrad_dot(k, eg, eg, d) * f;
+      }
+      l2 += el2, h1 += eh1;
+    }
+    out[0] = l2, out[1] = h1;
+  });
+}
+
+// physical coordinates of the points of element_rule(order) on every element, out[ne][npts][dim]
+int or_mesh_quadrature_points(const OrMesh* mesh, int order, double* out)
+{
+  int count = -1;
+  guarded([&] {
+    const Mesh& m = *mesh;
+    const Quadrature q = element_rule(m.ref, order);
+    count = int(q.w.size());
+    if (!out)
+      return;
+    for (int64_t e = 0; e < m.ne; ++e) {
+      const Geometry geo = m.geometry(e);
+      for (int qp = 0; qp < count; ++qp)
+        geo.global(q.x[qp].data(), &out[(e * count + qp) * m.ref.dim]);
+    }
+  });
+  return count;
+}
+
 int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const double* u, const double* lambda,
                                   double* grad)
 {

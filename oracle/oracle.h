@@ -109,6 +109,14 @@ double or_esv2007_h1_semi_error(const OrSpace* s, const double* u, int quad_orde
 int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const double* u, const double* lambda,
                                   double* grad);
 
+/* squared error norms of e = u_h - g (g: values [ne][npts], physical gradients [ne][npts][dim] at the
+ * points of the element rule of `order`, NULL for 0): out[0] = sum_e int e^2, out[1] = sum_e int kappa |grad e|^2
+ * (operators/bilinear-form.hh:344-452) */
+int or_error_norms(const OrSpace* s, const double* u, int order, const double* gval, const double* ggrad,
+                   const double* kappa, double kappa_const, double* out);
+/* physical coordinates of the element-rule points, out[ne][npts][dim] (NULL: size query); returns npts */
+int or_mesh_quadrature_points(const OrMesh* m, int order, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,8 @@ def lib():
         "or_quadrature": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]),
         "or_esv2007_h1_semi_error": (ctypes.c_double, [_vp, _dp, ctypes.c_int]),
         "or_adjoint_log_kappa_gradient": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+        "or_error_norms": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]),
+        "or_mesh_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -148,6 +150,15 @@ class Mesh:
         coords = np.ascontiguousarray(coords, np.float64)
         conn = np.ascontiguousarray(conn, np.int32)
         return cls(lib().or_mesh_create(geom, coords.shape[0], dptr(coords), conn.shape[0], iptr(conn)))
+
+    def quadrature_points(self, order):
+        """physical coordinates of the element-rule points, [ne, npts, dim]"""
+        n = lib().or_mesh_quadrature_points(self.h, order, None)
+        if n < 0:
+            raise RuntimeError(_err())
+        out = np.empty((self.ne, n, self.dim))
+        lib().or_mesh_quadrature_points(self.h, order, dptr(out))
+        return out
 
     def coords(self):
         out = np.empty((self.nv, self.dim))
@@ -338,6 +349,20 @@ def quadrature(geom, order):
 
 def esv2007_h1_semi_error(space, u, quad_order=8):
     return lib().or_esv2007_h1_semi_error(space.h, dptr(np.ascontiguousarray(u, np.float64)), quad_order)
+
+
+def error_norms(space, u, order, gval=None, ggrad=None, kappa=1.0):
+    """(||u_h - g||_L2^2, sum_e int kappa |grad(u_h - g)|^2) with g given at the element-rule points"""
+    out = np.empty(2)
+    keep = [np.ascontiguousarray(a, np.float64) if a is not None else None for a in (gval, ggrad)]
+    kap = None if np.isscalar(kappa) else np.ascontiguousarray(kappa, np.float64)
+    rc = lib().or_error_norms(space.h, dptr(np.ascontiguousarray(u, np.float64)), order,
+                              dptr(keep[0]) if keep[0] is not None else None,
+                              dptr(keep[1]) if keep[1] is not None else None,
+                              dptr(kap) if kap is not None else None, float(kappa) if kap is None else 0.0, dptr(out))
+    if rc != 0:
+        raise RuntimeError(_err())
+    return out
 
 
 def adjoint_log_kappa_gradient(space, kappa, u, lam):

@@ -1,0 +1,120 @@
+"""Device error norms (BilinearForm(e, e).apply2() with product / Laplace integrands) and the ESV2007
+SWIPDG verification run end to end on the HIP path.
+
+Bar: squared norms within 1e-12 relative of the oracle on the same DoF vector and the same function
+values at the same quadrature points; ESV2007 H1-semi errors equal to the published three significant
+figures (stationary_heat_equation__ESV2007__table_1.mini:31-33) and within 1e-8 of the oracle's.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
+
+
+def _g(dim):
+    """an order-3 analytic function with its gradient, X of shape (dim, N)"""
+    def ev(X):
+        return np.sin(X[0]) * np.cos(0.5 * X[1]) + (X[2] ** 2 if dim == 3 else 0.0)
+
+    def jac(X):
+        g = [np.cos(X[0]) * np.cos(0.5 * X[1]), -0.5 * np.sin(X[0]) * np.sin(0.5 * X[1])]
+        if dim == 3:
+            g.append(2 * X[2])
+        return np.stack(g)
+    return ev, jac
+
+
+@pytest.mark.parametrize("geometry,cells,geom_o", [("quad", [12, 10], O.QUAD), ("tet", [6, 5, 4], O.TET),
+                                                   ("tri", [7, 9], O.TRI), ("hex", [4, 5, 3], O.HEX)])
+def test_error_norms_parity(hgdt_mod, geometry, cells, geom_o):
+    print(f"case error norms {geometry} {cells}", flush=True)
+    H = hgdt_mod
+    dim = len(cells)
+    lo, hi = [0.0] * dim, [1.5, 1.0, 2.0][:dim]
+    grid = H.Grid.structured(geometry, lo, hi, cells)
+    space = H.ContinuousLagrangeSpace(grid, 1)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(space.num_DoFs)
+    kappa = H.fields.lognormal(6, grid.num_elements)
+    ev, jac = _g(dim)
+    g = H.GenericFunction(3, ev, jacobian=jac)
+    uh = H.make_discrete_function(space, torch.from_numpy(u))
+    h1f = H.make_bilinear_form(grid, uh - g)
+    h1f += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kappa)))
+    l2f = H.make_bilinear_form(grid, uh - g)
+    l2f += H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand())
+    h1, l2 = h1f.apply2(), l2f.apply2()
+    # oracle on the same inputs: same rule, same points
+    m = O.Mesh.structured(geom_o, cells, lo, hi)
+    s = O.Space(m, True, 1)
+    order = 6
+    P = m.quadrature_points(order)
+    assert np.abs(P - grid.quadrature_points(order)).max() <= 1e-15 * max(hi)
+    X = P.reshape(-1, dim).T
+    ol2, oh1 = O.error_norms(s, u, order, ev(X), np.ascontiguousarray(jac(X).T), kappa)
+    print(f"  l2^2 {l2:.12e} (oracle {ol2:.12e})  h1^2 {h1:.12e} (oracle {oh1:.12e})")
+    assert abs(l2 - ol2) <= 1e-12 * ol2 and abs(h1 - oh1) <= 1e-12 * oh1
+    # against g = 0 with a union of two copies: per-instance results equal the single-grid ones
+    union = H.Grid.union([grid, grid])
+    us = H.ContinuousLagrangeSpace(union, 1)
+    both = H.make_discrete_function(us, torch.from_numpy(np.concatenate([u, 2 * u])))
+    f = H.make_bilinear_form(union, both)
+    f += H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand())
+    per = f.apply2_per_instance()
+    single = H.make_bilinear_form(grid, uh)
+    single += H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand())
+    v1 = single.apply2()
+    assert abs(per[0] - v1) <= 1e-13 * v1 and abs(per[1] - 4 * v1) <= 1e-13 * v1
+
+
+def _esv2007(H, n):
+    grid = H.Grid.structured("quad", [-1, -1], [1, 1], [n, n])
+    space = H.DiscontinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0))
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(
+        H.LocalLaplaceIPDGInnerCoupling(1.0, 1.0, 1.0) + H.LocalIPDGInnerPenalty(8.0, 1.0)), {},
+        H.ApplyOn.InnerIntersectionsOnce())
+    op.append(H.LocalIntersectionIntegralBilinearForm(
+        H.LocalIPDGBoundaryPenalty(14.0, 1.0) + H.LocalLaplaceIPDGDirichletCoupling(1.0, 1.0)), {},
+        H.ApplyOn.CustomBoundaryIntersections(H.AllDirichletBoundaryInfo()))
+    force = H.GenericFunction(
+        4, lambda X: math.pi / 2 * math.pi * np.cos(math.pi / 2 * X[0]) * np.cos(math.pi / 2 * X[1]))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(force)))
+    op.append(rhs)
+    op.assemble()
+    x = op.apply_inverse("rhs", opts={"precision": 1e-12, "max_iter": 100000})
+    exact = H.GenericFunction(
+        3, lambda X: np.cos(math.pi / 2 * X[0]) * np.cos(math.pi / 2 * X[1]),
+        jacobian=lambda X: np.stack([-math.pi / 2 * np.sin(math.pi / 2 * X[0]) * np.cos(math.pi / 2 * X[1]),
+                                     -math.pi / 2 * np.cos(math.pi / 2 * X[0]) * np.sin(math.pi / 2 * X[1])]))
+    err = H.make_bilinear_form(grid, H.make_discrete_function(space, x) - exact)
+    err += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0), over_integrate=4)  # order 10
+    return math.sqrt(err.apply2()), x.cpu().numpy(), op
+
+
+def test_esv2007_swipdg_on_device(hgdt_mod):
+    """ESV2007 table 1 (Q1-DG SWIPDG, sigma 8 / 14, cube grids on [-1, 1]^2): the whole pipeline on the
+    GPU -- assembly with the force at quadrature points, PCG, device H1-semi error -- reproduces the
+    published errors to three significant figures"""
+    ref = GOLDEN["reference"]["esv2007_q1dg_cube"]
+    for n, e_ref in zip((8, 16, 32), ref["h1_semi"]):
+        print(f"case ESV2007 n={n}", flush=True)
+        err, x, op = _esv2007(hgdt_mod, n)
+        m = O.Mesh.structured(O.QUAD, [n, n], [-1, -1], [1, 1])
+        s = O.Space(m, False, 1)
+        oerr = O.esv2007_h1_semi_error(s, x, 10)
+        print(f"  h1 semi error {err:.6e} (published {e_ref}, oracle on the GPU solution {oerr:.6e}), "
+              f"{op.last_solve['iterations']} iterations")
+        assert float(f"{err:.2e}") == e_ref
+        assert abs(err - oerr) <= 1e-12 * oerr

@@ -299,3 +299,51 @@ def test_adjoint_gradient_central_differences():
         um, _ = solve(gm)
         fd = (0.5 * np.sum((up - u_obs) ** 2) - 0.5 * np.sum((um - u_obs) ** 2)) / (2 * h)
         assert abs(fd - grad[e]) <= 1e-6 * max(abs(grad).max(), 1e-30)
+
+
+def test_error_norms_closed_forms():
+    """BilinearForm(e, e).apply2() with product and Laplace integrands (operators/bilinear-form.hh:344-452):
+    interpolants of linear functions are exact on P1 / Q1, so against the function itself both norms vanish;
+    against 0 they are integrals known in closed form"""
+    for geom, dim in ((O.QUAD, 2), (O.TRI, 2), (O.HEX, 3), (O.TET, 3)):
+        cells = [3, 4] if dim == 2 else [2, 3, 2]
+        lo, hi = [0.0] * dim, [2.0, 1.0, 3.0][:dim]
+        m = O.Mesh.structured(geom, cells, lo, hi)
+        s = O.Space(m, True, 1)
+        X = m.coords()
+        a = np.array([0.5, -1.25, 2.0][:dim])
+        u = X @ a + 0.75  # interpolant of v(x) = a.x + 0.75
+        order = 4
+        P = m.quadrature_points(order)
+        gv = P @ a + 0.75
+        gg = np.broadcast_to(a, P.shape).copy()
+        l2, h1 = O.error_norms(s, u, order, gv, gg)
+        assert abs(l2) < 1e-26 and abs(h1) < 1e-26
+        vol = float(np.prod(hi))
+        # against g = 0: ||v||^2 over the box and |a|^2 vol, per-element kappa weighting the H1 part
+        kappa = 1.0 + np.arange(m.ne) % 3
+        l2, h1 = O.error_norms(s, u, order, kappa=kappa)
+        # int (a.x + c)^2 over [0, hi]: exact by expanding (independent coordinates)
+        c = 0.75
+        mean = sum(a[k] * hi[k] / 2 for k in range(dim)) + c
+        var = sum(a[k] ** 2 * hi[k] ** 2 / 12 for k in range(dim))
+        assert abs(l2 - vol * (mean ** 2 + var)) < 1e-12 * vol * (mean ** 2 + var)
+        ev = vol / m.ne  # structured grids: every element has the same volume
+        assert abs(h1 - float(a @ a) * ev * kappa.sum()) < 1e-12 * h1
+
+
+def test_error_norms_match_esv2007_h1_error():
+    """the H1-semi part against the exact gradient equals the ESV2007 error the fixture pins"""
+    n = 8
+    m, s, op = swipdg_operator(n)
+    rp, cols = op.pattern()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-12)
+    P = m.quadrature_points(10)
+    X, Y = P[..., 0], P[..., 1]
+    gg = np.stack([-np.pi / 2 * np.sin(np.pi / 2 * X) * np.cos(np.pi / 2 * Y),
+                   -np.pi / 2 * np.cos(np.pi / 2 * X) * np.sin(np.pi / 2 * Y)], axis=-1)
+    l2, h1 = O.error_norms(s, x, 10, np.cos(np.pi / 2 * X) * np.cos(np.pi / 2 * Y), gg)
+    ref = O.esv2007_h1_semi_error(s, x, 10)
+    assert abs(np.sqrt(h1) - ref) <= 1e-13 * ref
+    assert float(f"{np.sqrt(h1):.2e}") == GOLDEN["reference"]["esv2007_q1dg_cube"]["h1_semi"][0]
+    assert 0 < l2 < h1
