@@ -288,7 +288,7 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this size first (reference dispatch for HBM counters)")
     ap.add_argument("--spmv-sample", type=int, default=8,
-                    help="HIP events around every k-th operator application of the timed steps")
+                    help="HIP events around the operator application of every k-th PCG iteration of the timed steps")
     args = ap.parse_args()
     if args.no_tune:
         args.tune = "off"
@@ -362,10 +362,11 @@ def main():
     iters = stats[-1]["iterations"]
     fi = op.format_info()
     spmv_ms = float(np.mean([s["spmv_avg_ms"] for s in stats]))
-    achieved = fi["spmv_bytes"] / (spmv_ms * 1e-3) / 1e9
+    step_bytes = int(stats[-1]["step_bytes"])
+    achieved = step_bytes / (spmv_ms * 1e-3) / 1e9
     solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"pmc_spmv_{args.config}.json")
+    tfile = os.path.join(ROOT, "profiles", f"pmc_cg_spmv_{args.config}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -388,7 +389,7 @@ def main():
                    "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"k_pcg_spmv<{fi['format']}>", "bytes_per_launch": fi["spmv_bytes"],
+                     "kernel": f"k_cg_spmv<{fi['format']}>", "bytes_per_launch": step_bytes,
                      "avg_launch_us": spmv_ms * 1e3},
     }
     if args.extras:

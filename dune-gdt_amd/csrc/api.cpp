@@ -698,9 +698,11 @@ hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t b
 {
   return guarded([&] {
     require(op && kernel >= 0 && kernel < 5, HGDT_ERR_INVALID_ARGUMENT, "unknown kernel id");
-    require(block_size >= 64 && block_size % 64 == 0 && items_per_thread >= 1 && items_per_thread <= 8 &&
+    const int max_items = (kernel == 3) ? 16 : 8;
+    require(block_size >= 64 && block_size % 64 == 0 && items_per_thread >= 1 && items_per_thread <= max_items &&
                 (items_per_thread & (items_per_thread - 1)) == 0,
-            HGDT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64, items_per_thread 1, 2, 4 or 8");
+            HGDT_ERR_INVALID_ARGUMENT,
+            "block_size must be a multiple of 64, items_per_thread a power of two up to 8 (16 for kernel 3)");
     const int limit = (kernel <= 2) ? 256 : 1024; // assembly kernels are compiled for at most 256 lanes
     require(block_size <= limit, HGDT_ERR_INVALID_ARGUMENT, "block_size above the kernel's launch bound");
     Launch& L = op->op->launch[kernel];

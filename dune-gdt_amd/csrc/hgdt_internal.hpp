@@ -223,7 +223,11 @@ struct SolverFormat {
   DeviceArray<int32_t> d_sell_cols;
   DeviceArray<double> d_sell_vals;
   DeviceArray<int64_t> d_sell_pos; // nnz: CSR slot -> SELL position
-  DeviceArray<double> d_dinv;      // Jacobi preconditioner
+  // Jacobi preconditioning by symmetric scaling: h_i = 1 / sqrt(a_ii), s_i = sqrt(a_ii), and the scaled
+  // matrix D^{-1/2} A D^{-1/2} in the layout of the storage above (DIA or SELL values)
+  DeviceArray<double> d_dh, d_dsq;
+  DeviceArray<double> d_scaled;
+  int64_t step_bytes = 0; // algorithmic bytes of the CG solver's operator application
 };
 
 // device-side PCG state, one record per instance; two copies indexed by iteration parity so that

@@ -14,17 +14,19 @@
 // One lane owns a row and sums it in ascending column order (the CSR order), so DIA, symmetric
 // DIA and SELL give the same bits for the same matrix.
 //
-// PCG iteration = three launches, each one pass over its vectors:
-//   k_pcg_spmv   : q = A p and the block partials of p.q
-//   k_pcg_update : alpha from the p.q partials; x += alpha p, r -= alpha q, z = D^{-1} r (not stored),
-//                  block partials of r.z and r.r
-//   k_pcg_pupdate: beta and the stopping test from the r.z / r.r partials; p = D^{-1} r + beta p
+// Jacobi-PCG runs as CG on the symmetrically scaled matrix D^{-1/2} A D^{-1/2} (same iterates in
+// exact arithmetic; the scaled matrix has a unit diagonal that is never stored or read, and the
+// preconditioner costs no vector), three launches per iteration, each one pass over its vectors:
+//   k_cg_direction : stopping test from the previous update's partials; x += alpha p (deferred from
+//                    the previous iteration); p = r + beta p
+//   k_cg_spmv      : q = Ah p and the block partials of p.q
+//   k_cg_update    : alpha from the p.q partials; r -= alpha q; block partials of r.r and |D^{1/2} r|^2
 // Data cross blocks only at kernel boundaries.  Every block of an instance reduces that
 // instance's block partials itself, in block order, with a fixed butterfly / wave order, so all
 // blocks obtain bitwise identical alpha and beta and the result does not depend on scheduling;
 // no fence or atomic is used.  The per-instance scalars live in two records selected by the
-// parity of the iteration: iteration j reads record j%2 and its last kernel writes record
-// (j+1)%2.  Every block belongs to exactly one problem instance, so a batch of independent
+// parity of the iteration: iteration j's direction kernel reads record j%2 and writes record
+// (j+1)%2, which its update kernel then completes.  Every block belongs to exactly one problem instance, so a batch of independent
 // problems (a union grid) is solved in the same launches with per-instance scalars and stopping
 // tests; blocks of finished instances exit at once.
 //
@@ -71,17 +73,20 @@ struct PcgParams {
   const int64_t* slice_ptr;
   const int32_t* sell_cols;
   const double* sell_vals;
-  // vectors
+  // symmetric scaling D^{-1/2} A D^{-1/2}: values in the layout of the format, h = D^{-1/2}, s = D^{1/2}
+  const double* scaled;
+  const double* dh;
+  const double* dsq;
+  // vectors of the scaled system (x and r scaled)
   const double* b;
   const double* ax;
   double* x;
   double* r;
   double* p;
   double* q;
-  const double* dinv;
   double* part1; // 1 per A-block: p.q
-  double* part2; // 2 per V-block: r.z, r.r
-  double* part3; // 3 per V-block: b.b, r.r, r.z (initialisation)
+  double* part2; // 2 per V-block: r.r of the scaled residual, r.r of the residual
+  double* part3; // 1 per V-block: b.b (initialisation)
   PcgScalars* S; // 2 * ninst records
   int ninst;
   double rtol2;
@@ -138,11 +143,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
 
 // sum of an instance's block partials (written by an earlier kernel) in block order
 // (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
-// partials are issued together, so the prologue costs one memory round trip per U * B partials)
+// partials are issued together, so the prologue costs one memory round trip per U * B partials:
+// one round trip for up to 8192 single partials with B = 256)
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
 {
-  constexpr int U = 8;
+  constexpr int U = 32 / NV;
   const int B = blockDim.x;
 #pragma unroll
   for (int k = 0; k < NV; ++k)
@@ -165,10 +171,13 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
   block_sum<NV>(tot);
 }
 
-// y_i = sum_j A_ij v_j for the ITEMS rows r0 + threadIdx.x + j * blockDim.x of block b
-template <int FMT, int ITEMS>
-__device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, int64_t r0, int64_t r1,
-                                           const double* __restrict__ v, double (&acc)[ITEMS])
+// y_i = sum_j A_ij v(j) for the ITEMS rows r0 + threadIdx.x + j * blockDim.x (< r1) of block b, whose rows
+// start at rb (a block may process its rows in several chunks starting at r0).  `vals` is the
+// value storage of the format (DIA diagonals or SELL values); v(c) returns the vector entry at column c.
+// UNIT: the diagonal entry is 1 and not read (the symmetrically scaled matrix of the CG solver).
+template <int FMT, int ITEMS, bool UNIT, class VSRC>
+__device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __restrict__ vals, int b, int inst,
+                                           int64_t rb, int64_t r0, int64_t r1, VSRC v, double (&acc)[ITEMS])
 {
   const int64_t B = blockDim.x;
 #pragma unroll
@@ -177,29 +186,75 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, 
   if constexpr (FMT == FMT_DIA || FMT == FMT_DIAS) {
     const int64_t lo = P.inst_row0[inst], hi = P.inst_row0[inst + 1];
     const int64_t* __restrict__ off = P.dia_off + int64_t(inst) * P.ndiag;
-    const double* __restrict__ dia = P.dia;
+    const double* __restrict__ dia = vals;
     const int64_t n = P.n;
+    const int k0 = P.dia_k0[inst];
     if constexpr (FMT == FMT_DIA) {
 #pragma unroll 3
       for (int k = 0; k < P.ndiag; ++k) {
         const int64_t o = off[k];
         const double* __restrict__ dk = dia + int64_t(k) * n;
+        const bool unit = UNIT && (k == k0);
         double a[ITEMS], x[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
           const int64_t i = r0 + threadIdx.x + j * B;
           const int64_t c = i + o;
           const bool ok = (i < r1) && (c >= lo) && (c < hi);
-          a[j] = ok ? dk[i] : 0.;
-          x[j] = ok ? v[c] : 0.;
+          a[j] = (ok && !unit) ? dk[i] : 1.;
+          x[j] = ok ? v(c) : 0.;
         }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
-          acc[j] = fma(a[j], x[j], acc[j]);
+          acc[j] = unit ? acc[j] + x[j] : fma(a[j], x[j], acc[j]);
       }
     } else {
       // lower triangle from the upper diagonals (descending offset = ascending column), then upper
-      const int k0 = P.dia_k0[inst];
+      const int ku = P.ndiag - 1 - k0; // strict upper slots of the instance (trailing ones may be padding)
+      if constexpr (ITEMS == 1) {
+        if (ku <= 8) {
+          // every load of the row is issued before the first FMA (one memory round trip per row);
+          // the FMAs run in the order of the loop below, so the bits are the same
+          constexpr int MU = 8;
+          const int64_t i = r0 + threadIdx.x;
+          const bool in = i < r1;
+          double al[MU], xl[MU], au[MU], xu[MU];
+#pragma unroll
+          for (int u = 0; u < MU; ++u) {
+            al[u] = xl[u] = au[u] = xu[u] = 0.;
+            if (u < ku) {
+              const int64_t o = off[k0 + 1 + u];
+              const double* __restrict__ dk = dia + int64_t(k0 + 1 + u) * n;
+              const int64_t cl = i - o, cu = i + o;
+              const bool okl = in && (cl >= lo) && (cl < hi);
+              const bool oku = in && (cu >= lo) && (cu < hi);
+              al[u] = okl ? dk[cl] : 0.;
+              xl[u] = okl ? v(cl) : 0.;
+              au[u] = oku ? dk[i] : 0.;
+              xu[u] = oku ? v(cu) : 0.;
+            }
+          }
+          const double ad = (in && !UNIT) ? dia[int64_t(k0) * n + i] : 0.;
+          const double xd = in ? v(i) : 0.;
+          double s = 0.;
+#pragma unroll
+          for (int u = MU - 1; u >= 0; --u)
+            if (u < ku)
+              s = fma(al[u], xl[u], s);
+          if constexpr (UNIT) {
+            if (in)
+              s += xd;
+          } else {
+            s = fma(ad, xd, s);
+          }
+#pragma unroll
+          for (int u = 0; u < MU; ++u)
+            if (u < ku)
+              s = fma(au[u], xu[u], s);
+          acc[0] = s;
+          return;
+        }
+      }
 #pragma unroll 2
       for (int k = P.ndiag - 1; k > k0; --k) {
         const int64_t o = off[k];
@@ -211,14 +266,22 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, 
           const int64_t c = i - o;
           const bool ok = (i < r1) && (c >= lo) && (c < hi);
           a[j] = ok ? dk[c] : 0.;
-          x[j] = ok ? v[c] : 0.;
+          x[j] = ok ? v(c) : 0.;
         }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
           acc[j] = fma(a[j], x[j], acc[j]);
       }
+      if constexpr (UNIT) { // the unit diagonal, between the lower and the upper part
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const int64_t i = r0 + threadIdx.x + j * B;
+          if (i < r1)
+            acc[j] += v(i);
+        }
+      }
 #pragma unroll 2
-      for (int k = k0; k < P.ndiag; ++k) {
+      for (int k = (UNIT ? k0 + 1 : k0); k < P.ndiag; ++k) {
         const int64_t o = off[k];
         const double* __restrict__ dk = dia + int64_t(k) * n;
         double a[ITEMS], x[ITEMS];
@@ -228,7 +291,7 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, 
           const int64_t c = i + o;
           const bool ok = (i < r1) && (c >= lo) && (c < hi);
           a[j] = ok ? dk[i] : 0.;
-          x[j] = ok ? v[c] : 0.;
+          x[j] = ok ? v(c) : 0.;
         }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
@@ -240,16 +303,16 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, int b, int inst, 
     for (int j = 0; j < ITEMS; ++j) {
       const int64_t i = r0 + threadIdx.x + j * B;
       if (i < r1) {
-        const int64_t local = i - r0;
+        const int64_t local = i - rb;
         const int64_t sl = P.block_slice0[b] + (local >> 6);
         const int lane = int(local & 63);
         const int64_t s0 = P.slice_ptr[sl];
         const int w = int((P.slice_ptr[sl + 1] - s0) >> 6);
         const int32_t* __restrict__ cc = P.sell_cols + s0 + lane;
-        const double* __restrict__ vv = P.sell_vals + s0 + lane;
+        const double* __restrict__ vv = vals + s0 + lane;
         double s = 0.;
         for (int k = 0; k < w; ++k)
-          s = fma(vv[int64_t(k) * 64], v[cc[int64_t(k) * 64]], s);
+          s = fma(vv[int64_t(k) * 64], v(cc[int64_t(k) * 64]), s);
         acc[j] = s;
       }
     }
@@ -263,20 +326,30 @@ __global__ void __launch_bounds__(1024) k_apply(PcgParams P, const double* __res
   if (b < 0)
     return;
   const int inst = P.a_inst[b];
-  const int64_t r0 = P.a_row0[b], r1 = P.a_row0[b + 1];
-  double acc[ITEMS];
-  apply_rows<FMT, ITEMS>(P, b, inst, r0, r1, x, acc);
+  const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
+  const double* vals = (FMT == FMT_SELL) ? P.sell_vals : P.dia;
+  for (int64_t r0 = rb; r0 < r1; r0 += int64_t(blockDim.x) * ITEMS) {
+    double acc[ITEMS];
+    apply_rows<FMT, ITEMS, false>(P, vals, b, inst, rb, r0, r1, [&](int64_t c) { return x[c]; }, acc);
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-    if (i < r1)
-      y[i] = acc[j];
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      if (i < r1)
+        y[i] = acc[j];
+    }
   }
 }
 
-// r = b - A x0, p = z = D^{-1} r, block partials of b.b, r.r, r.z
+// Jacobi-preconditioned CG run as CG on the symmetrically scaled system
+//   Ah = D^{-1/2} A D^{-1/2},  xh = D^{1/2} x,  rh = D^{-1/2} r,
+// which in exact arithmetic has the iterates of Jacobi-PCG (z = D^{-1} r, rh.rh = r.z,
+// p = D^{-1/2} ph, the same alpha and beta).  The unpreconditioned residual for the stopping test is
+// r = D^{1/2} rh.
+//
+// initialisation: r = b - A x0, rh = D^{-1/2} r, xh = D^{1/2} x0 (0 for a zero guess);
+// block partials of rh.rh and r.r (the form step 0 reduces) and of b.b
 template <int ITEMS>
-__global__ void __launch_bounds__(1024) k_pcg_init(PcgParams P)
+__global__ void __launch_bounds__(1024) k_cg_init(PcgParams P)
 {
   const int b = logical_block(P.v_nb);
   if (b < 0)
@@ -289,56 +362,129 @@ __global__ void __launch_bounds__(1024) k_pcg_init(PcgParams P)
     if (i < r1) {
       const double bi = P.b[i];
       const double ri = P.ax ? bi - P.ax[i] : bi;
-      const double zi = P.dinv[i] * ri;
-      P.r[i] = ri;
-      P.p[i] = zi;
-      if (P.zero_guess)
-        P.x[i] = 0.;
-      v[0] += bi * bi, v[1] += ri * ri, v[2] += ri * zi;
+      const double rh = P.dh[i] * ri;
+      P.r[i] = rh;
+      P.x[i] = P.zero_guess ? 0. : P.dsq[i] * P.x[i];
+      v[0] += rh * rh, v[1] += ri * ri, v[2] += bi * bi;
     }
   }
   block_sum<3>(v);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      P.part3[int64_t(b) * 3 + k] = v[k];
+  if (threadIdx.x == 0) {
+    P.part2[int64_t(b) * 2 + 0] = v[0];
+    P.part2[int64_t(b) * 2 + 1] = v[1];
+    P.part3[b] = v[2];
+  }
 }
 
-// one block per instance: initial scalars into record 0
-__global__ void __launch_bounds__(256) k_pcg_init_scalars(PcgParams P)
+// one block per instance: record 0 (tolerance from b.b, no iteration done yet)
+__global__ void __launch_bounds__(256) k_cg_init_scalars(PcgParams P)
 {
   const int inst = blockIdx.x;
-  double t[3];
-  reduce_partials<3>(P.part3, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  double t[1];
+  reduce_partials<1>(P.part3, P.v_ib0[inst], P.v_ib0[inst + 1], t);
   if (threadIdx.x == 0) {
     PcgScalars S{};
-    S.bb = t[0], S.rr = t[1], S.rz = t[2];
+    S.bb = t[0];
     S.tol2 = P.rtol2 * t[0];
     S.iters = 0;
-    S.done = (t[1] <= S.tol2) ? 1 : 0;
+    S.done = 0;
     P.S[inst] = S;
   }
 }
 
+// CG direction update of step j (par = j & 1), on the vector row partition:
+//   rho_j = rh.rh and |r|^2 from the partials of the previous update (or of the initialisation),
+//   stopping test; xh += alpha_{j-1} p (the update deferred from step j-1);
+//   p = rh + beta p with beta = rho_j / rho_{j-1} (p = rh at step 0).
+// Reads record par, writes record par ^ 1.
+template <int ITEMS>
+__global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
+{
+  const int b = logical_block(P.v_nb);
+  if (b < 0)
+    return;
+  const int inst = P.v_inst[b];
+  const PcgScalars* cur = P.S + par * P.ninst + inst;
+  PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
+  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
+  const int done = cur->done;
+  if (done) {
+    if (writer)
+      *nxt = *cur;
+    return;
+  }
+  const int64_t it = cur->iters;
+  const double rho_prev = cur->rz, alpha = cur->alpha, tol2 = cur->tol2;
+  double t[2];
+  reduce_partials<2>(P.part2, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  const int status = (t[1] <= tol2) ? 1 : (it >= P.max_iter) ? 2 : 0;
+  const bool first = (it == 0);
+  const double beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+  double pv[ITEMS], xv[ITEMS], rv[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+    const bool ok = i < r1;
+    pv[j] = (ok && !first) ? P.p[i] : 0.;
+    xv[j] = (ok && !first) ? P.x[i] : 0.;
+    rv[j] = (ok && status == 0) ? P.r[i] : 0.;
+  }
+  if (!first) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      if (i < r1) {
+        P.x[i] = fma(alpha, pv[j], xv[j]);
+        if (status == 0)
+          P.p[i] = fma(beta, pv[j], rv[j]);
+      }
+    }
+  } else if (status == 0) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      if (i < r1)
+        P.p[i] = rv[j];
+    }
+  }
+  if (writer) {
+    nxt->bb = cur->bb;
+    nxt->tol2 = tol2;
+    nxt->alpha = alpha;
+    nxt->pq = cur->pq;
+    nxt->rr = t[1];
+    nxt->rz = t[0];
+    nxt->beta = beta;
+    nxt->iters = status ? it : it + 1;
+    nxt->done = status;
+  }
+}
+
+// q = Ah p and the block partials of p.q, on the operator's row partition (record par ^ 1, read only)
 template <int FMT, int ITEMS>
-__global__ void __launch_bounds__(1024) k_pcg_spmv(PcgParams P, int par)
+__global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 {
   const int b = logical_block(P.a_nb);
   if (b < 0)
     return;
   const int inst = P.a_inst[b];
-  if (P.S[par * P.ninst + inst].done)
+  if (P.S[(par ^ 1) * P.ninst + inst].done)
     return;
-  const int64_t r0 = P.a_row0[b], r1 = P.a_row0[b + 1];
-  double acc[ITEMS];
-  apply_rows<FMT, ITEMS>(P, b, inst, r0, r1, P.p, acc);
+  const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
+  const double* __restrict__ p = P.p;
   double v[1] = {0.};
+  for (int64_t r0 = rb; r0 < r1; r0 += int64_t(blockDim.x) * ITEMS) {
+    double acc[ITEMS];
+    apply_rows<FMT, ITEMS, FMT != FMT_SELL>(P, P.scaled, b, inst, rb, r0, r1, [&](int64_t c) { return p[c]; },
+                                            acc);
 #pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-    if (i < r1) {
-      P.q[i] = acc[j];
-      v[0] += P.p[i] * acc[j];
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      if (i < r1) {
+        P.q[i] = acc[j];
+        v[0] += p[i] * acc[j];
+      }
     }
   }
   block_sum<1>(v);
@@ -346,50 +492,50 @@ __global__ void __launch_bounds__(1024) k_pcg_spmv(PcgParams P, int par)
     P.part1[b] = v[0];
 }
 
+// CG update of step j on the vector row partition: alpha = rho_j / p.q, rh -= alpha q, block partials
+// of rh.rh and |D^{1/2} rh|^2.  Works on record par ^ 1 (written by the step kernel): stores alpha
+// there for the next step's deferred x update, or the breakdown state.
 template <int ITEMS>
-__global__ void __launch_bounds__(1024) k_pcg_update(PcgParams P, int par)
+__global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 {
   const int b = logical_block(P.v_nb);
   if (b < 0)
     return;
   const int inst = P.v_inst[b];
-  PcgScalars* S = P.S + par * P.ninst + inst;
+  PcgScalars* S = P.S + (par ^ 1) * P.ninst + inst;
   if (S->done)
     return;
-  const bool first = (b == P.v_ib0[inst]);
+  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
   double pq[1];
   reduce_partials<1>(P.part1, P.a_ib0[inst], P.a_ib0[inst + 1], pq);
   if (!(pq[0] > 0.)) {
     // breakdown (A not positive definite on the Krylov space): every block of the instance sees it
-    if (first && threadIdx.x == 0)
+    if (writer)
       S->done = 3;
     return;
   }
   const double alpha = S->rz / pq[0];
-  if (first && threadIdx.x == 0)
+  if (writer)
     S->alpha = alpha, S->pq = pq[0];
   const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-  double xv[ITEMS], pv[ITEMS], qv[ITEMS], rv[ITEMS], dv[ITEMS];
+  double qv[ITEMS], rv[ITEMS], sv[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     const bool ok = i < r1;
-    xv[j] = ok ? P.x[i] : 0.;
-    pv[j] = ok ? P.p[i] : 0.;
     qv[j] = ok ? P.q[i] : 0.;
     rv[j] = ok ? P.r[i] : 0.;
-    dv[j] = ok ? P.dinv[i] : 0.;
+    sv[j] = ok ? P.dsq[i] : 0.;
   }
   double v[2] = {0., 0.};
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     if (i < r1) {
-      P.x[i] = xv[j] + alpha * pv[j];
-      const double ri = rv[j] - alpha * qv[j];
+      const double ri = fma(-alpha, qv[j], rv[j]);
       P.r[i] = ri;
-      const double zi = dv[j] * ri;
-      v[0] += ri * zi, v[1] += ri * ri;
+      const double ru = sv[j] * ri;
+      v[0] += ri * ri, v[1] += ru * ru;
     }
   }
   block_sum<2>(v);
@@ -397,64 +543,47 @@ __global__ void __launch_bounds__(1024) k_pcg_update(PcgParams P, int par)
     P.part2[int64_t(b) * 2 + 0] = v[0], P.part2[int64_t(b) * 2 + 1] = v[1];
 }
 
-template <int ITEMS>
-__global__ void __launch_bounds__(1024) k_pcg_pupdate(PcgParams P, int par)
+// x = D^{-1/2} xh after the last step
+__global__ void k_cg_finish(int64_t n, const double* __restrict__ dh, double* __restrict__ x)
 {
-  const int b = logical_block(P.v_nb);
-  if (b < 0)
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n)
+    x[i] = dh[i] * x[i];
+}
+
+// scaled DIA storage, one diagonal after the other (coalesced): Ah(i, i+o) = a(i, i+o) (h_i h_{i+o});
+// entries outside the instance are 0.  Diagonals below `kfirst` (the lower ones, never read by the
+// symmetric format) and the main diagonal (unit, never read) are skipped.
+__global__ void k_scale_dia(int64_t n, int ndiag, const int64_t* __restrict__ off, const int32_t* __restrict__ k0v,
+                            int lower_too, const int32_t* __restrict__ row_inst, const int64_t* __restrict__ inst_row0,
+                            const double* __restrict__ dia, const double* __restrict__ dh, double* __restrict__ dst)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= n)
     return;
-  const int inst = P.v_inst[b];
-  const int b0 = P.v_ib0[inst], b1 = P.v_ib0[inst + 1];
-  const PcgScalars* cur = P.S + par * P.ninst + inst;
-  PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
-  const bool writer = (b == b0 && threadIdx.x == 0);
-  // scalar fields are read one by one (no per-lane copy of the record)
-  const int done = cur->done;
-  if (done) {
-    if (writer)
-      *nxt = *cur;
+  const int inst = row_inst[i];
+  const int k0 = k0v[inst];
+  if (k == k0 || (!lower_too && k < k0))
     return;
-  }
-  const double rz = cur->rz, tol2 = cur->tol2;
-  const int64_t it = cur->iters + 1;
-  double t[2];
-  reduce_partials<2>(P.part2, b0, b1, t);
-  int status = 0;
-  double beta = 0.;
-  if (t[1] <= tol2) {
-    status = 1;
-  } else if (it >= P.max_iter) {
-    status = 2;
-  } else {
-    beta = t[0] / rz;
-    const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-    double rv[ITEMS], dv[ITEMS], pv[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-      const bool ok = i < r1;
-      rv[j] = ok ? P.r[i] : 0.;
-      dv[j] = ok ? P.dinv[i] : 0.;
-      pv[j] = ok ? P.p[i] : 0.;
-    }
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-      if (i < r1)
-        P.p[i] = dv[j] * rv[j] + beta * pv[j];
-    }
-  }
-  if (writer) {
-    nxt->bb = cur->bb;
-    nxt->tol2 = tol2;
-    nxt->alpha = cur->alpha;
-    nxt->pq = cur->pq;
-    nxt->rr = t[1];
-    nxt->iters = it;
-    nxt->done = status;
-    nxt->rz = status ? rz : t[0];
-    nxt->beta = beta;
-  }
+  const int64_t lo = inst_row0[inst], hi = inst_row0[inst + 1];
+  const int64_t c = i + off[int64_t(inst) * ndiag + k];
+  const int64_t at = int64_t(k) * n + i;
+  dst[at] = (c >= lo && c < hi) ? dia[at] * (dh[i] * dh[c]) : 0.;
+}
+
+// values of the scaled matrix in the storage layout: Ah_ij = a_ij (h_i h_j); the product h_i h_j is
+// symmetric in i and j, so an exactly symmetric A gives an exactly symmetric Ah
+__global__ void k_scale_values(int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
+                               const double* __restrict__ vals, const int64_t* __restrict__ pos,
+                               const double* __restrict__ dh, double* __restrict__ dst)
+{
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double hi = dh[i];
+  for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+    dst[pos[k]] = vals[k] * (hi * dh[cols[k]]);
 }
 
 __global__ void k_scatter_values(int64_t nnz, const double* __restrict__ vals, const int64_t* __restrict__ pos,
@@ -479,9 +608,9 @@ __global__ void k_fill(int64_t n, double* __restrict__ dst, double v)
     dst[s] = v;
 }
 
-// Jacobi preconditioner from the CSR diagonal; flags a non-positive diagonal
+// Jacobi scaling from the CSR diagonal: h_i = 1 / sqrt(a_ii), s_i = sqrt(a_ii); flags a non-positive diagonal
 __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
-                         const double* __restrict__ vals, double* __restrict__ dinv, int* bad)
+                         const double* __restrict__ vals, double* __restrict__ dh, double* __restrict__ dsq, int* bad)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
@@ -494,16 +623,18 @@ __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const i
     bad[0] = 1;
     d = 1.;
   }
-  dinv[i] = 1. / d;
+  const double sq = sqrt(d);
+  dh[i] = 1. / sq;
+  dsq[i] = sq;
 }
 
-// DIA preparation in one pass over the rows: Jacobi diagonal from the offset-0 diagonal (flagging a
+// DIA preparation in one pass over the rows: Jacobi scaling from the offset-0 diagonal (flagging a
 // non-positive entry) and exact symmetry: for every stored negative offset o, A(i, i+o) must equal
 // A(i+o, i), stored on the mirrored diagonal mirror[k] (offset -o) at row i+o
 __global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ off, const int32_t* __restrict__ k0v,
                               const int32_t* __restrict__ mirror, const int32_t* __restrict__ row_inst,
                               const int64_t* __restrict__ inst_row0, const double* __restrict__ dia,
-                              double* __restrict__ dinv, int* flags)
+                              double* __restrict__ dh, double* __restrict__ dsq, int* flags)
 {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n)
@@ -518,7 +649,9 @@ __global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ 
     flags[0] = 1;
     d = 1.;
   }
-  dinv[i] = 1. / d;
+  const double sq = sqrt(d);
+  dh[i] = 1. / sq;
+  dsq[i] = sq;
   bool ok = true;
   for (int k = 0; k < k0; ++k) {
     const int64_t c = i + o[k];
@@ -550,6 +683,13 @@ void dispatch_items(int items, F&& f)
     default:
       HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "items_per_thread must be 1, 2, 4 or 8");
   }
+}
+
+// rows a lane of the operator application loads together: 2 for items = 2, else 1 (a block with
+// items = 4, 8, 16 rows per lane processes them in chunks of block rows)
+inline int apply_lane_rows(int items)
+{
+  return items == 2 ? 2 : 1;
 }
 
 inline unsigned blocks_for(int64_t n, int block)
@@ -806,12 +946,23 @@ static void build_structure(Operator& op)
   ++C.version;
 }
 
-// algorithmic bytes of one operator application in the format in use
+// algorithmic bytes of one operator application in the format in use: matrix values (+ int32 columns for
+// SELL), x read, y written
 static int64_t apply_bytes(const Operator& op)
 {
   const SolverFormat& F = op.fmt;
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
+  return 12 * op.nnz + 16 * op.n;
+}
+
+// algorithmic bytes of the CG solver's operator application: the scaled matrix without its unit diagonal
+// (DIA) or with it (SELL, + int32 columns), p read, q written
+static int64_t step_bytes(const Operator& op)
+{
+  const SolverFormat& F = op.fmt;
+  if (F.format == HGDT_FORMAT_DIA)
+    return int64_t((F.symmetric ? F.nup : F.ndiag) - 1) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
 }
 
@@ -838,7 +989,9 @@ static PcgParams make_params(Operator& op)
   P.slice_ptr = F.d_slice_ptr.ptr;
   P.sell_cols = F.d_sell_cols.ptr;
   P.sell_vals = F.d_sell_vals.ptr;
-  P.dinv = F.d_dinv.ptr;
+  P.scaled = F.d_scaled.ptr;
+  P.dh = F.d_dh.ptr;
+  P.dsq = F.d_dsq.ptr;
   P.ninst = int(op.space->inst_dof.size()) - 1;
   return P;
 }
@@ -867,27 +1020,41 @@ void Operator::prepare_solver(hipStream_t stream)
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
   hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
                      dia ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
-  F.d_dinv.allocate(std::max<int64_t>(n, 1));
+  F.d_dh.allocate(std::max<int64_t>(n, 1));
+  F.d_dsq.allocate(std::max<int64_t>(n, 1));
   C.flags.allocate(2);
   if (!C.host_flags)
     HGDT_HIP(hipHostMalloc((void**)&C.host_flags, 2 * sizeof(int)));
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
   if (dia)
     hipLaunchKernelGGL(k_dia_prepare, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, F.ndiag, F.d_dia_offsets.ptr,
-                       C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dinv.ptr,
-                       C.flags.ptr);
+                       C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dh.ptr,
+                       F.d_dsq.ptr, C.flags.ptr);
   else
     hipLaunchKernelGGL(k_jacobi, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
-                       d_values.ptr, F.d_dinv.ptr, C.flags.ptr);
+                       d_values.ptr, F.d_dh.ptr, F.d_dsq.ptr, C.flags.ptr);
   HGDT_HIP(hipGetLastError());
   HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
   HGDT_HIP(hipStreamSynchronize(stream));
   F.diag_positive = (C.host_flags[0] == 0);
   const bool sym = dia && C.host_flags[1] == 0 && allow_symmetric;
+  // the scaled matrix for the CG solver, in the same layout
+  F.d_scaled.allocate(std::max<int64_t>(total, 1));
+  if (dia) {
+    hipLaunchKernelGGL(k_scale_dia, dim3(blocks_for(n, B), unsigned(F.ndiag)), dim3(B), 0, stream, n, F.ndiag,
+                       F.d_dia_offsets.ptr, C.dia_k0.ptr, sym ? 0 : 1, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr,
+                       F.d_dh.ptr, F.d_scaled.ptr);
+  } else {
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, F.d_scaled.ptr, 0.);
+    hipLaunchKernelGGL(k_scale_values, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
+                       d_values.ptr, F.d_sell_pos.ptr, F.d_dh.ptr, F.d_scaled.ptr);
+  }
+  HGDT_HIP(hipGetLastError());
   if (sym != F.symmetric)
     ++C.version; // captured graphs hold the kernel variant
   F.symmetric = sym;
   F.bytes = apply_bytes(*this);
+  F.step_bytes = step_bytes(*this);
   F.valid = true;
 }
 
@@ -898,7 +1065,7 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
   const int block = fmt.A.block;
   const unsigned grid = hw_blocks(fmt.A.nblocks);
   const int kf = kernel_format(*this);
-  dispatch_items(fmt.A.items, [&](auto IT) {
+  dispatch_items(apply_lane_rows(fmt.A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     if (kf == FMT_DIA)
       hipLaunchKernelGGL((k_apply<FMT_DIA, I>), dim3(grid), dim3(block), 0, stream, P, x, y);
@@ -916,25 +1083,27 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   const RowPartition& A = op.fmt.A;
   const RowPartition& V = op.fmt.V;
   const int kf = kernel_format(op);
+  dispatch_items(V.items, [&](auto IT) {
+    constexpr int I = decltype(IT)::value;
+    hipLaunchKernelGGL((k_cg_direction<I>), dim3(hw_blocks(V.nblocks)), dim3(V.block), 0, stream, P, par);
+  });
   if (ev0)
     HGDT_HIP(hipEventRecord(ev0, stream));
-  dispatch_items(A.items, [&](auto IT) {
+  dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(A.nblocks);
     if (kf == FMT_DIA)
-      hipLaunchKernelGGL((k_pcg_spmv<FMT_DIA, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      hipLaunchKernelGGL((k_cg_spmv<FMT_DIA, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
     else if (kf == FMT_DIAS)
-      hipLaunchKernelGGL((k_pcg_spmv<FMT_DIAS, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      hipLaunchKernelGGL((k_cg_spmv<FMT_DIAS, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
     else
-      hipLaunchKernelGGL((k_pcg_spmv<FMT_SELL, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      hipLaunchKernelGGL((k_cg_spmv<FMT_SELL, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
   });
   if (ev1)
     HGDT_HIP(hipEventRecord(ev1, stream));
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    const unsigned grid = hw_blocks(V.nblocks);
-    hipLaunchKernelGGL((k_pcg_update<I>), dim3(grid), dim3(V.block), 0, stream, P, par);
-    hipLaunchKernelGGL((k_pcg_pupdate<I>), dim3(grid), dim3(V.block), 0, stream, P, par);
+    hipLaunchKernelGGL((k_cg_update<I>), dim3(hw_blocks(V.nblocks)), dim3(V.block), 0, stream, P, par);
   });
 }
 
@@ -951,7 +1120,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   d_r.allocate(std::max<int64_t>(n, 1));
   d_p.allocate(std::max<int64_t>(n, 1));
   d_q.allocate(std::max<int64_t>(n, 1));
-  d_partials.allocate(std::max<int64_t>(fmt.A.nblocks + 5 * fmt.V.nblocks, 1));
+  d_partials.allocate(std::max<int64_t>(fmt.A.nblocks + 3 * fmt.V.nblocks, 1));
   d_scalars.allocate(2 * ninst);
   if (C.host_capacity < ninst) {
     for (auto& h : C.host_scalars) {
@@ -994,14 +1163,15 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   }
   dispatch_items(fmt.V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    hipLaunchKernelGGL((k_pcg_init<I>), dim3(hw_blocks(fmt.V.nblocks)), dim3(fmt.V.block), 0, stream, P);
+    hipLaunchKernelGGL((k_cg_init<I>), dim3(hw_blocks(fmt.V.nblocks)), dim3(fmt.V.block), 0, stream, P);
   });
-  hipLaunchKernelGGL(k_pcg_init_scalars, dim3(unsigned(ninst)), dim3(256), 0, stream, P);
+  hipLaunchKernelGGL(k_cg_init_scalars, dim3(unsigned(ninst)), dim3(256), 0, stream, P);
   HGDT_HIP(hipGetLastError());
   P.ax = nullptr;
 
-  // an even number of iterations per chunk keeps the parity of every captured launch fixed and
-  // leaves the current scalars in record 0 after each chunk
+  // an even number of steps per chunk keeps the parity of every captured launch fixed and leaves the
+  // current scalars in record 0 after each chunk; the last step of a solve runs in the chunk after the
+  // one whose update met the tolerance (the stopping test is the first thing a step does)
   const int chunk = opt.check_every + (opt.check_every & 1);
   if (graph && (C.graph == nullptr || C.graph_x != x || C.graph_chunk != chunk || C.graph_version != C.version
                 || C.graph_b != b || C.graph_maxit != opt.max_iter)) {
@@ -1051,7 +1221,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     return true;
   };
   // double-buffered convergence checks: chunk k+1 is queued before the flags of chunk k are read
-  const int64_t max_chunks = (opt.max_iter + chunk - 1) / chunk + 1;
+  const int64_t max_chunks = (opt.max_iter + 1 + chunk - 1) / chunk + 1;
   int cur = 0;
   int64_t launched = 0;
   launch_chunk();
@@ -1080,6 +1250,8 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     }
     cur = nxt;
   }
+  hipLaunchKernelGGL(k_cg_finish, dim3(blocks_for(n, 256)), dim3(256), 0, stream, n, fmt.d_dh.ptr, x);
+  HGDT_HIP(hipGetLastError());
   HGDT_HIP(hipEventRecord(C.t1, stream));
   HGDT_HIP(hipEventSynchronize(C.t1));
   if (graph) {
@@ -1117,6 +1289,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     st.spmv_avg_ms = cnt ? total / cnt : 0.;
     st.spmv_count = cnt;
   }
+  st.step_bytes = fmt.step_bytes;
   if (!st.converged) {
     if (st.breakdown)
       HGDT_THROW(HGDT_ERR_SOLVER_FAILED, "Jacobi-PCG breakdown: p.Ap <= 0 (matrix not SPD)");

@@ -115,6 +115,7 @@ class SolverStats(ctypes.Structure):
         ("spmv_avg_ms", ctypes.c_double),
         ("spmv_count", ctypes.c_int64),
         ("solve_ms", ctypes.c_double),
+        ("step_bytes", ctypes.c_int64),
     ]
 
     def as_dict(self):

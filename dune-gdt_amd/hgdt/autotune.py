@@ -14,7 +14,8 @@ import torch
 
 from . import _lib as L
 
-APPLY_CANDIDATES = [(64, 1), (128, 1), (256, 1), (512, 1), (1024, 1), (256, 2), (512, 2), (128, 2)]
+APPLY_CANDIDATES = [(64, 1), (128, 1), (256, 1), (512, 1), (1024, 1), (256, 2), (512, 2), (128, 2), (256, 4),
+                    (512, 4), (256, 8), (512, 8), (1024, 4), (256, 16)]
 VECTOR_CANDIDATES = [(256, 1), (256, 2), (256, 4), (256, 8), (512, 1), (512, 2), (512, 4), (512, 8),
                      (1024, 1), (1024, 2), (1024, 4), (128, 4), (128, 8)]
 ASSEMBLY_CANDIDATES = [64, 128, 256]

@@ -229,7 +229,8 @@ typedef struct hgdt_solver_options {
   int64_t max_iter;       /* per instance */
   int32_t check_every;    /* iterations between host-side convergence checks (>= 1) */
   int32_t use_graph;      /* 1: replay the iteration block as a captured hipGraph */
-  int32_t time_spmv;      /* k > 0: record HIP events around every k-th operator application */
+  int32_t time_spmv;      /* k > 0: record HIP events around the operator application of every k-th
+                             iteration */
   int32_t zero_initial_guess; /* 1: start from x = 0, else from the given x */
 } hgdt_solver_options;
 
@@ -238,14 +239,16 @@ typedef struct hgdt_solver_stats {
   int32_t breakdown;      /* non-positive curvature or non-positive diagonal met */
   int64_t iterations;     /* max over instances */
   double relres;          /* max over instances of ||r_k|| / ||b|| (recursive residual) */
-  double spmv_avg_ms;     /* mean duration of one operator application (time_spmv = 1) */
+  double spmv_avg_ms;     /* mean duration of the timed operator applications (time_spmv > 0) */
   int64_t spmv_count;     /* applications timed */
   double solve_ms;        /* wall time of the solve measured with HIP events */
+  int64_t step_bytes;     /* algorithmic bytes of the solver's operator application (the kernel timed
+                             by time_spmv) */
 } hgdt_solver_stats;
 
 hgdt_status hgdt_solver_options_default(hgdt_solver_options* options);
 /* ConstMatrixOperator::apply_inverse (matrix-based.hh:148-159) with a Jacobi-preconditioned CG
- * (x0 = 0 by default); b and x are device vectors of length rows.  For a union grid every instance
+ * (x0 = 0 by default), run as CG on D^{-1/2} A D^{-1/2}; b and x are device vectors of length rows.  For a union grid every instance
  * is solved with its own scalars and its own stopping test in the same launches.  Blocks the host
  * until convergence is known.  Fails with HGDT_ERR_SOLVER_FAILED when an instance does not converge. */
 hgdt_status hgdt_operator_apply_inverse(hgdt_operator op, const double* b, double* x,
@@ -265,7 +268,9 @@ hgdt_status hgdt_operator_adjoint_log_diffusion_gradient(hgdt_operator op, int32
 
 /* launch configuration of the tunable kernels (used by the autotuner); kernel ids: 0 element assembly,
  * 1 intersection assembly, 2 scatter/gather, 3 operator application, 4 vector updates.
- * block_size multiple of 64 in [64, 1024]; items_per_thread in [1, 8]. */
+ * block_size multiple of 64 in [64, 1024] (assembly kernels: at most 256); items_per_thread = rows per
+ * lane, a power of two up to 8, or up to 16 for kernel 3, which loads 1 or 2 rows of a lane together and
+ * processes larger counts in chunks (fewer, larger blocks: fewer partial sums for the solver). */
 hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t block_size, int32_t items_per_thread);
 hgdt_status hgdt_operator_get_launch(hgdt_operator op, int32_t kernel, int32_t* block_size, int32_t* items_per_thread);
 

@@ -176,13 +176,23 @@ def test_launch_shapes_and_autotuner(hgdt_mod):
     orp, ocols = oop.pattern()
     xo, ito, relo, rc = O.pcg_jacobi(orp, ocols, oop.values(), oop.rhs(), rtol=1e-10)
     L = hgdt_mod._lib
-    for a, v in [((64, 1), (256, 8)), ((512, 2), (1024, 4)), ((128, 1), (128, 1)), ((1024, 1), (64, 2))]:
+    yo = op.apply(torch.from_numpy(xo).cuda()).cpu().numpy()  # default launch shapes
+    assert np.abs(yo - O.spmv(orp, ocols, oop.values(), xo)).max() <= 1e-13 * np.abs(yo).max()
+    # (256, 8) and (128, 16): operator-application blocks that process their rows in chunks
+    for a, v in [((64, 1), (256, 8)), ((512, 2), (1024, 4)), ((128, 1), (128, 1)), ((1024, 1), (64, 2)),
+                 ((256, 8), (256, 4)), ((128, 16), (512, 1))]:
         op.set_launch(L.KERNEL_APPLY, *a)
         op.set_launch(L.KERNEL_VECTOR, *v)
         x = op.apply_inverse("rhs").cpu().numpy()
         rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
         print(f"  apply {a} vector {v}: iters {op.last_solve['iterations']} rel {rel:.2e}")
         assert rel <= SOLUTION_TOL and abs(op.last_solve["iterations"] - ito) <= 2
+        # the operator application is the same sum in every launch shape and format
+        for fmt in (L.FORMAT_AUTO, L.FORMAT_SELL):
+            op.set_format(fmt)
+            y = op.apply(torch.from_numpy(xo).cuda()).cpu().numpy()
+            assert np.array_equal(y, yo), f"apply differs for launch {a} format {fmt}"
+        op.set_format(L.FORMAT_AUTO)
     choice = autotune.tune_solver(op, iterations=10, repeats=1)
     assert tuple(choice["apply"]) in autotune.APPLY_CANDIDATES
     assert tuple(choice["vector"]) in autotune.VECTOR_CANDIDATES
