@@ -42,6 +42,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "hgdt_internal.hpp"
 
 namespace hgdt {
@@ -1077,6 +1079,23 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
   HGDT_HIP(hipGetLastError());
 }
 
+// launch of k(P, par); with events, they are recorded by the dispatch itself (start and end of the
+// kernel, the interval a kernel trace reports), not by separate markers around it
+template <class K>
+static void launch_pcg(K kernel, unsigned grid, int block, hipStream_t stream, const PcgParams& P, int par,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+{
+  if (!ev0) {
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, P, par);
+    return;
+  }
+  PcgParams p = P;
+  int q = par;
+  void* args[] = {&p, &q};
+  HGDT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(block), args, 0, stream, ev0,
+                              ev1, 0));
+}
+
 static void launch_iteration(const Operator& op, const PcgParams& P, int par, hipStream_t stream, hipEvent_t ev0,
                              hipEvent_t ev1)
 {
@@ -1085,25 +1104,21 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   const int kf = kernel_format(op);
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    hipLaunchKernelGGL((k_cg_direction<I>), dim3(hw_blocks(V.nblocks)), dim3(V.block), 0, stream, P, par);
+    launch_pcg(k_cg_direction<I>, hw_blocks(V.nblocks), V.block, stream, P, par);
   });
-  if (ev0)
-    HGDT_HIP(hipEventRecord(ev0, stream));
   dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(A.nblocks);
     if (kf == FMT_DIA)
-      hipLaunchKernelGGL((k_cg_spmv<FMT_DIA, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      launch_pcg(k_cg_spmv<FMT_DIA, I>, grid, A.block, stream, P, par, ev0, ev1);
     else if (kf == FMT_DIAS)
-      hipLaunchKernelGGL((k_cg_spmv<FMT_DIAS, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      launch_pcg(k_cg_spmv<FMT_DIAS, I>, grid, A.block, stream, P, par, ev0, ev1);
     else
-      hipLaunchKernelGGL((k_cg_spmv<FMT_SELL, I>), dim3(grid), dim3(A.block), 0, stream, P, par);
+      launch_pcg(k_cg_spmv<FMT_SELL, I>, grid, A.block, stream, P, par, ev0, ev1);
   });
-  if (ev1)
-    HGDT_HIP(hipEventRecord(ev1, stream));
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    hipLaunchKernelGGL((k_cg_update<I>), dim3(hw_blocks(V.nblocks)), dim3(V.block), 0, stream, P, par);
+    launch_pcg(k_cg_update<I>, hw_blocks(V.nblocks), V.block, stream, P, par);
   });
 }
 
