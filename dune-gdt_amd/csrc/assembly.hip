@@ -73,6 +73,10 @@ struct FaceParams {
   QRule rule[MAX_FFORMS];
   double* staging;
   int64_t base;
+  // every appended form is symmetric (symmetry prefactor 1 or a penalty): the blocks are stored
+  // exactly symmetric (in_in, out_out and boundary blocks from their upper triangle, out_in as the
+  // transpose of in_out), so the assembled matrix is symmetric bit for bit
+  int symmetric;
 };
 
 template <int GEOM>
@@ -471,13 +475,25 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
     }
   }
   double* __restrict__ out = P.staging + P.base + fi * (4 * K2);
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
+  if (P.symmetric) {
 #pragma unroll
     for (int i = 0; i < ND; ++i)
 #pragma unroll
-      for (int j = 0; j < ND; ++j)
-        out[k * K2 + i * ND + j] = B[k][i][j];
+      for (int j = 0; j < ND; ++j) {
+        out[0 * K2 + i * ND + j] = (i <= j) ? B[0][i][j] : B[0][j][i];
+        out[1 * K2 + i * ND + j] = B[1][i][j];
+        out[2 * K2 + i * ND + j] = B[1][j][i];
+        out[3 * K2 + i * ND + j] = (i <= j) ? B[3][i][j] : B[3][j][i];
+      }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+          out[k * K2 + i * ND + j] = B[k][i][j];
+  }
 }
 
 template <int GEOM>
@@ -537,7 +553,7 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
   for (int i = 0; i < ND; ++i)
 #pragma unroll
     for (int j = 0; j < ND; ++j)
-      out[i * ND + j] = B[i][j];
+      out[i * ND + j] = (P.symmetric && i > j) ? B[j][i] : B[i][j];
 }
 
 // fixed-order gather of the staged local contributions into the CSR values
@@ -835,6 +851,14 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, Element
   };
   fill_face(op.cforms, C);
   fill_face(op.bforms, B);
+  // the SWIPDG terms are symmetric for symmetry prefactor 1 (laplace-ipdg.hh:107-186,340-368); the
+  // penalties always are (ipdg.hh:113-171,254-282)
+  bool sym = true;
+  for (const auto* forms : {&op.cforms, &op.bforms})
+    for (const auto& D : *forms)
+      if ((D.kind == FF_INNER_COUPLING || D.kind == FF_DIRICHLET_COUPLING) && D.param != 1.)
+        sym = false;
+  C.symmetric = B.symmetric = sym ? 1 : 0;
 }
 
 void Operator::prepare_assembly()

@@ -214,48 +214,60 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
       // lower triangle from the upper diagonals (descending offset = ascending column), then upper
       const int ku = P.ndiag - 1 - k0; // strict upper slots of the instance (trailing ones may be padding)
       if constexpr (ITEMS == 1) {
-        if (ku <= 8) {
-          // every load of the row is issued before the first FMA (one memory round trip per row);
-          // the FMAs run in the order of the loop below, so the bits are the same
-          constexpr int MU = 8;
-          const int64_t i = r0 + threadIdx.x;
-          const bool in = i < r1;
-          double al[MU], xl[MU], au[MU], xu[MU];
+        // the loads of up to 8 diagonals are issued before their FMAs (few memory round trips per
+        // row); the FMAs run in the order of the loops below, so the bits are the same
+        constexpr int MU = 8;
+        const int64_t i = r0 + threadIdx.x;
+        const bool in = i < r1;
+        double s = 0.;
+        for (int top = ku - 1; top >= 0; top -= MU) { // lower part, descending offset
+          double a[MU], x[MU];
 #pragma unroll
-          for (int u = 0; u < MU; ++u) {
-            al[u] = xl[u] = au[u] = xu[u] = 0.;
-            if (u < ku) {
-              const int64_t o = off[k0 + 1 + u];
-              const double* __restrict__ dk = dia + int64_t(k0 + 1 + u) * n;
-              const int64_t cl = i - o, cu = i + o;
-              const bool okl = in && (cl >= lo) && (cl < hi);
-              const bool oku = in && (cu >= lo) && (cu < hi);
-              al[u] = okl ? dk[cl] : 0.;
-              xl[u] = okl ? v(cl) : 0.;
-              au[u] = oku ? dk[i] : 0.;
-              xu[u] = oku ? v(cu) : 0.;
+          for (int t = 0; t < MU; ++t) {
+            const int u = top - t;
+            a[t] = x[t] = 0.;
+            if (u >= 0) {
+              const int64_t cl = i - off[k0 + 1 + u];
+              if (in && cl >= lo && cl < hi) {
+                a[t] = dia[int64_t(k0 + 1 + u) * n + cl];
+                x[t] = v(cl);
+              }
             }
           }
-          const double ad = (in && !UNIT) ? dia[int64_t(k0) * n + i] : 0.;
-          const double xd = in ? v(i) : 0.;
-          double s = 0.;
 #pragma unroll
-          for (int u = MU - 1; u >= 0; --u)
-            if (u < ku)
-              s = fma(al[u], xl[u], s);
-          if constexpr (UNIT) {
-            if (in)
-              s += xd;
-          } else {
-            s = fma(ad, xd, s);
+          for (int t = 0; t < MU; ++t)
+            if (top - t >= 0)
+              s = fma(a[t], x[t], s);
+        }
+        if constexpr (UNIT) {
+          if (in)
+            s += v(i);
+        } else {
+          const double ad = in ? dia[int64_t(k0) * n + i] : 0.;
+          const double xd = in ? v(i) : 0.;
+          s = fma(ad, xd, s);
+        }
+        for (int bot = 0; bot < ku; bot += MU) { // upper part, ascending offset
+          double a[MU], x[MU];
+#pragma unroll
+          for (int t = 0; t < MU; ++t) {
+            const int u = bot + t;
+            a[t] = x[t] = 0.;
+            if (u < ku) {
+              const int64_t cu = i + off[k0 + 1 + u];
+              if (in && cu >= lo && cu < hi) {
+                a[t] = dia[int64_t(k0 + 1 + u) * n + i];
+                x[t] = v(cu);
+              }
+            }
           }
 #pragma unroll
-          for (int u = 0; u < MU; ++u)
-            if (u < ku)
-              s = fma(au[u], xu[u], s);
-          acc[0] = s;
-          return;
+          for (int t = 0; t < MU; ++t)
+            if (bot + t < ku)
+              s = fma(a[t], x[t], s);
         }
+        acc[0] = s;
+        return;
       }
 #pragma unroll 2
       for (int k = P.ndiag - 1; k > k0; --k) {

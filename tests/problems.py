@@ -10,7 +10,7 @@ import numpy as np
 import oracle_lib as O
 
 
-def oracle_problem(kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None):
+def oracle_problem(kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None, symmetry=1.0):
     if kind == "heat_q1":
         m = O.Mesh.structured(O.QUAD, cells, lo or [0, 0], hi or [1, 1])
         s = O.Space(m, True, 1)
@@ -30,9 +30,9 @@ def oracle_problem(kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None):
         s = O.Space(m, False, 1)
         op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
         op.append_laplace(kappa)
-        op.append_inner_coupling(1.0, kappa, kappa)
+        op.append_inner_coupling(symmetry, kappa, kappa)
         op.append_inner_penalty(sigma, kappa)
-        op.append_dirichlet_coupling(1.0, kappa)
+        op.append_dirichlet_coupling(symmetry, kappa)
         op.append_boundary_penalty(sigma, kappa)
         op.append_source(f)
     else:
@@ -40,7 +40,7 @@ def oracle_problem(kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None):
     return m, s, op
 
 
-def hgdt_problem(hgdt, kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None, grid=None):
+def hgdt_problem(hgdt, kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None, grid=None, symmetry=1.0):
     """Same problem through the reference-named API of hgdt (the HIP path)."""
     if kind == "heat_q1":
         grid = grid or hgdt.Grid.structured("quad", lo or [0, 0], hi or [1, 1], cells)
@@ -58,10 +58,10 @@ def hgdt_problem(hgdt, kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None, 
         op = hgdt.make_matrix_operator(space, hgdt.Stencil.element_and_intersection)
         op += hgdt.LocalElementIntegralBilinearForm(hgdt.LocalLaplaceIntegrand(kappa))
         op.append(hgdt.LocalCouplingIntersectionIntegralBilinearForm(
-            hgdt.LocalLaplaceIPDGInnerCoupling(1.0, kappa, kappa)), {}, hgdt.ApplyOn.InnerIntersectionsOnce())
+            hgdt.LocalLaplaceIPDGInnerCoupling(symmetry, kappa, kappa)), {}, hgdt.ApplyOn.InnerIntersectionsOnce())
         op.append(hgdt.LocalCouplingIntersectionIntegralBilinearForm(hgdt.LocalIPDGInnerPenalty(sigma, kappa)), {},
                   hgdt.ApplyOn.InnerIntersectionsOnce())
-        op.append(hgdt.LocalIntersectionIntegralBilinearForm(hgdt.LocalLaplaceIPDGDirichletCoupling(1.0, kappa)),
+        op.append(hgdt.LocalIntersectionIntegralBilinearForm(hgdt.LocalLaplaceIPDGDirichletCoupling(symmetry, kappa)),
                   {}, hgdt.ApplyOn.CustomBoundaryIntersections(hgdt.AllDirichletBoundaryInfo()))
         op.append(hgdt.LocalIntersectionIntegralBilinearForm(hgdt.LocalIPDGBoundaryPenalty(sigma, kappa)), {},
                   hgdt.ApplyOn.CustomBoundaryIntersections(hgdt.AllDirichletBoundaryInfo()))

@@ -101,8 +101,6 @@ def test_apply_parity(hgdt_mod, kind, cells, seed):
     L = hgdt_mod._lib
     ys = {}
     for fmt in (L.FORMAT_AUTO, L.FORMAT_DIA_NONSYMMETRIC, L.FORMAT_SELL):
-        if fmt == L.FORMAT_DIA_NONSYMMETRIC and kind == "swipdg":
-            continue
         op.set_format(fmt)
         y = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
         err = np.abs(y - yo).max() / np.abs(yo).max()
@@ -110,19 +108,41 @@ def test_apply_parity(hgdt_mod, kind, cells, seed):
         ys[name] = y
         print(f"  {name}: max rel err {err:.3e}")
         assert err <= 1e-13
-    if kind != "swipdg":
-        # continuous Lagrange matrices are assembled exactly symmetric, so the symmetric DIA path is taken
-        # and, summing every row in column order, agrees bit for bit with full DIA and SELL
-        assert "dia_symmetric" in ys
-        assert np.array_equal(ys["dia_symmetric"], ys["dia"]) and np.array_equal(ys["dia"], ys["sell64"])
+    # the matrices (continuous Lagrange, and symmetric interior penalty) are assembled exactly symmetric, so
+    # the symmetric DIA path is taken and, summing every row in column order, agrees bit for bit with full
+    # DIA and SELL
+    assert "dia_symmetric" in ys
+    assert np.array_equal(ys["dia_symmetric"], ys["dia"]) and np.array_equal(ys["dia"], ys["sell64"])
 
 
-@pytest.mark.parametrize("kind,cells,seed", [CASES[1], CASES[4]])
+@pytest.mark.parametrize("kind,cells,seed", [CASES[1], CASES[4], CASES[6]])
 def test_assembled_matrix_exactly_symmetric(hgdt_mod, kind, cells, seed):
     print(f"case symmetry {kind} cells={cells} seed={seed}", flush=True)
     oop, op = _build(hgdt_mod, kind, cells, seed)
     A = op.matrix.to_scipy()
     assert (A != A.T).nnz == 0
+
+
+def test_nonsymmetric_interior_penalty(hgdt_mod):
+    """NIPG (symmetry prefactor -1): blocks stored as computed, no symmetric storage, parity with the oracle"""
+    print("case NIPG swipdg 10x8", flush=True)
+    cells, seed = [10, 8], 4
+    kappa = fields(hgdt_mod, "swipdg", cells, seed)
+    m, s, oop = oracle_problem("swipdg", cells, kappa, symmetry=-1.0)
+    oop.assemble(1)
+    grid, space, op = hgdt_problem(hgdt_mod, "swipdg", cells, torch.from_numpy(kappa).cuda(), symmetry=-1.0)
+    op.assemble()
+    orp, ocols = oop.pattern()
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), orp)
+    print(f"  matrix max rel err {err:.3e}")
+    assert err <= MATRIX_TOL
+    A = op.matrix.to_scipy()
+    assert (A != A.T).nnz > 0
+    x = np.random.default_rng(5).standard_normal(op.shape[0])
+    yo = O.spmv(orp, ocols, oop.values(), x)
+    y = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert op.format_info()["format"] == "dia"
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
 
 
 def test_bitwise_reproducible(hgdt_mod):
