@@ -4,8 +4,10 @@
 // keeps the message of the last failure in a thread-local string.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "hgdt_internal.hpp"
 
@@ -341,20 +343,55 @@ hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* s
     require(grid && space, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
     require(kind == HGDT_CONTINUOUS_LAGRANGE || kind == HGDT_DISCONTINUOUS_LAGRANGE, HGDT_ERR_INVALID_ARGUMENT,
             "unknown space kind");
-    if (order != 1)
-      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "device spaces are provided for polynomial order 1 (P1/Q1)");
+    if (order != 1 && order != 2)
+      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "device spaces are provided for polynomial order 1 and 2");
+    if (order == 2 && grid->g->ref.geom == HGDT_HEX)
+      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "order 2 on hexahedra (27 local DoFs) is not provided on the device");
     auto s = std::make_shared<Space>();
     s->grid = grid->g;
     s->kind = kind, s->order = order;
     const Grid& g = *s->grid;
-    s->local_size = g.ref.nverts;
+    const auto nodes = lagrange_node_vertices(g.ref, order);
+    s->local_size = int(nodes.size());
     const int ls = s->local_size;
     s->dofs.resize(g.ne * ls);
-    if (kind == HGDT_CONTINUOUS_LAGRANGE) {
+    if (kind == HGDT_CONTINUOUS_LAGRANGE && order == 1) {
       // P1/Q1 Lagrange nodes are the element vertices in reference order: global DoF = vertex index
       s->dofs = g.conn;
       s->size = g.nv;
       s->inst_dof = g.inst_vert;
+    } else if (kind == HGDT_CONTINUOUS_LAGRANGE) {
+      // one DoF per subentity (order 2 has one node per subentity), keyed by the sorted global vertices
+      // of the subentity and numbered in order of first appearance in the element walk
+      // (ContinuousMapper, spaces/mapper/continuous.hh:139-182, up to the order of its index sets)
+      struct KeyHash {
+        size_t operator()(const std::array<int32_t, 8>& k) const
+        {
+          uint64_t h = 1469598103934665603ull;
+          for (int32_t v : k)
+            h = (h ^ uint32_t(v)) * 1099511628211ull;
+          return size_t(h);
+        }
+      };
+      std::unordered_map<std::array<int32_t, 8>, int32_t, KeyHash> ids;
+      s->inst_dof.assign(1, 0);
+      for (int in = 0; in < g.ninst(); ++in) {
+        for (int64_t e = g.inst_elem[in]; e < g.inst_elem[in + 1]; ++e)
+          for (int i = 0; i < ls; ++i) {
+            std::array<int32_t, 8> key;
+            key.fill(-1);
+            int c = 0;
+            for (int lv : nodes[i])
+              key[c++] = g.conn[e * g.ref.nverts + lv];
+            std::sort(key.begin(), key.begin() + c);
+            auto it = ids.find(key);
+            if (it == ids.end())
+              it = ids.emplace(key, int32_t(ids.size())).first;
+            s->dofs[e * ls + i] = it->second;
+          }
+        s->inst_dof.push_back(int64_t(ids.size()));
+      }
+      s->size = int64_t(ids.size());
     } else {
       // DiscontinuousMapper: element offset + local index (spaces/mapper/discontinuous.hh:104-109)
       parallel_for(g.ne, [&](int64_t b, int64_t e) {

@@ -215,6 +215,84 @@ __device__ __forceinline__ void shape(const double* xh, double* phi, double (*dp
   }
 }
 
+// Lagrange basis of order 1 or 2 in the local node order of lagrange_node_vertices() (setup.cpp),
+// LagrangeLocalFiniteElement<EquidistantPointSet> as created by local/finite-elements/lagrange.hh:141
+template <int GEOM, int ORDER>
+struct Basis {
+  static constexpr int D = Traits<GEOM>::D;
+  static constexpr int ND = (ORDER == 1) ? Traits<GEOM>::NV
+                            : Traits<GEOM>::SIMPLEX ? (D + 1) * (D + 2) / 2
+                                                    : (D == 2 ? 9 : 27);
+};
+
+// 1d quadratic Lagrange polynomials on the nodes 0, 1/2, 1 and their derivatives
+__device__ __forceinline__ double lag2(int m, double t)
+{
+  return m == 0 ? 2. * (t - 0.5) * (t - 1.) : m == 1 ? -4. * t * (t - 1.) : 2. * t * (t - 0.5);
+}
+__device__ __forceinline__ double dlag2(int m, double t)
+{
+  return m == 0 ? 4. * t - 3. : m == 1 ? 4. - 8. * t : 4. * t - 1.;
+}
+
+template <int GEOM, int ORDER>
+__device__ __forceinline__ void basis(const double* xh, double* phi, double (*dphi)[Traits<GEOM>::D])
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
+  if constexpr (ORDER == 1) {
+    shape<GEOM>(xh, phi, dphi);
+  } else if constexpr (T::SIMPLEX) {
+    // barycentric coordinates: l_0 = 1 - sum x, l_{a+1} = x_a; vertices l (2l - 1), edges 4 l_a l_b
+    double lam[D + 1], gl[D + 1][D];
+    lam[0] = 1.;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      lam[0] -= xh[a], lam[a + 1] = xh[a];
+#pragma unroll
+    for (int v = 0; v <= D; ++v)
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        gl[v][a] = (v == 0) ? -1. : (v == a + 1 ? 1. : 0.);
+#pragma unroll
+    for (int v = 0; v <= D; ++v) {
+      phi[v] = lam[v] * (2. * lam[v] - 1.);
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        dphi[v][a] = (4. * lam[v] - 1.) * gl[v][a];
+    }
+    int t = D + 1;
+#pragma unroll
+    for (int va = 0; va <= D; ++va)
+#pragma unroll
+      for (int vb = va + 1; vb <= D; ++vb, ++t) {
+        phi[t] = 4. * lam[va] * lam[vb];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          dphi[t][a] = 4. * (lam[va] * gl[vb][a] + lam[vb] * gl[va][a]);
+      }
+  } else {
+#pragma unroll
+    for (int t = 0; t < ND; ++t) {
+      const int idx[3] = {t % 3, (t / 3) % 3, t / 9};
+      double v = 1.;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+        v *= lag2(idx[a], xh[a]);
+      phi[t] = v;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double g = dlag2(idx[a], xh[a]);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+          if (c != a)
+            g *= lag2(idx[c], xh[c]);
+        dphi[t][a] = g;
+      }
+    }
+  }
+}
+
 // physical gradient g = J^{-T} gh (spaces/basis/default.hh:152-175)
 template <int D>
 __device__ __forceinline__ void to_physical(const Geo<D>& g, const double* gh, double* out)
@@ -229,11 +307,11 @@ __device__ __forceinline__ void to_physical(const Geo<D>& g, const double* gh, d
   }
 }
 
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= P.ne)
     return;
@@ -250,7 +328,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
     for (int f = 0; f < P.nforms; ++f) {
       const double c = P.field[f] ? P.field[f][e] : P.value[f];
       const double s = P.scaling[f];
-      if (P.kind[f] == EF_LAPLACE && T::SIMPLEX) {
+      if (P.kind[f] == EF_LAPLACE && T::SIMPLEX && ORDER == 1) {
         // P1 gradients are constant: the integrand is constant on the element and any
         // rule of the order chosen by integrals.hh:115 reduces to one evaluation
         double phi[ND], dphi[ND][D], G[ND][D];
@@ -283,7 +361,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
           xh[a] = qt[R.offset * D + q * D + a];
         const double w = qt[R.offset * D + R.npts * D + q];
         double phi[ND], dphi[ND][D];
-        shape<GEOM>(xh, phi, dphi);
+        basis<GEOM, ORDER>(xh, phi, dphi);
         const double factor = g.det * w;
         if (P.kind[f] == EF_LAPLACE) {
           double G[ND][D];
@@ -334,7 +412,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
           xh[a] = qt[R.offset * D + q * D + a];
         const double w = qt[R.offset * D + R.npts * D + q];
         double phi[ND], dphi[ND][D];
-        shape<GEOM>(xh, phi, dphi);
+        basis<GEOM, ORDER>(xh, phi, dphi);
         double fv;
         if (P.src_kind[sidx] == HGDT_SOURCE_CONSTANT)
           fv = P.src_value[sidx];
@@ -679,7 +757,7 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
 // weight 1) and out[ne + e] = int (kappa grad e) . grad e (Laplace integrand), with g given by its values
 // and physical gradients at the points of rule R (operators/bilinear-form.hh:344-452,
 // local/bilinear-forms/integrals.hh:97-134)
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* __restrict__ coords,
                                                      const int32_t* __restrict__ conn,
                                                      const int32_t* __restrict__ dofs,
@@ -691,7 +769,7 @@ __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* _
                                                      double* __restrict__ out)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= ne)
     return;
@@ -710,7 +788,7 @@ __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* _
       xh[a] = qtab[R.offset * D + q * D + a];
     const double w = qtab[R.offset * D + R.npts * D + q];
     double phi[ND], dphi[ND][D];
-    shape<GEOM>(xh, phi, dphi);
+    basis<GEOM, ORDER>(xh, phi, dphi);
     double ev = 0., eg[D];
 #pragma unroll
     for (int a = 0; a < D; ++a)
@@ -795,6 +873,37 @@ struct TableCache {
   DeviceArray<double> elem, line;
 };
 
+// (geometry, order) pairs with device kernels: order 1 everywhere, order 2 on quadrilaterals,
+// triangles and tetrahedra
+template <class F>
+void dispatch_fe(int geom, int order, F&& f)
+{
+  using I = std::integral_constant<int, 1>;
+  using II = std::integral_constant<int, 2>;
+  if (order == 1) {
+    switch (geom) {
+      case HGDT_QUAD:
+        return f(std::integral_constant<int, HGDT_QUAD>{}, I{});
+      case HGDT_TRI:
+        return f(std::integral_constant<int, HGDT_TRI>{}, I{});
+      case HGDT_HEX:
+        return f(std::integral_constant<int, HGDT_HEX>{}, I{});
+      case HGDT_TET:
+        return f(std::integral_constant<int, HGDT_TET>{}, I{});
+    }
+  } else if (order == 2) {
+    switch (geom) {
+      case HGDT_QUAD:
+        return f(std::integral_constant<int, HGDT_QUAD>{}, II{});
+      case HGDT_TRI:
+        return f(std::integral_constant<int, HGDT_TRI>{}, II{});
+      case HGDT_TET:
+        return f(std::integral_constant<int, HGDT_TET>{}, II{});
+    }
+  }
+  HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "no device kernels for this geometry and polynomial order");
+}
+
 template <class F>
 void dispatch_geom(int geom, F&& f)
 {
@@ -868,6 +977,8 @@ void Operator::prepare_assembly()
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many local forms appended");
   if ((!cforms.empty() || !bforms.empty()) && space->grid->ref.dim != 2)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
+  if ((!cforms.empty() || !bforms.empty()) && space->order != 1)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for order 1 spaces");
   if (tables_ready)
     return;
   QTable etab, ltab;
@@ -897,8 +1008,9 @@ void launch_assembly(Operator& op, hipStream_t stream)
     P.elem_base = M.elem_base;
     P.rhs_base = M.staging_size - (op.sources.empty() ? 0 : g.ne * op.space->local_size);
     P.write_matrix = op.eforms.empty() ? 0 : 1;
-    dispatch_geom(g.ref.geom, [&](auto G) {
-      hipLaunchKernelGGL(k_element_local<decltype(G)::value>, dim3(grid_for(g.ne, eb)), dim3(eb), 0, stream, P);
+    dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)), dim3(eb),
+                         0, stream, P);
     });
     HGDT_HIP(hipGetLastError());
   }
@@ -948,6 +1060,8 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "form_index does not name an appended element form");
   if (F->kind != EF_LAPLACE)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "the adjoint gradient is defined for a Laplace form");
+  if (space->order != 1)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel is provided for order 1 spaces");
   prepare_assembly();
   QTable etab, ltab;
   ElementParams P{};
@@ -988,8 +1102,9 @@ void error_norms(const Space& s, const double* dofs, int order, const double* gv
   d_vals.allocate(std::max<int64_t>(2 * g.ne, 1));
   d_part.allocate(std::max<int64_t>(2 * nchunks, 1));
   if (g.ne > 0) {
-    dispatch_geom(g.ref.geom, [&](auto G) {
-      hipLaunchKernelGGL(k_error_local<decltype(G)::value>, dim3(grid_for(g.ne, 256)), dim3(256), 0, stream, g.ne,
+    dispatch_fe(g.ref.geom, s.order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_error_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, 256)), dim3(256),
+                         0, stream, g.ne,
                          g.d_coords.ptr, g.d_conn.ptr, s.d_dofs.ptr, d_tab.ptr, R, dofs, gval, ggrad, kappa,
                          kappa_value, d_vals.ptr);
     });

@@ -88,6 +88,11 @@ struct RefElement {
   double volume = 1.; // reference element measure
 };
 RefElement reference_element(int geom);
+// Lagrange nodes of order 1 or 2 on a reference element, in the local DoF order of the shape
+// functions: for every node the reference vertices spanning its subentity.  Cubes: tensor index
+// t = i0 + (k+1) i1 [+ (k+1)^2 i2], node at i/k; simplices: the vertices, then (order 2) the edge
+// midpoints (a, b), a < b, in lexicographic order.
+std::vector<std::vector<int>> lagrange_node_vertices(const RefElement& ref, int order);
 
 // Quadrature rule on a reference element or on [0,1]
 struct Quadrature {

@@ -96,6 +96,39 @@ RefElement reference_element(int geom)
   return r;
 }
 
+std::vector<std::vector<int>> lagrange_node_vertices(const RefElement& ref, int order)
+{
+  std::vector<std::vector<int>> nodes;
+  const int d = ref.dim;
+  if (!ref.simplex) {
+    const int n1 = order + 1;
+    const int count = (d == 2) ? n1 * n1 : n1 * n1 * n1;
+    for (int t = 0; t < count; ++t) {
+      const int idx[3] = {t % n1, (t / n1) % n1, t / (n1 * n1)};
+      std::vector<int> sub;
+      for (int b = 0; b < ref.nverts; ++b) {
+        bool ok = true;
+        for (int a = 0; a < d; ++a) {
+          const int bit = (b >> a) & 1;
+          if ((idx[a] == 0 && bit != 0) || (idx[a] == order && bit != 1))
+            ok = false;
+        }
+        if (ok)
+          sub.push_back(b);
+      }
+      nodes.push_back(sub);
+    }
+  } else {
+    for (int v = 0; v < ref.nverts; ++v)
+      nodes.push_back({v});
+    if (order == 2)
+      for (int a = 0; a < ref.nverts; ++a)
+        for (int b = a + 1; b < ref.nverts; ++b)
+          nodes.push_back({a, b});
+  }
+  return nodes;
+}
+
 // Gauss-Legendre points on [0,1] (Newton on the three-term recurrence), ascending
 void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w)
 {
@@ -569,12 +602,23 @@ void Operator::build_dirichlet()
   const int ls = s.local_size, nf = g.ref.nfaces, fk = g.ref.face_nverts;
   std::vector<uint8_t> mask(n, 0);
   if (s.kind == HGDT_CONTINUOUS_LAGRANGE) {
-    // order 1: local DoF i sits at reference vertex i
+    // local DoFs whose subentity lies in a boundary face (the face itself and its lower-dimensional
+    // subentities)
+    const auto nodes = lagrange_node_vertices(g.ref, s.order);
+    std::vector<std::vector<int>> face_nodes(nf);
+    for (int f = 0; f < nf; ++f)
+      for (int i = 0; i < ls; ++i) {
+        bool inside = true;
+        for (int v : nodes[i])
+          inside = inside && std::find(g.ref.faces[f], g.ref.faces[f] + fk, v) != g.ref.faces[f] + fk;
+        if (inside)
+          face_nodes[f].push_back(i);
+      }
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f)
         if (g.nb[e * nf + f] < 0)
-          for (int a = 0; a < fk; ++a)
-            mask[s.dofs[e * ls + g.ref.faces[f][a]]] = 1;
+          for (int i : face_nodes[f])
+            mask[s.dofs[e * ls + i]] = 1;
   }
   dirichlet_dofs.clear();
   for (int64_t i = 0; i < n; ++i)
