@@ -1,0 +1,189 @@
+"""Edge cases of the HIP path: degenerate sizes, general (non axis-aligned, unstructured) meshes,
+ragged batches and invalid input.
+
+Bar as in test_gpu_parity: pattern bit-exact, matrix within 1e-12 (row-relative), solution within
+1e-10 relative L2 of the oracle's Jacobi-PCG; invalid input raises the exception family the reference
+raises, never a crash.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+from problems import matrix_error
+
+pytestmark = pytest.mark.gpu
+
+GEO = {"quad": O.QUAD, "tri": O.TRI, "hex": O.HEX, "tet": O.TET}
+
+
+def _heat(H, grid, kappa=1.0, f=1.0, order=1, dirichlet=True):
+    space = H.ContinuousLagrangeSpace(grid, order)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(kappa)))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(f)))
+    op.append(rhs)
+    if dirichlet:
+        op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.assemble()
+    return space, op
+
+
+def _oracle_heat(mesh, kappa=1.0, f=1.0, order=1, dirichlet=True):
+    s = O.Space(mesh, True, order)
+    op = O.Operator(s, O.STENCIL_ELEMENT)
+    op.append_laplace(kappa)
+    op.append_source(f)
+    op.set_dirichlet(dirichlet)
+    op.assemble(1)
+    return s, op
+
+
+def _compare(op, oop, label):
+    rp, cols = op.pattern()
+    orp, ocols = oop.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols), label
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), orp)
+    assert err <= 1e-12, (label, err)
+    ob = oop.rhs()
+    if np.abs(ob).max() > 0:
+        assert np.abs(op.rhs.cpu().numpy() - ob).max() <= 1e-12 * np.abs(ob).max(), label
+    xo, ito, relo, rc = O.pcg_jacobi(orp, ocols, oop.values(), ob, rtol=1e-10)
+    x = op.apply_inverse("rhs").cpu().numpy()
+    nrm = np.linalg.norm(xo)
+    rel = np.linalg.norm(x - xo) / nrm if nrm > 0 else np.linalg.norm(x)
+    print(f"  {label}: iters {op.last_solve['iterations']} (oracle {ito}) rel {rel:.2e}")
+    assert rel <= 1e-10 and abs(op.last_solve["iterations"] - ito) <= 2, label
+
+
+@pytest.mark.parametrize("geometry", ["quad", "tri", "hex", "tet"])
+def test_single_cell_all_dirichlet(hgdt_mod, geometry):
+    """one cube (split into simplices): every DoF lies on the boundary, the matrix is the identity, x = 0"""
+    H = hgdt_mod
+    print(f"case single cell {geometry}", flush=True)
+    dim = 2 if geometry in ("quad", "tri") else 3
+    grid = H.Grid.structured(geometry, [0] * dim, [1] * dim, [1] * dim)
+    space, op = _heat(H, grid)
+    A = op.matrix.to_scipy().toarray()
+    assert np.array_equal(A, np.eye(space.num_DoFs))
+    assert np.array_equal(op.dirichlet_dofs(), np.arange(space.num_DoFs))
+    x = op.apply_inverse("rhs")
+    assert op.last_solve["iterations"] == 0 and torch.count_nonzero(x) == 0
+
+
+def test_zero_right_hand_side(hgdt_mod):
+    """b = 0: converged before the first iteration, x = 0 (relative residual reported 0)"""
+    H = hgdt_mod
+    print("case zero rhs", flush=True)
+    grid = H.Grid.structured("quad", [0, 0], [1, 1], [8, 8])
+    space, op = _heat(H, grid, f=0.0)
+    x = op.apply_inverse("rhs")
+    assert op.last_solve["iterations"] == 0 and op.last_solve["converged"] == 1
+    assert torch.count_nonzero(x) == 0
+
+
+def _affine_mesh(geometry, cells, A, shift):
+    """structured mesh mapped by x -> A x + shift (sheared, rotated: no axis-aligned cells)"""
+    m = O.Mesh.structured(GEO[geometry], cells, [0] * len(cells), [1] * len(cells))
+    X = m.coords() @ np.asarray(A).T + np.asarray(shift)
+    return X, m.conn()
+
+
+@pytest.mark.parametrize("geometry,cells", [("quad", [7, 5]), ("tri", [6, 6]), ("hex", [4, 3, 5]), ("tet", [4, 4, 3])])
+def test_sheared_explicit_mesh(hgdt_mod, geometry, cells):
+    """explicit grids from coordinates and connectivity, with parallelogram / parallelepiped cells"""
+    H = hgdt_mod
+    print(f"case sheared {geometry} {cells}", flush=True)
+    A = [[1.0, 0.35], [-0.2, 0.9]] if len(cells) == 2 else [[1.0, 0.3, -0.1], [0.1, 0.8, 0.25], [-0.2, 0.15, 1.1]]
+    X, conn = _affine_mesh(geometry, cells, A, [0.5] * len(cells))
+    grid = H.Grid.from_arrays(geometry, X, conn)
+    kappa = H.fields.lognormal(17, grid.num_elements)
+    space, op = _heat(H, grid, kappa=torch.from_numpy(kappa).cuda())
+    mesh = O.Mesh.explicit(GEO[geometry], X, conn)
+    s, oop = _oracle_heat(mesh, kappa=kappa)
+    _compare(op, oop, f"sheared {geometry}")
+
+
+@pytest.mark.parametrize("geometry,cells,order", [("tri", [9, 7], 1), ("tet", [5, 4, 4], 1), ("tri", [6, 5], 2)])
+def test_unstructured_simplices(hgdt_mod, geometry, cells, order):
+    """interior vertices moved at random (a general simplex mesh, no constant stencil offsets): the
+    operator falls back to SELL storage"""
+    H = hgdt_mod
+    print(f"case perturbed {geometry} {cells} order {order}", flush=True)
+    m = O.Mesh.structured(GEO[geometry], cells, [0] * len(cells), [1] * len(cells))
+    X = m.coords().copy()
+    h = 1.0 / max(cells)
+    interior = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    rng = np.random.default_rng(23)
+    X[interior] += rng.uniform(-0.15 * h, 0.15 * h, size=(interior.sum(), X.shape[1]))
+    # randomise the vertex numbering too: the pattern is no longer banded
+    perm = rng.permutation(X.shape[0])
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    Xp, conn = X[perm], inv[m.conn()]
+    grid = H.Grid.from_arrays(geometry, Xp, conn)
+    kappa = H.fields.lognormal(29, grid.num_elements)
+    space, op = _heat(H, grid, kappa=torch.from_numpy(kappa).cuda(), order=order)
+    mesh = O.Mesh.explicit(GEO[geometry], Xp, conn)
+    s, oop = _oracle_heat(mesh, kappa=kappa, order=order)
+    _compare(op, oop, f"perturbed {geometry} order {order}")
+    assert op.format_info()["format"] == "sell64"
+
+
+def test_ragged_batch_with_tiny_instances(hgdt_mod):
+    """instances of 1, 2, 3 and 40 cells per direction in one union: each solved to its own tolerance,
+    equal bit for bit to its plain solve"""
+    H = hgdt_mod
+    print("case ragged batch", flush=True)
+    sizes = [1, 40, 2, 3]
+    parts = [H.Grid.structured("quad", [0, 0], [1, 1], [n, n]) for n in sizes]
+    union = H.Grid.union(parts)
+    kappa = H.fields.lognormal(31, union.num_elements)
+    space, op = _heat(H, union, kappa=torch.from_numpy(kappa).cuda())
+    xb = op.apply_inverse("rhs").cpu().numpy()
+    its, rels = op.instance_stats()
+    eoff, doff = union.instance_elements(), space.instance_dofs()
+    assert its[0] == 0  # the single cell is all Dirichlet
+    for i, n in enumerate(sizes):
+        s1, op1 = _heat(H, parts[i], kappa=torch.from_numpy(kappa[eoff[i]:eoff[i + 1]]).cuda())
+        for k in (3, 4):
+            op1.set_launch(k, *op.get_launch(k))
+        x1 = op1.apply_inverse("rhs").cpu().numpy()
+        assert op1.last_solve["iterations"] == its[i]
+        assert np.array_equal(x1, xb[doff[i]:doff[i + 1]])
+
+
+def test_invalid_input_raises(hgdt_mod):
+    H = hgdt_mod
+    print("case invalid input", flush=True)
+    grid = H.Grid.structured("quad", [0, 0], [1, 1], [4, 4])
+    space = H.ContinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    # a coefficient field of the wrong length
+    with pytest.raises(H.shapes_do_not_match):
+        op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.ones(7, dtype=torch.float64))))
+    # non-affine quadrilateral
+    X = np.array([[0, 0], [1, 0], [0, 1], [1.3, 1.2]], dtype=np.float64)
+    with pytest.raises(H.not_supported):
+        H.Grid.from_arrays("quad", X, np.array([[0, 1, 2, 3]], dtype=np.int32))
+    # connectivity out of range
+    with pytest.raises(H.wrong_input_given):
+        H.Grid.from_arrays("tri", X[:3], np.array([[0, 1, 5]], dtype=np.int32))
+    # order 3 spaces and order 2 on hexahedra are not provided
+    with pytest.raises(H.not_supported):
+        H.ContinuousLagrangeSpace(grid, 3)
+    with pytest.raises(H.not_supported):
+        H.ContinuousLagrangeSpace(H.Grid.structured("hex", [0, 0, 0], [1, 1, 1], [2, 2, 2]), 2)
+    # intersection integrals on an order-2 space
+    dg2 = H.DiscontinuousLagrangeSpace(grid, 2)
+    op2 = H.make_matrix_operator(dg2, H.Stencil.element_and_intersection)
+    op2.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(8.0, 1.0)), {},
+               H.ApplyOn.InnerIntersectionsOnce())
+    with pytest.raises(H.not_supported):
+        op2.assemble()
+    # launch shapes outside the kernels' bounds
+    with pytest.raises(H.wrong_input_given):
+        op.set_launch(H._lib.KERNEL_APPLY, 96, 1)
+    with pytest.raises(H.wrong_input_given):
+        op.set_launch(H._lib.KERNEL_ELEMENT, 512, 1)
