@@ -337,6 +337,34 @@ hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_
   });
 }
 
+hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points)
+{
+  return guarded([&] {
+    require(grid && num_points && order >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    const Grid& g = *grid->g;
+    require(g.ref.dim == 2, HGDT_ERR_NOT_SUPPORTED, "boundary quadrature points are provided for 2d grids");
+    const Quadrature q = line_rule(order);
+    const int nq = q.size(), nf = g.ref.nfaces;
+    *num_points = nq;
+    if (!host_out)
+      return;
+    int64_t bi = 0;
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int f = 0; f < nf; ++f) {
+        if (g.nb[e * nf + f] >= 0)
+          continue;
+        const int32_t va = g.conn[e * g.ref.nverts + g.ref.faces[f][0]];
+        const int32_t vb = g.conn[e * g.ref.nverts + g.ref.faces[f][1]];
+        for (int p = 0; p < nq; ++p)
+          for (int a = 0; a < 2; ++a) {
+            const double pa = g.coords[int64_t(va) * 2 + a], pb = g.coords[int64_t(vb) * 2 + a];
+            host_out[(bi * nq + p) * 2 + a] = pa + q.x[p] * (pb - pa);
+          }
+        ++bi;
+      }
+  });
+}
+
 hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space)
 {
   return guarded([&] {
@@ -573,6 +601,20 @@ hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double
     require(kind == HGDT_SOURCE_CONSTANT || data, HGDT_ERR_INVALID_ARGUMENT, "source data is NULL");
     op->op->sources.push_back({kind, data, value, f_order, over_integrate});
     op->op->maps.built = false; // staging layout gains the right-hand side block
+    forms_changed(*op->op);
+  });
+}
+
+hgdt_status hgdt_operator_append_boundary_source(hgdt_operator op, int kind, double param, const double* coeff_field,
+                                                 double coeff_value, const double* data, double value, int32_t g_order,
+                                                 int32_t over_integrate)
+{
+  return guarded([&] {
+    require(op && g_order >= 0 && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    require(kind >= HGDT_BOUNDARY_DIRICHLET_COUPLING && kind <= HGDT_BOUNDARY_PRODUCT, HGDT_ERR_INVALID_ARGUMENT,
+            "unknown boundary functional kind");
+    op->op->bsources.push_back({kind, param, coeff_field, coeff_value, data, value, g_order, over_integrate});
+    op->op->maps.built = false; // staging layout gains the boundary right-hand side block
     forms_changed(*op->op);
   });
 }

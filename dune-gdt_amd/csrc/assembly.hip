@@ -29,6 +29,7 @@ namespace {
 constexpr int MAX_EFORMS = 8;
 constexpr int MAX_FFORMS = 8;
 constexpr int MAX_SOURCES = 4;
+constexpr int MAX_BSOURCES = 4;
 
 struct QRule {
   int offset; // into the point / weight table
@@ -73,10 +74,30 @@ struct FaceParams {
   QRule rule[MAX_FFORMS];
   double* staging;
   int64_t base;
+  QRule brule[MAX_BSOURCES]; // line rules of the boundary right-hand side terms
   // every appended form is symmetric (symmetry prefactor 1 or a penalty): the blocks are stored
   // exactly symmetric (in_in, out_out and boundary blocks from their upper triangle, out_in as the
   // transpose of in_out), so the assembled matrix is symmetric bit for bit
   int symmetric;
+};
+
+// right-hand side terms on the boundary intersections
+struct BoundaryRhsParams {
+  int64_t nfaces;
+  const int32_t* faces; // 2 ints per boundary face: element, local face
+  const double* coords;
+  const int32_t* conn;
+  const double* qtab; // line rules
+  int nterms;
+  int kind[MAX_BSOURCES];
+  double param[MAX_BSOURCES];
+  const double* coeff[MAX_BSOURCES];
+  double coeff_value[MAX_BSOURCES];
+  const double* data[MAX_BSOURCES];
+  double value[MAX_BSOURCES];
+  QRule rule[MAX_BSOURCES];
+  double* staging;
+  int64_t base;
 };
 
 template <int GEOM>
@@ -634,6 +655,63 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
       out[i * ND + j] = (P.symmetric && i > j) ? B[j][i] : B[i][j];
 }
 
+// boundary functionals (LocalIntersectionIntegralFunctional::apply, local/functionals/integrals.hh:157-185),
+// one boundary intersection per lane, all appended terms summed into the face's local vector:
+//   DirichletCoupling as a unary integrand: -s g (kappa grad psi_i) . n   (laplace-ipdg.hh:300-320)
+//   BoundaryPenalty(sigma, w).with_ansatz(g): sigma (n . w n) / h_F g psi_i (ipdg.hh:254-282)
+//   LocalIntersectionProductIntegrand(w).with_ansatz(g): (w g) psi_i     (product.hh:355-377)
+template <int GEOM>
+__global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
+{
+  using T = Traits<GEOM>;
+  constexpr int ND = T::ND;
+  const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (fi >= P.nfaces)
+    return;
+  const int64_t e = P.faces[2 * fi + 0];
+  const int f = P.faces[2 * fi + 1];
+  Geo<2> gin;
+  load_geometry<GEOM>(P.coords, P.conn, e, gin);
+  double Pa[2], tau[2], n[2], len;
+  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
+  double b[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    b[i] = 0.;
+  for (int tm = 0; tm < P.nterms; ++tm) {
+    const double c = P.coeff[tm] ? P.coeff[tm][e] : P.coeff_value[tm];
+    const double prm = P.param[tm];
+    const QRule R = P.rule[tm];
+    for (int q = 0; q < R.npts; ++q) {
+      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
+      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
+      double pin[ND], Gin[ND][2];
+      eval_at<GEOM>(gin, X, pin, Gin);
+      const double g = P.data[tm] ? P.data[tm][fi * R.npts + q] : P.value[tm];
+      const double factor = len * wq;
+      if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING) {
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          b[i] += (-1.0 * prm * g * ((c * Gin[i][0]) * n[0] + (c * Gin[i][1]) * n[1])) * factor;
+      } else if (P.kind[tm] == HGDT_BOUNDARY_PENALTY) {
+        const double nwn = n[0] * (c * n[0]) + n[1] * (c * n[1]);
+        const double pen = (prm * nwn) / len;
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          b[i] += (pen * g * pin[i]) * factor;
+      } else {
+#pragma unroll
+        for (int i = 0; i < ND; ++i)
+          b[i] += ((c * g) * pin[i]) * factor;
+      }
+    }
+  }
+  double* __restrict__ out = P.staging + P.base + fi * ND;
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+    out[i] = b[i];
+}
+
 // fixed-order gather of the staged local contributions into the CSR values
 __global__ void __launch_bounds__(256) k_gather_values(int64_t nnz, const uint32_t* __restrict__ slot_ptr,
                                                        const uint32_t* __restrict__ contrib,
@@ -960,6 +1038,8 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, Element
   };
   fill_face(op.cforms, C);
   fill_face(op.bforms, B);
+  for (size_t t = 0; t < op.bsources.size() && t < size_t(MAX_BSOURCES); ++t)
+    B.brule[t] = ltab.add(line_rule(op.boundary_source_order(op.bsources[t])));
   // the SWIPDG terms are symmetric for symmetry prefactor 1 (laplace-ipdg.hh:107-186,340-368); the
   // penalties always are (ipdg.hh:113-171,254-282)
   bool sym = true;
@@ -979,6 +1059,10 @@ void Operator::prepare_assembly()
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
   if ((!cforms.empty() || !bforms.empty()) && space->order != 1)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for order 1 spaces");
+  if (bsources.size() > size_t(MAX_BSOURCES))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
+  if (!bsources.empty() && (space->grid->ref.dim != 2 || space->order != 1))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "boundary functionals are provided for order 1 spaces on 2d grids");
   if (tables_ready)
     return;
   QTable etab, ltab;
@@ -1006,7 +1090,7 @@ void launch_assembly(Operator& op, hipStream_t stream)
     P.qtab = op.d_qtab_elem.ptr;
     P.staging = op.d_staging.ptr;
     P.elem_base = M.elem_base;
-    P.rhs_base = M.staging_size - (op.sources.empty() ? 0 : g.ne * op.space->local_size);
+    P.rhs_base = M.rhs_base;
     P.write_matrix = op.eforms.empty() ? 0 : 1;
     dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
       hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)), dim3(eb),
@@ -1039,7 +1123,23 @@ void launch_assembly(Operator& op, hipStream_t stream)
                        op.d_values.ptr);
     HGDT_HIP(hipGetLastError());
   }
-  if (!op.sources.empty()) {
+  if (!op.bsources.empty() && nbd > 0) {
+    BoundaryRhsParams BR{};
+    BR.nfaces = nbd, BR.faces = M.d_bnd.ptr, BR.coords = g.d_coords.ptr, BR.conn = g.d_conn.ptr;
+    BR.qtab = op.d_qtab_line.ptr, BR.staging = op.d_staging.ptr, BR.base = M.brhs_base;
+    BR.nterms = int(op.bsources.size());
+    for (int t = 0; t < BR.nterms; ++t) {
+      const auto& S = op.bsources[t];
+      BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
+      BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t];
+    }
+    if (g.ref.geom == HGDT_QUAD)
+      hipLaunchKernelGGL(k_boundary_rhs<HGDT_QUAD>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+    else
+      hipLaunchKernelGGL(k_boundary_rhs<HGDT_TRI>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+    HGDT_HIP(hipGetLastError());
+  }
+  if (!op.sources.empty() || !op.bsources.empty()) {
     hipLaunchKernelGGL(k_gather_rhs, dim3(grid_for(op.n, gb)), dim3(gb), 0, stream, op.n, M.d_rhs_ptr.ptr,
                        M.d_rhs_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_dof_mask.ptr : nullptr,
                        op.d_rhs.ptr);

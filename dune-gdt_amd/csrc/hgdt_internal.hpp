@@ -168,6 +168,18 @@ struct SourceDesc {
   int over_integrate;
 };
 
+// right-hand side term on the boundary intersections (LocalIntersectionIntegralFunctional)
+struct BoundarySourceDesc {
+  int kind;            // HGDT_BOUNDARY_*
+  double param;        // symmetry prefactor or penalty
+  const double* coeff; // diffusion (coupling) or weight, per element; NULL -> coeff_value
+  double coeff_value;
+  const double* data;  // g at the face quadrature points [boundary face][point]; NULL -> value
+  double value;
+  int g_order;
+  int over_integrate;
+};
+
 // launch configuration of one tunable kernel
 struct Launch {
   int block = 256;
@@ -179,7 +191,7 @@ struct Launch {
 // its contributions, as indices into the staging buffer of local matrices, in the
 // order the reference's grid walk would add them.
 struct ScatterMaps {
-  int64_t elem_base = 0, inner_base = 0, bnd_base = 0, staging_size = 0;
+  int64_t elem_base = 0, inner_base = 0, bnd_base = 0, rhs_base = 0, brhs_base = 0, staging_size = 0;
   std::vector<int32_t> inner_in, inner_fin, inner_out, inner_fout; // inner intersections visited once
   std::vector<int32_t> bnd_e, bnd_f;                               // boundary intersections
   DeviceArray<uint32_t> d_slot_ptr, d_contrib;
@@ -258,6 +270,16 @@ struct Operator {
   std::vector<ElementFormDesc> eforms;
   std::vector<FaceFormDesc> cforms, bforms;
   std::vector<SourceDesc> sources;
+  std::vector<BoundarySourceDesc> bsources;
+  // quadrature order of a boundary term (laplace-ipdg.hh:292-297; conversion.hh:194-197 with ipdg.hh:245-250
+  // and product.hh:336-341)
+  int boundary_source_order(const BoundarySourceDesc& S) const
+  {
+    const int k = space->order;
+    if (S.kind == HGDT_BOUNDARY_DIRICHLET_COUPLING)
+      return S.g_order + 0 + std::max(k - 1, 0) + S.over_integrate;
+    return 0 + k + S.g_order + S.over_integrate;
+  }
   bool dirichlet = false;
   std::vector<int32_t> dirichlet_dofs;
   DeviceArray<uint8_t> d_slot_flag; // 0 keep, 1 zero, 2 one

@@ -440,7 +440,8 @@ void Operator::build_scatter_maps()
   const int64_t ne = g.ne, k2 = int64_t(ls) * ls;
   ScatterMaps& M = maps;
   const bool has_e = !eforms.empty(), has_c = !cforms.empty(), has_b = !bforms.empty();
-  const int signature = (has_e ? 1 : 0) | (has_c ? 2 : 0) | (has_b ? 4 : 0) | (!sources.empty() ? 8 : 0);
+  const bool has_s = !sources.empty(), has_bs = !bsources.empty();
+  const int signature = (has_e ? 1 : 0) | (has_c ? 2 : 0) | (has_b ? 4 : 0) | (has_s ? 8 : 0) | (has_bs ? 16 : 0);
   if (M.built && M.signature == signature)
     return;
   // intersections in walk order
@@ -470,7 +471,9 @@ void Operator::build_scatter_maps()
   M.inner_base = has_e ? ne * k2s : 0;
   M.bnd_base = M.inner_base + (has_c ? nin * 4 * k2 : 0);
   const int64_t rhs_base = M.bnd_base + (has_b ? nbd * k2 : 0);
-  M.staging_size = rhs_base + (sources.empty() ? 0 : ne * ls);
+  M.rhs_base = rhs_base;
+  M.brhs_base = rhs_base + (has_s ? ne * ls : 0); // boundary right-hand sides, ls per boundary face
+  M.staging_size = M.brhs_base + (has_bs ? nbd * ls : 0);
   if (M.staging_size >= (int64_t(1) << 32))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "local-matrix staging exceeds 2^32 entries");
 
@@ -564,14 +567,27 @@ void Operator::build_scatter_maps()
   slot_count.shrink_to_fit();
   M.d_slot_ptr.upload(slot_ptr);
   M.d_contrib.upload(contrib);
-  // right-hand side: element functional contributions per DoF in element order
-  if (!sources.empty()) {
-    std::vector<uint32_t> rptr(n + 1), rc(std::max<int64_t>(ne * ls, 1));
-    for (int64_t i = 0; i <= n; ++i)
-      rptr[i] = uint32_t(A.ptr[i]);
-    for (int64_t i = 0; i < n; ++i)
-      for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p)
-        rc[p] = uint32_t(rhs_base + int64_t(A.local[p]) * ne + A.elem[p]);
+  // right-hand side: per DoF in walk order, for every element holding it (ascending) the element
+  // functional's contribution, then those of the element's boundary intersections in face order
+  if (has_s || has_bs) {
+    std::vector<uint32_t> rptr(n + 1, 0), rc;
+    rc.reserve(ne * ls * (has_s ? 1 : 0) + (has_bs ? nbd * ls : 0) + 1);
+    for (int64_t i = 0; i < n; ++i) {
+      rptr[i] = uint32_t(rc.size());
+      for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+        const int64_t e = A.elem[p];
+        const int l = A.local[p];
+        if (has_s)
+          rc.push_back(uint32_t(rhs_base + int64_t(l) * ne + e));
+        if (has_bs)
+          for (int f = 0; f < nf; ++f)
+            if (bnd_index[e * nf + f] >= 0)
+              rc.push_back(uint32_t(M.brhs_base + int64_t(bnd_index[e * nf + f]) * ls + l));
+      }
+    }
+    rptr[n] = uint32_t(rc.size());
+    if (rc.empty())
+      rc.push_back(0);
     M.d_rhs_ptr.upload(rptr);
     M.d_rhs_contrib.upload(rc);
   }

@@ -91,6 +91,7 @@ CONTINUOUS_LAGRANGE, DISCONTINUOUS_LAGRANGE = 0, 1
 STENCIL_ELEMENT, STENCIL_INTERSECTION, STENCIL_ELEMENT_AND_INTERSECTION = 0, 1, 2
 SOURCE_CONSTANT, SOURCE_PER_ELEMENT, SOURCE_PER_QUADRATURE_POINT = 0, 1, 2
 FORMAT_AUTO, FORMAT_SELL, FORMAT_DIA, FORMAT_DIA_NONSYMMETRIC = 0, 1, 2, 3
+BOUNDARY_DIRICHLET_COUPLING, BOUNDARY_PENALTY, BOUNDARY_PRODUCT = 0, 1, 2
 
 KERNEL_ELEMENT, KERNEL_INTERSECTION, KERNEL_GATHER, KERNEL_APPLY, KERNEL_VECTOR = 0, 1, 2, 3, 4
 
@@ -149,6 +150,7 @@ SIGNATURES = {
     "hgdt_grid_copy_connectivity": (ctypes.c_int, [_vp, _pi32]),
     "hgdt_grid_instance_elements": (ctypes.c_int, [_vp, _pi64]),
     "hgdt_grid_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _pdbl, _pi32]),
+    "hgdt_grid_boundary_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _pdbl, _pi32]),
     "hgdt_space_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _pvp]),
     "hgdt_space_destroy": (ctypes.c_int, [_vp]),
     "hgdt_space_size": (ctypes.c_int, [_vp, _pi64, _pi32]),
@@ -166,6 +168,7 @@ SIGNATURES = {
     "hgdt_operator_append_dirichlet_coupling": (ctypes.c_int, [_vp, _dbl, _vp, _dbl, _i32, _dbl]),
     "hgdt_operator_append_boundary_penalty": (ctypes.c_int, [_vp, _dbl, _vp, _dbl, _i32, _dbl]),
     "hgdt_operator_append_source": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _dbl, _i32, _i32]),
+    "hgdt_operator_append_boundary_source": (ctypes.c_int, [_vp, ctypes.c_int, _dbl, _vp, _dbl, _vp, _dbl, _i32, _i32]),
     "hgdt_operator_set_dirichlet": (ctypes.c_int, [_vp, _i32]),
     "hgdt_operator_dirichlet_dofs": (ctypes.c_int, [_vp, _pi64, _pi32]),
     "hgdt_operator_assemble": (ctypes.c_int, [_vp, _vp]),

@@ -33,6 +33,7 @@ __all__ = [
     "make_vector_functional", "DirichletConstraints", "make_dirichlet_constraints", "AllDirichletBoundaryInfo",
     "ApplyOn", "Stencil", "MatrixOperator", "make_matrix_operator", "CsrMatrix", "GenericFunction",
     "DiscreteFunction", "make_discrete_function", "BilinearForm", "make_bilinear_form",
+    "LocalIntersectionIntegralFunctional", "LocalIntersectionProductIntegrand",
 ]
 
 GEOMETRIES = {"quad": L.QUAD, "cube": L.QUAD, "tri": L.TRI, "simplex": L.TRI, "hex": L.HEX, "tet": L.TET}
@@ -151,6 +152,15 @@ class Grid:
     def instance_elements(self):
         out = np.empty(self.num_instances + 1, np.int64)
         L.call("hgdt_grid_instance_elements", self._h, out.ctypes.data_as(L._pi64))
+        return out
+
+    def boundary_quadrature_points(self, order):
+        """Physical coordinates of the line-rule points on the boundary intersections in walk order: (nb, nq, dim)."""
+        nq = ctypes.c_int32()
+        L.call("hgdt_grid_boundary_quadrature_points", self._h, order, None, ctypes.byref(nq))
+        out = np.empty((self.num_boundary_intersections, nq.value, self.dimension))
+        L.call("hgdt_grid_boundary_quadrature_points", self._h, order, out.ctypes.data_as(L._pdbl),
+               ctypes.byref(nq))
         return out
 
     def quadrature_points(self, order):
@@ -356,10 +366,13 @@ class LocalIPDGInnerPenalty(_Integrand):
 
 
 class LocalLaplaceIPDGDirichletCoupling(_Integrand):
-    """LocalLaplaceIPDGIntegrands::DirichletCoupling, local/integrands/laplace-ipdg.hh:213-382"""
+    """LocalLaplaceIPDGIntegrands::DirichletCoupling, local/integrands/laplace-ipdg.hh:213-382: a binary
+    integrand in a LocalIntersectionIntegralBilinearForm; in a LocalIntersectionIntegralFunctional the
+    unary one with the Dirichlet data (a constant or a GenericFunction)"""
 
-    def __init__(self, symmetry_prefactor, diffusion, logging_prefix=""):
+    def __init__(self, symmetry_prefactor, diffusion, dirichlet_data=0.0, logging_prefix=""):
         self.symmetry_prefactor, self.diffusion = float(symmetry_prefactor), _gf(diffusion)
+        self.dirichlet_data = dirichlet_data
 
 
 class LocalIPDGBoundaryPenalty(_Integrand):
@@ -367,6 +380,26 @@ class LocalIPDGBoundaryPenalty(_Integrand):
 
     def __init__(self, penalty, weight=1.0, logging_prefix=""):
         self.penalty, self.weight = float(penalty), _gf(weight)
+
+    def with_ansatz(self, function):
+        """LocalBinaryToUnaryIntersectionIntegrand (local/integrands/conversion.hh:133-221)"""
+        return _BoundaryWithAnsatz(L.BOUNDARY_PENALTY, self.penalty, self.weight, function)
+
+
+class LocalIntersectionProductIntegrand(_Integrand):
+    """(weight psi_i) phi_j on intersections, local/integrands/product.hh:280-380 (inside element)"""
+
+    def __init__(self, weight=1.0, logging_prefix=""):
+        self.weight = _gf(weight)
+
+    def with_ansatz(self, function):
+        """LocalBinaryToUnaryIntersectionIntegrand (local/integrands/conversion.hh:133-221)"""
+        return _BoundaryWithAnsatz(L.BOUNDARY_PRODUCT, 0.0, self.weight, function)
+
+
+class _BoundaryWithAnsatz(_Integrand):
+    def __init__(self, kind, param, coeff, function):
+        self.kind, self.param, self.coeff, self.function = kind, float(param), coeff, function
 
 
 # ---------------------------------------------------------------------------
@@ -394,6 +427,17 @@ class LocalElementIntegralFunctional:
         self.integrand, self.over_integrate = integrand, int(over_integrate)
 
 
+class LocalIntersectionIntegralFunctional:
+    """local/functionals/integrals.hh:108-191 on boundary intersections: the integrand is
+    LocalLaplaceIPDGDirichletCoupling(s, diffusion, dirichlet_data), LocalIPDGBoundaryPenalty(...).with_ansatz(g)
+    or LocalIntersectionProductIntegrand(weight).with_ansatz(g)"""
+
+    def __init__(self, integrand, over_integrate=0, logging_prefix=""):
+        if not isinstance(integrand, (LocalLaplaceIPDGDirichletCoupling, _BoundaryWithAnsatz)):
+            raise L.not_supported(9, "intersection functionals take the Dirichlet coupling or a with_ansatz integrand")
+        self.integrand, self.over_integrate = integrand, int(over_integrate)
+
+
 class VectorFunctional:
     """VectorBasedFunctional (functionals/vector-based.hh:132-286), assembled in the operator's walk."""
 
@@ -402,7 +446,10 @@ class VectorFunctional:
         self.local_functionals = []
         self._op = None
 
-    def append(self, local_functional):
+    def append(self, local_functional, param=None, filter=None):
+        if isinstance(local_functional, LocalIntersectionIntegralFunctional) and filter is not None and not isinstance(
+                filter, ApplyOn.CustomBoundaryIntersections):
+            raise L.not_supported(9, "intersection functionals are applied on the (Dirichlet) boundary")
         self.local_functionals.append(local_functional)
         return self
 
@@ -561,7 +608,29 @@ class MatrixOperator:
         else:
             raise L.not_supported(9, f"{type(integrand).__name__} is not a boundary integrand of this path")
 
+    def _append_boundary_source(self, lf):
+        it, over = lf.integrand, lf.over_integrate
+        k = self.space.order
+        if isinstance(it, LocalLaplaceIPDGDirichletCoupling):
+            kind, param, coeff, g = L.BOUNDARY_DIRICHLET_COUPLING, it.symmetry_prefactor, it.diffusion, it.dirichlet_data
+        else:
+            kind, param, coeff, g = it.kind, it.param, it.coeff, it.function
+        g_order = g.order() if isinstance(g, GenericFunction) else 0
+        order = (g_order + max(k - 1, 0) + over) if kind == L.BOUNDARY_DIRICHLET_COUPLING else (k + g_order + over)
+        cp, cv = self._field(coeff)
+        if isinstance(g, GenericFunction):
+            pts = self.grid.boundary_quadrature_points(order)
+            vals = _dvec(np.asarray(g.evaluate(pts.reshape(-1, self.grid.dimension).T), np.float64).reshape(-1),
+                         name="boundary data")
+            self._keep.append(vals)
+            L.call("hgdt_operator_append_boundary_source", self._h, kind, param, cp, cv,
+                   ctypes.c_void_p(vals.data_ptr()), 0.0, g_order, over)
+        else:
+            L.call("hgdt_operator_append_boundary_source", self._h, kind, param, cp, cv, None, float(g), 0, over)
+
     def _append_source(self, lf):
+        if isinstance(lf, LocalIntersectionIntegralFunctional):
+            return self._append_boundary_source(lf)
         f = lf.integrand.function
         over = lf.over_integrate
         if isinstance(f, QuadraturePointFunction):

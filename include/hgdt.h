@@ -51,6 +51,8 @@ enum { HGDT_CONTINUOUS_LAGRANGE = 0, HGDT_DISCONTINUOUS_LAGRANGE = 1 };
 enum { HGDT_STENCIL_ELEMENT = 0, HGDT_STENCIL_INTERSECTION = 1, HGDT_STENCIL_ELEMENT_AND_INTERSECTION = 2 };
 /* source data kinds for the right-hand side functional */
 enum { HGDT_SOURCE_CONSTANT = 0, HGDT_SOURCE_PER_ELEMENT = 1, HGDT_SOURCE_PER_QUADRATURE_POINT = 2 };
+/* right-hand side terms on boundary intersections */
+enum { HGDT_BOUNDARY_DIRICHLET_COUPLING = 0, HGDT_BOUNDARY_PENALTY = 1, HGDT_BOUNDARY_PRODUCT = 2 };
 /* matrix storage used for apply / apply_inverse */
 enum { HGDT_FORMAT_AUTO = 0, HGDT_FORMAT_SELL = 1, HGDT_FORMAT_DIA = 2, HGDT_FORMAT_DIA_NONSYMMETRIC = 3 };
 
@@ -105,6 +107,11 @@ hgdt_status hgdt_grid_instance_elements(hgdt_grid grid, int64_t* host_out);
  * host_out[num_elements x num_points x dim]; *num_points receives the rule size.
  * Pass host_out = NULL to query the size only. */
 hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
+
+/* physical coordinates of the quadrature points (line rule of the given order) on the boundary
+ * intersections, in walk order (elements ascending, their faces in reference order):
+ * host_out[num_boundary_intersections x num_points x dim] (2d grids); host_out = NULL queries the size. */
+hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
 
 /* ------------------------------------------------------------------------- */
 /* spaces                                                                    */
@@ -199,6 +206,22 @@ hgdt_status hgdt_operator_append_boundary_penalty(hgdt_operator op, double penal
  * with order = 1 + f_order + over_integrate.  `data` is a device pointer that must stay valid. */
 hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double* data, double value,
                                         int32_t f_order, int32_t over_integrate);
+/* right-hand side terms on the boundary intersections, assembled in the same walk:
+ * VectorBasedFunctional::append(LocalIntersectionIntegralFunctional(integrand, over_integrate), {},
+ * CustomBoundaryIntersections(all)) (local/functionals/integrals.hh:108-191) with
+ *   HGDT_BOUNDARY_DIRICHLET_COUPLING: LocalLaplaceIPDGIntegrands::DirichletCoupling(param, coeff, g) as a unary
+ *     integrand, -param g (coeff grad psi_i) . n (local/integrands/laplace-ipdg.hh:237-320);
+ *   HGDT_BOUNDARY_PENALTY: LocalIPDGIntegrands::BoundaryPenalty(param, coeff).with_ansatz(g),
+ *     param (n . coeff n) / h_F g psi_i (local/integrands/ipdg.hh:201-282, conversion.hh:133-221);
+ *   HGDT_BOUNDARY_PRODUCT: LocalIntersectionProductIntegrand(coeff).with_ansatz(g), (coeff g) psi_i
+ *     (local/integrands/product.hh:280-380).
+ * coeff: per-element device field or coeff_value (NULL field).  g: device values at the points of
+ * hgdt_grid_boundary_quadrature_points(order) with order = g_order + max(k - 1, 0) + over_integrate for the
+ * coupling term and k + g_order + over_integrate for the others (k the space order), or the constant
+ * `value` (NULL data).  Device arrays must stay valid.  2d grids, order-1 spaces. */
+hgdt_status hgdt_operator_append_boundary_source(hgdt_operator op, int kind, double param, const double* coeff_field,
+                                                 double coeff_value, const double* data, double value, int32_t g_order,
+                                                 int32_t over_integrate);
 /* DirichletConstraints(space, AllDirichletBoundaryInfo) applied after assembly with
  * ensure_symmetry (tools/dirichlet-constraints.hh:85-184): unit rows and columns, rhs entry 0 */
 hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled);

@@ -347,6 +347,10 @@ public:
   {
     return order_;
   }
+  double evaluate(const double* x) const
+  {
+    return evaluate_(x);
+  }
   std::vector<double> values_at(const Grid& grid, int quadrature_order) const
   {
     int32_t nq = 0;
@@ -540,12 +544,22 @@ struct InnerCoupling : public IntersectionIntegrandSum
     terms.push_back({IntersectionTerm::inner_coupling, symmetry_prefactor, std::move(diffusion), std::move(weight)});
   }
 };
+// in a LocalIntersectionIntegralBilinearForm the binary integrand; in a LocalIntersectionIntegralFunctional
+// the unary one with the Dirichlet data (laplace-ipdg.hh:237-320)
 struct DirichletCoupling : public IntersectionIntegrandSum
 {
-  DirichletCoupling(double symmetry_prefactor, GridFunction diffusion)
+  DirichletCoupling(double symmetry_prefactor, GridFunction diffusion, double dirichlet_data = 0.)
+    : data_value(dirichlet_data)
   {
     terms.push_back({IntersectionTerm::dirichlet_coupling, symmetry_prefactor, std::move(diffusion), 1.});
   }
+  DirichletCoupling(double symmetry_prefactor, GridFunction diffusion, const GenericFunction& dirichlet_data)
+    : data_function(std::make_shared<GenericFunction>(dirichlet_data))
+  {
+    terms.push_back({IntersectionTerm::dirichlet_coupling, symmetry_prefactor, std::move(diffusion), 1.});
+  }
+  double data_value = 0.;
+  std::shared_ptr<GenericFunction> data_function;
 };
 } // namespace LocalLaplaceIPDGIntegrands
 
@@ -557,14 +571,58 @@ struct InnerPenalty : public IntersectionIntegrandSum
     terms.push_back({IntersectionTerm::inner_penalty, penalty, 1., std::move(weight)});
   }
 };
+struct BoundaryPenalty;
+} // namespace LocalIPDGIntegrands
+
+// a binary boundary integrand with a function as ansatz: LocalBinaryToUnaryIntersectionIntegrand
+// (local/integrands/conversion.hh:133-221)
+struct BoundaryWithAnsatz
+{
+  int kind; // HGDT_BOUNDARY_PENALTY or HGDT_BOUNDARY_PRODUCT
+  double param;
+  GridFunction coeff;
+  double data_value = 0.;
+  std::shared_ptr<GenericFunction> data_function;
+};
+
+namespace LocalIPDGIntegrands {
 struct BoundaryPenalty : public IntersectionIntegrandSum
 {
-  BoundaryPenalty(double penalty, GridFunction weight = 1.)
+  BoundaryPenalty(double penalty_, GridFunction weight_ = 1.)
+    : penalty(penalty_)
+    , weight(weight_)
   {
-    terms.push_back({IntersectionTerm::boundary_penalty, penalty, 1., std::move(weight)});
+    terms.push_back({IntersectionTerm::boundary_penalty, penalty_, 1., std::move(weight_)});
   }
+  BoundaryWithAnsatz with_ansatz(double g) const
+  {
+    return {HGDT_BOUNDARY_PENALTY, penalty, weight, g, nullptr};
+  }
+  BoundaryWithAnsatz with_ansatz(const GenericFunction& g) const
+  {
+    return {HGDT_BOUNDARY_PENALTY, penalty, weight, 0., std::make_shared<GenericFunction>(g)};
+  }
+  double penalty;
+  GridFunction weight;
 };
 } // namespace LocalIPDGIntegrands
+
+// (weight psi_i) phi_j on intersections (local/integrands/product.hh:280-380), used with a function as ansatz
+struct LocalIntersectionProductIntegrand
+{
+  explicit LocalIntersectionProductIntegrand(GridFunction weight_ = 1.)
+    : weight(std::move(weight_))
+  {}
+  BoundaryWithAnsatz with_ansatz(double g) const
+  {
+    return {HGDT_BOUNDARY_PRODUCT, 0., weight, g, nullptr};
+  }
+  BoundaryWithAnsatz with_ansatz(const GenericFunction& g) const
+  {
+    return {HGDT_BOUNDARY_PRODUCT, 0., weight, 0., std::make_shared<GenericFunction>(g)};
+  }
+  GridFunction weight;
+};
 
 // ---------------------------------------------------------------------------------------------
 // local forms, filters, functionals, constraints
@@ -610,6 +668,24 @@ struct LocalElementIntegralFunctional
     , over_integrate(over_integrate_)
   {}
   LocalProductWithAnsatz integrand;
+  int over_integrate;
+};
+
+// LocalIntersectionIntegralFunctional(integrand, over_integrate) on boundary intersections
+// (local/functionals/integrals.hh:108-191)
+struct LocalIntersectionIntegralFunctional
+{
+  LocalIntersectionIntegralFunctional(const LocalLaplaceIPDGIntegrands::DirichletCoupling& integrand,
+                                      int over_integrate_ = 0)
+    : term{HGDT_BOUNDARY_DIRICHLET_COUPLING, integrand.terms.at(0).parameter, integrand.terms.at(0).diffusion,
+           integrand.data_value, integrand.data_function}
+    , over_integrate(over_integrate_)
+  {}
+  LocalIntersectionIntegralFunctional(const BoundaryWithAnsatz& integrand, int over_integrate_ = 0)
+    : term(integrand)
+    , over_integrate(over_integrate_)
+  {}
+  BoundaryWithAnsatz term;
   int over_integrate;
 };
 
@@ -667,9 +743,20 @@ public:
   {
     return append(local_functional);
   }
+  // on the boundary intersections (ApplyOn::CustomBoundaryIntersections)
+  template <class Filter>
+  VectorFunctional& append(const LocalIntersectionIntegralFunctional& local_functional, const Parameter&, const Filter&)
+  {
+    boundary_functionals_.push_back(local_functional);
+    return *this;
+  }
   const std::vector<LocalElementIntegralFunctional>& local_functionals() const
   {
     return functionals_;
+  }
+  const std::vector<LocalIntersectionIntegralFunctional>& boundary_functionals() const
+  {
+    return boundary_functionals_;
   }
   const SpaceInterface& space() const
   {
@@ -679,6 +766,7 @@ public:
 private:
   SpaceInterface space_;
   std::vector<LocalElementIntegralFunctional> functionals_;
+  std::vector<LocalIntersectionIntegralFunctional> boundary_functionals_;
 };
 inline VectorFunctional make_vector_functional(const SpaceInterface& space)
 {
@@ -833,6 +921,31 @@ public:
                                           in.grid_f.field() ? HGDT_SOURCE_PER_ELEMENT : HGDT_SOURCE_CONSTANT,
                                           in.grid_f.field(), in.grid_f.value(), 0, lf.over_integrate));
       }
+    }
+    for (const auto& bf : functional.boundary_functionals()) {
+      const auto& t = bf.term;
+      keep(t.coeff);
+      const int k = space_.order();
+      const int g_order = t.data_function ? t.data_function->order() : 0;
+      const int order = (t.kind == HGDT_BOUNDARY_DIRICHLET_COUPLING) ? g_order + std::max(k - 1, 0) + bf.over_integrate
+                                                                      : k + g_order + bf.over_integrate;
+      const double* data = nullptr;
+      if (t.data_function) {
+        int32_t nq = 0;
+        check(hgdt_grid_boundary_quadrature_points(space_.grid_view().handle(), order, nullptr, &nq));
+        int64_t info[9];
+        check(hgdt_grid_info(space_.grid_view().handle(), info));
+        std::vector<double> pts(size_t(info[6]) * size_t(nq) * 2);
+        check(hgdt_grid_boundary_quadrature_points(space_.grid_view().handle(), order, pts.data(), &nq));
+        std::vector<double> vals(size_t(info[6]) * size_t(nq));
+        for (size_t p = 0; p < vals.size(); ++p)
+          vals[p] = t.data_function->evaluate(&pts[2 * p]);
+        auto dv = std::make_shared<DeviceVector>(vals);
+        kept_.push_back(dv);
+        data = dv->data();
+      }
+      check(hgdt_operator_append_boundary_source(handle(), t.kind, t.param, t.coeff.field(), t.coeff.value(), data,
+                                                 t.data_value, g_order, bf.over_integrate));
     }
     return *this;
   }

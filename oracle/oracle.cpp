@@ -670,6 +670,17 @@ struct Source {
   int f_order;
   int over_integrate;
 };
+// right-hand side terms on boundary intersections (LocalIntersectionIntegralFunctional)
+enum { BS_DIRICHLET_COUPLING = 0, BS_BOUNDARY_PENALTY = 1, BS_PRODUCT = 2 };
+struct BoundarySource {
+  int kind;
+  double param;         // symmetry prefactor or penalty
+  Coefficient coeff;    // diffusion (coupling) or weight (penalty, product)
+  std::vector<double> g; // data at the face quadrature points [boundary face][point], empty -> g_const
+  double g_const = 0.;
+  int g_order = 0;
+  int over_integrate = 0;
+};
 
 // bound local basis: values and physical gradients at a reference point
 struct BoundBasis {
@@ -720,6 +731,19 @@ struct Operator {
   std::vector<ElementForm> eforms;
   std::vector<FaceForm> cforms, bforms;
   std::vector<Source> sources;
+  std::vector<BoundarySource> bsources;
+  std::vector<int64_t> bface_index; // boundary intersections numbered in walk order (element, then face)
+  void number_boundary_faces()
+  {
+    const Mesh& m = *space->mesh;
+    const int nf = m.ref.nfaces;
+    bface_index.assign(m.ne * nf, -1);
+    int64_t c = 0;
+    for (int64_t e = 0; e < m.ne; ++e)
+      for (int f = 0; f < nf; ++f)
+        if (m.nb[e * nf + f] < 0)
+          bface_index[e * nf + f] = c++;
+  }
   bool dirichlet = false;
   std::vector<int32_t> dirichlet_dofs;
 
@@ -970,6 +994,50 @@ struct Operator {
     }
   }
 
+  // quadrature order of a boundary functional term: DirichletCoupling as a unary integrand,
+  // laplace-ipdg.hh:292-297 (data + diffusion + max(test - 1, 0)); BoundaryPenalty and the intersection
+  // product with the data as ansatz (conversion.hh:194-197): weight + test + data
+  int boundary_source_order(const BoundarySource& S) const
+  {
+    const int k = space->fe.order;
+    if (S.kind == BS_DIRICHLET_COUPLING)
+      return S.g_order + 0 + std::max(k - 1, 0) + S.over_integrate;
+    return 0 + k + S.g_order + S.over_integrate;
+  }
+
+  // LocalIntersectionIntegralFunctional::apply (local/functionals/integrals.hh:157-185) with
+  //   DirichletCoupling (unary): -s g (kappa grad psi_i) . n            (laplace-ipdg.hh:300-320)
+  //   BoundaryPenalty(sigma, w).with_ansatz(g): sigma (n.w n)/h g psi_i  (ipdg.hh:254-282, conversion.hh:199-215)
+  //   LocalIntersectionProductIntegrand(w).with_ansatz(g): (w g) psi_i   (product.hh:355-377)
+  void boundary_source_local(const BoundarySource& S, int64_t e, int64_t bi, const FaceGeo& fg, BoundBasis& bin,
+                             std::vector<double>& result) const
+  {
+    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    result.assign(ls, 0.);
+    const double* n = fg.normal;
+    const double c = S.coeff.at(e);
+    const size_t nq = fg.w.size();
+    for (size_t qp = 0; qp < nq; ++qp) {
+      bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
+      const double g = S.g.empty() ? S.g_const : S.g[bi * nq + qp];
+      for (int i = 0; i < ls; ++i) {
+        double v;
+        if (S.kind == BS_DIRICHLET_COUPLING) {
+          v = -1.0 * S.param * g * kgrad_dot(c, &bin.grads[3 * i], n, d);
+        } else if (S.kind == BS_BOUNDARY_PENALTY) {
+          double nwn = 0.;
+          for (int a = 0; a < d; ++a)
+            nwn += n[a] * (c * n[a]);
+          const double penalty = (S.param * nwn) / fg.diameter;
+          v = penalty * g * bin.vals[i];
+        } else {
+          v = (c * g) * bin.vals[i];
+        }
+        result[i] += v * fg.integration_element * fg.w[qp];
+      }
+    }
+  }
+
   static double esv2007_force(const double* x)
   {
     return M_PI_2 * M_PI * std::cos(M_PI_2 * x[0]) * std::cos(M_PI_2 * x[1]);
@@ -1033,6 +1101,24 @@ struct Operator {
                 rhs[gi[i]] += lvec[i];
               } else {
                 rhs[gi[i]] += lvec[i];
+              }
+            }
+          }
+          // boundary functionals, the element's boundary intersections in face order
+          for (int f = 0; f < nf && !bsources.empty(); ++f) {
+            if (m.nb[e * nf + f] >= 0)
+              continue;
+            const int64_t bi = bface_index[e * nf + f];
+            for (const auto& S : bsources) {
+              const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, boundary_source_order(S));
+              boundary_source_local(S, e, bi, fg, basis, lvec);
+              for (int i = 0; i < ls; ++i) {
+                if (atomic) {
+#pragma omp atomic
+                  rhs[gi[i]] += lvec[i];
+                } else {
+                  rhs[gi[i]] += lvec[i];
+                }
               }
             }
           }
@@ -1550,6 +1636,69 @@ double or_esv2007_h1_semi_error(const OrSpace* s, const double* u, int quad_orde
 // (operators/bilinear-form.hh:344-452, local/bilinear-forms/integrals.hh:97-134,
 // as used in examples/stationary-heat-equation.cc:120-132).  g is given by its values
 // [ne][npts] and physical gradients [ne][npts][dim] at the points of element_rule(order).
+int or_operator_append_boundary_source(OrOperator* op, int kind, double param, const double* coeff, double coeff_const,
+                                      const double* g, double g_const, int g_order, int over_integrate)
+{
+  return guarded([&] {
+    if (kind < BS_DIRICHLET_COUPLING || kind > BS_PRODUCT)
+      throw Error("unknown boundary functional kind");
+    BoundarySource S;
+    S.kind = kind, S.param = param, S.g_const = g_const, S.g_order = g_order, S.over_integrate = over_integrate;
+    const Mesh& m = *op->space->mesh;
+    if (coeff)
+      S.coeff.values.assign(coeff, coeff + m.ne);
+    else
+      S.coeff.constant = coeff_const;
+    op->number_boundary_faces();
+    if (g) {
+      int64_t nb = 0;
+      for (int64_t v : op->bface_index)
+        nb += (v >= 0);
+      const int order = op->boundary_source_order(S);
+      const int64_t nq = int64_t(line_rule(order).w.size());
+      S.g.assign(g, g + nb * nq);
+    }
+    op->bsources.push_back(S);
+  });
+}
+
+// physical coordinates of the face quadrature points of the boundary intersections in walk order,
+// out[nb][npts][dim] (NULL: size query); returns npts (2d grids)
+int or_mesh_boundary_quadrature_points(const OrMesh* mesh, int order, double* out)
+{
+  int count = -1;
+  guarded([&] {
+    const Mesh& m = *mesh;
+    if (m.ref.dim != 2)
+      throw Error("boundary quadrature points are implemented for 2d grids");
+    const Quadrature q = line_rule(order);
+    count = int(q.w.size());
+    if (!out)
+      return;
+    const int nf = m.ref.nfaces;
+    int64_t bi = 0;
+    for (int64_t e = 0; e < m.ne; ++e) {
+      const Geometry geo = m.geometry(e);
+      for (int f = 0; f < nf; ++f) {
+        if (m.nb[e * nf + f] >= 0)
+          continue;
+        const int a = m.ref.faces[f][0], b = m.ref.faces[f][1];
+        for (int qp = 0; qp < count; ++qp) {
+          const double t = q.x[qp][0];
+          double xh[3] = {0, 0, 0}, X[3];
+          for (int c = 0; c < 2; ++c)
+            xh[c] = (1. - t) * m.ref.verts[a][c] + t * m.ref.verts[b][c];
+          geo.global(xh, X);
+          out[(bi * count + qp) * 2 + 0] = X[0];
+          out[(bi * count + qp) * 2 + 1] = X[1];
+        }
+        ++bi;
+      }
+    }
+  });
+  return count;
+}
+
 int or_error_norms(const OrSpace* s, const double* u, int order, const double* gval, const double* ggrad,
                    const double* kappa, double kappa_const, double* out)
 {

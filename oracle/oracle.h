@@ -117,6 +117,17 @@ int or_error_norms(const OrSpace* s, const double* u, int order, const double* g
 /* physical coordinates of the element-rule points, out[ne][npts][dim] (NULL: size query); returns npts */
 int or_mesh_quadrature_points(const OrMesh* m, int order, double* out);
 
+/* right-hand side on the boundary intersections: VectorFunctional::append(LocalIntersectionIntegralFunctional(...))
+ * kind 0: LocalLaplaceIPDGIntegrands::DirichletCoupling(param, coeff, g) as a unary integrand
+ * kind 1: LocalIPDGIntegrands::BoundaryPenalty(param, coeff).with_ansatz(g)
+ * kind 2: LocalIntersectionProductIntegrand(coeff).with_ansatz(g)
+ * g: values at the face quadrature points of the term's order, [boundary face (walk order)][point], or g_const */
+int or_operator_append_boundary_source(OrOperator* op, int kind, double param, const double* coeff, double coeff_const,
+                                      const double* g, double g_const, int g_order, int over_integrate);
+/* physical coordinates of the line-rule points on the boundary intersections in walk order, out[nb][npts][2]
+ * (NULL: size query); returns npts */
+int or_mesh_boundary_quadrature_points(const OrMesh* m, int order, double* out);
+
 #ifdef __cplusplus
 }
 #endif

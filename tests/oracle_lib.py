@@ -97,6 +97,10 @@ def lib():
         "or_adjoint_log_kappa_gradient": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
         "or_error_norms": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]),
         "or_mesh_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "or_operator_append_boundary_source": (
+            ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, _dp, ctypes.c_double,
+                           ctypes.c_int, ctypes.c_int]),
+        "or_mesh_boundary_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -158,6 +162,16 @@ class Mesh:
             raise RuntimeError(_err())
         out = np.empty((self.ne, n, self.dim))
         lib().or_mesh_quadrature_points(self.h, order, dptr(out))
+        return out
+
+    def boundary_quadrature_points(self, order):
+        """physical coordinates of the line-rule points on the boundary intersections (walk order), [nb, npts, 2]"""
+        n = lib().or_mesh_boundary_quadrature_points(self.h, order, None)
+        if n < 0:
+            raise RuntimeError(_err())
+        nb = int(np.sum(self.neighbors() < 0))
+        out = np.empty((nb, n, self.dim))
+        lib().or_mesh_boundary_quadrature_points(self.h, order, dptr(out))
         return out
 
     def coords(self):
@@ -256,6 +270,19 @@ class Operator:
     def append_source(self, f=1.0, f_order=0, over_integrate=0, esv2007=False):
         p, c = self._coef(f)
         self._check(lib().or_operator_append_source(self.h, 1 if esv2007 else 0, p, c, f_order, over_integrate))
+
+    BS_DIRICHLET_COUPLING, BS_BOUNDARY_PENALTY, BS_PRODUCT = 0, 1, 2
+
+    def append_boundary_source(self, kind, param=0.0, coeff=1.0, g=0.0, g_order=0, over_integrate=0):
+        """boundary functional term; g: constant or values at the face quadrature points [nb, npts]"""
+        p, c = self._coef(coeff)
+        if np.isscalar(g):
+            gp, gc = None, float(g)
+        else:
+            self._keep_g = getattr(self, "_keep_g", []) + [np.ascontiguousarray(g, np.float64).reshape(-1)]
+            gp, gc = dptr(self._keep_g[-1]), 0.0
+        self._check(lib().or_operator_append_boundary_source(self.h, kind, param, p, c, gp, gc, g_order,
+                                                             over_integrate))
 
     def set_dirichlet(self, enabled=True):
         lib().or_operator_set_dirichlet(self.h, int(enabled))

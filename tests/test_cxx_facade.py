@@ -65,3 +65,52 @@ def test_example_matches_oracle():
     assert diff <= 1e-10 and abs(res["iterations"] - it) <= 2
     assert abs(res["h1_semi_error"] - err_o) <= 1e-12 * err_o
     assert 0 < res["l2_error"] < res["h1_semi_error"]
+
+
+SWIPDG = os.path.join(ROOT, "dune-gdt_amd", "bin", "elliptic_swipdg")
+
+
+def _exact(X):
+    return np.cos(np.pi / 2 * X[..., 0]) * np.cos(np.pi / 2 * X[..., 1]) + 0.25 * X[..., 0] - 0.5 * X[..., 1]
+
+
+@pytest.mark.gpu
+def test_swipdg_example_with_dirichlet_data_matches_oracle():
+    """dune-gdt_amd/examples/elliptic_swipdg.cc: SWIPDG with nonzero Dirichlet data through boundary
+    functionals; solution parity with the oracle (1e-10), H1-semi error parity (1e-12) and first-order
+    convergence under refinement"""
+    errs = []
+    for n in (16, 32):
+        print(f"case C++ example elliptic_swipdg n={n}", flush=True)
+        with tempfile.TemporaryDirectory() as d:
+            out = os.path.join(d, "u.bin")
+            r = subprocess.run([SWIPDG, str(n), out], capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            u = np.fromfile(out, dtype=np.float64)
+        m = O.Mesh.structured(O.QUAD, [n, n], [-1, -1], [1, 1])
+        s = O.Space(m, False, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+        op.append_laplace(1.0)
+        op.append_inner_coupling(1.0, 1.0, 1.0)
+        op.append_inner_penalty(16.0, 1.0)
+        op.append_dirichlet_coupling(1.0, 1.0)
+        op.append_boundary_penalty(16.0, 1.0)
+        op.append_source(esv2007=True, f_order=3)
+        op.append_boundary_source(0, 1.0, 1.0, _exact(m.boundary_quadrature_points(3)), g_order=3)
+        op.append_boundary_source(1, 16.0, 1.0, _exact(m.boundary_quadrature_points(1 + 3)), g_order=3)
+        op.assemble(4)
+        rp, cols = op.pattern()
+        xo, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-10, nthreads=4)
+        diff = np.linalg.norm(u - xo) / np.linalg.norm(xo)
+        P = m.quadrature_points(10)
+        X, Y = P[..., 0], P[..., 1]
+        gg = np.stack([-np.pi / 2 * np.sin(np.pi / 2 * X) * np.cos(np.pi / 2 * Y) + 0.25,
+                       -np.pi / 2 * np.cos(np.pi / 2 * X) * np.sin(np.pi / 2 * Y) - 0.5], axis=-1)
+        l2o, h1o = O.error_norms(s, u, 10, _exact(P), gg)
+        print(f"  iterations {res['iterations']} (oracle {it}), rel L2 {diff:.2e}, h1 error {res['h1_semi_error']:.6e} "
+              f"(oracle on the same solution {np.sqrt(h1o):.6e})")
+        assert diff <= 1e-10 and abs(res["iterations"] - it) <= 2
+        assert abs(res["h1_semi_error"] - np.sqrt(h1o)) <= 1e-12 * np.sqrt(h1o)
+        errs.append(res["h1_semi_error"])
+    assert 1.8 < errs[0] / errs[1] < 2.2  # O(h) in the broken H1 seminorm

@@ -347,3 +347,66 @@ def test_error_norms_match_esv2007_h1_error():
     assert abs(np.sqrt(h1) - ref) <= 1e-13 * ref
     assert float(f"{np.sqrt(h1):.2e}") == GOLDEN["reference"]["esv2007_q1dg_cube"]["h1_semi"][0]
     assert 0 < l2 < h1
+
+
+def _boundary_points(m, order):
+    return m.boundary_quadrature_points(order)
+
+
+@pytest.mark.parametrize("geom,continuous", [(O.QUAD, False), (O.TRI, True)])
+def test_boundary_functional_sums(geom, continuous):
+    """LocalIntersectionIntegralFunctional on the boundary (local/functionals/integrals.hh:157-185): with
+    constant data the sum over all DoFs follows from the partition of unity -- product: w g |boundary|;
+    penalty: sigma w g per boundary face (penalty sigma w / h_F times |F| = h_F); Dirichlet coupling: the
+    gradient of the partition of unity vanishes, so the sum is 0"""
+    m = O.Mesh.structured(geom, [6, 3], [0, 0], [2, 1])
+    s = O.Space(m, continuous, 1)
+    nb = int(np.sum(m.neighbors() < 0))
+    for kind, param, coeff, g, expect in [(2, 0.0, 1.5, 0.7, 1.5 * 0.7 * 6.0), (1, 8.0, 2.0, 0.5, 8.0 * 2.0 * 0.5 * nb),
+                                          (0, 1.0, 3.0, 0.9, 0.0)]:
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_boundary_source(kind, param, coeff, g, g_order=0)
+        op.assemble(1)
+        total = op.rhs().sum()
+        assert abs(total - expect) <= 1e-13 * max(1.0, abs(expect)), (kind, total, expect)
+    # data given at the quadrature points instead of a constant gives the same vector
+    op1 = O.Operator(s, O.STENCIL_ELEMENT)
+    op1.append_boundary_source(1, 8.0, 2.0, 0.5, g_order=0)
+    op1.assemble(1)
+    P = _boundary_points(m, 0 + 1 + 0)
+    op2 = O.Operator(s, O.STENCIL_ELEMENT)
+    op2.append_boundary_source(1, 8.0, 2.0, np.full(P.shape[:2], 0.5), g_order=0)
+    op2.assemble(1)
+    assert np.array_equal(op1.rhs(), op2.rhs())
+
+
+@pytest.mark.parametrize("geom", [O.QUAD, O.TRI])
+def test_swipdg_reproduces_linear_solutions(geom):
+    """consistency of SIPG with weakly imposed Dirichlet data (laplace-ipdg.hh:292-320 for the coupling
+    right-hand side, ipdg.hh:254-282 with the data as ansatz for the penalty): u = 0.3 + 2x - 1.5y solves
+    -div(grad u) = 0 and lies in the discrete space, so the discrete solution reproduces it.  On quadrilaterals
+    the reference's order for the coupling term (data + diffusion + max(test - 1, 0)) under-integrates the
+    bilinear normal derivative along a face: one order more makes the face rule exact; at the reference's
+    own order the interpolant does not satisfy the system (residual 5.8e-2)"""
+    n, sigma = 5, 16.0
+    m = O.Mesh.structured(geom, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, False, 1)
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(1.0)
+    op.append_inner_coupling(1.0, 1.0, 1.0)
+    op.append_inner_penalty(sigma, 1.0)
+    op.append_dirichlet_coupling(1.0, 1.0)
+    op.append_boundary_penalty(sigma, 1.0)
+
+    def u(X):
+        return 0.3 + 2.0 * X[..., 0] - 1.5 * X[..., 1]
+
+    over = 1 if geom == O.QUAD else 0
+    op.append_boundary_source(0, 1.0, 1.0, u(_boundary_points(m, 1 + 0 + 0 + over)), g_order=1, over_integrate=over)
+    op.append_boundary_source(1, sigma, 1.0, u(_boundary_points(m, 0 + 1 + 1)), g_order=1)
+    op.assemble(1)
+    rp, cols = op.pattern()
+    x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-14, max_iter=10000)
+    X = m.coords()[m.conn()]  # [ne, local vertex, 2]: DG DoF (e, l) sits at vertex l of e
+    gi = s.global_indices()
+    assert np.abs(x[gi] - u(X)).max() <= 1e-10
