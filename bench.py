@@ -363,7 +363,8 @@ def main():
     fi = op.format_info()
     spmv_ms = float(np.mean([s["spmv_avg_ms"] for s in stats]))
     step_bytes = int(stats[-1]["step_bytes"])
-    achieved = step_bytes / (spmv_ms * 1e-3) / 1e9
+    # --spmv-sample 0 (or graph replay) records no events: no live roofline figure then
+    achieved = step_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
     solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"pmc_cg_spmv_{args.config}.json")
@@ -388,7 +389,7 @@ def main():
                    "t_cg_ms": solve_ms, "t_assembly_ms": ms_per_step - solve_ms, "setup_s": t_setup,
                    "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": f"k_cg_spmv<{fi['format']}>", "bytes_per_launch": step_bytes,
                      "avg_launch_us": spmv_ms * 1e3},
     }
