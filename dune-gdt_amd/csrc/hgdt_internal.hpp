@@ -38,6 +38,24 @@ struct Error : std::runtime_error {
                           std::string(#call) + ": " + hipGetErrorString(hgdt_e_));                            \
   } while (0)
 
+// XCD-aware index of 1-d launches.  The hardware deals workgroups round-robin to the 8 XCDs of the
+// chip, each with its own L2; hardware block hw runs logical block (hw % 8) * ceil(nb / 8) + hw / 8, so
+// every XCD works on a contiguous eighth of the index range and neighbouring blocks share one L2.  The
+// grid must have xcd_grid(n, block) blocks; lanes whose index is >= n are idle.
+__device__ __forceinline__ int64_t xcd_index(int64_t n)
+{
+  const int64_t nb = (n + blockDim.x - 1) / blockDim.x;
+  const int64_t q = (nb + 7) >> 3;
+  const int64_t hw = blockIdx.x;
+  const int64_t logical = (hw & 7) * q + (hw >> 3);
+  return logical * blockDim.x + threadIdx.x;
+}
+inline unsigned xcd_grid(int64_t n, int block)
+{
+  const int64_t nb = (n + block - 1) / block;
+  return unsigned(((nb + 7) / 8) * 8);
+}
+
 // device buffer owned by the library
 template <class T>
 struct DeviceArray {

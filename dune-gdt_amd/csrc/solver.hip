@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 // x = D^{-1/2} xh after the last step
 __global__ void k_cg_finish(int64_t n, const double* __restrict__ dh, double* __restrict__ x)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   if (i < n)
     x[i] = dh[i] * x[i];
 }
@@ -572,7 +572,7 @@ __global__ void k_scale_dia(int64_t n, int ndiag, const int64_t* __restrict__ of
                             int lower_too, const int32_t* __restrict__ row_inst, const int64_t* __restrict__ inst_row0,
                             const double* __restrict__ dia, const double* __restrict__ dh, double* __restrict__ dst)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   const int k = blockIdx.y;
   if (i >= n)
     return;
@@ -592,7 +592,7 @@ __global__ void k_scale_values(int64_t n, const int64_t* __restrict__ row_ptr, c
                                const double* __restrict__ vals, const int64_t* __restrict__ pos,
                                const double* __restrict__ dh, double* __restrict__ dst)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   if (i >= n)
     return;
   const double hi = dh[i];
@@ -603,7 +603,7 @@ __global__ void k_scale_values(int64_t n, const int64_t* __restrict__ row_ptr, c
 __global__ void k_scatter_values(int64_t nnz, const double* __restrict__ vals, const int64_t* __restrict__ pos,
                                  double* __restrict__ dst)
 {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_index(nnz);
   if (s < nnz)
     dst[pos[s]] = vals[s];
 }
@@ -617,7 +617,7 @@ __global__ void k_fill_int(int n, int* __restrict__ dst, int v)
 
 __global__ void k_fill(int64_t n, double* __restrict__ dst, double v)
 {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_index(n);
   if (s < n)
     dst[s] = v;
 }
@@ -626,7 +626,7 @@ __global__ void k_fill(int64_t n, double* __restrict__ dst, double v)
 __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
                          const double* __restrict__ vals, double* __restrict__ dh, double* __restrict__ dsq, int* bad)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   if (i >= n)
     return;
   double d = 0.;
@@ -650,7 +650,7 @@ __global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ 
                               const int64_t* __restrict__ inst_row0, const double* __restrict__ dia,
                               double* __restrict__ dh, double* __restrict__ dsq, int* flags)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   if (i >= n)
     return;
   const int inst = row_inst[i];
@@ -706,9 +706,10 @@ inline int apply_lane_rows(int items)
   return items == 2 ? 2 : 1;
 }
 
+// 1-d launches use the XCD-aware index (xcd_index)
 inline unsigned blocks_for(int64_t n, int block)
 {
-  return unsigned((n + block - 1) / block);
+  return xcd_grid(n, block);
 }
 
 inline unsigned hw_blocks(int64_t nb)
