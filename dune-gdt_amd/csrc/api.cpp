@@ -111,6 +111,10 @@ static std::shared_ptr<Grid> make_structured(int geom, const int32_t* cells, con
   });
   g->inst_elem = {0, g->ne};
   g->inst_vert = {0, g->nv};
+  Lattice L;
+  for (int a = 0; a < d; ++a)
+    L.n[a] = n[a], L.lo[a] = lo[a], L.h[a] = h[a];
+  g->lattice = {L};
   g->finalize();
   return g;
 }
@@ -121,7 +125,9 @@ void Operator::assemble(hipStream_t stream)
   build_scatter_maps();
   if (dirichlet)
     build_dirichlet();
-  launch_assembly(*this, stream);
+  // matrix-free mode: the right-hand side only, the operator is recomputed at every application
+  matrix_free_assembly = (requested_format == HGDT_FORMAT_MATRIX_FREE);
+  launch_assembly(*this, stream, !matrix_free_assembly);
   fmt.valid = false;
 }
 
@@ -231,6 +237,13 @@ hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_g
     auto u = std::make_shared<Grid>();
     u->ref = grids[0]->g->ref;
     u->structured = false;
+    // a union of structured grids keeps the lattices of its parts (matrix-free application)
+    bool lattices = true;
+    for (int i = 0; i < count; ++i)
+      lattices = lattices && grids[i] && grids[i]->g->lattice.size() == 1;
+    if (lattices)
+      for (int i = 0; i < count; ++i)
+        u->lattice.push_back(grids[i]->g->lattice[0]);
     int64_t nv = 0, ne = 0;
     for (int i = 0; i < count; ++i) {
       require(grids[i] && grids[i]->g->ref.geom == u->ref.geom, HGDT_ERR_INVALID_ARGUMENT,
@@ -523,9 +536,17 @@ hgdt_status hgdt_operator_copy_pattern(hgdt_operator op, int64_t* row_ptr, int32
   });
 }
 
+static void require_matrix(const Operator& o)
+{
+  require(!o.matrix_free_assembly, HGDT_ERR_OPERATOR,
+          "the operator was assembled in matrix-free mode and holds no matrix: set another format and assemble again");
+}
+
 static void forms_changed(Operator& o)
 {
   o.tables_ready = false;
+  if (o.requested_format == HGDT_FORMAT_MATRIX_FREE) // the reference matrices follow the forms
+    o.fmt.structure_ready = false, o.fmt.valid = false;
 }
 
 hgdt_status hgdt_operator_append_laplace(hgdt_operator op, const double* field, double value, int32_t over_integrate,
@@ -624,6 +645,7 @@ hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled)
   return guarded([&] {
     require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
     op->op->dirichlet = enabled != 0;
+    forms_changed(*op->op);
   });
 }
 
@@ -651,6 +673,7 @@ hgdt_status hgdt_operator_values_device(hgdt_operator op, double** values)
 {
   return guarded([&] {
     require(op && values, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_matrix(*op->op);
     *values = op->op->d_values.ptr;
   });
 }
@@ -667,6 +690,7 @@ hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out)
 {
   return guarded([&] {
     require(op && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_matrix(*op->op);
     HGDT_HIP(hipDeviceSynchronize());
     HGDT_HIP(hipMemcpy(host_out, op->op->d_values.ptr, op->op->nnz * sizeof(double), hipMemcpyDeviceToHost));
   });
@@ -684,7 +708,7 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out)
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format)
 {
   return guarded([&] {
-    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_DIA_NONSYMMETRIC, HGDT_ERR_INVALID_ARGUMENT,
+    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_MATRIX_FREE, HGDT_ERR_INVALID_ARGUMENT,
             "unknown format");
     op->op->requested_format = (format == HGDT_FORMAT_DIA_NONSYMMETRIC) ? HGDT_FORMAT_DIA : format;
     op->op->allow_symmetric = (format != HGDT_FORMAT_DIA_NONSYMMETRIC);

@@ -1074,7 +1074,7 @@ void Operator::prepare_assembly()
   tables_ready = true;
 }
 
-void launch_assembly(Operator& op, hipStream_t stream)
+void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
 {
   const Grid& g = *op.space->grid;
   const ScatterMaps& M = op.maps;
@@ -1083,7 +1083,8 @@ void launch_assembly(Operator& op, hipStream_t stream)
   FaceParams C{}, B{};
   build_tables(op, etab, ltab, P, C, B);
   const int eb = op.launch[0].block, fb = op.launch[1].block, gb = op.launch[2].block;
-  if (!op.eforms.empty() || !op.sources.empty()) {
+  const bool emat = matrix && !op.eforms.empty();
+  if (emat || !op.sources.empty()) {
     P.ne = g.ne;
     P.coords = g.d_coords.ptr;
     P.conn = g.d_conn.ptr;
@@ -1091,7 +1092,7 @@ void launch_assembly(Operator& op, hipStream_t stream)
     P.staging = op.d_staging.ptr;
     P.elem_base = M.elem_base;
     P.rhs_base = M.rhs_base;
-    P.write_matrix = op.eforms.empty() ? 0 : 1;
+    P.write_matrix = emat ? 1 : 0;
     dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
       hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)), dim3(eb),
                          0, stream, P);
@@ -1099,7 +1100,7 @@ void launch_assembly(Operator& op, hipStream_t stream)
     HGDT_HIP(hipGetLastError());
   }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
-  if (!op.cforms.empty() && nin > 0) {
+  if (matrix && !op.cforms.empty() && nin > 0) {
     C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
     C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
     if (g.ref.geom == HGDT_QUAD)
@@ -1108,7 +1109,7 @@ void launch_assembly(Operator& op, hipStream_t stream)
       hipLaunchKernelGGL(k_inner_local<HGDT_TRI>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
     HGDT_HIP(hipGetLastError());
   }
-  if (!op.bforms.empty() && nbd > 0) {
+  if (matrix && !op.bforms.empty() && nbd > 0) {
     B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
     B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
     if (g.ref.geom == HGDT_QUAD)
@@ -1117,7 +1118,7 @@ void launch_assembly(Operator& op, hipStream_t stream)
       hipLaunchKernelGGL(k_boundary_local<HGDT_TRI>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
     HGDT_HIP(hipGetLastError());
   }
-  if (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty()) {
+  if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
     hipLaunchKernelGGL(k_gather_values, dim3(grid_for(op.nnz, gb)), dim3(gb), 0, stream, op.nnz, M.d_slot_ptr.ptr,
                        M.d_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_slot_flag.ptr : nullptr,
                        op.d_values.ptr);

@@ -1,0 +1,59 @@
+// block_reduce.hpp -- launch-order and reduction helpers shared by the solver's kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hgdt {
+namespace {
+
+// XCD-aware logical block: hardware block hw runs logical block (hw % 8) * ceil(nb / 8) + hw / 8, so each
+// XCD walks one contiguous eighth of the blocks; -1 for the padding blocks of the last XCD
+__device__ __forceinline__ int logical_block(int nb)
+{
+  const int q = (nb + 7) >> 3;
+  const int hw = blockIdx.x;
+  const int L = (hw & 7) * q + (hw >> 3);
+  return L < nb ? L : -1;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// deterministic block reduction of NV values; the result is returned to every thread
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+  __shared__ double lds[NV][16];
+  __shared__ double res[NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      lds[k][wave] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.;
+      for (int w = 0; w < nw; ++w)
+        s += lds[k][w];
+      res[k] = s;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = res[k];
+  __syncthreads();
+}
+
+} // namespace
+} // namespace hgdt

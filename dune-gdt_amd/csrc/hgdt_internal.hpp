@@ -126,9 +126,17 @@ void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
 Quadrature line_rule(int order);
 Quadrature element_rule(const RefElement& ref, int order);
 
+// axis-parallel lattice of one instance of a structured grid: n[a] cells of width h[a] from lo[a]
+// (cells and vertices numbered lexicographically, x fastest)
+struct Lattice {
+  int64_t n[3] = {1, 1, 1};
+  double lo[3] = {0., 0., 0.}, h[3] = {0., 0., 0.};
+};
+
 struct Grid {
   RefElement ref;
   bool structured = false;
+  std::vector<Lattice> lattice; // one per instance when every instance is a structured grid, else empty
   int64_t nv = 0, ne = 0;
   std::vector<double> coords;  // nv * dim
   std::vector<int32_t> conn;   // ne * nverts
@@ -263,6 +271,13 @@ struct SolverFormat {
   DeviceArray<double> d_dh, d_dsq;
   DeviceArray<double> d_scaled;
   int64_t step_bytes = 0; // algorithmic bytes of the CG solver's operator application
+  // matrix-free application (format HGDT_FORMAT_MATRIX_FREE): per instance the lattice (cells n0 n1
+  // n2, first element, first DoF) and the reference local matrices of the appended forms
+  int mf_kind = -1;
+  int64_t mf_ref_stride = 0;
+  int mf_fields = 0; // distinct coefficient arrays read by one application
+  DeviceArray<int64_t> d_mf_lat;
+  DeviceArray<double> d_mf_ref;
 };
 
 // device-side PCG state, one record per instance; two copies indexed by iteration parity so that
@@ -299,6 +314,7 @@ struct Operator {
     return 0 + k + S.g_order + S.over_integrate;
   }
   bool dirichlet = false;
+  bool matrix_free_assembly = false; // the last assemble() ran in matrix-free mode: no CSR values
   std::vector<int32_t> dirichlet_dofs;
   DeviceArray<uint8_t> d_slot_flag; // 0 keep, 1 zero, 2 one
   DeviceArray<uint8_t> d_dof_mask;
@@ -333,8 +349,28 @@ struct Operator {
 // worker-thread helper for host set-up loops
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body);
 
+// matrix-free application (matrix_free.hip)
+enum { MF_APPLY = 0, MF_PCG = 1, MF_DIAG = 2 };
+struct MfVectors {
+  const double* x = nullptr; // MF_APPLY: x; MF_PCG: p
+  double* y = nullptr;       // MF_APPLY: y = A x; MF_PCG: q = D^{-1/2} A D^{-1/2} p
+  const double* dh = nullptr; // MF_PCG: D^{-1/2}
+  double* part = nullptr;     // MF_PCG: p.q per row block
+  const PcgScalars* S = nullptr;
+  int par = 0;
+  double* out_dh = nullptr;  // MF_DIAG: D^{-1/2}, D^{1/2} and the non-positive-diagonal flag
+  double* out_dsq = nullptr;
+  int* flags = nullptr;
+};
+// host set-up of the matrix-free data; throws HGDT_ERR_NOT_SUPPORTED (with the reason) for operators
+// it does not cover
+void mf_setup(Operator& op);
+int64_t mf_bytes(const Operator& op, bool scaled);
+void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t stream, hipEvent_t ev0 = nullptr,
+               hipEvent_t ev1 = nullptr);
+
 // kernel launchers (assembly.hip)
-void launch_assembly(Operator& op, hipStream_t stream);
+void launch_assembly(Operator& op, hipStream_t stream, bool matrix);
 // squared L2 and (kappa-weighted) H1-semi norms of u_h - g per instance (host_out[2 * ninst])
 void error_norms(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
                  const double* kappa, double kappa_value, double* host_out, hipStream_t stream);

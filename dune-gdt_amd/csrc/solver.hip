@@ -45,6 +45,7 @@
 #include <hip/hip_ext.h>
 
 #include "hgdt_internal.hpp"
+#include "block_reduce.hpp"
 
 namespace hgdt {
 namespace {
@@ -95,53 +96,6 @@ struct PcgParams {
   int64_t max_iter;
   int zero_guess;
 };
-
-__device__ __forceinline__ int logical_block(int nb)
-{
-  const int q = (nb + 7) >> 3;
-  const int hw = blockIdx.x;
-  const int L = (hw & 7) * q + (hw >> 3);
-  return L < nb ? L : -1;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-    v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// deterministic block reduction of NV values; the result is returned to every thread
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV])
-{
-  __shared__ double lds[NV][16];
-  __shared__ double res[NV];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-    v[k] = wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      lds[k][wave] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.;
-      for (int w = 0; w < nw; ++w)
-        s += lds[k][w];
-      res[k] = s;
-    }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-    v[k] = res[k];
-  __syncthreads();
-}
 
 // sum of an instance's block partials (written by an earlier kernel) in block order
 // (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
@@ -812,12 +766,15 @@ static void build_partitions(Operator& op)
   // instance's partials stays small
   if (!op.launch[3].user)
     op.launch[3].block = 256, op.launch[3].items = 1;
+  // the matrix-free kernel of discontinuous spaces gives a lane one element (4 rows) at a time
+  const bool mf_dg = op.requested_format == HGDT_FORMAT_MATRIX_FREE && s.kind == HGDT_DISCONTINUOUS_LAGRANGE;
+  const int items3 = mf_dg ? std::max(op.launch[3].items, 4) : op.launch[3].items;
   if (!op.launch[4].user) {
     op.launch[4].block = 256;
     const int64_t want = (op.n + 256 * 1024 - 1) / (256 * 1024);
     op.launch[4].items = want <= 1 ? 1 : want <= 2 ? 2 : want <= 4 ? 4 : 8;
   }
-  F.A.build(s.inst_dof, op.launch[3].block, op.launch[3].items);
+  F.A.build(s.inst_dof, op.launch[3].block, items3);
   F.V.build(s.inst_dof, op.launch[4].block, op.launch[4].items);
   F.d_inst_row0.upload(s.inst_dof);
   F.partitions_ready = true;
@@ -832,6 +789,19 @@ static void build_structure(Operator& op)
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
   const int ninst = int(s.inst_dof.size()) - 1;
+  F.d_dia.release(), F.d_dia_pos.release(), F.d_sell_vals.release(), F.d_sell_cols.release();
+  F.d_sell_pos.release(), F.d_scaled.release();
+  F.d_mf_ref.release(), F.d_mf_lat.release();
+  F.mf_kind = -1;
+  if (op.requested_format == HGDT_FORMAT_MATRIX_FREE) {
+    mf_setup(op);
+    F.format = HGDT_FORMAT_MATRIX_FREE;
+    F.ndiag = 0, F.nup = 0;
+    F.structure_ready = true;
+    F.valid = false;
+    ++C.version;
+    return;
+  }
 
   // DIA detection: distinct (col - row) offsets per instance
   std::vector<std::vector<int64_t>> offs(ninst);
@@ -866,8 +836,6 @@ static void build_structure(Operator& op)
   }
   if (op.requested_format == HGDT_FORMAT_DIA && !dia_ok)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "DIA storage requested but an instance has more than 32 diagonals");
-  F.d_dia.release(), F.d_dia_pos.release(), F.d_sell_vals.release(), F.d_sell_cols.release();
-  F.d_sell_pos.release();
   if (dia_ok) {
     F.format = HGDT_FORMAT_DIA;
     int nd = 0;
@@ -966,6 +934,8 @@ static void build_structure(Operator& op)
 static int64_t apply_bytes(const Operator& op)
 {
   const SolverFormat& F = op.fmt;
+  if (F.format == HGDT_FORMAT_MATRIX_FREE)
+    return mf_bytes(op, false);
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -976,6 +946,8 @@ static int64_t apply_bytes(const Operator& op)
 static int64_t step_bytes(const Operator& op)
 {
   const SolverFormat& F = op.fmt;
+  if (F.format == HGDT_FORMAT_MATRIX_FREE)
+    return mf_bytes(op, true); // coefficients, D^{-1/2}, p read, q written
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t((F.symmetric ? F.nup : F.ndiag) - 1) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -1028,6 +1000,29 @@ void Operator::prepare_solver(hipStream_t stream)
     build_structure(*this);
   if (F.valid)
     return;
+  F.d_dh.allocate(std::max<int64_t>(n, 1));
+  F.d_dsq.allocate(std::max<int64_t>(n, 1));
+  C.flags.allocate(2);
+  if (!C.host_flags)
+    HGDT_HIP(hipHostMalloc((void**)&C.host_flags, 2 * sizeof(int)));
+  if (F.format == HGDT_FORMAT_MATRIX_FREE) {
+    // Jacobi scaling from the diagonal of the operator, recomputed from the coefficients
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
+    MfVectors v;
+    v.out_dh = F.d_dh.ptr, v.out_dsq = F.d_dsq.ptr, v.flags = C.flags.ptr;
+    mf_launch(*this, MF_DIAG, v, stream);
+    HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HGDT_HIP(hipStreamSynchronize(stream));
+    F.diag_positive = (C.host_flags[0] == 0);
+    F.symmetric = false;
+    F.bytes = apply_bytes(*this);
+    F.step_bytes = step_bytes(*this);
+    F.valid = true;
+    return;
+  }
+  if (matrix_free_assembly)
+    HGDT_THROW(HGDT_ERR_OPERATOR,
+               "the operator was assembled in matrix-free mode and holds no matrix: assemble again in this format");
   const int B = 256;
   const bool dia = F.format == HGDT_FORMAT_DIA;
   double* dst = dia ? F.d_dia.ptr : F.d_sell_vals.ptr;
@@ -1035,11 +1030,6 @@ void Operator::prepare_solver(hipStream_t stream)
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
   hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
                      dia ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
-  F.d_dh.allocate(std::max<int64_t>(n, 1));
-  F.d_dsq.allocate(std::max<int64_t>(n, 1));
-  C.flags.allocate(2);
-  if (!C.host_flags)
-    HGDT_HIP(hipHostMalloc((void**)&C.host_flags, 2 * sizeof(int)));
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
   if (dia)
     hipLaunchKernelGGL(k_dia_prepare, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, F.ndiag, F.d_dia_offsets.ptr,
@@ -1076,6 +1066,12 @@ void Operator::prepare_solver(hipStream_t stream)
 void Operator::apply(const double* x, double* y, hipStream_t stream)
 {
   prepare_solver(stream);
+  if (fmt.format == HGDT_FORMAT_MATRIX_FREE) {
+    MfVectors v;
+    v.x = x, v.y = y;
+    mf_launch(*this, MF_APPLY, v, stream);
+    return;
+  }
   PcgParams P = make_params(*this);
   const int block = fmt.A.block;
   const unsigned grid = hw_blocks(fmt.A.nblocks);
@@ -1119,7 +1115,11 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
     constexpr int I = decltype(IT)::value;
     launch_pcg(k_cg_direction<I>, hw_blocks(V.nblocks), V.block, stream, P, par);
   });
-  dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
+  if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
+    MfVectors v;
+    v.x = P.p, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
+    mf_launch(op, MF_PCG, v, stream, ev0, ev1);
+  } else dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(A.nblocks);
     if (kf == FMT_DIA)

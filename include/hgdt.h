@@ -53,8 +53,14 @@ enum { HGDT_STENCIL_ELEMENT = 0, HGDT_STENCIL_INTERSECTION = 1, HGDT_STENCIL_ELE
 enum { HGDT_SOURCE_CONSTANT = 0, HGDT_SOURCE_PER_ELEMENT = 1, HGDT_SOURCE_PER_QUADRATURE_POINT = 2 };
 /* right-hand side terms on boundary intersections */
 enum { HGDT_BOUNDARY_DIRICHLET_COUPLING = 0, HGDT_BOUNDARY_PENALTY = 1, HGDT_BOUNDARY_PRODUCT = 2 };
-/* matrix storage used for apply / apply_inverse */
-enum { HGDT_FORMAT_AUTO = 0, HGDT_FORMAT_SELL = 1, HGDT_FORMAT_DIA = 2, HGDT_FORMAT_DIA_NONSYMMETRIC = 3 };
+/* matrix storage used for apply / apply_inverse (HGDT_FORMAT_MATRIX_FREE: no stored matrix) */
+enum {
+  HGDT_FORMAT_AUTO = 0,
+  HGDT_FORMAT_SELL = 1,
+  HGDT_FORMAT_DIA = 2,
+  HGDT_FORMAT_DIA_NONSYMMETRIC = 3,
+  HGDT_FORMAT_MATRIX_FREE = 4
+};
 
 typedef struct hgdt_grid_s* hgdt_grid;
 typedef struct hgdt_space_s* hgdt_space;
@@ -231,7 +237,8 @@ hgdt_status hgdt_operator_dirichlet_dofs(hgdt_operator op, int64_t* count, int32
 /* MatrixOperator::assemble(use_tbb) (matrix-based.hh:496-500): one device walk over elements and
  * intersections, deterministic fixed-order scatter, Dirichlet constraints, right-hand side. Async on `stream`. */
 hgdt_status hgdt_operator_assemble(hgdt_operator op, void* stream);
-/* device pointers of the assembled CSR values [nnz] and right-hand side [rows] (owned by the operator) */
+/* device pointers of the assembled CSR values [nnz] and right-hand side [rows] (owned by the operator);
+ * the values are refused (HGDT_ERR_OPERATOR) after an assembly in matrix-free mode */
 hgdt_status hgdt_operator_values_device(hgdt_operator op, double** values);
 hgdt_status hgdt_operator_rhs_device(hgdt_operator op, double** rhs);
 /* synchronous copies to host memory */
@@ -242,10 +249,23 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
  * instance has at most 32 distinct diagonals, else SELL-64).  DIA reads only the diagonals with
  * offset >= 0 when the assembled matrix is exactly (bitwise) symmetric;
  * HGDT_FORMAT_DIA_NONSYMMETRIC always reads every diagonal.  The format is rebuilt from the CSR
- * values at the next apply / apply_inverse after each assemble. */
+ * values at the next apply / apply_inverse after each assemble.
+ *
+ * HGDT_FORMAT_MATRIX_FREE: apply and apply_inverse recompute the operator from the coefficient fields
+ * at every application, element by element and intersection by intersection, as
+ * LocalizableOperator::apply does (operators/localizable-operator.hh:352-379 with the local operator
+ * applicators of local/assembler/operator-applicators.hh:82-87,219-230), and no matrix is stored: assemble()
+ * then builds the right-hand side and the Dirichlet DoF set only, and the CSR values are not available
+ * until the operator is assembled again in another format.  Provided for order-1 spaces on structured
+ * grids of quadrilaterals or hexahedra (and unions of them): continuous Lagrange with element forms and
+ * Dirichlet constraints, discontinuous Lagrange on quadrilaterals with element forms and the SWIPDG
+ * intersection forms.  Results agree with the assembled operator's to round-off (the sums are
+ * associated differently); set_format succeeds for any operator, and the first apply / apply_inverse
+ * fails with HGDT_ERR_NOT_SUPPORTED, naming the reason, for one outside this set. */
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format);
 /* format actually in use after the last preparation (HGDT_FORMAT_SELL, HGDT_FORMAT_DIA for symmetric
- * DIA, HGDT_FORMAT_DIA_NONSYMMETRIC) and the algorithmic bytes one application reads and writes */
+ * DIA, HGDT_FORMAT_DIA_NONSYMMETRIC, HGDT_FORMAT_MATRIX_FREE) and the algorithmic bytes one application
+ * reads and writes */
 hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t* spmv_bytes, int32_t* num_diagonals);
 
 /* ConstMatrixOperator::apply (matrix-based.hh:121-129): y = A x, device vectors of length rows */
