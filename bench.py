@@ -89,6 +89,7 @@ def run_batched(hgdt, torch, args):
     kappa = F.lognormal(41, union.num_elements)
     f = F.uniform(42, union.num_elements)
     space, op = batch_problem(hgdt, union, torch.from_numpy(kappa).cuda(), torch.from_numpy(f).cuda())
+    op.set_format(args.format)
     op.assemble()
     op.apply_inverse("rhs", use_graph=True)
     tuned = choose_launch(op, args)
@@ -97,6 +98,7 @@ def run_batched(hgdt, torch, args):
     for i, n in enumerate(sizes):
         s1, op1 = batch_problem(hgdt, parts[i], torch.from_numpy(kappa[eoff[i]:eoff[i + 1]]).cuda(),
                                 torch.from_numpy(f[eoff[i]:eoff[i + 1]]).cuda())
+        op1.set_format(args.format)
         op1.assemble()
         op1.apply_inverse("rhs", use_graph=True)
         plain.append((s1, op1))
@@ -130,7 +132,7 @@ def run_batched(hgdt, torch, args):
     return {"batched_dofs_per_s": n / t_batch, "plain_dofs_per_s": n / t_plain, "speedup_vs_plain": t_plain / t_batch,
             "ms_per_batch": t_batch * 1e3, "ms_plain_sequential": t_plain * 1e3, "instances": len(sizes),
             "dofs": n, "nnz": op.nnz, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
-            "relres_max": float(rels.max()), "launch": tuned, "setup_s": t_setup,
+            "relres_max": float(rels.max()), "launch": tuned, "setup_s": t_setup, "format": op.format_info()["format"],
             "workload": "256_q1_quads_n_U32-512_lognormal_kappa_uniform_f_dirichlet0"}
 
 
@@ -287,6 +289,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="same as --tune off")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this size first (reference dispatch for HBM counters)")
+    ap.add_argument("--format", default="auto", choices=["auto", "matrix_free", "dia_symmetric", "dia", "sell64"],
+                    help="operator storage of apply_inverse (matrix_free: recomputed from the coefficients at every "
+                         "application; the assembly then builds the right-hand side only)")
     ap.add_argument("--spmv-sample", type=int, default=8,
                     help="HIP events around the operator application of every k-th PCG iteration of the timed steps")
     args = ap.parse_args()
@@ -327,6 +332,7 @@ def main():
         return
     t_setup = time.perf_counter()
     grid, space, op, kappa, workload = build_problem(hgdt, torch, args.config)
+    op.set_format(args.format)
     op.assemble()
     x = op.apply_inverse("rhs")
     torch.cuda.synchronize()
@@ -366,8 +372,11 @@ def main():
     # --spmv-sample 0 (or graph replay) records no events: no live roofline figure then
     achieved = step_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
     solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
+    # the solver's operator application: the storage format's SpMV, or the matrix-free kernel
+    mf = fi["format"] == "matrix_free"
+    kernel = ({"c3": "k_mf_dg<1>"}.get(args.config, "k_mf_cg<2, 1>") if mf else f"k_cg_spmv<{fi['format']}>")
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"pmc_cg_spmv_{args.config}.json")
+    tfile = os.path.join(ROOT, "profiles", f"pmc_{'mf' if mf else 'cg_spmv'}_{args.config}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -390,7 +399,7 @@ def main():
                    "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "kernel": f"k_cg_spmv<{fi['format']}>", "bytes_per_launch": step_bytes,
+                     "kernel": kernel, "bytes_per_launch": step_bytes,
                      "avg_launch_us": spmv_ms * 1e3},
     }
     if args.extras:

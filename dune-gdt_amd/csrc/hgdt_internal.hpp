@@ -238,6 +238,9 @@ struct RowPartition {
   DeviceArray<int64_t> d_row0;
   DeviceArray<int32_t> d_inst, d_inst_block0;
   void build(const std::vector<int64_t>& inst_dof, int block_size, int items_per_thread);
+  // blocks given per instance (tiles of the matrix-free kernel): row0 holds the instance's first row
+  void build_counts(const std::vector<int64_t>& inst_dof, const std::vector<int64_t>& blocks_per_inst,
+                    int block_size, int items_per_thread);
 };
 
 // storage of the matrix used by apply / apply_inverse
@@ -271,12 +274,13 @@ struct SolverFormat {
   DeviceArray<double> d_dh, d_dsq;
   DeviceArray<double> d_scaled;
   int64_t step_bytes = 0; // algorithmic bytes of the CG solver's operator application
-  // matrix-free application (format HGDT_FORMAT_MATRIX_FREE): per instance the lattice (cells n0 n1
-  // n2, first element, first DoF) and the reference local matrices of the appended forms
+  // matrix-free application (format HGDT_FORMAT_MATRIX_FREE): per row block of A one record (first DoF
+  // and first element of its instance, the instance's cells n0 n1 n2, the instance, the block's rows
+  // relative to the instance), and per instance the reference local matrices of the appended forms
   int mf_kind = -1;
   int64_t mf_ref_stride = 0;
   int mf_fields = 0; // distinct coefficient arrays read by one application
-  DeviceArray<int64_t> d_mf_lat;
+  DeviceArray<int64_t> d_mf_blk;
   DeviceArray<double> d_mf_ref;
 };
 
@@ -365,6 +369,10 @@ struct MfVectors {
 // host set-up of the matrix-free data; throws HGDT_ERR_NOT_SUPPORTED (with the reason) for operators
 // it does not cover
 void mf_setup(Operator& op);
+// per-block records of the operator partition A (after every rebuild of the partition)
+void mf_block_table(Operator& op);
+// tiles per instance of the discontinuous matrix-free kernel (empty when the grid has no lattices)
+std::vector<int64_t> mf_tile_counts(const Operator& op);
 int64_t mf_bytes(const Operator& op, bool scaled);
 void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t stream, hipEvent_t ev0 = nullptr,
                hipEvent_t ev1 = nullptr);

@@ -51,9 +51,10 @@ enum { MFK_CG_QUAD = 0, MFK_CG_HEX = 1, MFK_DG_QUAD = 2 };
 
 struct MfParams {
   int ninst;
-  const int64_t* lat;  // per instance: cells n0 n1 n2, first element, first DoF
-  const double* ref;   // per instance (ref_stride doubles): h0 h1 h2 0, element, face, boundary matrices
-  int64_t ref_stride;
+  // per row block: first DoF and first element of the instance, cells n0 n1 n2, instance, first and last
+  // row of the block relative to the instance's first DoF
+  const int64_t* blk;
+  int64_t ref_stride; // doubles per instance of the reference data: h0 h1 1/h1 1/h0, element, face, boundary matrices
   int64_t eoff, coff, boff; // offsets of the three groups inside an instance's block
   int dirichlet;
   int nef;
@@ -66,14 +67,12 @@ struct MfParams {
   double fkv[2][MF_MAXF];
   const double* fomega[2][MF_MAXF];
   double fov[2][MF_MAXF];
-  // row blocks (the solver's operator partition; discontinuous spaces: multiples of 4 rows)
-  int nb;
-  const int64_t* row0;
-  const int32_t* binst;
+  int nb; // row blocks (the solver's operator partition): one row or element per lane
   MfVectors v;
 };
 
-__device__ __forceinline__ double coef(const double* f, double value, int64_t e)
+// coefficient of a form at element e: the field's value, or the constant
+__device__ __forceinline__ double cf(const double* __restrict__ f, double value, int64_t e)
 {
   return f ? f[e] : value;
 }
@@ -139,174 +138,402 @@ __device__ __forceinline__ void finish_block(const MfParams& P, int b, double pa
   }
 }
 
-// continuous Q1 on quadrilaterals (D = 2) or hexahedra (D = 3): one lane per DoF
-template <int D, int MODE>
-__global__ void __launch_bounds__(256) k_mf_cg(MfParams P)
+// the 4 rows i .. i + 3 of an element (discontinuous spaces, i a multiple of 4): raw source values xo and,
+// for the solver, D^{-1/2} (ho); VEC: the vectors are 16-byte aligned, two 16-byte accesses per array
+template <int MODE, bool VEC>
+__device__ __forceinline__ void load4(const MfParams& P, int64_t i, double* xo, double* ho)
+{
+  if constexpr (VEC) {
+    const double2* __restrict__ x = reinterpret_cast<const double2*>(P.v.x + i);
+    const double2 x0 = x[0], x1 = x[1];
+    xo[0] = x0.x, xo[1] = x0.y, xo[2] = x1.x, xo[3] = x1.y;
+    if constexpr (MODE == MF_PCG) {
+      const double2* __restrict__ h = reinterpret_cast<const double2*>(P.v.dh + i);
+      const double2 h0 = h[0], h1 = h[1];
+      ho[0] = h0.x, ho[1] = h0.y, ho[2] = h1.x, ho[3] = h1.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      xo[k] = P.v.x[i + k];
+      if constexpr (MODE == MF_PCG)
+        ho[k] = P.v.dh[i + k];
+    }
+  }
+}
+
+template <bool VEC>
+__device__ __forceinline__ void store4(double* __restrict__ out, int64_t i, const double* v)
+{
+  if constexpr (VEC) {
+    double2* o = reinterpret_cast<double2*>(out + i);
+    o[0] = make_double2(v[0], v[1]);
+    o[1] = make_double2(v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      out[i + k] = v[k];
+  }
+}
+
+// continuous Q1 on quadrilaterals (D = 2) or hexahedra (D = 3): one lane per DoF.  Every load of a lane
+// (the source values of the 3^D neighbourhood, the coefficients of the 2^D cells around the vertex) is
+// issued before the solver's convergence flag is examined.  NEF >= 0: the number of element forms is known
+// at compile time; -1: taken from the parameters.
+template <int D, int MODE, int NEF>
+__global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restrict__ ref)
 {
   constexpr int NV = 1 << D, NS = (D == 2) ? 9 : 27;
   constexpr bool DIAG = (MODE == MF_DIAG);
+  constexpr int EC = NEF > 0 ? NEF : 1;
   const int b = logical_block(P.nb);
   if (b < 0)
     return;
-  const int inst = P.binst[b];
-  if (MODE == MF_PCG && P.v.S[(P.v.par ^ 1) * P.ninst + inst].done)
-    return;
-  const int64_t* __restrict__ L = P.lat + 5 * int64_t(inst);
-  const int64_t e0 = L[3], d0 = L[4];
-  const uint32_t nx = uint32_t(L[0]), ny = uint32_t(L[1]), nz = uint32_t(L[2]);
+  const int64_t* __restrict__ B = P.blk + 8 * int64_t(b);
+  const int64_t d0 = B[0], e0 = B[1];
+  const uint32_t nx = uint32_t(B[2]), ny = uint32_t(B[3]), nz = uint32_t(B[4]);
+  const int inst = int(B[5]);
+  const uint32_t r1 = uint32_t(B[7]);
+  const uint32_t v = uint32_t(B[6]) + threadIdx.x; // row of the instance
+  const bool row = v < r1;
   const uint32_t vx = nx + 1, vy = ny + 1;
   const int64_t sy = vx, sz = int64_t(vx) * vy;
-  const double* __restrict__ R = P.ref + int64_t(inst) * P.ref_stride + P.eoff;
-  const int64_t r0 = P.row0[b], r1 = P.row0[b + 1];
-  double part = 0.;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    const uint32_t v = uint32_t(i - d0);
-    const uint32_t ix = v % vx, iy = (v / vx) % vy, iz = (D == 3) ? v / (vx * vy) : 0u;
-    auto boundary = [&](uint32_t x, uint32_t y, uint32_t z) {
-      return x == 0u || x == nx || y == 0u || y == ny || (D == 3 && (z == 0u || z == nz));
-    };
-    double y = 0.;
-    if (P.dirichlet && boundary(ix, iy, iz)) {
-      // unit row of a constrained DoF
-      y = DIAG ? 1. : src<MODE>(P, i);
-    } else {
-      // source on the 3^D neighbourhood; zero outside the instance and on constrained DoFs (zero columns)
-      double s[NS];
+  const uint32_t ix = v % vx, iy = (v / vx) % vy, iz = (D == 3) ? v / (vx * vy) : 0u;
+  auto boundary = [&](uint32_t x, uint32_t y, uint32_t z) {
+    return x == 0u || x == nx || y == 0u || y == ny || (D == 3 && (z == 0u || z == nz));
+  };
+  const bool fixed = P.dirichlet && boundary(ix, iy, iz); // unit row of a constrained DoF
+  const int64_t i = d0 + v;
+  // source on the 3^D neighbourhood; zero outside the instance and on constrained DoFs (zero columns)
+  double s[NS];
 #pragma unroll
-      for (int t = 0; t < NS; ++t) {
-        s[t] = 0.;
-        if constexpr (!DIAG) {
-          const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = (D == 3) ? t / 9 - 1 : 0;
-          const uint32_t jx = ix + dx, jy = iy + dy, jz = iz + dz; // wraps to > n below zero
-          const bool in = jx <= nx && jy <= ny && (D == 2 || jz <= nz);
-          if (in && !(P.dirichlet && boundary(jx, jy, jz)))
-            s[t] = src<MODE>(P, i + dx + dy * sy + dz * sz);
-        }
-      }
-      // the cells around the vertex: cell (ix - 1 + cx, ...) holds it as local vertex l
+  for (int t = 0; t < NS; ++t) {
+    s[t] = 0.;
+    if constexpr (!DIAG) {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = (D == 3) ? t / 9 - 1 : 0;
+      const uint32_t jx = ix + dx, jy = iy + dy, jz = iz + dz; // wraps to > n below zero
+      const bool in = jx <= nx && jy <= ny && (D == 2 || jz <= nz);
+      const bool self = dx == 0 && dy == 0 && dz == 0;
+      if (row && in && (self || !fixed) && !(P.dirichlet && !self && boundary(jx, jy, jz)))
+        s[t] = src<MODE>(P, i + dx + dy * sy + dz * sz);
+    }
+  }
+  // cell c = (cx, cy, cz) in {0, 1}^D is the cell (ix - 1 + cx, ...), holding the vertex as local vertex
+  // l = (1 - cx) + 2 (1 - cy) + 4 (1 - cz)
+  bool valid[NV];
+  int64_t cell[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+    const uint32_t ex = ix - 1 + cx, ey = iy - 1 + cy, ez = (D == 3) ? iz - 1 + cz : 0u;
+    valid[c] = row && !fixed && ex < nx && ey < ny && (D == 2 || ez < nz);
+    cell[c] = e0 + (int64_t(ez) * ny + ey) * nx + ex;
+  }
+  double kc[EC][NV];
+  if constexpr (NEF >= 0) {
+#pragma unroll
+    for (int f = 0; f < NEF; ++f)
+#pragma unroll
+      for (int c = 0; c < NV; ++c)
+        kc[f][c] = valid[c] ? cf(P.ecoef[f], P.evalue[f], cell[c]) : 0.;
+  }
+  if (MODE == MF_PCG && P.v.S[(P.v.par ^ 1) * P.ninst + inst].done)
+    return;
+  const double* __restrict__ R = ref + int64_t(inst) * P.ref_stride + P.eoff;
+  double y = 0.;
+  if (fixed) {
+    y = DIAG ? 1. : s[NS / 2];
+  } else {
+#pragma unroll
+    for (int f = 0; f < (NEF >= 0 ? NEF : P.nef); ++f) {
+      const double sc = P.escale[f];
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
-        const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-        const uint32_t ex = ix - 1 + cx, ey = iy - 1 + cy, ez = iz - 1 + cz;
-        if (ex >= nx || ey >= ny || (D == 3 && ez >= nz))
+        if (!valid[c])
           continue;
+        const double k = (NEF >= 0) ? kc[NEF >= 0 ? f : 0][c] : cf(P.ecoef[f], P.evalue[f], cell[c]);
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
         const int l = (1 - cx) | ((1 - cy) << 1) | ((D == 3 ? (1 - cz) : 0) << 2);
-        const int64_t e = e0 + (int64_t(D == 3 ? ez : 0u) * ny + ey) * nx + ex;
-        for (int f = 0; f < P.nef; ++f) {
-          const double sf = P.escale[f] * coef(P.ecoef[f], P.evalue[f], e);
-          const double* __restrict__ row = R + f * NV * NV + l * NV;
-          if constexpr (DIAG) {
-            y = fma(sf, row[l], y);
-          } else {
-            double t = 0.;
+        const double* __restrict__ rw = R + f * NV * NV + l * NV;
+        if constexpr (DIAG) {
+          y = fma(sc * k, rw[l], y);
+        } else {
+          double t = 0.;
 #pragma unroll
-            for (int j = 0; j < NV; ++j) {
-              const int tx = cx + (j & 1), ty = cy + ((j >> 1) & 1), tz = (D == 3) ? cz + (j >> 2) : 1;
-              t = fma(row[j], s[tx + 3 * ty + 9 * (D == 3 ? tz : 0)], t);
-            }
-            y = fma(sf, t, y);
+          for (int j = 0; j < NV; ++j) {
+            const int tx = cx + (j & 1), ty = cy + ((j >> 1) & 1), tz = (D == 3) ? cz + (j >> 2) : 0;
+            t = fma(rw[j], s[tx + 3 * ty + 9 * tz], t);
           }
+          y = fma(sc * k, t, y);
         }
       }
     }
-    store_row<MODE>(P, i, y, part);
   }
+  double part = 0.;
+  if (row)
+    store_row<MODE>(P, i, y, part);
   finish_block<MODE>(P, b, part);
 }
 
-// discontinuous Q1 on quadrilaterals: one lane per element (4 consecutive DoFs)
-template <int MODE>
-__global__ void __launch_bounds__(256) k_mf_dg(MfParams P)
+// Structural zeros of the Q1 face matrices.  On a face normal to x (o = 0) the inside element's shape
+// functions 1, 3 (reference x = 1) and the outside element's 0, 2 do not vanish; normal to y (o = 1) the
+// inside's 2, 3 and the outside's 0, 1.  A matrix entry (i, j) is bit 4 i + j of a mask.
+constexpr uint32_t face_dofs(int o, bool inside)
+{
+  return o == 0 ? (inside ? 0xAu : 0x5u) : (inside ? 0xCu : 0x3u);
+}
+constexpr uint32_t mask_of(uint32_t rows, uint32_t cols, bool either)
+{
+  uint32_t m = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const bool r = (rows >> i) & 1, c = (cols >> j) & 1;
+      if (either ? (r || c) : (r && c))
+        m |= 1u << (4 * i + j);
+    }
+  return m;
+}
+// masks of the 6 coupling slots (A_in, G_io, G_oi^T, A_out, G_oi, G_io^T) and the 4 penalty slots (M_ii,
+// M_io, M_io^T, M_oo) of orientation o, and of the boundary matrices of side sd (coupling / penalty)
+constexpr uint32_t coupling_mask(int o, int slot)
+{
+  const uint32_t fi = face_dofs(o, true), fo = face_dofs(o, false);
+  switch (slot) {
+    case 0: return mask_of(fi, fi, true);
+    case 1: return mask_of(fi, 0xFu, false);
+    case 2: return mask_of(0xFu, fo, false);
+    case 3: return mask_of(fo, fo, true);
+    case 4: return mask_of(fo, 0xFu, false);
+    default: return mask_of(0xFu, fi, false);
+  }
+}
+constexpr uint32_t penalty_mask(int o, int slot)
+{
+  const uint32_t fi = face_dofs(o, true), fo = face_dofs(o, false);
+  switch (slot) {
+    case 0: return mask_of(fi, fi, false);
+    case 1: return mask_of(fi, fo, false);
+    case 2: return mask_of(fo, fi, false);
+    default: return mask_of(fo, fo, false);
+  }
+}
+constexpr uint32_t boundary_dofs(int sd)
+{
+  return face_dofs(sd >> 1, sd & 1);
+}
+
+// y += s (M v) over the entries of MASK (DIAG: y_i += s M_ii)
+template <uint32_t MASK, bool DIAG>
+__device__ __forceinline__ void mvm(const double* __restrict__ M, double s, const double* v, double* y)
+{
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if constexpr (DIAG) {
+      if ((MASK >> (5 * i)) & 1u)
+        y[i] = fma(s, M[5 * i], y[i]);
+    } else {
+      if (((MASK >> (4 * i)) & 0xFu) == 0u)
+        continue;
+      double t = 0.;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if ((MASK >> (4 * i + j)) & 1u)
+          t = fma(M[4 * i + j], v[j], t);
+      y[i] = fma(s, t, y[i]);
+    }
+  }
+}
+
+// 1 / x for x > 0: hardware reciprocal and two Newton steps (correct to round-off)
+__device__ __forceinline__ double recip(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.), r, r);
+  return fma(fma(-x, r, 1.), r, r);
+}
+
+// the inner forms' contributions of side SD (0 x-, 1 x+, 2 y-, 3 y+) to the rows of element e
+template <int SD, int MODE, int NCF>
+__device__ __forceinline__ void dg_side(const MfParams& P, const double* __restrict__ R, int ncf, int64_t e, int64_t en,
+                                        const double* a, const double* bv, double* y)
 {
   constexpr bool DIAG = (MODE == MF_DIAG);
+  constexpr int o = SD >> 1;
+  constexpr bool inside = SD & 1; // the neighbour across x+ / y+ has the larger index: this element is inside
+  const double ilen = R[2 + o];
+#pragma unroll
+  for (int f = 0; f < (NCF >= 0 ? NCF : ncf); ++f) {
+    const double sc = P.fscale[0][f], prm = P.fparam[0][f];
+    const double* __restrict__ of = P.fomega[0][f];
+    const double* __restrict__ G = R + P.coff + (2 * f + o) * 96;
+    const double os = cf(of, P.fov[0][f], e), on = cf(of, P.fov[0][f], en);
+    // weights of the intersection from its inside (-) and outside (+) elements
+    const double dm = inside ? os : on, dp = inside ? on : os;
+    const double inv = recip(dp + dm);
+    if (P.fkind[0][f] == FF_INNER_COUPLING) {
+      const double* __restrict__ kf = P.fkappa[0][f];
+      const double ks = cf(kf, P.fkv[0][f], e), kn = cf(kf, P.fkv[0][f], en);
+      const double kin = inside ? ks : kn, kout = inside ? kn : ks;
+      const double wm = dp * inv, wp = dm * inv;
+      if constexpr (inside) {
+        mvm<coupling_mask(o, 0), DIAG>(G, -sc * wm * kin, a, y);
+        if constexpr (!DIAG) {
+          mvm<coupling_mask(o, 1), false>(G + 16, -sc * wp * kout, bv, y);
+          mvm<coupling_mask(o, 2), false>(G + 32, sc * prm * wm * kin, bv, y);
+        }
+      } else {
+        mvm<coupling_mask(o, 3), DIAG>(G + 48, sc * wp * kout, a, y);
+        if constexpr (!DIAG) {
+          mvm<coupling_mask(o, 4), false>(G + 64, sc * wm * kin, bv, y);
+          mvm<coupling_mask(o, 5), false>(G + 80, -sc * prm * wp * kout, bv, y);
+        }
+      }
+    } else {
+      const double pen = sc * (prm * (dp * dm * inv)) * ilen;
+      if constexpr (inside) {
+        mvm<penalty_mask(o, 0), DIAG>(G, pen, a, y);
+        if constexpr (!DIAG)
+          mvm<penalty_mask(o, 1), false>(G + 16, -pen, bv, y);
+      } else {
+        mvm<penalty_mask(o, 3), DIAG>(G + 48, pen, a, y);
+        if constexpr (!DIAG)
+          mvm<penalty_mask(o, 2), false>(G + 32, -pen, bv, y);
+      }
+    }
+  }
+}
+
+// the boundary forms' contributions of side SD
+template <int SD, int MODE, int NBF>
+__device__ __forceinline__ void dg_boundary(const MfParams& P, const double* __restrict__ R, int nbf, int64_t e,
+                                            const double* a, double* y)
+{
+  constexpr bool DIAG = (MODE == MF_DIAG);
+  const double ilen = R[2 + (SD >> 1)];
+#pragma unroll
+  for (int f = 0; f < (NBF >= 0 ? NBF : nbf); ++f) {
+    const double sc = P.fscale[1][f], prm = P.fparam[1][f];
+    const double* __restrict__ B = R + P.boff + (4 * f + SD) * 16;
+    if (P.fkind[1][f] == FF_DIRICHLET_COUPLING) // -s_c k (G + s G^T), stored combined
+      mvm<mask_of(boundary_dofs(SD), boundary_dofs(SD), true), DIAG>(B, -sc * cf(P.fkappa[1][f], P.fkv[1][f], e), a, y);
+    else // s_c sigma omega / h_F M
+      mvm<mask_of(boundary_dofs(SD), boundary_dofs(SD), false), DIAG>(
+          B, sc * (prm * cf(P.fomega[1][f], P.fov[1][f], e)) * ilen, a, y);
+  }
+}
+
+// discontinuous Q1 on quadrilaterals: a block works on a tile of TX x TY elements, one lane per element (its
+// 4 consecutive DoFs).  The source values of the tile and of a one-element halo (D^{-1/2} p for the solver)
+// are first staged in LDS with coalesced loads (the 4-DoF vectors of a row of elements are contiguous), so
+// each lane reads its own and its 4 neighbours' values from LDS.  Sides: 0 x-, 1 x+, 2 y-, 3 y+; across
+// x+ / y+ the neighbour has the larger index (this element is the inside of that intersection), across
+// x- / y- the smaller (the outside).  Face matrices per form and orientation (host set-up): coupling A_in =
+// G_ii + s G_ii^T, G_io, G_oi^T, A_out = G_oo + s G_oo^T, G_oi, G_io^T; penalty M_ii, M_io, M_io^T, M_oo;
+// only their structurally nonzero entries are read.  NEF, NCF, NBF >= 0: form counts known at compile time
+// (loops unrolled); -1: runtime loops.  VEC: the vectors are 16-byte aligned (16-byte accesses).
+constexpr int TX = 64, TY = 4;
+
+template <int MODE, int NEF, int NCF, int NBF, bool VEC>
+__global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __restrict__ ref)
+{
+  constexpr bool DIAG = (MODE == MF_DIAG);
+  __shared__ __align__(16) double sp[TY + 2][TX + 2][4]; // source of the tile and its halo
+  const int nef = NEF >= 0 ? NEF : P.nef, ncf = NCF >= 0 ? NCF : P.ncf, nbf = NBF >= 0 ? NBF : P.nbf;
   const int b = logical_block(P.nb);
   if (b < 0)
     return;
-  const int inst = P.binst[b];
+  const int64_t* __restrict__ Bk = P.blk + 8 * int64_t(b);
+  const int64_t d0 = Bk[0], e0 = Bk[1];
+  const uint32_t nx = uint32_t(Bk[2]), ny = uint32_t(Bk[3]);
+  const int inst = int(Bk[5]);
+  const uint32_t x0 = uint32_t(Bk[6]), y0 = uint32_t(Bk[7]);
   if (MODE == MF_PCG && P.v.S[(P.v.par ^ 1) * P.ninst + inst].done)
     return;
-  const int64_t* __restrict__ L = P.lat + 5 * int64_t(inst);
-  const int64_t nx = L[0], ny = L[1], e0 = L[3], d0 = L[4];
-  const double* __restrict__ R = P.ref + int64_t(inst) * P.ref_stride;
-  const double len[2] = {R[1], R[0]}; // faces normal to x have length h1, faces normal to y h0
-  const int64_t r0 = P.row0[b], r1 = P.row0[b + 1];
-  double part = 0.;
-  for (int64_t i0 = r0 + 4 * int64_t(threadIdx.x); i0 < r1; i0 += 4 * int64_t(blockDim.x)) {
-    const int64_t c = (i0 - d0) >> 2, e = e0 + c;
-    const int64_t ci = c % nx, cj = c / nx;
-    double a[4], y[4] = {0., 0., 0., 0.};
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      a[k] = DIAG ? 0. : src<MODE>(P, i0 + k);
-    for (int f = 0; f < P.nef; ++f)
-      mv<4, false, DIAG>(R + P.eoff + 16 * f, P.escale[f] * coef(P.ecoef[f], P.evalue[f], e), a, y);
-    // sides: 0 x-, 1 x+, 2 y-, 3 y+; the neighbour across x+ / y+ has the larger index (this element is
-    // the inside of that intersection), across x- / y- the smaller (this element is the outside)
-    for (int side = 0; side < 4; ++side) {
-      const int o = side >> 1;
-      const bool inside = side & 1;
-      const bool inner = (side == 0) ? ci > 0 : (side == 1) ? ci < nx - 1 : (side == 2) ? cj > 0 : cj < ny - 1;
-      if (inner) {
-        const int64_t step = (o == 0) ? 1 : nx;
-        const int64_t nb = inside ? e + step : e - step;
-        const int64_t ein = inside ? e : nb, eout = inside ? nb : e;
-        double bv[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          bv[k] = DIAG ? 0. : src<MODE>(P, (inside ? i0 + 4 * step : i0 - 4 * step) + k);
-        for (int f = 0; f < P.ncf; ++f) {
-          const double* __restrict__ G = R + P.coff + (2 * f + o) * 64; // G_ii G_io G_oi G_oo or M_ii M_io - M_oo
-          const double sc = P.fscale[0][f], prm = P.fparam[0][f];
-          const double dm = coef(P.fomega[0][f], P.fov[0][f], ein), dp = coef(P.fomega[0][f], P.fov[0][f], eout);
-          if (P.fkind[0][f] == FF_INNER_COUPLING) {
-            const double kin = coef(P.fkappa[0][f], P.fkv[0][f], ein);
-            const double kout = coef(P.fkappa[0][f], P.fkv[0][f], eout);
-            const double wm = dp / (dp + dm), wp = dm / (dp + dm);
-            if (inside) {
-              const double s0 = -sc * wm * kin;
-              mv<4, false, DIAG>(G, s0, a, y);
-              mv<4, true, DIAG>(G, s0 * prm, a, y);
-              if constexpr (!DIAG) {
-                mv<4, false, false>(G + 16, -sc * wp * kout, bv, y);
-                mv<4, true, false>(G + 32, sc * prm * wm * kin, bv, y);
-              }
-            } else {
-              const double t0 = sc * wp * kout;
-              mv<4, false, DIAG>(G + 48, t0, a, y);
-              mv<4, true, DIAG>(G + 48, t0 * prm, a, y);
-              if constexpr (!DIAG) {
-                mv<4, false, false>(G + 32, sc * wm * kin, bv, y);
-                mv<4, true, false>(G + 16, -sc * prm * wp * kout, bv, y);
-              }
-            }
-          } else {
-            const double pen = sc * ((prm * ((dp * dm) / (dp + dm))) / len[o]);
-            mv<4, false, DIAG>(G + (inside ? 0 : 48), pen, a, y);
-            if constexpr (!DIAG) {
-              if (inside)
-                mv<4, false, false>(G + 16, -pen, bv, y);
-              else
-                mv<4, true, false>(G + 16, -pen, bv, y);
-            }
-          }
+  const uint32_t tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const uint32_t ex = x0 + tx, ey = y0 + ty;
+  const bool row = ex < nx && ey < ny;
+  if constexpr (!DIAG) {
+    // staged columns x0 + xs .. x0 + xe - 1 of the rows y0 - 1 .. y0 + TY (clipped to the instance)
+    const int xs = x0 > 0u ? -1 : 0;
+    const int xe = int(min(uint32_t(TX + 1), nx - x0));
+    constexpr int U = VEC ? 2 : 4; // accesses per element and array
+    const int per_row = (xe - xs) * U;
+    for (int u = threadIdx.x; u < (TY + 2) * per_row; u += TX * TY) {
+      const int r = u / per_row, w = u % per_row;
+      const int64_t yy = int64_t(y0) - 1 + r;
+      if (yy < 0 || yy >= int64_t(ny))
+        continue;
+      const int col = xs + w / U; // relative to x0
+      const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + (4 / U) * (w % U);
+      double* dst = &sp[r][col + 1][(4 / U) * (w % U)];
+      if constexpr (VEC) {
+        double2 xv = *reinterpret_cast<const double2*>(P.v.x + dof);
+        if constexpr (MODE == MF_PCG) {
+          const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
+          xv.x *= hv.x, xv.y *= hv.y;
         }
+        *reinterpret_cast<double2*>(dst) = xv;
       } else {
-        for (int f = 0; f < P.nbf; ++f) {
-          const double* __restrict__ B = R + P.boff + (4 * f + side) * 16;
-          const double sc = P.fscale[1][f], prm = P.fparam[1][f];
-          if (P.fkind[1][f] == FF_DIRICHLET_COUPLING) {
-            const double s0 = -sc * coef(P.fkappa[1][f], P.fkv[1][f], e);
-            mv<4, false, DIAG>(B, s0, a, y);
-            mv<4, true, DIAG>(B, s0 * prm, a, y);
-          } else {
-            mv<4, false, DIAG>(B, sc * ((prm * coef(P.fomega[1][f], P.fov[1][f], e)) / len[o]), a, y);
-          }
-        }
+        dst[0] = src<MODE>(P, dof);
       }
     }
+    __syncthreads();
+  }
+  const double* __restrict__ R = ref + int64_t(inst) * P.ref_stride;
+  const uint32_t c = ey * nx + ex;
+  const int64_t e = e0 + c;
+  const bool has[4] = {row && ex > 0u, row && ex + 1u < nx, row && ey > 0u, row && ey + 1u < ny};
+  double a[4], bv[4][4], y[4] = {0., 0., 0., 0.};
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      store_row<MODE>(P, i0 + k, y[k], part);
+  for (int k = 0; k < 4; ++k) {
+    a[k] = (!DIAG && row) ? sp[ty + 1][tx + 1][k] : 0.;
+    bv[0][k] = (!DIAG && has[0]) ? sp[ty + 1][tx][k] : 0.;
+    bv[1][k] = (!DIAG && has[1]) ? sp[ty + 1][tx + 2][k] : 0.;
+    bv[2][k] = (!DIAG && has[2]) ? sp[ty][tx + 1][k] : 0.;
+    bv[3][k] = (!DIAG && has[3]) ? sp[ty + 2][tx + 1][k] : 0.;
+  }
+  if (row) {
+#pragma unroll
+    for (int f = 0; f < nef; ++f)
+      mv<4, false, DIAG>(R + P.eoff + 16 * f, P.escale[f] * cf(P.ecoef[f], P.evalue[f], e), a, y);
+    if (has[0])
+      dg_side<0, MODE, NCF>(P, R, ncf, e, e - 1, a, bv[0], y);
+    else
+      dg_boundary<0, MODE, NBF>(P, R, nbf, e, a, y);
+    if (has[1])
+      dg_side<1, MODE, NCF>(P, R, ncf, e, e + 1, a, bv[1], y);
+    else
+      dg_boundary<1, MODE, NBF>(P, R, nbf, e, a, y);
+    if (has[2])
+      dg_side<2, MODE, NCF>(P, R, ncf, e, e - nx, a, bv[2], y);
+    else
+      dg_boundary<2, MODE, NBF>(P, R, nbf, e, a, y);
+    if (has[3])
+      dg_side<3, MODE, NCF>(P, R, ncf, e, e + nx, a, bv[3], y);
+    else
+      dg_boundary<3, MODE, NBF>(P, R, nbf, e, a, y);
+  }
+  double part = 0.;
+  if (row) {
+    const int64_t i0 = d0 + 4 * int64_t(c);
+    if constexpr (MODE == MF_APPLY) {
+      store4<VEC>(P.v.y, i0, y);
+    } else if constexpr (MODE == MF_PCG) {
+      // q = D^{-1/2} y; p.q = (D^{-1/2} p).y
+      double xo[4], ho[4];
+      load4<MODE, VEC>(P, i0, xo, ho);
+      double q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        q[k] = ho[k] * y[k];
+        part += xo[k] * q[k];
+      }
+      store4<VEC>(P.v.y, i0, q);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        store_row<MODE>(P, i0 + k, y[k], part);
+    }
   }
   finish_block<MODE>(P, b, part);
 }
@@ -432,17 +659,15 @@ void mf_setup(Operator& op)
   // order 0 + 1 + 1 + over_integrate, face rules likewise: integrals.hh:115, laplace.hh:74-79,
   // product.hh:88-100, laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
   const int nef = int(op.eforms.size()), ncf = int(op.cforms.size()), nbf = int(op.bforms.size());
-  const int64_t eoff = 4, coff = eoff + int64_t(nef) * NV * NV, boff = coff + int64_t(ncf) * 2 * 64;
+  const int64_t eoff = 4, coff = eoff + int64_t(nef) * NV * NV, boff = coff + int64_t(ncf) * 2 * 96;
   const int64_t stride = boff + int64_t(nbf) * 4 * 16;
   std::vector<double> ref(size_t(stride) * ninst, 0.);
-  std::vector<int64_t> lat(5 * size_t(ninst));
   for (int in = 0; in < ninst; ++in) {
     const Lattice& L = g.lattice[in];
     double* r = &ref[size_t(in) * stride];
     const double h[3] = {L.h[0], L.h[1], D == 3 ? L.h[2] : 1.};
-    r[0] = h[0], r[1] = h[1], r[2] = h[2];
-    lat[5 * in + 0] = L.n[0], lat[5 * in + 1] = L.n[1], lat[5 * in + 2] = (D == 3) ? L.n[2] : 1;
-    lat[5 * in + 3] = g.inst_elem[in], lat[5 * in + 4] = s.inst_dof[in];
+    // header: h0, h1 and the inverse face lengths 1 / h1 (faces normal to x), 1 / h0 (normal to y)
+    r[0] = h[0], r[1] = h[1], r[2] = 1. / h[1], r[3] = 1. / h[0];
     const double vol = h[0] * h[1] * (D == 3 ? h[2] : 1.);
     for (int f = 0; f < nef; ++f) {
       const auto& E = op.eforms[f];
@@ -470,7 +695,8 @@ void mf_setup(Operator& op)
       const auto& C = op.cforms[f];
       const Quadrature q = line_rule(0 + 1 + 1 + C.over_integrate);
       for (int o = 0; o < 2; ++o) {
-        double* G = r + coff + int64_t(2 * f + o) * 64;
+        // face integrals G_ii, G_io, G_oi, G_oo (coupling) or M_ii, M_io, M_oo (penalty)
+        double I[4][16] = {};
         const double flen = h[1 - o];
         for (int p = 0; p < q.size(); ++p) {
           double xin[2], xout[2];
@@ -483,17 +709,36 @@ void mf_setup(Operator& op)
             for (int j = 0; j < 4; ++j) {
               if (C.kind == FF_INNER_COUPLING) {
                 const double gin_j = din[j][o] / h[o], gout_j = dout[j][o] / h[o];
-                G[0 + i * 4 + j] += w * (gin_j * pin[i]);
-                G[16 + i * 4 + j] += w * (gout_j * pin[i]);
-                G[32 + i * 4 + j] += w * (gin_j * pout[i]);
-                G[48 + i * 4 + j] += w * (gout_j * pout[i]);
+                I[0][i * 4 + j] += w * (gin_j * pin[i]);
+                I[1][i * 4 + j] += w * (gout_j * pin[i]);
+                I[2][i * 4 + j] += w * (gin_j * pout[i]);
+                I[3][i * 4 + j] += w * (gout_j * pout[i]);
               } else {
-                G[0 + i * 4 + j] += w * (pin[j] * pin[i]);
-                G[16 + i * 4 + j] += w * (pout[j] * pin[i]);
-                G[48 + i * 4 + j] += w * (pout[j] * pout[i]);
+                I[0][i * 4 + j] += w * (pin[j] * pin[i]);
+                I[1][i * 4 + j] += w * (pout[j] * pin[i]);
+                I[3][i * 4 + j] += w * (pout[j] * pout[i]);
               }
             }
         }
+        double* G = r + coff + int64_t(2 * f + o) * 96;
+        const double s_ = C.param;
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) {
+            const int ij = i * 4 + j, ji = j * 4 + i;
+            if (C.kind == FF_INNER_COUPLING) {
+              G[0 + ij] = I[0][ij] + s_ * I[0][ji]; // A_in
+              G[16 + ij] = I[1][ij];                // G_io
+              G[32 + ij] = I[2][ji];                // G_oi^T
+              G[48 + ij] = I[3][ij] + s_ * I[3][ji]; // A_out
+              G[64 + ij] = I[2][ij];                // G_oi
+              G[80 + ij] = I[1][ji];                // G_io^T
+            } else {
+              G[0 + ij] = I[0][ij];  // M_ii
+              G[16 + ij] = I[1][ij]; // M_io
+              G[32 + ij] = I[1][ji]; // M_io^T
+              G[48 + ij] = I[3][ij]; // M_oo
+            }
+          }
       }
     }
     // boundary sides 0 x-, 1 x+, 2 y-, 3 y+ with the outward normal
@@ -503,7 +748,7 @@ void mf_setup(Operator& op)
       for (int side = 0; side < 4; ++side) {
         const int o = side >> 1;
         const double sign = (side & 1) ? 1. : -1.;
-        double* B = r + boff + int64_t(4 * f + side) * 16;
+        double I[16] = {};
         const double flen = h[1 - o];
         for (int p = 0; p < q.size(); ++p) {
           double x[2];
@@ -513,16 +758,42 @@ void mf_setup(Operator& op)
           const double w = flen * q.w[p];
           for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j)
-              B[i * 4 + j] += (Bd.kind == FF_DIRICHLET_COUPLING) ? w * ((sign * dphi[j][o] / h[o]) * phi[i])
+              I[i * 4 + j] += (Bd.kind == FF_DIRICHLET_COUPLING) ? w * ((sign * dphi[j][o] / h[o]) * phi[i])
                                                                 : w * (phi[j] * phi[i]);
         }
+        // coupling: G + s G^T (G_ij = int (grad phi_j . n) phi_i); penalty: M
+        double* B = r + boff + int64_t(4 * f + side) * 16;
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j)
+            B[i * 4 + j] = (Bd.kind == FF_DIRICHLET_COUPLING) ? I[i * 4 + j] + Bd.param * I[j * 4 + i] : I[i * 4 + j];
       }
     }
+  }
+  // the kernels read only the structurally nonzero entries of the face matrices: the others must vanish
+  for (int in = 0; in < ninst && dg; ++in) {
+    const double* r = &ref[size_t(in) * stride];
+    for (int f = 0; f < ncf; ++f)
+      for (int o = 0; o < 2; ++o)
+        for (int slot = 0; slot < 6; ++slot) {
+          const bool coupling = op.cforms[f].kind == FF_INNER_COUPLING;
+          if (!coupling && slot >= 4)
+            continue;
+          const uint32_t m = coupling ? coupling_mask(o, slot) : penalty_mask(o, slot);
+          for (int t = 0; t < 16; ++t)
+            if (!((m >> t) & 1u) && r[coff + (2 * f + o) * 96 + slot * 16 + t] != 0.)
+              HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free set-up: face matrix entry outside the Q1 face pattern");
+        }
+    for (int f = 0; f < nbf; ++f)
+      for (int sd = 0; sd < 4; ++sd) {
+        const uint32_t m = mask_of(boundary_dofs(sd), boundary_dofs(sd), op.bforms[f].kind == FF_DIRICHLET_COUPLING);
+        for (int t = 0; t < 16; ++t)
+          if (!((m >> t) & 1u) && r[boff + (4 * f + sd) * 16 + t] != 0.)
+            HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free set-up: boundary matrix entry outside the Q1 face pattern");
+      }
   }
   F.mf_kind = dg ? MFK_DG_QUAD : (D == 2 ? MFK_CG_QUAD : MFK_CG_HEX);
   F.mf_ref_stride = stride;
   F.d_mf_ref.upload(ref);
-  F.d_mf_lat.upload(lat);
   // distinct coefficient arrays read per application
   std::vector<const double*> fields;
   for (const auto& E : op.eforms)
@@ -536,6 +807,44 @@ void mf_setup(Operator& op)
   F.mf_fields = int(std::unique(fields.begin(), fields.end()) - fields.begin());
 }
 
+void mf_block_table(Operator& op)
+{
+  const Space& s = *op.space;
+  const Grid& g = *s.grid;
+  const RowPartition& A = op.fmt.A;
+  if (int(g.lattice.size()) != g.ninst())
+    return; // not a structured grid: mf_setup refuses the operator
+  const int D = g.ref.dim;
+  const bool tiles = s.kind == HGDT_DISCONTINUOUS_LAGRANGE;
+  std::vector<int64_t> blk(8 * size_t(A.nblocks));
+  for (int64_t b = 0; b < A.nblocks; ++b) {
+    const int in = A.inst[b];
+    const Lattice& L = g.lattice[in];
+    int64_t* r = &blk[8 * size_t(b)];
+    r[0] = s.inst_dof[in], r[1] = g.inst_elem[in];
+    r[2] = L.n[0], r[3] = L.n[1], r[4] = (D == 3) ? L.n[2] : 1;
+    r[5] = in;
+    if (tiles) { // tile origin (element column and row)
+      const int64_t t = b - A.inst_block0[in], ntx = (L.n[0] + TX - 1) / TX;
+      r[6] = (t % ntx) * TX, r[7] = (t / ntx) * TY;
+    } else { // rows of the block relative to the instance
+      r[6] = A.row0[b] - s.inst_dof[in], r[7] = A.row0[b + 1] - s.inst_dof[in];
+    }
+  }
+  op.fmt.d_mf_blk.upload(blk);
+}
+
+std::vector<int64_t> mf_tile_counts(const Operator& op)
+{
+  const Grid& g = *op.space->grid;
+  if (int(g.lattice.size()) != g.ninst() || g.ref.dim != 2)
+    return {};
+  std::vector<int64_t> t(g.ninst());
+  for (int in = 0; in < g.ninst(); ++in)
+    t[in] = ((g.lattice[in].n[0] + TX - 1) / TX) * ((g.lattice[in].n[1] + TY - 1) / TY);
+  return t;
+}
+
 int64_t mf_bytes(const Operator& op, bool scaled)
 {
   return (scaled ? 24 : 16) * op.n + int64_t(8) * op.fmt.mf_fields * op.space->grid->ne;
@@ -546,13 +855,12 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
   const SolverFormat& F = op.fmt;
   MfParams P{};
   P.ninst = int(op.space->inst_dof.size()) - 1;
-  P.lat = F.d_mf_lat.ptr;
-  P.ref = F.d_mf_ref.ptr;
+  P.blk = F.d_mf_blk.ptr;
   P.ref_stride = F.mf_ref_stride;
   const int D = op.space->grid->ref.dim, NV = 1 << D;
   P.eoff = 4;
   P.coff = P.eoff + int64_t(op.eforms.size()) * NV * NV;
-  P.boff = P.coff + int64_t(op.cforms.size()) * 2 * 64;
+  P.boff = P.coff + int64_t(op.cforms.size()) * 2 * 96;
   P.dirichlet = op.dirichlet ? 1 : 0;
   P.nef = int(op.eforms.size());
   for (int f = 0; f < P.nef; ++f) {
@@ -570,25 +878,48 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
     }
   }
   P.nb = int(F.A.nblocks);
-  P.row0 = F.A.d_row0.ptr;
-  P.binst = F.A.d_inst.ptr;
   P.v = v;
   const unsigned grid = hw_grid(F.A.nblocks);
-  const int block = std::min(F.A.block, 256);
+  const int block = F.A.block; // one row / element per lane (the partition is built so)
+  const bool ok = (F.mf_kind == MFK_DG_QUAD)
+                      ? (block == TX * TY && F.A.items == 4 && int64_t(F.A.nblocks) * TX * TY * 4 >= op.n)
+                      : (block <= 256 && F.A.items == 1 && int64_t(block) * F.A.nblocks >= op.n);
+  if (!ok)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free application: row partition does not match the kernels");
   const void* k = nullptr;
   auto pick = [&](auto k0, auto k1, auto k2) {
     k = mode == MF_APPLY ? reinterpret_cast<const void*>(k0)
                          : mode == MF_PCG ? reinterpret_cast<const void*>(k1) : reinterpret_cast<const void*>(k2);
   };
-  if (F.mf_kind == MFK_CG_QUAD)
-    pick(k_mf_cg<2, MF_APPLY>, k_mf_cg<2, MF_PCG>, k_mf_cg<2, MF_DIAG>);
+  // compile-time form counts for the usual operators: one element form (Laplace), and for SWIPDG one
+  // element form, coupling + penalty on the inner and on the boundary intersections
+  // 16-byte vector accesses when every vector of the launch is 16-byte aligned
+  auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  const bool vec = aligned(v.x) && aligned(v.y) && aligned(v.dh);
+  const bool one = op.eforms.size() == 1;
+  const bool swipdg = one && op.cforms.size() == 2 && op.bforms.size() == 2;
+  if (F.mf_kind == MFK_CG_QUAD && one)
+    pick(k_mf_cg<2, MF_APPLY, 1>, k_mf_cg<2, MF_PCG, 1>, k_mf_cg<2, MF_DIAG, 1>);
+  else if (F.mf_kind == MFK_CG_QUAD)
+    pick(k_mf_cg<2, MF_APPLY, -1>, k_mf_cg<2, MF_PCG, -1>, k_mf_cg<2, MF_DIAG, -1>);
+  else if (F.mf_kind == MFK_CG_HEX && one)
+    pick(k_mf_cg<3, MF_APPLY, 1>, k_mf_cg<3, MF_PCG, 1>, k_mf_cg<3, MF_DIAG, 1>);
   else if (F.mf_kind == MFK_CG_HEX)
-    pick(k_mf_cg<3, MF_APPLY>, k_mf_cg<3, MF_PCG>, k_mf_cg<3, MF_DIAG>);
+    pick(k_mf_cg<3, MF_APPLY, -1>, k_mf_cg<3, MF_PCG, -1>, k_mf_cg<3, MF_DIAG, -1>);
+  else if (F.mf_kind == MFK_DG_QUAD && swipdg && vec)
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true>, k_mf_dg<MF_PCG, 1, 2, 2, true>, k_mf_dg<MF_DIAG, 1, 2, 2, true>);
+  else if (F.mf_kind == MFK_DG_QUAD && swipdg)
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, false>, k_mf_dg<MF_PCG, 1, 2, 2, false>, k_mf_dg<MF_DIAG, 1, 2, 2, false>);
+  else if (F.mf_kind == MFK_DG_QUAD && vec)
+    pick(k_mf_dg<MF_APPLY, -1, -1, -1, true>, k_mf_dg<MF_PCG, -1, -1, -1, true>,
+         k_mf_dg<MF_DIAG, -1, -1, -1, true>);
   else if (F.mf_kind == MFK_DG_QUAD)
-    pick(k_mf_dg<MF_APPLY>, k_mf_dg<MF_PCG>, k_mf_dg<MF_DIAG>);
+    pick(k_mf_dg<MF_APPLY, -1, -1, -1, false>, k_mf_dg<MF_PCG, -1, -1, -1, false>,
+         k_mf_dg<MF_DIAG, -1, -1, -1, false>);
   else
     HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free data not prepared");
-  void* args[] = {&P};
+  const double* ref = F.d_mf_ref.ptr;
+  void* args[] = {&P, &ref};
   if (ev0)
     HGDT_HIP(hipExtLaunchKernel(k, dim3(grid), dim3(block), args, 0, stream, ev0, ev1, 0));
   else

@@ -755,6 +755,30 @@ void RowPartition::build(const std::vector<int64_t>& inst_dof, int block_size, i
   d_inst_block0.upload(inst_block0);
 }
 
+void RowPartition::build_counts(const std::vector<int64_t>& inst_dof, const std::vector<int64_t>& blocks_per_inst,
+                                int block_size, int items_per_thread)
+{
+  block = block_size, items = items_per_thread;
+  row0.clear(), inst.clear(), inst_block0.assign(1, 0);
+  const int ninst = int(inst_dof.size()) - 1;
+  for (int in = 0; in < ninst; ++in) {
+    for (int64_t t = 0; t < blocks_per_inst[in]; ++t) {
+      row0.push_back(inst_dof[in]);
+      inst.push_back(in);
+    }
+    inst_block0.push_back(int32_t(row0.size()));
+  }
+  row0.push_back(inst_dof.back());
+  nblocks = int64_t(inst.size());
+  if (nblocks == 0)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "operator has no rows");
+  if (nblocks >= (int64_t(1) << 31) / 8)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many row blocks");
+  d_row0.upload(row0);
+  d_inst.upload(inst);
+  d_inst_block0.upload(inst_block0);
+}
+
 // row-block partitions of the operator (host set-up, O(number of blocks))
 static void build_partitions(Operator& op)
 {
@@ -766,17 +790,26 @@ static void build_partitions(Operator& op)
   // instance's partials stays small
   if (!op.launch[3].user)
     op.launch[3].block = 256, op.launch[3].items = 1;
-  // the matrix-free kernel of discontinuous spaces gives a lane one element (4 rows) at a time
-  const bool mf_dg = op.requested_format == HGDT_FORMAT_MATRIX_FREE && s.kind == HGDT_DISCONTINUOUS_LAGRANGE;
-  const int items3 = mf_dg ? std::max(op.launch[3].items, 4) : op.launch[3].items;
+  // the matrix-free kernels give every lane one row (continuous spaces, at most 256 lanes per block) or one
+  // element of a tile of elements (discontinuous spaces: 256 lanes, 4 rows each)
+  const bool mf = op.requested_format == HGDT_FORMAT_MATRIX_FREE;
+  const bool mf_tiles = mf && s.kind == HGDT_DISCONTINUOUS_LAGRANGE;
+  const int block3 = mf_tiles ? 256 : mf ? std::min(op.launch[3].block, 256) : op.launch[3].block;
+  const int items3 = mf_tiles ? 4 : mf ? 1 : op.launch[3].items;
   if (!op.launch[4].user) {
     op.launch[4].block = 256;
     const int64_t want = (op.n + 256 * 1024 - 1) / (256 * 1024);
     op.launch[4].items = want <= 1 ? 1 : want <= 2 ? 2 : want <= 4 ? 4 : 8;
   }
-  F.A.build(s.inst_dof, op.launch[3].block, items3);
+  const std::vector<int64_t> tiles = mf_tiles ? mf_tile_counts(op) : std::vector<int64_t>{};
+  if (!tiles.empty())
+    F.A.build_counts(s.inst_dof, tiles, block3, items3);
+  else
+    F.A.build(s.inst_dof, block3, items3);
   F.V.build(s.inst_dof, op.launch[4].block, op.launch[4].items);
   F.d_inst_row0.upload(s.inst_dof);
+  if (mf)
+    mf_block_table(op);
   F.partitions_ready = true;
   ++C.version; // captured graphs hold the launch shapes
 }
@@ -791,7 +824,7 @@ static void build_structure(Operator& op)
   const int ninst = int(s.inst_dof.size()) - 1;
   F.d_dia.release(), F.d_dia_pos.release(), F.d_sell_vals.release(), F.d_sell_cols.release();
   F.d_sell_pos.release(), F.d_scaled.release();
-  F.d_mf_ref.release(), F.d_mf_lat.release();
+  F.d_mf_ref.release();
   F.mf_kind = -1;
   if (op.requested_format == HGDT_FORMAT_MATRIX_FREE) {
     mf_setup(op);

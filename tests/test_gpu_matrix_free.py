@@ -90,6 +90,25 @@ def test_swipdg_variants_apply(hgdt_mod, symmetry):
     assert err <= 1e-12
 
 
+def test_matrix_free_unaligned_vectors(hgdt_mod):
+    """source and range at an 8-byte offset (no 16-byte accesses): the same bits as aligned vectors"""
+    H = hgdt_mod
+    print("case matrix-free unaligned vectors", flush=True)
+    cells = [70, 9]  # more than one tile of 64 elements per row
+    kappa = torch.from_numpy(H.fields.lognormal(23, cells[0] * cells[1])).cuda()
+    grid, space, op = hgdt_problem(H, "swipdg", cells, kappa)
+    op.set_format("matrix_free")
+    op.assemble()
+    n = space.num_DoFs
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y1 = op.apply(x)
+    xb = torch.empty(n + 1, dtype=torch.float64, device="cuda")
+    yb = torch.empty(n + 1, dtype=torch.float64, device="cuda")
+    xb[1:] = x
+    op.apply(xb[1:], yb[1:])
+    assert torch.equal(y1, yb[1:])
+
+
 def test_heat_q1_matrix_free_parity(hgdt_mod):
     H = hgdt_mod
     print("case matrix-free heat Q1", flush=True)
