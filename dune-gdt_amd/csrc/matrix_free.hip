@@ -47,6 +47,7 @@ namespace hgdt {
 namespace {
 
 constexpr int MF_MAXF = 8;
+constexpr int TX = 64, TY = 4; // tile of the discontinuous kernel (elements)
 enum { MFK_CG_QUAD = 0, MFK_CG_HEX = 1, MFK_DG_QUAD = 2 };
 
 struct MfParams {
@@ -55,9 +56,10 @@ struct MfParams {
   // row of the block relative to the instance's first DoF
   const int64_t* blk;
   int64_t ref_stride; // doubles per instance of the reference data: h0 h1 1/h1 1/h0, element, face, boundary matrices
-  int64_t eoff, coff, boff; // offsets of the three groups inside an instance's block
+  int64_t eoff; // offset of the element matrices inside an instance's block (continuous spaces)
   int dirichlet;
   int nef;
+  int ekind[MF_MAXF];
   const double* ecoef[MF_MAXF];
   double evalue[MF_MAXF], escale[MF_MAXF];
   int ncf, nbf;
@@ -244,7 +246,6 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
   if (fixed) {
     y = DIAG ? 1. : s[NS / 2];
   } else {
-#pragma unroll
     for (int f = 0; f < (NEF >= 0 ? NEF : P.nef); ++f) {
       const double sc = P.escale[f];
 #pragma unroll
@@ -275,75 +276,6 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
   finish_block<MODE>(P, b, part);
 }
 
-// Structural zeros of the Q1 face matrices.  On a face normal to x (o = 0) the inside element's shape
-// functions 1, 3 (reference x = 1) and the outside element's 0, 2 do not vanish; normal to y (o = 1) the
-// inside's 2, 3 and the outside's 0, 1.  A matrix entry (i, j) is bit 4 i + j of a mask.
-constexpr uint32_t face_dofs(int o, bool inside)
-{
-  return o == 0 ? (inside ? 0xAu : 0x5u) : (inside ? 0xCu : 0x3u);
-}
-constexpr uint32_t mask_of(uint32_t rows, uint32_t cols, bool either)
-{
-  uint32_t m = 0;
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      const bool r = (rows >> i) & 1, c = (cols >> j) & 1;
-      if (either ? (r || c) : (r && c))
-        m |= 1u << (4 * i + j);
-    }
-  return m;
-}
-// masks of the 6 coupling slots (A_in, G_io, G_oi^T, A_out, G_oi, G_io^T) and the 4 penalty slots (M_ii,
-// M_io, M_io^T, M_oo) of orientation o, and of the boundary matrices of side sd (coupling / penalty)
-constexpr uint32_t coupling_mask(int o, int slot)
-{
-  const uint32_t fi = face_dofs(o, true), fo = face_dofs(o, false);
-  switch (slot) {
-    case 0: return mask_of(fi, fi, true);
-    case 1: return mask_of(fi, 0xFu, false);
-    case 2: return mask_of(0xFu, fo, false);
-    case 3: return mask_of(fo, fo, true);
-    case 4: return mask_of(fo, 0xFu, false);
-    default: return mask_of(0xFu, fi, false);
-  }
-}
-constexpr uint32_t penalty_mask(int o, int slot)
-{
-  const uint32_t fi = face_dofs(o, true), fo = face_dofs(o, false);
-  switch (slot) {
-    case 0: return mask_of(fi, fi, false);
-    case 1: return mask_of(fi, fo, false);
-    case 2: return mask_of(fo, fi, false);
-    default: return mask_of(fo, fo, false);
-  }
-}
-constexpr uint32_t boundary_dofs(int sd)
-{
-  return face_dofs(sd >> 1, sd & 1);
-}
-
-// y += s (M v) over the entries of MASK (DIAG: y_i += s M_ii)
-template <uint32_t MASK, bool DIAG>
-__device__ __forceinline__ void mvm(const double* __restrict__ M, double s, const double* v, double* y)
-{
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if constexpr (DIAG) {
-      if ((MASK >> (5 * i)) & 1u)
-        y[i] = fma(s, M[5 * i], y[i]);
-    } else {
-      if (((MASK >> (4 * i)) & 0xFu) == 0u)
-        continue;
-      double t = 0.;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if ((MASK >> (4 * i + j)) & 1u)
-          t = fma(M[4 * i + j], v[j], t);
-      y[i] = fma(s, t, y[i]);
-    }
-  }
-}
-
 // 1 / x for x > 0: hardware reciprocal and two Newton steps (correct to round-off)
 __device__ __forceinline__ double recip(double x)
 {
@@ -352,87 +284,205 @@ __device__ __forceinline__ double recip(double x)
   return fma(fma(-x, r, 1.), r, r);
 }
 
-// the inner forms' contributions of side SD (0 x-, 1 x+, 2 y-, 3 y+) to the rows of element e
-template <int SD, int MODE, int NCF>
-__device__ __forceinline__ void dg_side(const MfParams& P, const double* __restrict__ R, int ncf, int64_t e, int64_t en,
-                                        const double* a, const double* bv, double* y)
+// ---- discontinuous Q1 on rectangles, in closed form ----
+// Every integral of the element and intersection forms is exact under the reference's rules (order
+// 0 + 1 + 1 + over_integrate >= 2: Gauss rules of >= 2 points per direction; the integrands are of degree
+// <= 2 per direction on a rectangle), so the local operators are applied in closed form.  Local DoF i has
+// bits (i & 1, i >> 1) = reference (x, y) vertex coordinates.  On a face normal to direction o the trace of
+// a Q1 function is linear along the face in the basis (1 - t, t); its two coefficients are the DoFs
+// fidx(o, side, m), m = 0, 1 (side 1: reference coordinate 1 in direction o).  With W = [2 1; 1 2]:
+//   face mass        int theta_m theta_n = len / 6 W[m][n]
+//   element Laplace  (h_y / 6 h_x) s_i s_j W[y_i][y_j] + (h_x / 6 h_y) s'_i s'_j W[x_i][x_j]
+//                    (s = +-1 by the x bit, s' by the y bit)
+//   element product  h_x h_y / 36 W[x_i][x_j] W[y_i][y_j]
+// and for an inner intersection with inside L (its side 1) and outside R (its side 0), jump J = u_L - u_R,
+// F = wm k_L dn u_L + wp k_R dn u_R (laplace-ipdg.hh:107-186, ipdg.hh:113-171):
+//   rows of L:  - int F psi_i - s wm k_L int J dn psi_i + p int J psi_i
+//   rows of R:  + int F psi_i - s wp k_R int J dn psi_i - p int J psi_i
+// and on a boundary intersection with outward normal (laplace-ipdg.hh:340-368, ipdg.hh:254-282):
+//   - int k dn u psi_i - s int k u dn psi_i + sigma omega / h_F int u psi_i
+constexpr int fidx(int o, int side, int m)
 {
-  constexpr bool DIAG = (MODE == MF_DIAG);
-  constexpr int o = SD >> 1;
-  constexpr bool inside = SD & 1; // the neighbour across x+ / y+ has the larger index: this element is inside
-  const double ilen = R[2 + o];
-#pragma unroll
-  for (int f = 0; f < (NCF >= 0 ? NCF : ncf); ++f) {
-    const double sc = P.fscale[0][f], prm = P.fparam[0][f];
-    const double* __restrict__ of = P.fomega[0][f];
-    const double* __restrict__ G = R + P.coff + (2 * f + o) * 96;
-    const double os = cf(of, P.fov[0][f], e), on = cf(of, P.fov[0][f], en);
-    // weights of the intersection from its inside (-) and outside (+) elements
-    const double dm = inside ? os : on, dp = inside ? on : os;
-    const double inv = recip(dp + dm);
-    if (P.fkind[0][f] == FF_INNER_COUPLING) {
-      const double* __restrict__ kf = P.fkappa[0][f];
-      const double ks = cf(kf, P.fkv[0][f], e), kn = cf(kf, P.fkv[0][f], en);
-      const double kin = inside ? ks : kn, kout = inside ? kn : ks;
-      const double wm = dp * inv, wp = dm * inv;
-      if constexpr (inside) {
-        mvm<coupling_mask(o, 0), DIAG>(G, -sc * wm * kin, a, y);
-        if constexpr (!DIAG) {
-          mvm<coupling_mask(o, 1), false>(G + 16, -sc * wp * kout, bv, y);
-          mvm<coupling_mask(o, 2), false>(G + 32, sc * prm * wm * kin, bv, y);
-        }
-      } else {
-        mvm<coupling_mask(o, 3), DIAG>(G + 48, sc * wp * kout, a, y);
-        if constexpr (!DIAG) {
-          mvm<coupling_mask(o, 4), false>(G + 64, sc * wm * kin, bv, y);
-          mvm<coupling_mask(o, 5), false>(G + 80, -sc * prm * wp * kout, bv, y);
-        }
-      }
+  return o == 0 ? side + 2 * m : 2 * side + m;
+}
+
+struct DgGeom {
+  double h[2], ih[2]; // cell widths and their inverses
+};
+
+// coefficients of the element and of its neighbours (a field's value or the constant)
+struct Coefs {
+  int64_t e; // the element
+  __device__ __forceinline__ double own(const double* f, double value) const
+  {
+    return cf(f, value, e);
+  }
+  __device__ __forceinline__ double nb(const double* f, double value, int64_t en) const
+  {
+    return cf(f, value, en);
+  }
+};
+
+// y += c (W v) on the face DoFs of `side` (v: 2 face coefficients already multiplied by len / 6)
+template <int O, int SIDE>
+__device__ __forceinline__ void add_face(double c, const double (&wv)[2], double* y)
+{
+  y[fidx(O, SIDE, 0)] = fma(c, wv[0], y[fidx(O, SIDE, 0)]);
+  y[fidx(O, SIDE, 1)] = fma(c, wv[1], y[fidx(O, SIDE, 1)]);
+}
+// y_i += c dn(psi_i) v[m_i] for all i: dn psi_i = +-1 / h_o by side (the derivative along +e_o)
+template <int O>
+__device__ __forceinline__ void add_dn(double c, const double (&wv)[2], double* y)
+{
+  y[fidx(O, 1, 0)] = fma(c, wv[0], y[fidx(O, 1, 0)]);
+  y[fidx(O, 1, 1)] = fma(c, wv[1], y[fidx(O, 1, 1)]);
+  y[fidx(O, 0, 0)] = fma(-c, wv[0], y[fidx(O, 0, 0)]);
+  y[fidx(O, 0, 1)] = fma(-c, wv[1], y[fidx(O, 0, 1)]);
+}
+__device__ __forceinline__ void wmul(double c6, const double (&v)[2], double (&out)[2])
+{
+  out[0] = c6 * (2. * v[0] + v[1]);
+  out[1] = c6 * (v[0] + 2. * v[1]);
+}
+
+// element forms
+__device__ __forceinline__ void dg_element(const MfParams& P, int nef, const Coefs& K, const DgGeom& g, const double* a,
+                                           double* y)
+{
+  for (int f = 0; f < nef; ++f) {
+    const double c = P.escale[f] * K.own(P.ecoef[f], P.evalue[f]);
+    if (P.ekind[f] == EF_LAPLACE) {
+      const double cx = c * (g.h[1] * g.ih[0]) * (1. / 6.), cy = c * (g.h[0] * g.ih[1]) * (1. / 6.);
+      const double dx[2] = {a[1] - a[0], a[3] - a[2]}, dy[2] = {a[2] - a[0], a[3] - a[1]};
+      const double wx0 = 2. * dx[0] + dx[1], wx1 = dx[0] + 2. * dx[1];
+      const double wy0 = 2. * dy[0] + dy[1], wy1 = dy[0] + 2. * dy[1];
+      y[0] = fma(-cx, wx0, fma(-cy, wy0, y[0]));
+      y[1] = fma(cx, wx0, fma(-cy, wy1, y[1]));
+      y[2] = fma(-cx, wx1, fma(cy, wy0, y[2]));
+      y[3] = fma(cx, wx1, fma(cy, wy1, y[3]));
     } else {
-      const double pen = sc * (prm * (dp * dm * inv)) * ilen;
-      if constexpr (inside) {
-        mvm<penalty_mask(o, 0), DIAG>(G, pen, a, y);
-        if constexpr (!DIAG)
-          mvm<penalty_mask(o, 1), false>(G + 16, -pen, bv, y);
-      } else {
-        mvm<penalty_mask(o, 3), DIAG>(G + 48, pen, a, y);
-        if constexpr (!DIAG)
-          mvm<penalty_mask(o, 2), false>(G + 32, -pen, bv, y);
-      }
+      const double cm = c * (g.h[0] * g.h[1]) * (1. / 36.);
+      double t[4]; // W in x, then W in y
+      t[0] = 2. * a[0] + a[1], t[1] = a[0] + 2. * a[1], t[2] = 2. * a[2] + a[3], t[3] = a[2] + 2. * a[3];
+      y[0] = fma(cm, 2. * t[0] + t[2], y[0]);
+      y[1] = fma(cm, 2. * t[1] + t[3], y[1]);
+      y[2] = fma(cm, t[0] + 2. * t[2], y[2]);
+      y[3] = fma(cm, t[1] + 2. * t[3], y[3]);
     }
   }
 }
 
-// the boundary forms' contributions of side SD
-template <int SD, int MODE, int NBF>
-__device__ __forceinline__ void dg_boundary(const MfParams& P, const double* __restrict__ R, int nbf, int64_t e,
-                                            const double* a, double* y)
+// the inner forms of side SD (0 x-, 1 x+, 2 y-, 3 y+; across x+ / y+ this element is the inside)
+template <int SD, int NCF>
+__device__ __forceinline__ void dg_inner(const MfParams& P, int ncf, const Coefs& K, int64_t en, const DgGeom& g,
+                                         const double* a, const double* bv, double* y)
 {
-  constexpr bool DIAG = (MODE == MF_DIAG);
-  const double ilen = R[2 + (SD >> 1)];
+  constexpr int O = SD >> 1;
+  constexpr bool INSIDE = SD & 1;
+  const double* uL = INSIDE ? a : bv;
+  const double* uR = INSIDE ? bv : a;
+  const double ihn = g.ih[O], c6 = g.h[1 - O] * (1. / 6.), ilen = g.ih[1 - O];
+  double J[2], DL[2], DR[2], YJ[2];
 #pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    J[m] = uL[fidx(O, 1, m)] - uR[fidx(O, 0, m)];
+    DL[m] = (uL[fidx(O, 1, m)] - uL[fidx(O, 0, m)]) * ihn;
+    DR[m] = (uR[fidx(O, 1, m)] - uR[fidx(O, 0, m)]) * ihn;
+  }
+  wmul(c6, J, YJ);
+  for (int f = 0; f < (NCF >= 0 ? NCF : ncf); ++f) {
+    const double sc = P.fscale[0][f], prm = P.fparam[0][f];
+    const double* __restrict__ of = P.fomega[0][f];
+    const double os = K.own(of, P.fov[0][f]), on = K.nb(of, P.fov[0][f], en);
+    // weights of the intersection from its inside (-) and outside (+) elements
+    const double dm = INSIDE ? os : on, dp = INSIDE ? on : os;
+    const double inv = recip(dp + dm);
+    if (P.fkind[0][f] == FF_INNER_COUPLING) {
+      const double* __restrict__ kf = P.fkappa[0][f];
+      const double ks = K.own(kf, P.fkv[0][f]), kn = K.nb(kf, P.fkv[0][f], en);
+      const double kin = INSIDE ? ks : kn, kout = INSIDE ? kn : ks;
+      const double wm = dp * inv, wp = dm * inv;
+      double F[2], YF[2];
+      F[0] = (wm * kin) * DL[0] + (wp * kout) * DR[0];
+      F[1] = (wm * kin) * DL[1] + (wp * kout) * DR[1];
+      wmul(c6, F, YF);
+      if constexpr (INSIDE) {
+        add_face<O, 1>(-sc, YF, y);
+        add_dn<O>(-sc * prm * wm * kin * ihn, YJ, y);
+      } else {
+        add_face<O, 0>(sc, YF, y);
+        add_dn<O>(-sc * prm * wp * kout * ihn, YJ, y);
+      }
+    } else {
+      const double pen = sc * (prm * (dp * dm * inv)) * ilen;
+      if constexpr (INSIDE)
+        add_face<O, 1>(pen, YJ, y);
+      else
+        add_face<O, 0>(-pen, YJ, y);
+    }
+  }
+}
+
+// the boundary forms on side SD (outward normal +-e_o)
+template <int SD, int NBF>
+__device__ __forceinline__ void dg_boundary(const MfParams& P, int nbf, const Coefs& K, const DgGeom& g, const double* a,
+                                            double* y)
+{
+  constexpr int O = SD >> 1, SIDE = SD & 1;
+  const double sign = SIDE ? 1. : -1.;
+  const double ihn = g.ih[O], c6 = g.h[1 - O] * (1. / 6.), ilen = g.ih[1 - O];
+  double T[2], Dn[2], YT[2], YD[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    T[m] = a[fidx(O, SIDE, m)];
+    Dn[m] = sign * ((a[fidx(O, 1, m)] - a[fidx(O, 0, m)]) * ihn);
+  }
+  wmul(c6, T, YT);
+  wmul(c6, Dn, YD);
   for (int f = 0; f < (NBF >= 0 ? NBF : nbf); ++f) {
     const double sc = P.fscale[1][f], prm = P.fparam[1][f];
-    const double* __restrict__ B = R + P.boff + (4 * f + SD) * 16;
-    if (P.fkind[1][f] == FF_DIRICHLET_COUPLING) // -s_c k (G + s G^T), stored combined
-      mvm<mask_of(boundary_dofs(SD), boundary_dofs(SD), true), DIAG>(B, -sc * cf(P.fkappa[1][f], P.fkv[1][f], e), a, y);
-    else // s_c sigma omega / h_F M
-      mvm<mask_of(boundary_dofs(SD), boundary_dofs(SD), false), DIAG>(
-          B, sc * (prm * cf(P.fomega[1][f], P.fov[1][f], e)) * ilen, a, y);
+    if (P.fkind[1][f] == FF_DIRICHLET_COUPLING) {
+      const double k = K.own(P.fkappa[1][f], P.fkv[1][f]);
+      add_face<O, SIDE>(-sc * k, YD, y);
+      add_dn<O>(-sc * prm * k * sign * ihn, YT, y);
+    } else {
+      add_face<O, SIDE>(sc * (prm * K.own(P.fomega[1][f], P.fov[1][f])) * ilen, YT, y);
+    }
   }
+}
+
+// y = (local operator of element e) applied to its own values a and its neighbours' bv
+template <int NEF, int NCF, int NBF>
+__device__ __forceinline__ void dg_local(const MfParams& P, int nef, int ncf, int nbf, const Coefs& K, uint32_t nx,
+                                         const bool (&has)[4], const DgGeom& g, const double* a,
+                                         const double (&bv)[4][4], double* y)
+{
+  const int64_t e = K.e;
+  dg_element(P, NEF >= 0 ? NEF : nef, K, g, a, y);
+  if (has[0])
+    dg_inner<0, NCF>(P, ncf, K, e - 1, g, a, bv[0], y);
+  else
+    dg_boundary<0, NBF>(P, nbf, K, g, a, y);
+  if (has[1])
+    dg_inner<1, NCF>(P, ncf, K, e + 1, g, a, bv[1], y);
+  else
+    dg_boundary<1, NBF>(P, nbf, K, g, a, y);
+  if (has[2])
+    dg_inner<2, NCF>(P, ncf, K, e - nx, g, a, bv[2], y);
+  else
+    dg_boundary<2, NBF>(P, nbf, K, g, a, y);
+  if (has[3])
+    dg_inner<3, NCF>(P, ncf, K, e + nx, g, a, bv[3], y);
+  else
+    dg_boundary<3, NBF>(P, nbf, K, g, a, y);
 }
 
 // discontinuous Q1 on quadrilaterals: a block works on a tile of TX x TY elements, one lane per element (its
 // 4 consecutive DoFs).  The source values of the tile and of a one-element halo (D^{-1/2} p for the solver)
 // are first staged in LDS with coalesced loads (the 4-DoF vectors of a row of elements are contiguous), so
-// each lane reads its own and its 4 neighbours' values from LDS.  Sides: 0 x-, 1 x+, 2 y-, 3 y+; across
-// x+ / y+ the neighbour has the larger index (this element is the inside of that intersection), across
-// x- / y- the smaller (the outside).  Face matrices per form and orientation (host set-up): coupling A_in =
-// G_ii + s G_ii^T, G_io, G_oi^T, A_out = G_oo + s G_oo^T, G_oi, G_io^T; penalty M_ii, M_io, M_io^T, M_oo;
-// only their structurally nonzero entries are read.  NEF, NCF, NBF >= 0: form counts known at compile time
-// (loops unrolled); -1: runtime loops.  VEC: the vectors are 16-byte aligned (16-byte accesses).
-constexpr int TX = 64, TY = 4;
-
+// each lane reads its own and its 4 neighbours' values from LDS.  The diagonal is the local operator applied
+// to the 4 unit vectors.  NEF, NCF, NBF >= 0: form counts known at compile time (loops unrolled); -1: runtime
+// loops.  VEC: the vectors are 16-byte aligned (16-byte accesses).
 template <int MODE, int NEF, int NCF, int NBF, bool VEC>
 __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __restrict__ ref)
 {
@@ -456,62 +506,67 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
     // staged columns x0 + xs .. x0 + xe - 1 of the rows y0 - 1 .. y0 + TY (clipped to the instance)
     const int xs = x0 > 0u ? -1 : 0;
     const int xe = int(min(uint32_t(TX + 1), nx - x0));
+    const int ncols = xe - xs;
     constexpr int U = VEC ? 2 : 4; // accesses per element and array
-    const int per_row = (xe - xs) * U;
+    const int per_row = ncols * U;
     for (int u = threadIdx.x; u < (TY + 2) * per_row; u += TX * TY) {
       const int r = u / per_row, w = u % per_row;
       const int64_t yy = int64_t(y0) - 1 + r;
       if (yy < 0 || yy >= int64_t(ny))
         continue;
       const int col = xs + w / U; // relative to x0
-      const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + (4 / U) * (w % U);
-      double* dst = &sp[r][col + 1][(4 / U) * (w % U)];
+      const int k0 = (4 / U) * (w % U);
+      const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + k0;
       if constexpr (VEC) {
         double2 xv = *reinterpret_cast<const double2*>(P.v.x + dof);
         if constexpr (MODE == MF_PCG) {
           const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
           xv.x *= hv.x, xv.y *= hv.y;
         }
-        *reinterpret_cast<double2*>(dst) = xv;
+        *reinterpret_cast<double2*>(&sp[r][col + 1][k0]) = xv;
       } else {
-        dst[0] = src<MODE>(P, dof);
+        double xv = P.v.x[dof];
+        if constexpr (MODE == MF_PCG) {
+          const double hv = P.v.dh[dof];
+          xv *= hv;
+        }
+        sp[r][col + 1][k0] = xv;
       }
     }
     __syncthreads();
   }
+  Coefs K;
   const double* __restrict__ R = ref + int64_t(inst) * P.ref_stride;
+  DgGeom g;
+  g.h[0] = R[0], g.h[1] = R[1], g.ih[0] = R[3], g.ih[1] = R[2];
   const uint32_t c = ey * nx + ex;
   const int64_t e = e0 + c;
+  K.e = e;
   const bool has[4] = {row && ex > 0u, row && ex + 1u < nx, row && ey > 0u, row && ey + 1u < ny};
-  double a[4], bv[4][4], y[4] = {0., 0., 0., 0.};
+  double y[4] = {0., 0., 0., 0.};
+  if constexpr (DIAG) {
+    // diagonal: column k of the element's own block, entry k
+    const double z[4][4] = {};
+    if (row)
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    a[k] = (!DIAG && row) ? sp[ty + 1][tx + 1][k] : 0.;
-    bv[0][k] = (!DIAG && has[0]) ? sp[ty + 1][tx][k] : 0.;
-    bv[1][k] = (!DIAG && has[1]) ? sp[ty + 1][tx + 2][k] : 0.;
-    bv[2][k] = (!DIAG && has[2]) ? sp[ty][tx + 1][k] : 0.;
-    bv[3][k] = (!DIAG && has[3]) ? sp[ty + 2][tx + 1][k] : 0.;
-  }
-  if (row) {
+      for (int k = 0; k < 4; ++k) {
+        double a[4] = {0., 0., 0., 0.}, yk[4] = {0., 0., 0., 0.};
+        a[k] = 1.;
+        dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, nx, has, g, a, z, yk);
+        y[k] = yk[k];
+      }
+  } else {
+    double a[4], bv[4][4];
 #pragma unroll
-    for (int f = 0; f < nef; ++f)
-      mv<4, false, DIAG>(R + P.eoff + 16 * f, P.escale[f] * cf(P.ecoef[f], P.evalue[f], e), a, y);
-    if (has[0])
-      dg_side<0, MODE, NCF>(P, R, ncf, e, e - 1, a, bv[0], y);
-    else
-      dg_boundary<0, MODE, NBF>(P, R, nbf, e, a, y);
-    if (has[1])
-      dg_side<1, MODE, NCF>(P, R, ncf, e, e + 1, a, bv[1], y);
-    else
-      dg_boundary<1, MODE, NBF>(P, R, nbf, e, a, y);
-    if (has[2])
-      dg_side<2, MODE, NCF>(P, R, ncf, e, e - nx, a, bv[2], y);
-    else
-      dg_boundary<2, MODE, NBF>(P, R, nbf, e, a, y);
-    if (has[3])
-      dg_side<3, MODE, NCF>(P, R, ncf, e, e + nx, a, bv[3], y);
-    else
-      dg_boundary<3, MODE, NBF>(P, R, nbf, e, a, y);
+    for (int k = 0; k < 4; ++k) {
+      a[k] = row ? sp[ty + 1][tx + 1][k] : 0.;
+      bv[0][k] = has[0] ? sp[ty + 1][tx][k] : 0.;
+      bv[1][k] = has[1] ? sp[ty + 1][tx + 2][k] : 0.;
+      bv[2][k] = has[2] ? sp[ty][tx + 1][k] : 0.;
+      bv[3][k] = has[3] ? sp[ty + 2][tx + 1][k] : 0.;
+    }
+    if (row)
+      dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, nx, has, g, a, bv, y);
   }
   double part = 0.;
   if (row) {
@@ -519,7 +574,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
     if constexpr (MODE == MF_APPLY) {
       store4<VEC>(P.v.y, i0, y);
     } else if constexpr (MODE == MF_PCG) {
-      // q = D^{-1/2} y; p.q = (D^{-1/2} p).y
+      // q = D^{-1/2} y; p.q
       double xo[4], ho[4];
       load4<MODE, VEC>(P, i0, xo, ho);
       double q[4];
@@ -655,19 +710,20 @@ void mf_setup(Operator& op)
       unsupported("the constrained DoFs are not the boundary vertices");
   }
 
-  // reference matrices per instance, with the assembly kernels' quadrature rules (element rules of
-  // order 0 + 1 + 1 + over_integrate, face rules likewise: integrals.hh:115, laplace.hh:74-79,
-  // product.hh:88-100, laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
-  const int nef = int(op.eforms.size()), ncf = int(op.cforms.size()), nbf = int(op.bforms.size());
-  const int64_t eoff = 4, coff = eoff + int64_t(nef) * NV * NV, boff = coff + int64_t(ncf) * 2 * 96;
-  const int64_t stride = boff + int64_t(nbf) * 4 * 16;
+  // per instance: header h0, h1, 1 / h1, 1 / h0; continuous spaces also the reference element matrices,
+  // with the assembly kernels' quadrature rules (order 0 + 1 + 1 + over_integrate: integrals.hh:115,
+  // laplace.hh:74-79, product.hh:88-100).  Discontinuous spaces apply their forms in closed form.
+  const int nef = int(op.eforms.size());
+  const int64_t eoff = 4;
+  const int64_t stride = dg ? 4 : eoff + int64_t(nef) * NV * NV;
   std::vector<double> ref(size_t(stride) * ninst, 0.);
   for (int in = 0; in < ninst; ++in) {
     const Lattice& L = g.lattice[in];
     double* r = &ref[size_t(in) * stride];
     const double h[3] = {L.h[0], L.h[1], D == 3 ? L.h[2] : 1.};
-    // header: h0, h1 and the inverse face lengths 1 / h1 (faces normal to x), 1 / h0 (normal to y)
     r[0] = h[0], r[1] = h[1], r[2] = 1. / h[1], r[3] = 1. / h[0];
+    if (dg)
+      continue;
     const double vol = h[0] * h[1] * (D == 3 ? h[2] : 1.);
     for (int f = 0; f < nef; ++f) {
       const auto& E = op.eforms[f];
@@ -689,107 +745,6 @@ void mf_setup(Operator& op)
           }
       }
     }
-    // faces: orientation o (0: normal +x, 1: normal +y); the inside element holds the face at reference
-    // coordinate 1 in direction o, the outside element at 0; t runs along the other direction
-    for (int f = 0; f < ncf; ++f) {
-      const auto& C = op.cforms[f];
-      const Quadrature q = line_rule(0 + 1 + 1 + C.over_integrate);
-      for (int o = 0; o < 2; ++o) {
-        // face integrals G_ii, G_io, G_oi, G_oo (coupling) or M_ii, M_io, M_oo (penalty)
-        double I[4][16] = {};
-        const double flen = h[1 - o];
-        for (int p = 0; p < q.size(); ++p) {
-          double xin[2], xout[2];
-          xin[o] = 1., xout[o] = 0., xin[1 - o] = xout[1 - o] = q.x[p];
-          double pin[8], pout[8], din[8][3], dout[8][3];
-          q1_shape(2, xin, pin, din);
-          q1_shape(2, xout, pout, dout);
-          const double w = flen * q.w[p];
-          for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) {
-              if (C.kind == FF_INNER_COUPLING) {
-                const double gin_j = din[j][o] / h[o], gout_j = dout[j][o] / h[o];
-                I[0][i * 4 + j] += w * (gin_j * pin[i]);
-                I[1][i * 4 + j] += w * (gout_j * pin[i]);
-                I[2][i * 4 + j] += w * (gin_j * pout[i]);
-                I[3][i * 4 + j] += w * (gout_j * pout[i]);
-              } else {
-                I[0][i * 4 + j] += w * (pin[j] * pin[i]);
-                I[1][i * 4 + j] += w * (pout[j] * pin[i]);
-                I[3][i * 4 + j] += w * (pout[j] * pout[i]);
-              }
-            }
-        }
-        double* G = r + coff + int64_t(2 * f + o) * 96;
-        const double s_ = C.param;
-        for (int i = 0; i < 4; ++i)
-          for (int j = 0; j < 4; ++j) {
-            const int ij = i * 4 + j, ji = j * 4 + i;
-            if (C.kind == FF_INNER_COUPLING) {
-              G[0 + ij] = I[0][ij] + s_ * I[0][ji]; // A_in
-              G[16 + ij] = I[1][ij];                // G_io
-              G[32 + ij] = I[2][ji];                // G_oi^T
-              G[48 + ij] = I[3][ij] + s_ * I[3][ji]; // A_out
-              G[64 + ij] = I[2][ij];                // G_oi
-              G[80 + ij] = I[1][ji];                // G_io^T
-            } else {
-              G[0 + ij] = I[0][ij];  // M_ii
-              G[16 + ij] = I[1][ij]; // M_io
-              G[32 + ij] = I[1][ji]; // M_io^T
-              G[48 + ij] = I[3][ij]; // M_oo
-            }
-          }
-      }
-    }
-    // boundary sides 0 x-, 1 x+, 2 y-, 3 y+ with the outward normal
-    for (int f = 0; f < nbf; ++f) {
-      const auto& Bd = op.bforms[f];
-      const Quadrature q = line_rule(0 + 1 + 1 + Bd.over_integrate);
-      for (int side = 0; side < 4; ++side) {
-        const int o = side >> 1;
-        const double sign = (side & 1) ? 1. : -1.;
-        double I[16] = {};
-        const double flen = h[1 - o];
-        for (int p = 0; p < q.size(); ++p) {
-          double x[2];
-          x[o] = (side & 1) ? 1. : 0., x[1 - o] = q.x[p];
-          double phi[8], dphi[8][3];
-          q1_shape(2, x, phi, dphi);
-          const double w = flen * q.w[p];
-          for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j)
-              I[i * 4 + j] += (Bd.kind == FF_DIRICHLET_COUPLING) ? w * ((sign * dphi[j][o] / h[o]) * phi[i])
-                                                                : w * (phi[j] * phi[i]);
-        }
-        // coupling: G + s G^T (G_ij = int (grad phi_j . n) phi_i); penalty: M
-        double* B = r + boff + int64_t(4 * f + side) * 16;
-        for (int i = 0; i < 4; ++i)
-          for (int j = 0; j < 4; ++j)
-            B[i * 4 + j] = (Bd.kind == FF_DIRICHLET_COUPLING) ? I[i * 4 + j] + Bd.param * I[j * 4 + i] : I[i * 4 + j];
-      }
-    }
-  }
-  // the kernels read only the structurally nonzero entries of the face matrices: the others must vanish
-  for (int in = 0; in < ninst && dg; ++in) {
-    const double* r = &ref[size_t(in) * stride];
-    for (int f = 0; f < ncf; ++f)
-      for (int o = 0; o < 2; ++o)
-        for (int slot = 0; slot < 6; ++slot) {
-          const bool coupling = op.cforms[f].kind == FF_INNER_COUPLING;
-          if (!coupling && slot >= 4)
-            continue;
-          const uint32_t m = coupling ? coupling_mask(o, slot) : penalty_mask(o, slot);
-          for (int t = 0; t < 16; ++t)
-            if (!((m >> t) & 1u) && r[coff + (2 * f + o) * 96 + slot * 16 + t] != 0.)
-              HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free set-up: face matrix entry outside the Q1 face pattern");
-        }
-    for (int f = 0; f < nbf; ++f)
-      for (int sd = 0; sd < 4; ++sd) {
-        const uint32_t m = mask_of(boundary_dofs(sd), boundary_dofs(sd), op.bforms[f].kind == FF_DIRICHLET_COUPLING);
-        for (int t = 0; t < 16; ++t)
-          if (!((m >> t) & 1u) && r[boff + (4 * f + sd) * 16 + t] != 0.)
-            HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free set-up: boundary matrix entry outside the Q1 face pattern");
-      }
   }
   F.mf_kind = dg ? MFK_DG_QUAD : (D == 2 ? MFK_CG_QUAD : MFK_CG_HEX);
   F.mf_ref_stride = stride;
@@ -857,15 +812,12 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
   P.ninst = int(op.space->inst_dof.size()) - 1;
   P.blk = F.d_mf_blk.ptr;
   P.ref_stride = F.mf_ref_stride;
-  const int D = op.space->grid->ref.dim, NV = 1 << D;
   P.eoff = 4;
-  P.coff = P.eoff + int64_t(op.eforms.size()) * NV * NV;
-  P.boff = P.coff + int64_t(op.cforms.size()) * 2 * 96;
   P.dirichlet = op.dirichlet ? 1 : 0;
   P.nef = int(op.eforms.size());
   for (int f = 0; f < P.nef; ++f) {
     const auto& E = op.eforms[f];
-    P.ecoef[f] = E.field, P.evalue[f] = E.value, P.escale[f] = E.scaling;
+    P.ekind[f] = E.kind, P.ecoef[f] = E.field, P.evalue[f] = E.value, P.escale[f] = E.scaling;
   }
   P.ncf = int(op.cforms.size()), P.nbf = int(op.bforms.size());
   for (int grp = 0; grp < 2; ++grp) {
