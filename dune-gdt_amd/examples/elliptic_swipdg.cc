@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
+#include <string>
 
 #include "hgdt.hh"
 
@@ -65,6 +66,9 @@ int main(int argc, char* argv[])
                     {},
                     ApplyOn::CustomBoundaryIntersections(boundary_info));
     lhs_op.append(rhs_func);
+    // third argument "matrix-free": no stored matrix, the operator is applied from the coefficients
+    if (argc > 3 && std::string(argv[3]) == "matrix-free")
+      lhs_op.set_format(Format::matrix_free);
     lhs_op.assemble(/*use_tbb=*/true);
 
     auto solution = make_discrete_function(dg_space);

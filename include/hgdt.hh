@@ -728,6 +728,15 @@ private:
 
 class MatrixOperator;
 
+// storage of an operator's apply / apply_inverse (HGDT_FORMAT_*)
+enum class Format {
+  automatic = HGDT_FORMAT_AUTO,
+  sell = HGDT_FORMAT_SELL,
+  dia = HGDT_FORMAT_DIA,
+  dia_nonsymmetric = HGDT_FORMAT_DIA_NONSYMMETRIC,
+  matrix_free = HGDT_FORMAT_MATRIX_FREE
+};
+
 class VectorFunctional
 {
 public:
@@ -960,6 +969,14 @@ public:
     return append(what);
   }
 
+  // storage used by apply / apply_inverse; Format::matrix_free stores no matrix and recomputes the operator
+  // from the coefficients at every application, as LocalizableOperator::apply does
+  // (operators/localizable-operator.hh:352-379); assemble() then builds the right-hand side only
+  MatrixOperator& set_format(Format format)
+  {
+    check(hgdt_operator_set_format(handle(), int(format)));
+    return *this;
+  }
   // assemble(use_tbb), matrix-based.hh:496-500; asynchronous on `stream` (NULL: default stream)
   void assemble(bool /*use_tbb*/ = false, void* stream = nullptr)
   {

@@ -75,16 +75,17 @@ def _exact(X):
 
 
 @pytest.mark.gpu
-def test_swipdg_example_with_dirichlet_data_matches_oracle():
+@pytest.mark.parametrize("mode", ["assembled", "matrix-free"])
+def test_swipdg_example_with_dirichlet_data_matches_oracle(mode):
     """dune-gdt_amd/examples/elliptic_swipdg.cc: SWIPDG with nonzero Dirichlet data through boundary
-    functionals; solution parity with the oracle (1e-10), H1-semi error parity (1e-12) and first-order
-    convergence under refinement"""
+    functionals, with the assembled matrix or matrix-free; solution parity with the oracle (1e-10), H1-semi
+    error parity (1e-12) and first-order convergence under refinement"""
     errs = []
     for n in (16, 32):
-        print(f"case C++ example elliptic_swipdg n={n}", flush=True)
+        print(f"case C++ example elliptic_swipdg n={n} {mode}", flush=True)
         with tempfile.TemporaryDirectory() as d:
             out = os.path.join(d, "u.bin")
-            r = subprocess.run([SWIPDG, str(n), out], capture_output=True, text=True, timeout=120)
+            r = subprocess.run([SWIPDG, str(n), out, mode], capture_output=True, text=True, timeout=120)
             assert r.returncode == 0, r.stderr
             res = json.loads(r.stdout.strip().splitlines()[-1])
             u = np.fromfile(out, dtype=np.float64)
