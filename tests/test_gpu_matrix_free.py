@@ -90,6 +90,48 @@ def test_swipdg_variants_apply(hgdt_mod, symmetry):
     assert err <= 1e-12
 
 
+def test_dg_generic_forms_matrix_free(hgdt_mod):
+    """the general form set (runtime form counts): a scaled product element form, omega different from
+    kappa, scaled and over-integrated intersection forms; apply against the oracle's assembled operator"""
+    H = hgdt_mod
+    print("case matrix-free DG generic forms", flush=True)
+    cells, hi = [19, 13], [1.3, 0.7]
+    ne = cells[0] * cells[1]
+    kappa = H.fields.lognormal(43, ne)
+    omega = H.fields.uniform(44, ne) + 0.25
+    w = H.fields.uniform(45, ne) + 0.5
+    tk, to, tw = (torch.from_numpy(v).cuda() for v in (kappa, omega, w))
+    grid = H.Grid.structured("quad", [0, 0], hi, cells)
+    space = H.DiscontinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(tk)))
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(tw)), {"matrixoperator.scaling": 0.3})
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalLaplaceIPDGInnerCoupling(1.0, tk, to),
+                                                              over_integrate=1),
+              {"matrixoperator.scaling": 0.5}, H.ApplyOn.InnerIntersectionsOnce())
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(12.0, to)),
+              {"matrixoperator.scaling": 2.0}, H.ApplyOn.InnerIntersectionsOnce())
+    boundary = H.ApplyOn.CustomBoundaryIntersections(H.AllDirichletBoundaryInfo())
+    op.append(H.LocalIntersectionIntegralBilinearForm(H.LocalLaplaceIPDGDirichletCoupling(1.0, tk), over_integrate=2),
+              {}, boundary)
+    op.append(H.LocalIntersectionIntegralBilinearForm(H.LocalIPDGBoundaryPenalty(20.0, to)), {}, boundary)
+    op.set_format("matrix_free")
+    op.assemble()
+    m = O.Mesh.structured(O.QUAD, cells, [0, 0], hi)
+    s = O.Space(m, False, 1)
+    oop = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    oop.append_laplace(kappa)
+    oop.append_product(w, scaling=0.3)
+    oop.append_inner_coupling(1.0, kappa, omega, over_integrate=1, scaling=0.5)
+    oop.append_inner_penalty(12.0, omega, scaling=2.0)
+    oop.append_dirichlet_coupling(1.0, kappa, over_integrate=2)
+    oop.append_boundary_penalty(20.0, omega)
+    oop.assemble(1)
+    err = _apply_error(H, op, _csr(oop), 46)
+    print(f"  apply row-relative error {err:.2e}")
+    assert err <= 1e-12
+
+
 def test_matrix_free_unaligned_vectors(hgdt_mod):
     """source and range at an 8-byte offset (no 16-byte accesses): the same bits as aligned vectors"""
     H = hgdt_mod
