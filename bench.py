@@ -292,7 +292,9 @@ def main():
     ap.add_argument("--format", default="auto", choices=["auto", "matrix_free", "dia_symmetric", "dia", "sell64"],
                     help="operator storage of apply_inverse (matrix_free: recomputed from the coefficients at every "
                          "application; the assembly then builds the right-hand side only)")
-    ap.add_argument("--spmv-sample", type=int, default=8,
+    ap.add_argument("--check-every", type=int, default=16,
+                    help="PCG iterations between host-side convergence checks")
+    ap.add_argument("--spmv-sample", type=int, default=32,
                     help="HIP events around the operator application of every k-th PCG iteration of the timed steps")
     args = ap.parse_args()
     if args.no_tune:
@@ -346,7 +348,7 @@ def main():
     def step(time_spmv):
         op.assemble()
         return op.apply_inverse("rhs", use_graph=args.graph and not time_spmv, time_spmv=time_spmv,
-                                check_every=16)
+                                check_every=args.check_every)
 
     for w in range(args.warmup):
         step(False)
