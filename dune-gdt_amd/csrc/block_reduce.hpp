@@ -16,6 +16,13 @@ __device__ __forceinline__ int logical_block(int nb)
   return L < nb ? L : -1;
 }
 
+// logical block of a launch over a list of active blocks (NULL: all blocks 0 .. nb - 1)
+__device__ __forceinline__ int active_block(int nb, const int32_t* __restrict__ list)
+{
+  const int L = logical_block(nb);
+  return (L < 0 || !list) ? L : list[L];
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

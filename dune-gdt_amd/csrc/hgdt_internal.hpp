@@ -362,6 +362,8 @@ struct MfVectors {
   double* part = nullptr;     // MF_PCG: p.q per row block
   const PcgScalars* S = nullptr;
   int par = 0;
+  const int32_t* list = nullptr; // MF_PCG: blocks of unfinished instances (nb of them; NULL: all blocks)
+  int nb = -1;
   double* out_dh = nullptr;  // MF_DIAG: D^{-1/2}, D^{1/2} and the non-positive-diagonal flag
   double* out_dsq = nullptr;
   int* flags = nullptr;

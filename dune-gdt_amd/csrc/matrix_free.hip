@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
   constexpr int NV = 1 << D, NS = (D == 2) ? 9 : 27;
   constexpr bool DIAG = (MODE == MF_DIAG);
   constexpr int EC = NEF > 0 ? NEF : 1;
-  const int b = logical_block(P.nb);
+  const int b = active_block(P.nb, P.v.list);
   if (b < 0)
     return;
   const int64_t* __restrict__ B = P.blk + 8 * int64_t(b);
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   constexpr bool DIAG = (MODE == MF_DIAG);
   __shared__ __align__(16) double sp[TY + 2][TX + 2][4]; // source of the tile and its halo
   const int nef = NEF >= 0 ? NEF : P.nef, ncf = NCF >= 0 ? NCF : P.ncf, nbf = NBF >= 0 ? NBF : P.nbf;
-  const int b = logical_block(P.nb);
+  const int b = active_block(P.nb, P.v.list);
   if (b < 0)
     return;
   const int64_t* __restrict__ Bk = P.blk + 8 * int64_t(b);
@@ -829,9 +829,9 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
       P.fomega[grp][f] = Fm.omega, P.fov[grp][f] = Fm.omega_value;
     }
   }
-  P.nb = int(F.A.nblocks);
+  P.nb = v.nb >= 0 ? v.nb : int(F.A.nblocks);
   P.v = v;
-  const unsigned grid = hw_grid(F.A.nblocks);
+  const unsigned grid = hw_grid(P.nb);
   const int block = F.A.block; // one row / element per lane (the partition is built so)
   const bool ok = (F.mf_kind == MFK_DG_QUAD)
                       ? (block == TX * TY && F.A.items == 4 && int64_t(F.A.nblocks) * TX * TY * 4 >= op.n)

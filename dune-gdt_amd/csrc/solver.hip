@@ -65,6 +65,9 @@ struct PcgParams {
   const int64_t* v_row0;
   const int32_t* v_inst;
   const int32_t* v_ib0;
+  // blocks of unfinished instances (NULL: every block); a_nb / v_nb are then the list lengths
+  const int32_t* a_list;
+  const int32_t* v_list;
   const int64_t* inst_row0;
   // DIA
   int ndiag;
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(256) k_cg_init_scalars(PcgParams P)
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
 {
-  const int b = logical_block(P.v_nb);
+  const int b = active_block(P.v_nb, P.v_list);
   if (b < 0)
     return;
   const int inst = P.v_inst[b];
@@ -433,7 +436,7 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
 template <int FMT, int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 {
-  const int b = logical_block(P.a_nb);
+  const int b = active_block(P.a_nb, P.a_list);
   if (b < 0)
     return;
   const int inst = P.a_inst[b];
@@ -466,7 +469,7 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 {
-  const int b = logical_block(P.v_nb);
+  const int b = active_block(P.v_nb, P.v_list);
   if (b < 0)
     return;
   const int inst = P.v_inst[b];
@@ -675,6 +678,9 @@ inline unsigned hw_blocks(int64_t nb)
 
 // per-operator solver workspace that the header does not need to know about
 struct SolverCache {
+  // lists of the blocks of unfinished instances (two buffers: one may still be read by a queued chunk)
+  DeviceArray<int32_t> active[2];
+  std::vector<int32_t> active_host[2];
   DeviceArray<int64_t> block_slice0;
   DeviceArray<int32_t> dia_k0;
   DeviceArray<int32_t> dia_mirror;
@@ -1146,15 +1152,16 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   const int kf = kernel_format(op);
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    launch_pcg(k_cg_direction<I>, hw_blocks(V.nblocks), V.block, stream, P, par);
+    launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
   });
   if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
     MfVectors v;
     v.x = P.p, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
+    v.list = P.a_list, v.nb = P.a_nb;
     mf_launch(op, MF_PCG, v, stream, ev0, ev1);
   } else dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    const unsigned grid = hw_blocks(A.nblocks);
+    const unsigned grid = hw_blocks(P.a_nb);
     if (kf == FMT_DIA)
       launch_pcg(k_cg_spmv<FMT_DIA, I>, grid, A.block, stream, P, par, ev0, ev1);
     else if (kf == FMT_DIAS)
@@ -1164,7 +1171,7 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   });
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    launch_pcg(k_cg_update<I>, hw_blocks(V.nblocks), V.block, stream, P, par);
+    launch_pcg(k_cg_update<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
   });
 }
 
@@ -1200,7 +1207,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     HGDT_HIP(hipEventCreate(&C.t1));
   // graphs cannot be captured on the legacy default stream: run on an owned stream joined to the caller's
   hipStream_t stream = user_stream;
-  const bool graph = opt.use_graph && !opt.time_spmv;
+  bool graph = opt.use_graph && !opt.time_spmv;
   if (graph) {
     if (!C.own_stream)
       HGDT_HIP(hipStreamCreateWithFlags(&C.own_stream, hipStreamNonBlocking));
@@ -1281,6 +1288,43 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
         return false;
     return true;
   };
+  // Batches: once at least 15 % of the launched blocks belong to finished instances, later chunks are
+  // launched over the blocks of the unfinished instances only (lists read through active_block), eagerly.
+  // An instance found finished after chunk k did its last step in chunk k and the copy of its record to
+  // the other parity in chunk k + 1 (queued before this check), so its blocks have nothing left to do from
+  // chunk k + 2 on, the first chunk launched with the new lists.
+  int slot = 0;
+  int64_t compactions = 0;
+  auto compact = [&](const PcgScalars* h) {
+    const RowPartition& A = fmt.A;
+    const RowPartition& V = fmt.V;
+    int64_t na = 0, nv = 0;
+    for (int i = 0; i < ninst; ++i)
+      if (h[i].done == 0) {
+        na += A.inst_block0[i + 1] - A.inst_block0[i];
+        nv += V.inst_block0[i + 1] - V.inst_block0[i];
+      }
+    if (!(na < 0.85 * P.a_nb || nv < 0.85 * P.v_nb) || na == 0 || nv == 0)
+      return;
+    slot ^= 1;
+    std::vector<int32_t>& L = C.active_host[slot];
+    L.clear();
+    for (int i = 0; i < ninst; ++i)
+      if (h[i].done == 0)
+        for (int32_t b = A.inst_block0[i]; b < A.inst_block0[i + 1]; ++b)
+          L.push_back(b);
+    for (int i = 0; i < ninst; ++i)
+      if (h[i].done == 0)
+        for (int32_t b = V.inst_block0[i]; b < V.inst_block0[i + 1]; ++b)
+          L.push_back(b);
+    DeviceArray<int32_t>& D = C.active[slot];
+    D.allocate(size_t(A.nblocks + V.nblocks));
+    HGDT_HIP(hipMemcpyAsync(D.ptr, L.data(), L.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    P.a_list = D.ptr, P.a_nb = int(na);
+    P.v_list = D.ptr + na, P.v_nb = int(nv);
+    graph = false;
+    ++compactions;
+  };
   // double-buffered convergence checks: chunk k+1 is queued before the flags of chunk k are read
   const int64_t max_chunks = (opt.max_iter + 1 + chunk - 1) / chunk + 1;
   int cur = 0;
@@ -1309,6 +1353,8 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
       HGDT_HIP(hipEventSynchronize(C.copied[cur]));
       break;
     }
+    if (ninst > 1)
+      compact(C.host_scalars[cur]);
     cur = nxt;
   }
   hipLaunchKernelGGL(k_cg_finish, dim3(blocks_for(n, 256)), dim3(256), 0, stream, n, fmt.d_dh.ptr, x);
@@ -1351,6 +1397,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     st.spmv_count = cnt;
   }
   st.step_bytes = fmt.step_bytes;
+  st.compactions = compactions;
   if (!st.converged) {
     if (st.breakdown)
       HGDT_THROW(HGDT_ERR_SOLVER_FAILED, "Jacobi-PCG breakdown: p.Ap <= 0 (matrix not SPD)");

@@ -117,6 +117,7 @@ class SolverStats(ctypes.Structure):
         ("spmv_count", ctypes.c_int64),
         ("solve_ms", ctypes.c_double),
         ("step_bytes", ctypes.c_int64),
+        ("compactions", ctypes.c_int64),
     ]
 
     def as_dict(self):

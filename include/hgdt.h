@@ -291,6 +291,8 @@ typedef struct hgdt_solver_stats {
   double solve_ms;        /* wall time of the solve measured with HIP events */
   int64_t step_bytes;     /* algorithmic bytes of the solver's operator application (the kernel timed
                              by time_spmv) */
+  int64_t compactions;    /* batches: times the launches were narrowed to the blocks of the instances
+                             still iterating */
 } hgdt_solver_stats;
 
 hgdt_status hgdt_solver_options_default(hgdt_solver_options* options);

@@ -145,6 +145,7 @@ def test_ragged_batch_with_tiny_instances(hgdt_mod):
     its, rels = op.instance_stats()
     eoff, doff = union.instance_elements(), space.instance_dofs()
     assert its[0] == 0  # the single cell is all Dirichlet
+    assert op.last_solve["compactions"] >= 1
     for i, n in enumerate(sizes):
         s1, op1 = _heat(H, parts[i], kappa=torch.from_numpy(kappa[eoff[i]:eoff[i + 1]]).cuda())
         for k in (3, 4):
