@@ -103,11 +103,12 @@ struct PcgParams {
 // sum of an instance's block partials (written by an earlier kernel) in block order
 // (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
 // partials are issued together, so the prologue costs one memory round trip per U * B partials:
-// one round trip for up to 8192 single partials with B = 256)
+// one round trip for up to 2048 single partials with B = 256)
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
 {
-  constexpr int U = 32 / NV;
+  constexpr int U = NV >= 8 ? 1 : 8 / NV; // partials per thread and trip: few registers, since callers keep
+                                          // their rows' loads in flight across the reduction
   const int B = blockDim.x;
 #pragma unroll
   for (int k = 0; k < NV; ++k)
@@ -386,12 +387,10 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
   }
   const int64_t it = cur->iters;
   const double rho_prev = cur->rz, alpha = cur->alpha, tol2 = cur->tol2;
-  double t[2];
-  reduce_partials<2>(P.part2, P.v_ib0[inst], P.v_ib0[inst + 1], t);
-  const int status = (t[1] <= tol2) ? 1 : (it >= P.max_iter) ? 2 : 0;
   const bool first = (it == 0);
-  const double beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
   const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+  // the block's rows are loaded before the scalars are reduced, so the loads overlap the reduction
+  // (r is read also at the step that finds the instance converged, and then not used)
   double pv[ITEMS], xv[ITEMS], rv[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -399,8 +398,12 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
     const bool ok = i < r1;
     pv[j] = (ok && !first) ? P.p[i] : 0.;
     xv[j] = (ok && !first) ? P.x[i] : 0.;
-    rv[j] = (ok && status == 0) ? P.r[i] : 0.;
+    rv[j] = ok ? P.r[i] : 0.;
   }
+  double t[2];
+  reduce_partials<2>(P.part2, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  const int status = (t[1] <= tol2) ? 1 : (it >= P.max_iter) ? 2 : 0;
+  const double beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
   if (!first) {
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
@@ -477,6 +480,17 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
   if (S->done)
     return;
   const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+  // rows loaded before the reduction of p.q, which they then overlap
+  double qv[ITEMS], rv[ITEMS], sv[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+    const bool ok = i < r1;
+    qv[j] = ok ? P.q[i] : 0.;
+    rv[j] = ok ? P.r[i] : 0.;
+    sv[j] = ok ? P.dsq[i] : 0.;
+  }
   double pq[1];
   reduce_partials<1>(P.part1, P.a_ib0[inst], P.a_ib0[inst + 1], pq);
   if (!(pq[0] > 0.)) {
@@ -488,16 +502,6 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
   const double alpha = S->rz / pq[0];
   if (writer)
     S->alpha = alpha, S->pq = pq[0];
-  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-  double qv[ITEMS], rv[ITEMS], sv[ITEMS];
-#pragma unroll
-  for (int j = 0; j < ITEMS; ++j) {
-    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-    const bool ok = i < r1;
-    qv[j] = ok ? P.q[i] : 0.;
-    rv[j] = ok ? P.r[i] : 0.;
-    sv[j] = ok ? P.dsq[i] : 0.;
-  }
   double v[2] = {0., 0.};
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
