@@ -713,26 +713,48 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
 }
 
 // fixed-order gather of the staged local contributions into the CSR values
+// sum of staging[contrib[c]] for c = c0 .. c1 - 1, added in that order (the walk order); the loads of U
+// contributions are issued together so that a lane keeps U index / value pairs in flight
+template <int U>
+__device__ __forceinline__ double ordered_sum(const uint32_t* __restrict__ contrib, const double* __restrict__ staging,
+                                              uint32_t c0, uint32_t c1)
+{
+  double v = 0.;
+  for (uint32_t c = c0; c < c1; c += U) {
+    uint32_t k[U];
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      k[u] = (c + u < c1) ? contrib[c + u] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      x[u] = (c + u < c1) ? staging[k[u]] : 0.;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c + u < c1)
+        v += x[u];
+  }
+  return v;
+}
+
+// blocks are mapped XCD-aware (xcd_index): each XCD sums a contiguous range of slots, whose contributions
+// come from a compact set of elements, so the staged lines they share are found in that XCD's L2
 __global__ void __launch_bounds__(256) k_gather_values(int64_t nnz, const uint32_t* __restrict__ slot_ptr,
                                                        const uint32_t* __restrict__ contrib,
                                                        const double* __restrict__ staging,
                                                        const uint8_t* __restrict__ flag, double* __restrict__ values)
 {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_index(nnz);
   if (s >= nnz)
     return;
   const uint8_t fl = flag ? flag[s] : 0;
   double v;
-  if (fl == 1) {
+  if (fl == 1)
     v = 0.;
-  } else if (fl == 2) {
+  else if (fl == 2)
     v = 1.;
-  } else {
-    v = 0.;
-    const uint32_t c1 = slot_ptr[s + 1];
-    for (uint32_t c = slot_ptr[s]; c < c1; ++c)
-      v += staging[contrib[c]];
-  }
+  else
+    v = ordered_sum<8>(contrib, staging, slot_ptr[s], slot_ptr[s + 1]);
   values[s] = v;
 }
 
@@ -741,15 +763,12 @@ __global__ void __launch_bounds__(256) k_gather_rhs(int64_t n, const uint32_t* _
                                                     const double* __restrict__ staging,
                                                     const uint8_t* __restrict__ mask, double* __restrict__ rhs)
 {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_index(n);
   if (i >= n)
     return;
   double v = 0.;
-  if (!(mask && mask[i])) {
-    const uint32_t c1 = ptr[i + 1];
-    for (uint32_t c = ptr[i]; c < c1; ++c)
-      v += staging[contrib[c]];
-  }
+  if (!(mask && mask[i]))
+    v = ordered_sum<8>(contrib, staging, ptr[i], ptr[i + 1]);
   rhs[i] = v;
 }
 
@@ -1119,7 +1138,7 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
-    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(op.nnz, gb)), dim3(gb), 0, stream, op.nnz, M.d_slot_ptr.ptr,
+    hipLaunchKernelGGL(k_gather_values, dim3(xcd_grid(op.nnz, gb)), dim3(gb), 0, stream, op.nnz, M.d_slot_ptr.ptr,
                        M.d_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_slot_flag.ptr : nullptr,
                        op.d_values.ptr);
     HGDT_HIP(hipGetLastError());
@@ -1141,7 +1160,7 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     HGDT_HIP(hipGetLastError());
   }
   if (!op.sources.empty() || !op.bsources.empty()) {
-    hipLaunchKernelGGL(k_gather_rhs, dim3(grid_for(op.n, gb)), dim3(gb), 0, stream, op.n, M.d_rhs_ptr.ptr,
+    hipLaunchKernelGGL(k_gather_rhs, dim3(xcd_grid(op.n, gb)), dim3(gb), 0, stream, op.n, M.d_rhs_ptr.ptr,
                        M.d_rhs_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_dof_mask.ptr : nullptr,
                        op.d_rhs.ptr);
     HGDT_HIP(hipGetLastError());
