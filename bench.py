@@ -376,7 +376,8 @@ def main():
     solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
     # the solver's operator application: the storage format's SpMV, or the matrix-free kernel
     mf = fi["format"] == "matrix_free"
-    kernel = ({"c3": "k_mf_dg<1, 1, 2, 2, true>"}.get(args.config, "k_mf_cg<2, 1, 1>") if mf
+    # (discontinuous: the application with the solver's direction update fused in, mode 3)
+    kernel = ({"c3": "k_mf_dg<3, 1, 2, 2, true>"}.get(args.config, "k_mf_cg<2, 1, 1>") if mf
               else f"k_cg_spmv<{fi['format']}>")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"pmc_{'mf' if mf else 'cg_spmv'}_{args.config}.json")

@@ -62,5 +62,36 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
   __syncthreads();
 }
 
+// sum of an instance's block partials (written by an earlier kernel) in block order
+// (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
+// partials are issued together, so the prologue costs one memory round trip per U * B partials:
+// one round trip for up to 2048 single partials with B = 256)
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
+{
+  constexpr int U = NV >= 8 ? 1 : 8 / NV; // partials per thread and trip: few registers, since callers keep
+                                          // their rows' loads in flight across the reduction
+  const int B = blockDim.x;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    tot[k] = 0.;
+  for (int base = b0 + threadIdx.x; base < b1; base += U * B) {
+    double v[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int bb = base + u * B;
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        v[u][k] = (bb < b1) ? part[int64_t(bb) * NV + k] : 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        tot[k] += v[u][k];
+  }
+  block_sum<NV>(tot);
+}
+
 } // namespace
 } // namespace hgdt

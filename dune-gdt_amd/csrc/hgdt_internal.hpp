@@ -332,6 +332,7 @@ struct Operator {
   Launch launch[5];
   // solver workspace
   DeviceArray<double> d_r, d_p, d_q;
+  DeviceArray<double> d_p2; // second p buffer of the matrix-free solver with the fused direction update
   DeviceArray<double> d_partials; // 2 per block
   DeviceArray<PcgScalars> d_scalars;
   std::shared_ptr<SolverCache> solver_cache;
@@ -354,16 +355,26 @@ struct Operator {
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body);
 
 // matrix-free application (matrix_free.hip)
-enum { MF_APPLY = 0, MF_PCG = 1, MF_DIAG = 2 };
+// MF_PCG_DIR: the solver's direction update fused into its operator application (discontinuous kernel)
+enum { MF_APPLY = 0, MF_PCG = 1, MF_DIAG = 2, MF_PCG_DIR = 3 };
 struct MfVectors {
-  const double* x = nullptr; // MF_APPLY: x; MF_PCG: p
-  double* y = nullptr;       // MF_APPLY: y = A x; MF_PCG: q = D^{-1/2} A D^{-1/2} p
-  const double* dh = nullptr; // MF_PCG: D^{-1/2}
-  double* part = nullptr;     // MF_PCG: p.q per row block
-  const PcgScalars* S = nullptr;
+  const double* x = nullptr; // MF_APPLY: x; MF_PCG: p; MF_PCG_DIR: p of the previous step
+  double* y = nullptr;       // MF_APPLY: y = A x; MF_PCG(_DIR): q = D^{-1/2} A D^{-1/2} p
+  const double* dh = nullptr; // MF_PCG(_DIR): D^{-1/2}
+  double* part = nullptr;     // MF_PCG(_DIR): p.q per row block
+  PcgScalars* S = nullptr;    // MF_PCG_DIR writes the next record (first block of every instance)
   int par = 0;
-  const int32_t* list = nullptr; // MF_PCG: blocks of unfinished instances (nb of them; NULL: all blocks)
+  const int32_t* list = nullptr; // MF_PCG(_DIR): blocks of unfinished instances (nb of them; NULL: all)
   int nb = -1;
+  // MF_PCG_DIR: new p, scaled residual, scaled iterate (x += alpha p deferred from the previous step),
+  // partials of the previous update (2 per V-block), first V- and A-block of every instance, max_iter
+  double* p_out = nullptr;
+  const double* r = nullptr;
+  double* xh = nullptr;
+  const double* part2 = nullptr;
+  const int32_t* v_ib0 = nullptr;
+  const int32_t* a_ib0 = nullptr;
+  int64_t max_iter = 0;
   double* out_dh = nullptr;  // MF_DIAG: D^{-1/2}, D^{1/2} and the non-positive-diagonal flag
   double* out_dsq = nullptr;
   int* flags = nullptr;
@@ -375,7 +386,10 @@ void mf_setup(Operator& op);
 void mf_block_table(Operator& op);
 // tiles per instance of the discontinuous matrix-free kernel (empty when the grid has no lattices)
 std::vector<int64_t> mf_tile_counts(const Operator& op);
-int64_t mf_bytes(const Operator& op, bool scaled);
+// algorithmic bytes of one application in mode MF_APPLY, MF_PCG or MF_PCG_DIR
+int64_t mf_bytes(const Operator& op, int mode);
+// the solver fuses its direction update into the application (MF_PCG_DIR) for this operator
+bool mf_fused_direction(const Operator& op);
 void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t stream, hipEvent_t ev0 = nullptr,
                hipEvent_t ev1 = nullptr);
 

@@ -132,7 +132,7 @@ __device__ __forceinline__ void store_row(const MfParams& P, int64_t i, double y
 template <int MODE>
 __device__ __forceinline__ void finish_block(const MfParams& P, int b, double part)
 {
-  if constexpr (MODE == MF_PCG) {
+  if constexpr (MODE == MF_PCG || MODE == MF_PCG_DIR) {
     double v[1] = {part};
     block_sum<1>(v);
     if (threadIdx.x == 0)
@@ -161,6 +161,20 @@ __device__ __forceinline__ void load4(const MfParams& P, int64_t i, double* xo, 
       if constexpr (MODE == MF_PCG)
         ho[k] = P.v.dh[i + k];
     }
+  }
+}
+
+// the 4 entries i .. i + 3 of a (16-byte accesses when VEC)
+template <bool VEC>
+__device__ __forceinline__ void ld4(const double* __restrict__ a, int64_t i, double* o)
+{
+  if constexpr (VEC) {
+    const double2 a0 = reinterpret_cast<const double2*>(a + i)[0], a1 = reinterpret_cast<const double2*>(a + i)[1];
+    o[0] = a0.x, o[1] = a0.y, o[2] = a1.x, o[3] = a1.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = a[i + k];
   }
 }
 
@@ -487,6 +501,8 @@ template <int MODE, int NEF, int NCF, int NBF, bool VEC>
 __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __restrict__ ref)
 {
   constexpr bool DIAG = (MODE == MF_DIAG);
+  constexpr bool DIR = (MODE == MF_PCG_DIR);
+  constexpr bool SCALED = (MODE == MF_PCG || DIR);
   __shared__ __align__(16) double sp[TY + 2][TX + 2][4]; // source of the tile and its halo
   const int nef = NEF >= 0 ? NEF : P.nef, ncf = NCF >= 0 ? NCF : P.ncf, nbf = NBF >= 0 ? NBF : P.nbf;
   const int b = active_block(P.nb, P.v.list);
@@ -502,6 +518,56 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   const uint32_t tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const uint32_t ex = x0 + tx, ey = y0 + ty;
   const bool row = ex < nx && ey < ny;
+  const uint32_t c = ey * nx + ex;
+  const int64_t i0 = d0 + 4 * int64_t(c); // first DoF of the lane's element
+  // DIR: the direction update of the step (as k_cg_direction: stopping test on the previous update's
+  // partials, p = r + beta p, x += alpha p deferred from the previous step) on the block's elements.  The
+  // staged source is formed from r and the previous p, which live in another buffer than the new p, so a
+  // neighbouring block's writes never reach this block's halo.
+  double alpha = 0., beta = 0.;
+  bool first = false;
+  if constexpr (DIR) {
+    const PcgScalars* cur = P.v.S + P.v.par * P.ninst + inst;
+    PcgScalars* nxt = P.v.S + (P.v.par ^ 1) * P.ninst + inst;
+    const bool writer = (b == P.v.a_ib0[inst] && threadIdx.x == 0);
+    if (cur->done) {
+      if (writer)
+        *nxt = *cur;
+      return;
+    }
+    const int64_t it = cur->iters;
+    const double rho_prev = cur->rz, tol2 = cur->tol2;
+    alpha = cur->alpha;
+    first = (it == 0);
+    double t[2];
+    reduce_partials<2>(P.v.part2, P.v.v_ib0[inst], P.v.v_ib0[inst + 1], t);
+    const int status = (t[1] <= tol2) ? 1 : (it >= P.v.max_iter) ? 2 : 0;
+    beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
+    if (writer) {
+      nxt->bb = cur->bb;
+      nxt->tol2 = tol2;
+      nxt->alpha = alpha;
+      nxt->pq = cur->pq;
+      nxt->rr = t[1];
+      nxt->rz = t[0];
+      nxt->beta = beta;
+      nxt->iters = status ? it : it + 1;
+      nxt->done = status;
+    }
+    if (status != 0) {
+      // the instance stops here: only the iterate's deferred update remains (uniform over the block)
+      if (row && !first) {
+        double pv[4], xv[4];
+        ld4<VEC>(P.v.x, i0, pv);
+        ld4<VEC>(P.v.xh, i0, xv);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          xv[k] = fma(alpha, pv[k], xv[k]);
+        store4<VEC>(P.v.xh, i0, xv);
+      }
+      return;
+    }
+  }
   if constexpr (!DIAG) {
     // staged columns x0 + xs .. x0 + xe - 1 of the rows y0 - 1 .. y0 + TY (clipped to the instance)
     const int xs = x0 > 0u ? -1 : 0;
@@ -518,15 +584,28 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       const int k0 = (4 / U) * (w % U);
       const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + k0;
       if constexpr (VEC) {
-        double2 xv = *reinterpret_cast<const double2*>(P.v.x + dof);
-        if constexpr (MODE == MF_PCG) {
+        double2 xv;
+        if constexpr (DIR) {
+          xv = *reinterpret_cast<const double2*>(P.v.r + dof);
+          if (!first) {
+            const double2 pv = *reinterpret_cast<const double2*>(P.v.x + dof);
+            xv.x = fma(beta, pv.x, xv.x), xv.y = fma(beta, pv.y, xv.y);
+          }
+        } else {
+          xv = *reinterpret_cast<const double2*>(P.v.x + dof);
+        }
+        if constexpr (SCALED) {
           const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
           xv.x *= hv.x, xv.y *= hv.y;
         }
         *reinterpret_cast<double2*>(&sp[r][col + 1][k0]) = xv;
       } else {
-        double xv = P.v.x[dof];
-        if constexpr (MODE == MF_PCG) {
+        double xv;
+        if constexpr (DIR)
+          xv = first ? P.v.r[dof] : fma(beta, P.v.x[dof], P.v.r[dof]);
+        else
+          xv = P.v.x[dof];
+        if constexpr (SCALED) {
           const double hv = P.v.dh[dof];
           xv *= hv;
         }
@@ -539,7 +618,6 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   const double* __restrict__ R = ref + int64_t(inst) * P.ref_stride;
   DgGeom g;
   g.h[0] = R[0], g.h[1] = R[1], g.ih[0] = R[3], g.ih[1] = R[2];
-  const uint32_t c = ey * nx + ex;
   const int64_t e = e0 + c;
   K.e = e;
   const bool has[4] = {row && ex > 0u, row && ex + 1u < nx, row && ey > 0u, row && ey + 1u < ny};
@@ -570,13 +648,30 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   }
   double part = 0.;
   if (row) {
-    const int64_t i0 = d0 + 4 * int64_t(c);
     if constexpr (MODE == MF_APPLY) {
       store4<VEC>(P.v.y, i0, y);
-    } else if constexpr (MODE == MF_PCG) {
+    } else if constexpr (SCALED) {
       // q = D^{-1/2} y; p.q
       double xo[4], ho[4];
-      load4<MODE, VEC>(P, i0, xo, ho);
+      if constexpr (DIR) {
+        // the element's new p (the same operations as in the staging) and its deferred iterate update
+        ld4<VEC>(P.v.r, i0, xo);
+        ld4<VEC>(P.v.dh, i0, ho);
+        if (!first) {
+          double pv[4], xv[4];
+          ld4<VEC>(P.v.x, i0, pv);
+          ld4<VEC>(P.v.xh, i0, xv);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            xo[k] = fma(beta, pv[k], xo[k]);
+            xv[k] = fma(alpha, pv[k], xv[k]);
+          }
+          store4<VEC>(P.v.xh, i0, xv);
+        }
+        store4<VEC>(P.v.p_out, i0, xo);
+      } else {
+        load4<MODE, VEC>(P, i0, xo, ho);
+      }
       double q[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -800,9 +895,18 @@ std::vector<int64_t> mf_tile_counts(const Operator& op)
   return t;
 }
 
-int64_t mf_bytes(const Operator& op, bool scaled)
+int64_t mf_bytes(const Operator& op, int mode)
 {
-  return (scaled ? 24 : 16) * op.n + int64_t(8) * op.fmt.mf_fields * op.space->grid->ne;
+  // apply: x read, y written; solver: p, D^{-1/2} read, q written; with the direction update: r, p of the
+  // previous step, D^{-1/2}, the iterate read, p, the iterate, q written.  Plus one coefficient per element
+  // and distinct field.
+  const int64_t per_dof = mode == MF_APPLY ? 16 : mode == MF_PCG ? 24 : 56;
+  return per_dof * op.n + int64_t(8) * op.fmt.mf_fields * op.space->grid->ne;
+}
+
+bool mf_fused_direction(const Operator& op)
+{
+  return op.fmt.format == HGDT_FORMAT_MATRIX_FREE && op.fmt.mf_kind == MFK_DG_QUAD;
 }
 
 void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1)
@@ -839,37 +943,45 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
   if (!ok)
     HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free application: row partition does not match the kernels");
   const void* k = nullptr;
-  auto pick = [&](auto k0, auto k1, auto k2) {
+  auto pick = [&](auto k0, auto k1, auto k2, const void* k3) {
     k = mode == MF_APPLY ? reinterpret_cast<const void*>(k0)
-                         : mode == MF_PCG ? reinterpret_cast<const void*>(k1) : reinterpret_cast<const void*>(k2);
+        : mode == MF_PCG ? reinterpret_cast<const void*>(k1)
+        : mode == MF_DIAG ? reinterpret_cast<const void*>(k2)
+                          : k3;
   };
+  auto kv = [](auto kernel) { return reinterpret_cast<const void*>(kernel); };
   // compile-time form counts for the usual operators: one element form (Laplace), and for SWIPDG one
   // element form, coupling + penalty on the inner and on the boundary intersections
   // 16-byte vector accesses when every vector of the launch is 16-byte aligned
   auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-  const bool vec = aligned(v.x) && aligned(v.y) && aligned(v.dh);
+  const bool vec = aligned(v.x) && aligned(v.y) && aligned(v.dh) && aligned(v.r) && aligned(v.xh) &&
+                   aligned(v.p_out);
   const bool one = op.eforms.size() == 1;
   const bool swipdg = one && op.cforms.size() == 2 && op.bforms.size() == 2;
   if (F.mf_kind == MFK_CG_QUAD && one)
-    pick(k_mf_cg<2, MF_APPLY, 1>, k_mf_cg<2, MF_PCG, 1>, k_mf_cg<2, MF_DIAG, 1>);
+    pick(k_mf_cg<2, MF_APPLY, 1>, k_mf_cg<2, MF_PCG, 1>, k_mf_cg<2, MF_DIAG, 1>, nullptr);
   else if (F.mf_kind == MFK_CG_QUAD)
-    pick(k_mf_cg<2, MF_APPLY, -1>, k_mf_cg<2, MF_PCG, -1>, k_mf_cg<2, MF_DIAG, -1>);
+    pick(k_mf_cg<2, MF_APPLY, -1>, k_mf_cg<2, MF_PCG, -1>, k_mf_cg<2, MF_DIAG, -1>, nullptr);
   else if (F.mf_kind == MFK_CG_HEX && one)
-    pick(k_mf_cg<3, MF_APPLY, 1>, k_mf_cg<3, MF_PCG, 1>, k_mf_cg<3, MF_DIAG, 1>);
+    pick(k_mf_cg<3, MF_APPLY, 1>, k_mf_cg<3, MF_PCG, 1>, k_mf_cg<3, MF_DIAG, 1>, nullptr);
   else if (F.mf_kind == MFK_CG_HEX)
-    pick(k_mf_cg<3, MF_APPLY, -1>, k_mf_cg<3, MF_PCG, -1>, k_mf_cg<3, MF_DIAG, -1>);
+    pick(k_mf_cg<3, MF_APPLY, -1>, k_mf_cg<3, MF_PCG, -1>, k_mf_cg<3, MF_DIAG, -1>, nullptr);
   else if (F.mf_kind == MFK_DG_QUAD && swipdg && vec)
-    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true>, k_mf_dg<MF_PCG, 1, 2, 2, true>, k_mf_dg<MF_DIAG, 1, 2, 2, true>);
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true>, k_mf_dg<MF_PCG, 1, 2, 2, true>, k_mf_dg<MF_DIAG, 1, 2, 2, true>,
+         kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, true>));
   else if (F.mf_kind == MFK_DG_QUAD && swipdg)
-    pick(k_mf_dg<MF_APPLY, 1, 2, 2, false>, k_mf_dg<MF_PCG, 1, 2, 2, false>, k_mf_dg<MF_DIAG, 1, 2, 2, false>);
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, false>, k_mf_dg<MF_PCG, 1, 2, 2, false>, k_mf_dg<MF_DIAG, 1, 2, 2, false>,
+         kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, false>));
   else if (F.mf_kind == MFK_DG_QUAD && vec)
     pick(k_mf_dg<MF_APPLY, -1, -1, -1, true>, k_mf_dg<MF_PCG, -1, -1, -1, true>,
-         k_mf_dg<MF_DIAG, -1, -1, -1, true>);
+         k_mf_dg<MF_DIAG, -1, -1, -1, true>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, true>));
   else if (F.mf_kind == MFK_DG_QUAD)
     pick(k_mf_dg<MF_APPLY, -1, -1, -1, false>, k_mf_dg<MF_PCG, -1, -1, -1, false>,
-         k_mf_dg<MF_DIAG, -1, -1, -1, false>);
+         k_mf_dg<MF_DIAG, -1, -1, -1, false>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, false>));
   else
     HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free data not prepared");
+  if (!k)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free application: mode not provided for this operator");
   const double* ref = F.d_mf_ref.ptr;
   void* args[] = {&P, &ref};
   if (ev0)

@@ -89,6 +89,8 @@ struct PcgParams {
   double* x;
   double* r;
   double* p;
+  double* p2; // second p buffer (matrix-free solver with the fused direction update: the p formed at step
+              // j is written to p2 for even j, to p for odd j)
   double* q;
   double* part1; // 1 per A-block: p.q
   double* part2; // 2 per V-block: r.r of the scaled residual, r.r of the residual
@@ -99,37 +101,6 @@ struct PcgParams {
   int64_t max_iter;
   int zero_guess;
 };
-
-// sum of an instance's block partials (written by an earlier kernel) in block order
-// (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
-// partials are issued together, so the prologue costs one memory round trip per U * B partials:
-// one round trip for up to 2048 single partials with B = 256)
-template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
-{
-  constexpr int U = NV >= 8 ? 1 : 8 / NV; // partials per thread and trip: few registers, since callers keep
-                                          // their rows' loads in flight across the reduction
-  const int B = blockDim.x;
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-    tot[k] = 0.;
-  for (int base = b0 + threadIdx.x; base < b1; base += U * B) {
-    double v[U][NV];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int bb = base + u * B;
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-        v[u][k] = (bb < b1) ? part[int64_t(bb) * NV + k] : 0.;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-        tot[k] += v[u][k];
-  }
-  block_sum<NV>(tot);
-}
 
 // y_i = sum_j A_ij v(j) for the ITEMS rows r0 + threadIdx.x + j * blockDim.x (< r1) of block b, whose rows
 // start at rb (a block may process its rows in several chunks starting at r0).  `vals` is the
@@ -978,7 +949,7 @@ static int64_t apply_bytes(const Operator& op)
 {
   const SolverFormat& F = op.fmt;
   if (F.format == HGDT_FORMAT_MATRIX_FREE)
-    return mf_bytes(op, false);
+    return mf_bytes(op, MF_APPLY);
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -989,8 +960,8 @@ static int64_t apply_bytes(const Operator& op)
 static int64_t step_bytes(const Operator& op)
 {
   const SolverFormat& F = op.fmt;
-  if (F.format == HGDT_FORMAT_MATRIX_FREE)
-    return mf_bytes(op, true); // coefficients, D^{-1/2}, p read, q written
+  if (F.format == HGDT_FORMAT_MATRIX_FREE) // coefficients and vectors (with the direction update when fused)
+    return mf_bytes(op, mf_fused_direction(op) ? MF_PCG_DIR : MF_PCG);
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t((F.symmetric ? F.nup : F.ndiag) - 1) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -1154,15 +1125,23 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   const RowPartition& A = op.fmt.A;
   const RowPartition& V = op.fmt.V;
   const int kf = kernel_format(op);
-  dispatch_items(V.items, [&](auto IT) {
-    constexpr int I = decltype(IT)::value;
-    launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
-  });
+  const bool fused = mf_fused_direction(op);
+  if (!fused)
+    dispatch_items(V.items, [&](auto IT) {
+      constexpr int I = decltype(IT)::value;
+      launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
+    });
   if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
     MfVectors v;
     v.x = P.p, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
     v.list = P.a_list, v.nb = P.a_nb;
-    mf_launch(op, MF_PCG, v, stream, ev0, ev1);
+    if (fused) {
+      // the direction update inside the application: p of the previous step in one buffer, the new p in
+      // the other (the parity of the step picks them)
+      v.x = par ? P.p2 : P.p, v.p_out = par ? P.p : P.p2;
+      v.r = P.r, v.xh = P.x, v.part2 = P.part2, v.v_ib0 = P.v_ib0, v.a_ib0 = P.a_ib0, v.max_iter = P.max_iter;
+    }
+    mf_launch(op, fused ? MF_PCG_DIR : MF_PCG, v, stream, ev0, ev1);
   } else dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(P.a_nb);
@@ -1191,6 +1170,8 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   const int ninst = int(space->inst_dof.size()) - 1;
   d_r.allocate(std::max<int64_t>(n, 1));
   d_p.allocate(std::max<int64_t>(n, 1));
+  if (mf_fused_direction(*this))
+    d_p2.allocate(std::max<int64_t>(n, 1));
   d_q.allocate(std::max<int64_t>(n, 1));
   d_partials.allocate(std::max<int64_t>(fmt.A.nblocks + 3 * fmt.V.nblocks, 1));
   d_scalars.allocate(2 * ninst);
@@ -1222,7 +1203,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     stream = C.own_stream;
   }
   PcgParams P = make_params(*this);
-  P.b = b, P.x = x, P.r = d_r.ptr, P.p = d_p.ptr, P.q = d_q.ptr;
+  P.b = b, P.x = x, P.r = d_r.ptr, P.p = d_p.ptr, P.p2 = d_p2.ptr, P.q = d_q.ptr;
   P.part1 = d_partials.ptr;
   P.part2 = d_partials.ptr + fmt.A.nblocks;
   P.part3 = d_partials.ptr + fmt.A.nblocks + 2 * fmt.V.nblocks;
