@@ -47,7 +47,7 @@ namespace hgdt {
 namespace {
 
 constexpr int MF_MAXF = 8;
-constexpr int TX = 64, TY = 4; // tile of the discontinuous kernel (elements)
+constexpr int TX = 32, TY = 8; // tile of the discontinuous kernel (elements): halo rows 1.3x the tile
 enum { MFK_CG_QUAD = 0, MFK_CG_HEX = 1, MFK_DG_QUAD = 2 };
 
 struct MfParams {
@@ -504,6 +504,9 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   constexpr bool DIR = (MODE == MF_PCG_DIR);
   constexpr bool SCALED = (MODE == MF_PCG || DIR);
   __shared__ __align__(16) double sp[TY + 2][TX + 2][4]; // source of the tile and its halo
+  // DIR: D^{-1/2} of the tile's own elements (staged with the source; the tile's own rows are touched
+  // only by the staging, which also writes their new p and updates their iterate)
+  __shared__ __align__(16) double sh[DIR ? TY : 1][DIR ? TX : 1][4];
   const int nef = NEF >= 0 ? NEF : P.nef, ncf = NCF >= 0 ? NCF : P.ncf, nbf = NBF >= 0 ? NBF : P.nbf;
   const int b = active_block(P.nb, P.v.list);
   if (b < 0)
@@ -583,31 +586,55 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       const int col = xs + w / U; // relative to x0
       const int k0 = (4 / U) * (w % U);
       const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + k0;
+      // DIR: an entry of the tile itself (not of the halo)
+      const bool own = DIR && r >= 1 && r <= TY && col >= 0 && col < TX;
       if constexpr (VEC) {
-        double2 xv;
+        double2 xv, pv;
         if constexpr (DIR) {
           xv = *reinterpret_cast<const double2*>(P.v.r + dof);
           if (!first) {
-            const double2 pv = *reinterpret_cast<const double2*>(P.v.x + dof);
+            pv = *reinterpret_cast<const double2*>(P.v.x + dof);
             xv.x = fma(beta, pv.x, xv.x), xv.y = fma(beta, pv.y, xv.y);
           }
         } else {
           xv = *reinterpret_cast<const double2*>(P.v.x + dof);
         }
+        if (own) {
+          *reinterpret_cast<double2*>(P.v.p_out + dof) = xv;
+          if (!first) {
+            double2 xh = *reinterpret_cast<const double2*>(P.v.xh + dof);
+            xh.x = fma(alpha, pv.x, xh.x), xh.y = fma(alpha, pv.y, xh.y);
+            *reinterpret_cast<double2*>(P.v.xh + dof) = xh;
+          }
+        }
         if constexpr (SCALED) {
           const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
           xv.x *= hv.x, xv.y *= hv.y;
+          if (own)
+            *reinterpret_cast<double2*>(&sh[r - 1][col][k0]) = hv;
         }
         *reinterpret_cast<double2*>(&sp[r][col + 1][k0]) = xv;
       } else {
-        double xv;
-        if constexpr (DIR)
-          xv = first ? P.v.r[dof] : fma(beta, P.v.x[dof], P.v.r[dof]);
-        else
+        double xv, pv = 0.;
+        if constexpr (DIR) {
+          xv = P.v.r[dof];
+          if (!first) {
+            pv = P.v.x[dof];
+            xv = fma(beta, pv, xv);
+          }
+        } else {
           xv = P.v.x[dof];
+        }
+        if (own) {
+          P.v.p_out[dof] = xv;
+          if (!first)
+            P.v.xh[dof] = fma(alpha, pv, P.v.xh[dof]);
+        }
         if constexpr (SCALED) {
           const double hv = P.v.dh[dof];
           xv *= hv;
+          if (own)
+            sh[r - 1][col][k0] = hv;
         }
         sp[r][col + 1][k0] = xv;
       }
@@ -652,31 +679,22 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       store4<VEC>(P.v.y, i0, y);
     } else if constexpr (SCALED) {
       // q = D^{-1/2} y; p.q
-      double xo[4], ho[4];
-      if constexpr (DIR) {
-        // the element's new p (the same operations as in the staging) and its deferred iterate update
-        ld4<VEC>(P.v.r, i0, xo);
-        ld4<VEC>(P.v.dh, i0, ho);
-        if (!first) {
-          double pv[4], xv[4];
-          ld4<VEC>(P.v.x, i0, pv);
-          ld4<VEC>(P.v.xh, i0, xv);
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            xo[k] = fma(beta, pv[k], xo[k]);
-            xv[k] = fma(alpha, pv[k], xv[k]);
-          }
-          store4<VEC>(P.v.xh, i0, xv);
-        }
-        store4<VEC>(P.v.p_out, i0, xo);
-      } else {
-        load4<MODE, VEC>(P, i0, xo, ho);
-      }
       double q[4];
+      if constexpr (DIR) {
+        // D^{-1/2} and D^{-1/2} p of the element from LDS: p.q = (D^{-1/2} p).y
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        q[k] = ho[k] * y[k];
-        part += xo[k] * q[k];
+        for (int k = 0; k < 4; ++k) {
+          q[k] = sh[ty][tx][k] * y[k];
+          part += sp[ty + 1][tx + 1][k] * y[k];
+        }
+      } else {
+        double xo[4], ho[4];
+        load4<MODE, VEC>(P, i0, xo, ho);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          q[k] = ho[k] * y[k];
+          part += xo[k] * q[k];
+        }
       }
       store4<VEC>(P.v.y, i0, q);
     } else {
