@@ -285,3 +285,51 @@ def test_matrix_free_mode_rules(hgdt_mod):
         o.assemble()
         with pytest.raises(H.not_supported):
             o.apply(torch.ones(o.shape[0], dtype=torch.float64).cuda())
+
+
+def test_matrix_free_max_iter_stop_matches_assembled(hgdt_mod):
+    """a solve stopped by max_iter: the matrix-free solver (direction update inside the application kernel,
+    x += alpha p deferred to the stopping step) leaves the same iterate as the assembled solver after the
+    same number of steps, up to the round-off of the differently associated applications"""
+    H = hgdt_mod
+    print("case matrix-free max_iter stop", flush=True)
+    cells = [20, 17]
+    kappa = torch.from_numpy(H.fields.lognormal(23, cells[0] * cells[1])).cuda()
+    xs = []
+    for fmt in ("matrix_free", "auto"):
+        grid, space, op = hgdt_problem(H, "swipdg", cells, kappa)
+        op.set_format(fmt)
+        op.assemble()
+        x = torch.zeros(space.num_DoFs, dtype=torch.float64, device="cuda")
+        with pytest.raises(H.linear_solver_failed):
+            op.apply_inverse("rhs", x, opts={"max_iter": 40}, check_every=6)
+        assert op.last_solve["iterations"] == 40 and not op.last_solve["converged"]
+        xs.append(x.cpu().numpy())
+    rel = np.linalg.norm(xs[0] - xs[1]) / np.linalg.norm(xs[1])
+    print(f"  iterate after 40 steps, matrix-free vs assembled: rel L2 {rel:.2e}")
+    assert rel <= 1e-9
+
+
+def test_matrix_free_zero_rhs_instance(hgdt_mod):
+    """an instance of a union with a zero right-hand side stops at step 0 with x = 0; the other instances'
+    iterates do not depend on it (bitwise equal to the solve with the full right-hand side)"""
+    H = hgdt_mod
+    print("case matrix-free zero rhs instance", flush=True)
+    sizes = [4, 9, 2]
+    parts = [H.Grid.structured("quad", [0, 0], [1, 1 + 0.25 * i], [n, n + 1]) for i, n in enumerate(sizes)]
+    union = H.Grid.union(parts)
+    kappa = torch.from_numpy(H.fields.lognormal(29, union.num_elements)).cuda()
+    _, space, op = hgdt_problem(H, "swipdg", None, kappa, grid=union)
+    op.set_format("matrix_free")
+    op.assemble()
+    doff = space.instance_dofs()
+    x_full = op.apply_inverse("rhs", check_every=2).cpu().numpy()
+    its_full, _ = op.instance_stats()
+    b = op.rhs.clone()
+    b[doff[1]:doff[2]] = 0.
+    x = op.apply_inverse(b, check_every=2).cpu().numpy()
+    its, _ = op.instance_stats()
+    assert its[1] == 0 and np.all(x[doff[1]:doff[2]] == 0.)
+    for i in (0, 2):
+        assert its[i] == its_full[i]
+        assert np.array_equal(x[doff[i]:doff[i + 1]], x_full[doff[i]:doff[i + 1]])
