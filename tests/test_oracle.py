@@ -241,7 +241,7 @@ def test_dirichlet_constraints():
 
 
 def test_pcg_against_direct_solve():
-    """Jacobi-PCG to 1e-10 vs scipy's direct solve; C1 (128^2, 16 641 DoFs): max u_h vs the
+    """Jacobi-PCG to 1e-10 vs scipy's direct solve; Q1 on 128^2 quads (16 641 DoFs): max u_h vs the
     continuous max 0.0736713 of -Laplace u = 1 on the unit square"""
     m, s, op = q1_heat(128)
     rp, cols = op.pattern()
