@@ -287,7 +287,8 @@ struct SolverFormat {
 // device-side PCG state, one record per instance; two copies indexed by iteration parity so that
 // no kernel reads a record it writes
 struct PcgScalars {
-  double rz, rr, bb, tol2, alpha, beta, pq, pad0;
+  double rz, rr, bb, tol2, alpha, beta, pq;
+  double alpha_prev; // alpha of the step before the one in alpha (the iterate is updated every second step)
   int64_t iters;
   int32_t done; // 0 running, 1 converged, 2 max_iter reached, 3 breakdown
   int32_t pad1;

@@ -527,8 +527,8 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   // partials, p = r + beta p, x += alpha p deferred from the previous step) on the block's elements.  The
   // staged source is formed from r and the previous p, which live in another buffer than the new p, so a
   // neighbouring block's writes never reach this block's halo.
-  double alpha = 0., beta = 0.;
-  bool first = false;
+  double alpha = 0., alpha_prev = 0., beta = 0.;
+  bool first = false, pair = false; // pair: the iterate's two pending updates are due (even step >= 2)
   if constexpr (DIR) {
     const PcgScalars* cur = P.v.S + P.v.par * P.ninst + inst;
     PcgScalars* nxt = P.v.S + (P.v.par ^ 1) * P.ninst + inst;
@@ -540,8 +540,9 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
     }
     const int64_t it = cur->iters;
     const double rho_prev = cur->rz, tol2 = cur->tol2;
-    alpha = cur->alpha;
+    alpha = cur->alpha, alpha_prev = cur->alpha_prev;
     first = (it == 0);
+    pair = !first && (it & 1) == 0;
     double t[2];
     reduce_partials<2>(P.v.part2, P.v.v_ib0[inst], P.v.v_ib0[inst + 1], t);
     const int status = (t[1] <= tol2) ? 1 : (it >= P.v.max_iter) ? 2 : 0;
@@ -550,6 +551,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       nxt->bb = cur->bb;
       nxt->tol2 = tol2;
       nxt->alpha = alpha;
+      nxt->alpha_prev = alpha;
       nxt->pq = cur->pq;
       nxt->rr = t[1];
       nxt->rz = t[0];
@@ -558,11 +560,18 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       nxt->done = status;
     }
     if (status != 0) {
-      // the instance stops here: only the iterate's deferred update remains (uniform over the block)
+      // the instance stops here: only the iterate's pending updates remain (uniform over the block)
       if (row && !first) {
         double pv[4], xv[4];
         ld4<VEC>(P.v.x, i0, pv);
         ld4<VEC>(P.v.xh, i0, xv);
+        if (pair) {
+          double ov[4];
+          ld4<VEC>(P.v.p_out, i0, ov);
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            xv[k] = fma(alpha_prev, ov[k], xv[k]);
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           xv[k] = fma(alpha, pv[k], xv[k]);
@@ -600,12 +609,14 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
           xv = *reinterpret_cast<const double2*>(P.v.x + dof);
         }
         if (own) {
-          *reinterpret_cast<double2*>(P.v.p_out + dof) = xv;
-          if (!first) {
+          if (pair) { // p_{j-2} is read before p_j replaces it
+            const double2 ov = *reinterpret_cast<const double2*>(P.v.p_out + dof);
             double2 xh = *reinterpret_cast<const double2*>(P.v.xh + dof);
-            xh.x = fma(alpha, pv.x, xh.x), xh.y = fma(alpha, pv.y, xh.y);
+            xh.x = fma(alpha, pv.x, fma(alpha_prev, ov.x, xh.x));
+            xh.y = fma(alpha, pv.y, fma(alpha_prev, ov.y, xh.y));
             *reinterpret_cast<double2*>(P.v.xh + dof) = xh;
           }
+          *reinterpret_cast<double2*>(P.v.p_out + dof) = xv;
         }
         if constexpr (SCALED) {
           const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
@@ -626,9 +637,9 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
           xv = P.v.x[dof];
         }
         if (own) {
+          if (pair)
+            P.v.xh[dof] = fma(alpha, pv, fma(alpha_prev, P.v.p_out[dof], P.v.xh[dof]));
           P.v.p_out[dof] = xv;
-          if (!first)
-            P.v.xh[dof] = fma(alpha, pv, P.v.xh[dof]);
         }
         if constexpr (SCALED) {
           const double hv = P.v.dh[dof];

@@ -17,8 +17,8 @@
 // Jacobi-PCG runs as CG on the symmetrically scaled matrix D^{-1/2} A D^{-1/2} (same iterates in
 // exact arithmetic; the scaled matrix has a unit diagonal that is never stored or read, and the
 // preconditioner costs no vector), three launches per iteration, each one pass over its vectors:
-//   k_cg_direction : stopping test from the previous update's partials; x += alpha p (deferred from
-//                    the previous iteration); p = r + beta p
+//   k_cg_direction : stopping test from the previous update's partials; p = r + beta p into the other
+//                    of two p buffers; every second step x += the two updates deferred since
 //   k_cg_spmv      : q = Ah p and the block partials of p.q
 //   k_cg_update    : alpha from the p.q partials; r -= alpha q; block partials of r.r and |D^{1/2} r|^2
 // Data cross blocks only at kernel boundaries.  Every block of an instance reduces that
@@ -89,8 +89,7 @@ struct PcgParams {
   double* x;
   double* r;
   double* p;
-  double* p2; // second p buffer (matrix-free solver with the fused direction update: the p formed at step
-              // j is written to p2 for even j, to p for odd j)
+  double* p2; // second p buffer: the p formed at step j is in p2 for even j, in p for odd j
   double* q;
   double* part1; // 1 per A-block: p.q
   double* part2; // 2 per V-block: r.r of the scaled residual, r.r of the residual
@@ -337,8 +336,11 @@ __global__ void __launch_bounds__(256) k_cg_init_scalars(PcgParams P)
 
 // CG direction update of step j (par = j & 1), on the vector row partition:
 //   rho_j = rh.rh and |r|^2 from the partials of the previous update (or of the initialisation),
-//   stopping test; xh += alpha_{j-1} p (the update deferred from step j-1);
-//   p = rh + beta p with beta = rho_j / rho_{j-1} (p = rh at step 0).
+//   stopping test; p_j = rh + beta p_{j-1} with beta = rho_j / rho_{j-1} (p_0 = rh).
+// p_j is written to p2 for even j and to p for odd j, so p_{j-1} and p_{j-2} are both at hand, and the
+// iterate is updated every second step: at even j >= 2, xh += alpha_{j-2} p_{j-2}, then += alpha_{j-1}
+// p_{j-1} (the operations of one update per step, in the same order), and at the step that stops the
+// instance, whatever is still pending (the update kernel does it after a breakdown).
 // Reads record par, writes record par ^ 1.
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
@@ -357,18 +359,21 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
     return;
   }
   const int64_t it = cur->iters;
-  const double rho_prev = cur->rz, alpha = cur->alpha, tol2 = cur->tol2;
+  const double rho_prev = cur->rz, alpha = cur->alpha, alpha_prev = cur->alpha_prev, tol2 = cur->tol2;
   const bool first = (it == 0);
+  const bool pair = !first && (it & 1) == 0; // both pending updates of the iterate are due
+  const double* __restrict__ pin = par ? P.p2 : P.p; // p_{j-1}
+  double* __restrict__ pout = par ? P.p : P.p2;      // p_j, over p_{j-2}
   const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-  // the block's rows are loaded before the scalars are reduced, so the loads overlap the reduction
-  // (r is read also at the step that finds the instance converged, and then not used)
-  double pv[ITEMS], xv[ITEMS], rv[ITEMS];
+  // the block's rows of r and p_{j-1} are loaded before the scalars are reduced, so the loads overlap the
+  // reduction (r is read also at the step that finds the instance converged, and then not used); those
+  // of the iterate's update after it (every second step, few registers across the reduction)
+  double pv[ITEMS], rv[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     const bool ok = i < r1;
-    pv[j] = (ok && !first) ? P.p[i] : 0.;
-    xv[j] = (ok && !first) ? P.x[i] : 0.;
+    pv[j] = (ok && !first) ? pin[i] : 0.;
     rv[j] = ok ? P.r[i] : 0.;
   }
   double t[2];
@@ -376,13 +381,26 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
   const int status = (t[1] <= tol2) ? 1 : (it >= P.max_iter) ? 2 : 0;
   const double beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
   if (!first) {
+    // the iterate's rows and p_{j-2} when an update is due (all loads issued before the stores)
+    const bool upd = pair || status != 0;
+    double ov[ITEMS], xv[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+      const bool ok = i < r1;
+      ov[j] = (ok && pair) ? pout[i] : 0.;
+      xv[j] = (ok && upd) ? P.x[i] : 0.;
+    }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
       if (i < r1) {
-        P.x[i] = fma(alpha, pv[j], xv[j]);
+        if (pair)
+          P.x[i] = fma(alpha, pv[j], fma(alpha_prev, ov[j], xv[j]));
+        else if (status != 0)
+          P.x[i] = fma(alpha, pv[j], xv[j]);
         if (status == 0)
-          P.p[i] = fma(beta, pv[j], rv[j]);
+          pout[i] = fma(beta, pv[j], rv[j]);
       }
     }
   } else if (status == 0) {
@@ -390,13 +408,14 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
     for (int j = 0; j < ITEMS; ++j) {
       const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
       if (i < r1)
-        P.p[i] = rv[j];
+        pout[i] = rv[j];
     }
   }
   if (writer) {
     nxt->bb = cur->bb;
     nxt->tol2 = tol2;
     nxt->alpha = alpha;
+    nxt->alpha_prev = alpha;
     nxt->pq = cur->pq;
     nxt->rr = t[1];
     nxt->rz = t[0];
@@ -417,7 +436,7 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
   if (P.S[(par ^ 1) * P.ninst + inst].done)
     return;
   const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
-  const double* __restrict__ p = P.p;
+  const double* __restrict__ p = par ? P.p : P.p2; // p_j (written by the direction update of step j)
   double v[1] = {0.};
   for (int64_t r0 = rb; r0 < r1; r0 += int64_t(blockDim.x) * ITEMS) {
     double acc[ITEMS];
@@ -465,7 +484,12 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
   double pq[1];
   reduce_partials<1>(P.part1, P.a_ib0[inst], P.a_ib0[inst + 1], pq);
   if (!(pq[0] > 0.)) {
-    // breakdown (A not positive definite on the Krylov space): every block of the instance sees it
+    // breakdown (A not positive definite on the Krylov space): every block of the instance sees it.  At
+    // an odd step j the direction update deferred alpha_{j-1} p_{j-1} (p_{j-1} in p2): applied here, so
+    // the stopped iterate is the one of an update at every step
+    if ((S->iters & 1) == 0)
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+        P.x[i] = fma(S->alpha, P.p2[i], P.x[i]);
     if (writer)
       S->done = 3;
     return;
@@ -1133,7 +1157,7 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
     });
   if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
     MfVectors v;
-    v.x = P.p, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
+    v.x = par ? P.p : P.p2, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
     v.list = P.a_list, v.nb = P.a_nb;
     if (fused) {
       // the direction update inside the application: p of the previous step in one buffer, the new p in
@@ -1170,8 +1194,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   const int ninst = int(space->inst_dof.size()) - 1;
   d_r.allocate(std::max<int64_t>(n, 1));
   d_p.allocate(std::max<int64_t>(n, 1));
-  if (mf_fused_direction(*this))
-    d_p2.allocate(std::max<int64_t>(n, 1));
+  d_p2.allocate(std::max<int64_t>(n, 1));
   d_q.allocate(std::max<int64_t>(n, 1));
   d_partials.allocate(std::max<int64_t>(fmt.A.nblocks + 3 * fmt.V.nblocks, 1));
   d_scalars.allocate(2 * ninst);
