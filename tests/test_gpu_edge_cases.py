@@ -188,3 +188,89 @@ def test_invalid_input_raises(hgdt_mod):
         op.set_launch(H._lib.KERNEL_APPLY, 96, 1)
     with pytest.raises(H.wrong_input_given):
         op.set_launch(H._lib.KERNEL_ELEMENT, 512, 1)
+
+
+def _first_nonpositive_curvature(A, b, rtol=1e-10, max_iter=2000):
+    """step j of textbook Jacobi-PCG (x0 = 0) at which p_j . A p_j <= 0 first occurs (None if it converges
+    first), found on the host to choose the oracle's comparison step; the margin is |cos(p, Ap)|"""
+    import scipy.sparse  # noqa: F401  (A is a scipy matrix)
+    d = A.diagonal()
+    x, r = np.zeros_like(b), b.copy()
+    z = r / d
+    p, rz, tol2 = z.copy(), r @ z, rtol * rtol * (b @ b)
+    for k in range(max_iter):
+        q = A @ p
+        pq = p @ q
+        if not pq > 0:
+            return k, abs(pq) / (np.linalg.norm(p) * np.linalg.norm(q))
+        alpha = rz / pq
+        x += alpha * p
+        r -= alpha * q
+        if r @ r <= tol2:
+            return None, 1.
+        z = r / d
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    return None, 1.
+
+
+def test_breakdown_leaves_the_iterate_of_the_last_full_step(hgdt_mod):
+    """indefinite operators with a positive diagonal (Laplace minus c times the mass matrix): the solve
+    stops each instance at its first non-positive curvature p_j . A p_j <= 0 (reported as breakdown) and
+    leaves the iterate after j updates, x_j, equal to the oracle's Jacobi-PCG run for j iterations.  The
+    breakdown steps (3, 2, 1, 0) cover both parities of the iterate's every-second-step update; the
+    instance with c = 0 converges and equals its plain solve"""
+    import scipy.sparse as sp
+    H = hgdt_mod
+    print("case breakdown iterates", flush=True)
+    n, cs = 16, [0., 25., 30., 40., 80.]
+    parts = [H.Grid.structured("quad", [0, 0], [1, 1], [n, n]) for _ in cs]
+    union = H.Grid.union(parts)
+    w = np.repeat(np.array(cs), n * n)
+    space = H.ContinuousLagrangeSpace(union, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(torch.from_numpy(w).cuda())),
+              {"matrixoperator.scaling": -1.0})
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.assemble()
+    x = torch.zeros(space.num_DoFs, dtype=torch.float64, device="cuda")
+    with pytest.raises(H.linear_solver_failed):
+        op.apply_inverse("rhs", x, check_every=2)
+    assert op.last_solve["breakdown"] == 1
+    x = x.cpu().numpy()
+    its, _ = op.instance_stats()
+    doff = space.instance_dofs()
+    seen = set()
+    for i, c in enumerate(cs):
+        m = O.Mesh.structured(O.QUAD, [n, n], [0, 0], [1, 1])
+        s = O.Space(m, True, 1)
+        oop = O.Operator(s, O.STENCIL_ELEMENT)
+        oop.append_laplace(1.0)
+        oop.append_product(c, scaling=-1.0)
+        oop.append_source(1.0)
+        oop.set_dirichlet(True)
+        oop.assemble(1)
+        rp, cols = oop.pattern()
+        A = sp.csr_matrix((oop.values(), cols, rp), shape=(len(rp) - 1, len(rp) - 1))
+        j, margin = _first_nonpositive_curvature(A, oop.rhs())
+        xi = x[doff[i]:doff[i + 1]]
+        if j is None:
+            xo, ito, relo, rc = O.pcg_jacobi(rp, cols, oop.values(), oop.rhs(), rtol=1e-10)
+            print(f"  c={c}: converged, iters {its[i]} (oracle {ito})")
+            assert rc == 0 and its[i] == ito
+        else:
+            assert margin > 1e-3  # the sign of p.Ap is not a rounding matter
+            seen.add(j)
+            xo = np.zeros_like(xi) if j == 0 else O.pcg_jacobi(rp, cols, oop.values(), oop.rhs(), rtol=1e-10,
+                                                                 max_iters_run=j)[0]
+            print(f"  c={c}: breakdown at step {j} (steps begun {its[i]})")
+            assert its[i] == j + 1
+        rel = np.linalg.norm(xi - xo) / max(np.linalg.norm(xo), 1e-300)
+        print(f"    iterate vs oracle: rel L2 {rel:.2e}")
+        assert (rel <= 1e-10) if np.linalg.norm(xo) > 0 else np.all(xi == 0.)
+    assert {0, 1, 2, 3} <= seen
