@@ -524,9 +524,9 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   const uint32_t c = ey * nx + ex;
   const int64_t i0 = d0 + 4 * int64_t(c); // first DoF of the lane's element
   // DIR: the direction update of the step (as k_cg_direction: stopping test on the previous update's
-  // partials, p = r + beta p, x += alpha p deferred from the previous step) on the block's elements.  The
-  // staged source is formed from r and the previous p, which live in another buffer than the new p, so a
-  // neighbouring block's writes never reach this block's halo.
+  // partials, p = r + beta p, and every second step the iterate's two deferred updates) on the block's
+  // elements.  The staged source is formed from r and the previous p, which live in another buffer than
+  // the new p, so a neighbouring block's writes never reach this block's halo.
   double alpha = 0., alpha_prev = 0., beta = 0.;
   bool first = false, pair = false; // pair: the iterate's two pending updates are due (even step >= 2)
   if constexpr (DIR) {

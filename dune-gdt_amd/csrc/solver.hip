@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 
 // CG update of step j on the vector row partition: alpha = rho_j / p.q, rh -= alpha q, block partials
 // of rh.rh and |D^{1/2} rh|^2.  Works on record par ^ 1 (written by the step kernel): stores alpha
-// there for the next step's deferred x update, or the breakdown state.
+// there for the iterate's deferred updates, or the breakdown state.
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 {
