@@ -7,19 +7,21 @@ namespace hgdt {
 namespace {
 
 // XCD-aware logical block: hardware block hw runs logical block (hw % 8) * ceil(nb / 8) + hw / 8, so each
-// XCD walks one contiguous eighth of the blocks; -1 for the padding blocks of the last XCD
-__device__ __forceinline__ int logical_block(int nb)
+// XCD walks one contiguous eighth of the blocks (rev: the same eighth from its top down); -1 for the
+// padding blocks of the last XCD
+__device__ __forceinline__ int logical_block(int nb, bool rev = false)
 {
   const int q = (nb + 7) >> 3;
   const int hw = blockIdx.x;
-  const int L = (hw & 7) * q + (hw >> 3);
+  const int s = hw >> 3;
+  const int L = (hw & 7) * q + (rev ? q - 1 - s : s);
   return L < nb ? L : -1;
 }
 
 // logical block of a launch over a list of active blocks (NULL: all blocks 0 .. nb - 1)
-__device__ __forceinline__ int active_block(int nb, const int32_t* __restrict__ list)
+__device__ __forceinline__ int active_block(int nb, const int32_t* __restrict__ list, bool rev = false)
 {
-  const int L = logical_block(nb);
+  const int L = logical_block(nb, rev);
   return (L < 0 || !list) ? L : list[L];
 }
 

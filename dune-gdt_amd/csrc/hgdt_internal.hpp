@@ -367,6 +367,7 @@ struct MfVectors {
   int par = 0;
   const int32_t* list = nullptr; // MF_PCG(_DIR): blocks of unfinished instances (nb of them; NULL: all)
   int nb = -1;
+  int rev = 0; // each XCD walks its blocks top down
   // MF_PCG_DIR: new p, scaled residual, scaled iterate (updated every second step with the deferred alpha p),
   // partials of the previous update (2 per V-block), first V- and A-block of every instance, max_iter
   double* p_out = nullptr;

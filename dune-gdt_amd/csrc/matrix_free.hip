@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
   constexpr int NV = 1 << D, NS = (D == 2) ? 9 : 27;
   constexpr bool DIAG = (MODE == MF_DIAG);
   constexpr int EC = NEF > 0 ? NEF : 1;
-  const int b = active_block(P.nb, P.v.list);
+  const int b = active_block(P.nb, P.v.list, P.v.rev);
   if (b < 0)
     return;
   const int64_t* __restrict__ B = P.blk + 8 * int64_t(b);
@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   // only by the staging, which also writes their new p and updates their iterate)
   __shared__ __align__(16) double sh[DIR ? TY : 1][DIR ? TX : 1][4];
   const int nef = NEF >= 0 ? NEF : P.nef, ncf = NCF >= 0 ? NCF : P.ncf, nbf = NBF >= 0 ? NBF : P.nbf;
-  const int b = active_block(P.nb, P.v.list);
+  const int b = active_block(P.nb, P.v.list, P.v.rev);
   if (b < 0)
     return;
   const int64_t* __restrict__ Bk = P.blk + 8 * int64_t(b);

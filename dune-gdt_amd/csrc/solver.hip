@@ -68,6 +68,9 @@ struct PcgParams {
   // blocks of unfinished instances (NULL: every block); a_nb / v_nb are then the list lengths
   const int32_t* a_list;
   const int32_t* v_list;
+  // the blocks of each XCD are walked top down (set per launch: successive kernels of an iteration walk
+  // in opposite orders, so each starts on the rows the previous one touched last)
+  int rev;
   const int64_t* inst_row0;
   // DIA
   int ndiag;
@@ -345,7 +348,7 @@ __global__ void __launch_bounds__(256) k_cg_init_scalars(PcgParams P)
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
 {
-  const int b = active_block(P.v_nb, P.v_list);
+  const int b = active_block(P.v_nb, P.v_list, P.rev);
   if (b < 0)
     return;
   const int inst = P.v_inst[b];
@@ -429,7 +432,7 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
 template <int FMT, int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 {
-  const int b = active_block(P.a_nb, P.a_list);
+  const int b = active_block(P.a_nb, P.a_list, P.rev);
   if (b < 0)
     return;
   const int inst = P.a_inst[b];
@@ -462,7 +465,7 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 {
-  const int b = active_block(P.v_nb, P.v_list);
+  const int b = active_block(P.v_nb, P.v_list, P.rev);
   if (b < 0)
     return;
   const int inst = P.v_inst[b];
@@ -1150,15 +1153,20 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   const RowPartition& V = op.fmt.V;
   const int kf = kernel_format(op);
   const bool fused = mf_fused_direction(op);
+  // block order: the operator application walks the blocks of every XCD upward, the vector kernels
+  // downward, so the direction update ends and the update starts on the rows the application touches
+  // first and last
+  PcgParams Pd = P, Pa = P, Pu = P;
+  Pd.rev = 1, Pa.rev = 0, Pu.rev = 1;
   if (!fused)
     dispatch_items(V.items, [&](auto IT) {
       constexpr int I = decltype(IT)::value;
-      launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
+      launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, Pd, par);
     });
   if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
     MfVectors v;
     v.x = par ? P.p : P.p2, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
-    v.list = P.a_list, v.nb = P.a_nb;
+    v.list = P.a_list, v.nb = P.a_nb, v.rev = Pa.rev;
     if (fused) {
       // the direction update inside the application: p of the previous step in one buffer, the new p in
       // the other (the parity of the step picks them)
@@ -1170,15 +1178,15 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(P.a_nb);
     if (kf == FMT_DIA)
-      launch_pcg(k_cg_spmv<FMT_DIA, I>, grid, A.block, stream, P, par, ev0, ev1);
+      launch_pcg(k_cg_spmv<FMT_DIA, I>, grid, A.block, stream, Pa, par, ev0, ev1);
     else if (kf == FMT_DIAS)
-      launch_pcg(k_cg_spmv<FMT_DIAS, I>, grid, A.block, stream, P, par, ev0, ev1);
+      launch_pcg(k_cg_spmv<FMT_DIAS, I>, grid, A.block, stream, Pa, par, ev0, ev1);
     else
-      launch_pcg(k_cg_spmv<FMT_SELL, I>, grid, A.block, stream, P, par, ev0, ev1);
+      launch_pcg(k_cg_spmv<FMT_SELL, I>, grid, A.block, stream, Pa, par, ev0, ev1);
   });
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    launch_pcg(k_cg_update<I>, hw_blocks(P.v_nb), V.block, stream, P, par);
+    launch_pcg(k_cg_update<I>, hw_blocks(P.v_nb), V.block, stream, Pu, par);
   });
 }
 
