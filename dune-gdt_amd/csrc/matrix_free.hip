@@ -927,9 +927,10 @@ std::vector<int64_t> mf_tile_counts(const Operator& op)
 int64_t mf_bytes(const Operator& op, int mode)
 {
   // apply: x read, y written; solver: p, D^{-1/2} read, q written; with the direction update: r, p of the
-  // previous step, D^{-1/2}, the iterate read, p, the iterate, q written.  Plus one coefficient per element
-  // and distinct field.
-  const int64_t per_dof = mode == MF_APPLY ? 16 : mode == MF_PCG ? 24 : 56;
+  // previous step, D^{-1/2} read, p, q written (40 B), and every second step also the iterate and the p
+  // before read and the iterate written (64 B): 52 B per launch on average.  Plus one coefficient per
+  // element and distinct field.
+  const int64_t per_dof = mode == MF_APPLY ? 16 : mode == MF_PCG ? 24 : 52;
   return per_dof * op.n + int64_t(8) * op.fmt.mf_fields * op.space->grid->ne;
 }
 
