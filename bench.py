@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="same as --tune off")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this size first (reference dispatch for HBM counters)")
-    ap.add_argument("--format", default="auto", choices=["auto", "matrix_free", "dia_symmetric", "dia", "sell64"],
+    ap.add_argument("--format", default="auto", choices=["auto", "matrix_free", "block4", "dia_symmetric", "dia", "sell64"],
                     help="operator storage of apply_inverse (matrix_free: recomputed from the coefficients at every "
                          "application; the assembly then builds the right-hand side only)")
     ap.add_argument("--check-every", type=int, default=16,
@@ -378,7 +378,7 @@ def main():
     mf = fi["format"] == "matrix_free"
     # (discontinuous: the application with the solver's direction update fused in, mode 3)
     kernel = ({"c3": "k_mf_dg<3, 1, 2, 2, true>"}.get(args.config, "k_mf_cg<2, 1, 1>") if mf
-              else f"k_cg_spmv<{fi['format']}>")
+              else "k_cg_spmv_blk<symmetric>" if fi["format"] == "block4" else f"k_cg_spmv<{fi['format']}>")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"pmc_{'mf' if mf else 'cg_spmv'}_{args.config}.json")
     if os.path.exists(tfile):

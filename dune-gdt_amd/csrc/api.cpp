@@ -708,7 +708,7 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out)
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format)
 {
   return guarded([&] {
-    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_MATRIX_FREE, HGDT_ERR_INVALID_ARGUMENT,
+    require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_BLOCK, HGDT_ERR_INVALID_ARGUMENT,
             "unknown format");
     op->op->requested_format = (format == HGDT_FORMAT_DIA_NONSYMMETRIC) ? HGDT_FORMAT_DIA : format;
     op->op->allow_symmetric = (format != HGDT_FORMAT_DIA_NONSYMMETRIC);

@@ -245,7 +245,7 @@ struct RowPartition {
 
 // storage of the matrix used by apply / apply_inverse
 struct SolverFormat {
-  int format = 0;      // HGDT_FORMAT_SELL or HGDT_FORMAT_DIA
+  int format = 0;      // HGDT_FORMAT_SELL, HGDT_FORMAT_DIA, HGDT_FORMAT_BLOCK or HGDT_FORMAT_MATRIX_FREE
   bool partitions_ready = false; // row partitions A and V built for the current launch shapes
   bool structure_ready = false;  // storage layout (DIA offsets / SELL slices) built
   bool valid = false;  // values converted since the last assemble
@@ -269,6 +269,16 @@ struct SolverFormat {
   DeviceArray<int32_t> d_sell_cols;
   DeviceArray<double> d_sell_vals;
   DeviceArray<int64_t> d_sell_pos; // nnz: CSR slot -> SELL position
+  // 4 x 4 element blocks (discontinuous spaces with 4 DoFs per element): per element its own block and
+  // the blocks of its neighbours (ascending), value planes: value m of element E at m * blk_ne + E, m < 16
+  // the own block (row-major), 16 + 16 s + 4 a + b entry (a, b) of the block with neighbour slot s
+  int64_t blk_ne = 0;
+  int blk_mn = 0;                // neighbour slots per element
+  int64_t blk_up = 0, blk_all = 0; // neighbour blocks above the element / all, summed over the elements
+  DeviceArray<int32_t> d_blk_nb;  // blk_mn * blk_ne: neighbour of each slot (ascending, -1 past the last)
+  DeviceArray<int32_t> d_blk_mir; // blk_mn * blk_ne: slot of the element in its neighbour's list
+  DeviceArray<double> d_blk;      // (16 + 16 blk_mn) * blk_ne
+  DeviceArray<int64_t> d_blk_pos; // nnz: CSR slot -> block position
   // Jacobi preconditioning by symmetric scaling: h_i = 1 / sqrt(a_ii), s_i = sqrt(a_ii), and the scaled
   // matrix D^{-1/2} A D^{-1/2} in the layout of the storage above (DIA or SELL values)
   DeviceArray<double> d_dh, d_dsq;

@@ -51,7 +51,7 @@ namespace hgdt {
 namespace {
 
 constexpr int MAXDIAG = 32;
-enum { FMT_DIA = HGDT_FORMAT_DIA, FMT_SELL = HGDT_FORMAT_SELL, FMT_DIAS = 3 };
+enum { FMT_DIA = HGDT_FORMAT_DIA, FMT_SELL = HGDT_FORMAT_SELL, FMT_DIAS = 3, FMT_BLK = HGDT_FORMAT_BLOCK };
 
 struct PcgParams {
   int64_t n;
@@ -82,6 +82,12 @@ struct PcgParams {
   const int64_t* slice_ptr;
   const int32_t* sell_cols;
   const double* sell_vals;
+  // 4 x 4 element blocks (value planes of blk_ne elements, blk_mn neighbour slots)
+  int64_t blk_ne;
+  int blk_mn;
+  const int32_t* blk_nb;
+  const int32_t* blk_mir;
+  const double* blk;
   // symmetric scaling D^{-1/2} A D^{-1/2}: values in the layout of the format, h = D^{-1/2}, s = D^{1/2}
   const double* scaled;
   const double* dh;
@@ -285,6 +291,259 @@ __global__ void __launch_bounds__(1024) k_apply(PcgParams P, const double* __res
   }
 }
 
+// 4 x 4 element blocks: y = A x over the 4 rows of element E (lane per element).  Each row sums its
+// terms in ascending column order (neighbours below E, the own block, neighbours above E), the order of
+// DIA and SELL, so the formats give the same bits.  SYM (exactly symmetric matrix): only the own
+// block's upper triangle and the blocks of the neighbours above E are read from E's record; the block
+// with a neighbour F below E is F's block with E (read from F's record, transposed).  UNIT: the unit
+// diagonal of the scaled matrix, not read.
+template <bool SYM, bool UNIT, class XS>
+__device__ __forceinline__ void blk_rows_seq(const PcgParams& P, const double* __restrict__ v, int64_t E, XS xs,
+                                         double (&y)[4])
+{
+  const int64_t ne = P.blk_ne;
+  const int mn = P.blk_mn;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    y[i] = 0.;
+  int s = 0;
+  for (; s < mn; ++s) { // neighbours below E
+    const int32_t F = P.blk_nb[int64_t(s) * ne + E];
+    if (F < 0 || F > E)
+      break;
+    const double* __restrict__ B =
+        SYM ? v + int64_t(16 + 16 * P.blk_mir[int64_t(s) * ne + E]) * ne + F : v + int64_t(16 + 16 * s) * ne + E;
+    double b[16], x[4];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      b[k] = B[int64_t(k) * ne];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      x[j] = xs(4 * int64_t(F) + j);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        y[i] = fma(SYM ? b[4 * j + i] : b[4 * i + j], x[j], y[i]);
+  }
+  { // own block
+    double o[16], x[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        o[4 * a + c] = ((SYM && c < a) || (UNIT && c == a)) ? 0. : v[int64_t(4 * a + c) * ne + E];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      x[j] = xs(4 * E + j);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (UNIT && j == i)
+          y[i] += x[j];
+        else
+          y[i] = fma((SYM && j < i) ? o[4 * j + i] : o[4 * i + j], x[j], y[i]);
+      }
+  }
+  for (; s < mn; ++s) { // neighbours above E
+    const int32_t F = P.blk_nb[int64_t(s) * ne + E];
+    if (F < 0)
+      break;
+    const double* __restrict__ B = v + int64_t(16 + 16 * s) * ne + E;
+    double b[16], x[4];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      b[k] = B[int64_t(k) * ne];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      x[j] = xs(4 * int64_t(F) + j);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        y[i] = fma(b[4 * i + j], x[j], y[i]);
+  }
+}
+
+// the same sums for elements with at most MAXS neighbours: the slots' neighbours are read first, and the
+// blocks are loaded two at a time (loads of two blocks in flight per lane; the FMAs keep the order)
+template <int MAXS, bool SYM, bool UNIT, class XS>
+__device__ __forceinline__ void blk_rows_pairs(const PcgParams& P, const double* __restrict__ v, int64_t E, XS xs,
+                                               double (&y)[4])
+{
+  const int64_t ne = P.blk_ne;
+  const int mn = P.blk_mn;
+  int32_t Fs[MAXS], ts[MAXS];
+#pragma unroll
+  for (int s = 0; s < MAXS; ++s) {
+    Fs[s] = s < mn ? P.blk_nb[int64_t(s) * ne + E] : -1;
+    ts[s] = (SYM && s < mn) ? P.blk_mir[int64_t(s) * ne + E] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    y[i] = 0.;
+  // the own block's loads are issued first, so they are in flight with the first neighbour blocks
+  double o[16], xo[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      o[4 * a + c] = ((SYM && c < a) || (UNIT && c == a)) ? 0. : v[int64_t(4 * a + c) * ne + E];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    xo[j] = xs(4 * E + j);
+  // neighbours below E (a prefix of the slots), two blocks at a time
+#pragma unroll
+  for (int s0 = 0; s0 < MAXS; s0 += 2) {
+    double b[2][16], x[2][4];
+    bool on[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int s = s0 + g;
+      on[g] = s < MAXS && Fs[s] >= 0 && Fs[s] < E;
+      if (on[g]) {
+        const double* __restrict__ B =
+            SYM ? v + int64_t(16 + 16 * ts[s]) * ne + Fs[s] : v + int64_t(16 + 16 * s) * ne + E;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          b[g][k] = B[int64_t(k) * ne];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          x[g][j] = xs(4 * int64_t(Fs[s]) + j);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+      if (on[g])
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            y[i] = fma(SYM ? b[g][4 * j + i] : b[g][4 * i + j], x[g][j], y[i]);
+  }
+  { // own block (loaded before the neighbours below)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (UNIT && j == i)
+          y[i] += xo[j];
+        else
+          y[i] = fma((SYM && j < i) ? o[4 * j + i] : o[4 * i + j], xo[j], y[i]);
+      }
+  }
+  // neighbours above E, two blocks at a time
+#pragma unroll
+  for (int s0 = 0; s0 < MAXS; s0 += 2) {
+    double b[2][16], x[2][4];
+    bool on[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int s = s0 + g;
+      on[g] = s < MAXS && Fs[s] > E;
+      if (on[g]) {
+        const double* __restrict__ B = v + int64_t(16 + 16 * s) * ne + E;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          b[g][k] = B[int64_t(k) * ne];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          x[g][j] = xs(4 * int64_t(Fs[s]) + j);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+      if (on[g])
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            y[i] = fma(b[g][4 * i + j], x[g][j], y[i]);
+  }
+}
+
+template <bool SYM, bool UNIT, class XS>
+__device__ __forceinline__ void blk_rows(const PcgParams& P, const double* __restrict__ v, int64_t E, XS xs,
+                                         double (&y)[4])
+{
+  if (P.blk_mn <= 4)
+    blk_rows_pairs<4, SYM, UNIT>(P, v, E, xs, y);
+  else
+    blk_rows_seq<SYM, UNIT>(P, v, E, xs, y);
+}
+
+// y = A x with element blocks (unscaled values), one lane per element of the operator's row blocks
+template <bool SYM>
+__global__ void __launch_bounds__(256) k_blk_apply(PcgParams P, const double* __restrict__ x, double* __restrict__ y)
+{
+  const int b = logical_block(P.a_nb);
+  if (b < 0)
+    return;
+  const int64_t i0 = P.a_row0[b] + 4 * int64_t(threadIdx.x);
+  if (i0 >= P.a_row0[b + 1])
+    return;
+  double acc[4];
+  blk_rows<SYM, false>(P, P.blk, i0 / 4, [&](int64_t c) { return x[c]; }, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    y[i0 + i] = acc[i];
+}
+
+// diagonal scaling and exact-symmetry test of the element blocks: h = D^{-1/2}, s = D^{1/2} from the own
+// blocks' diagonals (flags[0]: a non-positive diagonal); flags[1]: an entry differs from its transposed
+// one (own block, or the block with a neighbour below against that neighbour's block)
+__global__ void k_blk_prepare(PcgParams P, double* __restrict__ dh, double* __restrict__ dsq, int* flags)
+{
+  const int64_t E = xcd_index(P.blk_ne);
+  if (E >= P.blk_ne)
+    return;
+  const int64_t ne = P.blk_ne;
+  const double* __restrict__ v = P.blk;
+  bool ok = true;
+  for (int a = 0; a < 4; ++a) {
+    double d = v[int64_t(5 * a) * ne + E];
+    if (!(d > 0.)) {
+      flags[0] = 1;
+      d = 1.;
+    }
+    const double sq = sqrt(d);
+    dh[4 * E + a] = 1. / sq;
+    dsq[4 * E + a] = sq;
+    for (int c = a + 1; c < 4; ++c)
+      ok = ok && v[int64_t(4 * a + c) * ne + E] == v[int64_t(4 * c + a) * ne + E];
+  }
+  for (int s = 0; s < P.blk_mn; ++s) {
+    const int32_t F = P.blk_nb[int64_t(s) * ne + E];
+    if (F < 0 || F > E)
+      break;
+    const int t = P.blk_mir[int64_t(s) * ne + E];
+    for (int k = 0; k < 16; ++k)
+      ok = ok && v[int64_t(16 + 16 * s + k) * ne + E] == v[int64_t(16 + 16 * t + 4 * (k % 4) + k / 4) * ne + F];
+  }
+  if (!ok)
+    flags[1] = 1;
+}
+
+// scaled element blocks: Ah(i, j) = a_ij (h_i h_j); slots past the last neighbour hold 0
+__global__ void k_blk_scale(PcgParams P, const double* __restrict__ dh, double* __restrict__ dst)
+{
+  const int64_t E = xcd_index(P.blk_ne);
+  if (E >= P.blk_ne)
+    return;
+  const int64_t ne = P.blk_ne;
+  const double* __restrict__ v = P.blk;
+  for (int k = 0; k < 16; ++k)
+    dst[int64_t(k) * ne + E] = v[int64_t(k) * ne + E] * (dh[4 * E + k / 4] * dh[4 * E + k % 4]);
+  for (int s = 0; s < P.blk_mn; ++s) {
+    const int32_t F = P.blk_nb[int64_t(s) * ne + E];
+    for (int k = 0; k < 16; ++k) {
+      const int64_t m = int64_t(16 + 16 * s + k) * ne + E;
+      dst[m] = F < 0 ? 0. : v[m] * (dh[4 * E + k / 4] * dh[4 * int64_t(F) + k % 4]);
+    }
+  }
+}
+
 // Jacobi-preconditioned CG run as CG on the symmetrically scaled system
 //   Ah = D^{-1/2} A D^{-1/2},  xh = D^{1/2} x,  rh = D^{-1/2} r,
 // which in exact arithmetic has the iterates of Jacobi-PCG (z = D^{-1} r, rh.rh = r.z,
@@ -452,6 +711,33 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
         P.q[i] = acc[j];
         v[0] += p[i] * acc[j];
       }
+    }
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0)
+    P.part1[b] = v[0];
+}
+
+// q = Ah p and the block partials of p.q with element blocks (lane per element of the operator's row blocks)
+template <bool SYM>
+__global__ void __launch_bounds__(256) k_cg_spmv_blk(PcgParams P, int par)
+{
+  const int b = active_block(P.a_nb, P.a_list, P.rev);
+  if (b < 0)
+    return;
+  const int inst = P.a_inst[b];
+  if (P.S[(par ^ 1) * P.ninst + inst].done)
+    return;
+  const double* __restrict__ p = par ? P.p : P.p2; // p_j (written by the direction update of step j)
+  const int64_t i0 = P.a_row0[b] + 4 * int64_t(threadIdx.x);
+  double v[1] = {0.};
+  if (i0 < P.a_row0[b + 1]) {
+    double acc[4];
+    blk_rows<SYM, true>(P, P.scaled, i0 / 4, [&](int64_t c) { return p[c]; }, acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      P.q[i0 + i] = acc[i];
+      v[0] += p[i0 + i] * acc[i];
     }
   }
   block_sum<1>(v);
@@ -787,6 +1073,8 @@ void RowPartition::build_counts(const std::vector<int64_t>& inst_dof, const std:
   d_inst_block0.upload(inst_block0);
 }
 
+static bool blk_wanted(const Operator& op);
+
 // row-block partitions of the operator (host set-up, O(number of blocks))
 static void build_partitions(Operator& op)
 {
@@ -802,8 +1090,10 @@ static void build_partitions(Operator& op)
   // element of a tile of elements (discontinuous spaces: 256 lanes, 4 rows each)
   const bool mf = op.requested_format == HGDT_FORMAT_MATRIX_FREE;
   const bool mf_tiles = mf && s.kind == HGDT_DISCONTINUOUS_LAGRANGE;
-  const int block3 = mf_tiles ? 256 : mf ? std::min(op.launch[3].block, 256) : op.launch[3].block;
-  const int items3 = mf_tiles ? 4 : mf ? 1 : op.launch[3].items;
+  // element blocks: one element (4 rows) per lane, at most 256 lanes per block (registers for two blocks
+  // of loads in flight)
+  const int block3 = mf_tiles ? 256 : (mf || blk_wanted(op)) ? std::min(op.launch[3].block, 256) : op.launch[3].block;
+  const int items3 = (mf_tiles || blk_wanted(op)) ? 4 : mf ? 1 : op.launch[3].items;
   if (!op.launch[4].user) {
     op.launch[4].block = 256;
     const int64_t want = (op.n + 256 * 1024 - 1) / (256 * 1024);
@@ -824,6 +1114,108 @@ static void build_partitions(Operator& op)
 
 // storage structure of the operator (host set-up): DIA offsets and CSR -> DIA positions, or the
 // SELL-64 slices of the A partition
+// element-block storage is the default for discontinuous spaces with 4 DoFs per element
+static bool blk_wanted(const Operator& op)
+{
+  const Space& s = *op.space;
+  const bool fits = s.kind == HGDT_DISCONTINUOUS_LAGRANGE && s.local_size == 4 && op.n % 4 == 0 && op.n > 0;
+  return fits && (op.requested_format == HGDT_FORMAT_BLOCK || op.requested_format == HGDT_FORMAT_AUTO);
+}
+
+// 4 x 4 element blocks from the CSR pattern: false when the rows of an element do not share their columns
+// in whole aligned blocks of 4 including the element's own block (then another storage is used)
+static bool build_blocks(Operator& op)
+{
+  SolverFormat& F = op.fmt;
+  const int64_t ne = op.n / 4;
+  std::vector<int32_t> cnt(ne, 0);
+  std::atomic<bool> ok{true};
+  std::mutex m;
+  int mn = 0;
+  parallel_for(ne, [&](int64_t e0, int64_t e1) {
+    int lmax = 0;
+    for (int64_t E = e0; E < e1 && ok; ++E) {
+      const int64_t t0 = op.row_ptr[4 * E], len = op.row_ptr[4 * E + 1] - t0;
+      bool self = false;
+      bool good = len > 0 && len % 4 == 0;
+      for (int a = 0; a < 4 && good; ++a) {
+        const int64_t ta = op.row_ptr[4 * E + a];
+        good = op.row_ptr[4 * E + a + 1] - ta == len;
+        for (int64_t k = 0; k < len && good; ++k) {
+          const int64_t c0 = op.cols[t0 + (k / 4) * 4];
+          good = c0 % 4 == 0 && op.cols[ta + k] == c0 + k % 4;
+          self = self || c0 == 4 * E;
+        }
+      }
+      if (!good || !self) {
+        ok = false;
+        break;
+      }
+      cnt[E] = int32_t(len / 4 - 1);
+      lmax = std::max(lmax, cnt[E]);
+    }
+    std::lock_guard<std::mutex> lock(m);
+    mn = std::max(mn, lmax);
+  });
+  if (!ok)
+    return false;
+  // neighbour slots in ascending order; slot of the element in each neighbour's list
+  std::vector<int32_t> nb(std::max<int64_t>(int64_t(mn) * ne, 1), -1), mir(std::max<int64_t>(int64_t(mn) * ne, 1), -1);
+  parallel_for(ne, [&](int64_t e0, int64_t e1) {
+    for (int64_t E = e0; E < e1; ++E) {
+      const int64_t t0 = op.row_ptr[4 * E], nq = (op.row_ptr[4 * E + 1] - t0) / 4;
+      int s = 0;
+      for (int64_t q = 0; q < nq; ++q) {
+        const int32_t Fb = op.cols[t0 + 4 * q] / 4;
+        if (Fb != E)
+          nb[int64_t(s++) * ne + E] = Fb;
+      }
+    }
+  });
+  std::atomic<bool> sym_pattern{true};
+  int64_t up = 0, all = 0;
+  parallel_for(ne, [&](int64_t e0, int64_t e1) {
+    int64_t lup = 0, lall = 0;
+    for (int64_t E = e0; E < e1; ++E)
+      for (int s = 0; s < cnt[E]; ++s) {
+        const int32_t Fb = nb[int64_t(s) * ne + E];
+        ++lall;
+        lup += Fb > E;
+        int t = -1;
+        for (int u = 0; u < cnt[Fb]; ++u)
+          if (nb[int64_t(u) * ne + Fb] == E)
+            t = u;
+        if (t < 0)
+          sym_pattern = false;
+        mir[int64_t(s) * ne + E] = t;
+      }
+    std::lock_guard<std::mutex> lock(m);
+    up += lup, all += lall;
+  });
+  if (!sym_pattern)
+    return false;
+  std::vector<int64_t> pos(std::max<int64_t>(op.nnz, 1));
+  parallel_for(ne, [&](int64_t e0, int64_t e1) {
+    for (int64_t E = e0; E < e1; ++E)
+      for (int a = 0; a < 4; ++a) {
+        const int64_t ta = op.row_ptr[4 * E + a], len = op.row_ptr[4 * E + a + 1] - ta;
+        int s = 0;
+        for (int64_t q = 0; q < len / 4; ++q) {
+          const int32_t Fb = op.cols[ta + 4 * q] / 4;
+          const int64_t base = (Fb == E) ? 0 : 16 + 16 * int64_t(s++);
+          for (int c = 0; c < 4; ++c)
+            pos[ta + 4 * q + c] = (base + 4 * a + c) * ne + E;
+        }
+      }
+  });
+  F.blk_ne = ne, F.blk_mn = mn, F.blk_up = up, F.blk_all = all;
+  F.d_blk_nb.upload(nb);
+  F.d_blk_mir.upload(mir);
+  F.d_blk_pos.upload(pos);
+  F.d_blk.allocate(std::max<int64_t>(int64_t(16 + 16 * mn) * ne, 1));
+  return true;
+}
+
 static void build_structure(Operator& op)
 {
   SolverFormat& F = op.fmt;
@@ -842,6 +1234,22 @@ static void build_structure(Operator& op)
     F.valid = false;
     ++C.version;
     return;
+  }
+
+  F.d_blk_nb.release(), F.d_blk_mir.release(), F.d_blk.release(), F.d_blk_pos.release();
+  if (op.requested_format == HGDT_FORMAT_BLOCK && !blk_wanted(op))
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "block storage needs a discontinuous space with 4 DoFs per element");
+  if (blk_wanted(op)) {
+    if (build_blocks(op)) {
+      F.format = HGDT_FORMAT_BLOCK;
+      F.ndiag = F.blk_mn, F.nup = 0;
+      F.structure_ready = true;
+      F.valid = false;
+      ++C.version;
+      return;
+    }
+    if (op.requested_format == HGDT_FORMAT_BLOCK)
+      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "block storage: the pattern does not consist of whole 4 x 4 element blocks");
   }
 
   // DIA detection: distinct (col - row) offsets per instance
@@ -977,6 +1385,8 @@ static int64_t apply_bytes(const Operator& op)
   const SolverFormat& F = op.fmt;
   if (F.format == HGDT_FORMAT_MATRIX_FREE)
     return mf_bytes(op, MF_APPLY);
+  if (F.format == HGDT_FORMAT_BLOCK) // own block (upper triangle if symmetric), neighbour blocks read
+    return 8 * ((F.symmetric ? 10 : 16) * F.blk_ne + 16 * (F.symmetric ? F.blk_up : F.blk_all)) + 16 * op.n;
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -989,6 +1399,8 @@ static int64_t step_bytes(const Operator& op)
   const SolverFormat& F = op.fmt;
   if (F.format == HGDT_FORMAT_MATRIX_FREE) // coefficients and vectors (with the direction update when fused)
     return mf_bytes(op, mf_fused_direction(op) ? MF_PCG_DIR : MF_PCG);
+  if (F.format == HGDT_FORMAT_BLOCK) // the unit diagonal is not read
+    return 8 * ((F.symmetric ? 6 : 12) * F.blk_ne + 16 * (F.symmetric ? F.blk_up : F.blk_all)) + 16 * op.n;
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t((F.symmetric ? F.nup : F.ndiag) - 1) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
@@ -1017,6 +1429,11 @@ static PcgParams make_params(Operator& op)
   P.slice_ptr = F.d_slice_ptr.ptr;
   P.sell_cols = F.d_sell_cols.ptr;
   P.sell_vals = F.d_sell_vals.ptr;
+  P.blk_ne = F.blk_ne;
+  P.blk_mn = F.blk_mn;
+  P.blk_nb = F.d_blk_nb.ptr;
+  P.blk_mir = F.d_blk_mir.ptr;
+  P.blk = F.d_blk.ptr;
   P.scaled = F.d_scaled.ptr;
   P.dh = F.d_dh.ptr;
   P.dsq = F.d_dsq.ptr;
@@ -1028,6 +1445,8 @@ static int kernel_format(const Operator& op)
 {
   if (op.fmt.format == HGDT_FORMAT_SELL)
     return FMT_SELL;
+  if (op.fmt.format == HGDT_FORMAT_BLOCK)
+    return FMT_BLK;
   return op.fmt.symmetric ? FMT_DIAS : FMT_DIA;
 }
 
@@ -1065,6 +1484,32 @@ void Operator::prepare_solver(hipStream_t stream)
     HGDT_THROW(HGDT_ERR_OPERATOR,
                "the operator was assembled in matrix-free mode and holds no matrix: assemble again in this format");
   const int B = 256;
+  if (F.format == HGDT_FORMAT_BLOCK) {
+    const int64_t total = int64_t(16 + 16 * F.blk_mn) * F.blk_ne;
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, F.d_blk.ptr, 0.);
+    hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
+                       F.d_blk_pos.ptr, F.d_blk.ptr);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
+    const PcgParams P = make_params(*this);
+    hipLaunchKernelGGL(k_blk_prepare, dim3(blocks_for(F.blk_ne, B)), dim3(B), 0, stream, P, F.d_dh.ptr, F.d_dsq.ptr,
+                       C.flags.ptr);
+    HGDT_HIP(hipGetLastError());
+    HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HGDT_HIP(hipStreamSynchronize(stream));
+    F.diag_positive = (C.host_flags[0] == 0);
+    const bool sym = C.host_flags[1] == 0 && allow_symmetric;
+    F.d_scaled.allocate(std::max<int64_t>(total, 1));
+    hipLaunchKernelGGL(k_blk_scale, dim3(blocks_for(F.blk_ne, B)), dim3(B), 0, stream, P, F.d_dh.ptr,
+                       F.d_scaled.ptr);
+    HGDT_HIP(hipGetLastError());
+    if (sym != F.symmetric)
+      ++C.version; // captured graphs hold the kernel variant
+    F.symmetric = sym;
+    F.bytes = apply_bytes(*this);
+    F.step_bytes = step_bytes(*this);
+    F.valid = true;
+    return;
+  }
   const bool dia = F.format == HGDT_FORMAT_DIA;
   double* dst = dia ? F.d_dia.ptr : F.d_sell_vals.ptr;
   const int64_t total = dia ? int64_t(F.ndiag) * n : int64_t(F.d_sell_vals.count);
@@ -1117,6 +1562,14 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
   const int block = fmt.A.block;
   const unsigned grid = hw_blocks(fmt.A.nblocks);
   const int kf = kernel_format(*this);
+  if (kf == FMT_BLK) {
+    if (fmt.symmetric)
+      hipLaunchKernelGGL(k_blk_apply<true>, dim3(grid), dim3(block), 0, stream, P, x, y);
+    else
+      hipLaunchKernelGGL(k_blk_apply<false>, dim3(grid), dim3(block), 0, stream, P, x, y);
+    HGDT_HIP(hipGetLastError());
+    return;
+  }
   dispatch_items(apply_lane_rows(fmt.A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     if (kf == FMT_DIA)
@@ -1174,6 +1627,12 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
       v.r = P.r, v.xh = P.x, v.part2 = P.part2, v.v_ib0 = P.v_ib0, v.a_ib0 = P.a_ib0, v.max_iter = P.max_iter;
     }
     mf_launch(op, fused ? MF_PCG_DIR : MF_PCG, v, stream, ev0, ev1);
+  } else if (kf == FMT_BLK) {
+    const unsigned grid = hw_blocks(P.a_nb);
+    if (op.fmt.symmetric)
+      launch_pcg(k_cg_spmv_blk<true>, grid, A.block, stream, Pa, par, ev0, ev1);
+    else
+      launch_pcg(k_cg_spmv_blk<false>, grid, A.block, stream, Pa, par, ev0, ev1);
   } else dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(P.a_nb);

@@ -90,7 +90,7 @@ QUAD, TRI, HEX, TET = 0, 1, 2, 3
 CONTINUOUS_LAGRANGE, DISCONTINUOUS_LAGRANGE = 0, 1
 STENCIL_ELEMENT, STENCIL_INTERSECTION, STENCIL_ELEMENT_AND_INTERSECTION = 0, 1, 2
 SOURCE_CONSTANT, SOURCE_PER_ELEMENT, SOURCE_PER_QUADRATURE_POINT = 0, 1, 2
-FORMAT_AUTO, FORMAT_SELL, FORMAT_DIA, FORMAT_DIA_NONSYMMETRIC, FORMAT_MATRIX_FREE = 0, 1, 2, 3, 4
+FORMAT_AUTO, FORMAT_SELL, FORMAT_DIA, FORMAT_DIA_NONSYMMETRIC, FORMAT_MATRIX_FREE, FORMAT_BLOCK = 0, 1, 2, 3, 4, 5
 BOUNDARY_DIRICHLET_COUPLING, BOUNDARY_PENALTY, BOUNDARY_PRODUCT = 0, 1, 2
 
 KERNEL_ELEMENT, KERNEL_INTERSECTION, KERNEL_GATHER, KERNEL_APPLY, KERNEL_VECTOR = 0, 1, 2, 3, 4

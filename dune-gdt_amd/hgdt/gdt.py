@@ -768,19 +768,21 @@ class MatrixOperator:
 
     # -- storage / launch configuration --
     _FORMATS = {"auto": L.FORMAT_AUTO, "sell64": L.FORMAT_SELL, "dia_symmetric": L.FORMAT_DIA,
-                "dia": L.FORMAT_DIA_NONSYMMETRIC, "matrix_free": L.FORMAT_MATRIX_FREE}
+                "dia": L.FORMAT_DIA_NONSYMMETRIC, "matrix_free": L.FORMAT_MATRIX_FREE, "block4": L.FORMAT_BLOCK}
 
     def set_format(self, fmt):
-        """storage of apply / apply_inverse: "auto", "sell64", "dia_symmetric", "dia", or "matrix_free"
-        (no stored matrix: the operator is recomputed from the coefficient fields at every application, as
-        LocalizableOperator::apply does; assemble() then builds the right-hand side only)"""
+        """storage of apply / apply_inverse: "auto", "sell64", "dia_symmetric", "dia", "block4" (4 x 4
+        element blocks of a discontinuous space with 4 DoFs per element, the default for such spaces) or
+        "matrix_free" (no stored matrix: the operator is recomputed from the coefficient fields at every
+        application, as LocalizableOperator::apply does; assemble() then builds the right-hand side only)"""
         L.call("hgdt_operator_set_format", self._h, self._FORMATS[fmt] if isinstance(fmt, str) else int(fmt))
 
     def format_info(self):
         f, b, d = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
         L.call("hgdt_operator_format_info", self._h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(d))
         return {"format": {L.FORMAT_SELL: "sell64", L.FORMAT_DIA: "dia_symmetric",
-                           L.FORMAT_DIA_NONSYMMETRIC: "dia", L.FORMAT_MATRIX_FREE: "matrix_free"}[f.value],
+                           L.FORMAT_DIA_NONSYMMETRIC: "dia", L.FORMAT_MATRIX_FREE: "matrix_free",
+                           L.FORMAT_BLOCK: "block4"}[f.value],
                 "spmv_bytes": int(b.value),
                 "num_diagonals": int(d.value)}
 

@@ -59,7 +59,8 @@ enum {
   HGDT_FORMAT_SELL = 1,
   HGDT_FORMAT_DIA = 2,
   HGDT_FORMAT_DIA_NONSYMMETRIC = 3,
-  HGDT_FORMAT_MATRIX_FREE = 4
+  HGDT_FORMAT_MATRIX_FREE = 4,
+  HGDT_FORMAT_BLOCK = 5
 };
 
 typedef struct hgdt_grid_s* hgdt_grid;
@@ -245,8 +246,9 @@ hgdt_status hgdt_operator_rhs_device(hgdt_operator op, double** rhs);
 hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out);
 hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
 
-/* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: DIA when every
- * instance has at most 32 distinct diagonals, else SELL-64).  DIA reads only the diagonals with
+/* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: element blocks for a
+ * discontinuous space with 4 DoFs per element, else DIA when every instance has at most 32 distinct
+ * diagonals, else SELL-64).  DIA reads only the diagonals with
  * offset >= 0 when the assembled matrix is exactly (bitwise) symmetric;
  * HGDT_FORMAT_DIA_NONSYMMETRIC always reads every diagonal.  The format is rebuilt from the CSR
  * values at the next apply / apply_inverse after each assemble.
@@ -261,11 +263,19 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
  * Dirichlet constraints, discontinuous Lagrange on quadrilaterals with element forms and the SWIPDG
  * intersection forms.  Results agree with the assembled operator's to round-off (the sums are
  * associated differently); set_format succeeds for any operator, and the first apply / apply_inverse
- * fails with HGDT_ERR_NOT_SUPPORTED, naming the reason, for one outside this set. */
+ * fails with HGDT_ERR_NOT_SUPPORTED, naming the reason, for one outside this set.
+ *
+ * HGDT_FORMAT_BLOCK: the 4 x 4 blocks of a discontinuous space with 4 DoFs per element (Q1 on
+ * quadrilaterals), one record per element (its own block, then the blocks of its neighbours in ascending
+ * order).  When the assembled matrix is exactly symmetric, an application reads only the own block's
+ * upper triangle and the blocks of the neighbours with larger index, and takes the others from the
+ * neighbours' records (transposed).  The default for such spaces (HGDT_FORMAT_AUTO); every row sums its
+ * terms in ascending column order, as DIA and SELL do, so apply gives the same bits in every format. */
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format);
 /* format actually in use after the last preparation (HGDT_FORMAT_SELL, HGDT_FORMAT_DIA for symmetric
- * DIA, HGDT_FORMAT_DIA_NONSYMMETRIC, HGDT_FORMAT_MATRIX_FREE) and the algorithmic bytes one application
- * reads and writes */
+ * DIA, HGDT_FORMAT_DIA_NONSYMMETRIC, HGDT_FORMAT_MATRIX_FREE, HGDT_FORMAT_BLOCK) and the algorithmic
+ * bytes one application reads and writes (num_diagonals: neighbour blocks per element for
+ * HGDT_FORMAT_BLOCK) */
 hgdt_status hgdt_operator_format_info(hgdt_operator op, int32_t* format, int64_t* spmv_bytes, int32_t* num_diagonals);
 
 /* ConstMatrixOperator::apply (matrix-based.hh:121-129): y = A x, device vectors of length rows */

@@ -734,7 +734,8 @@ enum class Format {
   sell = HGDT_FORMAT_SELL,
   dia = HGDT_FORMAT_DIA,
   dia_nonsymmetric = HGDT_FORMAT_DIA_NONSYMMETRIC,
-  matrix_free = HGDT_FORMAT_MATRIX_FREE
+  matrix_free = HGDT_FORMAT_MATRIX_FREE,
+  block = HGDT_FORMAT_BLOCK
 };
 
 class VectorFunctional

@@ -100,7 +100,10 @@ def test_apply_parity(hgdt_mod, kind, cells, seed):
     yo = O.spmv(orp, ocols, oop.values(), x)
     L = hgdt_mod._lib
     ys = {}
-    for fmt in (L.FORMAT_AUTO, L.FORMAT_DIA_NONSYMMETRIC, L.FORMAT_SELL):
+    fmts = [L.FORMAT_AUTO, L.FORMAT_DIA, L.FORMAT_DIA_NONSYMMETRIC, L.FORMAT_SELL]
+    if kind == "swipdg":
+        fmts.append(L.FORMAT_BLOCK)
+    for fmt in fmts:
         op.set_format(fmt)
         y = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
         err = np.abs(y - yo).max() / np.abs(yo).max()
@@ -109,10 +112,11 @@ def test_apply_parity(hgdt_mod, kind, cells, seed):
         print(f"  {name}: max rel err {err:.3e}")
         assert err <= 1e-13
     # the matrices (continuous Lagrange, and symmetric interior penalty) are assembled exactly symmetric, so
-    # the symmetric DIA path is taken and, summing every row in column order, agrees bit for bit with full
-    # DIA and SELL
-    assert "dia_symmetric" in ys
-    assert np.array_equal(ys["dia_symmetric"], ys["dia"]) and np.array_equal(ys["dia"], ys["sell64"])
+    # the symmetric DIA (and, for the discontinuous space, element-block) paths are taken and, summing every
+    # row in column order, agree bit for bit with full DIA and SELL
+    assert "dia_symmetric" in ys and ("block4" in ys) == (kind == "swipdg")
+    for name, y in ys.items():
+        assert np.array_equal(y, ys["sell64"]), name
 
 
 @pytest.mark.parametrize("kind,cells,seed", [CASES[1], CASES[4], CASES[6]])
@@ -141,8 +145,13 @@ def test_nonsymmetric_interior_penalty(hgdt_mod):
     x = np.random.default_rng(5).standard_normal(op.shape[0])
     yo = O.spmv(orp, ocols, oop.values(), x)
     y = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
-    assert op.format_info()["format"] == "dia"
+    # element blocks read in full (the transposed blocks differ), the same bits as DIA
+    assert op.format_info()["format"] == "block4"
     assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    op.set_format(hgdt_mod._lib.FORMAT_DIA)
+    y2 = op.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert op.format_info()["format"] == "dia"
+    assert np.array_equal(y, y2)
 
 
 def test_bitwise_reproducible(hgdt_mod):
