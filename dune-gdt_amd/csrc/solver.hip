@@ -473,18 +473,16 @@ __device__ __forceinline__ void blk_rows(const PcgParams& P, const double* __res
     blk_rows_seq<SYM, UNIT>(P, v, E, xs, y);
 }
 
-// y = A x with element blocks (unscaled values), one lane per element of the operator's row blocks
+// y = A x with element blocks (unscaled values), one lane per element (XCD-aware 1-d launch of 256 lanes)
 template <bool SYM>
 __global__ void __launch_bounds__(256) k_blk_apply(PcgParams P, const double* __restrict__ x, double* __restrict__ y)
 {
-  const int b = logical_block(P.a_nb);
-  if (b < 0)
+  const int64_t E = xcd_index(P.blk_ne);
+  if (E >= P.blk_ne)
     return;
-  const int64_t i0 = P.a_row0[b] + 4 * int64_t(threadIdx.x);
-  if (i0 >= P.a_row0[b + 1])
-    return;
+  const int64_t i0 = 4 * E;
   double acc[4];
-  blk_rows<SYM, false>(P, P.blk, i0 / 4, [&](int64_t c) { return x[c]; }, acc);
+  blk_rows<SYM, false>(P, P.blk, E, [&](int64_t c) { return x[c]; }, acc);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     y[i0 + i] = acc[i];
@@ -1563,10 +1561,11 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
   const unsigned grid = hw_blocks(fmt.A.nblocks);
   const int kf = kernel_format(*this);
   if (kf == FMT_BLK) {
+    const unsigned g1 = xcd_grid(fmt.blk_ne, 256);
     if (fmt.symmetric)
-      hipLaunchKernelGGL(k_blk_apply<true>, dim3(grid), dim3(block), 0, stream, P, x, y);
+      hipLaunchKernelGGL(k_blk_apply<true>, dim3(g1), dim3(256), 0, stream, P, x, y);
     else
-      hipLaunchKernelGGL(k_blk_apply<false>, dim3(grid), dim3(block), 0, stream, P, x, y);
+      hipLaunchKernelGGL(k_blk_apply<false>, dim3(g1), dim3(256), 0, stream, P, x, y);
     HGDT_HIP(hipGetLastError());
     return;
   }
