@@ -261,3 +261,41 @@ def test_element_blocks_on_a_union(hgdt_mod):
     rel = np.linalg.norm(xs[L.FORMAT_BLOCK] - xs[L.FORMAT_DIA]) / np.linalg.norm(xs[L.FORMAT_DIA])
     print(f"  block vs symmetric DIA solution: rel L2 {rel:.2e}")
     assert rel <= 1e-12
+
+
+def test_element_blocks_without_neighbours(hgdt_mod):
+    """a discontinuous operator with element forms only (block diagonal, no neighbour slots): element-block
+    storage gives SELL's bits and the oracle's solution"""
+    H = hgdt_mod
+    L = H._lib
+    print("case element blocks, block-diagonal DG mass + Laplace", flush=True)
+    cells = [9, 7]
+    kappa = H.fields.lognormal(23, cells[0] * cells[1])
+    grid = H.Grid.structured("quad", [0, 0], [1, 1], cells)
+    space = H.DiscontinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kappa).cuda())))
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(1.0)))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    op.assemble()
+    m = O.Mesh.structured(O.QUAD, cells, [0, 0], [1, 1])
+    s = O.Space(m, False, 1)
+    oop = O.Operator(s, O.STENCIL_ELEMENT)
+    oop.append_laplace(kappa)
+    oop.append_product(1.0)
+    oop.append_source(1.0)
+    oop.assemble(1)
+    x = torch.from_numpy(np.random.default_rng(2).standard_normal(space.num_DoFs)).cuda()
+    y = op.apply(x).cpu().numpy()
+    assert op.format_info()["format"] == "block4" and op.format_info()["num_diagonals"] == 0
+    op.set_format(L.FORMAT_SELL)
+    assert np.array_equal(y, op.apply(x).cpu().numpy())
+    op.set_format(L.FORMAT_AUTO)
+    rp, cols = oop.pattern()
+    xo, ito, relo, rc = O.pcg_jacobi(rp, cols, oop.values(), oop.rhs(), rtol=1e-10)
+    xg = op.apply_inverse("rhs").cpu().numpy()
+    rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print(f"  iterations {op.last_solve['iterations']} (oracle {ito}), rel L2 {rel:.2e}")
+    assert rel <= 1e-10 and abs(op.last_solve["iterations"] - ito) <= 2
