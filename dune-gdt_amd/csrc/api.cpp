@@ -822,6 +822,16 @@ hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t b
   });
 }
 
+hgdt_status hgdt_operator_set_column_staging(hgdt_operator op, int32_t mode)
+{
+  return guarded([&] {
+    require(op && mode >= -1 && mode <= 1, HGDT_ERR_INVALID_ARGUMENT, "column staging mode must be -1, 0 or 1");
+    if (op->op->column_staging != mode)
+      op->op->fmt.partitions_ready = false; // rebuilt before the next solve (captured graphs too)
+    op->op->column_staging = mode;
+  });
+}
+
 hgdt_status hgdt_operator_get_launch(hgdt_operator op, int32_t kernel, int32_t* block_size, int32_t* items_per_thread)
 {
   return guarded([&] {

@@ -341,6 +341,9 @@ struct Operator {
   bool allow_symmetric = true;
   SolverFormat fmt;
   Launch launch[5];
+  // p entries of the solver's symmetric DIA application staged in LDS: -1 automatic, 0 never, 1 wherever
+  // the stencil allows it
+  int column_staging = -1;
   // solver workspace
   DeviceArray<double> d_r, d_p, d_q;
   DeviceArray<double> d_p2; // second p buffer of the matrix-free solver with the fused direction update
@@ -356,6 +359,7 @@ struct Operator {
   void prepare_assembly();
   void assemble(hipStream_t stream);
   void prepare_solver(hipStream_t stream);
+  void prepare_storage(hipStream_t stream); // storage format, Jacobi scaling, scaled values
   void apply(const double* x, double* y, hipStream_t stream);
   void apply_inverse(const double* b, double* x, const hgdt_solver_options& opt, hgdt_solver_stats& st,
                      hipStream_t stream);

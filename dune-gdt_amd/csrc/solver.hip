@@ -53,6 +53,25 @@ namespace {
 constexpr int MAXDIAG = 32;
 enum { FMT_DIA = HGDT_FORMAT_DIA, FMT_SELL = HGDT_FORMAT_SELL, FMT_DIAS = 3, FMT_BLK = HGDT_FORMAT_BLOCK };
 
+// column windows of an instance's symmetric DIA stencil: the offsets 0 and +-o of its strict upper
+// diagonals, grouped into at most WIN_MAX ranges [lo, hi] (relative to the row) with gaps of at most
+// WIN_GAP inside a range; offsets in no range never meet a column of the instance
+constexpr int WIN_MAX = 4, WIN_GAP = 512, WIN_MAXU = 8;
+constexpr int64_t WIN_LDS = 64 * 1024;
+// automatic choice: staged when the application moves more than the 256 MiB last-level cache (MALL) of the
+// GPU holds; below that, the direct kernel, whose cache-line requests the MALL then serves, is faster
+constexpr int64_t WIN_AUTO_BYTES = int64_t(256) << 20;
+struct DiaWindows {
+  int32_t nwin;
+  int32_t span[WIN_MAX];  // hi_w - lo_w
+  int32_t cspan[WIN_MAX]; // sum of the spans of the windows before w
+  int32_t zero_w, zero_rel; // the window holding offset 0, and its position in it (cspan_w + 0 - lo_w)
+  // the window of p(i - o_u) and p(i + o_u) of strict upper slot u (-1: no column of the instance) and its
+  // position cspan_w + (+-o_u - lo_w): LDS index = lane + w R + position for blocks of R rows
+  int32_t dn_w[WIN_MAXU], dn_rel[WIN_MAXU], up_w[WIN_MAXU], up_rel[WIN_MAXU];
+  int64_t lo[WIN_MAX];
+};
+
 struct PcgParams {
   int64_t n;
   // A: row partition of the operator application
@@ -108,6 +127,7 @@ struct PcgParams {
   double rtol2;
   int64_t max_iter;
   int zero_guess;
+  const DiaWindows* win; // ninst: column windows of the DIA stencils (p staged in LDS)
 };
 
 // y_i = sum_j A_ij v(j) for the ITEMS rows r0 + threadIdx.x + j * blockDim.x (< r1) of block b, whose rows
@@ -716,6 +736,101 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
     P.part1[b] = v[0];
 }
 
+// q = Ah p and the block partials of p.q with symmetric DIA storage, one row per lane (a block's rows at
+// most blockDim.x), at most WIN_MAXU strict upper diagonals, and the p entries of the block's rows staged
+// in LDS: every column window row + [lo_w, hi_w] of the instance is loaded once per block (coalesced),
+// and each row reads its 2 ku + 1 entries of p from LDS instead of from the cache hierarchy.  The FMAs
+// run in the order of the direct kernel (lower part by descending offset, the unit diagonal, the upper
+// part by ascending offset), so q and the partials have the same bits.
+__global__ void __launch_bounds__(1024) k_cg_spmv_win(PcgParams P, int par)
+{
+  extern __shared__ double win[];
+  const int b = active_block(P.a_nb, P.a_list, P.rev);
+  if (b < 0)
+    return;
+  const int inst = P.a_inst[b];
+  if (P.S[(par ^ 1) * P.ninst + inst].done)
+    return;
+  const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
+  const double* __restrict__ p = par ? P.p : P.p2; // p_j (written by the direction update of step j)
+  const int64_t lo = P.inst_row0[inst], hi = P.inst_row0[inst + 1];
+  const int64_t* __restrict__ off = P.dia_off + int64_t(inst) * P.ndiag;
+  const double* __restrict__ dia = P.scaled;
+  const int64_t n = P.n;
+  const int k0 = P.dia_k0[inst];
+  const int ku = P.ndiag - 1 - k0; // <= WIN_MAXU (checked by the host)
+  const DiaWindows& W = P.win[inst];
+  const int t = threadIdx.x, R = blockDim.x;
+  const int64_t i = rb + t;
+  const bool in = i < r1;
+  const int rows = int(r1 - rb);
+  // the windows of p first, into registers (a window spans at most R + rows <= 2 R entries: two per
+  // lane): window w holds p(rb + lo_w + s), s < rows + span_w, at w R + cspan_w + s (0 outside the
+  // instance)
+  double pw[WIN_MAX][2];
+#pragma unroll
+  for (int w = 0; w < WIN_MAX; ++w)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      pw[w][j] = 0.;
+      if (w < W.nwin) {
+        const int s = t + j * R;
+        const int64_t g = rb + W.lo[w] + s;
+        if (s < rows + W.span[w] && g >= lo && g < hi)
+          pw[w][j] = p[g];
+      }
+    }
+  // then the matrix values (the lower part from the mirrored rows), whose loads stay in flight across the
+  // staging of p
+  double al[WIN_MAXU], au[WIN_MAXU];
+#pragma unroll
+  for (int u = 0; u < WIN_MAXU; ++u) {
+    al[u] = au[u] = 0.;
+    if (u < ku) {
+      const int64_t o = off[k0 + 1 + u];
+      const double* __restrict__ du = dia + int64_t(k0 + 1 + u) * n;
+      if (in && i - o >= lo)
+        al[u] = du[i - o];
+      if (in && i + o < hi)
+        au[u] = du[i];
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < WIN_MAX; ++w)
+    if (w < W.nwin)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        if (t + j * R < rows + W.span[w])
+          win[w * R + W.cspan[w] + t + j * R] = pw[w][j];
+  __syncthreads();
+  double xl[WIN_MAXU], xu[WIN_MAXU];
+#pragma unroll
+  for (int u = 0; u < WIN_MAXU; ++u) {
+    xl[u] = (u < ku && W.dn_w[u] >= 0) ? win[t + W.dn_w[u] * R + W.dn_rel[u]] : 0.;
+    xu[u] = (u < ku && W.up_w[u] >= 0) ? win[t + W.up_w[u] * R + W.up_rel[u]] : 0.;
+  }
+  const double pi = win[t + W.zero_w * R + W.zero_rel];
+  double s = 0.;
+#pragma unroll
+  for (int u = WIN_MAXU - 1; u >= 0; --u) // lower part, descending offset
+    if (u < ku)
+      s = fma(al[u], xl[u], s);
+  if (in)
+    s += pi;
+#pragma unroll
+  for (int u = 0; u < WIN_MAXU; ++u) // upper part, ascending offset
+    if (u < ku)
+      s = fma(au[u], xu[u], s);
+  double v[1] = {0.};
+  if (in) {
+    P.q[i] = s;
+    v[0] += pi * s;
+  }
+  block_sum<1>(v);
+  if (t == 0)
+    P.part1[b] = v[0];
+}
+
 // q = Ah p and the block partials of p.q with element blocks (lane per element of the operator's row blocks)
 template <bool SYM>
 __global__ void __launch_bounds__(256) k_cg_spmv_blk(PcgParams P, int par)
@@ -970,6 +1085,15 @@ struct SolverCache {
   DeviceArray<int64_t> block_slice0;
   DeviceArray<int32_t> dia_k0;
   DeviceArray<int32_t> dia_mirror;
+  // column windows of the DIA stencils (k_cg_spmv_win): usable when every instance has at most WIN_MAX
+  // windows and WIN_MAXU strict upper diagonals; the LDS of a block of R rows is at most
+  // (win_nmax R + win_span) doubles, and blocks of R rows load a window with two entries per lane when its
+  // span is at most R (win_wmax: the largest span)
+  std::vector<DiaWindows> win_host;
+  DeviceArray<DiaWindows> win; // uploaded when the staged kernel is first used
+  bool win_ok = false;
+  int win_nmax = 0;
+  int64_t win_span = 0, win_wmax = 0;
   DeviceArray<int32_t> row_inst;
   DeviceArray<int> flags; // [0] non-positive diagonal, [1] asymmetric
   int* host_flags = nullptr;
@@ -1214,12 +1338,96 @@ static bool build_blocks(Operator& op)
   return true;
 }
 
+// column windows of every instance's stencil for the symmetric DIA kernel with p staged in LDS: the
+// sorted offsets {0, +-o : o a strict upper offset inside the instance} split where two neighbours are
+// more than WIN_GAP apart.  Not usable (win_ok false) when an instance has more than WIN_MAXU strict upper
+// slots or more than WIN_MAX windows.
+static void plan_windows(Operator& op, const std::vector<std::vector<int64_t>>& offs, int nd,
+                         const std::vector<int32_t>& k0)
+{
+  SolverCache& C = cache_of(op);
+  const Space& s = *op.space;
+  const int ninst = int(offs.size());
+  std::vector<DiaWindows> w(ninst);
+  bool ok = true;
+  int nmax = 0;
+  int64_t span = 0, wmax = 0;
+  for (int in = 0; in < ninst && ok; ++in) {
+    const int64_t size = s.inst_dof[in + 1] - s.inst_dof[in];
+    ok = nd - 1 - k0[in] <= WIN_MAXU;
+    std::vector<int64_t> o{0};
+    for (size_t k = size_t(k0[in]) + 1; k < offs[in].size(); ++k)
+      if (offs[in][k] < size)
+        o.push_back(offs[in][k]), o.push_back(-offs[in][k]);
+    std::sort(o.begin(), o.end());
+    std::vector<int64_t> lo, hi;
+    for (size_t j = 0; j < o.size(); ++j)
+      if (j == 0 || o[j] - o[j - 1] > WIN_GAP)
+        lo.push_back(o[j]), hi.push_back(o[j]);
+      else
+        hi.back() = o[j];
+    if (lo.size() > size_t(WIN_MAX)) {
+      ok = false;
+      break;
+    }
+    DiaWindows& d = w[in];
+    d = DiaWindows{};
+    d.nwin = int32_t(lo.size());
+    int64_t cs = 0;
+    for (int v = 0; v < d.nwin; ++v) {
+      d.lo[v] = lo[v];
+      d.span[v] = int32_t(std::min<int64_t>(hi[v] - lo[v], INT32_MAX / 2));
+      d.cspan[v] = int32_t(std::min<int64_t>(cs, INT32_MAX / 2));
+      cs += hi[v] - lo[v];
+      wmax = std::max(wmax, hi[v] - lo[v]);
+    }
+    // window and position of an offset (-1: in no window, i.e. never a column of the instance)
+    auto place = [&](int64_t x, int32_t& win, int32_t& rel) {
+      win = -1, rel = 0;
+      for (int v = 0; v < d.nwin; ++v)
+        if (x >= d.lo[v] && x <= d.lo[v] + d.span[v])
+          win = v, rel = int32_t(d.cspan[v] + (x - d.lo[v]));
+    };
+    place(0, d.zero_w, d.zero_rel);
+    for (int u = 0; u < WIN_MAXU; ++u) {
+      d.dn_w[u] = d.up_w[u] = -1;
+      const size_t k = size_t(k0[in]) + 1 + u;
+      if (k < offs[in].size() && offs[in][k] < size) {
+        place(-offs[in][k], d.dn_w[u], d.dn_rel[u]);
+        place(offs[in][k], d.up_w[u], d.up_rel[u]);
+      }
+    }
+    nmax = std::max(nmax, int(d.nwin));
+    span = std::max(span, cs);
+  }
+  C.win_ok = ok;
+  C.win_nmax = nmax, C.win_span = span, C.win_wmax = wmax;
+  C.win.release();
+  C.win_host = ok ? std::move(w) : std::vector<DiaWindows>{};
+}
+
+// bytes of LDS that k_cg_spmv_win needs for row blocks of `block` rows (0: the kernel is not used)
+static int64_t win_lds_bytes(const Operator& op)
+{
+  const SolverFormat& F = op.fmt;
+  if (!op.solver_cache)
+    return 0;
+  const SolverCache& C = *op.solver_cache;
+  if (!C.win_ok || !F.symmetric || F.format != HGDT_FORMAT_DIA || F.A.items != 1 || C.win_wmax > F.A.block)
+    return 0;
+  if (op.column_staging == 0 || (op.column_staging < 0 && F.step_bytes <= WIN_AUTO_BYTES))
+    return 0;
+  const int64_t bytes = (int64_t(C.win_nmax) * F.A.block + C.win_span) * int64_t(sizeof(double));
+  return bytes <= WIN_LDS ? bytes : 0;
+}
+
 static void build_structure(Operator& op)
 {
   SolverFormat& F = op.fmt;
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
   const int ninst = int(s.inst_dof.size()) - 1;
+  C.win_ok = false;
   F.d_dia.release(), F.d_dia_pos.release(), F.d_sell_vals.release(), F.d_sell_cols.release();
   F.d_sell_pos.release(), F.d_scaled.release();
   F.d_mf_ref.release();
@@ -1322,6 +1530,7 @@ static void build_structure(Operator& op)
           mirror[int64_t(in) * nd + k] = int32_t(it - offs[in].begin());
       }
     C.dia_mirror.upload(mirror);
+    plan_windows(op, offs, nd, k0);
     F.d_dia_offsets.upload(table);
     C.dia_k0.upload(k0);
     C.row_inst.upload(row_inst);
@@ -1423,6 +1632,7 @@ static PcgParams make_params(Operator& op)
   P.dia_off = F.d_dia_offsets.ptr;
   P.dia_k0 = C.dia_k0.ptr;
   P.dia = F.d_dia.ptr;
+  P.win = C.win.ptr;
   P.block_slice0 = C.block_slice0.ptr;
   P.slice_ptr = F.d_slice_ptr.ptr;
   P.sell_cols = F.d_sell_cols.ptr;
@@ -1449,6 +1659,15 @@ static int kernel_format(const Operator& op)
 }
 
 void Operator::prepare_solver(hipStream_t stream)
+{
+  prepare_storage(stream);
+  // the column windows on the device once the staged application is chosen (set-up, never captured)
+  SolverCache& C = cache_of(*this);
+  if (win_lds_bytes(*this) > 0 && !C.win.ptr)
+    C.win.upload(C.win_host);
+}
+
+void Operator::prepare_storage(hipStream_t stream)
 {
   SolverFormat& F = fmt;
   SolverCache& C = cache_of(*this);
@@ -1585,17 +1804,17 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
 // kernel, the interval a kernel trace reports), not by separate markers around it
 template <class K>
 static void launch_pcg(K kernel, unsigned grid, int block, hipStream_t stream, const PcgParams& P, int par,
-                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, size_t lds = 0)
 {
   if (!ev0) {
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, P, par);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, P, par);
     return;
   }
   PcgParams p = P;
   int q = par;
   void* args[] = {&p, &q};
-  HGDT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(block), args, 0, stream, ev0,
-                              ev1, 0));
+  HGDT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(block), args, lds, stream,
+                              ev0, ev1, 0));
 }
 
 static void launch_iteration(const Operator& op, const PcgParams& P, int par, hipStream_t stream, hipEvent_t ev0,
@@ -1635,8 +1854,11 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   } else dispatch_items(apply_lane_rows(A.items), [&](auto IT) {
     constexpr int I = decltype(IT)::value;
     const unsigned grid = hw_blocks(P.a_nb);
+    const int64_t lds = win_lds_bytes(op); // > 0: the p entries of a row block staged in LDS
     if (kf == FMT_DIA)
       launch_pcg(k_cg_spmv<FMT_DIA, I>, grid, A.block, stream, Pa, par, ev0, ev1);
+    else if (kf == FMT_DIAS && lds > 0 && P.win)
+      launch_pcg(k_cg_spmv_win, grid, A.block, stream, Pa, par, ev0, ev1, size_t(lds));
     else if (kf == FMT_DIAS)
       launch_pcg(k_cg_spmv<FMT_DIAS, I>, grid, A.block, stream, Pa, par, ev0, ev1);
     else

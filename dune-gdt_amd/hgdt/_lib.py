@@ -189,6 +189,7 @@ SIGNATURES = {
     "hgdt_operator_adjoint_log_diffusion_gradient": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "hgdt_operator_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "hgdt_operator_get_launch": (ctypes.c_int, [_vp, _i32, _pi32, _pi32]),
+    "hgdt_operator_set_column_staging": (ctypes.c_int, [_vp, _i32]),
 }
 
 _lib = None

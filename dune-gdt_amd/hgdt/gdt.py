@@ -789,6 +789,11 @@ class MatrixOperator:
     def set_launch(self, kernel, block_size, items_per_thread=1):
         L.call("hgdt_operator_set_launch", self._h, int(kernel), int(block_size), int(items_per_thread))
 
+    def set_column_staging(self, mode):
+        """p entries of the solver's symmetric DIA application staged in LDS: -1 automatic (the default:
+        when the application moves more than 256 MiB), 0 never, 1 wherever the stencil allows it"""
+        L.call("hgdt_operator_set_column_staging", self._h, int(mode))
+
     def get_launch(self, kernel):
         b, i = ctypes.c_int32(), ctypes.c_int32()
         L.call("hgdt_operator_get_launch", self._h, int(kernel), ctypes.byref(b), ctypes.byref(i))

@@ -333,6 +333,14 @@ hgdt_status hgdt_operator_adjoint_log_diffusion_gradient(hgdt_operator op, int32
 hgdt_status hgdt_operator_set_launch(hgdt_operator op, int32_t kernel, int32_t block_size, int32_t items_per_thread);
 hgdt_status hgdt_operator_get_launch(hgdt_operator op, int32_t kernel, int32_t* block_size, int32_t* items_per_thread);
 
+/* Column staging of the solver's operator application (no reference counterpart; a launch setting like
+ * the one above).  With symmetric DIA storage, one row per lane (kernel 3 with items_per_thread 1), at most
+ * 8 strict upper diagonals and the stencil's column offsets in at most 4 windows no wider than the block,
+ * the entries of p a row block needs are loaded into LDS once per block and read from there.  mode -1
+ * (default): staged when the application moves more than 256 MiB (beyond the GPU's last-level cache);
+ * 0: never; 1: wherever it applies.  Results are the same bits either way. */
+hgdt_status hgdt_operator_set_column_staging(hgdt_operator op, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

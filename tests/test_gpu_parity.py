@@ -299,3 +299,46 @@ def test_element_blocks_without_neighbours(hgdt_mod):
     rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
     print(f"  iterations {op.last_solve['iterations']} (oracle {ito}), rel L2 {rel:.2e}")
     assert rel <= 1e-10 and abs(op.last_solve["iterations"] - ito) <= 2
+
+
+@pytest.mark.parametrize("case", ["p1_tets_three_windows", "heat_q1_union"])
+def test_symmetric_dia_solve_bits(hgdt_mod, case):
+    """the symmetric DIA solve with its application's p entries staged in LDS (forced on: these problems are
+    below the automatic threshold) and without gives the bits and iteration counts of the full DIA solve
+    (every diagonal read, the same sums in the same order) in launch shapes of one row per lane, eagerly
+    and replayed as captured chunks"""
+    H = hgdt_mod
+    L = H._lib
+    print(f"case symmetric DIA solve bits {case}", flush=True)
+    if case == "p1_tets_three_windows":
+        # 25 x 25 vertices per plane: the column offsets +-1, +-25, +-26 and +-(625 .. 651) form three windows
+        cells = [24, 24, 6]
+        kappa = torch.from_numpy(fields(H, "p1_tets", cells, 3)).cuda()
+        _, space, op = hgdt_problem(H, "p1_tets", cells, kappa)
+    else:
+        parts = [H.Grid.structured("quad", [0, 0], [1, 1 + 0.2 * i], [n, n + 2]) for i, n in enumerate([7, 30, 4])]
+        union = H.Grid.union(parts)
+        kappa = torch.from_numpy(H.fields.lognormal(29, union.num_elements)).cuda()
+        _, space, op = hgdt_problem(H, "heat_q1", None, kappa, grid=union)
+    op.assemble()
+    with pytest.raises(H._lib.HgdtError):
+        op.set_column_staging(2)
+    for block in (64, 256, 1024):
+        op.set_launch(L.KERNEL_APPLY, block, 1)
+        op.set_format(L.FORMAT_DIA_NONSYMMETRIC)
+        xd = op.apply_inverse("rhs").cpu().numpy()
+        itd = op.instance_stats()[0].copy()
+        assert op.format_info()["format"] == "dia"
+        op.set_format(L.FORMAT_AUTO)
+        for staging in (1, 0):
+            op.set_column_staging(staging)
+            x = op.apply_inverse("rhs").cpu().numpy()
+            its = op.instance_stats()[0].copy()
+            assert op.format_info()["format"] == "dia_symmetric"
+            print(f"  apply block {block}, staging {staging}: iterations {list(its)}")
+            assert np.array_equal(its, itd)
+            assert np.array_equal(x, xd)
+            for _ in range(2):
+                xg = op.apply_inverse("rhs", use_graph=True, check_every=8).cpu().numpy()
+                assert np.array_equal(xg, xd)
+        op.set_column_staging(-1)
