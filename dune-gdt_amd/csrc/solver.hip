@@ -54,9 +54,9 @@ constexpr int MAXDIAG = 32;
 enum { FMT_DIA = HGDT_FORMAT_DIA, FMT_SELL = HGDT_FORMAT_SELL, FMT_DIAS = 3, FMT_BLK = HGDT_FORMAT_BLOCK };
 
 // column windows of an instance's symmetric DIA stencil: the offsets 0 and +-o of its strict upper
-// diagonals, grouped into at most WIN_MAX ranges [lo, hi] (relative to the row) with gaps of at most
-// WIN_GAP inside a range; offsets in no range never meet a column of the instance
-constexpr int WIN_MAX = 4, WIN_GAP = 512, WIN_MAXU = 8;
+// diagonals, grouped into at most WIN_MAX ranges [lo, hi] (relative to the row) no wider than the row
+// block; offsets in no range never meet a column of the instance
+constexpr int WIN_MAX = 4, WIN_MAXU = 8;
 constexpr int64_t WIN_LDS = 64 * 1024;
 // automatic choice: staged when the application moves more than the 256 MiB last-level cache (MALL) of the
 // GPU holds; below that, the direct kernel, whose cache-line requests the MALL then serves, is faster
@@ -1089,6 +1089,9 @@ struct SolverCache {
   // windows and WIN_MAXU strict upper diagonals; the LDS of a block of R rows is at most
   // (win_nmax R + win_span) doubles, and blocks of R rows load a window with two entries per lane when its
   // span is at most R (win_wmax: the largest span)
+  std::vector<std::vector<int64_t>> dia_offs; // per instance: its DIA offsets, ascending
+  std::vector<int32_t> dia_k0_host;
+  int win_block = 0; // row block size the windows were planned for (0: not planned)
   std::vector<DiaWindows> win_host;
   DeviceArray<DiaWindows> win; // uploaded when the staged kernel is first used
   bool win_ok = false;
@@ -1338,46 +1341,68 @@ static bool build_blocks(Operator& op)
   return true;
 }
 
-// column windows of every instance's stencil for the symmetric DIA kernel with p staged in LDS: the
-// sorted offsets {0, +-o : o a strict upper offset inside the instance} split where two neighbours are
-// more than WIN_GAP apart.  Not usable (win_ok false) when an instance has more than WIN_MAXU strict upper
-// slots or more than WIN_MAX windows.
-static void plan_windows(Operator& op, const std::vector<std::vector<int64_t>>& offs, int nd,
-                         const std::vector<int32_t>& k0)
+// column windows of every instance's stencil for the symmetric DIA kernel with p staged in LDS, for row
+// blocks of R rows: the sorted offsets {0, +-o : o a strict upper offset inside the instance} split into at
+// most WIN_MAX contiguous groups of span at most R that load the fewest entries, sum of (R + span) over the
+// groups (dynamic programming over at most 2 WIN_MAXU + 1 offsets).  Not usable (win_ok false) when an
+// instance has more than WIN_MAXU strict upper slots or no such split exists.
+static void plan_windows(Operator& op, int R)
 {
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
+  const auto& offs = C.dia_offs;
+  const int nd = op.fmt.ndiag;
   const int ninst = int(offs.size());
   std::vector<DiaWindows> w(ninst);
-  bool ok = true;
+  bool ok = ninst > 0;
   int nmax = 0;
   int64_t span = 0, wmax = 0;
   for (int in = 0; in < ninst && ok; ++in) {
+    const int k0 = C.dia_k0_host[in];
     const int64_t size = s.inst_dof[in + 1] - s.inst_dof[in];
-    ok = nd - 1 - k0[in] <= WIN_MAXU;
-    std::vector<int64_t> o{0};
-    for (size_t k = size_t(k0[in]) + 1; k < offs[in].size(); ++k)
-      if (offs[in][k] < size)
-        o.push_back(offs[in][k]), o.push_back(-offs[in][k]);
-    std::sort(o.begin(), o.end());
-    std::vector<int64_t> lo, hi;
-    for (size_t j = 0; j < o.size(); ++j)
-      if (j == 0 || o[j] - o[j - 1] > WIN_GAP)
-        lo.push_back(o[j]), hi.push_back(o[j]);
-      else
-        hi.back() = o[j];
-    if (lo.size() > size_t(WIN_MAX)) {
+    if (nd - 1 - k0 > WIN_MAXU) {
       ok = false;
       break;
     }
+    std::vector<int64_t> o{0};
+    for (size_t k = size_t(k0) + 1; k < offs[in].size(); ++k)
+      if (offs[in][k] < size)
+        o.push_back(offs[in][k]), o.push_back(-offs[in][k]);
+    std::sort(o.begin(), o.end());
+    const int m = int(o.size());
+    constexpr int64_t INF = INT64_MAX / 4;
+    // best[j][k]: fewest entries for the first j offsets in k groups; from[j][k]: start of the last group
+    std::vector<std::array<int64_t, WIN_MAX + 1>> best(m + 1), from(m + 1);
+    for (auto& b : best)
+      b.fill(INF);
+    best[0][0] = 0;
+    for (int j = 1; j <= m; ++j)
+      for (int k = 1; k <= WIN_MAX; ++k)
+        for (int a0 = j - 1; a0 >= 0 && o[j - 1] - o[a0] <= R; --a0)
+          if (best[a0][k - 1] < INF && best[a0][k - 1] + R + (o[j - 1] - o[a0]) < best[j][k])
+            best[j][k] = best[a0][k - 1] + R + (o[j - 1] - o[a0]), from[j][k] = a0;
+    int kb = 0;
+    for (int k = 1; k <= WIN_MAX; ++k)
+      if (best[m][k] < INF && (kb == 0 || best[m][k] < best[m][kb]))
+        kb = k;
+    if (kb == 0) {
+      ok = false;
+      break;
+    }
+    std::vector<int64_t> lo(kb), hi(kb);
+    for (int j = m, k = kb; k > 0; --k) {
+      const int a0 = int(from[j][k]);
+      lo[k - 1] = o[a0], hi[k - 1] = o[j - 1];
+      j = a0;
+    }
     DiaWindows& d = w[in];
     d = DiaWindows{};
-    d.nwin = int32_t(lo.size());
+    d.nwin = kb;
     int64_t cs = 0;
     for (int v = 0; v < d.nwin; ++v) {
       d.lo[v] = lo[v];
-      d.span[v] = int32_t(std::min<int64_t>(hi[v] - lo[v], INT32_MAX / 2));
-      d.cspan[v] = int32_t(std::min<int64_t>(cs, INT32_MAX / 2));
+      d.span[v] = int32_t(hi[v] - lo[v]); // <= R
+      d.cspan[v] = int32_t(cs);
       cs += hi[v] - lo[v];
       wmax = std::max(wmax, hi[v] - lo[v]);
     }
@@ -1391,7 +1416,7 @@ static void plan_windows(Operator& op, const std::vector<std::vector<int64_t>>& 
     place(0, d.zero_w, d.zero_rel);
     for (int u = 0; u < WIN_MAXU; ++u) {
       d.dn_w[u] = d.up_w[u] = -1;
-      const size_t k = size_t(k0[in]) + 1 + u;
+      const size_t k = size_t(k0) + 1 + u;
       if (k < offs[in].size() && offs[in][k] < size) {
         place(-offs[in][k], d.dn_w[u], d.dn_rel[u]);
         place(offs[in][k], d.up_w[u], d.up_rel[u]);
@@ -1402,6 +1427,7 @@ static void plan_windows(Operator& op, const std::vector<std::vector<int64_t>>& 
   }
   C.win_ok = ok;
   C.win_nmax = nmax, C.win_span = span, C.win_wmax = wmax;
+  C.win_block = R;
   C.win.release();
   C.win_host = ok ? std::move(w) : std::vector<DiaWindows>{};
 }
@@ -1413,7 +1439,8 @@ static int64_t win_lds_bytes(const Operator& op)
   if (!op.solver_cache)
     return 0;
   const SolverCache& C = *op.solver_cache;
-  if (!C.win_ok || !F.symmetric || F.format != HGDT_FORMAT_DIA || F.A.items != 1 || C.win_wmax > F.A.block)
+  if (!C.win_ok || C.win_block != F.A.block || !F.symmetric || F.format != HGDT_FORMAT_DIA || F.A.items != 1 ||
+      C.win_wmax > F.A.block)
     return 0;
   if (op.column_staging == 0 || (op.column_staging < 0 && F.step_bytes <= WIN_AUTO_BYTES))
     return 0;
@@ -1427,7 +1454,8 @@ static void build_structure(Operator& op)
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
   const int ninst = int(s.inst_dof.size()) - 1;
-  C.win_ok = false;
+  C.win_ok = false, C.win_block = 0;
+  C.win.release();
   F.d_dia.release(), F.d_dia_pos.release(), F.d_sell_vals.release(), F.d_sell_cols.release();
   F.d_sell_pos.release(), F.d_scaled.release();
   F.d_mf_ref.release();
@@ -1530,7 +1558,8 @@ static void build_structure(Operator& op)
           mirror[int64_t(in) * nd + k] = int32_t(it - offs[in].begin());
       }
     C.dia_mirror.upload(mirror);
-    plan_windows(op, offs, nd, k0);
+    C.dia_offs = offs, C.dia_k0_host = k0;
+    C.win_block = 0; // windows planned for the row blocks of the solver
     F.d_dia_offsets.upload(table);
     C.dia_k0.upload(k0);
     C.row_inst.upload(row_inst);
@@ -1661,8 +1690,12 @@ static int kernel_format(const Operator& op)
 void Operator::prepare_solver(hipStream_t stream)
 {
   prepare_storage(stream);
-  // the column windows on the device once the staged application is chosen (set-up, never captured)
+  // column windows for the current row blocks, on the device once the staged application is chosen
+  // (set-up, never captured)
   SolverCache& C = cache_of(*this);
+  if (fmt.format == HGDT_FORMAT_DIA && fmt.symmetric && fmt.A.items == 1 && column_staging != 0 &&
+      C.win_block != fmt.A.block)
+    plan_windows(*this, fmt.A.block);
   if (win_lds_bytes(*this) > 0 && !C.win.ptr)
     C.win.upload(C.win_host);
 }
