@@ -152,6 +152,7 @@ def run_adjoint(hgdt, torch, args):
     prob = DiffusionProblem(union, source=1.0, rtol=1e-10)
     with torch.no_grad():
         u_obs = solve_log_diffusion(logk_obs, prob).clone()
+    tuned = choose_launch(prob.op, args)  # the batched operator's solver launch shapes
     singles = [DiffusionProblem(part, source=1.0, rtol=1e-10) for _ in range(ninst)]
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
@@ -195,7 +196,7 @@ def run_adjoint(hgdt, torch, args):
             "ms_per_gradient_batch": t_batch * 1e3, "ms_plain_sequential": t_plain * 1e3, "instances": ninst,
             "dofs": n, "forward_iterations": prob.last_forward["iterations"],
             "adjoint_iterations": prob.last_adjoint["iterations"], "batched_vs_plain_max_rel_diff": agree,
-            "setup_s": t_setup, "workload": "32_p1_kuhn_tets_64x64x64x6_logkappa_N01_J_half_l2_misfit"}
+            "setup_s": t_setup, "launch": tuned, "workload": "32_p1_kuhn_tets_64x64x64x6_logkappa_N01_J_half_l2_misfit"}
 
 
 def choose_launch(op, args):
