@@ -19,7 +19,9 @@
 // preconditioner costs no vector), three launches per iteration, each one pass over its vectors:
 //   k_cg_direction : stopping test from the previous update's partials; p = r + beta p into the other
 //                    of two p buffers; every second step x += the two updates deferred since
-//   k_cg_spmv      : q = Ah p and the block partials of p.q
+//   k_cg_spmv      : q = Ah p and the block partials of p.q (k_cg_spmv_win: the same with the p entries
+//                    of a row block staged in LDS, for operators larger than the last-level cache;
+//                    k_cg_spmv_blk: 4 x 4 element blocks)
 //   k_cg_update    : alpha from the p.q partials; r -= alpha q; block partials of r.r and |D^{1/2} r|^2
 // Data cross blocks only at kernel boundaries.  Every block of an instance reduces that
 // instance's block partials itself, in block order, with a fixed butterfly / wave order, so all
