@@ -4,8 +4,9 @@ A candidate launch shape is timed on the operator itself: for the solver, a fixe
 iterations (relative tolerance 0, so exactly `iterations` iterations run) measured with HIP events
 inside libhgdt; for assembly, MatrixOperator.assemble() bracketed by torch events.  The operator
 application (kernel 3) and the vector kernels (kernel 4) have separate row partitions and are tuned
-one after the other.  Results are cached per problem signature in memory and optionally in a JSON
-file.
+one after the other; with symmetric DIA storage the tuner then times the application with and without
+column staging (p entries of a row block read from LDS) at the chosen shapes and keeps the faster.
+Results are cached per problem signature in memory and optionally in a JSON file.
 """
 import json
 import os
@@ -59,6 +60,8 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
     if cached is not None:
         op.set_launch(L.KERNEL_APPLY, *cached["apply"])
         op.set_launch(L.KERNEL_VECTOR, *cached["vector"])
+        if "staging" in cached:
+            op.set_column_staging(cached["staging"])
         return dict(cached, cached=True)
     timings = {}
 
@@ -81,6 +84,19 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
     timings = {"apply": {f"{b}x{i}": t for (b, i), t in ra.items()},
                "vector": {f"{b}x{i}": t for (b, i), t in rv.items()}}
     choice = {"apply": list(best_a), "vector": list(best_v), "us_per_iteration": rv[best_v]}
+    if op.format_info()["format"] == "dia_symmetric":
+        # column staging off / on at the chosen shapes (on falls back to the direct kernel where the
+        # stencil does not allow it, so both then time the same kernel)
+        rs = {}
+        for mode in (0, 1):
+            op.set_column_staging(mode)
+            rs[mode] = min(_solver_us_per_iteration(op, iterations) for _ in range(repeats))
+            if verbose:
+                print(f"  column staging {mode}: {rs[mode]:8.2f} us/iteration", flush=True)
+        best_s = min(rs, key=rs.get)
+        op.set_column_staging(best_s)
+        timings["staging"] = {str(m): t for m, t in rs.items()}
+        choice["staging"], choice["us_per_iteration"] = best_s, rs[best_s]
     _memory[key] = choice
     if cache_file:
         data = {}
@@ -104,6 +120,8 @@ def apply_table(op, table_file=TABLE_FILE):
         return None
     op.set_launch(L.KERNEL_APPLY, *entry["apply"])
     op.set_launch(L.KERNEL_VECTOR, *entry["vector"])
+    if "staging" in entry:
+        op.set_column_staging(entry["staging"])
     return dict(entry, cached=True)
 
 
@@ -115,6 +133,8 @@ def save_table(op, choice, table_file):
             data = json.load(f)
     data[signature(op)] = {"apply": list(choice["apply"]), "vector": list(choice["vector"]),
                            "us_per_iteration": choice["us_per_iteration"]}
+    if "staging" in choice:
+        data[signature(op)]["staging"] = choice["staging"]
     os.makedirs(os.path.dirname(os.path.abspath(table_file)), exist_ok=True)
     with open(table_file, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
