@@ -195,7 +195,7 @@ def test_solver_failure_is_reported(hgdt_mod):
         op.apply_inverse(op.rhs)
 
 
-def test_launch_shapes_and_autotuner(hgdt_mod):
+def test_launch_shapes_and_autotuner(hgdt_mod, tmp_path):
     """every launch shape gives the oracle's solution (1e-10 rel. L2); the tuner picks one of its candidates"""
     print("case launch shapes p1_tets 8^3", flush=True)
     from hgdt import autotune
@@ -225,8 +225,15 @@ def test_launch_shapes_and_autotuner(hgdt_mod):
     choice = autotune.tune_solver(op, iterations=10, repeats=1)
     assert tuple(choice["apply"]) in autotune.APPLY_CANDIDATES
     assert tuple(choice["vector"]) in autotune.VECTOR_CANDIDATES
+    assert choice["staging"] in (0, 1)  # symmetric DIA: column staging timed off and on
     x = op.apply_inverse("rhs").cpu().numpy()
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= SOLUTION_TOL
+    # a recorded choice is applied again from a launch table, staging included
+    table = str(tmp_path / "table.json")
+    autotune.save_table(op, choice, table)
+    entry = autotune.apply_table(op, table)
+    assert entry["apply"] == list(choice["apply"]) and entry["staging"] == choice["staging"]
+    assert np.array_equal(op.apply_inverse("rhs").cpu().numpy(), x)
 
 
 def test_element_blocks_on_a_union(hgdt_mod):
