@@ -751,6 +751,7 @@ hgdt_status hgdt_solver_options_default(hgdt_solver_options* o)
     o->use_graph = 0;
     o->time_spmv = 0;
     o->zero_initial_guess = 1;
+    o->true_residual = 1;
   });
 }
 
@@ -785,6 +786,18 @@ hgdt_status hgdt_operator_instance_stats(hgdt_operator op, int64_t* iterations, 
       std::copy(op->op->last_iters.begin(), op->op->last_iters.end(), iterations);
     if (relres)
       std::copy(op->op->last_relres.begin(), op->op->last_relres.end(), relres);
+  });
+}
+
+hgdt_status hgdt_operator_instance_true_residuals(hgdt_operator op, double* relres)
+{
+  return guarded([&] {
+    require(op && relres, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    const auto& v = op->op->last_true_relres;
+    if (v.empty())
+      std::fill(relres, relres + (op->op->space->inst_dof.size() - 1), -1.);
+    else
+      std::copy(v.begin(), v.end(), relres);
   });
 }
 

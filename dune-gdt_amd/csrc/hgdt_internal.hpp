@@ -300,9 +300,17 @@ struct PcgScalars {
   double rz, rr, bb, tol2, alpha, beta, pq;
   double alpha_prev; // alpha of the step before the one in alpha (the iterate is updated every second step)
   int64_t iters;
-  int32_t done; // 0 running, 1 converged, 2 max_iter reached, 3 breakdown
-  int32_t pad1;
+  int32_t done; // 0 running, 1 converged, 2 max_iter reached, 3 breakdown (set by the direction update)
+  // breakdown met by the update kernel of this record's step.  Only the next direction update reads it
+  // (and then sets done = 3): the update kernel's own blocks test `done`, which it never writes, so no
+  // block of that launch sees a flag another block of the same launch may still be writing.
+  int32_t brk;
 };
+// state of a record as the next direction update sees it
+__host__ __device__ inline int pcg_state(const PcgScalars& s)
+{
+  return s.done ? s.done : (s.brk ? 3 : 0);
+}
 
 struct SolverCache; // solver.hip: events, pinned buffers, captured graph
 
@@ -352,6 +360,7 @@ struct Operator {
   std::shared_ptr<SolverCache> solver_cache;
   std::vector<int64_t> last_iters;
   std::vector<double> last_relres;
+  std::vector<double> last_true_relres; // |b - A x| / |b| at exit (-1: not computed)
 
   void build_pattern();
   void build_scatter_maps();

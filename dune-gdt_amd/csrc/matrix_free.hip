@@ -533,9 +533,12 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
     const PcgScalars* cur = P.v.S + P.v.par * P.ninst + inst;
     PcgScalars* nxt = P.v.S + (P.v.par ^ 1) * P.ninst + inst;
     const bool writer = (b == P.v.a_ib0[inst] && threadIdx.x == 0);
-    if (cur->done) {
-      if (writer)
+    const int done = pcg_state(*cur);
+    if (done) {
+      if (writer) {
         *nxt = *cur;
+        nxt->done = done;
+      }
       return;
     }
     const int64_t it = cur->iters;
@@ -558,6 +561,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       nxt->beta = beta;
       nxt->iters = status ? it : it + 1;
       nxt->done = status;
+      nxt->brk = 0;
     }
     if (status != 0) {
       // the instance stops here: only the iterate's pending updates remain (uniform over the block)

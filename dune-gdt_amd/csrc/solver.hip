@@ -634,10 +634,12 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
   const PcgScalars* cur = P.S + par * P.ninst + inst;
   PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
   const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
-  const int done = cur->done;
+  const int done = pcg_state(*cur);
   if (done) {
-    if (writer)
+    if (writer) {
       *nxt = *cur;
+      nxt->done = done;
+    }
     return;
   }
   const int64_t it = cur->iters;
@@ -704,6 +706,7 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
     nxt->beta = beta;
     nxt->iters = status ? it : it + 1;
     nxt->done = status;
+    nxt->brk = 0;
   }
 }
 
@@ -862,7 +865,10 @@ __global__ void __launch_bounds__(256) k_cg_spmv_blk(PcgParams P, int par)
 
 // CG update of step j on the vector row partition: alpha = rho_j / p.q, rh -= alpha q, block partials
 // of rh.rh and |D^{1/2} rh|^2.  Works on record par ^ 1 (written by the step kernel): stores alpha
-// there for the iterate's deferred updates, or the breakdown state.
+// there for the iterate's deferred updates, or the breakdown flag `brk`.  Its blocks test only `done`,
+// which this kernel never writes, so a block that starts after the writer block has flagged a breakdown
+// still applies the pending half update of the iterate (the next direction update turns `brk` into
+// done = 3).
 template <int ITEMS>
 __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
 {
@@ -895,7 +901,7 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
       for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
         P.x[i] = fma(S->alpha, P.p2[i], P.x[i]);
     if (writer)
-      S->done = 3;
+      S->brk = 1;
     return;
   }
   const double alpha = S->rz / pq[0];
@@ -923,6 +929,41 @@ __global__ void k_cg_finish(int64_t n, const double* __restrict__ dh, double* __
   const int64_t i = xcd_index(n);
   if (i < n)
     x[i] = dh[i] * x[i];
+}
+
+// true residual at exit, on the vector row partition: block partials of |b - A x|^2 with ax = A x computed
+// by the (unscaled) operator application
+template <int ITEMS>
+__global__ void __launch_bounds__(1024) k_true_residual(PcgParams P)
+{
+  const int b = logical_block(P.v_nb);
+  if (b < 0)
+    return;
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
+  double v[1] = {0.};
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+    if (i < r1) {
+      const double ri = P.b[i] - P.ax[i];
+      v[0] += ri * ri;
+    }
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0)
+    P.part3[b] = v[0];
+}
+
+// one block per instance: |b - A x| / |b| (|b - A x| when b = 0), with b.b from the instance's record
+__global__ void __launch_bounds__(256) k_true_reduce(PcgParams P, double* __restrict__ out)
+{
+  const int inst = blockIdx.x;
+  double t[1];
+  reduce_partials<1>(P.part3, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  if (threadIdx.x == 0) {
+    const double bb = P.S[inst].bb;
+    out[inst] = bb > 0. ? sqrt(t[0] / bb) : sqrt(t[0]);
+  }
 }
 
 // scaled DIA storage, one diagonal after the other (coalesced): Ah(i, i+o) = a(i, i+o) (h_i h_{i+o});
@@ -1104,6 +1145,10 @@ struct SolverCache {
   int* host_flags = nullptr;
   PcgScalars* host_scalars[2] = {nullptr, nullptr};
   int host_capacity = 0;
+  // true relative residual per instance at the solver's exit (device and pinned host copy)
+  DeviceArray<double> true_relres;
+  double* host_true = nullptr;
+  int host_true_capacity = 0;
   hipEvent_t copied[2] = {nullptr, nullptr};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   std::vector<hipEvent_t> timing_events;
@@ -1121,6 +1166,8 @@ struct SolverCache {
     for (auto& h : host_scalars)
       if (h)
         (void)hipHostFree(h);
+    if (host_true)
+      (void)hipHostFree(host_true);
     if (host_flags)
       (void)hipHostFree(host_flags);
     for (auto& e : copied)
@@ -2015,7 +2062,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   };
   auto all_done = [&](const PcgScalars* h) {
     for (int i = 0; i < ninst; ++i)
-      if (h[i].done == 0)
+      if (pcg_state(h[i]) == 0)
         return false;
     return true;
   };
@@ -2031,7 +2078,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     const RowPartition& V = fmt.V;
     int64_t na = 0, nv = 0;
     for (int i = 0; i < ninst; ++i)
-      if (h[i].done == 0) {
+      if (pcg_state(h[i]) == 0) {
         na += A.inst_block0[i + 1] - A.inst_block0[i];
         nv += V.inst_block0[i + 1] - V.inst_block0[i];
       }
@@ -2041,11 +2088,11 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     std::vector<int32_t>& L = C.active_host[slot];
     L.clear();
     for (int i = 0; i < ninst; ++i)
-      if (h[i].done == 0)
+      if (pcg_state(h[i]) == 0)
         for (int32_t b = A.inst_block0[i]; b < A.inst_block0[i + 1]; ++b)
           L.push_back(b);
     for (int i = 0; i < ninst; ++i)
-      if (h[i].done == 0)
+      if (pcg_state(h[i]) == 0)
         for (int32_t b = V.inst_block0[i]; b < V.inst_block0[i + 1]; ++b)
           L.push_back(b);
     DeviceArray<int32_t>& D = C.active[slot];
@@ -2090,6 +2137,28 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   }
   hipLaunchKernelGGL(k_cg_finish, dim3(blocks_for(n, 256)), dim3(256), 0, stream, n, fmt.d_dh.ptr, x);
   HGDT_HIP(hipGetLastError());
+  // true residual |b - A x| / |b| of every instance at exit, with the unscaled operator (one application and
+  // one pass over b and A x, over all blocks whatever the narrowed launches held)
+  if (opt.true_residual) {
+    if (C.host_true_capacity < ninst) {
+      if (C.host_true)
+        HGDT_HIP(hipHostFree(C.host_true));
+      C.host_true = nullptr;
+      HGDT_HIP(hipHostMalloc((void**)&C.host_true, sizeof(double) * ninst));
+      C.host_true_capacity = ninst;
+    }
+    C.true_relres.allocate(size_t(ninst));
+    apply(x, d_q.ptr, stream);
+    PcgParams T = make_params(*this);
+    T.b = b, T.ax = d_q.ptr, T.part3 = P.part3, T.S = P.S;
+    dispatch_items(fmt.V.items, [&](auto IT) {
+      constexpr int I = decltype(IT)::value;
+      hipLaunchKernelGGL((k_true_residual<I>), dim3(hw_blocks(fmt.V.nblocks)), dim3(fmt.V.block), 0, stream, T);
+    });
+    hipLaunchKernelGGL(k_true_reduce, dim3(unsigned(ninst)), dim3(256), 0, stream, T, C.true_relres.ptr);
+    HGDT_HIP(hipGetLastError());
+    HGDT_HIP(hipMemcpyAsync(C.host_true, C.true_relres.ptr, sizeof(double) * ninst, hipMemcpyDeviceToHost, stream));
+  }
   HGDT_HIP(hipEventRecord(C.t1, stream));
   HGDT_HIP(hipEventSynchronize(C.t1));
   if (graph) {
@@ -2098,16 +2167,23 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   }
   const PcgScalars* h = C.host_scalars[cur];
   last_iters.assign(ninst, 0), last_relres.assign(ninst, 0.);
+  last_true_relres.assign(ninst, opt.true_residual ? 0. : -1.);
   st = hgdt_solver_stats{};
   st.converged = 1;
+  st.true_relres = opt.true_residual ? 0. : -1.;
   for (int i = 0; i < ninst; ++i) {
     last_iters[i] = h[i].iters;
     last_relres[i] = h[i].bb > 0. ? std::sqrt(h[i].rr / h[i].bb) : 0.;
+    if (opt.true_residual) {
+      last_true_relres[i] = C.host_true[i];
+      if (!(C.host_true[i] <= st.true_relres)) // a NaN propagates
+        st.true_relres = C.host_true[i];
+    }
     st.iterations = std::max<int64_t>(st.iterations, h[i].iters);
     st.relres = std::max(st.relres, last_relres[i]);
-    if (h[i].done != 1)
+    if (pcg_state(h[i]) != 1)
       st.converged = 0;
-    if (h[i].done == 3)
+    if (pcg_state(h[i]) == 3)
       st.breakdown = 1;
   }
   float ms = 0.f;
@@ -2129,6 +2205,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   }
   st.step_bytes = fmt.step_bytes;
   st.compactions = compactions;
+  st.column_staged = (kernel_format(*this) == FMT_DIAS && win_lds_bytes(*this) > 0 && C.win.ptr) ? 1 : 0;
   if (!st.converged) {
     if (st.breakdown)
       HGDT_THROW(HGDT_ERR_SOLVER_FAILED, "Jacobi-PCG breakdown: p.Ap <= 0 (matrix not SPD)");

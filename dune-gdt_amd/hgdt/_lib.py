@@ -104,6 +104,7 @@ class SolverOptions(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("time_spmv", ctypes.c_int32),
         ("zero_initial_guess", ctypes.c_int32),
+        ("true_residual", ctypes.c_int32),
     ]
 
 
@@ -118,6 +119,8 @@ class SolverStats(ctypes.Structure):
         ("solve_ms", ctypes.c_double),
         ("step_bytes", ctypes.c_int64),
         ("compactions", ctypes.c_int64),
+        ("true_relres", ctypes.c_double),
+        ("column_staged", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -186,6 +189,7 @@ SIGNATURES = {
         [_vp, _vp, _vp, ctypes.POINTER(SolverOptions), ctypes.POINTER(SolverStats), _vp],
     ),
     "hgdt_operator_instance_stats": (ctypes.c_int, [_vp, _pi64, _pdbl]),
+    "hgdt_operator_instance_true_residuals": (ctypes.c_int, [_vp, _pdbl]),
     "hgdt_operator_adjoint_log_diffusion_gradient": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "hgdt_operator_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "hgdt_operator_get_launch": (ctypes.c_int, [_vp, _i32, _pi32, _pi32]),

@@ -6,6 +6,11 @@ dropped because the constrained rows and columns do not depend on kappa), then t
 element kernel grad_e = -kappa_e lambda_e^T Khat_e u_e, recomputing the unit-coefficient local
 matrix element-locally instead of storing local matrices.  No reference counterpart: dune-gdt has
 no adjoint path; the result is checked against central finite differences and the CPU oracle.
+
+A DiffusionProblem holds one assembled operator.  Every forward solve gets a new id and keeps its
+kappa in the autograd context; a backward whose forward is no longer the one assembled (another solve
+on the same problem ran in between) copies its own kappa back and re-assembles before the adjoint
+solve, so each graph's gradient is taken at its own kappa.
 """
 import numpy as np
 import torch
@@ -34,10 +39,22 @@ class DiffusionProblem:
         self.free = torch.from_numpy(~mask).to("cuda").to(torch.float64)
         self.last_forward = None
         self.last_adjoint = None
+        self._solves = 0      # ids handed to forward solves
+        self.assembled = None  # id of the solve whose kappa the operator holds
+        self.reassemblies = 0  # backward passes that had to restore their kappa
 
-    def solve(self, kappa):
+    def assemble_for(self, kappa, solve_id=None):
+        """Assemble A(kappa); returns the id under which the operator now holds it."""
         self.kappa.copy_(kappa)
         self.op.assemble()
+        if solve_id is None:
+            self._solves += 1
+            solve_id = self._solves
+        self.assembled = solve_id
+        return solve_id
+
+    def solve(self, kappa):
+        self.assemble_for(kappa)
         u = self.op.apply_inverse(self.op.rhs, opts={"precision": self.rtol})
         self.last_forward = dict(self.op.last_solve)
         return u
@@ -46,16 +63,21 @@ class DiffusionProblem:
 class LogDiffusionToSolution(torch.autograd.Function):
     @staticmethod
     def forward(ctx, log_kappa, problem):
-        kappa = torch.exp(log_kappa.detach())
+        kappa = torch.exp(log_kappa.detach()).to(torch.float64)
         u = problem.solve(kappa)
         ctx.problem = problem
-        ctx.save_for_backward(u)
+        ctx.solve_id = problem.assembled
+        ctx.save_for_backward(u, kappa)
         return u
 
     @staticmethod
     def backward(ctx, grad_u):
-        (u,) = ctx.saved_tensors
+        u, kappa = ctx.saved_tensors
         problem = ctx.problem
+        if problem.assembled != ctx.solve_id:
+            # another forward solve re-assembled the operator (and overwrote the form's kappa) since this one
+            problem.assemble_for(kappa, ctx.solve_id)
+            problem.reassemblies += 1
         g = (grad_u.detach().to(torch.float64) * problem.free).contiguous()
         lam = problem.op.apply_inverse(g, opts={"precision": problem.rtol})
         problem.last_adjoint = dict(problem.op.last_solve)

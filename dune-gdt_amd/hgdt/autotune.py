@@ -49,6 +49,13 @@ def _solver_us_per_iteration(op, iterations):
     return (t2 - t1) * 1e3 / (i2 - i1)
 
 
+def _staging_applies(op):
+    """whether the column-staged application runs for this operator at its current launch shapes"""
+    op.set_column_staging(1)
+    _solve_ms(op, 2)
+    return bool(op.last_solve["column_staged"])
+
+
 def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
     """Pick (block, items) for operator application and vector kernels; returns the choice and timings."""
     op.apply_inverse("rhs")  # prepares the storage format, warms the kernels
@@ -60,8 +67,7 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
     if cached is not None:
         op.set_launch(L.KERNEL_APPLY, *cached["apply"])
         op.set_launch(L.KERNEL_VECTOR, *cached["vector"])
-        if "staging" in cached:
-            op.set_column_staging(cached["staging"])
+        op.set_column_staging(cached.get("staging", -1))
         return dict(cached, cached=True)
     timings = {}
 
@@ -84,9 +90,10 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
     timings = {"apply": {f"{b}x{i}": t for (b, i), t in ra.items()},
                "vector": {f"{b}x{i}": t for (b, i), t in rv.items()}}
     choice = {"apply": list(best_a), "vector": list(best_v), "us_per_iteration": rv[best_v]}
-    if op.format_info()["format"] == "dia_symmetric":
-        # column staging off / on at the chosen shapes (on falls back to the direct kernel where the
-        # stencil does not allow it, so both then time the same kernel)
+    if op.format_info()["format"] == "dia_symmetric" and best_a[1] == 1 and _staging_applies(op):
+        # column staging off / on at the chosen shapes (only where the staged kernel applies: one row per
+        # lane and a stencil whose windows fit; elsewhere both settings would time the same kernel and the
+        # automatic choice, -1, is kept)
         rs = {}
         for mode in (0, 1):
             op.set_column_staging(mode)
@@ -97,6 +104,9 @@ def tune_solver(op, iterations=40, repeats=2, cache_file=None, verbose=False):
         op.set_column_staging(best_s)
         timings["staging"] = {str(m): t for m, t in rs.items()}
         choice["staging"], choice["us_per_iteration"] = best_s, rs[best_s]
+    else:
+        op.set_column_staging(-1)
+        choice["staging"] = -1
     _memory[key] = choice
     if cache_file:
         data = {}
@@ -120,8 +130,7 @@ def apply_table(op, table_file=TABLE_FILE):
         return None
     op.set_launch(L.KERNEL_APPLY, *entry["apply"])
     op.set_launch(L.KERNEL_VECTOR, *entry["vector"])
-    if "staging" in entry:
-        op.set_column_staging(entry["staging"])
+    op.set_column_staging(entry.get("staging", -1))
     return dict(entry, cached=True)
 
 

@@ -289,6 +289,8 @@ typedef struct hgdt_solver_options {
   int32_t time_spmv;      /* k > 0: record HIP events around the operator application of every k-th
                              iteration */
   int32_t zero_initial_guess; /* 1: start from x = 0, else from the given x */
+  int32_t true_residual;  /* 1 (default): compute |b - A x|_2 / |b|_2 of every instance at exit with the
+                             unscaled operator (one more application; reported in true_relres) */
 } hgdt_solver_options;
 
 typedef struct hgdt_solver_stats {
@@ -303,6 +305,10 @@ typedef struct hgdt_solver_stats {
                              by time_spmv) */
   int64_t compactions;    /* batches: times the launches were narrowed to the blocks of the instances
                              still iterating */
+  double true_relres;     /* max over instances of |b - A x| / |b| at exit, recomputed from the returned x
+                             (-1 when options.true_residual is 0) */
+  int32_t column_staged;  /* 1: the solver's operator application ran with column staging (see
+                             hgdt_operator_set_column_staging) */
 } hgdt_solver_stats;
 
 hgdt_status hgdt_solver_options_default(hgdt_solver_options* options);
@@ -315,6 +321,10 @@ hgdt_status hgdt_operator_apply_inverse(hgdt_operator op, const double* b, doubl
                                         void* stream);
 /* per-instance iteration counts and relative residuals of the last apply_inverse, host arrays */
 hgdt_status hgdt_operator_instance_stats(hgdt_operator op, int64_t* iterations, double* relres);
+/* per-instance true relative residuals |b - A x| / |b| of the last apply_inverse (host array [num_instances];
+ * -1 when it was not computed).  The reference's apply_inverse checks the solver's result the same way
+ * (operators/matrix-based.hh:148-159 with XT::LA::Solver's post-check of the residual). */
+hgdt_status hgdt_operator_instance_true_residuals(hgdt_operator op, double* relres);
 
 /* Adjoint element kernel (no reference counterpart): for the Laplace form appended at position
  * `form_index` among the element forms, with per-element diffusion kappa_e,

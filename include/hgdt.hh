@@ -812,6 +812,7 @@ struct SolverStats
   int64_t iterations = 0;
   double relres = 0.;
   double solve_ms = 0.;
+  double true_relres = -1.; // |b - A x| / |b| recomputed at exit
 };
 
 // assembled CSR matrix of an operator (values in device memory)
@@ -1017,7 +1018,7 @@ public:
     o.use_graph = opts.use_graph ? 1 : 0;
     hgdt_solver_stats st{};
     const hgdt_status s = hgdt_operator_apply_inverse(handle(), range_device, source.data(), &o, &st, stream);
-    last_ = SolverStats{st.converged != 0, st.iterations, st.relres, st.solve_ms};
+    last_ = SolverStats{st.converged != 0, st.iterations, st.relres, st.solve_ms, st.true_relres};
     check(s);
     return last_;
   }

@@ -147,3 +147,36 @@ def test_adjoint_gradient_matches_oracle_and_fd(hgdt_mod):
         fd = (vals[0] - vals[1]) / (2 * h)
         print(f"  element {e}: adjoint {g[e]:.6e} fd {fd:.6e}")
         assert abs(fd - g[e]) <= 1e-5 * gmax
+
+
+def test_adjoint_backward_after_a_second_forward(hgdt_mod):
+    """two forward solves on one DiffusionProblem (kappa_1, then kappa_2) and the backward of the first:
+    its gradient must be the oracle's at kappa_1, not at the kappa the operator was last assembled with;
+    the second graph's gradient must then be the oracle's at kappa_2"""
+    print("case adjoint: forward(k1), forward(k2), backward(first), backward(second)", flush=True)
+    from hgdt.autograd import DiffusionProblem, solve_log_diffusion
+
+    F = hgdt_mod.fields
+    n = 4
+    grid = hgdt_mod.Grid.structured("tet", [0, 0, 0], [1, 1, 1], [n, n, n])
+    ne = grid.num_elements
+    lk1, lk2, lk_obs = F.normal(61, ne), F.normal(62, ne), F.normal(63, ne)
+    g1, u_obs, _ = _oracle_gradient(n, lk1, lk_obs)
+    g2, _, _ = _oracle_gradient(n, lk2, lk_obs)
+    prob = DiffusionProblem(grid, source=1.0, rtol=1e-12)
+    u_obs = torch.from_numpy(u_obs).cuda()
+    t1 = torch.tensor(lk1, device="cuda", requires_grad=True)
+    t2 = torch.tensor(lk2, device="cuda", requires_grad=True)
+    J1 = 0.5 * torch.sum((solve_log_diffusion(t1, prob) - u_obs) ** 2)
+    J2 = 0.5 * torch.sum((solve_log_diffusion(t2, prob) - u_obs) ** 2)
+    J1.backward()
+    assert prob.reassemblies == 1
+    J2.backward()
+    assert prob.reassemblies == 2
+    for t, go, label in ((t1, g1, "first"), (t2, g2, "second")):
+        g = t.grad.detach().cpu().numpy()
+        rel = np.abs(g - go).max() / np.abs(go).max()
+        print(f"  {label} graph: gradient vs oracle max rel {rel:.2e}")
+        assert rel <= 1e-8, label
+    # the two gradients differ, so the check above can tell them apart
+    assert np.abs(g1 - g2).max() > 1e-3 * np.abs(g1).max()

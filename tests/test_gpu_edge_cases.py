@@ -274,3 +274,59 @@ def test_breakdown_leaves_the_iterate_of_the_last_full_step(hgdt_mod):
         print(f"    iterate vs oracle: rel L2 {rel:.2e}")
         assert (rel <= 1e-10) if np.linalg.norm(xo) > 0 else np.all(xi == 0.)
     assert {0, 1, 2, 3} <= seen
+
+
+def test_breakdown_with_more_blocks_than_resident(hgdt_mod):
+    """the breakdown flag of the update kernel at full size: four indefinite instances (Laplace minus c
+    times the mass matrix on 600 x 600 cells, 1.44 M DoFs) with vector blocks of 64 rows, about 22 600
+    blocks, far more than the GPU holds at once, so most blocks of an instance start after the block that
+    flags the breakdown has finished.  Every instance must stop at its first non-positive curvature with the
+    oracle's iterate after j steps, including the pending half update at odd j"""
+    import scipy.sparse as sp
+    H = hgdt_mod
+    n, cs = 600, [2910.4, 1862.6, 953.7, 762.9]  # breakdown at steps 0, 1, 2, 3 (found with the oracle below)
+    print(f"case breakdown at full size: 4 x Q1 {n}^2, vector blocks 64 x 1", flush=True)
+    parts = [H.Grid.structured("quad", [0, 0], [1, 1], [n, n]) for _ in cs]
+    union = H.Grid.union(parts)
+    w = np.repeat(np.array(cs), n * n)
+    space = H.ContinuousLagrangeSpace(union, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(torch.from_numpy(w).cuda())),
+              {"matrixoperator.scaling": -1.0})
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.set_launch(H._lib.KERNEL_VECTOR, 64, 1)
+    op.assemble()
+    x = torch.zeros(space.num_DoFs, dtype=torch.float64, device="cuda")
+    with pytest.raises(H.linear_solver_failed):
+        op.apply_inverse("rhs", x, check_every=2)
+    assert op.last_solve["breakdown"] == 1 and op.last_solve["converged"] == 0
+    x = x.cpu().numpy()
+    its, _ = op.instance_stats()
+    doff = space.instance_dofs()
+    m = O.Mesh.structured(O.QUAD, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, True, 1)
+    seen = set()
+    for i, c in enumerate(cs):
+        oop = O.Operator(s, O.STENCIL_ELEMENT)
+        oop.append_laplace(1.0)
+        oop.append_product(c, scaling=-1.0)
+        oop.append_source(1.0)
+        oop.set_dirichlet(True)
+        oop.assemble(4)
+        rp, cols = oop.pattern()
+        A = sp.csr_matrix((oop.values(), cols, rp), shape=(len(rp) - 1, len(rp) - 1))
+        j, margin = _first_nonpositive_curvature(A, oop.rhs(), max_iter=16)
+        assert j is not None and margin > 1e-3, (c, j, margin)
+        seen.add(j)
+        xi = x[doff[i]:doff[i + 1]]
+        xo = np.zeros_like(xi) if j == 0 else O.pcg_jacobi(rp, cols, oop.values(), oop.rhs(), rtol=1e-10,
+                                                             max_iters_run=j)[0]
+        rel = np.linalg.norm(xi - xo) / max(np.linalg.norm(xo), 1e-300)
+        print(f"  c={c}: breakdown at step {j} (steps begun {its[i]}), iterate vs oracle rel L2 {rel:.2e}")
+        assert its[i] == j + 1
+        assert (rel <= 1e-10) if j > 0 else np.all(xi == 0.)
+    assert {1, 3} <= seen

@@ -84,10 +84,16 @@ def test_solve_parity(hgdt_mod, kind, cells, seed):
         assert st["converged"] == 1 and st["relres"] <= 1e-10
         assert abs(st["iterations"] - ito) <= 2
         assert rel <= SOLUTION_TOL
-        # true residual of the GPU solution with the oracle's matrix
+        # true residual of the GPU solution with the oracle's matrix, and the one the solver reports at exit
         A = oop.csr()
         b = oop.rhs()
-        assert np.linalg.norm(b - A @ xg) / np.linalg.norm(b) <= 2e-10
+        true_host = np.linalg.norm(b - A @ xg) / np.linalg.norm(b)
+        print(f"  true relres: device {st['true_relres']:.3e}, host {true_host:.3e}")
+        assert true_host <= 2e-10
+        assert abs(st["true_relres"] - true_host) <= 0.05 * true_host + 1e-14
+        tr = np.empty(1)
+        hgdt_mod._lib.call("hgdt_operator_instance_true_residuals", op._h, tr.ctypes.data_as(hgdt_mod._lib._pdbl))
+        assert tr[0] == st["true_relres"]
 
 
 @pytest.mark.parametrize("kind,cells,seed", [CASES[1], CASES[3], CASES[5]])
@@ -225,7 +231,17 @@ def test_launch_shapes_and_autotuner(hgdt_mod, tmp_path):
     choice = autotune.tune_solver(op, iterations=10, repeats=1)
     assert tuple(choice["apply"]) in autotune.APPLY_CANDIDATES
     assert tuple(choice["vector"]) in autotune.VECTOR_CANDIDATES
-    assert choice["staging"] in (0, 1)  # symmetric DIA: column staging timed off and on
+    # symmetric DIA: column staging is timed off and on where the staged kernel runs at the chosen shapes (one
+    # row per lane), else the automatic setting (-1) is recorded
+    staging = choice["staging"]
+    op.set_column_staging(1)
+    op.apply_inverse("rhs")
+    applies = bool(op.last_solve["column_staged"])
+    print(f"  tuned apply {choice['apply']} vector {choice['vector']} staging {staging} (staged kernel "
+          f"{'applies' if applies else 'does not apply'})")
+    assert applies == (choice["apply"][1] == 1)
+    assert (staging in (0, 1)) if applies else (staging == -1)
+    op.set_column_staging(staging)
     x = op.apply_inverse("rhs").cpu().numpy()
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= SOLUTION_TOL
     # a recorded choice is applied again from a launch table, staging included
@@ -342,7 +358,10 @@ def test_symmetric_dia_solve_bits(hgdt_mod, case):
             x = op.apply_inverse("rhs").cpu().numpy()
             its = op.instance_stats()[0].copy()
             assert op.format_info()["format"] == "dia_symmetric"
-            print(f"  apply block {block}, staging {staging}: iterations {list(its)}")
+            print(f"  apply block {block}, staging {staging}: iterations {list(its)}, "
+                  f"staged {op.last_solve['column_staged']}")
+            # the staged kernel really ran (staging 1) or did not (staging 0)
+            assert op.last_solve["column_staged"] == staging
             assert np.array_equal(its, itd)
             assert np.array_equal(x, xd)
             for _ in range(2):
