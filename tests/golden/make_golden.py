@@ -148,6 +148,94 @@ def p2_triangle_matrices():
     return K, M, n
 
 
+# P2 on Kuhn tetrahedra: grid of TNX x TNY x TNZ cells of widths TH (anisotropic), exact rationals
+TCELLS, TH = (2, 1, 1), (F(1, 2), F(1, 2), F(3, 4))
+
+
+def p2_tet_matrices():
+    """P2 Laplace and mass matrices on the Kuhn tetrahedra of TCELLS cells (every cube split into the six
+    tetrahedra (v0, v0 + e_p0, v0 + e_p0 + e_p1, v7) for the axis permutations p), by exact integration of
+    the monomials over the reference tetrahedron (int x^a y^b z^c = a! b! c! / (a + b + c + 3)!).  The
+    basis is lambda_a (2 lambda_a - 1) at the vertices and 4 lambda_a lambda_b at the edge midpoints; DoFs
+    are identified by their node's coordinates on the half-cell lattice."""
+    from itertools import permutations
+    from math import factorial
+
+    def mul(p, q):
+        r = {}
+        for a, c in p.items():
+            for b, d in q.items():
+                k = (a[0] + b[0], a[1] + b[1], a[2] + b[2])
+                r[k] = r.get(k, F(0)) + c * d
+        return r
+
+    def der(p, axis):
+        r = {}
+        for a, c in p.items():
+            if a[axis] > 0:
+                k = list(a)
+                k[axis] -= 1
+                r[tuple(k)] = r.get(tuple(k), F(0)) + a[axis] * c
+        return r
+
+    def integrate(p):
+        return sum(c * F(factorial(a[0]) * factorial(a[1]) * factorial(a[2]), factorial(sum(a) + 3))
+                   for a, c in p.items())
+
+    lam = [{(0, 0, 0): F(1), (1, 0, 0): F(-1), (0, 1, 0): F(-1), (0, 0, 1): F(-1)},
+           {(1, 0, 0): F(1)}, {(0, 1, 0): F(1)}, {(0, 0, 1): F(1)}]
+    basis = [mul(lam[a], {k: 2 * v for k, v in lam[a].items()} | {(0, 0, 0): 2 * lam[a].get((0, 0, 0), F(0)) - 1})
+             for a in range(4)]
+    edges = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    basis += [{k: 4 * v for k, v in mul(lam[a], lam[b]).items()} for a, b in edges]
+    ref_vert = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    grads = [[der(p, ax) for ax in range(3)] for p in basis]
+    # integrals of products of reference derivatives / values (the same for every tetrahedron)
+    DD = [[[[integrate(mul(grads[i][a], grads[j][b])) for b in range(3)] for a in range(3)] for j in range(10)]
+          for i in range(10)]
+    VV = [[integrate(mul(basis[i], basis[j])) for j in range(10)] for i in range(10)]
+    K, Mm = {}, {}
+    for cz in range(TCELLS[2]):
+        for cy in range(TCELLS[1]):
+            for cx in range(TCELLS[0]):
+                v0 = (cx, cy, cz)
+                for perm in permutations(range(3)):
+                    pts = [v0]
+                    cur = list(v0)
+                    for ax in perm:
+                        cur[ax] += 1
+                        pts.append(tuple(cur))
+                    # columns of J: physical edge vectors from vertex 0 (lattice units times TH)
+                    J = [[(pts[c + 1][r] - pts[0][r]) * TH[r] for c in range(3)] for r in range(3)]
+                    det = (J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                           + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]))
+                    cof = [[(J[(r + 1) % 3][(c + 1) % 3] * J[(r + 2) % 3][(c + 2) % 3]
+                             - J[(r + 1) % 3][(c + 2) % 3] * J[(r + 2) % 3][(c + 1) % 3]) for c in range(3)] for r in range(3)]
+                    Ji = [[cof[c][r] / det for c in range(3)] for r in range(3)]  # J^{-1}
+                    G = [[sum(Ji[a][k] * Ji[b][k] for k in range(3)) for b in range(3)] for a in range(3)]
+                    # node keys: doubled lattice coordinates of vertices and edge midpoints
+                    lat = [tuple(2 * p[r] for r in range(3)) for p in pts]
+                    nodes = lat + [tuple((lat[a][r] + lat[b][r]) // 2 for r in range(3)) for a, b in edges]
+                    for i in range(10):
+                        for j in range(10):
+                            kv = abs(det) * sum(G[a][b] * DD[i][j][a][b] for a in range(3) for b in range(3))
+                            key = (nodes[i], nodes[j])
+                            K[key] = K.get(key, F(0)) + kv
+                            Mm[key] = Mm.get(key, F(0)) + abs(det) * VV[i][j]
+    keys = sorted({k[0] for k in K})
+    return K, Mm, keys
+
+
+def dense_eigenvalues(A, keys):
+    import numpy as np
+
+    idx = {k: i for i, k in enumerate(keys)}
+    D = np.zeros((len(keys), len(keys)))
+    for (a, b), v in A.items():
+        D[idx[a], idx[b]] = float(v)
+    return [float(x) for x in np.linalg.eigvalsh(D)]
+
+
 def main():
     out = {"reference": {
         # dune/gdt/test/integrands/integrands_laplace.cc:131-133 (tol 1e-13, 1e-13, 5e-12)
@@ -162,6 +250,8 @@ def main():
                             "tol": [1e-13, 1e-13, 1e-13]},
         # dune/gdt/test/stationary-heat-equation/stationary_heat_equation__ESV2007__table_1.mini:31-33
         "esv2007_q1dg_cube": {"h": [3.54e-01, 1.77e-01, 8.84e-02], "h1_semi": [2.52e-01, 1.26e-01, 6.30e-02]},
+        # same file, :11-13: P1-DG on the ALU conforming simplex grid (4 x 4 cubes, Kuhn split, bisection-refined)
+        "esv2007_p1dg_simplex": {"h": [3.54e-01, 1.77e-01, 8.84e-02], "h1_semi": [3.28e-01, 1.62e-01, 8.04e-02]},
     }, "exact": {}}
     K, M, n = q2_matrices()
     for name, A in (("p2_laplace_cube", K), ("p2_mass_cube", M)):
@@ -173,10 +263,17 @@ def main():
         mn, mx, ss = stats(A, n)
         out["exact"][name] = {"min": [mn.numerator, mn.denominator], "max": [mx.numerator, mx.denominator],
                               "sumsq": [ss.numerator, ss.denominator]}
+    K, M, keys = p2_tet_matrices()
+    for name, A in (("p2_laplace_tet_kuhn", K), ("p2_mass_tet_kuhn", M)):
+        mn, mx, ss = stats(A, len(keys))
+        out["exact"][name] = {"min": [mn.numerator, mn.denominator], "max": [mx.numerator, mx.denominator],
+                              "sumsq": [ss.numerator, ss.denominator], "size": len(keys),
+                              "cells": list(TCELLS), "widths": [[w.numerator, w.denominator] for w in TH],
+                              "eigenvalues": dense_eigenvalues(A, keys)}
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump(out, f, indent=1)
     for k, v in out["exact"].items():
-        print(k, {kk: float(F(*vv)) for kk, vv in v.items()})
+        print(k, {kk: float(F(*vv)) for kk, vv in v.items() if kk in ("min", "max", "sumsq")})
 
 
 if __name__ == "__main__":

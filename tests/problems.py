@@ -75,6 +75,27 @@ def hgdt_problem(hgdt, kind, cells, kappa, f=1.0, sigma=16.0, lo=None, hi=None, 
     return grid, space, op
 
 
+def bisection_refined_triangles(n, lo=-1.0, hi=1.0):
+    """The simplex grid of the reference's ESV2007 test (ESV2007.hh:66-69): a 4 x 4 cube grid split into
+    Kuhn triangles, then refined by newest-vertex bisection (ALU conforming, two bisections per halving of
+    h).  Each bisection pair splits a triangle's hypotenuse and then its legs, so every 2 x 2 block of
+    cells of width (hi - lo) / n ends up with its diagonals meeting at the block's centre ("union jack");
+    n = 8, 16, 32 are the grids after 2, 4 and 6 bisections.  Returns (coords [N, 2], connectivity [E, 3])."""
+    assert n % 2 == 0
+    xs = np.linspace(lo, hi, n + 1)
+    X = np.array([[x, y] for y in xs for x in xs])
+    conn = []
+    for j in range(n):
+        for i in range(n):
+            v00, v10 = j * (n + 1) + i, j * (n + 1) + i + 1
+            v01, v11 = v00 + n + 1, v10 + n + 1
+            if i % 2 == j % 2:  # the block centre is this cell's corner v11 or v00
+                conn += [[v00, v10, v11], [v00, v01, v11]]
+            else:  # the block centre is at v10 or v01
+                conn += [[v10, v11, v01], [v10, v00, v01]]
+    return X, np.array(conn, np.int32)
+
+
 def matrix_error(vals, ref_vals, row_ptr):
     """max over rows of max_k |a_k - r_k| / max_k |r_k| (entries compared relative to their row's scale)"""
     err = np.abs(vals - ref_vals)
