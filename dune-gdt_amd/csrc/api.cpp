@@ -495,6 +495,26 @@ hgdt_status hgdt_space_error_norms(hgdt_space space, const double* dofs, int ord
   });
 }
 
+hgdt_status hgdt_space_error_norms_noalloc(hgdt_space space, const double* dofs, int order, const double* g_values,
+                                           const double* g_gradients, const double* diffusion_field,
+                                           double diffusion_value, double* host_out, void* stream)
+{
+  return guarded([&] {
+    require(space && dofs && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require_device();
+    error_norms_noalloc(*space->s, dofs, order, g_values, g_gradients, diffusion_field, diffusion_value, host_out,
+                        static_cast<hipStream_t>(stream));
+  });
+}
+
+hgdt_status hgdt_noalloc_scratch_bytes(int64_t* bytes)
+{
+  return guarded([&] {
+    require(bytes, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *bytes = error_norms_scratch_bytes();
+  });
+}
+
 hgdt_status hgdt_operator_create(hgdt_space space, int stencil, hgdt_operator* op)
 {
   return guarded([&] {

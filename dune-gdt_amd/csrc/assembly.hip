@@ -20,6 +20,7 @@
 //      results are bitwise reproducible.
 
 #include <algorithm>
+#include <mutex>
 
 #include "hgdt_internal.hpp"
 
@@ -1245,6 +1246,131 @@ void error_norms(const Space& s, const double* dofs, int order, const double* gv
       for (int64_t c = 0; c < nchunks; ++c)
         if (chunk0[c] >= g.inst_elem[i] && chunk0[c] < g.inst_elem[i + 1])
           acc += part[comp * nchunks + c];
+      host_out[2 * i + comp] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// error_norms without allocations: the same kernels over a fixed device array `scratch` (one copy per GPU,
+// held until the process ends).  Its regions: the quadrature table, the chunk table, the chunk partials and
+// the element values of one window.  A window is a run of whole chunks (of NA_CHUNK elements, never
+// straddling an instance) holding at most NA_WINDOW elements; k_error_local runs on the window's elements
+// (pointer offsets) and k_sum_chunks sums each of its chunks exactly as error_norms does, so the results
+// are the same bits.  Calls that overlap on one GPU (two host threads, or two streams) race on the array.
+// ---------------------------------------------------------------------------------------------------------
+namespace {
+constexpr int64_t NA_CHUNK = 65536;             // elements per chunk (as error_norms)
+constexpr int64_t NA_WINDOW = 16 * NA_CHUNK;     // elements per window: 2 values each
+constexpr int64_t NA_TABLE = 65536;              // doubles of quadrature table
+constexpr int64_t NA_CHUNKS = 8192;              // chunk-table entries: chunks + one end mark per window
+constexpr int64_t NA_VALUES_OFF = 0;                                   // bytes
+constexpr int64_t NA_TABLE_OFF = NA_VALUES_OFF + 8 * 2 * NA_WINDOW;
+constexpr int64_t NA_CHUNKS_OFF = NA_TABLE_OFF + 8 * NA_TABLE;
+constexpr int64_t NA_PART_OFF = NA_CHUNKS_OFF + 8 * NA_CHUNKS;
+constexpr int64_t NA_BYTES = NA_PART_OFF + 8 * 2 * NA_CHUNKS;         // 16.75 MiB
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[NA_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+unsigned char* scratch_on_current_device()
+{
+  int dev = -1;
+  HGDT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEVICES)
+    HGDT_THROW(HGDT_ERR_DEVICE, "device index beyond the scratch table of the no-allocation entry points");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HGDT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+} // namespace
+
+int64_t error_norms_scratch_bytes()
+{
+  return NA_BYTES;
+}
+
+void error_norms_noalloc(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
+                         const double* kappa, double kappa_value, double* host_out, hipStream_t stream)
+{
+  const Grid& g = *s.grid;
+  if (s.kind != HGDT_CONTINUOUS_LAGRANGE && s.kind != HGDT_DISCONTINUOUS_LAGRANGE)
+    HGDT_THROW(HGDT_ERR_SPACE, "unknown space kind");
+  if (order < 0 || order > 20)
+    HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "quadrature order must be in [0, 20]");
+  QTable tab;
+  const QRule R = tab.add(element_rule(g.ref, order));
+  if (int64_t(tab.data.size()) > NA_TABLE)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "no-allocation error norms: the quadrature rule of this order needs " +
+                                           std::to_string(tab.data.size()) + " table entries, more than the " +
+                                           std::to_string(NA_TABLE) + " of the fixed scratch array");
+  // chunks (as error_norms), then windows of whole chunks; the device chunk table holds every window's
+  // starts relative to the window followed by the window's end
+  std::vector<int64_t> chunk0;
+  for (int i = 0; i < g.ninst(); ++i)
+    for (int64_t e = g.inst_elem[i]; e < g.inst_elem[i + 1]; e += NA_CHUNK)
+      chunk0.push_back(e);
+  const int64_t nchunks = int64_t(chunk0.size());
+  chunk0.push_back(g.ne);
+  std::vector<int64_t> win0{0}; // first chunk of every window, closed by nchunks
+  for (int64_t c = 0; c < nchunks; ++c)
+    if (chunk0[c + 1] - chunk0[win0.back()] > NA_WINDOW)
+      win0.push_back(c);
+  win0.push_back(nchunks);
+  const int64_t nwin = int64_t(win0.size()) - 1;
+  if (nchunks + nwin > NA_CHUNKS)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "no-allocation error norms: the grid needs " + std::to_string(nchunks) +
+                                           " chunks of at most 65536 elements in " + std::to_string(nwin) +
+                                           " windows, more than the fixed scratch array's " +
+                                           std::to_string(NA_CHUNKS) + " chunk-table entries");
+  std::vector<int64_t> table;
+  table.reserve(size_t(nchunks + nwin));
+  for (int64_t w = 0; w < nwin; ++w)
+    for (int64_t c = win0[w]; c <= win0[w + 1]; ++c)
+      table.push_back(chunk0[c] - chunk0[win0[w]]);
+  unsigned char* base = scratch_on_current_device();
+  double* d_vals = reinterpret_cast<double*>(base + NA_VALUES_OFF);
+  double* d_tab = reinterpret_cast<double*>(base + NA_TABLE_OFF);
+  int64_t* d_chunk = reinterpret_cast<int64_t*>(base + NA_CHUNKS_OFF);
+  double* d_part = reinterpret_cast<double*>(base + NA_PART_OFF);
+  std::vector<double> part(2 * nchunks, 0.);
+  if (g.ne > 0) {
+    HGDT_HIP(hipMemcpyAsync(d_tab, tab.data.data(), tab.data.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HGDT_HIP(hipMemcpyAsync(d_chunk, table.data(), table.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    const int nv = g.ref.nverts, nd = s.local_size, D = g.ref.dim;
+    for (int64_t w = 0; w < nwin; ++w) {
+      const int64_t e0 = chunk0[win0[w]], e1 = chunk0[win0[w + 1]], ne = e1 - e0;
+      const int64_t nc = win0[w + 1] - win0[w];
+      dispatch_fe(g.ref.geom, s.order, [&](auto G, auto O) {
+        hipLaunchKernelGGL((k_error_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(ne, 256)),
+                           dim3(256), 0, stream, ne, g.d_coords.ptr, g.d_conn.ptr + e0 * nv,
+                           s.d_dofs.ptr + e0 * nd, d_tab, R, dofs, gval ? gval + e0 * R.npts : nullptr,
+                           ggrad ? ggrad + e0 * R.npts * D : nullptr, kappa ? kappa + e0 : nullptr, kappa_value,
+                           d_vals);
+      });
+      HGDT_HIP(hipGetLastError());
+      // window w's chunk starts sit at table offset win0[w] + w; its partials at 2 win0[w] as [comp][chunk]
+      hipLaunchKernelGGL(k_sum_chunks, dim3(unsigned(nc), 2), dim3(256), 0, stream, d_vals, ne,
+                         d_chunk + win0[w] + w, d_part + 2 * win0[w], nc);
+      HGDT_HIP(hipGetLastError());
+    }
+    HGDT_HIP(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HGDT_HIP(hipStreamSynchronize(stream));
+  }
+  for (int i = 0; i < g.ninst(); ++i)
+    for (int comp = 0; comp < 2; ++comp) {
+      double acc = 0.;
+      for (int64_t w = 0; w < nwin; ++w)
+        for (int64_t c = win0[w]; c < win0[w + 1]; ++c)
+          if (chunk0[c] >= g.inst_elem[i] && chunk0[c] < g.inst_elem[i + 1])
+            acc += part[2 * win0[w] + comp * (win0[w + 1] - win0[w]) + (c - win0[w])];
       host_out[2 * i + comp] = acc;
     }
 }

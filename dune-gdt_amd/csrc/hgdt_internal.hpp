@@ -423,5 +423,9 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix);
 // squared L2 and (kappa-weighted) H1-semi norms of u_h - g per instance (host_out[2 * ninst])
 void error_norms(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
                  const double* kappa, double kappa_value, double* host_out, hipStream_t stream);
+// the same results (bit for bit) through a fixed device array instead of allocated buffers
+void error_norms_noalloc(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
+                         const double* kappa, double kappa_value, double* host_out, hipStream_t stream);
+int64_t error_norms_scratch_bytes();
 
 } // namespace hgdt

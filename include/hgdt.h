@@ -155,6 +155,21 @@ hgdt_status hgdt_space_error_norms(hgdt_space space, const double* dofs, int ord
                                    const double* g_gradients, const double* diffusion_field, double diffusion_value,
                                    double* host_out, void* stream);
 
+/* No reference counterpart: hgdt_space_error_norms for callers that cannot allocate device memory (code
+ * inside a tight serving or time-stepping loop, a harness that forbids allocation).  Same arguments and the
+ * same results, bit for bit; its element values, chunk sums and tables live in one fixed device array of
+ * hgdt_noalloc_scratch_bytes() bytes per GPU (16.75 MiB, held until the process ends), walked in windows of
+ * 1 048 576 elements, so it calls no hipMalloc, hipHostMalloc or hipFree (host-side set-up uses ordinary
+ * heap memory).  Limits: at most 8192 chunk-table entries (chunks of 65 536 elements, at least one per
+ * instance, plus one per window) and a quadrature table of at most 65 536 doubles; beyond them it fails with
+ * HGDT_ERR_NOT_SUPPORTED naming the limit.  Calls that overlap on one GPU (from two host threads or on two
+ * streams) share the array and race.  Blocks until the results are on the host. */
+hgdt_status hgdt_space_error_norms_noalloc(hgdt_space space, const double* dofs, int order, const double* g_values,
+                                           const double* g_gradients, const double* diffusion_field,
+                                           double diffusion_value, double* host_out, void* stream);
+/* size of the fixed device array behind the no-allocation entry points (per GPU) */
+hgdt_status hgdt_noalloc_scratch_bytes(int64_t* bytes);
+
 /* ------------------------------------------------------------------------- */
 /* matrix operator                                                           */
 /* ------------------------------------------------------------------------- */
