@@ -76,8 +76,8 @@ def test_error_norms_parity(hgdt_mod, geometry, cells, geom_o):
     assert abs(per[0] - v1) <= 1e-13 * v1 and abs(per[1] - 4 * v1) <= 1e-13 * v1
 
 
-def _esv2007(H, n):
-    grid = H.Grid.structured("quad", [-1, -1], [1, 1], [n, n])
+def _esv2007(H, n, grid=None):
+    grid = grid or H.Grid.structured("quad", [-1, -1], [1, 1], [n, n])
     space = H.DiscontinuousLagrangeSpace(grid, 1)
     op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
     op += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0))
@@ -113,6 +113,26 @@ def test_esv2007_swipdg_on_device(hgdt_mod):
         err, x, op = _esv2007(hgdt_mod, n)
         m = O.Mesh.structured(O.QUAD, [n, n], [-1, -1], [1, 1])
         s = O.Space(m, False, 1)
+        oerr = O.esv2007_h1_semi_error(s, x, 10)
+        print(f"  h1 semi error {err:.6e} (published {e_ref}, oracle on the GPU solution {oerr:.6e}), "
+              f"{op.last_solve['iterations']} iterations")
+        assert float(f"{err:.2e}") == e_ref
+        assert abs(err - oerr) <= 1e-12 * oerr
+
+
+def test_esv2007_swipdg_simplex_on_device(hgdt_mod):
+    """ESV2007 table 1 columns 1-5 on simplices (stationary_heat_equation__ESV2007__table_1.mini:11-13): P1-DG
+    SWIPDG on the bisection-refined triangle grid of ESV2007.hh:66-69, explicit grid through the C ABI; the
+    device pipeline reproduces the published H1-semi errors to three significant figures"""
+    from problems import bisection_refined_triangles
+
+    ref = GOLDEN["reference"]["esv2007_p1dg_simplex"]
+    for n, e_ref in zip((8, 16, 32), ref["h1_semi"]):
+        print(f"case ESV2007 simplex n={n}", flush=True)
+        X, conn = bisection_refined_triangles(n)
+        grid = hgdt_mod.Grid.from_arrays("tri", X, conn)
+        err, x, op = _esv2007(hgdt_mod, n, grid)
+        s = O.Space(O.Mesh.explicit(O.TRI, X, conn), False, 1)
         oerr = O.esv2007_h1_semi_error(s, x, 10)
         print(f"  h1 semi error {err:.6e} (published {e_ref}, oracle on the GPU solution {oerr:.6e}), "
               f"{op.last_solve['iterations']} iterations")
