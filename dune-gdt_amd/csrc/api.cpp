@@ -122,13 +122,21 @@ static std::shared_ptr<Grid> make_structured(int geom, const int32_t* cells, con
 void Operator::assemble(hipStream_t stream)
 {
   prepare_assembly();
+  // matrix-free mode: the right-hand side only, the operator is recomputed at every application
+  matrix_free_assembly = (requested_format == HGDT_FORMAT_MATRIX_FREE);
+  if (!matrix_free_assembly && direct_applicable()) {
+    if (dirichlet)
+      build_dirichlet();
+    assemble_direct(stream);
+    return;
+  }
   build_scatter_maps();
   if (dirichlet)
     build_dirichlet();
-  // matrix-free mode: the right-hand side only, the operator is recomputed at every application
-  matrix_free_assembly = (requested_format == HGDT_FORMAT_MATRIX_FREE);
   launch_assembly(*this, stream, !matrix_free_assembly);
   fmt.valid = false;
+  direct_storage = false;
+  values_stale = false;
 }
 
 } // namespace hgdt
@@ -350,6 +358,49 @@ hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_
   });
 }
 
+hgdt_status hgdt_grid_boundary_intersections(hgdt_grid grid, double* centers, double* normals)
+{
+  return guarded([&] {
+    require(grid, HGDT_ERR_INVALID_ARGUMENT, "NULL grid");
+    const Grid& g = *grid->g;
+    const int d = g.ref.dim, nf = g.ref.nfaces, nv = g.ref.nverts, fk = g.ref.face_nverts;
+    int64_t bi = 0;
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int f = 0; f < nf; ++f) {
+        if (g.nb[e * nf + f] >= 0)
+          continue;
+        double cen[3] = {0, 0, 0}, ecen[3] = {0, 0, 0}, v[4][3] = {};
+        for (int k = 0; k < fk; ++k) {
+          const int32_t vid = g.conn[e * nv + g.ref.faces[f][k]];
+          for (int a = 0; a < d; ++a)
+            v[k][a] = g.coords[int64_t(vid) * d + a], cen[a] += v[k][a] / fk;
+        }
+        for (int k = 0; k < nv; ++k)
+          for (int a = 0; a < d; ++a)
+            ecen[a] += g.coords[int64_t(g.conn[e * nv + k]) * d + a] / nv;
+        double nrm[3] = {0, 0, 0};
+        if (d == 2) {
+          nrm[0] = v[1][1] - v[0][1], nrm[1] = -(v[1][0] - v[0][0]);
+        } else {
+          const double a[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+          const double b[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+          nrm[0] = a[1] * b[2] - a[2] * b[1], nrm[1] = a[2] * b[0] - a[0] * b[2], nrm[2] = a[0] * b[1] - a[1] * b[0];
+        }
+        double len = 0., side = 0.;
+        for (int a = 0; a < d; ++a)
+          len += nrm[a] * nrm[a], side += nrm[a] * (cen[a] - ecen[a]);
+        len = std::sqrt(len) * (side < 0. ? -1. : 1.); // outward: away from the element's centre
+        for (int a = 0; a < d; ++a) {
+          if (centers)
+            centers[bi * d + a] = cen[a];
+          if (normals)
+            normals[bi * d + a] = nrm[a] / len;
+        }
+        ++bi;
+      }
+  });
+}
+
 hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points)
 {
   return guarded([&] {
@@ -562,9 +613,20 @@ static void require_matrix(const Operator& o)
           "the operator was assembled in matrix-free mode and holds no matrix: set another format and assemble again");
 }
 
+// CSR values of a structured assembly, computed from its storage once the assembly's work is done (the
+// assembly's stream may be gone: the device is synchronized and the default stream used)
+static void materialize(Operator& o)
+{
+  if (!o.values_stale)
+    return;
+  HGDT_HIP(hipDeviceSynchronize());
+  o.materialize_values(nullptr);
+}
+
 static void forms_changed(Operator& o)
 {
   o.tables_ready = false;
+  o.direct_state = -1;
   if (o.requested_format == HGDT_FORMAT_MATRIX_FREE) // the reference matrices follow the forms
     o.fmt.structure_ready = false, o.fmt.valid = false;
 }
@@ -596,12 +658,13 @@ static void append_face(hgdt_operator op, int kind, double param, const double* 
   const Grid& g = *op->op->space->grid;
   if (g.ref.dim != 2)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
-  FaceFormDesc F{kind, param, kappa, kv, omega, ov, over_integrate, scaling};
+  FaceFormDesc F{kind, param, kappa, kv, omega, ov, over_integrate, scaling, nullptr};
   if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY) {
     if (op->op->stencil == HGDT_STENCIL_ELEMENT)
       HGDT_THROW(HGDT_ERR_OPERATOR, "coupling intersection forms need an intersection stencil");
     op->op->cforms.push_back(F);
   } else {
+    F.mask = op->op->pending_mask;
     op->op->bforms.push_back(F);
   }
   forms_changed(*op->op);
@@ -654,9 +717,48 @@ hgdt_status hgdt_operator_append_boundary_source(hgdt_operator op, int kind, dou
     require(op && g_order >= 0 && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
     require(kind >= HGDT_BOUNDARY_DIRICHLET_COUPLING && kind <= HGDT_BOUNDARY_PRODUCT, HGDT_ERR_INVALID_ARGUMENT,
             "unknown boundary functional kind");
-    op->op->bsources.push_back({kind, param, coeff_field, coeff_value, data, value, g_order, over_integrate});
+    op->op->bsources.push_back(
+        {kind, param, coeff_field, coeff_value, data, value, g_order, over_integrate, op->op->pending_mask});
     op->op->maps.built = false; // staging layout gains the boundary right-hand side block
     forms_changed(*op->op);
+  });
+}
+
+static void check_boundary_mask(const Operator& o, const uint8_t* mask, int64_t count)
+{
+  require(!mask || count == o.space->grid->n_boundary, HGDT_ERR_SHAPES_DO_NOT_MATCH,
+          "the boundary mask needs one entry per boundary intersection");
+}
+
+hgdt_status hgdt_operator_set_boundary_filter(hgdt_operator op, const uint8_t* mask, int64_t count)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    Operator& o = *op->op;
+    check_boundary_mask(o, mask, count);
+    if (!mask) {
+      o.pending_mask = nullptr;
+      return;
+    }
+    auto d = std::make_unique<DeviceArray<uint8_t>>();
+    d->upload(std::vector<uint8_t>(mask, mask + count));
+    o.pending_mask = d->ptr;
+    o.face_masks.push_back(std::move(d));
+  });
+}
+
+hgdt_status hgdt_operator_set_dirichlet_faces(hgdt_operator op, const uint8_t* mask, int64_t count)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    Operator& o = *op->op;
+    check_boundary_mask(o, mask, count);
+    if (mask)
+      o.dirichlet_faces.assign(mask, mask + count);
+    else
+      o.dirichlet_faces.clear();
+    o.dirichlet_built = false;
+    forms_changed(o);
   });
 }
 
@@ -694,6 +796,7 @@ hgdt_status hgdt_operator_values_device(hgdt_operator op, double** values)
   return guarded([&] {
     require(op && values, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
     require_matrix(*op->op);
+    materialize(*op->op);
     *values = op->op->d_values.ptr;
   });
 }
@@ -711,6 +814,7 @@ hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out)
   return guarded([&] {
     require(op && host_out, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
     require_matrix(*op->op);
+    materialize(*op->op);
     HGDT_HIP(hipDeviceSynchronize());
     HGDT_HIP(hipMemcpy(host_out, op->op->d_values.ptr, op->op->nnz * sizeof(double), hipMemcpyDeviceToHost));
   });
@@ -725,11 +829,26 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out)
   });
 }
 
+hgdt_status hgdt_operator_assembly_info(hgdt_operator op, int32_t* structured, int32_t* solver_upper_diagonals)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    if (structured)
+      *structured = op->op->direct_storage ? 1 : 0;
+    if (solver_upper_diagonals)
+      *solver_upper_diagonals = op->op->solver_upper_diagonals();
+  });
+}
+
 hgdt_status hgdt_operator_set_format(hgdt_operator op, int format)
 {
   return guarded([&] {
     require(op && format >= HGDT_FORMAT_AUTO && format <= HGDT_FORMAT_BLOCK, HGDT_ERR_INVALID_ARGUMENT,
             "unknown format");
+    // the CSR values of a structured assembly are computed from its storage before that storage is rebuilt
+    materialize(*op->op);
+    op->op->direct_storage = false;
+    op->op->direct_state = -1;
     op->op->requested_format = (format == HGDT_FORMAT_DIA_NONSYMMETRIC) ? HGDT_FORMAT_DIA : format;
     op->op->allow_symmetric = (format != HGDT_FORMAT_DIA_NONSYMMETRIC);
     op->op->fmt.partitions_ready = false;

@@ -73,6 +73,7 @@ struct FaceParams {
   double omega_value[MAX_FFORMS];
   double scaling[MAX_FFORMS];
   QRule rule[MAX_FFORMS];
+  const uint8_t* mask[MAX_FFORMS]; // boundary forms: applied on the intersections with a nonzero byte (NULL: all)
   double* staging;
   int64_t base;
   QRule brule[MAX_BSOURCES]; // line rules of the boundary right-hand side terms
@@ -97,6 +98,7 @@ struct BoundaryRhsParams {
   const double* data[MAX_BSOURCES];
   double value[MAX_BSOURCES];
   QRule rule[MAX_BSOURCES];
+  const uint8_t* mask[MAX_BSOURCES]; // applied on the intersections with a nonzero byte (NULL: all)
   double* staging;
   int64_t base;
 };
@@ -617,6 +619,8 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
     for (int j = 0; j < ND; ++j)
       B[i][j] = 0.;
   for (int fm = 0; fm < P.nforms; ++fm) {
+    if (P.mask[fm] && !P.mask[fm][fi]) // ApplyOn::CustomBoundaryIntersections: not one of this form's faces
+      continue;
     const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
     const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
     const double sc = P.scaling[fm], prm = P.param[fm];
@@ -680,6 +684,8 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
   for (int i = 0; i < ND; ++i)
     b[i] = 0.;
   for (int tm = 0; tm < P.nterms; ++tm) {
+    if (P.mask[tm] && !P.mask[tm][fi])
+      continue;
     const double c = P.coeff[tm] ? P.coeff[tm][e] : P.coeff_value[tm];
     const double prm = P.param[tm];
     const QRule R = P.rule[tm];
@@ -1050,7 +1056,7 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, Element
     for (int f = 0; f < F.nforms; ++f) {
       const auto& D = forms[f];
       F.kind[f] = D.kind, F.param[f] = D.param, F.kappa[f] = D.kappa, F.kappa_value[f] = D.kappa_value;
-      F.omega[f] = D.omega, F.omega_value[f] = D.omega_value, F.scaling[f] = D.scaling;
+      F.omega[f] = D.omega, F.omega_value[f] = D.omega_value, F.scaling[f] = D.scaling, F.mask[f] = D.mask;
       // coupling / penalty orders: 0 + 0 + test + ansatz (laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
       Quadrature q = line_rule(0 + k + k + D.over_integrate);
       F.rule[f] = ltab.add(q);
@@ -1152,7 +1158,7 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     for (int t = 0; t < BR.nterms; ++t) {
       const auto& S = op.bsources[t];
       BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
-      BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t];
+      BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
     if (g.ref.geom == HGDT_QUAD)
       hipLaunchKernelGGL(k_boundary_rhs<HGDT_QUAD>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);

@@ -64,6 +64,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
   __syncthreads();
 }
 
+// *flag = 1 when `pred` holds for an active lane of the wave: one store per wave (by its first active lane)
+// when every active lane names the same flag, else one store per lane whose predicate holds.  Every store
+// writes the same value, so the flag needs no atomic.
+__device__ __forceinline__ void wave_flag(bool pred, int* flag)
+{
+  if (!__ballot(pred))
+    return;
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  const long long mine = (long long)(uintptr_t)flag;
+  const bool same = !__ballot(mine != __shfl(mine, leader, 64));
+  if (same ? int(threadIdx.x & 63) == leader : pred)
+    *flag = 1;
+}
+
 // sum of an instance's block partials (written by an earlier kernel) in block order
 // (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
 // partials are issued together, so the prologue costs one memory round trip per U * B partials:

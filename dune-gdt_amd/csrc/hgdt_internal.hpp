@@ -185,6 +185,9 @@ struct FaceFormDesc {
   double omega_value;
   int over_integrate;
   double scaling;
+  // boundary forms: the boundary intersections (walk order) the form is applied on, one byte each (device,
+  // owned by the operator); NULL: every boundary intersection
+  const uint8_t* mask;
 };
 struct SourceDesc {
   int kind;
@@ -204,6 +207,7 @@ struct BoundarySourceDesc {
   double value;
   int g_order;
   int over_integrate;
+  const uint8_t* mask; // as FaceFormDesc::mask
 };
 
 // launch configuration of one tunable kernel
@@ -314,6 +318,19 @@ __host__ __device__ inline int pcg_state(const PcgScalars& s)
 
 struct SolverCache; // solver.hip: events, pinned buffers, captured graph
 
+// one instance of a structured grid of Kuhn tetrahedra for the structured assembly: its first DoF and
+// element, its cells per direction, and its unit-coefficient local quantities (the same for every
+// tetrahedron of an axis-parallel lattice): wT[a] = |T| / h_a^2 (the P1 Laplace matrix of a tetrahedron with
+// path v0 -> v0 + e_p0 -> v0 + e_p0 + e_p1 -> v7 has diagonal wT[p0], wT[p0] + wT[p1], wT[p1] + wT[p2], wT[p2]
+// and the entries -wT[p_k] between consecutive path vertices, nothing else) and |T| / 4 (the local
+// right-hand side of a unit source)
+struct DirectInst {
+  int64_t dof0, elem0;
+  int32_t nx, ny, nz, pad;
+  double wT[3];
+  double quarter_vol;
+};
+
 struct Operator {
   std::shared_ptr<Space> space;
   int stencil = HGDT_STENCIL_ELEMENT;
@@ -337,6 +354,13 @@ struct Operator {
     return 0 + k + S.g_order + S.over_integrate;
   }
   bool dirichlet = false;
+  // the Dirichlet intersections of the constraints (boundary_info.type == DirichletBoundary), one byte per
+  // boundary intersection in walk order; empty: every boundary intersection
+  std::vector<uint8_t> dirichlet_faces;
+  // boundary-intersection masks of the appended boundary forms and functionals, and the one the next appends
+  // take (ApplyOn::CustomBoundaryIntersections); NULL: every boundary intersection
+  std::vector<std::unique_ptr<DeviceArray<uint8_t>>> face_masks;
+  const uint8_t* pending_mask = nullptr;
   bool matrix_free_assembly = false; // the last assemble() ran in matrix-free mode: no CSR values
   std::vector<int32_t> dirichlet_dofs;
   DeviceArray<uint8_t> d_slot_flag; // 0 keep, 1 zero, 2 one
@@ -361,6 +385,19 @@ struct Operator {
   std::vector<int64_t> last_iters;
   std::vector<double> last_relres;
   std::vector<double> last_true_relres; // |b - A x| / |b| at exit (-1: not computed)
+
+  // Structured assembly (P1 on the Kuhn tetrahedra of axis-parallel lattices with Laplace element forms and
+  // constant or per-element sources): the symmetric DIA storage of the solver (diagonal and upper diagonals),
+  // D^{-1/2}, D^{1/2} and the right-hand side are written by one pass over the DoFs, and the CSR values are
+  // computed from that storage only when they are asked for.
+  bool direct_storage = false; // the DIA storage holds the last assembly (no CSR conversion needed)
+  bool values_stale = false;   // d_values does not hold the last assembly yet
+  int direct_state = -1;       // -1 not checked since the last change of forms or format, 0 no, 1 yes
+  DeviceArray<DirectInst> d_direct_inst;
+  bool direct_applicable();
+  void assemble_direct(hipStream_t stream);
+  void materialize_values(hipStream_t stream); // CSR values from the structured assembly's storage
+  int solver_upper_diagonals() const;          // strict upper diagonals of the solver's symmetric DIA copy
 
   void build_pattern();
   void build_scatter_maps();

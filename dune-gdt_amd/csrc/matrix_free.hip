@@ -774,6 +774,11 @@ void mf_setup(Operator& op)
     unsupported("Dirichlet constraints are provided on continuous spaces");
   if (op.eforms.size() > size_t(MF_MAXF) || op.cforms.size() > size_t(MF_MAXF) || op.bforms.size() > size_t(MF_MAXF))
     unsupported("too many forms appended");
+  for (const auto& b : op.bforms)
+    if (b.mask)
+      unsupported("boundary forms are applied on every boundary intersection (no boundary-info filter)");
+  if (!op.dirichlet_faces.empty())
+    unsupported("Dirichlet constraints are applied on every boundary intersection (no boundary info)");
   const int D = g.ref.dim, NV = 1 << D;
   const int ninst = g.ninst();
 

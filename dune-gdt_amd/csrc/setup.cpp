@@ -630,11 +630,17 @@ void Operator::build_dirichlet()
         if (inside)
           face_nodes[f].push_back(i);
       }
+    // boundary intersections in walk order; only the Dirichlet ones (all, without a boundary info) constrain
+    int64_t bi = 0;
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f)
-        if (g.nb[e * nf + f] < 0)
-          for (int i : face_nodes[f])
-            mask[s.dofs[e * ls + i]] = 1;
+        if (g.nb[e * nf + f] < 0) {
+          const bool dir = dirichlet_faces.empty() || dirichlet_faces[bi] != 0;
+          ++bi;
+          if (dir)
+            for (int i : face_nodes[f])
+              mask[s.dofs[e * ls + i]] = 1;
+        }
   }
   dirichlet_dofs.clear();
   for (int64_t i = 0; i < n; ++i)

@@ -98,6 +98,10 @@ struct PcgParams {
   const int64_t* dia_off; // ninst * ndiag, ascending, sentinel n for unused slots
   const int32_t* dia_k0;  // ninst: index of offset 0
   const double* dia;      // ndiag * n
+  // the solver's scaled copy of a symmetric DIA matrix: the strict upper diagonals with a nonzero entry,
+  // s_nu slots (values scaled[u * n + row]), offsets s_off[inst * s_nu + u] ascending (sentinel n: padding)
+  int s_nu;
+  const int64_t* s_off;
   // SELL
   const int64_t* block_slice0; // per A-block
   const int64_t* slice_ptr;
@@ -170,8 +174,14 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
           acc[j] = unit ? acc[j] + x[j] : fma(a[j], x[j], acc[j]);
       }
     } else {
-      // lower triangle from the upper diagonals (descending offset = ascending column), then upper
-      const int ku = P.ndiag - 1 - k0; // strict upper slots of the instance (trailing ones may be padding)
+      // lower triangle from the upper diagonals (descending offset = ascending column), then upper.
+      // Strict upper slot u has offset uo[u] and its values at ub[u * n + row].  The assembled matrix's
+      // storage (!UNIT) holds every diagonal of the pattern; the solver's scaled copy (UNIT) holds only the
+      // strict upper diagonals with a nonzero entry in the instance (s_nu slots, offsets s_off), which
+      // leaves out FMAs that would add 0 * x: the same sums.
+      const int ku = UNIT ? P.s_nu : P.ndiag - 1 - k0; // (trailing slots may be padding: offset n)
+      const int64_t* __restrict__ uo = UNIT ? P.s_off + int64_t(inst) * P.s_nu : off + k0 + 1;
+      const double* __restrict__ ub = UNIT ? dia : dia + int64_t(k0 + 1) * n;
       if constexpr (ITEMS == 1) {
         // the loads of up to 8 diagonals are issued before their FMAs (few memory round trips per
         // row); the FMAs run in the order of the loops below, so the bits are the same
@@ -186,9 +196,9 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
             const int u = top - t;
             a[t] = x[t] = 0.;
             if (u >= 0) {
-              const int64_t cl = i - off[k0 + 1 + u];
+              const int64_t cl = i - uo[u];
               if (in && cl >= lo && cl < hi) {
-                a[t] = dia[int64_t(k0 + 1 + u) * n + cl];
+                a[t] = ub[int64_t(u) * n + cl];
                 x[t] = v(cl);
               }
             }
@@ -213,9 +223,9 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
             const int u = bot + t;
             a[t] = x[t] = 0.;
             if (u < ku) {
-              const int64_t cu = i + off[k0 + 1 + u];
+              const int64_t cu = i + uo[u];
               if (in && cu >= lo && cu < hi) {
-                a[t] = dia[int64_t(k0 + 1 + u) * n + i];
+                a[t] = ub[int64_t(u) * n + i];
                 x[t] = v(cu);
               }
             }
@@ -229,9 +239,9 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
         return;
       }
 #pragma unroll 2
-      for (int k = P.ndiag - 1; k > k0; --k) {
-        const int64_t o = off[k];
-        const double* __restrict__ dk = dia + int64_t(k) * n;
+      for (int u = ku - 1; u >= 0; --u) {
+        const int64_t o = uo[u];
+        const double* __restrict__ dk = ub + int64_t(u) * n;
         double a[ITEMS], x[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
@@ -245,18 +255,21 @@ __device__ __forceinline__ void apply_rows(const PcgParams& P, const double* __r
         for (int j = 0; j < ITEMS; ++j)
           acc[j] = fma(a[j], x[j], acc[j]);
       }
-      if constexpr (UNIT) { // the unit diagonal, between the lower and the upper part
+      // the diagonal, between the lower and the upper part (unit in the solver's scaled copy)
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-          const int64_t i = r0 + threadIdx.x + j * B;
-          if (i < r1)
+      for (int j = 0; j < ITEMS; ++j) {
+        const int64_t i = r0 + threadIdx.x + j * B;
+        if (i < r1) {
+          if constexpr (UNIT)
             acc[j] += v(i);
+          else
+            acc[j] = fma(dia[int64_t(k0) * n + i], v(i), acc[j]);
         }
       }
 #pragma unroll 2
-      for (int k = (UNIT ? k0 + 1 : k0); k < P.ndiag; ++k) {
-        const int64_t o = off[k];
-        const double* __restrict__ dk = dia + int64_t(k) * n;
+      for (int u = 0; u < ku; ++u) {
+        const int64_t o = uo[u];
+        const double* __restrict__ dk = ub + int64_t(u) * n;
         double a[ITEMS], x[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
@@ -759,11 +772,10 @@ __global__ void __launch_bounds__(1024) k_cg_spmv_win(PcgParams P, int par)
   const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
   const double* __restrict__ p = par ? P.p : P.p2; // p_j (written by the direction update of step j)
   const int64_t lo = P.inst_row0[inst], hi = P.inst_row0[inst + 1];
-  const int64_t* __restrict__ off = P.dia_off + int64_t(inst) * P.ndiag;
+  const int64_t* __restrict__ off = P.s_off + int64_t(inst) * P.s_nu; // the scaled copy's strict upper slots
   const double* __restrict__ dia = P.scaled;
   const int64_t n = P.n;
-  const int k0 = P.dia_k0[inst];
-  const int ku = P.ndiag - 1 - k0; // <= WIN_MAXU (checked by the host)
+  const int ku = P.s_nu; // <= WIN_MAXU (checked by the host)
   const DiaWindows& W = P.win[inst];
   const int t = threadIdx.x, R = blockDim.x;
   const int64_t i = rb + t;
@@ -792,8 +804,8 @@ __global__ void __launch_bounds__(1024) k_cg_spmv_win(PcgParams P, int par)
   for (int u = 0; u < WIN_MAXU; ++u) {
     al[u] = au[u] = 0.;
     if (u < ku) {
-      const int64_t o = off[k0 + 1 + u];
-      const double* __restrict__ du = dia + int64_t(k0 + 1 + u) * n;
+      const int64_t o = off[u];
+      const double* __restrict__ du = dia + int64_t(u) * n;
       if (in && i - o >= lo)
         al[u] = du[i - o];
       if (in && i + o < hi)
@@ -1049,7 +1061,7 @@ __global__ void k_jacobi(int64_t n, const int64_t* __restrict__ row_ptr, const i
 __global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ off, const int32_t* __restrict__ k0v,
                               const int32_t* __restrict__ mirror, const int32_t* __restrict__ row_inst,
                               const int64_t* __restrict__ inst_row0, const double* __restrict__ dia,
-                              double* __restrict__ dh, double* __restrict__ dsq, int* flags)
+                              double* __restrict__ dh, double* __restrict__ dsq, int* flags, int* nz)
 {
   const int64_t i = xcd_index(n);
   if (i >= n)
@@ -1073,10 +1085,194 @@ __global__ void k_dia_prepare(int64_t n, int ndiag, const int64_t* __restrict__ 
     if (c < lo || c >= hi)
       continue; // outside the instance: padding, never read
     const int m = mir[k];
-    ok = ok && (m >= 0) && (dia[int64_t(m) * n + c] == dia[int64_t(k) * n + i]);
+    if (m < 0) {
+      ok = false;
+      continue;
+    }
+    // A(c, c - o) on the upper diagonal m: every upper entry inside the instance is met once here, so the
+    // upper diagonals that hold a nonzero entry are flagged on the way (same value stored by every writer)
+    const double up = dia[int64_t(m) * n + c];
+    ok = ok && (up == dia[int64_t(k) * n + i]);
+    wave_flag(up != 0., nz + int64_t(inst) * ndiag + m);
   }
   if (!ok)
     flags[1] = 1;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// structured assembly: P1 Laplace forms and constant / per-element sources on the Kuhn tetrahedra of
+// axis-parallel lattices (the grid walk of MatrixOperator::assemble, matrix-based.hh:496-500, with the local
+// matrices of LocalElementIntegralBilinearForm(LocalLaplaceIntegrand), local/bilinear-forms/integrals.hh:97-134,
+// local/integrands/laplace.hh:74-102, and the scatter of bilinear-form-assemblers.hh:110-128), written straight
+// into the solver's symmetric DIA storage.  One lane per DoF (vertex) visits the 24 tetrahedra around it in the
+// walk order (cells ascending, the six tetrahedra of a cell in permutation order) and adds each one's entries
+// of its row: the diagonal and the entry with the next vertex on the tetrahedron's path, the only nonzero
+// strict upper entries of a path tetrahedron's Laplace matrix (so the face and body diagonals of the stencil
+// stay exactly 0, as the reference's assembly also leaves them).  The DIA slots k0 + u, u = dx + 2 dy + 4 dz,
+// hold the entries with the vertex (x + dx, y + dy, z + dz) (ascending offsets 1, nx+1, nx+2, (nx+1)(ny+1), ...).
+// Dirichlet rows get a unit diagonal, Dirichlet columns 0 (dirichlet-constraints.hh:153-184); the row's
+// D^{-1/2} and D^{1/2} and the nonzero flags of the diagonals are written on the way.
+// ---------------------------------------------------------------------------------------------------------
+constexpr int DIRECT_MAX_FORMS = 8, DIRECT_MAX_SOURCES = 4;
+struct DirectParams {
+  int64_t n;
+  int ninst;
+  const DirectInst* inst;
+  int nforms;
+  const double* field[DIRECT_MAX_FORMS];
+  double value[DIRECT_MAX_FORMS];
+  double scaling[DIRECT_MAX_FORMS];
+  int nsrc;
+  const double* src_data[DIRECT_MAX_SOURCES]; // per element (NULL: the constant)
+  double src_value[DIRECT_MAX_SOURCES];
+  const uint8_t* mask; // Dirichlet DoFs (NULL: no constraints)
+  double* dia;         // row i of slot u at dia[u * n + i]: u = 0 the diagonal, u = 1 .. 7 the upper diagonals
+  double* rhs;
+  double* dh;
+  double* dsq;
+  int* flags; // [0] non-positive diagonal; nonzero flag of slot u of instance t at nz[t * ndiag + k0 + u]
+  int* nz;
+  int ndiag, k0;
+};
+
+__global__ void __launch_bounds__(256) k_direct_p1_tets(DirectParams P)
+{
+  constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  const int64_t i = xcd_index(P.n);
+  if (i >= P.n)
+    return;
+  int t0 = 0, t1 = P.ninst - 1; // the instance: last first-DoF <= i
+  while (t0 < t1) {
+    const int mid = (t0 + t1 + 1) >> 1;
+    if (P.inst[mid].dof0 <= i)
+      t0 = mid;
+    else
+      t1 = mid - 1;
+  }
+  const DirectInst I = P.inst[t0];
+  const int64_t nvx = I.nx + 1, nvy = I.ny + 1, local = i - I.dof0;
+  const int64_t x = local % nvx, y = (local / nvx) % nvy, z = local / (nvx * nvy);
+  double ad = 0., ax = 0., ay = 0., az = 0., b = 0.; // diagonal, entries with +e_x, +e_y, +e_z
+  for (int64_t cz = z - 1; cz <= z; ++cz) {
+    if (cz < 0 || cz >= I.nz)
+      continue;
+    for (int64_t cy = y - 1; cy <= y; ++cy) {
+      if (cy < 0 || cy >= I.ny)
+        continue;
+      for (int64_t cx = x - 1; cx <= x; ++cx) {
+        if (cx < 0 || cx >= I.nx)
+          continue;
+        const int corner = int(x - cx) | (int(y - cy) << 1) | (int(z - cz) << 2);
+        const int64_t e0 = I.elem0 + 6 * (cx + I.nx * (cy + I.ny * cz));
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+          const int c1 = 1 << PERM[p][0], c2 = c1 | (1 << PERM[p][1]);
+          const int li = corner == 0 ? 0 : corner == c1 ? 1 : corner == c2 ? 2 : corner == 7 ? 3 : -1;
+          if (li < 0)
+            continue;
+          const int64_t e = e0 + p;
+          // unit-coefficient row li: diagonal wT[prev axis] + wT[next axis], entry -wT[next axis] with the next
+          // path vertex (one step along the next axis)
+          const int prev = li > 0 ? PERM[p][li - 1] : -1, next = li < 3 ? PERM[p][li] : -1;
+          const double kd = (prev >= 0 ? I.wT[prev] : 0.) + (next >= 0 ? I.wT[next] : 0.);
+          const double ku = next >= 0 ? -I.wT[next] : 0.;
+          double cd = 0., cu = 0.;
+          for (int f = 0; f < P.nforms; ++f) {
+            const double c = P.field[f] ? P.field[f][e] : P.value[f];
+            cd += P.scaling[f] * (c * kd);
+            cu += P.scaling[f] * (c * ku);
+          }
+          ad += cd;
+          if (next == 0)
+            ax += cu;
+          else if (next == 1)
+            ay += cu;
+          else if (next == 2)
+            az += cu;
+          for (int sidx = 0; sidx < P.nsrc; ++sidx)
+            b += (P.src_data[sidx] ? P.src_data[sidx][e] : P.src_value[sidx]) * I.quarter_vol;
+        }
+      }
+    }
+  }
+  // Dirichlet constraints: unit row, zero columns, b_i = 0
+  const int64_t S = nvx * nvy;
+  if (P.mask) {
+    if (P.mask[i]) {
+      ad = 1., ax = ay = az = 0., b = 0.;
+    } else {
+      if (x + 1 < nvx && P.mask[i + 1])
+        ax = 0.;
+      if (y + 1 < nvy && P.mask[i + nvx])
+        ay = 0.;
+      if (z < I.nz && P.mask[i + S])
+        az = 0.;
+    }
+  }
+  const int64_t n = P.n;
+  P.dia[i] = ad;
+  P.dia[1 * n + i] = ax;
+  P.dia[2 * n + i] = ay;
+  P.dia[3 * n + i] = 0.;
+  P.dia[4 * n + i] = az;
+  P.dia[5 * n + i] = 0.;
+  P.dia[6 * n + i] = 0.;
+  P.dia[7 * n + i] = 0.;
+  P.rhs[i] = b;
+  double d = ad;
+  if (!(d > 0.)) {
+    P.flags[0] = 1;
+    d = 1.;
+  }
+  const double sq = sqrt(d);
+  P.dh[i] = 1. / sq;
+  P.dsq[i] = sq;
+  int* nzi = P.nz + int64_t(t0) * P.ndiag + P.k0;
+  wave_flag(ax != 0., nzi + 1);
+  wave_flag(ay != 0., nzi + 2);
+  wave_flag(az != 0., nzi + 4);
+}
+
+// CSR values from symmetric DIA storage of which only the diagonal and the upper diagonals are written: slot s
+// sits at pos[s] = k n + i; a lower entry (k < k0) is the upper entry of its mirrored diagonal at row i + o
+__global__ void k_values_from_upper(int64_t nnz, int64_t n, int ndiag, const int64_t* __restrict__ pos,
+                                    const int64_t* __restrict__ off, const int32_t* __restrict__ k0v,
+                                    const int32_t* __restrict__ mirror, const int32_t* __restrict__ row_inst,
+                                    const double* __restrict__ dia, double* __restrict__ values)
+{
+  const int64_t s = xcd_index(nnz);
+  if (s >= nnz)
+    return;
+  const int64_t p = pos[s];
+  const int k = int(p / n);
+  const int64_t i = p - int64_t(k) * n;
+  const int inst = row_inst[i];
+  if (k >= k0v[inst]) {
+    values[s] = dia[p];
+  } else {
+    const int64_t t = int64_t(inst) * ndiag + k;
+    values[s] = dia[int64_t(mirror[t]) * n + i + off[t]];
+  }
+}
+
+// the solver's scaled copy of a symmetric DIA matrix: Ah(i, i+o) = a(i, i+o) (h_i h_{i+o}) for the kept strict
+// upper diagonals (kidx: index of slot u's diagonal in the full layout, -1 for padding), compact
+// (dst[u * n + i]); entries outside the instance are 0
+__global__ void k_scale_upper(int64_t n, int s_nu, const int64_t* __restrict__ s_off, const int32_t* __restrict__ kidx,
+                              const int32_t* __restrict__ row_inst, const int64_t* __restrict__ inst_row0,
+                              const double* __restrict__ dia, const double* __restrict__ dh, double* __restrict__ dst)
+{
+  const int64_t i = xcd_index(n);
+  if (i >= n)
+    return;
+  const int inst = row_inst[i];
+  const int64_t lo = inst_row0[inst], hi = inst_row0[inst + 1];
+  const double hi_i = dh[i];
+  for (int u = 0; u < s_nu; ++u) {
+    const int k = kidx[int64_t(inst) * s_nu + u];
+    const int64_t c = i + s_off[int64_t(inst) * s_nu + u];
+    dst[int64_t(u) * n + i] = (k >= 0 && c >= lo && c < hi) ? dia[int64_t(k) * n + i] * (hi_i * dh[c]) : 0.;
+  }
 }
 
 template <class F>
@@ -1141,8 +1337,18 @@ struct SolverCache {
   int win_nmax = 0;
   int64_t win_span = 0, win_wmax = 0;
   DeviceArray<int32_t> row_inst;
-  DeviceArray<int> flags; // [0] non-positive diagonal, [1] asymmetric
+  // [0] non-positive diagonal, [1] asymmetric, then (DIA) per instance and diagonal: holds a nonzero entry
+  DeviceArray<int> flags;
   int* host_flags = nullptr;
+  int64_t host_flags_capacity = 0;
+  // the solver's scaled copy of a symmetric DIA matrix keeps the strict upper diagonals with a nonzero entry:
+  // per instance their offsets (host), and as tables [ninst][s_nu] their offsets (sentinel n) and their index
+  // in the full layout (-1 for padding)
+  int s_nu = -1;
+  std::vector<std::vector<int64_t>> s_offs;
+  std::vector<int32_t> s_kidx_host;
+  DeviceArray<int64_t> s_off;
+  DeviceArray<int32_t> s_kidx;
   PcgScalars* host_scalars[2] = {nullptr, nullptr};
   int host_capacity = 0;
   // true relative residual per instance at the solver's exit (device and pinned host copy)
@@ -1399,24 +1605,18 @@ static void plan_windows(Operator& op, int R)
 {
   SolverCache& C = cache_of(op);
   const Space& s = *op.space;
-  const auto& offs = C.dia_offs;
-  const int nd = op.fmt.ndiag;
+  const auto& offs = C.s_offs; // the strict upper offsets of the solver's scaled copy, per instance
   const int ninst = int(offs.size());
   std::vector<DiaWindows> w(ninst);
-  bool ok = ninst > 0;
+  bool ok = ninst > 0 && C.s_nu <= WIN_MAXU;
   int nmax = 0;
   int64_t span = 0, wmax = 0;
   for (int in = 0; in < ninst && ok; ++in) {
-    const int k0 = C.dia_k0_host[in];
     const int64_t size = s.inst_dof[in + 1] - s.inst_dof[in];
-    if (nd - 1 - k0 > WIN_MAXU) {
-      ok = false;
-      break;
-    }
     std::vector<int64_t> o{0};
-    for (size_t k = size_t(k0) + 1; k < offs[in].size(); ++k)
-      if (offs[in][k] < size)
-        o.push_back(offs[in][k]), o.push_back(-offs[in][k]);
+    for (size_t u = 0; u < offs[in].size(); ++u)
+      if (offs[in][u] < size)
+        o.push_back(offs[in][u]), o.push_back(-offs[in][u]);
     std::sort(o.begin(), o.end());
     const int m = int(o.size());
     constexpr int64_t INF = INT64_MAX / 4;
@@ -1465,10 +1665,9 @@ static void plan_windows(Operator& op, int R)
     place(0, d.zero_w, d.zero_rel);
     for (int u = 0; u < WIN_MAXU; ++u) {
       d.dn_w[u] = d.up_w[u] = -1;
-      const size_t k = size_t(k0) + 1 + u;
-      if (k < offs[in].size() && offs[in][k] < size) {
-        place(-offs[in][k], d.dn_w[u], d.dn_rel[u]);
-        place(offs[in][k], d.up_w[u], d.up_rel[u]);
+      if (size_t(u) < offs[in].size() && offs[in][u] < size) {
+        place(-offs[in][u], d.dn_w[u], d.dn_rel[u]);
+        place(offs[in][u], d.up_w[u], d.up_rel[u]);
       }
     }
     nmax = std::max(nmax, int(d.nwin));
@@ -1495,6 +1694,58 @@ static int64_t win_lds_bytes(const Operator& op)
     return 0;
   const int64_t bytes = (int64_t(C.win_nmax) * F.A.block + C.win_span) * int64_t(sizeof(double));
   return bytes <= WIN_LDS ? bytes : 0;
+}
+
+// device flag array and its pinned host copy, at least `count` ints
+static void flag_buffers(SolverCache& C, int64_t count)
+{
+  if (int64_t(C.flags.count) < count)
+    C.flags.allocate(size_t(count));
+  if (C.host_flags_capacity < count) {
+    if (C.host_flags)
+      HGDT_HIP(hipHostFree(C.host_flags));
+    C.host_flags = nullptr;
+    HGDT_HIP(hipHostMalloc((void**)&C.host_flags, size_t(count) * sizeof(int)));
+    C.host_flags_capacity = count;
+  }
+}
+
+// tables of the solver's scaled copy of a symmetric DIA matrix from the per-instance nonzero flags of the
+// diagonals (nz[inst * ndiag + k]): the strict upper diagonals that hold a nonzero entry, ascending.  A
+// diagonal whose entries are all exactly zero adds only 0 * p to every row, so leaving it out changes no
+// sum.  The tables are uploaded (and captured graphs and column windows invalidated) only when they change.
+static void compact_upper(Operator& op, const int* nz)
+{
+  SolverCache& C = cache_of(op);
+  const SolverFormat& F = op.fmt;
+  const int ninst = int(C.dia_offs.size()), nd = F.ndiag;
+  std::vector<std::vector<int32_t>> kept(ninst);
+  int snu = 0;
+  for (int in = 0; in < ninst; ++in) {
+    for (int k = C.dia_k0_host[in] + 1; k < int(C.dia_offs[in].size()); ++k)
+      if (nz[int64_t(in) * nd + k])
+        kept[in].push_back(k);
+    snu = std::max(snu, int(kept[in].size()));
+  }
+  std::vector<int32_t> kidx(std::max<int64_t>(int64_t(ninst) * snu, 1), -1);
+  std::vector<int64_t> off(std::max<int64_t>(int64_t(ninst) * snu, 1), op.n);
+  std::vector<std::vector<int64_t>> offs(ninst);
+  for (int in = 0; in < ninst; ++in)
+    for (size_t u = 0; u < kept[in].size(); ++u) {
+      kidx[int64_t(in) * snu + u] = kept[in][u];
+      off[int64_t(in) * snu + u] = C.dia_offs[in][kept[in][u]];
+      offs[in].push_back(C.dia_offs[in][kept[in][u]]);
+    }
+  if (snu == C.s_nu && kidx == C.s_kidx_host)
+    return;
+  C.s_kidx.upload(kidx);
+  C.s_off.upload(off);
+  C.s_kidx_host = std::move(kidx);
+  C.s_offs = std::move(offs);
+  C.s_nu = snu;
+  C.win_ok = false, C.win_block = 0; // the column windows follow the kept offsets
+  C.win.release();
+  ++C.version; // captured graphs hold the tables and the slot count
 }
 
 static void build_structure(Operator& op)
@@ -1608,6 +1859,7 @@ static void build_structure(Operator& op)
       }
     C.dia_mirror.upload(mirror);
     C.dia_offs = offs, C.dia_k0_host = k0;
+    C.s_nu = -1, C.s_kidx_host.clear(), C.s_offs.clear(); // compact tables rebuilt at the next preparation
     C.win_block = 0; // windows planned for the row blocks of the solver
     F.d_dia_offsets.upload(table);
     C.dia_k0.upload(k0);
@@ -1686,8 +1938,10 @@ static int64_t step_bytes(const Operator& op)
     return mf_bytes(op, mf_fused_direction(op) ? MF_PCG_DIR : MF_PCG);
   if (F.format == HGDT_FORMAT_BLOCK) // the unit diagonal is not read
     return 8 * ((F.symmetric ? 6 : 12) * F.blk_ne + 16 * (F.symmetric ? F.blk_up : F.blk_all)) + 16 * op.n;
+  if (F.format == HGDT_FORMAT_DIA && F.symmetric) // the strict upper diagonals with a nonzero entry
+    return int64_t(std::max(op.solver_cache ? op.solver_cache->s_nu : 0, 0)) * 8 * op.n + 16 * op.n;
   if (F.format == HGDT_FORMAT_DIA)
-    return int64_t((F.symmetric ? F.nup : F.ndiag) - 1) * 8 * op.n + 16 * op.n;
+    return int64_t(F.ndiag - 1) * 8 * op.n + 16 * op.n;
   return 12 * op.nnz + 16 * op.n;
 }
 
@@ -1709,6 +1963,8 @@ static PcgParams make_params(Operator& op)
   P.ndiag = F.ndiag;
   P.dia_off = F.d_dia_offsets.ptr;
   P.dia_k0 = C.dia_k0.ptr;
+  P.s_nu = std::max(C.s_nu, 0);
+  P.s_off = C.s_off.ptr;
   P.dia = F.d_dia.ptr;
   P.win = C.win.ptr;
   P.block_slice0 = C.block_slice0.ptr;
@@ -1736,6 +1992,121 @@ static int kernel_format(const Operator& op)
   return op.fmt.symmetric ? FMT_DIAS : FMT_DIA;
 }
 
+// the structured assembly applies: P1 on the Kuhn tetrahedra of axis-parallel lattices (every instance), Laplace
+// element forms only, constant or per-element sources, symmetric DIA storage with the Kuhn stencil's 15 diagonals
+bool Operator::direct_applicable()
+{
+  if (direct_state >= 0)
+    return direct_state == 1;
+  direct_state = 0;
+  const Space& s = *space;
+  const Grid& g = *s.grid;
+  // every instance a lattice (a structured grid, or a union of structured grids, which keeps their lattices)
+  if (g.ref.geom != HGDT_TET || int(g.lattice.size()) != g.ninst() || g.ninst() < 1)
+    return false;
+  if (s.kind != HGDT_CONTINUOUS_LAGRANGE || s.order != 1 || stencil != HGDT_STENCIL_ELEMENT)
+    return false;
+  if (eforms.empty() || eforms.size() > size_t(DIRECT_MAX_FORMS) || !cforms.empty() || !bforms.empty() ||
+      !bsources.empty() || sources.size() > size_t(DIRECT_MAX_SOURCES))
+    return false;
+  for (const auto& f : eforms)
+    if (f.kind != EF_LAPLACE)
+      return false;
+  for (const auto& src : sources)
+    if (src.kind != HGDT_SOURCE_CONSTANT && src.kind != HGDT_SOURCE_PER_ELEMENT)
+      return false;
+  if (!(requested_format == HGDT_FORMAT_AUTO || requested_format == HGDT_FORMAT_DIA) || !allow_symmetric)
+    return false;
+  if (!fmt.partitions_ready)
+    build_partitions(*this);
+  if (!fmt.structure_ready)
+    build_structure(*this);
+  SolverCache& C = cache_of(*this);
+  if (fmt.format != HGDT_FORMAT_DIA || fmt.ndiag != 15)
+    return false;
+  std::vector<DirectInst> rec(g.ninst());
+  for (int in = 0; in < g.ninst(); ++in) {
+    const Lattice& L = g.lattice[in];
+    const int64_t nvx = L.n[0] + 1, nvy = L.n[1] + 1, S = nvx * nvy;
+    const int64_t up[7] = {1, nvx, nvx + 1, S, S + 1, S + nvx, S + nvx + 1};
+    std::vector<int64_t> expect;
+    for (int u = 6; u >= 0; --u)
+      expect.push_back(-up[u]);
+    expect.push_back(0);
+    for (int u = 0; u < 7; ++u)
+      expect.push_back(up[u]);
+    if (C.dia_offs[in] != expect || C.dia_k0_host[in] != 7 ||
+        s.inst_dof[in + 1] - s.inst_dof[in] != S * (L.n[2] + 1) || g.inst_elem[in + 1] - g.inst_elem[in] != 6 * L.n[0] * L.n[1] * L.n[2])
+      return false;
+    DirectInst& r = rec[in];
+    r = DirectInst{};
+    r.dof0 = s.inst_dof[in], r.elem0 = g.inst_elem[in];
+    r.nx = int32_t(L.n[0]), r.ny = int32_t(L.n[1]), r.nz = int32_t(L.n[2]);
+    const double vol = L.h[0] * L.h[1] * L.h[2] / 6.;
+    for (int a = 0; a < 3; ++a)
+      r.wT[a] = vol / (L.h[a] * L.h[a]);
+    r.quarter_vol = vol / 4.;
+  }
+  d_direct_inst.upload(rec);
+  direct_state = 1;
+  return true;
+}
+
+void Operator::assemble_direct(hipStream_t stream)
+{
+  SolverFormat& F = fmt;
+  SolverCache& C = cache_of(*this);
+  const int ninst = int(space->inst_dof.size()) - 1;
+  F.d_dh.allocate(std::max<int64_t>(n, 1));
+  F.d_dsq.allocate(std::max<int64_t>(n, 1));
+  const int64_t nflags = 2 + int64_t(ninst) * F.ndiag;
+  flag_buffers(C, nflags);
+  const int B = 256;
+  hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(nflags, B)), dim3(B), 0, stream, int(nflags), C.flags.ptr, 0);
+  DirectParams P{};
+  P.n = n, P.ninst = ninst, P.inst = d_direct_inst.ptr;
+  P.nforms = int(eforms.size());
+  for (int f = 0; f < P.nforms; ++f)
+    P.field[f] = eforms[f].field, P.value[f] = eforms[f].value, P.scaling[f] = eforms[f].scaling;
+  P.nsrc = int(sources.size());
+  for (int k = 0; k < P.nsrc; ++k) {
+    P.src_data[k] = sources[k].kind == HGDT_SOURCE_PER_ELEMENT ? sources[k].data : nullptr;
+    P.src_value[k] = sources[k].value;
+  }
+  P.mask = dirichlet ? d_dof_mask.ptr : nullptr;
+  P.ndiag = F.ndiag, P.k0 = 7;
+  P.dia = F.d_dia.ptr + int64_t(7) * n;
+  P.rhs = d_rhs.ptr, P.dh = F.d_dh.ptr, P.dsq = F.d_dsq.ptr;
+  P.flags = C.flags.ptr, P.nz = C.flags.ptr + 2;
+  hipLaunchKernelGGL(k_direct_p1_tets, dim3(blocks_for(n, B)), dim3(B), 0, stream, P);
+  HGDT_HIP(hipGetLastError());
+  F.valid = false;
+  direct_storage = true;
+  values_stale = true;
+}
+
+int Operator::solver_upper_diagonals() const
+{
+  if (!fmt.valid || fmt.format != HGDT_FORMAT_DIA || !fmt.symmetric || !solver_cache)
+    return -1;
+  return solver_cache->s_nu;
+}
+
+void Operator::materialize_values(hipStream_t stream)
+{
+  if (!values_stale)
+    return;
+  if (!direct_storage || fmt.format != HGDT_FORMAT_DIA)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "CSR values requested from a storage that does not hold the assembly");
+  SolverCache& C = cache_of(*this);
+  hipLaunchKernelGGL(k_values_from_upper, dim3(blocks_for(nnz, 256)), dim3(256), 0, stream, nnz, n, fmt.ndiag,
+                     fmt.d_dia_pos.ptr, fmt.d_dia_offsets.ptr, C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr,
+                     fmt.d_dia.ptr, d_values.ptr);
+  HGDT_HIP(hipGetLastError());
+  HGDT_HIP(hipStreamSynchronize(stream)); // the values may be read on another stream or by a copy next
+  values_stale = false;
+}
+
 void Operator::prepare_solver(hipStream_t stream)
 {
   prepare_storage(stream);
@@ -1761,9 +2132,9 @@ void Operator::prepare_storage(hipStream_t stream)
     return;
   F.d_dh.allocate(std::max<int64_t>(n, 1));
   F.d_dsq.allocate(std::max<int64_t>(n, 1));
-  C.flags.allocate(2);
-  if (!C.host_flags)
-    HGDT_HIP(hipHostMalloc((void**)&C.host_flags, 2 * sizeof(int)));
+  const int ninst = int(space->inst_dof.size()) - 1;
+  const int64_t nflags = 2 + (F.format == HGDT_FORMAT_DIA ? int64_t(ninst) * F.ndiag : 0);
+  flag_buffers(C, nflags);
   if (F.format == HGDT_FORMAT_MATRIX_FREE) {
     // Jacobi scaling from the diagonal of the operator, recomputed from the coefficients
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
@@ -1812,29 +2183,45 @@ void Operator::prepare_storage(hipStream_t stream)
   const bool dia = F.format == HGDT_FORMAT_DIA;
   double* dst = dia ? F.d_dia.ptr : F.d_sell_vals.ptr;
   const int64_t total = dia ? int64_t(F.ndiag) * n : int64_t(F.d_sell_vals.count);
-  hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
-  hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
-                     dia ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, 2, C.flags.ptr, 0);
-  if (dia)
-    hipLaunchKernelGGL(k_dia_prepare, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, F.ndiag, F.d_dia_offsets.ptr,
-                       C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dh.ptr,
-                       F.d_dsq.ptr, C.flags.ptr);
-  else
-    hipLaunchKernelGGL(k_jacobi, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
-                       d_values.ptr, F.d_dh.ptr, F.d_dsq.ptr, C.flags.ptr);
-  HGDT_HIP(hipGetLastError());
-  HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (dia && direct_storage) {
+    // the structured assembly wrote the diagonal and the upper diagonals, D^{-1/2}, D^{1/2} and the flags
+    // (symmetric by construction); nothing to convert
+  } else {
+    if (values_stale)
+      materialize_values(stream);
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
+    hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
+                       dia ? F.d_dia_pos.ptr : F.d_sell_pos.ptr, dst);
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(nflags, B)), dim3(B), 0, stream, int(nflags), C.flags.ptr, 0);
+    if (dia)
+      hipLaunchKernelGGL(k_dia_prepare, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, F.ndiag, F.d_dia_offsets.ptr,
+                         C.dia_k0.ptr, C.dia_mirror.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dh.ptr,
+                         F.d_dsq.ptr, C.flags.ptr, C.flags.ptr + 2);
+    else
+      hipLaunchKernelGGL(k_jacobi, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
+                         d_values.ptr, F.d_dh.ptr, F.d_dsq.ptr, C.flags.ptr);
+    HGDT_HIP(hipGetLastError());
+  }
+  HGDT_HIP(hipMemcpyAsync(C.host_flags, C.flags.ptr, nflags * sizeof(int), hipMemcpyDeviceToHost, stream));
   HGDT_HIP(hipStreamSynchronize(stream));
   F.diag_positive = (C.host_flags[0] == 0);
   const bool sym = dia && C.host_flags[1] == 0 && allow_symmetric;
-  // the scaled matrix for the CG solver, in the same layout
-  F.d_scaled.allocate(std::max<int64_t>(total, 1));
-  if (dia) {
+  if (dia && !sym && direct_storage)
+    HGDT_THROW(HGDT_ERR_OPERATOR, "structured assembly into symmetric storage met a non-symmetric operator");
+  // the scaled matrix for the CG solver: symmetric DIA as its strict upper diagonals with a nonzero entry,
+  // otherwise in the layout of the storage
+  if (dia && sym) {
+    compact_upper(*this, C.host_flags + 2);
+    F.d_scaled.allocate(std::max<int64_t>(int64_t(C.s_nu) * n, 1));
+    hipLaunchKernelGGL(k_scale_upper, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, C.s_nu, C.s_off.ptr,
+                       C.s_kidx.ptr, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr, F.d_dh.ptr, F.d_scaled.ptr);
+  } else if (dia) {
+    F.d_scaled.allocate(std::max<int64_t>(total, 1));
     hipLaunchKernelGGL(k_scale_dia, dim3(blocks_for(n, B), unsigned(F.ndiag)), dim3(B), 0, stream, n, F.ndiag,
-                       F.d_dia_offsets.ptr, C.dia_k0.ptr, sym ? 0 : 1, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr,
+                       F.d_dia_offsets.ptr, C.dia_k0.ptr, 1, C.row_inst.ptr, F.d_inst_row0.ptr, F.d_dia.ptr,
                        F.d_dh.ptr, F.d_scaled.ptr);
   } else {
+    F.d_scaled.allocate(std::max<int64_t>(total, 1));
     hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, F.d_scaled.ptr, 0.);
     hipLaunchKernelGGL(k_scale_values, dim3(blocks_for(n, B)), dim3(B), 0, stream, n, d_row_ptr.ptr, d_cols.ptr,
                        d_values.ptr, F.d_sell_pos.ptr, F.d_dh.ptr, F.d_scaled.ptr);

@@ -192,6 +192,10 @@ SIGNATURES = {
     "hgdt_operator_instance_true_residuals": (ctypes.c_int, [_vp, _pdbl]),
     "hgdt_space_error_norms_noalloc": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _dbl, _pdbl, _vp]),
     "hgdt_noalloc_scratch_bytes": (ctypes.c_int, [_pi64]),
+    "hgdt_operator_assembly_info": (ctypes.c_int, [_vp, _pi32, _pi32]),
+    "hgdt_grid_boundary_intersections": (ctypes.c_int, [_vp, _pdbl, _pdbl]),
+    "hgdt_operator_set_boundary_filter": (ctypes.c_int, [_vp, _vp, _i64]),
+    "hgdt_operator_set_dirichlet_faces": (ctypes.c_int, [_vp, _vp, _i64]),
     "hgdt_operator_adjoint_log_diffusion_gradient": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "hgdt_operator_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "hgdt_operator_get_launch": (ctypes.c_int, [_vp, _i32, _pi32, _pi32]),

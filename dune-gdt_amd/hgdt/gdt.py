@@ -31,6 +31,8 @@ __all__ = [
     "LocalIPDGBoundaryPenalty", "LocalElementIntegralBilinearForm", "LocalCouplingIntersectionIntegralBilinearForm",
     "LocalIntersectionIntegralBilinearForm", "LocalElementIntegralFunctional", "VectorFunctional",
     "make_vector_functional", "DirichletConstraints", "make_dirichlet_constraints", "AllDirichletBoundaryInfo",
+    "AllNeumannBoundaryInfo", "NormalBasedBoundaryInfo", "BoundaryInfo", "DirichletBoundary", "NeumannBoundary",
+    "NoBoundary",
     "ApplyOn", "Stencil", "MatrixOperator", "make_matrix_operator", "CsrMatrix", "GenericFunction",
     "DiscreteFunction", "make_discrete_function", "BilinearForm", "make_bilinear_form",
     "LocalIntersectionIntegralFunctional", "LocalIntersectionProductIntegrand",
@@ -73,6 +75,74 @@ class Stencil(enum.IntEnum):
     element_and_intersection = L.STENCIL_ELEMENT_AND_INTERSECTION
 
 
+class NoBoundary:
+    """XT::Grid::NoBoundary"""
+    id = 0
+
+
+class DirichletBoundary:
+    """XT::Grid::DirichletBoundary"""
+    id = 1
+
+
+class NeumannBoundary:
+    """XT::Grid::NeumannBoundary"""
+    id = 2
+
+
+def _type_id(t):
+    return (t if isinstance(t, type) else type(t)).id
+
+
+class BoundaryInfo:
+    """XT::Grid::BoundaryInfo: the boundary type of every boundary intersection.  type_ids(grid) gives one id per
+    boundary intersection in walk order (elements ascending, their faces in reference order)."""
+
+    def type_ids(self, grid):
+        raise NotImplementedError
+
+    def mask(self, grid, boundary_type):
+        """one byte per boundary intersection: 1 where type(intersection) == boundary_type"""
+        return (self.type_ids(grid) == _type_id(boundary_type)).astype(np.uint8)
+
+
+class AllDirichletBoundaryInfo(BoundaryInfo):
+    """XT::Grid::AllDirichletBoundaryInfo: every boundary intersection is a Dirichlet one."""
+
+    def type_ids(self, grid):
+        return np.full(grid.num_boundary_intersections, DirichletBoundary.id, np.int8)
+
+
+class AllNeumannBoundaryInfo(BoundaryInfo):
+    """XT::Grid::AllNeumannBoundaryInfo: every boundary intersection is a Neumann one."""
+
+    def type_ids(self, grid):
+        return np.full(grid.num_boundary_intersections, NeumannBoundary.id, np.int8)
+
+
+class NormalBasedBoundaryInfo(BoundaryInfo):
+    """XT::Grid::NormalBasedBoundaryInfo: the type of an intersection is the one registered for its outward unit
+    normal (within tol), else the default (as used in dune/gdt/test/interpolations/boundary.hh:79-81)."""
+
+    def __init__(self, tol=1e-10, default_boundary_type=None):
+        self.tol = tol
+        self.default = NoBoundary if default_boundary_type is None else default_boundary_type
+        self.normals = []
+
+    def register_new_normal(self, normal, boundary_type):
+        n = np.asarray(normal, np.float64)
+        self.normals.append((n / np.linalg.norm(n), boundary_type))
+        return self
+
+    def type_ids(self, grid):
+        _, normals = grid.boundary_intersections()
+        ids = np.full(len(normals), _type_id(self.default), np.int8)
+        for n, t in self.normals:
+            d = n[: normals.shape[1]]
+            ids[np.linalg.norm(normals - d, axis=1) < self.tol] = _type_id(t)
+        return ids
+
+
 class ApplyOn:
     """Element / intersection filters of XT::Grid::ApplyOn used on this path."""
 
@@ -86,12 +156,18 @@ class ApplyOn:
         pass
 
     class CustomBoundaryIntersections:
+        """the boundary intersections with boundary_info.type(intersection) == boundary_type"""
+
         def __init__(self, boundary_info=None, boundary_type=None):
             self.boundary_info = boundary_info
+            self.boundary_type = DirichletBoundary if boundary_type is None else boundary_type
 
-
-class AllDirichletBoundaryInfo:
-    """XT::Grid::AllDirichletBoundaryInfo: every boundary intersection is a Dirichlet one."""
+        def mask(self, grid):
+            """None when every boundary intersection qualifies, else one byte per boundary intersection"""
+            if self.boundary_info is None:
+                return None
+            m = self.boundary_info.mask(grid, self.boundary_type)
+            return None if m.all() else m
 
 
 # ---------------------------------------------------------------------------
@@ -153,6 +229,13 @@ class Grid:
         out = np.empty(self.num_instances + 1, np.int64)
         L.call("hgdt_grid_instance_elements", self._h, out.ctypes.data_as(L._pi64))
         return out
+
+    def boundary_intersections(self):
+        """(centres, outward unit normals) of the boundary intersections in walk order, each [nb, dim]"""
+        nb, d = self.num_boundary_intersections, self.dimension
+        c, n = np.empty((nb, d)), np.empty((nb, d))
+        L.call("hgdt_grid_boundary_intersections", self._h, c.ctypes.data_as(L._pdbl), n.ctypes.data_as(L._pdbl))
+        return c, n
 
     def boundary_quadrature_points(self, order):
         """Physical coordinates of the line-rule points on the boundary intersections in walk order: (nb, nq, dim)."""
@@ -448,8 +531,9 @@ class VectorFunctional:
 
     def append(self, local_functional, param=None, filter=None):
         if isinstance(local_functional, LocalIntersectionIntegralFunctional) and filter is not None and not isinstance(
-                filter, ApplyOn.CustomBoundaryIntersections):
-            raise L.not_supported(9, "intersection functionals are applied on the (Dirichlet) boundary")
+                filter, (ApplyOn.CustomBoundaryIntersections, ApplyOn.BoundaryIntersections)):
+            raise L.not_supported(9, "intersection functionals are applied on boundary intersections")
+        local_functional._filter = filter
         self.local_functionals.append(local_functional)
         return self
 
@@ -467,10 +551,13 @@ def make_vector_functional(space):
 
 
 class DirichletConstraints:
-    """DirichletConstraints (tools/dirichlet-constraints.hh:47-211), applied inside the operator's assembly."""
+    """DirichletConstraints (tools/dirichlet-constraints.hh:47-211), applied inside the operator's assembly: the DoFs
+    on the intersections with boundary_info.type(intersection) == DirichletBoundary() (every boundary intersection
+    without a boundary info)."""
 
     def __init__(self, space, boundary_info=None):
         self.space = space
+        self.boundary_info = boundary_info
         self._op = None
 
     @property
@@ -547,8 +634,12 @@ class MatrixOperator:
             for part in _parts(what.integrand):
                 self._append_coupling(part, what.over_integrate, _scaling(param))
         elif isinstance(what, LocalIntersectionIntegralBilinearForm):
-            for part in _parts(what.integrand):
-                self._append_boundary(part, what.over_integrate, _scaling(param))
+            self._set_boundary_filter(filter)
+            try:
+                for part in _parts(what.integrand):
+                    self._append_boundary(part, what.over_integrate, _scaling(param))
+            finally:
+                self._set_boundary_filter(None)
         elif isinstance(what, VectorFunctional):
             if what.space is not self.space:
                 raise L.not_supported(9, "the functional must live on the operator's space")
@@ -556,6 +647,10 @@ class MatrixOperator:
                 self._append_source(lf)
             what._op = self
         elif isinstance(what, DirichletConstraints):
+            mask = (None if what.boundary_info is None
+                    else ApplyOn.CustomBoundaryIntersections(what.boundary_info, DirichletBoundary).mask(self.grid))
+            L.call("hgdt_operator_set_dirichlet_faces", self._h,
+                   None if mask is None else ctypes.c_void_p(mask.ctypes.data), 0 if mask is None else len(mask))
             L.call("hgdt_operator_set_dirichlet", self._h, 1)
             what._op = self
         elif isinstance(what, tuple):
@@ -567,6 +662,15 @@ class MatrixOperator:
 
     def __iadd__(self, what):
         return self.append(what)
+
+    def _set_boundary_filter(self, filter):
+        """ApplyOn filter of the boundary forms / functionals appended next (None or BoundaryIntersections: all)"""
+        mask = filter.mask(self.grid) if isinstance(filter, ApplyOn.CustomBoundaryIntersections) else None
+        if filter is not None and not isinstance(filter, (ApplyOn.CustomBoundaryIntersections,
+                                                          ApplyOn.BoundaryIntersections)):
+            raise L.not_supported(9, "boundary forms are applied on boundary intersections")
+        L.call("hgdt_operator_set_boundary_filter", self._h,
+               None if mask is None else ctypes.c_void_p(mask.ctypes.data), 0 if mask is None else len(mask))
 
     def _field(self, gf):
         ptr, val = gf.args(self.grid)
@@ -630,7 +734,11 @@ class MatrixOperator:
 
     def _append_source(self, lf):
         if isinstance(lf, LocalIntersectionIntegralFunctional):
-            return self._append_boundary_source(lf)
+            self._set_boundary_filter(getattr(lf, "_filter", None))
+            try:
+                return self._append_boundary_source(lf)
+            finally:
+                self._set_boundary_filter(None)
         f = lf.integrand.function
         over = lf.over_integrate
         if isinstance(f, QuadraturePointFunction):
@@ -785,6 +893,14 @@ class MatrixOperator:
                            L.FORMAT_BLOCK: "block4"}[f.value],
                 "spmv_bytes": int(b.value),
                 "num_diagonals": int(d.value)}
+
+    def assembly_info(self):
+        """{"structured": the last assemble() wrote the solver's storage directly (P1 Kuhn tetrahedra on
+        axis-parallel lattices, Laplace forms), "solver_upper_diagonals": strict upper diagonals kept in the
+        solver's scaled symmetric DIA copy (-1: other storage, or not prepared since the assembly)}"""
+        st, su = ctypes.c_int32(), ctypes.c_int32()
+        L.call("hgdt_operator_assembly_info", self._h, ctypes.byref(st), ctypes.byref(su))
+        return {"structured": bool(st.value), "solver_upper_diagonals": int(su.value)}
 
     def set_launch(self, kernel, block_size, items_per_thread=1):
         L.call("hgdt_operator_set_launch", self._h, int(kernel), int(block_size), int(items_per_thread))

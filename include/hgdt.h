@@ -120,6 +120,12 @@ hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_
  * host_out[num_boundary_intersections x num_points x dim] (2d grids); host_out = NULL queries the size. */
 hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
 
+/* centres [num_boundary_intersections x dim] and outward unit normals [num_boundary_intersections x dim] of the
+ * boundary intersections in walk order (elements ascending, their faces in reference order): what a boundary
+ * info (XT::Grid::BoundaryInfo::type(intersection), e.g. NormalBasedBoundaryInfo) decides on.  Either output may
+ * be NULL. */
+hgdt_status hgdt_grid_boundary_intersections(hgdt_grid grid, double* centers, double* normals);
+
 /* ------------------------------------------------------------------------- */
 /* spaces                                                                    */
 /* ------------------------------------------------------------------------- */
@@ -244,6 +250,16 @@ hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double
 hgdt_status hgdt_operator_append_boundary_source(hgdt_operator op, int kind, double param, const double* coeff_field,
                                                  double coeff_value, const double* data, double value, int32_t g_order,
                                                  int32_t over_integrate);
+/* ApplyOn::CustomBoundaryIntersections(boundary_info, type) for the boundary bilinear forms and boundary
+ * functionals appended from now on (test/stationary-eocstudies/diffusion-ipdg.hh:302-311): one byte per boundary
+ * intersection in walk order (see hgdt_grid_boundary_intersections), nonzero where boundary_info.type(intersection)
+ * is the type; NULL: every boundary intersection (the default).  The mask is copied. */
+hgdt_status hgdt_operator_set_boundary_filter(hgdt_operator op, const uint8_t* mask, int64_t count);
+/* the Dirichlet intersections of the constraints, boundary_info.type(intersection) == DirichletBoundary()
+ * (tools/dirichlet-constraints.hh:85-110): DoFs on those intersections and their subentities are constrained; one
+ * byte per boundary intersection in walk order; NULL: every boundary intersection (AllDirichletBoundaryInfo, the
+ * default).  The mask is copied. */
+hgdt_status hgdt_operator_set_dirichlet_faces(hgdt_operator op, const uint8_t* mask, int64_t count);
 /* DirichletConstraints(space, AllDirichletBoundaryInfo) applied after assembly with
  * ensure_symmetry (tools/dirichlet-constraints.hh:85-184): unit rows and columns, rhs entry 0 */
 hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled);
@@ -260,6 +276,15 @@ hgdt_status hgdt_operator_rhs_device(hgdt_operator op, double** rhs);
 /* synchronous copies to host memory */
 hgdt_status hgdt_operator_copy_values(hgdt_operator op, double* host_out);
 hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
+
+/* How the last assemble() ran and what the solver holds (no reference counterpart; for tests and reports):
+ * *structured = 1 when the structured assembly wrote the solver's storage directly (P1 on the Kuhn tetrahedra of
+ * axis-parallel lattices with Laplace element forms and constant or per-element sources: one pass over the
+ * DoFs, the CSR values computed from that storage only when they are asked for), else 0;
+ * *solver_upper_diagonals = strict upper diagonals in the solver's scaled copy of a symmetric DIA matrix (the
+ * diagonals whose entries are all exactly zero are left out: they add only 0 * p), -1 for other storages or
+ * before the first apply / apply_inverse after an assembly.  Either pointer may be NULL. */
+hgdt_status hgdt_operator_assembly_info(hgdt_operator op, int32_t* structured, int32_t* solver_upper_diagonals);
 
 /* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: element blocks for a
  * discontinuous space with 4 DoFs per element, else DIA when every instance has at most 32 distinct

@@ -263,6 +263,19 @@ public:
     check(hgdt_grid_quadrature_points(handle(), order, out.data(), &num_points));
     return out;
   }
+  int64_t num_boundary_intersections() const
+  {
+    int64_t info[9];
+    check(hgdt_grid_info(handle(), info));
+    return info[6];
+  }
+  // centres and outward unit normals of the boundary intersections in walk order, [num][dim] each
+  void boundary_intersections(std::vector<double>& centers, std::vector<double>& normals) const
+  {
+    const size_t nb = size_t(num_boundary_intersections());
+    centers.resize(nb * size_t(dim_)), normals.resize(nb * size_t(dim_));
+    check(hgdt_grid_boundary_intersections(handle(), centers.data(), normals.data()));
+  }
   hgdt_grid handle() const
   {
     return h_.get();
@@ -689,16 +702,122 @@ struct LocalIntersectionIntegralFunctional
   int over_integrate;
 };
 
-struct AllDirichletBoundaryInfo
-{};
+// XT::Grid boundary types and boundary infos: the type of every boundary intersection of a grid, in walk order
+struct NoBoundary
+{
+  static constexpr int id = 0;
+};
+struct DirichletBoundary
+{
+  static constexpr int id = 1;
+};
+struct NeumannBoundary
+{
+  static constexpr int id = 2;
+};
+class BoundaryInfo
+{
+public:
+  virtual ~BoundaryInfo() = default;
+  virtual std::vector<int8_t> type_ids(const Grid& grid) const = 0;
+  // one byte per boundary intersection: 1 where type(intersection) is the type
+  std::vector<uint8_t> mask(const Grid& grid, int type_id) const
+  {
+    const auto ids = type_ids(grid);
+    std::vector<uint8_t> m(ids.size());
+    for (size_t i = 0; i < ids.size(); ++i)
+      m[i] = ids[i] == type_id ? 1 : 0;
+    return m;
+  }
+};
+struct AllDirichletBoundaryInfo : BoundaryInfo
+{
+  std::vector<int8_t> type_ids(const Grid& grid) const override
+  {
+    return std::vector<int8_t>(size_t(grid.num_boundary_intersections()), int8_t(DirichletBoundary::id));
+  }
+};
+struct AllNeumannBoundaryInfo : BoundaryInfo
+{
+  std::vector<int8_t> type_ids(const Grid& grid) const override
+  {
+    return std::vector<int8_t>(size_t(grid.num_boundary_intersections()), int8_t(NeumannBoundary::id));
+  }
+};
+// XT::Grid::NormalBasedBoundaryInfo: the type registered for an intersection's outward unit normal (within tol),
+// else the default
+class NormalBasedBoundaryInfo : public BoundaryInfo
+{
+public:
+  explicit NormalBasedBoundaryInfo(double tol = 1e-10, int default_type_id = NoBoundary::id)
+    : tol_(tol)
+    , default_(default_type_id)
+  {}
+  template <class Type>
+  NormalBasedBoundaryInfo& register_new_normal(std::vector<double> normal, const Type&)
+  {
+    double len = 0.;
+    for (double v : normal)
+      len += v * v;
+    for (double& v : normal)
+      v /= std::sqrt(len);
+    normals_.push_back({normal, Type::id});
+    return *this;
+  }
+  std::vector<int8_t> type_ids(const Grid& grid) const override
+  {
+    std::vector<double> c, n;
+    grid.boundary_intersections(c, n);
+    const int d = grid.dimension();
+    std::vector<int8_t> ids(n.size() / size_t(d), int8_t(default_));
+    for (size_t b = 0; b < ids.size(); ++b)
+      for (const auto& reg : normals_) {
+        double dist = 0.;
+        for (int a = 0; a < d; ++a) {
+          const double r = a < int(reg.first.size()) ? reg.first[a] : 0.;
+          dist += (n[b * d + a] - r) * (n[b * d + a] - r);
+        }
+        if (std::sqrt(dist) < tol_)
+          ids[b] = int8_t(reg.second);
+      }
+    return ids;
+  }
+
+private:
+  double tol_;
+  int default_;
+  std::vector<std::pair<std::vector<double>, int>> normals_;
+};
+
 namespace ApplyOn {
 struct AllElements
 {};
 struct InnerIntersectionsOnce
 {};
+struct BoundaryIntersections
+{};
+// the boundary intersections with boundary_info.type(intersection) == type
 struct CustomBoundaryIntersections
 {
-  CustomBoundaryIntersections(const AllDirichletBoundaryInfo& = {}) {}
+  CustomBoundaryIntersections() = default;
+  template <class Info, class Type = DirichletBoundary>
+  CustomBoundaryIntersections(const Info& info, const Type& = {})
+    : boundary_info(std::make_shared<Info>(info))
+    , type_id(Type::id)
+  {}
+  // one byte per boundary intersection, or empty for every boundary intersection
+  std::vector<uint8_t> mask(const Grid& grid) const
+  {
+    if (!boundary_info)
+      return {};
+    auto m = boundary_info->mask(grid, type_id);
+    for (uint8_t v : m)
+      if (!v)
+        return m;
+    return {};
+  }
+  std::shared_ptr<const BoundaryInfo> boundary_info;
+  int type_id = DirichletBoundary::id;
 };
 } // namespace ApplyOn
 
@@ -753,12 +872,23 @@ public:
   {
     return append(local_functional);
   }
-  // on the boundary intersections (ApplyOn::CustomBoundaryIntersections)
-  template <class Filter>
-  VectorFunctional& append(const LocalIntersectionIntegralFunctional& local_functional, const Parameter&, const Filter&)
+  // on the boundary intersections the filter selects (ApplyOn::CustomBoundaryIntersections; every boundary
+  // intersection with ApplyOn::BoundaryIntersections)
+  VectorFunctional& append(const LocalIntersectionIntegralFunctional& local_functional, const Parameter&,
+                           const ApplyOn::CustomBoundaryIntersections& filter)
   {
     boundary_functionals_.push_back(local_functional);
+    boundary_filters_.push_back(filter);
     return *this;
+  }
+  VectorFunctional& append(const LocalIntersectionIntegralFunctional& local_functional, const Parameter& param,
+                           const ApplyOn::BoundaryIntersections&)
+  {
+    return append(local_functional, param, ApplyOn::CustomBoundaryIntersections{});
+  }
+  const std::vector<ApplyOn::CustomBoundaryIntersections>& boundary_filters() const
+  {
+    return boundary_filters_;
   }
   const std::vector<LocalElementIntegralFunctional>& local_functionals() const
   {
@@ -777,22 +907,36 @@ private:
   SpaceInterface space_;
   std::vector<LocalElementIntegralFunctional> functionals_;
   std::vector<LocalIntersectionIntegralFunctional> boundary_functionals_;
+  std::vector<ApplyOn::CustomBoundaryIntersections> boundary_filters_;
 };
 inline VectorFunctional make_vector_functional(const SpaceInterface& space)
 {
   return VectorFunctional(space);
 }
 
+// DirichletConstraints(boundary_info, space), tools/dirichlet-constraints.hh:69-110: the DoFs on the intersections
+// with boundary_info.type(intersection) == DirichletBoundary()
 struct DirichletConstraints
 {
   explicit DirichletConstraints(const SpaceInterface& space_)
     : space(space_)
   {}
+  template <class Info>
+  DirichletConstraints(const SpaceInterface& space_, const Info& info)
+    : space(space_)
+    , faces(info, DirichletBoundary())
+  {}
   SpaceInterface space;
+  ApplyOn::CustomBoundaryIntersections faces;
 };
-inline DirichletConstraints make_dirichlet_constraints(const SpaceInterface& space, const AllDirichletBoundaryInfo& = {})
+inline DirichletConstraints make_dirichlet_constraints(const SpaceInterface& space)
 {
   return DirichletConstraints(space);
+}
+template <class Info>
+DirichletConstraints make_dirichlet_constraints(const SpaceInterface& space, const Info& boundary_info)
+{
+  return DirichletConstraints(space, boundary_info);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -899,8 +1043,14 @@ public:
   }
   // append(LocalIntersectionBilinearFormInterface, param, filter), matrix-based.hh:395-408
   MatrixOperator& append(const LocalIntersectionIntegralBilinearForm& form, const Parameter& param,
-                         const ApplyOn::CustomBoundaryIntersections&)
+                         const ApplyOn::BoundaryIntersections&)
   {
+    return append(form, param, ApplyOn::CustomBoundaryIntersections{});
+  }
+  MatrixOperator& append(const LocalIntersectionIntegralBilinearForm& form, const Parameter& param,
+                         const ApplyOn::CustomBoundaryIntersections& filter)
+  {
+    const BoundaryFilter scope(*this, filter);
     const double scaling = param.get("matrixoperator.scaling", 1.);
     for (const auto& t : form.integrand.terms) {
       keep(t.diffusion), keep(t.weight);
@@ -933,7 +1083,9 @@ public:
                                           in.grid_f.field(), in.grid_f.value(), 0, lf.over_integrate));
       }
     }
-    for (const auto& bf : functional.boundary_functionals()) {
+    for (size_t b = 0; b < functional.boundary_functionals().size(); ++b) {
+      const auto& bf = functional.boundary_functionals()[b];
+      const BoundaryFilter scope(*this, functional.boundary_filters()[b]);
       const auto& t = bf.term;
       keep(t.coeff);
       const int k = space_.order();
@@ -960,8 +1112,10 @@ public:
     }
     return *this;
   }
-  MatrixOperator& append(const DirichletConstraints&)
+  MatrixOperator& append(const DirichletConstraints& constraints)
   {
+    const auto mask = constraints.faces.mask(space_.grid_view());
+    check(hgdt_operator_set_dirichlet_faces(handle(), mask.empty() ? nullptr : mask.data(), int64_t(mask.size())));
     check(hgdt_operator_set_dirichlet(handle(), 1));
     return *this;
   }
@@ -1041,6 +1195,24 @@ public:
   }
 
 private:
+  // the ApplyOn filter of the boundary forms and functionals appended while it lives (restored to every
+  // boundary intersection afterwards)
+  struct BoundaryFilter
+  {
+    BoundaryFilter(const MatrixOperator& op, const ApplyOn::CustomBoundaryIntersections& filter)
+      : h(op.handle())
+    {
+      const auto mask = filter.mask(op.space_.grid_view());
+      check(hgdt_operator_set_boundary_filter(h, mask.empty() ? nullptr : mask.data(), int64_t(mask.size())));
+    }
+    ~BoundaryFilter()
+    {
+      hgdt_operator_set_boundary_filter(h, nullptr, 0);
+    }
+    BoundaryFilter(const BoundaryFilter&) = delete;
+    BoundaryFilter& operator=(const BoundaryFilter&) = delete;
+    hgdt_operator h;
+  };
   void keep(const GridFunction& f)
   {
     if (f.storage())

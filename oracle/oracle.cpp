@@ -663,6 +663,9 @@ struct FaceForm {
   Coefficient kappa, omega;
   int over_integrate;
   double scaling;
+  // boundary forms: the intersections the form is applied on, by boundary intersection in walk order
+  // (ApplyOn::CustomBoundaryIntersections(boundary_info, type)); empty: every boundary intersection
+  std::vector<uint8_t> mask;
 };
 struct Source {
   int kind;
@@ -680,6 +683,7 @@ struct BoundarySource {
   double g_const = 0.;
   int g_order = 0;
   int over_integrate = 0;
+  std::vector<uint8_t> mask; // as FaceForm::mask
 };
 
 // bound local basis: values and physical gradients at a reference point
@@ -746,6 +750,15 @@ struct Operator {
   }
   bool dirichlet = false;
   std::vector<int32_t> dirichlet_dofs;
+  // boundary_info_.type(intersection) == DirichletBoundary() (dirichlet-constraints.hh:93), by boundary
+  // intersection in walk order; empty: every boundary intersection is a Dirichlet intersection
+  std::vector<uint8_t> dirichlet_faces;
+  // filter of the boundary forms and functionals appended next (empty: every boundary intersection)
+  std::vector<uint8_t> pending_mask;
+  bool on_boundary_subset(const std::vector<uint8_t>& mask, int64_t e, int f) const
+  {
+    return mask.empty() || mask[bface_index[e * space->mesh->ref.nfaces + f]] != 0;
+  }
 
   // sparsity pattern, tools/sparsity-pattern.hh:33-50 (element) and :117-144 (element+intersection)
   void build_pattern()
@@ -847,8 +860,10 @@ struct Operator {
   FaceGeo face_geometry(int64_t e, int f, int64_t o, const Geometry& gin, const Geometry* gout, int order) const
   {
     const Mesh& m = *space->mesh;
+    if (m.ref.dim == 3)
+      return face_geometry_3d(f, gin, gout, order);
     if (m.ref.dim != 2)
-      throw Error("oracle face integrals are implemented for 2d grids");
+      throw Error("oracle face integrals are implemented for 2d and 3d grids");
     FaceGeo fg;
     const int a = m.ref.faces[f][0], b = m.ref.faces[f][1];
     double Pa[3], Pb[3];
@@ -884,6 +899,69 @@ struct Operator {
       fg.w.push_back(q.w[qp]);
     }
     (void)e, (void)o;
+    return fg;
+  }
+
+  // 3d faces of affine elements (integrals.hh:226-233, 352-356): a triangle (tetrahedra) or a parallelogram
+  // (hexahedra) X(s, t) = P0 + s (P1 - P0) + t (P2 - P0) over the corners P0, P1, P2 (P3) of reference face f,
+  // integration element |(P1 - P0) x (P2 - P0)|, the simplex rule of the reference triangle or the tensor Gauss
+  // rule of the square; the normal points away from the inside element, the diameter is the largest distance
+  // between two corners (XT::Grid::diameter, the default of ipdg.hh:27-38 for d > 1)
+  FaceGeo face_geometry_3d(int f, const Geometry& gin, const Geometry* gout, int order) const
+  {
+    const Mesh& m = *space->mesh;
+    const auto& fv = m.ref.faces[f];
+    const int nc = int(fv.size());
+    FaceGeo fg;
+    double P[4][3], cen[3] = {0, 0, 0};
+    for (int k = 0; k < nc; ++k)
+      gin.global(m.ref.verts[fv[k]].data(), P[k]);
+    for (int v = 0; v < m.ref.nverts; ++v) {
+      double X[3];
+      gin.global(m.ref.verts[v].data(), X);
+      for (int c = 0; c < 3; ++c)
+        cen[c] += X[c] / m.ref.nverts;
+    }
+    double E1[3], E2[3], N[3];
+    for (int c = 0; c < 3; ++c)
+      E1[c] = P[1][c] - P[0][c], E2[c] = P[2][c] - P[0][c];
+    N[0] = E1[1] * E2[2] - E1[2] * E2[1];
+    N[1] = E1[2] * E2[0] - E1[0] * E2[2];
+    N[2] = E1[0] * E2[1] - E1[1] * E2[0];
+    const double area = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    fg.integration_element = area;
+    double side = 0.;
+    for (int c = 0; c < 3; ++c)
+      side += N[c] * (P[0][c] - cen[c]);
+    for (int c = 0; c < 3; ++c)
+      fg.normal[c] = (side < 0. ? -N[c] : N[c]) / area + 0.;
+    double diam = 0.;
+    for (int a = 0; a < nc; ++a)
+      for (int b = a + 1; b < nc; ++b) {
+        double d2 = 0.;
+        for (int c = 0; c < 3; ++c)
+          d2 += (P[a][c] - P[b][c]) * (P[a][c] - P[b][c]);
+        diam = std::max(diam, std::sqrt(d2));
+      }
+    fg.diameter = diam;
+    const bool simplex = m.ref.simplex;
+    const Quadrature q = simplex ? simplex_rule(2, order) : cube_rule(2, order);
+    for (size_t qp = 0; qp < q.w.size(); ++qp) {
+      const double s = q.x[qp][0], t = q.x[qp][1];
+      std::array<double, 3> xh = {0, 0, 0};
+      for (int c = 0; c < 3; ++c)
+        xh[c] = simplex ? (1. - s - t) * m.ref.verts[fv[0]][c] + s * m.ref.verts[fv[1]][c] + t * m.ref.verts[fv[2]][c]
+                        : (1. - s) * (1. - t) * m.ref.verts[fv[0]][c] + s * (1. - t) * m.ref.verts[fv[1]][c] +
+                              (1. - s) * t * m.ref.verts[fv[2]][c] + s * t * m.ref.verts[fv[3]][c];
+      fg.xin.push_back(xh);
+      if (gout) {
+        double X[3], xo[3] = {0, 0, 0};
+        gin.global(xh.data(), X);
+        gout->local(X, xo);
+        fg.xout.push_back({xo[0], xo[1], xo[2]});
+      }
+      fg.w.push_back(q.w[qp]);
+    }
     return fg;
   }
 
@@ -1110,6 +1188,8 @@ struct Operator {
               continue;
             const int64_t bi = bface_index[e * nf + f];
             for (const auto& S : bsources) {
+              if (!on_boundary_subset(S.mask, e, f))
+                continue;
               const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, boundary_source_order(S));
               boundary_source_local(S, e, bi, fg, basis, lvec);
               for (int i = 0; i < ls; ++i) {
@@ -1149,6 +1229,8 @@ struct Operator {
               }
             } else if (o < 0 && !bforms.empty()) {
               for (const auto& F : bforms) {
+                if (!on_boundary_subset(F.mask, e, f))
+                  continue;
                 const int order = 0 + space->fe.order + space->fe.order + F.over_integrate;
                 const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, order);
                 boundary_local(F, e, fg, basis, local);
@@ -1180,7 +1262,7 @@ struct Operator {
     if (space->continuous) {
       for (int64_t e = 0; e < m.ne; ++e)
         for (int f = 0; f < nf; ++f) {
-          if (m.nb[e * nf + f] >= 0)
+          if (m.nb[e * nf + f] >= 0 || !on_boundary_subset(dirichlet_faces, e, f))
             continue;
           const auto& fv = m.ref.faces[f];
           for (int i = 0; i < ls; ++i) {
@@ -1382,13 +1464,15 @@ static int append_face(OrOperator* op, int kind, double param, const double* kap
                        double oc, int over_integrate, double scaling)
 {
   return guarded([&] {
-    FaceForm F{kind, param, {}, {}, over_integrate, scaling};
+    FaceForm F{kind, param, {}, {}, over_integrate, scaling, {}};
     F.kappa.set(kappa, kc, op->space->mesh->ne);
     F.omega.set(omega, oc, op->space->mesh->ne);
     if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY)
       op->cforms.push_back(F);
-    else
+    else {
+      F.mask = op->pending_mask;
       op->bforms.push_back(F);
+    }
   });
 }
 int or_operator_append_inner_coupling(OrOperator* op, double s, const double* kappa, double kc, const double* omega,
@@ -1655,11 +1739,39 @@ int or_operator_append_boundary_source(OrOperator* op, int kind, double param, c
       for (int64_t v : op->bface_index)
         nb += (v >= 0);
       const int order = op->boundary_source_order(S);
-      const int64_t nq = int64_t(line_rule(order).w.size());
+      const int64_t nq = int64_t((m.ref.dim == 3 ? (m.ref.simplex ? simplex_rule(2, order) : cube_rule(2, order))
+                                                 : line_rule(order)).w.size());
       S.g.assign(g, g + nb * nq);
     }
+    S.mask = op->pending_mask;
     op->bsources.push_back(S);
   });
+}
+
+static void boundary_mask(OrOperator* op, const uint8_t* mask, int64_t count, std::vector<uint8_t>& out)
+{
+  op->number_boundary_faces();
+  int64_t nb = 0;
+  for (int64_t v : op->bface_index)
+    nb += (v >= 0);
+  if (!mask) {
+    out.clear();
+    return;
+  }
+  if (count != nb)
+    throw Error("boundary mask length differs from the number of boundary intersections");
+  out.assign(mask, mask + count);
+}
+// ApplyOn::CustomBoundaryIntersections for the boundary forms and functionals appended from now on: one byte
+// per boundary intersection in walk order (nonzero: applied); NULL: every boundary intersection
+int or_operator_set_boundary_filter(OrOperator* op, const uint8_t* mask, int64_t count)
+{
+  return guarded([&] { boundary_mask(op, mask, count, op->pending_mask); });
+}
+// the Dirichlet intersections of the constraints (boundary_info.type == DirichletBoundary); NULL: all
+int or_operator_set_dirichlet_faces(OrOperator* op, const uint8_t* mask, int64_t count)
+{
+  return guarded([&] { boundary_mask(op, mask, count, op->dirichlet_faces); });
 }
 
 // physical coordinates of the face quadrature points of the boundary intersections in walk order,
@@ -1669,8 +1781,39 @@ int or_mesh_boundary_quadrature_points(const OrMesh* mesh, int order, double* ou
   int count = -1;
   guarded([&] {
     const Mesh& m = *mesh;
+    if (m.ref.dim == 3) {
+      // the points of Operator::face_geometry_3d, in walk order
+      const Quadrature q = m.ref.simplex ? simplex_rule(2, order) : cube_rule(2, order);
+      count = int(q.w.size());
+      if (!out)
+        return;
+      const int nf = m.ref.nfaces;
+      int64_t bi = 0;
+      for (int64_t e = 0; e < m.ne; ++e) {
+        const Geometry geo = m.geometry(e);
+        for (int f = 0; f < nf; ++f) {
+          if (m.nb[e * nf + f] >= 0)
+            continue;
+          const auto& fv = m.ref.faces[f];
+          for (int qp = 0; qp < count; ++qp) {
+            const double s = q.x[qp][0], t = q.x[qp][1];
+            double xh[3], X[3];
+            for (int c = 0; c < 3; ++c)
+              xh[c] = m.ref.simplex
+                          ? (1. - s - t) * m.ref.verts[fv[0]][c] + s * m.ref.verts[fv[1]][c] + t * m.ref.verts[fv[2]][c]
+                          : (1. - s) * (1. - t) * m.ref.verts[fv[0]][c] + s * (1. - t) * m.ref.verts[fv[1]][c] +
+                                (1. - s) * t * m.ref.verts[fv[2]][c] + s * t * m.ref.verts[fv[3]][c];
+            geo.global(xh, X);
+            for (int c = 0; c < 3; ++c)
+              out[(bi * count + qp) * 3 + c] = X[c];
+          }
+          ++bi;
+        }
+      }
+      return;
+    }
     if (m.ref.dim != 2)
-      throw Error("boundary quadrature points are implemented for 2d grids");
+      throw Error("boundary quadrature points are implemented for 2d and 3d grids");
     const Quadrature q = line_rule(order);
     count = int(q.w.size());
     if (!out)

@@ -127,6 +127,10 @@ int or_operator_append_boundary_source(OrOperator* op, int kind, double param, c
 /* physical coordinates of the line-rule points on the boundary intersections in walk order, out[nb][npts][2]
  * (NULL: size query); returns npts */
 int or_mesh_boundary_quadrature_points(const OrMesh* m, int order, double* out);
+/* boundary intersections of the boundary forms / functionals appended next, and the Dirichlet intersections of
+ * the constraints: one byte per boundary intersection in walk order (element, then face); NULL: all */
+int or_operator_set_boundary_filter(OrOperator* op, const uint8_t* mask, int64_t count);
+int or_operator_set_dirichlet_faces(OrOperator* op, const uint8_t* mask, int64_t count);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,8 @@ def lib():
             ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, _dp, ctypes.c_double,
                            ctypes.c_int, ctypes.c_int]),
         "or_mesh_boundary_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "or_operator_set_boundary_filter": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int64]),
+        "or_operator_set_dirichlet_faces": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -283,6 +285,21 @@ class Operator:
             gp, gc = dptr(self._keep_g[-1]), 0.0
         self._check(lib().or_operator_append_boundary_source(self.h, kind, param, p, c, gp, gc, g_order,
                                                              over_integrate))
+
+    def _mask(self, fn, mask):
+        if mask is None:
+            self._check(fn(self.h, None, 0))
+            return
+        m = np.ascontiguousarray(mask, np.uint8)
+        self._check(fn(self.h, m.ctypes.data_as(ctypes.c_void_p), len(m)))
+
+    def set_boundary_filter(self, mask):
+        """boundary intersections (walk order) of the boundary forms and functionals appended next; None: all"""
+        self._mask(lib().or_operator_set_boundary_filter, mask)
+
+    def set_dirichlet_faces(self, mask):
+        """the Dirichlet intersections of the constraints (walk order); None: every boundary intersection"""
+        self._mask(lib().or_operator_set_dirichlet_faces, mask)
 
     def set_dirichlet(self, enabled=True):
         lib().or_operator_set_dirichlet(self.h, int(enabled))

@@ -330,3 +330,35 @@ def test_breakdown_with_more_blocks_than_resident(hgdt_mod):
         assert its[i] == j + 1
         assert (rel <= 1e-10) if j > 0 else np.all(xi == 0.)
     assert {1, 3} <= seen
+
+
+def test_coefficient_deleted_before_assemble(hgdt_mod):
+    """python/test/segfault_laplace_integrand.py:1-16: the coefficient function (here a device field), its
+    integrand and its form are dropped before assemble(); the operator keeps what the appended forms reference,
+    so the assembled matrix is the oracle's even after the freed memory of the caller's objects was reused"""
+    import gc
+    H = hgdt_mod
+    print("case coefficient deleted before assemble", flush=True)
+    grid = H.Grid.structured("quad", [0, 0], [1, 1], [12, 10])
+    space = H.ContinuousLagrangeSpace(grid, 1)
+    op = H.MatrixOperator(grid, space, space)
+    kappa = H.fields.lognormal(81, grid.num_elements)
+    func = torch.from_numpy(kappa).cuda()
+    form = H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(H.GridFunction(func)))
+    op += form
+    del func, form
+    gc.collect()
+    # reuse what the caching allocator freed: same-size blocks filled with garbage
+    junk = [torch.full((grid.num_elements,), float("nan"), dtype=torch.float64, device="cuda") for _ in range(8)]
+    torch.cuda.synchronize()
+    op.assemble()
+    m = O.Mesh.structured(O.QUAD, [12, 10], [0, 0], [1, 1])
+    s = O.Space(m, True, 1)
+    oop = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    oop.append_laplace(kappa)
+    oop.assemble(1)
+    rp, _ = oop.pattern()
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), rp)
+    print(f"  matrix max rel err {err:.2e}")
+    assert err <= 1e-12
+    del junk

@@ -619,3 +619,81 @@ def test_q1_hexahedron_closed_form():
     op.assemble(1)
     np.testing.assert_allclose(op.csr() @ np.ones(s.size), 0.0, atol=1e-14)
     assert op.nnz == (3 * 3 + 1) * (3 * 2 + 1) * (3 * 2 + 1)  # tensor product of the 1-d counts 3 n + 1
+
+
+def _face_midpoints(m):
+    """midpoints of the boundary intersections in walk order (one-point line rule)"""
+    return m.boundary_quadrature_points(1)[:, 0, :]
+
+
+def test_mixed_dirichlet_neumann_q1_closed_form():
+    """DirichletConstraints with a boundary info (tools/dirichlet-constraints.hh:85-110): Dirichlet on x = 0, 1,
+    no constraint (homogeneous Neumann) on y = 0, 1.  -u'' = 1 reduces to the 1-d problem, whose P1/Q1 nodal
+    values are exact: u_h = x (1 - x) / 2 at every vertex.  With every boundary intersection Dirichlet the
+    solution vanishes on y = 0, 1 instead, so the face selection is what is checked"""
+    import scipy.sparse.linalg as spla
+
+    n = 8
+    m = O.Mesh.structured(O.QUAD, [n, 6], [0, 0], [1, 0.75])
+    s = O.Space(m, True, 1)
+    mid = _face_midpoints(m)
+    dmask = (np.abs(mid[:, 0]) < 1e-12) | (np.abs(mid[:, 0] - 1) < 1e-12)
+    assert dmask.sum() == 2 * 6
+    op = O.Operator(s, O.STENCIL_ELEMENT)
+    op.append_laplace(1.0)
+    op.append_source(1.0)
+    op.set_dirichlet(True)
+    op.set_dirichlet_faces(dmask)
+    op.assemble(1)
+    X = m.coords()
+    on_x = (np.abs(X[:, 0]) < 1e-12) | (np.abs(X[:, 0] - 1) < 1e-12)
+    assert np.array_equal(np.sort(op.dirichlet_dofs()), np.nonzero(on_x)[0])
+    x = spla.spsolve(op.csr().tocsc(), op.rhs())
+    assert np.abs(x - X[:, 0] * (1 - X[:, 0]) / 2).max() <= 1e-13
+    op2 = O.Operator(s, O.STENCIL_ELEMENT)
+    op2.append_laplace(1.0)
+    op2.append_source(1.0)
+    op2.set_dirichlet(True)
+    op2.assemble(1)
+    assert len(op2.dirichlet_dofs()) > len(op.dirichlet_dofs())
+
+
+@pytest.mark.parametrize("geom", [O.QUAD, O.TRI])
+def test_swipdg_mixed_boundary_reproduces_linear_solution(geom):
+    """SWIPDG with the weak Dirichlet forms and data only on the Dirichlet intersections (x = 0, 1:
+    ApplyOn::CustomBoundaryIntersections(boundary_info, DirichletBoundary()), diffusion-ipdg.hh:302-311) and
+    natural homogeneous Neumann conditions on y = 0, 1: u = 0.3 + 2 x has zero normal flux there, lies in the
+    space and is reproduced; applying the Dirichlet forms on every boundary intersection instead does not
+    reproduce it (u != 0.3 + 2 x is imposed on y = 0, 1 only weakly, with data 0 there)"""
+    import scipy.sparse.linalg as spla
+
+    n, sigma = 5, 16.0
+    m = O.Mesh.structured(geom, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, False, 1)
+    mid = _face_midpoints(m)
+    dmask = ((np.abs(mid[:, 0]) < 1e-12) | (np.abs(mid[:, 0] - 1) < 1e-12)).astype(np.uint8)
+
+    def u(X):
+        return 0.3 + 2.0 * X[..., 0]
+
+    over = 1 if geom == O.QUAD else 0
+    results = []
+    for mask in (dmask, None):
+        op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+        op.append_laplace(1.0)
+        op.append_inner_coupling(1.0, 1.0, 1.0)
+        op.append_inner_penalty(sigma, 1.0)
+        op.set_boundary_filter(mask)
+        op.append_dirichlet_coupling(1.0, 1.0)
+        op.append_boundary_penalty(sigma, 1.0)
+        g1 = u(m.boundary_quadrature_points(1 + over))
+        g2 = u(m.boundary_quadrature_points(2))
+        if mask is None:  # data 0 on the y faces, as a Dirichlet condition there would impose
+            g1[dmask == 0] = 0.
+            g2[dmask == 0] = 0.
+        op.append_boundary_source(0, 1.0, 1.0, g1, g_order=1, over_integrate=over)
+        op.append_boundary_source(1, sigma, 1.0, g2, g_order=1)
+        op.assemble(1)
+        x = spla.spsolve(op.csr().tocsc(), op.rhs())
+        results.append(np.abs(x[s.global_indices()] - u(m.coords()[m.conn()])).max())
+    assert results[0] <= 1e-11 and results[1] > 1e-3, results
