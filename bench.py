@@ -351,7 +351,8 @@ def run_single(hgdt, torch, args, config, fmt, steps, warmup, spmv_sample):
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "traffic_source": tsrc, "kernel": spmv_kernel_name(op, config),
                      "bytes_per_launch": step_bytes, "avg_launch_us": spmv_ms * 1e3,
-                     "timing": f"HIP events recorded by the dispatch around every {spmv_sample}th application"},
+                     "timing": f"HIP events bound to the dispatches, every {spmv_sample}th application: from the end of "
+                               "the kernel before it (the direction update) to its own end"},
     }
     return res, (grid, space, op, kappa, x)
 

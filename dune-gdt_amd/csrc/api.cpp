@@ -406,23 +406,32 @@ hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, doub
   return guarded([&] {
     require(grid && num_points && order >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
     const Grid& g = *grid->g;
-    require(g.ref.dim == 2, HGDT_ERR_NOT_SUPPORTED, "boundary quadrature points are provided for 2d grids");
-    const Quadrature q = line_rule(order);
+    const int d = g.ref.dim;
+    const Quadrature q = face_rule(g.ref, order);
     const int nq = q.size(), nf = g.ref.nfaces;
     *num_points = nq;
     if (!host_out)
       return;
+    // the points of the device face kernels: X = P0 + s (P1 - P0) (+ t (P2 - P0) in 3d) over the corners of
+    // the reference face, faces in walk order (elements ascending, faces in reference order)
     int64_t bi = 0;
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f) {
         if (g.nb[e * nf + f] >= 0)
           continue;
-        const int32_t va = g.conn[e * g.ref.nverts + g.ref.faces[f][0]];
-        const int32_t vb = g.conn[e * g.ref.nverts + g.ref.faces[f][1]];
+        double P[3][3];
+        for (int k = 0; k < d; ++k) {
+          const int32_t v = g.conn[e * g.ref.nverts + g.ref.faces[f][k]];
+          for (int a = 0; a < d; ++a)
+            P[k][a] = g.coords[int64_t(v) * d + a];
+        }
         for (int p = 0; p < nq; ++p)
-          for (int a = 0; a < 2; ++a) {
-            const double pa = g.coords[int64_t(va) * 2 + a], pb = g.coords[int64_t(vb) * 2 + a];
-            host_out[(bi * nq + p) * 2 + a] = pa + q.x[p] * (pb - pa);
+          for (int a = 0; a < d; ++a) {
+            const double s = q.x[size_t(p) * (d - 1)];
+            double X = P[0][a] + s * (P[1][a] - P[0][a]);
+            if (d == 3)
+              X += q.x[size_t(p) * 2 + 1] * (P[2][a] - P[0][a]);
+            host_out[(bi * nq + p) * d + a] = X;
           }
         ++bi;
       }
@@ -655,9 +664,6 @@ static void append_face(hgdt_operator op, int kind, double param, const double* 
                         double ov, int over_integrate, double scaling)
 {
   require(op && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
-  const Grid& g = *op->op->space->grid;
-  if (g.ref.dim != 2)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
   FaceFormDesc F{kind, param, kappa, kv, omega, ov, over_integrate, scaling, nullptr};
   if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY) {
     if (op->op->stencil == HGDT_STENCIL_ELEMENT)

@@ -457,144 +457,252 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 }
 
 // ---------------------------------------------------------------------------
-// intersections (2d: quadrilaterals and triangles)
+// intersections: segments (2d), triangles and parallelograms (3d)
 // ---------------------------------------------------------------------------
+// local vertex k of reference face f: simplices number face f opposite vertex D - f, cubes hold face
+// 2a + s on the vertices whose bit a equals s (the reference's face numbering; vertices ascending)
 template <int GEOM>
-__device__ __forceinline__ void face_frame(const double* __restrict__ coords, const int32_t* __restrict__ conn,
-                                           int64_t e, int f, double* Pa, double* tau, double* normal, double& len)
+__device__ __forceinline__ int face_vertex(int f, int k)
 {
   using T = Traits<GEOM>;
-  constexpr int fa[4][2] = {{0, 2}, {1, 3}, {0, 1}, {2, 3}};
-  constexpr int ft[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-  const int la = T::SIMPLEX ? ft[f][0] : fa[f][0];
-  const int lb = T::SIMPLEX ? ft[f][1] : fa[f][1];
-  const int32_t va = conn[e * T::NV + la], vb = conn[e * T::NV + lb];
-  double Pb[2], cen[2] = {0., 0.};
-  for (int a = 0; a < 2; ++a) {
-    Pa[a] = coords[int64_t(va) * 2 + a];
-    Pb[a] = coords[int64_t(vb) * 2 + a];
+  if constexpr (T::SIMPLEX)
+    return k + (k >= T::D - f ? 1 : 0);
+  const int a = f >> 1, s = f & 1;
+  return (k & ((1 << a) - 1)) | (s << a) | ((k >> a) << (a + 1));
+}
+
+// affine face X(s, t) = P0 + s E1 + t E2 over the corners P0, P1, P2 of the face (2d: X(s) = P0 + s E1);
+// integration element |E1| (2d) or |E1 x E2| (3d); unit normal pointing away from the inside element's
+// centroid; diameter h = largest distance between two corners (the length in 2d)
+template <int D>
+struct FaceFrame {
+  double P0[D], E1[D], E2[D], n[D];
+  double ie, h;
+  __device__ __forceinline__ void point(double s, double t, double* X) const
+  {
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      X[a] = (D == 2) ? P0[a] + s * E1[a] : P0[a] + s * E1[a] + t * E2[a];
   }
+};
+
+template <int GEOM>
+__device__ __forceinline__ void face_frame(const double* __restrict__ coords, const int32_t* __restrict__ conn,
+                                           int64_t e, int f, FaceFrame<Traits<GEOM>::D>& F)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, NC = (D == 2) ? 2 : (T::SIMPLEX ? 3 : 4);
+  double P[NC][D], cen[D];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int32_t v = conn[e * T::NV + face_vertex<GEOM>(f, k)];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      P[k][a] = coords[int64_t(v) * D + a];
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    cen[a] = 0.;
   for (int v = 0; v < T::NV; ++v) {
     const int32_t vv = conn[e * T::NV + v];
-    cen[0] += coords[int64_t(vv) * 2 + 0] / T::NV;
-    cen[1] += coords[int64_t(vv) * 2 + 1] / T::NV;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      cen[a] += coords[int64_t(vv) * D + a] / T::NV;
   }
-  tau[0] = Pb[0] - Pa[0], tau[1] = Pb[1] - Pa[1];
-  len = sqrt(tau[0] * tau[0] + tau[1] * tau[1]);
-  double nx = tau[1] / len, ny = -tau[0] / len;
-  if (nx * (Pa[0] - cen[0]) + ny * (Pa[1] - cen[1]) < 0.)
-    nx = -nx, ny = -ny;
-  normal[0] = nx + 0., normal[1] = ny + 0.;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    F.P0[a] = P[0][a], F.E1[a] = P[1][a] - P[0][a], F.E2[a] = 0.;
+  if constexpr (D == 2) {
+    F.ie = F.h = sqrt(F.E1[0] * F.E1[0] + F.E1[1] * F.E1[1]);
+    double nx = F.E1[1] / F.ie, ny = -F.E1[0] / F.ie;
+    if (nx * (P[0][0] - cen[0]) + ny * (P[0][1] - cen[1]) < 0.)
+      nx = -nx, ny = -ny;
+    F.n[0] = nx + 0., F.n[1] = ny + 0.;
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      F.E2[a] = P[2][a] - P[0][a];
+    double N[3];
+    N[0] = F.E1[1] * F.E2[2] - F.E1[2] * F.E2[1];
+    N[1] = F.E1[2] * F.E2[0] - F.E1[0] * F.E2[2];
+    N[2] = F.E1[0] * F.E2[1] - F.E1[1] * F.E2[0];
+    F.ie = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    double side = 0.;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      side += N[a] * (P[0][a] - cen[a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      F.n[a] = (side < 0. ? -N[a] : N[a]) / F.ie + 0.;
+    double h2 = 0.;
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+      for (int l = k + 1; l < NC; ++l) {
+        double d2 = 0.;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          d2 += (P[k][a] - P[l][a]) * (P[k][a] - P[l][a]);
+        h2 = fmax(h2, d2);
+      }
+    F.h = sqrt(h2);
+  }
+}
+
+// point q of a face rule (FD = D - 1 coordinates per point, weights after the points)
+template <int FD>
+__device__ __forceinline__ void face_qp(const double* __restrict__ qtab, const QRule& R, int q, double& s, double& t,
+                                        double& w)
+{
+  const double* __restrict__ b = qtab + int64_t(R.offset) * FD;
+  s = b[q * FD];
+  t = (FD == 2) ? b[q * FD + 1] : 0.;
+  w = b[R.npts * FD + q];
 }
 
 template <int GEOM>
-__device__ __forceinline__ void eval_at(const Geo<2>& g, const double* X, double* phi, double (*G)[2])
+__device__ __forceinline__ void eval_at(const Geo<Traits<GEOM>::D>& g, const double* X, double* phi,
+                                        double (*G)[Traits<GEOM>::D])
 {
-  constexpr int ND = Traits<GEOM>::ND;
-  double xh[2], dx[2] = {X[0] - g.x0[0], X[1] - g.x0[1]};
-  for (int a = 0; a < 2; ++a)
-    xh[a] = g.Ji[a][0] * dx[0] + g.Ji[a][1] * dx[1];
-  double dphi[ND][2];
+  constexpr int D = Traits<GEOM>::D, ND = Traits<GEOM>::ND;
+  double xh[D], dx[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    dx[a] = X[a] - g.x0[a];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double v = g.Ji[a][0] * dx[0];
+#pragma unroll
+    for (int b = 1; b < D; ++b)
+      v += g.Ji[a][b] * dx[b];
+    xh[a] = v;
+  }
+  double dphi[ND][D];
   shape<GEOM>(xh, phi, dphi);
+#pragma unroll
   for (int i = 0; i < ND; ++i)
-    to_physical<2>(g, dphi[i], G[i]);
+    to_physical<D>(g, dphi[i], G[i]);
 }
 
+template <int D>
+__device__ __forceinline__ double flux(double k, const double* G, const double* n)
+{
+  double v = (k * G[0]) * n[0];
+#pragma unroll
+  for (int a = 1; a < D; ++a)
+    v += (k * G[a]) * n[a];
+  return v;
+}
+
+template <int D>
+__device__ __forceinline__ double n_w_n(const double* n, double w)
+{
+  double v = n[0] * (w * n[0]);
+#pragma unroll
+  for (int a = 1; a < D; ++a)
+    v += n[a] * (w * n[a]);
+  return v;
+}
+
+// one inner intersection per lane: the four blocks (in_in, in_out, out_in, out_out) of every appended
+// coupling form, RB rows of each block at a time (all rows at once up to 4 basis functions).  With
+// symmetric forms the diagonal blocks are accumulated from their upper triangle and out_in as the transpose
+// of in_out: the stored blocks are symmetric bit for bit
 template <int GEOM>
 __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int ND = T::ND, K2 = ND * ND;
+  constexpr int D = T::D, ND = T::ND, K2 = ND * ND;
+  constexpr int RB = ND > 4 ? 2 : ND;
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
   const int64_t e = P.faces[4 * fi + 0], o = P.faces[4 * fi + 2];
   const int f = P.faces[4 * fi + 1];
-  Geo<2> gin, gout;
+  Geo<D> gin, gout;
   load_geometry<GEOM>(P.coords, P.conn, e, gin);
   load_geometry<GEOM>(P.coords, P.conn, o, gout);
-  double Pa[2], tau[2], n[2], len;
-  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
-  double B[4][ND][ND];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-      for (int j = 0; j < ND; ++j)
-        B[k][i][j] = 0.;
-  for (int fm = 0; fm < P.nforms; ++fm) {
-    const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
-    const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
-    const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
-    const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
-    const double sc = P.scaling[fm], prm = P.param[fm];
-    const QRule R = P.rule[fm];
-    for (int q = 0; q < R.npts; ++q) {
-      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
-      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
-      double pin[ND], pout[ND], Gin[ND][2], Gout[ND][2];
-      eval_at<GEOM>(gin, X, pin, Gin);
-      eval_at<GEOM>(gout, X, pout, Gout);
-      const double delta_plus = n[0] * (wout * n[0]) + n[1] * (wout * n[1]);
-      const double delta_minus = n[0] * (win * n[0]) + n[1] * (win * n[1]);
-      const double factor = len * wq;
-      if (P.kind[fm] == FF_INNER_COUPLING) {
-        const double wm = delta_plus / (delta_plus + delta_minus);
-        const double wp = delta_minus / (delta_plus + delta_minus);
-        double fin[ND], fout[ND]; // (kappa grad phi) . n
-#pragma unroll
-        for (int i = 0; i < ND; ++i) {
-          fin[i] = (kin * Gin[i][0]) * n[0] + (kin * Gin[i][1]) * n[1];
-          fout[i] = (kout * Gout[i][0]) * n[0] + (kout * Gout[i][1]) * n[1];
-        }
-#pragma unroll
-        for (int i = 0; i < ND; ++i)
-#pragma unroll
-          for (int j = 0; j < ND; ++j) {
-            const double ii = -1.0 * wm * fin[j] * pin[i] + -1.0 * prm * wm * pin[j] * fin[i];
-            const double io = -1.0 * wp * fout[j] * pin[i] + prm * wm * pout[j] * fin[i];
-            const double oi = wm * fin[j] * pout[i] + -1.0 * prm * wp * pin[j] * fout[i];
-            const double oo = wp * fout[j] * pout[i] + prm * wp * pout[j] * fout[i];
-            B[0][i][j] += sc * (ii * factor);
-            B[1][i][j] += sc * (io * factor);
-            B[2][i][j] += sc * (oi * factor);
-            B[3][i][j] += sc * (oo * factor);
-          }
-      } else {
-        const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus);
-        const double pen = (prm * weight) / len;
-#pragma unroll
-        for (int i = 0; i < ND; ++i)
-#pragma unroll
-          for (int j = 0; j < ND; ++j) {
-            B[0][i][j] += sc * ((pen * pin[j] * pin[i]) * factor);
-            B[1][i][j] += sc * ((-1.0 * pen * pout[j] * pin[i]) * factor);
-            B[2][i][j] += sc * ((-1.0 * pen * pin[j] * pout[i]) * factor);
-            B[3][i][j] += sc * ((pen * pout[j] * pout[i]) * factor);
-          }
-      }
-    }
-  }
+  FaceFrame<D> F;
+  face_frame<GEOM>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
+  const bool sym = P.symmetric != 0;
   double* __restrict__ out = P.staging + P.base + fi * (4 * K2);
-  if (P.symmetric) {
 #pragma unroll
-    for (int i = 0; i < ND; ++i)
-#pragma unroll
-      for (int j = 0; j < ND; ++j) {
-        out[0 * K2 + i * ND + j] = (i <= j) ? B[0][i][j] : B[0][j][i];
-        out[1 * K2 + i * ND + j] = B[1][i][j];
-        out[2 * K2 + i * ND + j] = B[1][j][i];
-        out[3 * K2 + i * ND + j] = (i <= j) ? B[3][i][j] : B[3][j][i];
-      }
-  } else {
+  for (int i0 = 0; i0 < ND; i0 += RB) {
+    double B[4][RB][ND];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int i = 0; i < ND; ++i)
+      for (int r = 0; r < RB; ++r)
 #pragma unroll
         for (int j = 0; j < ND; ++j)
-          out[k * K2 + i * ND + j] = B[k][i][j];
+          B[k][r][j] = 0.;
+    for (int fm = 0; fm < P.nforms; ++fm) {
+      const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+      const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
+      const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+      const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+      const double sc = P.scaling[fm], prm = P.param[fm];
+      const QRule R = P.rule[fm];
+      for (int q = 0; q < R.npts; ++q) {
+        double s, t, wq, X[D];
+        face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+        F.point(s, t, X);
+        double pin[ND], pout[ND], Gin[ND][D], Gout[ND][D];
+        eval_at<GEOM>(gin, X, pin, Gin);
+        eval_at<GEOM>(gout, X, pout, Gout);
+        const double delta_plus = n_w_n<D>(n, wout);
+        const double delta_minus = n_w_n<D>(n, win);
+        const double factor = F.ie * wq;
+        if (P.kind[fm] == FF_INNER_COUPLING) {
+          const double wm = delta_plus / (delta_plus + delta_minus);
+          const double wp = delta_minus / (delta_plus + delta_minus);
+          double fin[ND], fout[ND]; // (kappa grad phi) . n
+#pragma unroll
+          for (int i = 0; i < ND; ++i) {
+            fin[i] = flux<D>(kin, Gin[i], n);
+            fout[i] = flux<D>(kout, Gout[i], n);
+          }
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+              const int i = i0 + r;
+              const int a = (sym && i > j) ? j : i, b = (sym && i > j) ? i : j;
+              const double ii = -1.0 * wm * fin[b] * pin[a] + -1.0 * prm * wm * pin[b] * fin[a];
+              const double io = -1.0 * wp * fout[j] * pin[i] + prm * wm * pout[j] * fin[i];
+              const double oi = sym ? -1.0 * wp * fout[i] * pin[j] + prm * wm * pout[i] * fin[j]
+                                    : wm * fin[j] * pout[i] + -1.0 * prm * wp * pin[j] * fout[i];
+              const double oo = wp * fout[b] * pout[a] + prm * wp * pout[b] * fout[a];
+              B[0][r][j] += sc * (ii * factor);
+              B[1][r][j] += sc * (io * factor);
+              B[2][r][j] += sc * (oi * factor);
+              B[3][r][j] += sc * (oo * factor);
+            }
+        } else {
+          const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus);
+          const double pen = (prm * weight) / F.h;
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+              const int i = i0 + r;
+              const int a = (sym && i > j) ? j : i, b = (sym && i > j) ? i : j;
+              B[0][r][j] += sc * ((pen * pin[b] * pin[a]) * factor);
+              B[1][r][j] += sc * ((-1.0 * pen * pout[j] * pin[i]) * factor);
+              B[2][r][j] += sc * ((sym ? -1.0 * pen * pout[i] * pin[j] : -1.0 * pen * pin[j] * pout[i]) * factor);
+              B[3][r][j] += sc * ((pen * pout[b] * pout[a]) * factor);
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+          out[k * K2 + (i0 + r) * ND + j] = B[k][r][j];
   }
 }
 
@@ -602,16 +710,17 @@ template <int GEOM>
 __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int ND = T::ND;
+  constexpr int D = T::D, ND = T::ND;
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
   const int64_t e = P.faces[2 * fi + 0];
   const int f = P.faces[2 * fi + 1];
-  Geo<2> gin;
+  Geo<D> gin;
   load_geometry<GEOM>(P.coords, P.conn, e, gin);
-  double Pa[2], tau[2], n[2], len;
-  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
+  FaceFrame<D> F;
+  face_frame<GEOM>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
   double B[ND][ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i)
@@ -626,24 +735,24 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
     const double sc = P.scaling[fm], prm = P.param[fm];
     const QRule R = P.rule[fm];
     for (int q = 0; q < R.npts; ++q) {
-      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
-      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
-      double pin[ND], Gin[ND][2];
+      double s, t, wq, X[D];
+      face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+      F.point(s, t, X);
+      double pin[ND], Gin[ND][D];
       eval_at<GEOM>(gin, X, pin, Gin);
-      const double factor = len * wq;
+      const double factor = F.ie * wq;
       if (P.kind[fm] == FF_DIRICHLET_COUPLING) {
         double fin[ND];
 #pragma unroll
         for (int i = 0; i < ND; ++i)
-          fin[i] = (kin * Gin[i][0]) * n[0] + (kin * Gin[i][1]) * n[1];
+          fin[i] = flux<D>(kin, Gin[i], n);
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
           for (int j = 0; j < ND; ++j)
             B[i][j] += sc * ((-1.0 * fin[j] * pin[i] + -1.0 * prm * pin[j] * fin[i]) * factor);
       } else {
-        const double nwn = n[0] * (win * n[0]) + n[1] * (win * n[1]);
-        const double pen = (prm * nwn) / len;
+        const double pen = (prm * n_w_n<D>(n, win)) / F.h;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
@@ -669,16 +778,17 @@ template <int GEOM>
 __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int ND = T::ND;
+  constexpr int D = T::D, ND = T::ND;
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
   const int64_t e = P.faces[2 * fi + 0];
   const int f = P.faces[2 * fi + 1];
-  Geo<2> gin;
+  Geo<D> gin;
   load_geometry<GEOM>(P.coords, P.conn, e, gin);
-  double Pa[2], tau[2], n[2], len;
-  face_frame<GEOM>(P.coords, P.conn, e, f, Pa, tau, n, len);
+  FaceFrame<D> F;
+  face_frame<GEOM>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
   double b[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i)
@@ -690,19 +800,19 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
     const double prm = P.param[tm];
     const QRule R = P.rule[tm];
     for (int q = 0; q < R.npts; ++q) {
-      const double t = P.qtab[R.offset + q], wq = P.qtab[R.offset + R.npts + q];
-      const double X[2] = {Pa[0] + t * tau[0], Pa[1] + t * tau[1]};
-      double pin[ND], Gin[ND][2];
+      double s, t, wq, X[D];
+      face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+      F.point(s, t, X);
+      double pin[ND], Gin[ND][D];
       eval_at<GEOM>(gin, X, pin, Gin);
       const double g = P.data[tm] ? P.data[tm][fi * R.npts + q] : P.value[tm];
-      const double factor = len * wq;
+      const double factor = F.ie * wq;
       if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING) {
 #pragma unroll
         for (int i = 0; i < ND; ++i)
-          b[i] += (-1.0 * prm * g * ((c * Gin[i][0]) * n[0] + (c * Gin[i][1]) * n[1])) * factor;
+          b[i] += (-1.0 * prm * g * flux<D>(c, Gin[i], n)) * factor;
       } else if (P.kind[tm] == HGDT_BOUNDARY_PENALTY) {
-        const double nwn = n[0] * (c * n[0]) + n[1] * (c * n[1]);
-        const double pen = (prm * nwn) / len;
+        const double pen = (prm * n_w_n<D>(n, c)) / F.h;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           b[i] += (pen * g * pin[i]) * factor;
@@ -1031,7 +1141,7 @@ void dispatch_geom(int geom, F&& f)
 
 } // namespace
 
-// quadrature tables of the appended forms: element rules and face (line) rules
+// quadrature tables of the appended forms: element rules and face rules (dim - 1 coordinates per point)
 static void build_tables(const Operator& op, QTable& etab, QTable& ltab, ElementParams& P, FaceParams& C,
                          FaceParams& B)
 {
@@ -1058,14 +1168,13 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, Element
       F.kind[f] = D.kind, F.param[f] = D.param, F.kappa[f] = D.kappa, F.kappa_value[f] = D.kappa_value;
       F.omega[f] = D.omega, F.omega_value[f] = D.omega_value, F.scaling[f] = D.scaling, F.mask[f] = D.mask;
       // coupling / penalty orders: 0 + 0 + test + ansatz (laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
-      Quadrature q = line_rule(0 + k + k + D.over_integrate);
-      F.rule[f] = ltab.add(q);
+      F.rule[f] = ltab.add(face_rule(g.ref, 0 + k + k + D.over_integrate));
     }
   };
   fill_face(op.cforms, C);
   fill_face(op.bforms, B);
   for (size_t t = 0; t < op.bsources.size() && t < size_t(MAX_BSOURCES); ++t)
-    B.brule[t] = ltab.add(line_rule(op.boundary_source_order(op.bsources[t])));
+    B.brule[t] = ltab.add(face_rule(g.ref, op.boundary_source_order(op.bsources[t])));
   // the SWIPDG terms are symmetric for symmetry prefactor 1 (laplace-ipdg.hh:107-186,340-368); the
   // penalties always are (ipdg.hh:113-171,254-282)
   bool sym = true;
@@ -1081,14 +1190,12 @@ void Operator::prepare_assembly()
   if (eforms.size() > size_t(MAX_EFORMS) || sources.size() > size_t(MAX_SOURCES) || cforms.size() > size_t(MAX_FFORMS)
       || bforms.size() > size_t(MAX_FFORMS))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many local forms appended");
-  if ((!cforms.empty() || !bforms.empty()) && space->grid->ref.dim != 2)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for 2d grids");
   if ((!cforms.empty() || !bforms.empty()) && space->order != 1)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for order 1 spaces");
   if (bsources.size() > size_t(MAX_BSOURCES))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
-  if (!bsources.empty() && (space->grid->ref.dim != 2 || space->order != 1))
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "boundary functionals are provided for order 1 spaces on 2d grids");
+  if (!bsources.empty() && space->order != 1)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "boundary functionals are provided for order 1 spaces");
   if (tables_ready)
     return;
   QTable etab, ltab;
@@ -1129,19 +1236,17 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
   if (matrix && !op.cforms.empty() && nin > 0) {
     C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
     C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
-    if (g.ref.geom == HGDT_QUAD)
-      hipLaunchKernelGGL(k_inner_local<HGDT_QUAD>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
-    else
-      hipLaunchKernelGGL(k_inner_local<HGDT_TRI>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+    dispatch_geom(g.ref.geom, [&](auto G) {
+      hipLaunchKernelGGL(k_inner_local<decltype(G)::value>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+    });
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && !op.bforms.empty() && nbd > 0) {
     B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
     B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
-    if (g.ref.geom == HGDT_QUAD)
-      hipLaunchKernelGGL(k_boundary_local<HGDT_QUAD>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
-    else
-      hipLaunchKernelGGL(k_boundary_local<HGDT_TRI>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+    dispatch_geom(g.ref.geom, [&](auto G) {
+      hipLaunchKernelGGL(k_boundary_local<decltype(G)::value>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+    });
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
@@ -1160,10 +1265,9 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
       BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
       BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
-    if (g.ref.geom == HGDT_QUAD)
-      hipLaunchKernelGGL(k_boundary_rhs<HGDT_QUAD>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
-    else
-      hipLaunchKernelGGL(k_boundary_rhs<HGDT_TRI>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+    dispatch_geom(g.ref.geom, [&](auto G) {
+      hipLaunchKernelGGL(k_boundary_rhs<decltype(G)::value>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+    });
     HGDT_HIP(hipGetLastError());
   }
   if (!op.sources.empty() || !op.bsources.empty()) {

@@ -125,6 +125,7 @@ struct Quadrature {
 void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
 Quadrature line_rule(int order);
 Quadrature element_rule(const RefElement& ref, int order);
+Quadrature face_rule(const RefElement& ref, int order); // rule of the faces of ref (dim - 1 coordinates)
 
 // axis-parallel lattice of one instance of a structured grid: n[a] cells of width h[a] from lo[a]
 // (cells and vertices numbered lexicographically, x fastest)

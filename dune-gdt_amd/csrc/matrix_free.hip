@@ -1023,7 +1023,7 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
     HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free application: mode not provided for this operator");
   const double* ref = F.d_mf_ref.ptr;
   void* args[] = {&P, &ref};
-  if (ev0)
+  if (ev0 || ev1)
     HGDT_HIP(hipExtLaunchKernel(k, dim3(grid), dim3(block), args, 0, stream, ev0, ev1, 0));
   else
     HGDT_HIP(hipLaunchKernel(k, dim3(grid), dim3(block), args, 0, stream));

@@ -177,6 +177,15 @@ Quadrature line_rule(int order)
   return q;
 }
 
+// quadrature of the faces of `ref` (integrals.hh:226-233,352-356 use the rule of the intersection's
+// geometry type): Gauss-Legendre on [0, 1] in 2d, the triangle or square rule of element_rule in 3d
+Quadrature face_rule(const RefElement& ref, int order)
+{
+  if (ref.dim == 2)
+    return line_rule(order);
+  return element_rule(reference_element(ref.simplex ? HGDT_TRI : HGDT_QUAD), order);
+}
+
 Quadrature element_rule(const RefElement& ref, int order)
 {
   Quadrature q;

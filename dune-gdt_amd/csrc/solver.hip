@@ -2269,13 +2269,13 @@ void Operator::apply(const double* x, double* y, hipStream_t stream)
   HGDT_HIP(hipGetLastError());
 }
 
-// launch of k(P, par); with events, they are recorded by the dispatch itself (start and end of the
-// kernel, the interval a kernel trace reports), not by separate markers around it
+// launch of k(P, par); with events, they are bound to the dispatch itself (ev0 to its start, ev1 to its end;
+// either may be NULL), not recorded as separate markers around it
 template <class K>
 static void launch_pcg(K kernel, unsigned grid, int block, hipStream_t stream, const PcgParams& P, int par,
                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, size_t lds = 0)
 {
-  if (!ev0) {
+  if (!ev0 && !ev1) {
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, P, par);
     return;
   }
@@ -2286,8 +2286,10 @@ static void launch_pcg(K kernel, unsigned grid, int block, hipStream_t stream, c
                               ev0, ev1, 0));
 }
 
+// ev0 / ev1 time the iteration's operator application (ev0 NULL: only ev1 is bound, to its end), ev_upd is
+// bound to the end of the iteration's update
 static void launch_iteration(const Operator& op, const PcgParams& P, int par, hipStream_t stream, hipEvent_t ev0,
-                             hipEvent_t ev1)
+                             hipEvent_t ev1, hipEvent_t ev_upd = nullptr)
 {
   const RowPartition& A = op.fmt.A;
   const RowPartition& V = op.fmt.V;
@@ -2298,11 +2300,17 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   // first and last
   PcgParams Pd = P, Pa = P, Pu = P;
   Pd.rev = 1, Pa.rev = 0, Pu.rev = 1;
-  if (!fused)
+  // timed application: from the end of the direction update to the end of the application (the kernels
+  // run back to back, so this is the application's own interval; an event bound to the start of a
+  // dispatch also counts the dispatch's set-up); the caller times an application fused with the direction
+  // update from the end of the previous update
+  if (!fused) {
     dispatch_items(V.items, [&](auto IT) {
       constexpr int I = decltype(IT)::value;
-      launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, Pd, par);
+      launch_pcg(k_cg_direction<I>, hw_blocks(P.v_nb), V.block, stream, Pd, par, nullptr, ev0);
     });
+    ev0 = nullptr;
+  }
   if (op.fmt.format == HGDT_FORMAT_MATRIX_FREE) {
     MfVectors v;
     v.x = par ? P.p : P.p2, v.y = P.q, v.dh = P.dh, v.part = P.part1, v.S = P.S, v.par = par;
@@ -2335,7 +2343,7 @@ static void launch_iteration(const Operator& op, const PcgParams& P, int par, hi
   });
   dispatch_items(V.items, [&](auto IT) {
     constexpr int I = decltype(IT)::value;
-    launch_pcg(k_cg_update<I>, hw_blocks(P.v_nb), V.block, stream, Pu, par);
+    launch_pcg(k_cg_update<I>, hw_blocks(P.v_nb), V.block, stream, Pu, par, nullptr, ev_upd);
   });
 }
 
@@ -2424,25 +2432,37 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   size_t nev = 0;
   int64_t issued = 0; // iterations launched so far
   std::vector<int64_t> timed_iter;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed_pair; // (start, end) of each timed application
   const int64_t stride = opt.time_spmv > 0 ? opt.time_spmv : 0;
+  const bool fused_dir = mf_fused_direction(*this);
+  auto timing_event = [&]() {
+    if (C.timing_events.size() < nev + 1) {
+      hipEvent_t e;
+      HGDT_HIP(hipEventCreate(&e));
+      C.timing_events.push_back(e);
+    }
+    return C.timing_events[nev++];
+  };
+  hipEvent_t prev_upd = nullptr; // bound to the end of the previous iteration's update (fused application)
   auto launch_chunk = [&]() {
     if (graph) {
       HGDT_HIP(hipGraphLaunch(C.graph, stream));
       issued += chunk;
     } else {
       for (int it = 0; it < chunk; ++it, ++issued) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr, eu = nullptr;
         if (stride && issued % stride == 0) {
           timed_iter.push_back(issued);
-          while (C.timing_events.size() < nev + 2) {
-            hipEvent_t e;
-            HGDT_HIP(hipEventCreate(&e));
-            C.timing_events.push_back(e);
-          }
-          e0 = C.timing_events[nev], e1 = C.timing_events[nev + 1];
-          nev += 2;
+          e1 = timing_event();
+          // unfused: e0 is bound to the end of the direction update; fused: the previous update's end, or
+          // the start of the first iteration's application
+          e0 = (fused_dir && prev_upd) ? nullptr : timing_event();
+          timed_pair.emplace_back(e0 ? e0 : prev_upd, e1);
         }
-        launch_iteration(*this, P, it & 1, stream, e0, e1);
+        if (fused_dir && stride && (issued + 1) % stride == 0)
+          eu = timing_event();
+        launch_iteration(*this, P, it & 1, stream, e0, e1, eu);
+        prev_upd = eu;
       }
     }
     HGDT_HIP(hipGetLastError());
@@ -2580,10 +2600,10 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     // only iterations that did work: the early-exit launches after convergence are skipped
     double total = 0.;
     int64_t cnt = 0;
-    for (size_t k = 0; k + 1 < nev; k += 2) {
-      if (timed_iter[k / 2] < st.iterations) {
+    for (size_t k = 0; k < timed_pair.size(); ++k) {
+      if (timed_iter[k] < st.iterations) {
         float t = 0.f;
-        HGDT_HIP(hipEventElapsedTime(&t, C.timing_events[k], C.timing_events[k + 1]));
+        HGDT_HIP(hipEventElapsedTime(&t, timed_pair[k].first, timed_pair[k].second));
         total += t, ++cnt;
       }
     }

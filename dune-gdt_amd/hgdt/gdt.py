@@ -238,7 +238,7 @@ class Grid:
         return c, n
 
     def boundary_quadrature_points(self, order):
-        """Physical coordinates of the line-rule points on the boundary intersections in walk order: (nb, nq, dim)."""
+        """Physical coordinates of the face-rule points on the boundary intersections in walk order: (nb, nq, dim)."""
         nq = ctypes.c_int32()
         L.call("hgdt_grid_boundary_quadrature_points", self._h, order, None, ctypes.byref(nq))
         out = np.empty((self.num_boundary_intersections, nq.value, self.dimension))

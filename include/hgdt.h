@@ -115,9 +115,10 @@ hgdt_status hgdt_grid_instance_elements(hgdt_grid grid, int64_t* host_out);
  * Pass host_out = NULL to query the size only. */
 hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
 
-/* physical coordinates of the quadrature points (line rule of the given order) on the boundary
+/* physical coordinates of the quadrature points (face rule of the given order: Gauss-Legendre on segments
+ * in 2d, the triangle or tensor-Gauss square rule on the faces of tetrahedra / hexahedra) on the boundary
  * intersections, in walk order (elements ascending, their faces in reference order):
- * host_out[num_boundary_intersections x num_points x dim] (2d grids); host_out = NULL queries the size. */
+ * host_out[num_boundary_intersections x num_points x dim]; host_out = NULL queries the size. */
 hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
 
 /* centres [num_boundary_intersections x dim] and outward unit normals [num_boundary_intersections x dim] of the
@@ -246,7 +247,7 @@ hgdt_status hgdt_operator_append_source(hgdt_operator op, int kind, const double
  * coeff: per-element device field or coeff_value (NULL field).  g: device values at the points of
  * hgdt_grid_boundary_quadrature_points(order) with order = g_order + max(k - 1, 0) + over_integrate for the
  * coupling term and k + g_order + over_integrate for the others (k the space order), or the constant
- * `value` (NULL data).  Device arrays must stay valid.  2d grids, order-1 spaces. */
+ * `value` (NULL data).  Device arrays must stay valid.  Order-1 spaces. */
 hgdt_status hgdt_operator_append_boundary_source(hgdt_operator op, int kind, double param, const double* coeff_field,
                                                  double coeff_value, const double* data, double value, int32_t g_order,
                                                  int32_t over_integrate);

@@ -167,7 +167,7 @@ class Mesh:
         return out
 
     def boundary_quadrature_points(self, order):
-        """physical coordinates of the line-rule points on the boundary intersections (walk order), [nb, npts, 2]"""
+        """physical coordinates of the face-rule points on the boundary intersections (walk order), [nb, npts, dim]"""
         n = lib().or_mesh_boundary_quadrature_points(self.h, order, None)
         if n < 0:
             raise RuntimeError(_err())

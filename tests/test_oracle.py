@@ -697,3 +697,153 @@ def test_swipdg_mixed_boundary_reproduces_linear_solution(geom):
         x = spla.spsolve(op.csr().tocsc(), op.rhs())
         results.append(np.abs(x[s.global_indices()] - u(m.coords()[m.conn()])).max())
     assert results[0] <= 1e-11 and results[1] > 1e-3, results
+
+
+def _face_form_from_traces(jump, avg_flux, M, area, hF, s, sigma, gamma):
+    """a_F(u, v) = -int {kappa grad u}_w . n [v] - s int {kappa grad v}_w . n [u] + sigma gamma / h_F int [u][v]
+    for traces expanded in a face basis with mass matrix M (of the unit face; scaled by area): jump[g] and
+    avg_flux[g] are the coefficient vectors of [phi_g] and {kappa grad phi_g}_w . n"""
+    G = len(jump)
+    A = np.empty((G, G))
+    for i in range(G):
+        for j in range(G):
+            A[i, j] = area * (-avg_flux[j] @ M @ jump[i] - s * avg_flux[i] @ M @ jump[j]
+                              + sigma * gamma / hF * jump[j] @ M @ jump[i])
+    return A
+
+
+def _swip_face_matrix_hex(a, a2, b, c, kappa, omega, s, sigma):
+    """two Q1 hexahedra [0, a] x [0, b] x [0, c] (-) and [a, a + a2] x [0, b] x [0, c] (+), face x = a with
+    n = e_x, area b c and diameter h_F = sqrt(b^2 + c^2) (its diagonal): traces are bilinear in (y, z), so
+    the face mass matrix of the unit square is M1 (x) M1; the x-derivative of a trilinear basis function on
+    the face is +-1/h_x times its (y, z) factor.  Global DoF 8 e + local vertex (x bit 0, y bit 1, z bit 2)"""
+    M1 = np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+    M = np.kron(M1, M1)
+    km, kp = kappa
+    dm, dp = omega
+    wm, wp = dp / (dm + dp), dm / (dm + dp)
+    jump, avg = np.zeros((16, 4)), np.zeros((16, 4))
+    for g in range(16):
+        e, loc = divmod(g, 8)
+        bx, by, bz = loc & 1, (loc >> 1) & 1, loc >> 2
+        eyz = np.kron(np.eye(2)[by], np.eye(2)[bz])
+        hx = a if e == 0 else a2
+        on_face = bx == 1 if e == 0 else bx == 0
+        jump[g] = (1.0 if on_face else 0.0) * eyz * (1 if e == 0 else -1)
+        avg[g] = (wm * km if e == 0 else wp * kp) * (1 / hx if bx == 1 else -1 / hx) * eyz
+    return _face_form_from_traces(jump, avg, M, b * c, np.hypot(b, c), s, sigma, dm * dp / (dm + dp))
+
+
+@pytest.mark.parametrize("kappa,omega,s", [((3.0, 0.25), (3.0, 0.25), 1.0), ((3.0, 0.25), (2.0, 5.0), 1.0),
+                                           ((1.0, 1e-4), (1.0, 1e-4), -1.0), ((3.0, 0.25), (2.0, 5.0), 0.0)])
+def test_swipdg_face_closed_form_hexahedra(kappa, omega, s):
+    """3-d intersections (integrals.hh:197-267 over the faces of hexahedra): the oracle's coupling + penalty
+    blocks of two box cells of different widths equal the textbook face form with the bilinear face traces;
+    exchanging the weights, or taking the face's edge instead of its diagonal as h_F, changes the matrix"""
+    a, a2, b, c = 0.5, 1.25, 0.8, 0.6
+    m = O.Mesh.explicit(O.HEX, np.array([[x, y, z] for z in (0, c) for y in (0, b) for x in (0, a, a + a2)], float),
+                        np.array([[0, 1, 3, 4, 6, 7, 9, 10], [1, 2, 4, 5, 7, 8, 10, 11]], np.int32))
+    op = O.Operator(O.Space(m, False, 1), O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_inner_coupling(s, np.array(kappa), np.array(omega))
+    op.append_inner_penalty(16.0, np.array(omega))
+    op.assemble(1)
+    A = op.csr().toarray()
+    E = _swip_face_matrix_hex(a, a2, b, c, kappa, omega, s, 16.0)
+    assert np.abs(A - E).max() <= 1e-14 * np.abs(E).max()
+    W = _swip_face_matrix_hex(a, a2, b, c, kappa, omega[::-1], s, 16.0)
+    assert np.abs(A - W).max() >= 0.05 * np.abs(E).max()
+    H = _swip_face_matrix_hex(a, a2, b, c, kappa, omega, s, 16.0 * np.hypot(b, c) / b)
+    assert np.abs(A - H).max() >= 0.05 * np.abs(E).max()
+
+
+def _p1_gradients(V):
+    """gradients of the barycentric coordinates of the tetrahedron with vertex rows V"""
+    T = np.hstack([np.ones((4, 1)), V])
+    return np.linalg.inv(T)[1:].T  # row i: grad lambda_i
+
+
+def test_swipdg_face_closed_form_tetrahedra():
+    """3-d intersections over the faces of tetrahedra: two P1 tetrahedra sharing the face F = {(1,0,0),
+    (0,1,0), (0,0,1)}; traces are linear on F (face mass matrix |F| / 12 [[2,1,1],[1,2,1],[1,1,2]]), fluxes
+    constant per element (gradients from the barycentric system), n = (1,1,1)/sqrt(3) out of the first
+    element, h_F = the longest edge sqrt(2)"""
+    V0 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    apex = np.array([0.9, 0.7, 0.8])
+    X = np.vstack([V0, apex])
+    conn = np.array([[0, 1, 2, 3], [1, 2, 3, 4]], np.int32)
+    m = O.Mesh.explicit(O.TET, X, conn)
+    kappa, omega, s, sigma = (2.0, 0.5), (1.5, 4.0), 1.0, 16.0
+    op = O.Operator(O.Space(m, False, 1), O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_inner_coupling(s, np.array(kappa), np.array(omega))
+    op.append_inner_penalty(sigma, np.array(omega))
+    op.assemble(1)
+    A = op.csr().toarray()
+    n = np.ones(3) / np.sqrt(3)
+    area = np.sqrt(3) / 2
+    Mf = np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]]) / 12  # P1 mass matrix of a triangle of area 1
+    dm, dp = omega
+    wm, wp = dp / (dm + dp), dm / (dm + dp)
+    face = [1, 2, 3]  # global vertices of F, the face basis
+    jump, avg = np.zeros((8, 3)), np.zeros((8, 3))
+    for e in range(2):
+        G = _p1_gradients(X[conn[e]])
+        for loc in range(4):
+            g = 4 * e + loc
+            v = conn[e][loc]
+            tr = np.zeros(3)
+            if v in face:
+                tr[face.index(v)] = 1.0
+            jump[g] = tr * (1 if e == 0 else -1)
+            # a constant flux: its coefficient vector in the (partition of unity) face basis is constant
+            avg[g] = (wm * kappa[0] if e == 0 else wp * kappa[1]) * (G[loc] @ n) * np.ones(3)
+    E = _face_form_from_traces(jump, avg, Mf, area, np.sqrt(2), s, sigma, dm * dp / (dm + dp))
+    assert np.abs(A - E).max() <= 1e-13 * np.abs(E).max()
+
+
+def _swipdg_3d(m, s, kappa, sigma, g_vals, over):
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(kappa)
+    op.append_inner_coupling(1.0, kappa, kappa)
+    op.append_inner_penalty(sigma, kappa)
+    op.append_dirichlet_coupling(1.0, kappa)
+    op.append_boundary_penalty(sigma, kappa)
+    op.append_boundary_source(0, 1.0, kappa, g_vals(1 + over), g_order=1, over_integrate=over)
+    op.append_boundary_source(1, sigma, kappa, g_vals(2), g_order=1)
+    return op
+
+
+@pytest.mark.parametrize("geom", [O.HEX, O.TET])
+def test_swipdg_3d_reproduces_linear_and_layered_solutions(geom):
+    """SWIPDG on 3-d grids with weak Dirichlet data g = u (boundary functionals over the faces,
+    local/functionals/integrals.hh:157-185): a linear u with kappa = 1, and u(z) = 0.2 + int_0^z 1 / kappa
+    for kappa constant on layers of cells (contrast 1e3, continuous flux kappa u' = 1).  Both lie in the
+    P1 / Q1-DG space and are reproduced; the linear u with the layered kappa is not a solution (its flux
+    jumps across the layers) and is not reproduced"""
+    import scipy.sparse.linalg as spla
+
+    m = O.Mesh.structured(geom, [3, 3, 4], [0, 0, 0], [1, 1, 1])
+    s = O.Space(m, False, 1)
+    over = 1 if geom == O.HEX else 0
+    Xe = m.coords()[m.conn()]
+    layer = np.minimum((Xe[..., 2].mean(axis=1) * 4).astype(int), 3)
+    kl = np.array([1.0, 1e-3, 1.0, 0.2])
+    zb = np.concatenate([[0], np.cumsum(0.25 / kl)])
+
+    def u_lin(P):
+        return 0.3 + 2.0 * P[..., 0] - 1.0 * P[..., 1] + 0.5 * P[..., 2]
+
+    def u_layer(P):
+        z = P[..., 2]
+        k = np.minimum((z * 4).astype(int), 3)
+        return 0.2 + zb[k] + (z - 0.25 * k) / kl[k]
+
+    # vertices nudged into their element, so that a vertex on a layer interface takes its element's layer
+    Pn = Xe + 1e-12 * (Xe.mean(axis=1, keepdims=True) - Xe)
+    res = {}
+    for name, kappa, u in (("linear", np.ones(m.ne), u_lin), ("layered", kl[layer], u_layer),
+                           ("linear, layered kappa", kl[layer], u_lin)):
+        op = _swipdg_3d(m, s, kappa, 16.0, lambda order: u(m.boundary_quadrature_points(order)), over)
+        op.assemble(1)
+        x = spla.spsolve(op.csr().tocsc(), op.rhs())
+        res[name] = np.abs(x[s.global_indices()] - u(Pn)).max()
+    assert res["linear"] <= 1e-10 and res["layered"] <= 1e-9 and res["linear, layered kappa"] > 1e-3, res
