@@ -446,8 +446,6 @@ hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* s
             "unknown space kind");
     if (order != 1 && order != 2)
       HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "device spaces are provided for polynomial order 1 and 2");
-    if (order == 2 && grid->g->ref.geom == HGDT_HEX)
-      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "order 2 on hexahedra (27 local DoFs) is not provided on the device");
     auto s = std::make_shared<Space>();
     s->grid = grid->g;
     s->kind = kind, s->order = order;
@@ -582,9 +580,62 @@ hgdt_status hgdt_operator_create(hgdt_space space, int stencil, hgdt_operator* o
     require(stencil >= HGDT_STENCIL_ELEMENT && stencil <= HGDT_STENCIL_ELEMENT_AND_INTERSECTION,
             HGDT_ERR_INVALID_ARGUMENT, "unknown stencil");
     auto o = std::make_unique<Operator>();
-    o->space = space->s;
+    o->space = o->source = space->s;
     o->stencil = stencil;
     o->build_pattern();
+    *op = new hgdt_operator_s{std::move(o)};
+  });
+}
+
+static void check_space_pair(hgdt_space source, hgdt_space range)
+{
+  require(source && range, HGDT_ERR_INVALID_ARGUMENT, "NULL space");
+  const Space& s = *source->s;
+  const Space& r = *range->s;
+  require(s.grid == r.grid, HGDT_ERR_NOT_SUPPORTED, "the source and range spaces must live on the same grid");
+  require(s.order == r.order && s.local_size == r.local_size, HGDT_ERR_NOT_SUPPORTED,
+          "the source and range spaces must have the same local finite element (polynomial order)");
+}
+
+hgdt_status hgdt_operator_create_between(hgdt_space source, hgdt_space range, int stencil, hgdt_operator* op)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_space_pair(source, range);
+    require(stencil >= HGDT_STENCIL_ELEMENT && stencil <= HGDT_STENCIL_ELEMENT_AND_INTERSECTION,
+            HGDT_ERR_INVALID_ARGUMENT, "unknown stencil");
+    auto o = std::make_unique<Operator>();
+    o->space = range->s;
+    o->source = source->s == range->s ? range->s : source->s;
+    o->stencil = stencil;
+    o->build_pattern();
+    *op = new hgdt_operator_s{std::move(o)};
+  });
+}
+
+hgdt_status hgdt_operator_create_on_matrix(hgdt_space source, hgdt_space range, int stencil, const int64_t* row_ptr,
+                                           const int32_t* cols, int64_t rows, int64_t nnz, double* values,
+                                           hgdt_operator* op)
+{
+  return guarded([&] {
+    require(op && row_ptr && (cols || nnz == 0) && (values || nnz == 0) && rows >= 0 && nnz >= 0,
+            HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    check_space_pair(source, range);
+    require(stencil >= HGDT_STENCIL_ELEMENT && stencil <= HGDT_STENCIL_ELEMENT_AND_INTERSECTION,
+            HGDT_ERR_INVALID_ARGUMENT, "unknown stencil");
+    require(rows == range->s->size, HGDT_ERR_SHAPES_DO_NOT_MATCH, "the matrix needs one row per DoF of the range space");
+    require(row_ptr[rows] == nnz, HGDT_ERR_SHAPES_DO_NOT_MATCH, "row_ptr[rows] must equal nnz");
+    auto o = std::make_unique<Operator>();
+    o->space = range->s;
+    o->source = source->s == range->s ? range->s : source->s;
+    o->stencil = stencil;
+    o->borrowed = true;
+    o->borrowed_row_ptr.assign(row_ptr, row_ptr + rows + 1);
+    o->borrowed_cols.assign(cols, cols + nnz);
+    o->d_values.borrow(values, size_t(nnz));
+    o->build_pattern();
+    o->borrowed_row_ptr.clear(), o->borrowed_cols.clear();
+    o->borrowed_row_ptr.shrink_to_fit(), o->borrowed_cols.shrink_to_fit();
     *op = new hgdt_operator_s{std::move(o)};
   });
 }
@@ -601,7 +652,7 @@ hgdt_status hgdt_operator_shape(hgdt_operator op, int64_t* rows, int64_t* cols, 
     if (rows)
       *rows = op->op->n;
     if (cols)
-      *cols = op->op->n;
+      *cols = op->op->ncols;
     if (nnz)
       *nnz = op->op->nnz;
   });
@@ -772,6 +823,8 @@ hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled)
 {
   return guarded([&] {
     require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    require(!enabled || op->op->square(), HGDT_ERR_NOT_SUPPORTED,
+            "Dirichlet constraints need an operator whose source and range spaces are the same");
     op->op->dirichlet = enabled != 0;
     forms_changed(*op->op);
   });

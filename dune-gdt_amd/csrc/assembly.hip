@@ -456,6 +456,120 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
   }
 }
 
+// element matrices of an element with many local DoFs (Q2 on hexahedra: 27, whose 378 upper-triangle
+// entries do not fit a lane's registers): one lane per (element, row i) accumulates the row's entries over
+// the quadrature points and writes those with j >= i (and the right-hand side's entry i).  The basis
+// functions are the products of their 1-d factors, formed per point in the order of basis<>() (the same
+// values and gradients), and every entry is accumulated as in k_element_local.
+__device__ __forceinline__ double pick3(int m, const double* v)
+{
+  return m == 0 ? v[0] : (m == 1 ? v[1] : v[2]);
+}
+
+template <int GEOM, int ORDER>
+__global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
+{
+  using T = Traits<GEOM>;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
+  static_assert(!T::SIMPLEX && ORDER == 2 && D == 3, "row lanes are provided for Q2 hexahedra");
+  const int64_t lane = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= P.ne * ND)
+    return;
+  const int64_t e = lane / ND;
+  const int i = int(lane % ND);
+  const int ix = i % 3, iy = (i / 3) % 3, iz = i / 9;
+  Geo<D> g;
+  load_geometry<GEOM>(P.coords, P.conn, e, g);
+  const double* __restrict__ qt = P.qtab;
+  // 1-d factors at a point: l[a][m] = lag2(m, x_a), dl[a][m] = lag2'(m, x_a)
+  auto factors = [&](const double* xh, double (&l)[3][3], double (&dl)[3][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        l[a][m] = lag2(m, xh[a]), dl[a][m] = dlag2(m, xh[a]);
+  };
+  if (P.write_matrix) {
+    double row[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+      row[j] = 0.;
+    for (int f = 0; f < P.nforms; ++f) {
+      const double c = P.field[f] ? P.field[f][e] : P.value[f];
+      const double s = P.scaling[f];
+      const QRule R = P.rule[f];
+      for (int q = 0; q < R.npts; ++q) {
+        double xh[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          xh[a] = qt[R.offset * D + q * D + a];
+        const double w = qt[R.offset * D + R.npts * D + q];
+        double l[3][3], dl[3][3];
+        factors(xh, l, dl);
+        const double lx = pick3(ix, l[0]), ly = pick3(iy, l[1]), lz = pick3(iz, l[2]);
+        const double factor = g.det * w;
+        if (P.kind[f] == EF_LAPLACE) {
+          const double gh[3] = {(pick3(ix, dl[0]) * ly) * lz, (pick3(iy, dl[1]) * lx) * lz,
+                                (pick3(iz, dl[2]) * lx) * ly};
+          double Gi[3];
+          to_physical<3>(g, gh, Gi);
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
+            const double ghj[3] = {(dl[0][jx] * l[1][jy]) * l[2][jz], (dl[1][jy] * l[0][jx]) * l[2][jz],
+                                   (dl[2][jz] * l[0][jx]) * l[1][jy]};
+            double Gj[3];
+            to_physical<3>(g, ghj, Gj);
+            double dotv = 0.;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+              dotv += (c * Gj[a]) * Gi[a];
+            row[j] += s * (dotv * factor);
+          }
+        } else {
+          const double phii = (lx * ly) * lz;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
+            const double phij = (l[0][jx] * l[1][jy]) * l[2][jz];
+            row[j] += s * (((c * phii) * phij) * factor);
+          }
+        }
+      }
+    }
+    // packed upper triangle, entry-major: entry (i, j >= i) at t = i ND - i (i - 1) / 2 + (j - i)
+    double* __restrict__ out = P.staging + P.elem_base;
+    const int t0 = i * ND - i * (i - 1) / 2 - i;
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+      if (j >= i)
+        out[int64_t(t0 + j) * P.ne + e] = row[j];
+  }
+  if (P.nsrc > 0) {
+    double b = 0.;
+    for (int sidx = 0; sidx < P.nsrc; ++sidx) {
+      const QRule R = P.src_rule[sidx];
+      for (int q = 0; q < R.npts; ++q) {
+        double xh[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          xh[a] = qt[R.offset * D + q * D + a];
+        const double w = qt[R.offset * D + R.npts * D + q];
+        const double phii = (lag2(ix, xh[0]) * lag2(iy, xh[1])) * lag2(iz, xh[2]);
+        double fv;
+        if (P.src_kind[sidx] == HGDT_SOURCE_CONSTANT)
+          fv = P.src_value[sidx];
+        else if (P.src_kind[sidx] == HGDT_SOURCE_PER_ELEMENT)
+          fv = P.src_data[sidx][e];
+        else
+          fv = P.src_data[sidx][e * R.npts + q];
+        b += (phii * fv) * g.det * w;
+      }
+    }
+    P.staging[P.rhs_base + int64_t(i) * P.ne + e] = b;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // intersections: segments (2d), triangles and parallelograms (3d)
 // ---------------------------------------------------------------------------
@@ -560,11 +674,11 @@ __device__ __forceinline__ void face_qp(const double* __restrict__ qtab, const Q
   w = b[R.npts * FD + q];
 }
 
-template <int GEOM>
+template <int GEOM, int ORDER>
 __device__ __forceinline__ void eval_at(const Geo<Traits<GEOM>::D>& g, const double* X, double* phi,
                                         double (*G)[Traits<GEOM>::D])
 {
-  constexpr int D = Traits<GEOM>::D, ND = Traits<GEOM>::ND;
+  constexpr int D = Traits<GEOM>::D, ND = Basis<GEOM, ORDER>::ND;
   double xh[D], dx[D];
 #pragma unroll
   for (int a = 0; a < D; ++a)
@@ -578,7 +692,7 @@ __device__ __forceinline__ void eval_at(const Geo<Traits<GEOM>::D>& g, const dou
     xh[a] = v;
   }
   double dphi[ND][D];
-  shape<GEOM>(xh, phi, dphi);
+  basis<GEOM, ORDER>(xh, phi, dphi);
 #pragma unroll
   for (int i = 0; i < ND; ++i)
     to_physical<D>(g, dphi[i], G[i]);
@@ -608,12 +722,13 @@ __device__ __forceinline__ double n_w_n(const double* n, double w)
 // coupling form, RB rows of each block at a time (all rows at once up to 4 basis functions).  With
 // symmetric forms the diagonal blocks are accumulated from their upper triangle and out_in as the transpose
 // of in_out: the stored blocks are symmetric bit for bit
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND, K2 = ND * ND;
-  constexpr int RB = ND > 4 ? 2 : ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND, K2 = ND * ND;
+  // rows of the four blocks held at once: all of them up to 4 basis functions, else at most 16 / ND
+  constexpr int RB = ND <= 4 ? ND : (16 / ND > 0 ? 16 / ND : 1);
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
@@ -649,8 +764,8 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
         face_qp<D - 1>(P.qtab, R, q, s, t, wq);
         F.point(s, t, X);
         double pin[ND], pout[ND], Gin[ND][D], Gout[ND][D];
-        eval_at<GEOM>(gin, X, pin, Gin);
-        eval_at<GEOM>(gout, X, pout, Gout);
+        eval_at<GEOM, ORDER>(gin, X, pin, Gin);
+        eval_at<GEOM, ORDER>(gout, X, pout, Gout);
         const double delta_plus = n_w_n<D>(n, wout);
         const double delta_minus = n_w_n<D>(n, win);
         const double factor = F.ie * wq;
@@ -706,11 +821,11 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
   }
 }
 
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
@@ -739,7 +854,7 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
       face_qp<D - 1>(P.qtab, R, q, s, t, wq);
       F.point(s, t, X);
       double pin[ND], Gin[ND][D];
-      eval_at<GEOM>(gin, X, pin, Gin);
+      eval_at<GEOM, ORDER>(gin, X, pin, Gin);
       const double factor = F.ie * wq;
       if (P.kind[fm] == FF_DIRICHLET_COUPLING) {
         double fin[ND];
@@ -774,11 +889,11 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
 //   DirichletCoupling as a unary integrand: -s g (kappa grad psi_i) . n   (laplace-ipdg.hh:300-320)
 //   BoundaryPenalty(sigma, w).with_ansatz(g): sigma (n . w n) / h_F g psi_i (ipdg.hh:254-282)
 //   LocalIntersectionProductIntegrand(w).with_ansatz(g): (w g) psi_i     (product.hh:355-377)
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (fi >= P.nfaces)
     return;
@@ -804,7 +919,7 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
       face_qp<D - 1>(P.qtab, R, q, s, t, wq);
       F.point(s, t, X);
       double pin[ND], Gin[ND][D];
-      eval_at<GEOM>(gin, X, pin, Gin);
+      eval_at<GEOM, ORDER>(gin, X, pin, Gin);
       const double g = P.data[tm] ? P.data[tm][fi * R.npts + q] : P.value[tm];
       const double factor = F.ie * wq;
       if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING) {
@@ -834,9 +949,8 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
 // contributions are issued together so that a lane keeps U index / value pairs in flight
 template <int U>
 __device__ __forceinline__ double ordered_sum(const uint32_t* __restrict__ contrib, const double* __restrict__ staging,
-                                              uint32_t c0, uint32_t c1)
+                                              uint32_t c0, uint32_t c1, double v = 0.)
 {
-  double v = 0.;
   for (uint32_t c = c0; c < c1; c += U) {
     uint32_t k[U];
     double x[U];
@@ -856,10 +970,13 @@ __device__ __forceinline__ double ordered_sum(const uint32_t* __restrict__ contr
 
 // blocks are mapped XCD-aware (xcd_index): each XCD sums a contiguous range of slots, whose contributions
 // come from a compact set of elements, so the staged lines they share are found in that XCD's L2
+// accumulate: the contributions are added to the value held, in walk order (add_to_entry on a caller's
+// matrix, bilinear-form-assemblers.hh:126); otherwise the slot starts at 0
 __global__ void __launch_bounds__(256) k_gather_values(int64_t nnz, const uint32_t* __restrict__ slot_ptr,
                                                        const uint32_t* __restrict__ contrib,
                                                        const double* __restrict__ staging,
-                                                       const uint8_t* __restrict__ flag, double* __restrict__ values)
+                                                       const uint8_t* __restrict__ flag, double* __restrict__ values,
+                                                       int accumulate)
 {
   const int64_t s = xcd_index(nnz);
   if (s >= nnz)
@@ -871,7 +988,7 @@ __global__ void __launch_bounds__(256) k_gather_values(int64_t nnz, const uint32
   else if (fl == 2)
     v = 1.;
   else
-    v = ordered_sum<8>(contrib, staging, slot_ptr[s], slot_ptr[s + 1]);
+    v = ordered_sum<8>(contrib, staging, slot_ptr[s], slot_ptr[s + 1], accumulate ? values[s] : 0.);
   values[s] = v;
 }
 
@@ -1087,8 +1204,7 @@ struct TableCache {
   DeviceArray<double> elem, line;
 };
 
-// (geometry, order) pairs with device kernels: order 1 everywhere, order 2 on quadrilaterals,
-// triangles and tetrahedra
+// (geometry, order) pairs with device kernels: orders 1 and 2 on every geometry
 template <class F>
 void dispatch_fe(int geom, int order, F&& f)
 {
@@ -1111,6 +1227,8 @@ void dispatch_fe(int geom, int order, F&& f)
         return f(std::integral_constant<int, HGDT_QUAD>{}, II{});
       case HGDT_TRI:
         return f(std::integral_constant<int, HGDT_TRI>{}, II{});
+      case HGDT_HEX:
+        return f(std::integral_constant<int, HGDT_HEX>{}, II{});
       case HGDT_TET:
         return f(std::integral_constant<int, HGDT_TET>{}, II{});
     }
@@ -1137,6 +1255,37 @@ void dispatch_geom(int geom, F&& f)
     default:
       HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "unknown geometry");
   }
+}
+
+// (geometry, order) pairs with intersection kernels: every geometry at order 1, simplices and quadrilaterals
+// at order 2 (a Q2 hexahedron's 27 basis functions on both sides of a face do not fit a lane)
+template <class F>
+void dispatch_faces(int geom, int order, F&& f)
+{
+  using I = std::integral_constant<int, 1>;
+  using II = std::integral_constant<int, 2>;
+  if (order == 1) {
+    switch (geom) {
+      case HGDT_QUAD:
+        return f(std::integral_constant<int, HGDT_QUAD>{}, I{});
+      case HGDT_TRI:
+        return f(std::integral_constant<int, HGDT_TRI>{}, I{});
+      case HGDT_HEX:
+        return f(std::integral_constant<int, HGDT_HEX>{}, I{});
+      case HGDT_TET:
+        return f(std::integral_constant<int, HGDT_TET>{}, I{});
+    }
+  } else if (order == 2) {
+    switch (geom) {
+      case HGDT_QUAD:
+        return f(std::integral_constant<int, HGDT_QUAD>{}, II{});
+      case HGDT_TRI:
+        return f(std::integral_constant<int, HGDT_TRI>{}, II{});
+      case HGDT_TET:
+        return f(std::integral_constant<int, HGDT_TET>{}, II{});
+    }
+  }
+  HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "no intersection kernels for this geometry and polynomial order");
 }
 
 } // namespace
@@ -1190,12 +1339,12 @@ void Operator::prepare_assembly()
   if (eforms.size() > size_t(MAX_EFORMS) || sources.size() > size_t(MAX_SOURCES) || cforms.size() > size_t(MAX_FFORMS)
       || bforms.size() > size_t(MAX_FFORMS))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many local forms appended");
-  if ((!cforms.empty() || !bforms.empty()) && space->order != 1)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for order 1 spaces");
+  if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->order == 2 &&
+      space->grid->ref.geom == HGDT_HEX)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for Q2 on quadrilaterals, not hexahedra");
   if (bsources.size() > size_t(MAX_BSOURCES))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
-  if (!bsources.empty() && space->order != 1)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "boundary functionals are provided for order 1 spaces");
+
   if (tables_ready)
     return;
   QTable etab, ltab;
@@ -1226,33 +1375,39 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     P.elem_base = M.elem_base;
     P.rhs_base = M.rhs_base;
     P.write_matrix = emat ? 1 : 0;
-    dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
-      hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)), dim3(eb),
-                         0, stream, P);
-    });
+    if (g.ref.geom == HGDT_HEX && op.space->order == 2) // 27 local DoFs: one lane per element row
+      hipLaunchKernelGGL((k_element_rows<HGDT_HEX, 2>), dim3(grid_for(g.ne * 27, eb)), dim3(eb), 0, stream, P);
+    else
+      dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
+        if constexpr (decltype(G)::value == HGDT_HEX && decltype(O)::value == 2)
+          HGDT_THROW(HGDT_ERR_OPERATOR, "Q2 hexahedra are assembled by row lanes");
+        else
+          hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)),
+                             dim3(eb), 0, stream, P);
+      });
     HGDT_HIP(hipGetLastError());
   }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
   if (matrix && !op.cforms.empty() && nin > 0) {
     C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
     C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
-    dispatch_geom(g.ref.geom, [&](auto G) {
-      hipLaunchKernelGGL(k_inner_local<decltype(G)::value>, dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_inner_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
     });
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && !op.bforms.empty() && nbd > 0) {
     B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
     B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
-    dispatch_geom(g.ref.geom, [&](auto G) {
-      hipLaunchKernelGGL(k_boundary_local<decltype(G)::value>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_boundary_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
     });
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
     hipLaunchKernelGGL(k_gather_values, dim3(xcd_grid(op.nnz, gb)), dim3(gb), 0, stream, op.nnz, M.d_slot_ptr.ptr,
                        M.d_contrib.ptr, op.d_staging.ptr, op.dirichlet ? op.d_slot_flag.ptr : nullptr,
-                       op.d_values.ptr);
+                       op.d_values.ptr, op.borrowed ? 1 : 0);
     HGDT_HIP(hipGetLastError());
   }
   if (!op.bsources.empty() && nbd > 0) {
@@ -1265,8 +1420,8 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
       BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
       BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
-    dispatch_geom(g.ref.geom, [&](auto G) {
-      hipLaunchKernelGGL(k_boundary_rhs<decltype(G)::value>, dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_boundary_rhs<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
     });
     HGDT_HIP(hipGetLastError());
   }

@@ -81,32 +81,68 @@ __device__ __forceinline__ void wave_flag(bool pred, int* flag)
 // sum of an instance's block partials (written by an earlier kernel) in block order
 // (each lane sums the partials b0 + lane, b0 + lane + B, ... in that order; the loads of U such
 // partials are issued together, so the prologue costs one memory round trip per U * B partials:
-// one round trip for up to 2048 single partials with B = 256)
+// one round trip for up to 2048 single partials with B = 256).  Split in two so that a kernel can issue
+// the first trip's loads together with its other loads and wait for all of them once:
+// Partials::issue loads the first trip, Partials::sum adds it (and any further trips) and reduces the block.
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
-{
-  constexpr int U = NV >= 8 ? 1 : 8 / NV; // partials per thread and trip: few registers, since callers keep
-                                          // their rows' loads in flight across the reduction
-  const int B = blockDim.x;
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-    tot[k] = 0.;
-  for (int base = b0 + threadIdx.x; base < b1; base += U * B) {
-    double v[U][NV];
+struct Partials {
+  static constexpr int U = NV >= 8 ? 1 : 8 / NV; // partials per thread and trip: few registers, since callers
+                                                 // keep their rows' loads in flight across the reduction
+  double v[U][NV];
+  int b0, b1;
+  // the loads are unconditional (a lane past the range reads the partial b0, or 0 of an empty range, and
+  // discards it in sum), so no branch separates them from the caller's other loads
+  __device__ __forceinline__ void issue(const double* __restrict__ part, int first, int last)
+  {
+    b0 = first, b1 = last;
+    const int base = b0 + int(threadIdx.x), B = blockDim.x;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int bb = base + u * B;
+      const int64_t src = (bb < b1) ? bb : b0;
 #pragma unroll
       for (int k = 0; k < NV; ++k)
-        v[u][k] = (bb < b1) ? part[int64_t(bb) * NV + k] : 0.;
+        v[u][k] = part[src * NV + k];
     }
+  }
+  __device__ __forceinline__ void sum(const double* __restrict__ part, double (&tot)[NV])
+  {
+    const int B = blockDim.x;
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int k = 0; k < NV; ++k)
+      tot[k] = 0.;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = b0 + int(threadIdx.x) + u * B < b1;
 #pragma unroll
       for (int k = 0; k < NV; ++k)
-        tot[k] += v[u][k];
+        tot[k] += in ? v[u][k] : 0.;
+    }
+    for (int base = b0 + int(threadIdx.x) + U * B; base < b1; base += U * B) {
+      double w[U][NV];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int bb = base + u * B;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          w[u][k] = (bb < b1) ? part[int64_t(bb) * NV + k] : 0.;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          tot[k] += w[u][k];
+    }
+    block_sum<NV>(tot);
   }
-  block_sum<NV>(tot);
+};
+
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int b0, int b1, double (&tot)[NV])
+{
+  Partials<NV> p;
+  p.issue(part, b0, b1);
+  p.sum(part, tot);
 }
 
 } // namespace

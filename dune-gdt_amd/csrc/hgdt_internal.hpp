@@ -61,6 +61,7 @@ template <class T>
 struct DeviceArray {
   T* ptr = nullptr;
   size_t count = 0;
+  bool owned = true; // false: memory of the caller (borrow), never freed here
   DeviceArray() = default;
   DeviceArray(const DeviceArray&) = delete;
   DeviceArray& operator=(const DeviceArray&) = delete;
@@ -70,14 +71,22 @@ struct DeviceArray {
   }
   void release()
   {
-    if (ptr)
+    if (ptr && owned)
       (void)hipFree(ptr);
-    ptr = nullptr, count = 0;
+    ptr = nullptr, count = 0, owned = true;
+  }
+  // use the caller's device memory of n elements (it must outlive this array's use)
+  void borrow(T* p, size_t n)
+  {
+    release();
+    ptr = p, count = n, owned = false;
   }
   void allocate(size_t n)
   {
     if (n == count && ptr)
       return;
+    if (!owned)
+      HGDT_THROW(HGDT_ERR_OPERATOR, "borrowed device memory cannot be reallocated");
     release();
     if (n == 0)
       return;
@@ -333,9 +342,17 @@ struct DirectInst {
 };
 
 struct Operator {
-  std::shared_ptr<Space> space;
+  std::shared_ptr<Space> space;  // the range space: rows (test functions), right-hand side, constraints
+  std::shared_ptr<Space> source; // the source space: columns (ansatz functions); the range space by default
   int stencil = HGDT_STENCIL_ELEMENT;
   int64_t n = 0, nnz = 0;
+  int64_t ncols = 0; // source space size (n for a square operator)
+  bool square() const { return source == space; }
+  // MatrixOperator on a caller's matrix (matrix-based.hh:279-293): its CSR pattern (host copy) and device
+  // values; assemble() adds to the values held (add_to_entry)
+  bool borrowed = false;
+  std::vector<int64_t> borrowed_row_ptr;
+  std::vector<int32_t> borrowed_cols;
   std::vector<int64_t> row_ptr;
   std::vector<int32_t> cols;
   DeviceArray<int64_t> d_row_ptr;

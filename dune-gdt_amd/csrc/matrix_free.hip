@@ -50,6 +50,20 @@ constexpr int MF_MAXF = 8;
 constexpr int TX = 32, TY = 8; // tile of the discontinuous kernel (elements): halo rows 1.3x the tile
 enum { MFK_CG_QUAD = 0, MFK_CG_HEX = 1, MFK_DG_QUAD = 2 };
 
+// the parameters of an appended form, in one record (a kernel's few forms then span few lines of its argument
+// block, which the kernel reads before anything else)
+struct MfElementForm {
+  const double* coef;
+  double value, scale;
+  int kind;
+};
+struct MfFaceForm {
+  const double* kappa;
+  const double* omega;
+  double kv, ov, param, scale;
+  int kind;
+};
+
 struct MfParams {
   int ninst;
   // per row block: first DoF and first element of the instance, cells n0 n1 n2, instance, first and last
@@ -58,19 +72,13 @@ struct MfParams {
   int64_t ref_stride; // doubles per instance of the reference data: h0 h1 1/h1 1/h0, element, face, boundary matrices
   int64_t eoff; // offset of the element matrices inside an instance's block (continuous spaces)
   int dirichlet;
-  int nef;
-  int ekind[MF_MAXF];
-  const double* ecoef[MF_MAXF];
-  double evalue[MF_MAXF], escale[MF_MAXF];
-  int ncf, nbf;
-  int fkind[2][MF_MAXF]; // [0] inner, [1] boundary forms
-  double fparam[2][MF_MAXF], fscale[2][MF_MAXF];
-  const double* fkappa[2][MF_MAXF];
-  double fkv[2][MF_MAXF];
-  const double* fomega[2][MF_MAXF];
-  double fov[2][MF_MAXF];
+  int nef, ncf, nbf;
   int nb; // row blocks (the solver's operator partition): one row or element per lane
+  const double* single; // the one coefficient field of every form (kernels that prefetch it), or NULL
   MfVectors v;
+  // the forms last: a kernel reads the first nef / ncf / nbf records of each array
+  MfElementForm ef[MF_MAXF];
+  MfFaceForm ff[2][MF_MAXF]; // [0] inner, [1] boundary forms
 };
 
 // coefficient of a form at element e: the field's value, or the constant
@@ -251,7 +259,7 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
     for (int f = 0; f < NEF; ++f)
 #pragma unroll
       for (int c = 0; c < NV; ++c)
-        kc[f][c] = valid[c] ? cf(P.ecoef[f], P.evalue[f], cell[c]) : 0.;
+        kc[f][c] = valid[c] ? cf(P.ef[f].coef, P.ef[f].value, cell[c]) : 0.;
   }
   if (MODE == MF_PCG && P.v.S[(P.v.par ^ 1) * P.ninst + inst].done)
     return;
@@ -261,12 +269,12 @@ __global__ void __launch_bounds__(256) k_mf_cg(MfParams P, const double* __restr
     y = DIAG ? 1. : s[NS / 2];
   } else {
     for (int f = 0; f < (NEF >= 0 ? NEF : P.nef); ++f) {
-      const double sc = P.escale[f];
+      const double sc = P.ef[f].scale;
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
         if (!valid[c])
           continue;
-        const double k = (NEF >= 0) ? kc[NEF >= 0 ? f : 0][c] : cf(P.ecoef[f], P.evalue[f], cell[c]);
+        const double k = (NEF >= 0) ? kc[NEF >= 0 ? f : 0][c] : cf(P.ef[f].coef, P.ef[f].value, cell[c]);
         const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
         const int l = (1 - cx) | ((1 - cy) << 1) | ((D == 3 ? (1 - cz) : 0) << 2);
         const double* __restrict__ rw = R + f * NV * NV + l * NV;
@@ -324,16 +332,27 @@ struct DgGeom {
   double h[2], ih[2]; // cell widths and their inverses
 };
 
-// coefficients of the element and of its neighbours (a field's value or the constant)
+// coefficients of the element and of its neighbours across sides 0 .. 3 (x-, x+, y-, y+): a field's value or
+// the constant.  PRE: every form's coefficient is one field (MfParams::single) or a constant, and that
+// field's values at the element and at its neighbours are loaded once, at the start of the kernel
+template <bool PRE>
 struct Coefs {
-  int64_t e; // the element
+  int64_t e;     // the element
+  int64_t en[4]; // its neighbours
+  double own_v = 0., nb_v[4] = {0., 0., 0., 0.};
   __device__ __forceinline__ double own(const double* f, double value) const
   {
-    return cf(f, value, e);
+    if constexpr (PRE)
+      return f ? own_v : value;
+    else
+      return cf(f, value, e);
   }
-  __device__ __forceinline__ double nb(const double* f, double value, int64_t en) const
+  __device__ __forceinline__ double nb(const double* f, double value, int side) const
   {
-    return cf(f, value, en);
+    if constexpr (PRE)
+      return f ? nb_v[side] : value;
+    else
+      return cf(f, value, en[side]);
   }
 };
 
@@ -360,12 +379,13 @@ __device__ __forceinline__ void wmul(double c6, const double (&v)[2], double (&o
 }
 
 // element forms
-__device__ __forceinline__ void dg_element(const MfParams& P, int nef, const Coefs& K, const DgGeom& g, const double* a,
+template <class KC>
+__device__ __forceinline__ void dg_element(const MfParams& P, int nef, const KC& K, const DgGeom& g, const double* a,
                                            double* y)
 {
   for (int f = 0; f < nef; ++f) {
-    const double c = P.escale[f] * K.own(P.ecoef[f], P.evalue[f]);
-    if (P.ekind[f] == EF_LAPLACE) {
+    const double c = P.ef[f].scale * K.own(P.ef[f].coef, P.ef[f].value);
+    if (P.ef[f].kind == EF_LAPLACE) {
       const double cx = c * (g.h[1] * g.ih[0]) * (1. / 6.), cy = c * (g.h[0] * g.ih[1]) * (1. / 6.);
       const double dx[2] = {a[1] - a[0], a[3] - a[2]}, dy[2] = {a[2] - a[0], a[3] - a[1]};
       const double wx0 = 2. * dx[0] + dx[1], wx1 = dx[0] + 2. * dx[1];
@@ -387,9 +407,9 @@ __device__ __forceinline__ void dg_element(const MfParams& P, int nef, const Coe
 }
 
 // the inner forms of side SD (0 x-, 1 x+, 2 y-, 3 y+; across x+ / y+ this element is the inside)
-template <int SD, int NCF>
-__device__ __forceinline__ void dg_inner(const MfParams& P, int ncf, const Coefs& K, int64_t en, const DgGeom& g,
-                                         const double* a, const double* bv, double* y)
+template <int SD, int NCF, class KC>
+__device__ __forceinline__ void dg_inner(const MfParams& P, int ncf, const KC& K, const DgGeom& g, const double* a,
+                                         const double* bv, double* y)
 {
   constexpr int O = SD >> 1;
   constexpr bool INSIDE = SD & 1;
@@ -405,15 +425,15 @@ __device__ __forceinline__ void dg_inner(const MfParams& P, int ncf, const Coefs
   }
   wmul(c6, J, YJ);
   for (int f = 0; f < (NCF >= 0 ? NCF : ncf); ++f) {
-    const double sc = P.fscale[0][f], prm = P.fparam[0][f];
-    const double* __restrict__ of = P.fomega[0][f];
-    const double os = K.own(of, P.fov[0][f]), on = K.nb(of, P.fov[0][f], en);
+    const double sc = P.ff[0][f].scale, prm = P.ff[0][f].param;
+    const double* __restrict__ of = P.ff[0][f].omega;
+    const double os = K.own(of, P.ff[0][f].ov), on = K.nb(of, P.ff[0][f].ov, SD);
     // weights of the intersection from its inside (-) and outside (+) elements
     const double dm = INSIDE ? os : on, dp = INSIDE ? on : os;
     const double inv = recip(dp + dm);
-    if (P.fkind[0][f] == FF_INNER_COUPLING) {
-      const double* __restrict__ kf = P.fkappa[0][f];
-      const double ks = K.own(kf, P.fkv[0][f]), kn = K.nb(kf, P.fkv[0][f], en);
+    if (P.ff[0][f].kind == FF_INNER_COUPLING) {
+      const double* __restrict__ kf = P.ff[0][f].kappa;
+      const double ks = K.own(kf, P.ff[0][f].kv), kn = K.nb(kf, P.ff[0][f].kv, SD);
       const double kin = INSIDE ? ks : kn, kout = INSIDE ? kn : ks;
       const double wm = dp * inv, wp = dm * inv;
       double F[2], YF[2];
@@ -438,8 +458,8 @@ __device__ __forceinline__ void dg_inner(const MfParams& P, int ncf, const Coefs
 }
 
 // the boundary forms on side SD (outward normal +-e_o)
-template <int SD, int NBF>
-__device__ __forceinline__ void dg_boundary(const MfParams& P, int nbf, const Coefs& K, const DgGeom& g, const double* a,
+template <int SD, int NBF, class KC>
+__device__ __forceinline__ void dg_boundary(const MfParams& P, int nbf, const KC& K, const DgGeom& g, const double* a,
                                             double* y)
 {
   constexpr int O = SD >> 1, SIDE = SD & 1;
@@ -454,50 +474,53 @@ __device__ __forceinline__ void dg_boundary(const MfParams& P, int nbf, const Co
   wmul(c6, T, YT);
   wmul(c6, Dn, YD);
   for (int f = 0; f < (NBF >= 0 ? NBF : nbf); ++f) {
-    const double sc = P.fscale[1][f], prm = P.fparam[1][f];
-    if (P.fkind[1][f] == FF_DIRICHLET_COUPLING) {
-      const double k = K.own(P.fkappa[1][f], P.fkv[1][f]);
+    const double sc = P.ff[1][f].scale, prm = P.ff[1][f].param;
+    if (P.ff[1][f].kind == FF_DIRICHLET_COUPLING) {
+      const double k = K.own(P.ff[1][f].kappa, P.ff[1][f].kv);
       add_face<O, SIDE>(-sc * k, YD, y);
       add_dn<O>(-sc * prm * k * sign * ihn, YT, y);
     } else {
-      add_face<O, SIDE>(sc * (prm * K.own(P.fomega[1][f], P.fov[1][f])) * ilen, YT, y);
+      add_face<O, SIDE>(sc * (prm * K.own(P.ff[1][f].omega, P.ff[1][f].ov)) * ilen, YT, y);
     }
   }
 }
 
 // y = (local operator of element e) applied to its own values a and its neighbours' bv
-template <int NEF, int NCF, int NBF>
-__device__ __forceinline__ void dg_local(const MfParams& P, int nef, int ncf, int nbf, const Coefs& K, uint32_t nx,
+template <int NEF, int NCF, int NBF, class KC>
+__device__ __forceinline__ void dg_local(const MfParams& P, int nef, int ncf, int nbf, const KC& K,
                                          const bool (&has)[4], const DgGeom& g, const double* a,
                                          const double (&bv)[4][4], double* y)
 {
-  const int64_t e = K.e;
   dg_element(P, NEF >= 0 ? NEF : nef, K, g, a, y);
   if (has[0])
-    dg_inner<0, NCF>(P, ncf, K, e - 1, g, a, bv[0], y);
+    dg_inner<0, NCF>(P, ncf, K, g, a, bv[0], y);
   else
     dg_boundary<0, NBF>(P, nbf, K, g, a, y);
   if (has[1])
-    dg_inner<1, NCF>(P, ncf, K, e + 1, g, a, bv[1], y);
+    dg_inner<1, NCF>(P, ncf, K, g, a, bv[1], y);
   else
     dg_boundary<1, NBF>(P, nbf, K, g, a, y);
   if (has[2])
-    dg_inner<2, NCF>(P, ncf, K, e - nx, g, a, bv[2], y);
+    dg_inner<2, NCF>(P, ncf, K, g, a, bv[2], y);
   else
     dg_boundary<2, NBF>(P, nbf, K, g, a, y);
   if (has[3])
-    dg_inner<3, NCF>(P, ncf, K, e + nx, g, a, bv[3], y);
+    dg_inner<3, NCF>(P, ncf, K, g, a, bv[3], y);
   else
     dg_boundary<3, NBF>(P, nbf, K, g, a, y);
 }
 
 // discontinuous Q1 on quadrilaterals: a block works on a tile of TX x TY elements, one lane per element (its
 // 4 consecutive DoFs).  The source values of the tile and of a one-element halo (D^{-1/2} p for the solver)
-// are first staged in LDS with coalesced loads (the 4-DoF vectors of a row of elements are contiguous), so
-// each lane reads its own and its 4 neighbours' values from LDS.  The diagonal is the local operator applied
-// to the 4 unit vectors.  NEF, NCF, NBF >= 0: form counts known at compile time (loops unrolled); -1: runtime
-// loops.  VEC: the vectors are 16-byte aligned (16-byte accesses).
-template <int MODE, int NEF, int NCF, int NBF, bool VEC>
+// are staged in LDS with coalesced loads (the 4-DoF vectors of a row of elements are contiguous), so each
+// lane reads its own and its 4 neighbours' values from LDS.  The diagonal is the local operator applied to
+// the 4 unit vectors.  NEF, NCF, NBF >= 0: form counts known at compile time (loops unrolled); -1: runtime
+// loops.  VEC: the vectors are 16-byte aligned (16-byte accesses).  PRE: the coefficients come from one field
+// (Coefs).
+// Latency: every load that depends only on the block's record is issued before any result is waited for
+// (the staged vectors, the coefficients, the solver's scalars and partials), so the kernel pays about one
+// memory round trip before its arithmetic instead of one per stage.
+template <int MODE, int NEF, int NCF, int NBF, bool VEC, bool PRE>
 __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __restrict__ ref)
 {
   constexpr bool DIAG = (MODE == MF_DIAG);
@@ -523,39 +546,125 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
   const bool row = ex < nx && ey < ny;
   const uint32_t c = ey * nx + ex;
   const int64_t i0 = d0 + 4 * int64_t(c); // first DoF of the lane's element
+  const int64_t e = e0 + c;
+  const bool has[4] = {row && ex > 0u, row && ex + 1u < nx, row && ey > 0u, row && ey + 1u < ny};
+  Coefs<PRE> K;
+  K.e = e;
+  K.en[0] = e - 1, K.en[1] = e + 1, K.en[2] = e - int64_t(nx), K.en[3] = e + int64_t(nx);
+  if constexpr (PRE) {
+    if (P.single) {
+      K.own_v = row ? P.single[e] : 0.;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        K.nb_v[s] = has[s] ? P.single[K.en[s]] : 0.;
+    }
+  }
+  // staged entries: columns x0 + xs .. x0 + xe - 1 of the rows y0 - 1 .. y0 + TY (clipped to the instance),
+  // U accesses of W doubles per element and array; entry u of the block is (row u / per_row, access
+  // u % per_row), lane t takes the entries t + k TX TY
+  constexpr int U = VEC ? 2 : 4, W = 4 / U;
+  constexpr int NR = ((TY + 2) * (TX + 2) * U + TX * TY - 1) / (TX * TY);
+  const int xs = x0 > 0u ? -1 : 0;
+  const int xe = int(min(uint32_t(TX + 1), nx - x0));
+  const int per_row = (xe - xs) * U;
+  auto entry = [&](int k, int& r, int& col, int& k0, int64_t& dof) {
+    const int u = int(threadIdx.x) + k * TX * TY;
+    r = u / per_row;
+    const int w = u % per_row;
+    col = xs + w / U; // relative to x0
+    k0 = W * (w % U);
+    const int64_t yy = int64_t(y0) - 1 + r;
+    dof = d0 + 4 * (yy * nx + x0 + col) + k0;
+    return u < (TY + 2) * per_row && yy >= 0 && yy < int64_t(ny);
+  };
+  auto ld = [](const double* __restrict__ a, int64_t i, double (&o)[W]) {
+    if constexpr (VEC) {
+      const double2 v = *reinterpret_cast<const double2*>(a + i);
+      o[0] = v.x, o[1] = v.y;
+    } else {
+      o[0] = a[i];
+    }
+  };
+  auto st = [](double* __restrict__ a, int64_t i, const double (&v)[W]) {
+    if constexpr (VEC)
+      *reinterpret_cast<double2*>(a + i) = make_double2(v[0], v[1]);
+    else
+      a[i] = v[0];
+  };
+  // the staged vectors' loads: the source (DIR: r, and the previous p, read even at the first step, where it
+  // is not used) and D^{-1/2}
+  double lx[NR][W], lp[NR][W], lh[NR][W];
+  if constexpr (!DIAG) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      int r, col, k0;
+      int64_t dof;
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        lx[k][j] = lp[k][j] = lh[k][j] = 0.;
+      if (entry(k, r, col, k0, dof)) {
+        ld(DIR ? P.v.r : P.v.x, dof, lx[k]);
+        if constexpr (DIR)
+          ld(P.v.x, dof, lp[k]);
+        if constexpr (SCALED)
+          ld(P.v.dh, dof, lh[k]);
+      }
+    }
+  }
+  // DIR: the step's scalars and the instance's partial range, read after the staged vectors' loads are
+  // issued (which need only the block's record) and before any store of the kernel
+  PcgScalars S{};
+  int pb0 = 0, pb1 = 0;
+  if constexpr (DIR) {
+    S = P.v.S[P.v.par * P.ninst + inst];
+    pb0 = P.v.v_ib0[inst], pb1 = P.v.v_ib0[inst + 1];
+  }
   // DIR: the direction update of the step (as k_cg_direction: stopping test on the previous update's
   // partials, p = r + beta p, and every second step the iterate's two deferred updates) on the block's
   // elements.  The staged source is formed from r and the previous p, which live in another buffer than
   // the new p, so a neighbouring block's writes never reach this block's halo.
   double alpha = 0., alpha_prev = 0., beta = 0.;
   bool first = false, pair = false; // pair: the iterate's two pending updates are due (even step >= 2)
+  double lo[DIR ? NR : 1][W], lxh[DIR ? NR : 1][W]; // pair steps: p_{j-2} and the iterate of the own entries
   if constexpr (DIR) {
-    const PcgScalars* cur = P.v.S + P.v.par * P.ninst + inst;
     PcgScalars* nxt = P.v.S + (P.v.par ^ 1) * P.ninst + inst;
     const bool writer = (b == P.v.a_ib0[inst] && threadIdx.x == 0);
-    const int done = pcg_state(*cur);
+    const int64_t it = S.iters;
+    first = (it == 0);
+    pair = !first && (it & 1) == 0;
+    const int done = pcg_state(S);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        lo[k][j] = lxh[k][j] = 0.;
+      int r, col, k0;
+      int64_t dof;
+      if (!done && pair && entry(k, r, col, k0, dof) && r >= 1 && r <= TY && col >= 0 && col < TX) {
+        ld(P.v.p_out, dof, lo[k]);
+        ld(P.v.xh, dof, lxh[k]);
+      }
+    }
+    Partials<2> pt;
+    pt.issue(P.v.part2, pb0, done ? pb0 : pb1);
     if (done) {
       if (writer) {
-        *nxt = *cur;
+        *nxt = S;
         nxt->done = done;
       }
       return;
     }
-    const int64_t it = cur->iters;
-    const double rho_prev = cur->rz, tol2 = cur->tol2;
-    alpha = cur->alpha, alpha_prev = cur->alpha_prev;
-    first = (it == 0);
-    pair = !first && (it & 1) == 0;
+    alpha = S.alpha, alpha_prev = S.alpha_prev;
     double t[2];
-    reduce_partials<2>(P.v.part2, P.v.v_ib0[inst], P.v.v_ib0[inst + 1], t);
-    const int status = (t[1] <= tol2) ? 1 : (it >= P.v.max_iter) ? 2 : 0;
-    beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
+    pt.sum(P.v.part2, t);
+    const int status = (t[1] <= S.tol2) ? 1 : (it >= P.v.max_iter) ? 2 : 0;
+    beta = (status == 0 && !first) ? t[0] / S.rz : 0.;
     if (writer) {
-      nxt->bb = cur->bb;
-      nxt->tol2 = tol2;
+      nxt->bb = S.bb;
+      nxt->tol2 = S.tol2;
       nxt->alpha = alpha;
       nxt->alpha_prev = alpha;
-      nxt->pq = cur->pq;
+      nxt->pq = S.pq;
       nxt->rr = t[1];
       nxt->rz = t[0];
       nxt->beta = beta;
@@ -585,84 +694,41 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
     }
   }
   if constexpr (!DIAG) {
-    // staged columns x0 + xs .. x0 + xe - 1 of the rows y0 - 1 .. y0 + TY (clipped to the instance)
-    const int xs = x0 > 0u ? -1 : 0;
-    const int xe = int(min(uint32_t(TX + 1), nx - x0));
-    const int ncols = xe - xs;
-    constexpr int U = VEC ? 2 : 4; // accesses per element and array
-    const int per_row = ncols * U;
-    for (int u = threadIdx.x; u < (TY + 2) * per_row; u += TX * TY) {
-      const int r = u / per_row, w = u % per_row;
-      const int64_t yy = int64_t(y0) - 1 + r;
-      if (yy < 0 || yy >= int64_t(ny))
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      int r, col, k0;
+      int64_t dof;
+      if (!entry(k, r, col, k0, dof))
         continue;
-      const int col = xs + w / U; // relative to x0
-      const int k0 = (4 / U) * (w % U);
-      const int64_t dof = d0 + 4 * (yy * nx + x0 + col) + k0;
-      // DIR: an entry of the tile itself (not of the halo)
-      const bool own = DIR && r >= 1 && r <= TY && col >= 0 && col < TX;
-      if constexpr (VEC) {
-        double2 xv, pv;
-        if constexpr (DIR) {
-          xv = *reinterpret_cast<const double2*>(P.v.r + dof);
-          if (!first) {
-            pv = *reinterpret_cast<const double2*>(P.v.x + dof);
-            xv.x = fma(beta, pv.x, xv.x), xv.y = fma(beta, pv.y, xv.y);
+      double xv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        xv[j] = (DIR && !first) ? fma(beta, lp[k][j], lx[k][j]) : lx[k][j];
+      if constexpr (DIR) {
+        // an entry of the tile itself (not of the halo): its new p, and the iterate on pair steps
+        if (r >= 1 && r <= TY && col >= 0 && col < TX) {
+          if (pair) {
+            double xh[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+              xh[j] = fma(alpha, lp[k][j], fma(alpha_prev, lo[k][j], lxh[k][j]));
+            st(P.v.xh, dof, xh);
           }
-        } else {
-          xv = *reinterpret_cast<const double2*>(P.v.x + dof);
+          st(P.v.p_out, dof, xv);
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            sh[r - 1][col][k0 + j] = lh[k][j];
         }
-        if (own) {
-          if (pair) { // p_{j-2} is read before p_j replaces it
-            const double2 ov = *reinterpret_cast<const double2*>(P.v.p_out + dof);
-            double2 xh = *reinterpret_cast<const double2*>(P.v.xh + dof);
-            xh.x = fma(alpha, pv.x, fma(alpha_prev, ov.x, xh.x));
-            xh.y = fma(alpha, pv.y, fma(alpha_prev, ov.y, xh.y));
-            *reinterpret_cast<double2*>(P.v.xh + dof) = xh;
-          }
-          *reinterpret_cast<double2*>(P.v.p_out + dof) = xv;
-        }
-        if constexpr (SCALED) {
-          const double2 hv = *reinterpret_cast<const double2*>(P.v.dh + dof);
-          xv.x *= hv.x, xv.y *= hv.y;
-          if (own)
-            *reinterpret_cast<double2*>(&sh[r - 1][col][k0]) = hv;
-        }
-        *reinterpret_cast<double2*>(&sp[r][col + 1][k0]) = xv;
-      } else {
-        double xv, pv = 0.;
-        if constexpr (DIR) {
-          xv = P.v.r[dof];
-          if (!first) {
-            pv = P.v.x[dof];
-            xv = fma(beta, pv, xv);
-          }
-        } else {
-          xv = P.v.x[dof];
-        }
-        if (own) {
-          if (pair)
-            P.v.xh[dof] = fma(alpha, pv, fma(alpha_prev, P.v.p_out[dof], P.v.xh[dof]));
-          P.v.p_out[dof] = xv;
-        }
-        if constexpr (SCALED) {
-          const double hv = P.v.dh[dof];
-          xv *= hv;
-          if (own)
-            sh[r - 1][col][k0] = hv;
-        }
-        sp[r][col + 1][k0] = xv;
       }
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        sp[r][col + 1][k0 + j] = SCALED ? xv[j] * lh[k][j] : xv[j];
     }
     __syncthreads();
   }
-  Coefs K;
   const double* __restrict__ R = ref + int64_t(inst) * P.ref_stride;
   DgGeom g;
   g.h[0] = R[0], g.h[1] = R[1], g.ih[0] = R[3], g.ih[1] = R[2];
-  const int64_t e = e0 + c;
-  K.e = e;
-  const bool has[4] = {row && ex > 0u, row && ex + 1u < nx, row && ey > 0u, row && ey + 1u < ny};
   double y[4] = {0., 0., 0., 0.};
   if constexpr (DIAG) {
     // diagonal: column k of the element's own block, entry k
@@ -672,7 +738,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       for (int k = 0; k < 4; ++k) {
         double a[4] = {0., 0., 0., 0.}, yk[4] = {0., 0., 0., 0.};
         a[k] = 1.;
-        dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, nx, has, g, a, z, yk);
+        dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, has, g, a, z, yk);
         y[k] = yk[k];
       }
   } else {
@@ -686,7 +752,7 @@ __global__ void __launch_bounds__(TX * TY) k_mf_dg(MfParams P, const double* __r
       bv[3][k] = has[3] ? sp[ty + 2][tx + 1][k] : 0.;
     }
     if (row)
-      dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, nx, has, g, a, bv, y);
+      dg_local<NEF, NCF, NBF>(P, nef, ncf, nbf, K, has, g, a, bv, y);
   }
   double part = 0.;
   if (row) {
@@ -759,6 +825,8 @@ void mf_setup(Operator& op)
   const Space& s = *op.space;
   const Grid& g = *s.grid;
   SolverFormat& F = op.fmt;
+  if (op.borrowed || !op.square())
+    unsupported("an operator on a caller's matrix or between two spaces keeps its matrix");
   if (g.lattice.empty() || int(g.lattice.size()) != g.ninst())
     unsupported("needs a structured grid or a union of structured grids");
   if (g.ref.geom != HGDT_QUAD && g.ref.geom != HGDT_HEX)
@@ -960,20 +1028,35 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
   P.nef = int(op.eforms.size());
   for (int f = 0; f < P.nef; ++f) {
     const auto& E = op.eforms[f];
-    P.ekind[f] = E.kind, P.ecoef[f] = E.field, P.evalue[f] = E.value, P.escale[f] = E.scaling;
+    P.ef[f].kind = E.kind, P.ef[f].coef = E.field, P.ef[f].value = E.value, P.ef[f].scale = E.scaling;
   }
   P.ncf = int(op.cforms.size()), P.nbf = int(op.bforms.size());
   for (int grp = 0; grp < 2; ++grp) {
     const auto& forms = grp ? op.bforms : op.cforms;
     for (size_t f = 0; f < forms.size(); ++f) {
       const auto& Fm = forms[f];
-      P.fkind[grp][f] = Fm.kind, P.fparam[grp][f] = Fm.param, P.fscale[grp][f] = Fm.scaling;
-      P.fkappa[grp][f] = Fm.kappa, P.fkv[grp][f] = Fm.kappa_value;
-      P.fomega[grp][f] = Fm.omega, P.fov[grp][f] = Fm.omega_value;
+      P.ff[grp][f].kind = Fm.kind, P.ff[grp][f].param = Fm.param, P.ff[grp][f].scale = Fm.scaling;
+      P.ff[grp][f].kappa = Fm.kappa, P.ff[grp][f].kv = Fm.kappa_value;
+      P.ff[grp][f].omega = Fm.omega, P.ff[grp][f].ov = Fm.omega_value;
     }
   }
   P.nb = v.nb >= 0 ? v.nb : int(F.A.nblocks);
   P.v = v;
+  // the coefficient field shared by every form, if there is one (every other coefficient a constant)
+  bool one_field = true;
+  auto see = [&](const double* f) {
+    if (f && P.single && f != P.single)
+      one_field = false;
+    else if (f)
+      P.single = f;
+  };
+  for (const auto& E : op.eforms)
+    see(E.field);
+  for (const auto* forms : {&op.cforms, &op.bforms})
+    for (const auto& Fm : *forms)
+      see(Fm.kappa), see(Fm.omega);
+  if (!one_field)
+    P.single = nullptr;
   const unsigned grid = hw_grid(P.nb);
   const int block = F.A.block; // one row / element per lane (the partition is built so)
   const bool ok = (F.mf_kind == MFK_DG_QUAD)
@@ -1005,18 +1088,21 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
     pick(k_mf_cg<3, MF_APPLY, 1>, k_mf_cg<3, MF_PCG, 1>, k_mf_cg<3, MF_DIAG, 1>, nullptr);
   else if (F.mf_kind == MFK_CG_HEX)
     pick(k_mf_cg<3, MF_APPLY, -1>, k_mf_cg<3, MF_PCG, -1>, k_mf_cg<3, MF_DIAG, -1>, nullptr);
+  else if (F.mf_kind == MFK_DG_QUAD && swipdg && vec && one_field)
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true, true>, k_mf_dg<MF_PCG, 1, 2, 2, true, true>,
+         k_mf_dg<MF_DIAG, 1, 2, 2, true, true>, kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, true, true>));
   else if (F.mf_kind == MFK_DG_QUAD && swipdg && vec)
-    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true>, k_mf_dg<MF_PCG, 1, 2, 2, true>, k_mf_dg<MF_DIAG, 1, 2, 2, true>,
-         kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, true>));
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, true, false>, k_mf_dg<MF_PCG, 1, 2, 2, true, false>,
+         k_mf_dg<MF_DIAG, 1, 2, 2, true, false>, kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, true, false>));
   else if (F.mf_kind == MFK_DG_QUAD && swipdg)
-    pick(k_mf_dg<MF_APPLY, 1, 2, 2, false>, k_mf_dg<MF_PCG, 1, 2, 2, false>, k_mf_dg<MF_DIAG, 1, 2, 2, false>,
-         kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, false>));
+    pick(k_mf_dg<MF_APPLY, 1, 2, 2, false, false>, k_mf_dg<MF_PCG, 1, 2, 2, false, false>,
+         k_mf_dg<MF_DIAG, 1, 2, 2, false, false>, kv(k_mf_dg<MF_PCG_DIR, 1, 2, 2, false, false>));
   else if (F.mf_kind == MFK_DG_QUAD && vec)
-    pick(k_mf_dg<MF_APPLY, -1, -1, -1, true>, k_mf_dg<MF_PCG, -1, -1, -1, true>,
-         k_mf_dg<MF_DIAG, -1, -1, -1, true>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, true>));
+    pick(k_mf_dg<MF_APPLY, -1, -1, -1, true, false>, k_mf_dg<MF_PCG, -1, -1, -1, true, false>,
+         k_mf_dg<MF_DIAG, -1, -1, -1, true, false>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, true, false>));
   else if (F.mf_kind == MFK_DG_QUAD)
-    pick(k_mf_dg<MF_APPLY, -1, -1, -1, false>, k_mf_dg<MF_PCG, -1, -1, -1, false>,
-         k_mf_dg<MF_DIAG, -1, -1, -1, false>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, false>));
+    pick(k_mf_dg<MF_APPLY, -1, -1, -1, false, false>, k_mf_dg<MF_PCG, -1, -1, -1, false, false>,
+         k_mf_dg<MF_DIAG, -1, -1, -1, false, false>, kv(k_mf_dg<MF_PCG_DIR, -1, -1, -1, false, false>));
   else
     HGDT_THROW(HGDT_ERR_OPERATOR, "matrix-free data not prepared");
   if (!k)

@@ -379,14 +379,18 @@ static Adjacency dof_adjacency(const Space& s)
   return A;
 }
 
+// sparsity pattern of the stencil (make_element_sparsity_pattern and friends, tools/sparsity-pattern.hh):
+// rows are the range space's DoFs, the columns of row i the source space's DoFs of the elements holding i
+// (and, with an intersection stencil, of their neighbours).  On a caller's matrix the caller's pattern is
+// the operator's, and it must hold every entry of the stencil's.
 void Operator::build_pattern()
 {
   const Space& s = *space;
+  const Space& src = *source;
   const Grid& g = *s.grid;
   const int ls = s.local_size, nf = g.ref.nfaces;
-  n = s.size;
+  n = s.size, ncols = src.size;
   const Adjacency A = dof_adjacency(s);
-  row_ptr.assign(n + 1, 0);
   std::vector<int32_t> counts(n);
   auto row_columns = [&](int64_t i, std::vector<int32_t>& c) {
     c.clear();
@@ -394,40 +398,75 @@ void Operator::build_pattern()
       const int64_t e = A.elem[p];
       if (stencil != HGDT_STENCIL_INTERSECTION)
         for (int j = 0; j < ls; ++j)
-          c.push_back(s.dofs[e * ls + j]);
+          c.push_back(src.dofs[e * ls + j]);
       if (stencil != HGDT_STENCIL_ELEMENT)
         for (int f = 0; f < nf; ++f) {
           const int64_t o = g.nb[e * nf + f];
           if (o >= 0)
             for (int j = 0; j < ls; ++j)
-              c.push_back(s.dofs[o * ls + j]);
+              c.push_back(src.dofs[o * ls + j]);
         }
     }
     std::sort(c.begin(), c.end());
     c.erase(std::unique(c.begin(), c.end()), c.end());
   };
-  parallel_for(n, [&](int64_t b, int64_t e) {
-    std::vector<int32_t> c;
-    for (int64_t i = b; i < e; ++i) {
-      row_columns(i, c);
-      counts[i] = int32_t(c.size());
-    }
-  });
-  for (int64_t i = 0; i < n; ++i)
-    row_ptr[i + 1] = row_ptr[i] + counts[i];
-  nnz = row_ptr[n];
-  cols.resize(nnz);
-  parallel_for(n, [&](int64_t b, int64_t e) {
-    std::vector<int32_t> c;
-    for (int64_t i = b; i < e; ++i) {
-      row_columns(i, c);
-      std::copy(c.begin(), c.end(), cols.begin() + row_ptr[i]);
-    }
-  });
+  if (borrowed) {
+    const std::vector<int64_t>& rp = borrowed_row_ptr;
+    const std::vector<int32_t>& cc = borrowed_cols;
+    if (int64_t(rp.size()) != n + 1 || rp[0] != 0)
+      HGDT_THROW(HGDT_ERR_SHAPES_DO_NOT_MATCH, "the matrix needs one row per DoF of the range space");
+    std::atomic<int> bad{0}; // 1: row pointer / columns malformed, 2: a stencil entry is missing
+    parallel_for(n, [&](int64_t b, int64_t e) {
+      std::vector<int32_t> c;
+      for (int64_t i = b; i < e && !bad; ++i) {
+        const int64_t r0 = rp[i], r1 = rp[i + 1];
+        if (r1 < r0 || r1 > int64_t(cc.size())) {
+          bad = 1;
+          break;
+        }
+        for (int64_t t = r0; t < r1; ++t)
+          if (cc[t] < 0 || cc[t] >= ncols || (t > r0 && cc[t] <= cc[t - 1]))
+            bad = 1;
+        row_columns(i, c);
+        for (int32_t col : c)
+          if (!std::binary_search(cc.begin() + r0, cc.begin() + r1, col))
+            bad = 2;
+      }
+    });
+    if (bad == 1 || rp[n] != int64_t(cc.size()))
+      HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT,
+                 "the matrix pattern needs ascending row pointers and ascending, unique, valid columns per row");
+    if (bad == 2)
+      HGDT_THROW(HGDT_ERR_SHAPES_DO_NOT_MATCH, "the matrix pattern does not hold every entry of the stencil");
+    row_ptr = rp, cols = cc;
+    nnz = row_ptr[n];
+  } else {
+    row_ptr.assign(n + 1, 0);
+    parallel_for(n, [&](int64_t b, int64_t e) {
+      std::vector<int32_t> c;
+      for (int64_t i = b; i < e; ++i) {
+        row_columns(i, c);
+        counts[i] = int32_t(c.size());
+      }
+    });
+    for (int64_t i = 0; i < n; ++i)
+      row_ptr[i + 1] = row_ptr[i] + counts[i];
+    nnz = row_ptr[n];
+    cols.resize(nnz);
+    parallel_for(n, [&](int64_t b, int64_t e) {
+      std::vector<int32_t> c;
+      for (int64_t i = b; i < e; ++i) {
+        row_columns(i, c);
+        std::copy(c.begin(), c.end(), cols.begin() + row_ptr[i]);
+      }
+    });
+  }
   d_row_ptr.upload(row_ptr);
   d_cols.upload(cols);
-  d_values.allocate(std::max<int64_t>(nnz, 1));
-  HGDT_HIP(hipMemset(d_values.ptr, 0, nnz * sizeof(double)));
+  if (!borrowed) {
+    d_values.allocate(std::max<int64_t>(nnz, 1));
+    HGDT_HIP(hipMemset(d_values.ptr, 0, nnz * sizeof(double)));
+  }
   d_rhs.allocate(std::max<int64_t>(n, 1));
   HGDT_HIP(hipMemset(d_rhs.ptr, 0, n * sizeof(double)));
 }
@@ -443,7 +482,8 @@ void Operator::build_pattern()
 // ----------------------------------------------------------------------------
 void Operator::build_scatter_maps()
 {
-  const Space& s = *space;
+  const Space& s = *space;   // rows (test functions)
+  const Space& src = *source; // columns (ansatz functions)
   const Grid& g = *s.grid;
   const int ls = s.local_size, nf = g.ref.nfaces;
   const int64_t ne = g.ne, k2 = int64_t(ls) * ls;
@@ -497,7 +537,7 @@ void Operator::build_scatter_maps()
     for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
       const int64_t e = A.elem[p];
       const int li = A.local[p];
-      const int32_t* de = &s.dofs[e * ls];
+      const int32_t* de = &src.dofs[e * ls];
       if (has_e)
         for (int lj = 0; lj < ls; ++lj)
           out.push_back({e * (nf + 1), uint32_t(M.elem_base + tri(li, lj) * ne + e), de[lj]});
@@ -506,7 +546,7 @@ void Operator::build_scatter_maps()
         if (o >= 0 && has_c) {
           if (e < o) {
             const int64_t fi = inner_index[e * nf + f];
-            const int32_t* dout = &s.dofs[o * ls];
+            const int32_t* dout = &src.dofs[o * ls];
             const int64_t key = e * (nf + 1) + 1 + f;
             for (int lj = 0; lj < ls; ++lj) // in_in
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 0 * k2 + li * ls + lj), de[lj]});
@@ -515,7 +555,7 @@ void Operator::build_scatter_maps()
           } else {
             const int fo = g.nbface[e * nf + f];
             const int64_t fi = inner_index[o * nf + fo];
-            const int32_t* din = &s.dofs[o * ls];
+            const int32_t* din = &src.dofs[o * ls];
             const int64_t key = o * (nf + 1) + 1 + fo;
             for (int lj = 0; lj < ls; ++lj) // out_in
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 2 * k2 + li * ls + lj), din[lj]});

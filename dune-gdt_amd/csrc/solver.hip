@@ -643,51 +643,57 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
   const int b = active_block(P.v_nb, P.v_list, P.rev);
   if (b < 0)
     return;
-  const int inst = P.v_inst[b];
-  const PcgScalars* cur = P.S + par * P.ninst + inst;
-  PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
-  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
-  const int done = pcg_state(*cur);
-  if (done) {
-    if (writer) {
-      *nxt = *cur;
-      nxt->done = done;
-    }
-    return;
-  }
-  const int64_t it = cur->iters;
-  const double rho_prev = cur->rz, alpha = cur->alpha, alpha_prev = cur->alpha_prev, tol2 = cur->tol2;
-  const bool first = (it == 0);
-  const bool pair = !first && (it & 1) == 0; // both pending updates of the iterate are due
+  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
   const double* __restrict__ pin = par ? P.p2 : P.p; // p_{j-1}
   double* __restrict__ pout = par ? P.p : P.p2;      // p_j, over p_{j-2}
-  const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-  // the block's rows of r and p_{j-1} are loaded before the scalars are reduced, so the loads overlap the
-  // reduction (r is read also at the step that finds the instance converged, and then not used); those
-  // of the iterate's update after it (every second step, few registers across the reduction)
+  // every load that needs only the block's rows or record is issued before any is waited for: the rows
+  // of r and p_{j-1} (p_{j-1} also at the first step, where it is not used), the scalars, the partials of
+  // the previous update and, on pair steps, the iterate's rows and p_{j-2}
   double pv[ITEMS], rv[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
     const bool ok = i < r1;
-    pv[j] = (ok && !first) ? pin[i] : 0.;
+    pv[j] = ok ? pin[i] : 0.;
     rv[j] = ok ? P.r[i] : 0.;
   }
+  const int inst = P.v_inst[b];
+  const PcgScalars cur = P.S[par * P.ninst + inst];
+  PcgScalars* nxt = P.S + (par ^ 1) * P.ninst + inst;
+  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
+  const int done = pcg_state(cur);
+  Partials<2> pt;
+  pt.issue(P.part2, P.v_ib0[inst], done ? P.v_ib0[inst] : P.v_ib0[inst + 1]);
+  const int64_t it = cur.iters;
+  const bool first = (it == 0);
+  const bool pair = !first && (it & 1) == 0; // both pending updates of the iterate are due
+  double ov[ITEMS], xv[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+    const bool ok = i < r1 && pair && !done;
+    ov[j] = ok ? pout[i] : 0.;
+    xv[j] = ok ? P.x[i] : 0.;
+  }
+  if (done) {
+    if (writer) {
+      *nxt = cur;
+      nxt->done = done;
+    }
+    return;
+  }
+  const double rho_prev = cur.rz, alpha = cur.alpha, alpha_prev = cur.alpha_prev, tol2 = cur.tol2;
   double t[2];
-  reduce_partials<2>(P.part2, P.v_ib0[inst], P.v_ib0[inst + 1], t);
+  pt.sum(P.part2, t);
   const int status = (t[1] <= tol2) ? 1 : (it >= P.max_iter) ? 2 : 0;
   const double beta = (status == 0 && !first) ? t[0] / rho_prev : 0.;
   if (!first) {
-    // the iterate's rows and p_{j-2} when an update is due (all loads issued before the stores)
-    const bool upd = pair || status != 0;
-    double ov[ITEMS], xv[ITEMS];
+    if (!pair && status != 0) // the instance stops at an odd step: its one pending update
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
-      const bool ok = i < r1;
-      ov[j] = (ok && pair) ? pout[i] : 0.;
-      xv[j] = (ok && upd) ? P.x[i] : 0.;
-    }
+      for (int j = 0; j < ITEMS; ++j) {
+        const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
+        xv[j] = i < r1 ? P.x[i] : 0.;
+      }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
@@ -709,11 +715,11 @@ __global__ void __launch_bounds__(1024) k_cg_direction(PcgParams P, int par)
     }
   }
   if (writer) {
-    nxt->bb = cur->bb;
+    nxt->bb = cur.bb;
     nxt->tol2 = tol2;
     nxt->alpha = alpha;
     nxt->alpha_prev = alpha;
-    nxt->pq = cur->pq;
+    nxt->pq = cur.pq;
     nxt->rr = t[1];
     nxt->rz = t[0];
     nxt->beta = beta;
@@ -730,16 +736,20 @@ __global__ void __launch_bounds__(1024) k_cg_spmv(PcgParams P, int par)
   const int b = active_block(P.a_nb, P.a_list, P.rev);
   if (b < 0)
     return;
-  const int inst = P.a_inst[b];
-  if (P.S[(par ^ 1) * P.ninst + inst].done)
-    return;
+  // a single instance is instance 0: its offsets and bounds are then read without waiting for a_inst
+  const int inst = P.ninst == 1 ? 0 : P.a_inst[b];
   const int64_t rb = P.a_row0[b], r1 = P.a_row0[b + 1];
   const double* __restrict__ p = par ? P.p : P.p2; // p_j (written by the direction update of step j)
+  // the stopping flag is read with the block's first rows and tested before the first store, so its load
+  // does not delay theirs (a block of a finished instance reads its rows and discards them)
+  const int done = P.S[(par ^ 1) * P.ninst + inst].done;
   double v[1] = {0.};
   for (int64_t r0 = rb; r0 < r1; r0 += int64_t(blockDim.x) * ITEMS) {
     double acc[ITEMS];
     apply_rows<FMT, ITEMS, FMT != FMT_SELL>(P, P.scaled, b, inst, rb, r0, r1, [&](int64_t c) { return p[c]; },
                                             acc);
+    if (done)
+      return;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const int64_t i = r0 + threadIdx.x + int64_t(j) * blockDim.x;
@@ -887,13 +897,10 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
   const int b = active_block(P.v_nb, P.v_list, P.rev);
   if (b < 0)
     return;
-  const int inst = P.v_inst[b];
-  PcgScalars* S = P.S + (par ^ 1) * P.ninst + inst;
-  if (S->done)
-    return;
-  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
   const int64_t r0 = P.v_row0[b], r1 = P.v_row0[b + 1];
-  // rows loaded before the reduction of p.q, which they then overlap
+  // the rows, the record (read whole, before any store: the writer block's stores below touch only alpha,
+  // pq and brk, of which only alpha is read, and only on a breakdown, which no block writes alpha at) and
+  // the partials of p.q are all issued before any is waited for
   double qv[ITEMS], rv[ITEMS], sv[ITEMS];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -903,20 +910,28 @@ __global__ void __launch_bounds__(1024) k_cg_update(PcgParams P, int par)
     rv[j] = ok ? P.r[i] : 0.;
     sv[j] = ok ? P.dsq[i] : 0.;
   }
+  const int inst = P.v_inst[b];
+  PcgScalars* S = P.S + (par ^ 1) * P.ninst + inst;
+  const PcgScalars Sv = *S;
+  Partials<1> pp;
+  pp.issue(P.part1, P.a_ib0[inst], Sv.done ? P.a_ib0[inst] : P.a_ib0[inst + 1]);
+  if (Sv.done)
+    return;
+  const bool writer = (b == P.v_ib0[inst] && threadIdx.x == 0);
   double pq[1];
-  reduce_partials<1>(P.part1, P.a_ib0[inst], P.a_ib0[inst + 1], pq);
+  pp.sum(P.part1, pq);
   if (!(pq[0] > 0.)) {
     // breakdown (A not positive definite on the Krylov space): every block of the instance sees it.  At
     // an odd step j the direction update deferred alpha_{j-1} p_{j-1} (p_{j-1} in p2): applied here, so
     // the stopped iterate is the one of an update at every step
-    if ((S->iters & 1) == 0)
+    if ((Sv.iters & 1) == 0)
       for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
-        P.x[i] = fma(S->alpha, P.p2[i], P.x[i]);
+        P.x[i] = fma(Sv.alpha, P.p2[i], P.x[i]);
     if (writer)
       S->brk = 1;
     return;
   }
-  const double alpha = S->rz / pq[0];
+  const double alpha = Sv.rz / pq[0];
   if (writer)
     S->alpha = alpha, S->pq = pq[0];
   double v[2] = {0., 0.};
@@ -1498,7 +1513,8 @@ static void build_partitions(Operator& op)
 static bool blk_wanted(const Operator& op)
 {
   const Space& s = *op.space;
-  const bool fits = s.kind == HGDT_DISCONTINUOUS_LAGRANGE && s.local_size == 4 && op.n % 4 == 0 && op.n > 0;
+  const bool fits = s.kind == HGDT_DISCONTINUOUS_LAGRANGE && s.local_size == 4 && op.n % 4 == 0 && op.n > 0 &&
+                    op.square();
   return fits && (op.requested_format == HGDT_FORMAT_BLOCK || op.requested_format == HGDT_FORMAT_AUTO);
 }
 
@@ -1760,6 +1776,8 @@ static void build_structure(Operator& op)
   F.d_sell_pos.release(), F.d_scaled.release();
   F.d_mf_ref.release();
   F.mf_kind = -1;
+  if (!op.square() && op.requested_format != HGDT_FORMAT_AUTO && op.requested_format != HGDT_FORMAT_SELL)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "an operator between two spaces is stored in SELL format only");
   if (op.requested_format == HGDT_FORMAT_MATRIX_FREE) {
     mf_setup(op);
     F.format = HGDT_FORMAT_MATRIX_FREE;
@@ -1788,7 +1806,7 @@ static void build_structure(Operator& op)
 
   // DIA detection: distinct (col - row) offsets per instance
   std::vector<std::vector<int64_t>> offs(ninst);
-  bool dia_ok = op.requested_format != HGDT_FORMAT_SELL;
+  bool dia_ok = op.requested_format != HGDT_FORMAT_SELL && op.square();
   for (int in = 0; in < ninst && dia_ok; ++in) {
     std::vector<int64_t>& o = offs[in];
     std::mutex m;
@@ -1897,7 +1915,7 @@ static void build_structure(Operator& op)
                 scols[at] = op.cols[op.row_ptr[i] + k];
                 pos[op.row_ptr[i] + k] = at;
               } else {
-                scols[at] = int32_t(std::min<int64_t>(i, op.n - 1)); // padding: zero value, valid column
+                scols[at] = int32_t(std::min<int64_t>(i, op.ncols - 1)); // padding: zero value, valid column
               }
             }
           }
@@ -1926,7 +1944,7 @@ static int64_t apply_bytes(const Operator& op)
     return 8 * ((F.symmetric ? 10 : 16) * F.blk_ne + 16 * (F.symmetric ? F.blk_up : F.blk_all)) + 16 * op.n;
   if (F.format == HGDT_FORMAT_DIA)
     return int64_t(F.symmetric ? F.nup : F.ndiag) * 8 * op.n + 16 * op.n;
-  return 12 * op.nnz + 16 * op.n;
+  return 12 * op.nnz + 8 * op.n + 8 * op.ncols;
 }
 
 // algorithmic bytes of the CG solver's operator application: the scaled matrix without its unit diagonal
@@ -1999,6 +2017,8 @@ bool Operator::direct_applicable()
   if (direct_state >= 0)
     return direct_state == 1;
   direct_state = 0;
+  if (borrowed || !square()) // a caller's matrix or an operator between two spaces: the general path
+    return false;
   const Space& s = *space;
   const Grid& g = *s.grid;
   // every instance a lattice (a structured grid, or a union of structured grids, which keeps their lattices)
@@ -2183,6 +2203,19 @@ void Operator::prepare_storage(hipStream_t stream)
   const bool dia = F.format == HGDT_FORMAT_DIA;
   double* dst = dia ? F.d_dia.ptr : F.d_sell_vals.ptr;
   const int64_t total = dia ? int64_t(F.ndiag) * n : int64_t(F.d_sell_vals.count);
+  if (!square()) {
+    // an operator between two spaces: its SELL values for apply(); no Jacobi scaling, no solver copy
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(total, B)), dim3(B), 0, stream, total, dst, 0.);
+    hipLaunchKernelGGL(k_scatter_values, dim3(blocks_for(nnz, B)), dim3(B), 0, stream, nnz, d_values.ptr,
+                       F.d_sell_pos.ptr, dst);
+    HGDT_HIP(hipGetLastError());
+    F.diag_positive = false;
+    F.symmetric = false;
+    F.bytes = apply_bytes(*this);
+    F.step_bytes = 0;
+    F.valid = true;
+    return;
+  }
   if (dia && direct_storage) {
     // the structured assembly wrote the diagonal and the upper diagonals, D^{-1/2}, D^{1/2} and the flags
     // (symmetric by construction); nothing to convert
@@ -2353,6 +2386,8 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   SolverCache& C = cache_of(*this);
   if (opt.check_every < 1 || opt.max_iter < 1 || !(opt.rtol >= 0.))
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "invalid solver options");
+  if (!square())
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "apply_inverse needs an operator whose source and range spaces are the same");
   prepare_solver(user_stream);
   if (!fmt.diag_positive)
     HGDT_THROW(HGDT_ERR_SOLVER_FAILED, "Jacobi-PCG needs a positive diagonal: the matrix is not SPD");

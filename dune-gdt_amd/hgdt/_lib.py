@@ -162,6 +162,9 @@ SIGNATURES = {
     "hgdt_space_instance_dofs": (ctypes.c_int, [_vp, _pi64]),
     "hgdt_space_error_norms": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _dbl, _pdbl, _vp]),
     "hgdt_operator_create": (ctypes.c_int, [_vp, ctypes.c_int, _pvp]),
+    "hgdt_operator_create_between": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _pvp]),
+    "hgdt_operator_create_on_matrix": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _pi64, _pi32, ctypes.c_int64,
+                                                      ctypes.c_int64, _vp, _pvp]),
     "hgdt_operator_destroy": (ctypes.c_int, [_vp]),
     "hgdt_operator_shape": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64]),
     "hgdt_operator_copy_pattern": (ctypes.c_int, [_vp, _pi64, _pi32]),

@@ -310,12 +310,12 @@ class _LagrangeSpace:
 
 
 class ContinuousLagrangeSpace(_LagrangeSpace):
-    """dune/gdt/spaces/h1/continuous-lagrange.hh:49-183 (order 1 on the device)."""
+    """dune/gdt/spaces/h1/continuous-lagrange.hh:49-183 (orders 1 and 2 on the device)."""
     _kind = L.CONTINUOUS_LAGRANGE
 
 
 class DiscontinuousLagrangeSpace(_LagrangeSpace):
-    """dune/gdt/spaces/l2/discontinuous-lagrange.hh:60-183 (order 1 on the device)."""
+    """dune/gdt/spaces/l2/discontinuous-lagrange.hh:60-183 (orders 1 and 2 on the device)."""
     _kind = L.DISCONTINUOUS_LAGRANGE
 
 
@@ -605,13 +605,30 @@ class MatrixOperator:
     """MatrixOperator (dune/gdt/operators/matrix-based.hh:245-508) on the GPU."""
 
     def __init__(self, grid, source_space, range_space=None, stencil=Stencil.element_and_intersection,
-                 logging_prefix=""):
+                 logging_prefix="", matrix=None):
+        """MatrixOperator(grid, source_space, range_space[, matrix]) (matrix-based.hh:279-311; python bindings
+        :197-240): rows are the range space's DoFs, columns the source space's.  With `matrix` (a CsrMatrix
+        whose values are a float64 device tensor) the operator assembles into that matrix, adding to the
+        values it holds, and `matrix` is the operator's matrix; its pattern must hold the stencil's."""
         range_space = source_space if range_space is None else range_space
-        if range_space is not source_space:
-            raise L.not_supported(9, "test and ansatz space must be the same space")
-        self.grid, self.space = grid, source_space
+        self.grid, self.space, self.source_space = grid, range_space, source_space
+        self._borrowed = None
         h = ctypes.c_void_p()
-        L.call("hgdt_operator_create", source_space._h, int(stencil), ctypes.byref(h))
+        if matrix is not None:
+            rp = np.ascontiguousarray(matrix.row_ptr, np.int64)
+            cols = np.ascontiguousarray(matrix.cols, np.int32)
+            vals = matrix.values
+            if not (isinstance(vals, torch.Tensor) and vals.is_cuda and vals.dtype == torch.float64
+                    and vals.is_contiguous() and vals.numel() == cols.size):
+                raise L.wrong_input_given(1, "matrix values: a contiguous float64 device tensor of nnz entries")
+            L.call("hgdt_operator_create_on_matrix", source_space._h, range_space._h, int(stencil),
+                   rp.ctypes.data_as(L._pi64), cols.ctypes.data_as(L._pi32), len(rp) - 1, cols.size,
+                   ctypes.c_void_p(vals.data_ptr()), ctypes.byref(h))
+            self._borrowed = matrix
+        elif range_space is source_space:
+            L.call("hgdt_operator_create", source_space._h, int(stencil), ctypes.byref(h))
+        else:
+            L.call("hgdt_operator_create_between", source_space._h, range_space._h, int(stencil), ctypes.byref(h))
         self._h = h
         r, c, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         L.call("hgdt_operator_shape", h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(nnz))
@@ -785,6 +802,8 @@ class MatrixOperator:
 
     @property
     def matrix(self):
+        if self._borrowed is not None:  # the caller's matrix itself (its values are the operator's)
+            return self._borrowed
         rp, cols = self.pattern()
         vals = torch.empty(self.nnz, dtype=torch.float64, device=_device())
         L.call("hgdt_memcpy", ctypes.c_void_p(vals.data_ptr()), self.values_device_ptr(), 8 * self.nnz,
@@ -827,6 +846,8 @@ class MatrixOperator:
 
         `range` may be the string "rhs" to solve with the operator's own assembled right-hand side
         (no copy)."""
+        if self.source_space is not self.space:
+            raise L.not_supported(9, "apply_inverse needs an operator whose source and range spaces are the same")
         if isinstance(range, str) and range == "rhs":
             p = ctypes.c_void_p()
             L.call("hgdt_operator_rhs_device", self._h, ctypes.byref(p))
@@ -933,9 +954,12 @@ def _scaling(param):
     return float(v[0] if isinstance(v, (list, tuple)) else v)
 
 
-def make_matrix_operator(space, stencil=Stencil.element_and_intersection, logging_prefix=""):
-    """make_matrix_operator<M>(space, stencil), matrix-based.hh:650-658"""
-    return MatrixOperator(space.grid, space, space, stencil=stencil)
+def make_matrix_operator(space, stencil=Stencil.element_and_intersection, logging_prefix="", range_space=None,
+                         matrix=None):
+    """make_matrix_operator<M>(space, stencil) (matrix-based.hh:650-658), make_matrix_operator<M>(source_space,
+    range_space, stencil) (:200-216) with `range_space`, and the variants on a given matrix (:511-546)"""
+    return MatrixOperator(space.grid, space, space if range_space is None else range_space, stencil=stencil,
+                          matrix=matrix)
 
 
 # ---------------------------------------------------------------------------

@@ -185,6 +185,25 @@ hgdt_status hgdt_noalloc_scratch_bytes(int64_t* bytes);
  * sparsity pattern (dune/gdt/tools/sparsity-pattern.hh:33-144) and a zero matrix on it
  * (MatrixOperator ctor, matrix-based.hh:298-311).  Test and ansatz space are the same space. */
 hgdt_status hgdt_operator_create(hgdt_space space, int stencil, hgdt_operator* op);
+/* make_matrix_operator<M>(source_space, range_space, stencil) (matrix-based.hh:200-216 with the pattern
+ * constructor :295-311): an operator from the source space (ansatz functions, columns) to the range
+ * space (test functions, rows), rows x cols = range size x source size.  Both spaces on the same grid
+ * with the same local finite element (the same polynomial order; a continuous and a discontinuous space
+ * may be paired).  apply() maps source vectors to range vectors; apply_inverse and Dirichlet constraints
+ * need source == range (HGDT_ERR_NOT_SUPPORTED otherwise).  Stored in SELL format. */
+hgdt_status hgdt_operator_create_between(hgdt_space source, hgdt_space range, int stencil, hgdt_operator* op);
+/* MatrixOperator(assembly_grid_view, source_space, range_space, matrix) (matrix-based.hh:279-293,
+ * make_matrix_operator(..., matrix) :511-546): assembles into the caller's CSR matrix.  Host row_ptr[rows + 1]
+ * and cols[nnz] (ascending and unique per row) give its pattern, which must hold every entry of the
+ * stencil's (HGDT_ERR_SHAPES_DO_NOT_MATCH otherwise); rows = the range space's size.  The device values[nnz]
+ * stay the caller's and must outlive the operator.  As the reference's local assemblers
+ * (add_to_entry, local/assembler/bilinear-form-assemblers.hh:126,264-276,394), assemble() adds every
+ * contribution to the value held, in walk order; with Dirichlet constraints the constrained rows and
+ * columns are then set (tools/dirichlet-constraints.hh:153-184).  The operator's matrix is the caller's
+ * whole matrix: apply, apply_inverse and copy_values use every entry, those outside the stencil included. */
+hgdt_status hgdt_operator_create_on_matrix(hgdt_space source, hgdt_space range, int stencil, const int64_t* row_ptr,
+                                           const int32_t* cols, int64_t rows, int64_t nnz, double* values,
+                                           hgdt_operator* op);
 hgdt_status hgdt_operator_destroy(hgdt_operator op);
 /* rows, cols, nnz of the pattern */
 hgdt_status hgdt_operator_shape(hgdt_operator op, int64_t* rows, int64_t* cols, int64_t* nnz);

@@ -998,6 +998,14 @@ private:
   int64_t rows_ = 0, cols_ = 0, nnz_ = 0;
 };
 
+// a caller's CSR matrix for MatrixOperator(grid_view, source_space, range_space, matrix): the pattern on the
+// host (ascending, unique columns per row), the values on the device (nnz entries, owned by the caller)
+struct DeviceCsrMatrix {
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> col_idx;
+  double* values = nullptr;
+};
+
 class MatrixOperator
 {
 public:
@@ -1006,6 +1014,29 @@ public:
   {
     hgdt_operator op = nullptr;
     check(hgdt_operator_create(space.handle(), int(stencil), &op));
+    h_ = std::shared_ptr<hgdt_operator_s>(op, [](hgdt_operator p) { hgdt_operator_destroy(p); });
+  }
+  // MatrixOperator(grid_view, source_space, range_space, pattern), matrix-based.hh:295-311: rows of the range
+  // space, columns of the source space (same grid and polynomial order)
+  MatrixOperator(const SpaceInterface& source_space, const SpaceInterface& range_space,
+                 Stencil stencil = Stencil::element_and_intersection)
+    : space_(range_space)
+  {
+    hgdt_operator op = nullptr;
+    check(hgdt_operator_create_between(source_space.handle(), range_space.handle(), int(stencil), &op));
+    h_ = std::shared_ptr<hgdt_operator_s>(op, [](hgdt_operator p) { hgdt_operator_destroy(p); });
+  }
+  // MatrixOperator(grid_view, source_space, range_space, matrix), matrix-based.hh:279-293: assembles into the
+  // caller's matrix (its values must outlive the operator), adding to the values it holds
+  MatrixOperator(const SpaceInterface& source_space, const SpaceInterface& range_space, DeviceCsrMatrix& matrix,
+                 Stencil stencil = Stencil::element_and_intersection)
+    : space_(range_space)
+  {
+    hgdt_operator op = nullptr;
+    const int64_t rows = int64_t(matrix.row_ptr.size()) - 1;
+    check(hgdt_operator_create_on_matrix(source_space.handle(), range_space.handle(), int(stencil),
+                                         matrix.row_ptr.data(), matrix.col_idx.data(), rows,
+                                         int64_t(matrix.col_idx.size()), matrix.values, &op));
     h_ = std::shared_ptr<hgdt_operator_s>(op, [](hgdt_operator p) { hgdt_operator_destroy(p); });
   }
 
@@ -1227,6 +1258,18 @@ private:
 inline MatrixOperator make_matrix_operator(const SpaceInterface& space, Stencil stencil = Stencil::element_and_intersection)
 {
   return MatrixOperator(space, stencil);
+}
+// make_matrix_operator(source_space, range_space, stencil), matrix-based.hh:200-216
+inline MatrixOperator make_matrix_operator(const SpaceInterface& source_space, const SpaceInterface& range_space,
+                                           Stencil stencil = Stencil::element_and_intersection)
+{
+  return MatrixOperator(source_space, range_space, stencil);
+}
+// make_matrix_operator(grid_view, source_space, range_space, matrix), matrix-based.hh:511-546
+inline MatrixOperator make_matrix_operator(const SpaceInterface& source_space, const SpaceInterface& range_space,
+                                           DeviceCsrMatrix& matrix, Stencil stencil = Stencil::element_and_intersection)
+{
+  return MatrixOperator(source_space, range_space, matrix, stencil);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -726,7 +726,8 @@ inline double kgrad_dot(double kappa, const double* g, const double* h, int d)
 }
 
 struct Operator {
-  Space* space = nullptr;
+  Space* space = nullptr;  // range space: rows (test functions), right-hand side, constraints
+  Space* source = nullptr; // source space: columns (ansatz functions); the range space unless set
   int stencil = 0;
   int64_t n = 0;
   std::vector<int64_t> row_ptr;
@@ -769,16 +770,17 @@ struct Operator {
     std::vector<std::set<int32_t>> pattern(n);
     for (int64_t e = 0; e < m.ne; ++e) {
       const int32_t* gi = &space->indices[e * ls];
+      const int32_t* si = &source->indices[e * ls];
       if (stencil != OR_STENCIL_INTERSECTION)
         for (int i = 0; i < ls; ++i)
           for (int j = 0; j < ls; ++j)
-            pattern[gi[i]].insert(gi[j]);
+            pattern[gi[i]].insert(si[j]);
       if (stencil != OR_STENCIL_ELEMENT)
         for (int f = 0; f < nf; ++f) {
           const int32_t o = m.nb[e * nf + f];
           if (o < 0)
             continue;
-          const int32_t* gj = &space->indices[int64_t(o) * ls];
+          const int32_t* gj = &source->indices[int64_t(o) * ls];
           for (int i = 0; i < ls; ++i)
             for (int j = 0; j < ls; ++j)
               pattern[gi[i]].insert(gj[j]);
@@ -1164,12 +1166,13 @@ struct Operator {
         try {
           const Geometry geo = m.geometry(e);
           basis.bind(&space->fe, &geo);
-          const int32_t* gi = &space->indices[e * ls];
+          const int32_t* gi = &space->indices[e * ls];  // rows: test functions of the range space
+          const int32_t* si = &source->indices[e * ls]; // columns: ansatz functions of the source space
           for (const auto& F : eforms) {
             element_local(F, e, geo, basis, local);
             for (int i = 0; i < ls; ++i)
               for (int j = 0; j < ls; ++j)
-                add_to_entry(gi[i], gi[j], F.scaling * local[i * ls + j], atomic);
+                add_to_entry(gi[i], si[j], F.scaling * local[i * ls + j], atomic);
           }
           for (const auto& S : sources) {
             source_local(S, e, geo, basis, lvec);
@@ -1209,6 +1212,7 @@ struct Operator {
               const Geometry gout = m.geometry(o);
               bout.bind(&space->fe, &gout);
               const int32_t* gj = &space->indices[o * ls];
+              const int32_t* sj = &source->indices[o * ls];
               for (const auto& F : cforms) {
                 const int order = 0 + 0 + space->fe.order + space->fe.order + F.over_integrate;
                 const FaceGeo fg = face_geometry(e, f, o, geo, &gout, order);
@@ -1216,15 +1220,15 @@ struct Operator {
                 // four scatter loops, bilinear-form-assemblers.hh:262-277
                 for (int i = 0; i < ls; ++i) {
                   for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], gi[j], F.scaling * blocks[0][i * ls + j], atomic);
+                    add_to_entry(gi[i], si[j], F.scaling * blocks[0][i * ls + j], atomic);
                   for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], gj[j], F.scaling * blocks[1][i * ls + j], atomic);
+                    add_to_entry(gi[i], sj[j], F.scaling * blocks[1][i * ls + j], atomic);
                 }
                 for (int i = 0; i < ls; ++i) {
                   for (int j = 0; j < ls; ++j)
-                    add_to_entry(gj[i], gi[j], F.scaling * blocks[2][i * ls + j], atomic);
+                    add_to_entry(gj[i], si[j], F.scaling * blocks[2][i * ls + j], atomic);
                   for (int j = 0; j < ls; ++j)
-                    add_to_entry(gj[i], gj[j], F.scaling * blocks[3][i * ls + j], atomic);
+                    add_to_entry(gj[i], sj[j], F.scaling * blocks[3][i * ls + j], atomic);
                 }
               }
             } else if (o < 0 && !bforms.empty()) {
@@ -1236,7 +1240,7 @@ struct Operator {
                 boundary_local(F, e, fg, basis, local);
                 for (int i = 0; i < ls; ++i)
                   for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], gi[j], F.scaling * local[i * ls + j], atomic);
+                    add_to_entry(gi[i], si[j], F.scaling * local[i * ls + j], atomic);
               }
             }
           }
@@ -1422,11 +1426,30 @@ OrOperator* or_operator_create(OrSpace* s, int stencil)
   OrOperator* out = nullptr;
   guarded([&] {
     auto* op = new OrOperator;
-    op->space = s, op->stencil = stencil;
+    op->space = op->source = s, op->stencil = stencil;
     op->build_pattern();
     out = op;
   });
   return out;
+}
+// make_matrix_operator(source_space, range_space, stencil) (operators/matrix-based.hh:200-216): rows from the
+// range space, columns from the source space (same mesh and local finite element)
+OrOperator* or_operator_create_between(OrSpace* source, OrSpace* range, int stencil)
+{
+  OrOperator* out = nullptr;
+  guarded([&] {
+    if (source->mesh != range->mesh || source->fe.size != range->fe.size || source->fe.order != range->fe.order)
+      throw Error("source and range spaces need the same mesh and local finite element");
+    auto* op = new OrOperator;
+    op->space = range, op->source = source, op->stencil = stencil;
+    op->build_pattern();
+    out = op;
+  });
+  return out;
+}
+int64_t or_operator_cols(const OrOperator* op)
+{
+  return op->source->size;
 }
 void or_operator_destroy(OrOperator* op)
 {
@@ -1508,6 +1531,10 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
 }
 int or_operator_set_dirichlet(OrOperator* op, int enabled)
 {
+  if (enabled && op->source != op->space) {
+    g_error = "Dirichlet constraints need an operator whose source and range spaces are the same";
+    return -1;
+  }
   op->dirichlet = enabled != 0;
   return 0;
 }

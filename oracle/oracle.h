@@ -52,6 +52,8 @@ void or_space_global_indices(const OrSpace* s, int32_t* out /* ne * local_size *
 
 /* ---- matrix operator (pattern + forms + functional + Dirichlet) ---- */
 OrOperator* or_operator_create(OrSpace* s, int stencil);
+OrOperator* or_operator_create_between(OrSpace* source, OrSpace* range, int stencil);
+int64_t or_operator_cols(const OrOperator* op);
 void or_operator_destroy(OrOperator* op);
 int64_t or_operator_nnz(const OrOperator* op);
 void or_operator_pattern(const OrOperator* op, int64_t* row_ptr, int32_t* cols);

@@ -53,6 +53,8 @@ def lib():
         "or_space_local_size": (ctypes.c_int, [_vp]),
         "or_space_global_indices": (None, [_vp, _ip]),
         "or_operator_create": (_vp, [_vp, ctypes.c_int]),
+        "or_operator_create_between": (_vp, [_vp, _vp, ctypes.c_int]),
+        "or_operator_cols": (ctypes.c_int64, [_vp]),
         "or_operator_destroy": (None, [_vp]),
         "or_operator_nnz": (ctypes.c_int64, [_vp]),
         "or_operator_pattern": (None, [_vp, _lp, _ip]),
@@ -219,12 +221,18 @@ class Space:
 class Operator:
     """MatrixOperator restated on the CPU: pattern, appended local forms, functional, Dirichlet."""
 
-    def __init__(self, space, stencil=STENCIL_ELEMENT):
+    def __init__(self, space, stencil=STENCIL_ELEMENT, source=None):
+        """rows of `space` (range: test functions), columns of `source` (ansatz functions; `space` if None)"""
         self.space = space
-        self.h = lib().or_operator_create(space.h, stencil)
+        self.source = space if source is None else source
+        if source is None:
+            self.h = lib().or_operator_create(space.h, stencil)
+        else:
+            self.h = lib().or_operator_create_between(source.h, space.h, stencil)
         if not self.h:
             raise RuntimeError(_err())
         self.n = space.size
+        self.ncols = lib().or_operator_cols(self.h)
         self.nnz = lib().or_operator_nnz(self.h)
         self._keep = []
 
@@ -302,7 +310,7 @@ class Operator:
         self._mask(lib().or_operator_set_dirichlet_faces, mask)
 
     def set_dirichlet(self, enabled=True):
-        lib().or_operator_set_dirichlet(self.h, int(enabled))
+        self._check(lib().or_operator_set_dirichlet(self.h, int(enabled)))
 
     def assemble(self, nthreads=1):
         self._check(lib().or_operator_assemble(self.h, nthreads))
@@ -333,7 +341,7 @@ class Operator:
         import scipy.sparse as sp
 
         rp, cols = self.pattern()
-        return sp.csr_matrix((self.values(), cols, rp), shape=(self.n, self.n))
+        return sp.csr_matrix((self.values(), cols, rp), shape=(self.n, self.ncols))
 
     def __del__(self):
         if getattr(self, "h", None):

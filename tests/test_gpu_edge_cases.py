@@ -171,18 +171,20 @@ def test_invalid_input_raises(hgdt_mod):
     # connectivity out of range
     with pytest.raises(H.wrong_input_given):
         H.Grid.from_arrays("tri", X[:3], np.array([[0, 1, 5]], dtype=np.int32))
-    # order 3 spaces and order 2 on hexahedra are not provided
+    # order 3 spaces are not provided
     with pytest.raises(H.not_supported):
         H.ContinuousLagrangeSpace(grid, 3)
-    with pytest.raises(H.not_supported):
-        H.ContinuousLagrangeSpace(H.Grid.structured("hex", [0, 0, 0], [1, 1, 1], [2, 2, 2]), 2)
-    # intersection integrals on an order-2 space
-    dg2 = H.DiscontinuousLagrangeSpace(grid, 2)
+    # intersection integrals on Q2 hexahedra (27 basis functions on each side of a face)
+    hex_grid = H.Grid.structured("hex", [0, 0, 0], [1, 1, 1], [2, 2, 2])
+    dg2 = H.DiscontinuousLagrangeSpace(hex_grid, 2)
     op2 = H.make_matrix_operator(dg2, H.Stencil.element_and_intersection)
     op2.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(8.0, 1.0)), {},
                H.ApplyOn.InnerIntersectionsOnce())
     with pytest.raises(H.not_supported):
         op2.assemble()
+    # an operator between spaces of different orders
+    with pytest.raises(H.not_supported):
+        H.make_matrix_operator(space, H.Stencil.element, range_space=H.ContinuousLagrangeSpace(grid, 2))
     # launch shapes outside the kernels' bounds
     with pytest.raises(H.wrong_input_given):
         op.set_launch(H._lib.KERNEL_APPLY, 96, 1)

@@ -1,4 +1,4 @@
-"""Intersection integrals on 3-d grids on the HIP path (integrals.hh:197-267,338-369 over the faces of
+"""Intersection integrals on 3-d grids, and of order-2 spaces, on the HIP path (integrals.hh:197-267,338-369 over the faces of
 hexahedra and tetrahedra; boundary functionals, local/functionals/integrals.hh:157-185).
 
 Bar: pattern bit-exact, matrix and right-hand side within 1e-12 (row-relative / max-relative) of the oracle,
@@ -120,3 +120,62 @@ def test_swipdg_3d_dirichlet_forms_on_x_faces(hgdt_mod, name, geom, cells):
     berr = np.abs(op.rhs.cpu().numpy() - oop.rhs()).max() / np.abs(oop.rhs()).max()
     print(f"  {int(mask.sum())} of {len(mask)} boundary faces Dirichlet, matrix {err:.2e}, rhs {berr:.2e}")
     assert err <= 1e-12 and berr <= 1e-12
+
+
+@pytest.mark.parametrize("geometry,geom", [("tri", O.TRI), ("quad", O.QUAD)])
+def test_swipdg_order2_matches_oracle(hgdt_mod, geometry, geom):
+    """SWIPDG on P2 / Q2 discontinuous spaces (order-2 intersection integrals): matrix and rhs within 1e-12
+    of the oracle's, and the quadratic solution u = x^2 - xy + y^2/2 + 0.3 x (-Laplace u = -3, g = u)
+    reproduced to 1e-9"""
+    H = hgdt_mod
+    print(f"case SWIPDG order 2 on {geometry}", flush=True)
+    n, sigma = 4, 40.0
+    over = 1 if geometry == "quad" else 0
+
+    def u2(X):
+        return X[0] * X[0] - X[0] * X[1] + 0.5 * X[1] * X[1] + 0.3 * X[0]
+
+    grid = H.Grid.structured(geometry, [0, 0], [1, 1], [n, n])
+    space = H.DiscontinuousLagrangeSpace(grid, 2)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0))
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(
+        H.LocalLaplaceIPDGInnerCoupling(1.0, 1.0, 1.0) + H.LocalIPDGInnerPenalty(sigma, 1.0)), {},
+        H.ApplyOn.InnerIntersectionsOnce())
+    op.append(H.LocalIntersectionIntegralBilinearForm(
+        H.LocalIPDGBoundaryPenalty(sigma, 1.0) + H.LocalLaplaceIPDGDirichletCoupling(1.0, 1.0)), {},
+        H.ApplyOn.BoundaryIntersections())
+    g = H.GenericFunction(2, u2)
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(-3.0)))
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalLaplaceIPDGDirichletCoupling(1.0, 1.0, g),
+                                                     over_integrate=over), {}, H.ApplyOn.BoundaryIntersections())
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalIPDGBoundaryPenalty(sigma, 1.0).with_ansatz(g)), {},
+               H.ApplyOn.BoundaryIntersections())
+    op.append(rhs)
+    op.assemble()
+    m = O.Mesh.structured(geom, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, False, 2)
+    oop = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    oop.append_laplace(1.0)
+    oop.append_inner_coupling(1.0, 1.0, 1.0)
+    oop.append_inner_penalty(sigma, 1.0)
+    oop.append_boundary_penalty(sigma, 1.0)
+    oop.append_dirichlet_coupling(1.0, 1.0)
+    oop.append_source(-3.0)
+    oop.append_boundary_source(0, 1.0, 1.0, u2(np.moveaxis(m.boundary_quadrature_points(3 + over), -1, 0)),
+                               g_order=2, over_integrate=over)
+    oop.append_boundary_source(1, sigma, 1.0, u2(np.moveaxis(m.boundary_quadrature_points(4), -1, 0)), g_order=2)
+    oop.assemble(1)
+    rp, cols = op.pattern()
+    orp, ocols = oop.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), orp)
+    berr = np.abs(op.rhs.cpu().numpy() - oop.rhs()).max() / np.abs(oop.rhs()).max()
+    x = op.apply_inverse("rhs", opts={"precision": 1e-14, "max_iter": 100000}).cpu().numpy()
+    import scipy.sparse.linalg as spla
+
+    xo = spla.spsolve(oop.csr().tocsc(), oop.rhs())
+    rep = np.abs(x - xo).max() / np.abs(xo).max()
+    print(f"  matrix {err:.2e}, rhs {berr:.2e}, solution vs the oracle's exact solve {rep:.2e}")
+    assert err <= 1e-12 and berr <= 1e-12 and rep <= 1e-9

@@ -1,11 +1,12 @@
-"""Order-2 Lagrange spaces (P2 triangles and tetrahedra, Q2 quadrilaterals) on the HIP path.
+"""Order-2 Lagrange spaces (P2 triangles and tetrahedra, Q2 quadrilaterals and hexahedra) on the HIP path.
 
 The reference's own known-answer tests run on the device: P2 continuous Lagrange Laplace and mass
 matrices on the grid of dune/gdt/test/integrands/integrands.hh:64-70 (9 x 2 cells on [0, 3] x [0, 1]),
 statistics min / max / sum of squares against integrands_laplace.cc:115-135 and
 integrands_product.cc:107-126 with the reference's tolerances (and against the exact rational values of
 tests/golden/make_golden.py).  Entry-wise parity with the oracle (same DoF numbering) for assembly,
-Dirichlet DoFs, solve and error norms.
+Dirichlet DoFs, solve and error norms; Q2 hexahedra (27 local DoFs, assembled by one lane per element row)
+also against the Kronecker closed form of a single element.
 """
 import json
 import os
@@ -47,13 +48,13 @@ def test_p2_known_answers_on_device(hgdt_mod, geometry, kind):
     assert abs(ss - ref["sumsq"]) <= (1e-11 if name == "p2_laplace_simplex" else tss)
 
 
-CASES = [("tri", O.TRI, [7, 5]), ("quad", O.QUAD, [6, 9]), ("tet", O.TET, [3, 4, 3])]
+CASES = [("tri", O.TRI, [7, 5]), ("quad", O.QUAD, [6, 9]), ("tet", O.TET, [3, 4, 3]), ("hex", O.HEX, [3, 4, 3])]
 
 
 def _pair(H, geometry, geom_o, cells, seed):
     dim = len(cells)
     lo, hi = [0.0] * dim, [1.0, 2.0, 1.5][:dim]
-    kappa = H.fields.lognormal(seed, int(np.prod(cells)) * (1 if geometry == "quad" else (2 if dim == 2 else 6)))
+    kappa = H.fields.lognormal(seed, int(np.prod(cells)) * (1 if geometry in ("quad", "hex") else (2 if dim == 2 else 6)))
     m = O.Mesh.structured(geom_o, cells, lo, hi)
     s = O.Space(m, True, 2)
     oop = O.Operator(s, O.STENCIL_ELEMENT)
@@ -136,3 +137,31 @@ def test_order2_interpolates_quadratics_exactly(hgdt_mod):
     f += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0))
     f += H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand())
     assert f.apply2() < 1e-26
+
+
+def test_q2_hexahedron_element_matrices(hgdt_mod):
+    """one Q2 hexahedron [0, a] x [0, b] x [0, c] in a discontinuous space (global = local numbering): the
+    device's stiffness and mass matrices equal the Kronecker closed forms of the 1-d quadratic matrices"""
+    H = hgdt_mod
+    print("case Q2 hexahedron element matrices against the closed form", flush=True)
+    a, b, c = 0.7, 1.1, 0.45
+
+    def K1(h):
+        return np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / (3 * h)
+
+    def M1(h):
+        return np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) * h / 30
+
+    K = (np.kron(M1(c), np.kron(M1(b), K1(a))) + np.kron(M1(c), np.kron(K1(b), M1(a)))
+         + np.kron(K1(c), np.kron(M1(b), M1(a))))
+    M = np.kron(M1(c), np.kron(M1(b), M1(a)))
+    grid = H.Grid.structured("hex", [0, 0, 0], [a, b, c], [1, 1, 1])
+    space = H.DiscontinuousLagrangeSpace(grid, 2)
+    for integrand, E in ((H.LocalLaplaceIntegrand(1.0), K), (H.LocalElementProductIntegrand(1.0), M)):
+        op = H.make_matrix_operator(space, H.Stencil.element)
+        op.append(H.LocalElementIntegralBilinearForm(integrand))
+        op.assemble()
+        A = op.matrix.to_scipy().toarray()
+        err = np.abs(A - E).max() / np.abs(E).max()
+        print(f"  max rel err {err:.2e}")
+        assert A.shape == (27, 27) and err <= 1e-14

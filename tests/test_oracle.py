@@ -847,3 +847,107 @@ def test_swipdg_3d_reproduces_linear_and_layered_solutions(geom):
         x = spla.spsolve(op.csr().tocsc(), op.rhs())
         res[name] = np.abs(x[s.global_indices()] - u(Pn)).max()
     assert res["linear"] <= 1e-10 and res["layered"] <= 1e-9 and res["linear, layered kappa"] > 1e-3, res
+
+
+@pytest.mark.parametrize("geom,cells", [(O.QUAD, [4, 3]), (O.TRI, [3, 3]), (O.TET, [2, 2, 2])])
+def test_operator_between_continuous_and_discontinuous_spaces(geom, cells):
+    """make_matrix_operator(source_space, range_space) (operators/matrix-based.hh:200-216): rows are the range
+    space's test functions, columns the source space's ansatz functions.  A continuous function u_c is the
+    same function in the discontinuous space, u_d[e, i] = u_c[cg(e, i)], so with a product and a Laplace
+    form  A(CG <- DG) u_d = A(CG <- CG) u_c  and  A(DG <- CG) u_c = A(DG <- DG) u_d  (an exchanged row or column
+    mapping breaks both); the rectangular shapes and the nnz of the element pattern follow the mappers"""
+    rng = np.random.default_rng(5)
+    m = O.Mesh.structured(geom, cells, [0] * len(cells), [1.0] * len(cells))
+    cg, dg = O.Space(m, True, 1), O.Space(m, False, 1)
+    kappa = np.exp(rng.standard_normal(m.ne))
+    ops = {}
+    for name, rng_space, src in (("cc", cg, None), ("cd", cg, dg), ("dc", dg, cg), ("dd", dg, None)):
+        op = O.Operator(rng_space, O.STENCIL_ELEMENT, source=src)
+        op.append_laplace(kappa)
+        op.append_product(2.5)
+        op.assemble(1)
+        ops[name] = op.csr()
+    gi = cg.global_indices()  # [ne, ls] continuous DoFs of each element's local DoFs
+    u_c = rng.standard_normal(cg.size)
+    u_d = u_c[gi].reshape(-1)
+    assert ops["cd"].shape == (cg.size, dg.size) and ops["dc"].shape == (dg.size, cg.size)
+    a = ops["cc"] @ u_c
+    assert np.abs(ops["cd"] @ u_d - a).max() <= 1e-13 * np.abs(a).max()
+    b = ops["dd"] @ u_d
+    assert np.abs(ops["dc"] @ u_c - b).max() <= 1e-13 * np.abs(b).max()
+    # element pattern of CG <- DG: every continuous DoF couples to all DoFs of its elements
+    ls = gi.shape[1]
+    counts = np.bincount(gi.reshape(-1), minlength=cg.size)
+    assert ops["cd"].nnz == int((counts * ls).sum())
+    with pytest.raises(RuntimeError):
+        O.Operator(cg, O.STENCIL_ELEMENT, source=dg).set_dirichlet(True)
+
+
+def test_q2_hexahedron_closed_form():
+    """Q2 on a hexahedron [0, a] x [0, b] x [0, c] (27 nodes, x fastest): the stiffness matrix is
+    Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz and the mass matrix Mx(x)My(x)Mz with the 1-d quadratic
+    Lagrange matrices K = 1/(3h) [[7,-8,1],[-8,16,-8],[1,-8,7]], M = h/30 [[4,2,-1],[2,16,2],[-1,2,4]]
+    (Kronecker products in the order z, y, x of the node index iz*9 + iy*3 + ix)"""
+    a, b, c = 0.7, 1.1, 0.45
+
+    def K1(h):
+        return np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / (3 * h)
+
+    def M1(h):
+        return np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) * h / 30
+
+    K = (np.kron(M1(c), np.kron(M1(b), K1(a))) + np.kron(M1(c), np.kron(K1(b), M1(a)))
+         + np.kron(K1(c), np.kron(M1(b), M1(a))))
+    M = np.kron(M1(c), np.kron(M1(b), M1(a)))
+    m = O.Mesh.structured(O.HEX, [1, 1, 1], [0, 0, 0], [a, b, c])
+    s = O.Space(m, False, 2)
+    for form, E in (("laplace", K), ("product", M)):
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        (op.append_laplace if form == "laplace" else op.append_product)(1.0)
+        op.assemble(1)
+        A = op.csr().toarray()
+        assert A.shape == (27, 27)
+        assert np.abs(A - E).max() <= 1e-14 * np.abs(E).max(), form
+
+
+@pytest.mark.parametrize("geom", [O.TRI, O.QUAD])
+def test_swipdg_order2_reproduces_quadratic_solution(geom):
+    """SWIPDG with P2 / Q2 discontinuous spaces (the intersection integrands of laplace-ipdg.hh / ipdg.hh
+    with order-2 bases, quadrature of order test + ansatz): u = x^2 - xy + y^2/2 + 0.3 x, -Laplace(u) = -3,
+    weak Dirichlet data g = u, lies in the space and is reproduced (every term integrated exactly: for
+    quadrilaterals the Dirichlet coupling functional over-integrates by one)"""
+    import scipy.sparse.linalg as spla
+
+    n, sigma = 4, 40.0
+    m = O.Mesh.structured(geom, [n, n], [0, 0], [1, 1])
+    s = O.Space(m, False, 2)
+
+    def u(P):
+        x, y = P[..., 0], P[..., 1]
+        return x * x - x * y + 0.5 * y * y + 0.3 * x
+
+    over = 1 if geom == O.QUAD else 0
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(1.0)
+    op.append_inner_coupling(1.0, 1.0, 1.0)
+    op.append_inner_penalty(sigma, 1.0)
+    op.append_dirichlet_coupling(1.0, 1.0)
+    op.append_boundary_penalty(sigma, 1.0)
+    op.append_source(-3.0)
+    op.append_boundary_source(0, 1.0, 1.0, u(m.boundary_quadrature_points(2 + 1 + over)), g_order=2,
+                              over_integrate=over)
+    op.append_boundary_source(1, sigma, 1.0, u(m.boundary_quadrature_points(2 + 2)), g_order=2)
+    op.assemble(1)
+    x = spla.spsolve(op.csr().tocsc(), op.rhs())
+    # nodal values: the Lagrange nodes of P2 (vertices, edge midpoints) / Q2 (3 x 3 lattice) per element
+    Xe = m.coords()[m.conn()]
+    gi = s.global_indices()
+    if geom == O.TRI:
+        nodes = np.concatenate([Xe, (Xe[:, [0, 0, 1]] + Xe[:, [1, 2, 2]]) / 2], axis=1)
+    else:
+        t = np.array([0, 0.5, 1.0])
+        lo, hi = Xe[:, 0], Xe[:, 3]
+        nodes = np.stack([np.stack([lo[:, 0] + t[i % 3] * (hi[:, 0] - lo[:, 0]),
+                                    lo[:, 1] + t[i // 3] * (hi[:, 1] - lo[:, 1])], -1) for i in range(9)], 1)
+    err = np.abs(x[gi] - u(nodes)).max()
+    assert err <= 1e-10, err
