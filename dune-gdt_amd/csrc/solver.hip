@@ -2469,6 +2469,16 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
   std::vector<int64_t> timed_iter;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> timed_pair; // (start, end) of each timed application
   const int64_t stride = opt.time_spmv > 0 ? opt.time_spmv : 0;
+  // timed iterations: issued % stride == phase, and the next one when stride > 2 (both parities: the
+  // iterate is updated every second step), phase 1 when stride > 1 (the first iteration of a chunk follows
+  // the convergence check's copy, which an interval starting at the previous iteration's end would include)
+  const int64_t phase = stride > 1 ? 1 : 0;
+  auto timed = [&](int64_t i) {
+    if (!stride)
+      return false;
+    const int64_t m = i % stride;
+    return m == phase || (stride > 2 && m == phase + 1);
+  };
   const bool fused_dir = mf_fused_direction(*this);
   auto timing_event = [&]() {
     if (C.timing_events.size() < nev + 1) {
@@ -2486,7 +2496,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     } else {
       for (int it = 0; it < chunk; ++it, ++issued) {
         hipEvent_t e0 = nullptr, e1 = nullptr, eu = nullptr;
-        if (stride && issued % stride == 0) {
+        if (timed(issued)) {
           timed_iter.push_back(issued);
           e1 = timing_event();
           // unfused: e0 is bound to the end of the direction update; fused: the previous update's end, or
@@ -2494,7 +2504,7 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
           e0 = (fused_dir && prev_upd) ? nullptr : timing_event();
           timed_pair.emplace_back(e0 ? e0 : prev_upd, e1);
         }
-        if (fused_dir && stride && (issued + 1) % stride == 0)
+        if (fused_dir && timed(issued + 1))
           eu = timing_event();
         launch_iteration(*this, P, it & 1, stream, e0, e1, eu);
         prev_upd = eu;
