@@ -7,6 +7,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <unordered_map>
 
 #include "hgdt_internal.hpp"
@@ -237,6 +238,64 @@ hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* c
   });
 }
 
+hgdt_status hgdt_grid_create_mixed(int64_t num_vertices, const double* coords, int64_t num_elements,
+                                   const int32_t* geometry, const int64_t* offsets, const int32_t* vertices,
+                                   hgdt_grid* grid)
+{
+  return guarded([&] {
+    require(coords && geometry && offsets && vertices && grid, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    require(num_vertices > 0 && num_elements > 0, HGDT_ERR_INVALID_ARGUMENT, "empty grid");
+    require(num_vertices < (int64_t(1) << 31) && num_elements < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED,
+            "too many vertices or elements");
+    require(offsets[0] == 0, HGDT_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+    bool present[2] = {false, false};
+    for (int64_t e = 0; e < num_elements; ++e) {
+      require(geometry[e] == HGDT_QUAD || geometry[e] == HGDT_TRI, HGDT_ERR_INVALID_ARGUMENT,
+              "mixed grids hold triangles and quadrilaterals");
+      require(offsets[e + 1] - offsets[e] == (geometry[e] == HGDT_QUAD ? 4 : 3), HGDT_ERR_INVALID_ARGUMENT,
+              "an element's vertex count does not match its geometry type");
+      present[geometry[e]] = true;
+    }
+    for (int64_t t = 0; t < offsets[num_elements]; ++t)
+      require(vertices[t] >= 0 && vertices[t] < num_vertices, HGDT_ERR_INVALID_ARGUMENT,
+              "connectivity references a vertex out of range");
+    require_device();
+    auto g = std::make_shared<Grid>();
+    g->nv = num_vertices, g->ne = num_elements;
+    g->coords.assign(coords, coords + num_vertices * 2);
+    if (!(present[0] && present[1])) { // one element type: the plain grid of that type
+      g->ref = reference_element(present[0] ? HGDT_QUAD : HGDT_TRI);
+      g->conn.assign(vertices, vertices + offsets[num_elements]);
+    } else {
+      // strides of the largest type (the quadrilateral), parts in ascending geometry id (quadrilaterals,
+      // then triangles), each with its elements in grid order
+      g->mixed = true;
+      g->ref = reference_element(HGDT_QUAD);
+      g->ref.geom = HGDT_MIXED;
+      g->parts = std::vector<GridPart>(2);
+      g->parts[0].ref = reference_element(HGDT_QUAD);
+      g->parts[1].ref = reference_element(HGDT_TRI);
+      g->etype.resize(num_elements), g->elocal.resize(num_elements);
+      g->conn.assign(num_elements * 4, -1);
+      for (int64_t e = 0; e < num_elements; ++e) {
+        const int p = geometry[e] == HGDT_QUAD ? 0 : 1;
+        GridPart& P = g->parts[p];
+        g->etype[e] = int8_t(p), g->elocal[e] = int32_t(P.eid.size());
+        P.eid.push_back(int32_t(e));
+        for (int64_t t = offsets[e]; t < offsets[e + 1]; ++t) {
+          P.conn.push_back(vertices[t]);
+          g->conn[e * 4 + (t - offsets[e])] = vertices[t];
+        }
+      }
+    }
+    g->check_affine();
+    g->inst_elem = {0, g->ne};
+    g->inst_vert = {0, g->nv};
+    g->finalize();
+    *grid = new hgdt_grid_s{g};
+  });
+}
+
 hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_grid* grid)
 {
   return guarded([&] {
@@ -257,6 +316,7 @@ hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_g
       require(grids[i] && grids[i]->g->ref.geom == u->ref.geom, HGDT_ERR_INVALID_ARGUMENT,
               "all grids of a union must have the same geometry type");
       require(grids[i]->g->ninst() == 1, HGDT_ERR_INVALID_ARGUMENT, "grids of a union must not be unions");
+      require(!grids[i]->g->mixed, HGDT_ERR_NOT_SUPPORTED, "unions of mixed grids are not provided");
       nv += grids[i]->g->nv, ne += grids[i]->g->ne;
     }
     require(nv < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED, "union too large for 32-bit vertex indices");
@@ -330,31 +390,46 @@ hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_
   return guarded([&] {
     require(grid && num_points && order >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
     const Grid& g = *grid->g;
-    const Quadrature q = element_rule(g.ref, order);
-    *num_points = q.size();
+    *num_points = g.mixed ? 0 : element_rule(g.ref, order).size();
     if (!host_out)
       return;
-    const int d = g.ref.dim, nq = q.size();
+    const std::vector<int64_t> off = element_point_offsets(g, order);
+    std::vector<Quadrature> rules;
+    for (const GridPart& P : g.parts)
+      rules.push_back(element_rule(P.ref, order));
+    const Quadrature q0 = element_rule(g.mixed ? g.parts[0].ref : g.ref, order);
+    const int d = g.ref.dim;
     parallel_for(g.ne, [&](int64_t b, int64_t e) {
       for (int64_t el = b; el < e; ++el) {
+        const RefElement& r = g.eref(el);
+        const Quadrature& q = g.mixed ? rules[g.etype[el]] : q0;
         const int32_t* c = &g.conn[el * g.ref.nverts];
         double x0[3], J[3][3];
         for (int a = 0; a < d; ++a)
           x0[a] = g.coords[int64_t(c[0]) * d + a];
         for (int k = 0; k < d; ++k) {
-          const int lv = g.ref.simplex ? (k + 1) : (1 << k);
+          const int lv = r.simplex ? (k + 1) : (1 << k);
           for (int a = 0; a < d; ++a)
             J[a][k] = g.coords[int64_t(c[lv]) * d + a] - x0[a];
         }
-        for (int p = 0; p < nq; ++p)
+        for (int p = 0; p < q.size(); ++p)
           for (int a = 0; a < d; ++a) {
             double s = x0[a];
             for (int k = 0; k < d; ++k)
               s += J[a][k] * q.x[p * d + k];
-            host_out[(el * nq + p) * d + a] = s;
+            host_out[(off[el] + p) * d + a] = s;
           }
       }
     });
+  });
+}
+
+hgdt_status hgdt_grid_quadrature_offsets(hgdt_grid grid, int order, int64_t* host_out)
+{
+  return guarded([&] {
+    require(grid && host_out && order >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    const std::vector<int64_t> off = element_point_offsets(*grid->g, order);
+    std::copy(off.begin(), off.end(), host_out);
   });
 }
 
@@ -367,17 +442,18 @@ hgdt_status hgdt_grid_boundary_intersections(hgdt_grid grid, double* centers, do
     int64_t bi = 0;
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f) {
-        if (g.nb[e * nf + f] >= 0)
+        if (g.nb[e * nf + f] != -1)
           continue;
+        const RefElement& r = g.eref(e);
         double cen[3] = {0, 0, 0}, ecen[3] = {0, 0, 0}, v[4][3] = {};
         for (int k = 0; k < fk; ++k) {
-          const int32_t vid = g.conn[e * nv + g.ref.faces[f][k]];
+          const int32_t vid = g.conn[e * nv + r.faces[f][k]];
           for (int a = 0; a < d; ++a)
             v[k][a] = g.coords[int64_t(vid) * d + a], cen[a] += v[k][a] / fk;
         }
-        for (int k = 0; k < nv; ++k)
+        for (int k = 0; k < r.nverts; ++k)
           for (int a = 0; a < d; ++a)
-            ecen[a] += g.coords[int64_t(g.conn[e * nv + k]) * d + a] / nv;
+            ecen[a] += g.coords[int64_t(g.conn[e * nv + k]) * d + a] / r.nverts;
         double nrm[3] = {0, 0, 0};
         if (d == 2) {
           nrm[0] = v[1][1] - v[0][1], nrm[1] = -(v[1][0] - v[0][0]);
@@ -417,11 +493,11 @@ hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, doub
     int64_t bi = 0;
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f) {
-        if (g.nb[e * nf + f] >= 0)
+        if (g.nb[e * nf + f] != -1)
           continue;
         double P[3][3];
         for (int k = 0; k < d; ++k) {
-          const int32_t v = g.conn[e * g.ref.nverts + g.ref.faces[f][k]];
+          const int32_t v = g.conn[e * g.ref.nverts + g.eref(e).faces[f][k]];
           for (int a = 0; a < d; ++a)
             P[k][a] = g.coords[int64_t(v) * d + a];
         }
@@ -438,6 +514,66 @@ hgdt_status hgdt_grid_boundary_quadrature_points(hgdt_grid grid, int order, doub
   });
 }
 
+// the DoFs of a space on a mixed grid: as on one element type (continuous order 1: the vertices; order 2:
+// one DoF per subentity in order of first appearance in the element walk; discontinuous: element offsets
+// and local index), every element with the Lagrange nodes of its own type (the local finite element of its
+// geometry type, ContinuousMapper / DiscontinuousMapper over the grid's types,
+// spaces/mapper/continuous.hh:117-182, spaces/mapper/discontinuous.hh:102-125)
+static void make_mixed_space(Space& s)
+{
+  const Grid& g = *s.grid;
+  const int np = int(g.parts.size());
+  std::vector<std::vector<std::vector<int>>> nodes(np);
+  s.parts = std::vector<SpacePart>(np);
+  s.local_size = 0;
+  for (int p = 0; p < np; ++p) {
+    nodes[p] = lagrange_node_vertices(g.parts[p].ref, s.order);
+    s.parts[p].local_size = int(nodes[p].size());
+    s.local_size = std::max(s.local_size, s.parts[p].local_size);
+  }
+  const int ls = s.local_size, nv = g.ref.nverts;
+  s.dofs.assign(g.ne * ls, -1);
+  if (s.kind == HGDT_CONTINUOUS_LAGRANGE && s.order == 1) {
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int i = 0; i < s.elem_size(e); ++i)
+        s.dofs[e * ls + i] = g.conn[e * nv + nodes[g.etype[e]][i][0]];
+    s.size = g.nv;
+  } else if (s.kind == HGDT_CONTINUOUS_LAGRANGE) {
+    std::map<std::array<int32_t, 4>, int32_t> ids;
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int i = 0; i < s.elem_size(e); ++i) {
+        std::array<int32_t, 4> key;
+        key.fill(-1);
+        int c = 0;
+        for (int lv : nodes[g.etype[e]][i])
+          key[c++] = g.conn[e * nv + lv];
+        std::sort(key.begin(), key.begin() + c);
+        auto it = ids.find(key);
+        if (it == ids.end())
+          it = ids.emplace(key, int32_t(ids.size())).first;
+        s.dofs[e * ls + i] = it->second;
+      }
+    s.size = int64_t(ids.size());
+  } else {
+    int64_t off = 0;
+    for (int64_t e = 0; e < g.ne; ++e)
+      for (int i = 0; i < s.elem_size(e); ++i)
+        s.dofs[e * ls + i] = int32_t(off++);
+    s.size = off;
+    require(s.size < (int64_t(1) << 31), HGDT_ERR_NOT_SUPPORTED, "too many DoFs for 32-bit indices");
+  }
+  s.inst_dof = {0, s.size};
+  for (int p = 0; p < np; ++p) {
+    SpacePart& P = s.parts[p];
+    P.dofs.reserve(g.parts[p].ne() * P.local_size);
+    for (int32_t e : g.parts[p].eid)
+      for (int i = 0; i < P.local_size; ++i)
+        P.dofs.push_back(s.dofs[int64_t(e) * ls + i]);
+    P.d_dofs.upload(P.dofs);
+  }
+  s.d_dofs.upload(s.dofs);
+}
+
 hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space)
 {
   return guarded([&] {
@@ -450,6 +586,11 @@ hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* s
     s->grid = grid->g;
     s->kind = kind, s->order = order;
     const Grid& g = *s->grid;
+    if (g.mixed) {
+      make_mixed_space(*s);
+      *space = new hgdt_space_s{s};
+      return;
+    }
     const auto nodes = lagrange_node_vertices(g.ref, order);
     s->local_size = int(nodes.size());
     const int ls = s->local_size;

@@ -41,6 +41,11 @@ struct ElementParams {
   int64_t ne;
   const double* coords;
   const int32_t* conn;
+  // one part of a mixed grid: the grid index of every element (coefficient fields and per-element sources
+  // are indexed by it) and, per source given at quadrature points, where each element's points start
+  // (NULL: element e, point q at e * npts + q)
+  const int32_t* eid;
+  const int64_t* src_qoff[MAX_SOURCES];
   const double* qtab; // points (dim each) then weights, per rule
   int nforms;
   int kind[MAX_EFORMS];
@@ -339,6 +344,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= P.ne)
     return;
+  const int64_t ge = P.eid ? P.eid[e] : e;
   Geo<D> g;
   load_geometry<GEOM>(P.coords, P.conn, e, g);
   const double* __restrict__ qt = P.qtab;
@@ -350,7 +356,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
       for (int j = 0; j < ND; ++j)
         A[i][j] = 0.;
     for (int f = 0; f < P.nforms; ++f) {
-      const double c = P.field[f] ? P.field[f][e] : P.value[f];
+      const double c = P.field[f] ? P.field[f][ge] : P.value[f];
       const double s = P.scaling[f];
       if (P.kind[f] == EF_LAPLACE && T::SIMPLEX && ORDER == 1) {
         // P1 gradients are constant: the integrand is constant on the element and any
@@ -441,9 +447,9 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
         if (P.src_kind[sidx] == HGDT_SOURCE_CONSTANT)
           fv = P.src_value[sidx];
         else if (P.src_kind[sidx] == HGDT_SOURCE_PER_ELEMENT)
-          fv = P.src_data[sidx][e];
+          fv = P.src_data[sidx][ge];
         else
-          fv = P.src_data[sidx][e * R.npts + q];
+          fv = P.src_data[sidx][(P.src_qoff[sidx] ? P.src_qoff[sidx][e] : e * R.npts) + q];
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           b[i] += (phi[i] * fv) * g.det * w;
@@ -476,6 +482,7 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
   if (lane >= P.ne * ND)
     return;
   const int64_t e = lane / ND;
+  const int64_t ge = P.eid ? P.eid[e] : e;
   const int i = int(lane % ND);
   const int ix = i % 3, iy = (i / 3) % 3, iz = i / 9;
   Geo<D> g;
@@ -495,7 +502,7 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
     for (int j = 0; j < ND; ++j)
       row[j] = 0.;
     for (int f = 0; f < P.nforms; ++f) {
-      const double c = P.field[f] ? P.field[f][e] : P.value[f];
+      const double c = P.field[f] ? P.field[f][ge] : P.value[f];
       const double s = P.scaling[f];
       const QRule R = P.rule[f];
       for (int q = 0; q < R.npts; ++q) {
@@ -560,9 +567,9 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
         if (P.src_kind[sidx] == HGDT_SOURCE_CONSTANT)
           fv = P.src_value[sidx];
         else if (P.src_kind[sidx] == HGDT_SOURCE_PER_ELEMENT)
-          fv = P.src_data[sidx][e];
+          fv = P.src_data[sidx][ge];
         else
-          fv = P.src_data[sidx][e * R.npts + q];
+          fv = P.src_data[sidx][(P.src_qoff[sidx] ? P.src_qoff[sidx][e] : e * R.npts) + q];
         b += (phii * fv) * g.det * w;
       }
     }
@@ -1097,20 +1104,24 @@ __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* _
                                                      const double* __restrict__ gval,
                                                      const double* __restrict__ ggrad,
                                                      const double* __restrict__ kappa, double kappa_value,
-                                                     double* __restrict__ out)
+                                                     double* __restrict__ out, const int32_t* __restrict__ eid,
+                                                     const int64_t* __restrict__ qoff, int64_t ne_out)
 {
+  // eid, qoff (one part of a mixed grid): the grid index of every element, where its points start in
+  // gval / ggrad; the element's values go to out[eid], out[ne_out + eid] (NULL: the element's own index)
   using T = Traits<GEOM>;
   constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= ne)
     return;
+  const int64_t ge = eid ? eid[e] : e;
   Geo<D> g;
   load_geometry<GEOM>(coords, conn, e, g);
   double ue[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i)
     ue[i] = u[dofs[e * ND + i]];
-  const double k = kappa ? kappa[e] : kappa_value;
+  const double k = kappa ? kappa[ge] : kappa_value;
   double l2 = 0., h1 = 0.;
   for (int q = 0; q < R.npts; ++q) {
     double xh[D];
@@ -1133,7 +1144,7 @@ __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* _
       for (int a = 0; a < D; ++a)
         eg[a] += ue[i] * G[a];
     }
-    const int64_t qi = e * R.npts + q;
+    const int64_t qi = (qoff ? qoff[e] : e * R.npts) + q;
     if (gval)
       ev -= gval[qi];
     if (ggrad)
@@ -1148,8 +1159,8 @@ __global__ void __launch_bounds__(256) k_error_local(int64_t ne, const double* _
     l2 += (ev * ev) * f;
     h1 += kd * f;
   }
-  out[e] = l2;
-  out[ne + e] = h1;
+  out[ge] = l2;
+  out[ne_out + ge] = h1;
 }
 
 // fixed-order sums of element values over chunks (one block of 256 per chunk and component);
@@ -1291,24 +1302,30 @@ void dispatch_faces(int geom, int order, F&& f)
 } // namespace
 
 // quadrature tables of the appended forms: element rules and face rules (dim - 1 coordinates per point)
-static void build_tables(const Operator& op, QTable& etab, QTable& ltab, ElementParams& P, FaceParams& C,
-                         FaceParams& B)
+// (one ElementParams per part of a mixed grid, each with the rules of its element type)
+static void build_tables(const Operator& op, QTable& etab, QTable& ltab, std::vector<ElementParams>& PE,
+                         FaceParams& C, FaceParams& B)
 {
   const Grid& g = *op.space->grid;
   const int k = op.space->order;
-  P.nforms = int(op.eforms.size());
-  for (int f = 0; f < P.nforms; ++f) {
-    const auto& F = op.eforms[f];
-    P.kind[f] = F.kind, P.field[f] = F.field, P.value[f] = F.value, P.scaling[f] = F.scaling;
-    // integrand order: 0 (piecewise constant coefficient) + test + ansatz (laplace.hh:74-79, product.hh:88-100)
-    P.rule[f] = etab.add(element_rule(g.ref, 0 + k + k + F.over_integrate));
-  }
-  P.nsrc = int(op.sources.size());
-  for (int s = 0; s < P.nsrc; ++s) {
-    const auto& S = op.sources[s];
-    P.src_kind[s] = S.kind, P.src_data[s] = S.data, P.src_value[s] = S.value;
-    // product integrand with the source as ansatz: 0 + test order + f order (conversion.hh:90-93)
-    P.src_rule[s] = etab.add(element_rule(g.ref, 0 + k + S.f_order + S.over_integrate));
+  PE.assign(std::max<size_t>(1, g.parts.size()), ElementParams{});
+  for (size_t part = 0; part < PE.size(); ++part) {
+    ElementParams& P = PE[part];
+    const RefElement& r = g.mixed ? g.parts[part].ref : g.ref;
+    P.nforms = int(op.eforms.size());
+    for (int f = 0; f < P.nforms; ++f) {
+      const auto& F = op.eforms[f];
+      P.kind[f] = F.kind, P.field[f] = F.field, P.value[f] = F.value, P.scaling[f] = F.scaling;
+      // integrand order: 0 (piecewise constant coefficient) + test + ansatz (laplace.hh:74-79, product.hh:88-100)
+      P.rule[f] = etab.add(element_rule(r, 0 + k + k + F.over_integrate));
+    }
+    P.nsrc = int(op.sources.size());
+    for (int s = 0; s < P.nsrc; ++s) {
+      const auto& S = op.sources[s];
+      P.src_kind[s] = S.kind, P.src_data[s] = S.data, P.src_value[s] = S.value;
+      // product integrand with the source as ansatz: 0 + test order + f order (conversion.hh:90-93)
+      P.src_rule[s] = etab.add(element_rule(r, 0 + k + S.f_order + S.over_integrate));
+    }
   }
   auto fill_face = [&](const std::vector<FaceFormDesc>& forms, FaceParams& F) {
     F.nforms = int(forms.size());
@@ -1342,17 +1359,38 @@ void Operator::prepare_assembly()
   if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->order == 2 &&
       space->grid->ref.geom == HGDT_HEX)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for Q2 on quadrilaterals, not hexahedra");
+  if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->grid->mixed)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals need a grid of one element type");
   if (bsources.size() > size_t(MAX_BSOURCES))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
 
   if (tables_ready)
     return;
   QTable etab, ltab;
-  ElementParams P{};
+  std::vector<ElementParams> PE;
   FaceParams C{}, B{};
-  build_tables(*this, etab, ltab, P, C, B);
+  build_tables(*this, etab, ltab, PE, C, B);
   d_qtab_elem.upload(etab.data.empty() ? std::vector<double>{0.} : etab.data);
   d_qtab_line.upload(ltab.data.empty() ? std::vector<double>{0.} : ltab.data);
+  // mixed grids: the start of every element's points in the element-ordered quadrature-point data
+  const Grid& g = *space->grid;
+  part_qoff.clear();
+  if (g.mixed)
+    for (size_t s = 0; s < sources.size(); ++s) {
+      const std::vector<int64_t> off =
+          sources[s].kind == HGDT_SOURCE_PER_QUADRATURE_POINT
+              ? element_point_offsets(g, 0 + space->order + sources[s].f_order + sources[s].over_integrate)
+              : std::vector<int64_t>{};
+      for (const GridPart& P : g.parts) {
+        part_qoff.push_back(std::make_unique<DeviceArray<int64_t>>());
+        if (off.empty())
+          continue;
+        std::vector<int64_t> po(P.eid.size());
+        for (size_t el = 0; el < po.size(); ++el)
+          po[el] = off[P.eid[el]];
+        part_qoff.back()->upload(po);
+      }
+    }
   tables_ready = true;
 }
 
@@ -1361,31 +1399,42 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
   const Grid& g = *op.space->grid;
   const ScatterMaps& M = op.maps;
   QTable etab, ltab;
-  ElementParams P{};
+  std::vector<ElementParams> PE;
   FaceParams C{}, B{};
-  build_tables(op, etab, ltab, P, C, B);
+  build_tables(op, etab, ltab, PE, C, B);
   const int eb = op.launch[0].block, fb = op.launch[1].block, gb = op.launch[2].block;
   const bool emat = matrix && !op.eforms.empty();
   if (emat || !op.sources.empty()) {
-    P.ne = g.ne;
-    P.coords = g.d_coords.ptr;
-    P.conn = g.d_conn.ptr;
-    P.qtab = op.d_qtab_elem.ptr;
-    P.staging = op.d_staging.ptr;
-    P.elem_base = M.elem_base;
-    P.rhs_base = M.rhs_base;
-    P.write_matrix = emat ? 1 : 0;
-    if (g.ref.geom == HGDT_HEX && op.space->order == 2) // 27 local DoFs: one lane per element row
-      hipLaunchKernelGGL((k_element_rows<HGDT_HEX, 2>), dim3(grid_for(g.ne * 27, eb)), dim3(eb), 0, stream, P);
-    else
-      dispatch_fe(g.ref.geom, op.space->order, [&](auto G, auto O) {
-        if constexpr (decltype(G)::value == HGDT_HEX && decltype(O)::value == 2)
-          HGDT_THROW(HGDT_ERR_OPERATOR, "Q2 hexahedra are assembled by row lanes");
-        else
-          hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, eb)),
-                             dim3(eb), 0, stream, P);
-      });
-    HGDT_HIP(hipGetLastError());
+    // one launch per element type (a mixed grid's parts, each over its own elements and staging)
+    for (size_t part = 0; part < PE.size(); ++part) {
+      ElementParams& P = PE[part];
+      const RefElement& r = g.mixed ? g.parts[part].ref : g.ref;
+      const int64_t ne = g.mixed ? g.parts[part].ne() : g.ne;
+      if (ne == 0)
+        continue;
+      P.ne = ne;
+      P.coords = g.d_coords.ptr;
+      P.conn = g.mixed ? g.parts[part].d_conn.ptr : g.d_conn.ptr;
+      P.eid = g.mixed ? g.parts[part].d_eid.ptr : nullptr;
+      for (int s = 0; s < P.nsrc && g.mixed; ++s)
+        P.src_qoff[s] = op.part_qoff[s * g.parts.size() + part]->ptr;
+      P.qtab = op.d_qtab_elem.ptr;
+      P.staging = op.d_staging.ptr;
+      P.elem_base = M.part_elem_base[part];
+      P.rhs_base = M.part_rhs_base[part];
+      P.write_matrix = emat ? 1 : 0;
+      if (r.geom == HGDT_HEX && op.space->order == 2) // 27 local DoFs: one lane per element row
+        hipLaunchKernelGGL((k_element_rows<HGDT_HEX, 2>), dim3(grid_for(ne * 27, eb)), dim3(eb), 0, stream, P);
+      else
+        dispatch_fe(r.geom, op.space->order, [&](auto G, auto O) {
+          if constexpr (decltype(G)::value == HGDT_HEX && decltype(O)::value == 2)
+            HGDT_THROW(HGDT_ERR_OPERATOR, "Q2 hexahedra are assembled by row lanes");
+          else
+            hipLaunchKernelGGL((k_element_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(ne, eb)),
+                               dim3(eb), 0, stream, P);
+        });
+      HGDT_HIP(hipGetLastError());
+    }
   }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
   if (matrix && !op.cforms.empty() && nin > 0) {
@@ -1448,12 +1497,14 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "the adjoint gradient is defined for a Laplace form");
   if (space->order != 1)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel is provided for order 1 spaces");
+  if (g.mixed)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel needs a grid of one element type");
   prepare_assembly();
   QTable etab, ltab;
-  ElementParams P{};
+  std::vector<ElementParams> PE;
   FaceParams C{}, B{};
-  build_tables(*this, etab, ltab, P, C, B);
-  const QRule R = P.rule[form_index];
+  build_tables(*this, etab, ltab, PE, C, B);
+  const QRule R = PE[0].rule[form_index];
   const int block = launch[0].block;
   dispatch_geom(g.ref.geom, [&](auto G) {
     hipLaunchKernelGGL(k_adjoint_gradient<decltype(G)::value>, dim3(grid_for(g.ne, block)), dim3(block), 0, stream,
@@ -1472,7 +1523,10 @@ void error_norms(const Space& s, const double* dofs, int order, const double* gv
   if (order < 0 || order > 20)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "quadrature order must be in [0, 20]");
   QTable tab;
-  const QRule R = tab.add(element_rule(g.ref, order));
+  const QRule R = tab.add(element_rule(g.mixed ? g.parts[0].ref : g.ref, order));
+  std::vector<QRule> part_rule; // mixed grids: the rule of every part
+  for (const GridPart& P : g.parts)
+    part_rule.push_back(tab.add(element_rule(P.ref, order)));
   DeviceArray<double> d_tab, d_vals, d_part;
   DeviceArray<int64_t> d_chunk;
   d_tab.upload(tab.data);
@@ -1487,12 +1541,47 @@ void error_norms(const Space& s, const double* dofs, int order, const double* gv
   d_chunk.upload(chunk0);
   d_vals.allocate(std::max<int64_t>(2 * g.ne, 1));
   d_part.allocate(std::max<int64_t>(2 * nchunks, 1));
+  if (g.ne > 0 && g.mixed) {
+    // one launch per element type, every element's values at its grid index
+    const std::vector<int64_t> off = element_point_offsets(g, order);
+    std::vector<DeviceArray<int64_t>> d_qoff(g.parts.size());
+    for (size_t p = 0; p < g.parts.size(); ++p) {
+      const GridPart& P = g.parts[p];
+      std::vector<int64_t> po(P.eid.size());
+      for (size_t el = 0; el < po.size(); ++el)
+        po[el] = off[P.eid[el]];
+      d_qoff[p].upload(po);
+      if (P.ne() == 0)
+        continue;
+      dispatch_fe(P.ref.geom, s.order, [&](auto G, auto O) {
+        hipLaunchKernelGGL((k_error_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(P.ne(), 256)),
+                           dim3(256), 0, stream, P.ne(), g.d_coords.ptr, P.d_conn.ptr, s.parts[p].d_dofs.ptr,
+                           d_tab.ptr, part_rule[p], dofs, gval, ggrad, kappa, kappa_value, d_vals.ptr, P.d_eid.ptr,
+                           d_qoff[p].ptr, g.ne);
+      });
+      HGDT_HIP(hipGetLastError());
+    }
+    // the offsets are freed at return, after the stream synchronization below
+    hipLaunchKernelGGL(k_sum_chunks, dim3(unsigned(nchunks), 2), dim3(256), 0, stream, d_vals.ptr, g.ne, d_chunk.ptr,
+                       d_part.ptr, nchunks);
+    HGDT_HIP(hipGetLastError());
+    std::vector<double> part(2 * nchunks, 0.);
+    HGDT_HIP(hipMemcpyAsync(part.data(), d_part.ptr, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HGDT_HIP(hipStreamSynchronize(stream));
+    for (int comp = 0; comp < 2; ++comp) {
+      double acc = 0.;
+      for (int64_t c = 0; c < nchunks; ++c)
+        acc += part[comp * nchunks + c];
+      host_out[comp] = acc;
+    }
+    return;
+  }
   if (g.ne > 0) {
     dispatch_fe(g.ref.geom, s.order, [&](auto G, auto O) {
       hipLaunchKernelGGL((k_error_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, 256)), dim3(256),
                          0, stream, g.ne,
                          g.d_coords.ptr, g.d_conn.ptr, s.d_dofs.ptr, d_tab.ptr, R, dofs, gval, ggrad, kappa,
-                         kappa_value, d_vals.ptr);
+                         kappa_value, d_vals.ptr, nullptr, nullptr, g.ne);
     });
     HGDT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sum_chunks, dim3(unsigned(nchunks), 2), dim3(256), 0, stream, d_vals.ptr, g.ne, d_chunk.ptr,
@@ -1570,6 +1659,8 @@ void error_norms_noalloc(const Space& s, const double* dofs, int order, const do
     HGDT_THROW(HGDT_ERR_SPACE, "unknown space kind");
   if (order < 0 || order > 20)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "quadrature order must be in [0, 20]");
+  if (g.mixed)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "no-allocation error norms need a grid of one element type");
   QTable tab;
   const QRule R = tab.add(element_rule(g.ref, order));
   if (int64_t(tab.data.size()) > NA_TABLE)
@@ -1618,7 +1709,7 @@ void error_norms_noalloc(const Space& s, const double* dofs, int order, const do
                            dim3(256), 0, stream, ne, g.d_coords.ptr, g.d_conn.ptr + e0 * nv,
                            s.d_dofs.ptr + e0 * nd, d_tab, R, dofs, gval ? gval + e0 * R.npts : nullptr,
                            ggrad ? ggrad + e0 * R.npts * D : nullptr, kappa ? kappa + e0 : nullptr, kappa_value,
-                           d_vals);
+                           d_vals, nullptr, nullptr, ne);
       });
       HGDT_HIP(hipGetLastError());
       // window w's chunk starts sit at table offset win0[w] + w; its partials at 2 win0[w] as [comp][chunk]

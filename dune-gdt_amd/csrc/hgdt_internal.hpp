@@ -143,8 +143,32 @@ struct Lattice {
   double lo[3] = {0., 0., 0.}, h[3] = {0., 0., 0.};
 };
 
-struct Grid {
+// the elements of one geometry type of a mixed grid: their global indices (ascending) and their
+// connectivity in that type's reference vertex order
+struct GridPart {
   RefElement ref;
+  std::vector<int32_t> eid;  // global element index of every element of the part
+  std::vector<int32_t> conn; // eid.size() * ref.nverts
+  DeviceArray<int32_t> d_eid, d_conn;
+  int64_t ne() const
+  {
+    return int64_t(eid.size());
+  }
+};
+
+struct Grid {
+  // homogeneous grids: the element type.  Mixed grids (triangles and quadrilaterals): geom HGDT_MIXED,
+  // nverts / nfaces the largest of the types (the strides of conn, nb and nbface, padded with -1 vertices
+  // and -2 neighbours), no faces or vertex coordinates: every element's own type is eref(e)
+  RefElement ref;
+  bool mixed = false;
+  std::vector<int8_t> etype;    // mixed: part of every element
+  std::vector<int32_t> elocal;  // mixed: index of every element in its part
+  std::vector<GridPart> parts;  // mixed: one per geometry type present, ascending geometry id
+  const RefElement& eref(int64_t e) const
+  {
+    return mixed ? parts[etype[e]].ref : ref;
+  }
   bool structured = false;
   std::vector<Lattice> lattice; // one per instance when every instance is a structured grid, else empty
   int64_t nv = 0, ne = 0;
@@ -166,14 +190,30 @@ struct Grid {
   void check_affine() const;
 };
 
+// where the quadrature points of every element (rule of the given order of its own type) start in the
+// element-ordered layout of quadrature-point data: ne + 1 offsets
+std::vector<int64_t> element_point_offsets(const Grid& g, int order);
+
+// the DoFs of the elements of one part of a mixed grid, in the part's element order
+struct SpacePart {
+  int local_size = 0;
+  std::vector<int32_t> dofs; // part ne * local_size
+  DeviceArray<int32_t> d_dofs;
+};
+
 struct Space {
   std::shared_ptr<Grid> grid;
   int kind = HGDT_CONTINUOUS_LAGRANGE, order = 1;
-  int local_size = 0;
+  int local_size = 0;               // mixed grids: the largest of the parts'
   int64_t size = 0;
-  std::vector<int32_t> dofs;        // ne * local_size
+  std::vector<int32_t> dofs;        // ne * local_size (mixed grids: -1 past an element's own local size)
   std::vector<int64_t> inst_dof{0}; // DoF offsets of the instances
   DeviceArray<int32_t> d_dofs;
+  std::vector<SpacePart> parts;     // mixed grids: one per grid part
+  int elem_size(int64_t e) const    // local size of element e
+  {
+    return parts.empty() ? local_size : parts[grid->etype[e]].local_size;
+  }
 };
 
 enum ElementFormKind { EF_LAPLACE = 0, EF_PRODUCT = 1 };
@@ -232,6 +272,8 @@ struct Launch {
 // order the reference's grid walk would add them.
 struct ScatterMaps {
   int64_t elem_base = 0, inner_base = 0, bnd_base = 0, rhs_base = 0, brhs_base = 0, staging_size = 0;
+  // mixed grids: the element matrices and right-hand sides of every part start here (entry-major per part)
+  std::vector<int64_t> part_elem_base, part_rhs_base;
   std::vector<int32_t> inner_in, inner_fin, inner_out, inner_fout; // inner intersections visited once
   std::vector<int32_t> bnd_e, bnd_f;                               // boundary intersections
   DeviceArray<uint32_t> d_slot_ptr, d_contrib;
@@ -386,6 +428,8 @@ struct Operator {
   bool dirichlet_built = false;
   ScatterMaps maps;
   DeviceArray<double> d_qtab_elem, d_qtab_line; // quadrature tables of the appended forms
+  // mixed grids: per (part, source given at quadrature points) where every element's points start
+  std::vector<std::unique_ptr<DeviceArray<int64_t>>> part_qoff;
   bool tables_ready = false;
   int requested_format = HGDT_FORMAT_AUTO;
   bool allow_symmetric = true;

@@ -248,6 +248,23 @@ Quadrature element_rule(const RefElement& ref, int order)
   return q;
 }
 
+std::vector<int64_t> element_point_offsets(const Grid& g, int order)
+{
+  std::vector<int64_t> off(g.ne + 1, 0);
+  if (!g.mixed) {
+    const int64_t nq = element_rule(g.ref, order).size();
+    for (int64_t e = 0; e <= g.ne; ++e)
+      off[e] = e * nq;
+    return off;
+  }
+  std::vector<int64_t> nq;
+  for (const GridPart& P : g.parts)
+    nq.push_back(element_rule(P.ref, order).size());
+  for (int64_t e = 0; e < g.ne; ++e)
+    off[e + 1] = off[e] + nq[g.etype[e]];
+  return off;
+}
+
 void Grid::check_affine() const
 {
   if (ref.simplex)
@@ -256,6 +273,8 @@ void Grid::check_affine() const
   std::atomic<bool> bad{false};
   parallel_for(ne, [&](int64_t b, int64_t e) {
     for (int64_t el = b; el < e && !bad; ++el) {
+      if (eref(el).simplex)
+        continue;
       const int32_t* c = &conn[el * k];
       for (int v = 0; v < k; ++v)
         for (int a = 0; a < d; ++a) {
@@ -281,27 +300,38 @@ void Grid::finalize(bool neighbours_known)
     count_faces_and_upload();
     return;
   }
+  // on a mixed grid the faces past an element's own count are absent: neighbour -2, never matched
   const int nf = ref.nfaces, fk = ref.face_nverts, k = ref.nverts;
   const int64_t F = ne * nf;
   nb.assign(F, -1);
   nbface.assign(F, -1);
+  auto present = [&](int64_t ef) { return !mixed || int(ef % nf) < eref(ef / nf).nfaces; };
   std::vector<int64_t> start(nv + 1, 0);
   auto face_key = [&](int64_t ef, int32_t* key) {
     const int64_t e = ef / nf;
     const int f = int(ef % nf);
+    const RefElement& r = eref(e);
     for (int a = 0; a < fk; ++a)
-      key[a] = conn[e * k + ref.faces[f][a]];
+      key[a] = conn[e * k + r.faces[f][a]];
     std::sort(key, key + fk);
   };
+  int64_t nfaces = 0;
   for (int64_t ef = 0; ef < F; ++ef) {
+    if (!present(ef)) {
+      nb[ef] = -2;
+      continue;
+    }
     int32_t key[4];
     face_key(ef, key);
     ++start[key[0] + 1];
+    ++nfaces;
   }
   for (int64_t v = 0; v < nv; ++v)
     start[v + 1] += start[v];
-  std::vector<int64_t> bucket(F), fill(start.begin(), start.end() - 1);
+  std::vector<int64_t> bucket(std::max<int64_t>(nfaces, 1)), fill(start.begin(), start.end() - 1);
   for (int64_t ef = 0; ef < F; ++ef) {
+    if (!present(ef))
+      continue;
     int32_t key[4];
     face_key(ef, key);
     bucket[fill[key[0]]++] = ef;
@@ -339,13 +369,15 @@ void Grid::count_faces_and_upload()
   const int64_t F = ne * nf;
   n_inner = 0, n_boundary = 0;
   for (int64_t ef = 0; ef < F; ++ef) {
-    if (nb[ef] < 0)
+    if (nb[ef] == -1)
       ++n_boundary;
     else if (nb[ef] > ef / nf)
       ++n_inner;
   }
   d_coords.upload(coords);
   d_conn.upload(conn);
+  for (GridPart& p : parts)
+    p.d_eid.upload(p.eid), p.d_conn.upload(p.conn);
 }
 
 // ----------------------------------------------------------------------------
@@ -366,13 +398,14 @@ static Adjacency dof_adjacency(const Space& s)
   Adjacency A;
   A.ptr.assign(n + 1, 0);
   for (int64_t i = 0; i < ne * ls; ++i)
-    ++A.ptr[s.dofs[i] + 1];
+    if (s.dofs[i] >= 0) // (mixed grids: -1 past an element's own DoFs)
+      ++A.ptr[s.dofs[i] + 1];
   for (int64_t i = 0; i < n; ++i)
     A.ptr[i + 1] += A.ptr[i];
-  A.elem.resize(ne * ls), A.local.resize(ne * ls);
+  A.elem.resize(A.ptr[n]), A.local.resize(A.ptr[n]);
   std::vector<int64_t> fill(A.ptr.begin(), A.ptr.end() - 1);
   for (int64_t e = 0; e < ne; ++e)
-    for (int i = 0; i < ls; ++i) {
+    for (int i = 0; i < s.elem_size(e); ++i) {
       const int64_t p = fill[s.dofs[e * ls + i]]++;
       A.elem[p] = int32_t(e), A.local[p] = int8_t(i);
     }
@@ -397,13 +430,13 @@ void Operator::build_pattern()
     for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
       const int64_t e = A.elem[p];
       if (stencil != HGDT_STENCIL_INTERSECTION)
-        for (int j = 0; j < ls; ++j)
+        for (int j = 0; j < src.elem_size(e); ++j)
           c.push_back(src.dofs[e * ls + j]);
       if (stencil != HGDT_STENCIL_ELEMENT)
         for (int f = 0; f < nf; ++f) {
           const int64_t o = g.nb[e * nf + f];
           if (o >= 0)
-            for (int j = 0; j < ls; ++j)
+            for (int j = 0; j < src.elem_size(o); ++j)
               c.push_back(src.dofs[o * ls + j]);
         }
     }
@@ -504,24 +537,48 @@ void Operator::build_scatter_maps()
         inner_index[e * nf + f] = int32_t(M.inner_in.size());
         M.inner_in.push_back(int32_t(e)), M.inner_fin.push_back(f);
         M.inner_out.push_back(int32_t(o)), M.inner_fout.push_back(g.nbface[e * nf + f]);
-      } else if (o < 0) {
+      } else if (o == -1) {
         bnd_index[e * nf + f] = int32_t(M.bnd_e.size());
         M.bnd_e.push_back(int32_t(e)), M.bnd_f.push_back(f);
       }
     }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
-  // element staging holds the upper triangle of the (symmetric) element matrices, entry-major
-  const int64_t k2s = int64_t(ls) * (ls + 1) / 2;
-  auto tri = [ls](int i, int j) {
-    const int a = std::min(i, j), b = std::max(i, j);
-    return a * ls - a * (a - 1) / 2 + (b - a);
-  };
+  // mixed grids: the element matrices (and right-hand sides) part after part, each entry-major over the
+  // part's elements with the part's local size (the intersection forms need one element type)
+  const int np = std::max<int>(1, int(s.parts.size()));
+  auto part_ls = [&](int p) { return s.parts.empty() ? ls : s.parts[p].local_size; };
+  auto part_ne = [&](int p) { return s.parts.empty() ? ne : g.parts[p].ne(); };
+  M.part_elem_base.assign(np, 0), M.part_rhs_base.assign(np, 0);
+  int64_t elem_entries = 0, rhs_entries = 0;
+  for (int p = 0; p < np; ++p) {
+    M.part_elem_base[p] = elem_entries, M.part_rhs_base[p] = rhs_entries;
+    elem_entries += part_ne(p) * (int64_t(part_ls(p)) * (part_ls(p) + 1) / 2);
+    rhs_entries += part_ne(p) * part_ls(p);
+  }
   M.elem_base = 0;
-  M.inner_base = has_e ? ne * k2s : 0;
+  M.inner_base = has_e ? elem_entries : 0;
   M.bnd_base = M.inner_base + (has_c ? nin * 4 * k2 : 0);
   const int64_t rhs_base = M.bnd_base + (has_b ? nbd * k2 : 0);
   M.rhs_base = rhs_base;
-  M.brhs_base = rhs_base + (has_s ? ne * ls : 0); // boundary right-hand sides, ls per boundary face
+  for (int p = 0; p < np; ++p)
+    M.part_elem_base[p] += M.elem_base, M.part_rhs_base[p] += rhs_base;
+  M.brhs_base = rhs_base + (has_s ? rhs_entries : 0); // boundary right-hand sides, ls per boundary face
+  // element staging holds the upper triangle of the (symmetric) element matrices, entry-major
+  auto tri = [](int i, int j, int ls) {
+    const int a = std::min(i, j), b = std::max(i, j);
+    return a * ls - a * (a - 1) / 2 + (b - a);
+  };
+  // staging entry of element e's matrix entry (i, j) and right-hand side entry i
+  auto elem_slot = [&](int64_t e, int i, int j) {
+    const int p = s.parts.empty() ? 0 : g.etype[e];
+    const int64_t el = s.parts.empty() ? e : g.elocal[e];
+    return M.part_elem_base[p] + tri(i, j, part_ls(p)) * part_ne(p) + el;
+  };
+  auto rhs_slot = [&](int64_t e, int i) {
+    const int p = s.parts.empty() ? 0 : g.etype[e];
+    const int64_t el = s.parts.empty() ? e : g.elocal[e];
+    return M.part_rhs_base[p] + int64_t(i) * part_ne(p) + el;
+  };
   M.staging_size = M.brhs_base + (has_bs ? nbd * ls : 0);
   if (M.staging_size >= (int64_t(1) << 32))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "local-matrix staging exceeds 2^32 entries");
@@ -539,8 +596,8 @@ void Operator::build_scatter_maps()
       const int li = A.local[p];
       const int32_t* de = &src.dofs[e * ls];
       if (has_e)
-        for (int lj = 0; lj < ls; ++lj)
-          out.push_back({e * (nf + 1), uint32_t(M.elem_base + tri(li, lj) * ne + e), de[lj]});
+        for (int lj = 0; lj < src.elem_size(e); ++lj)
+          out.push_back({e * (nf + 1), uint32_t(elem_slot(e, li, lj)), de[lj]});
       for (int f = 0; f < nf; ++f) {
         const int64_t o = g.nb[e * nf + f];
         if (o >= 0 && has_c) {
@@ -562,7 +619,7 @@ void Operator::build_scatter_maps()
             for (int lj = 0; lj < ls; ++lj) // out_out
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 3 * k2 + li * ls + lj), de[lj]});
           }
-        } else if (o < 0 && has_b) {
+        } else if (o == -1 && has_b) {
           const int64_t bi = bnd_index[e * nf + f];
           for (int lj = 0; lj < ls; ++lj)
             out.push_back({e * (nf + 1) + 1 + f, uint32_t(M.bnd_base + bi * k2 + li * ls + lj), de[lj]});
@@ -620,14 +677,14 @@ void Operator::build_scatter_maps()
   // functional's contribution, then those of the element's boundary intersections in face order
   if (has_s || has_bs) {
     std::vector<uint32_t> rptr(n + 1, 0), rc;
-    rc.reserve(ne * ls * (has_s ? 1 : 0) + (has_bs ? nbd * ls : 0) + 1);
+    rc.reserve((has_s ? rhs_entries : 0) + (has_bs ? nbd * ls : 0) + 1);
     for (int64_t i = 0; i < n; ++i) {
       rptr[i] = uint32_t(rc.size());
       for (int64_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
         const int64_t e = A.elem[p];
         const int l = A.local[p];
         if (has_s)
-          rc.push_back(uint32_t(rhs_base + int64_t(l) * ne + e));
+          rc.push_back(uint32_t(rhs_slot(e, l)));
         if (has_bs)
           for (int f = 0; f < nf; ++f)
             if (bnd_index[e * nf + f] >= 0)
@@ -668,26 +725,30 @@ void Operator::build_dirichlet()
   std::vector<uint8_t> mask(n, 0);
   if (s.kind == HGDT_CONTINUOUS_LAGRANGE) {
     // local DoFs whose subentity lies in a boundary face (the face itself and its lower-dimensional
-    // subentities)
-    const auto nodes = lagrange_node_vertices(g.ref, s.order);
-    std::vector<std::vector<int>> face_nodes(nf);
-    for (int f = 0; f < nf; ++f)
-      for (int i = 0; i < ls; ++i) {
-        bool inside = true;
-        for (int v : nodes[i])
-          inside = inside && std::find(g.ref.faces[f], g.ref.faces[f] + fk, v) != g.ref.faces[f] + fk;
-        if (inside)
-          face_nodes[f].push_back(i);
-      }
+    // subentities), for every element type of the grid
+    std::vector<std::vector<std::vector<int>>> face_nodes;
+    for (size_t p = 0; p < std::max<size_t>(1, g.parts.size()); ++p) {
+      const RefElement& r = g.mixed ? g.parts[p].ref : g.ref;
+      const auto nodes = lagrange_node_vertices(r, s.order);
+      face_nodes.emplace_back(r.nfaces);
+      for (int f = 0; f < r.nfaces; ++f)
+        for (int i = 0; i < int(nodes.size()); ++i) {
+          bool inside = true;
+          for (int v : nodes[i])
+            inside = inside && std::find(r.faces[f], r.faces[f] + fk, v) != r.faces[f] + fk;
+          if (inside)
+            face_nodes.back()[f].push_back(i);
+        }
+    }
     // boundary intersections in walk order; only the Dirichlet ones (all, without a boundary info) constrain
     int64_t bi = 0;
     for (int64_t e = 0; e < g.ne; ++e)
       for (int f = 0; f < nf; ++f)
-        if (g.nb[e * nf + f] < 0) {
+        if (g.nb[e * nf + f] == -1) {
           const bool dir = dirichlet_faces.empty() || dirichlet_faces[bi] != 0;
           ++bi;
           if (dir)
-            for (int i : face_nodes[f])
+            for (int i : face_nodes[g.mixed ? g.etype[e] : 0][f])
               mask[s.dofs[e * ls + i]] = 1;
         }
   }

@@ -1513,8 +1513,8 @@ static void build_partitions(Operator& op)
 static bool blk_wanted(const Operator& op)
 {
   const Space& s = *op.space;
-  const bool fits = s.kind == HGDT_DISCONTINUOUS_LAGRANGE && s.local_size == 4 && op.n % 4 == 0 && op.n > 0 &&
-                    op.square();
+  const bool fits = s.kind == HGDT_DISCONTINUOUS_LAGRANGE && s.local_size == 4 && !s.grid->mixed && op.n % 4 == 0 &&
+                    op.n > 0 && op.square();
   return fits && (op.requested_format == HGDT_FORMAT_BLOCK || op.requested_format == HGDT_FORMAT_AUTO);
 }
 

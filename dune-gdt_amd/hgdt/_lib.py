@@ -86,7 +86,7 @@ _EXC = {
     10: out_of_memory,
 }
 
-QUAD, TRI, HEX, TET = 0, 1, 2, 3
+QUAD, TRI, HEX, TET, MIXED = 0, 1, 2, 3, 4
 CONTINUOUS_LAGRANGE, DISCONTINUOUS_LAGRANGE = 0, 1
 STENCIL_ELEMENT, STENCIL_INTERSECTION, STENCIL_ELEMENT_AND_INTERSECTION = 0, 1, 2
 SOURCE_CONSTANT, SOURCE_PER_ELEMENT, SOURCE_PER_QUADRATURE_POINT = 0, 1, 2
@@ -147,6 +147,7 @@ SIGNATURES = {
     "hgdt_stream_synchronize": (ctypes.c_int, [_vp]),
     "hgdt_grid_create_structured": (ctypes.c_int, [ctypes.c_int, _pi32, _pdbl, _pdbl, _pvp]),
     "hgdt_grid_create": (ctypes.c_int, [ctypes.c_int, _i64, _pdbl, _i64, _pi32, _pvp]),
+    "hgdt_grid_create_mixed": (ctypes.c_int, [_i64, _pdbl, _i64, _pi32, _pi64, _pi32, _pvp]),
     "hgdt_grid_create_union": (ctypes.c_int, [_pvp, _i32, _pvp]),
     "hgdt_grid_destroy": (ctypes.c_int, [_vp]),
     "hgdt_grid_info": (ctypes.c_int, [_vp, _pi64]),
@@ -154,6 +155,7 @@ SIGNATURES = {
     "hgdt_grid_copy_connectivity": (ctypes.c_int, [_vp, _pi32]),
     "hgdt_grid_instance_elements": (ctypes.c_int, [_vp, _pi64]),
     "hgdt_grid_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _pdbl, _pi32]),
+    "hgdt_grid_quadrature_offsets": (ctypes.c_int, [_vp, ctypes.c_int, _pi64]),
     "hgdt_grid_boundary_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _pdbl, _pi32]),
     "hgdt_space_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _pvp]),
     "hgdt_space_destroy": (ctypes.c_int, [_vp]),

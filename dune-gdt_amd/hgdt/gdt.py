@@ -208,6 +208,32 @@ class Grid:
         return cls(h)
 
     @classmethod
+    def mixed(cls, coords, elements):
+        """Triangles and quadrilaterals in one grid, element by element as GridFactory<G>::insertElement(
+        GeometryTypes::simplex(2) / cube(2), vertices) builds the mixed grids of
+        dune/gdt/test/spaces/base.hh:355-376: `elements` lists every element's vertices in the reference
+        vertex order (3: a triangle, 4: a quadrilateral)."""
+        c = np.ascontiguousarray(coords, np.float64)
+        if c.ndim != 2 or c.shape[1] != 2:
+            raise L.shapes_do_not_match(2, "mixed grids are two-dimensional: coords must be [num_vertices, 2]")
+        sizes = np.array([len(v) for v in elements], np.int64)
+        if np.any((sizes != 3) & (sizes != 4)):
+            raise L.wrong_input_given(1, "an element of a mixed grid has 3 (triangle) or 4 (quadrilateral) vertices")
+        geometry = np.where(sizes == 4, L.QUAD, L.TRI).astype(np.int32)
+        offsets = np.zeros(len(elements) + 1, np.int64)
+        offsets[1:] = np.cumsum(sizes)
+        verts = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32) for v in elements]), np.int32)
+        h = ctypes.c_void_p()
+        L.call("hgdt_grid_create_mixed", c.shape[0], c.ctypes.data_as(L._pdbl), len(elements),
+               geometry.ctypes.data_as(L._pi32), offsets.ctypes.data_as(L._pi64), verts.ctypes.data_as(L._pi32),
+               ctypes.byref(h))
+        return cls(h)
+
+    @property
+    def is_mixed(self):
+        return self.geometry == L.MIXED
+
+    @classmethod
     def union(cls, grids):
         """Disjoint union of independent grids: the batch of problem instances."""
         arr = (ctypes.c_void_p * len(grids))(*[g._h.value for g in grids])
@@ -247,11 +273,22 @@ class Grid:
         return out
 
     def quadrature_points(self, order):
-        """Physical coordinates of the element quadrature points of the given order: (E, nq, dim)."""
+        """Physical coordinates of the element quadrature points of the given order: (E, nq, dim); on a mixed
+        grid, whose element types have rules of different sizes, (P, dim) in element order (element e holds
+        the points quadrature_offsets(order)[e] .. [e + 1])."""
         nq = ctypes.c_int32()
         L.call("hgdt_grid_quadrature_points", self._h, order, None, ctypes.byref(nq))
-        out = np.empty((self.num_elements, nq.value, self.dimension))
+        if self.is_mixed:
+            out = np.empty((int(self.quadrature_offsets(order)[-1]), self.dimension))
+        else:
+            out = np.empty((self.num_elements, nq.value, self.dimension))
         L.call("hgdt_grid_quadrature_points", self._h, order, out.ctypes.data_as(L._pdbl), ctypes.byref(nq))
+        return out
+
+    def quadrature_offsets(self, order):
+        """Start of every element's quadrature points (rule of the given order) in the point layout: (E + 1,)."""
+        out = np.empty(self.num_elements + 1, np.int64)
+        L.call("hgdt_grid_quadrature_offsets", self._h, order, out.ctypes.data_as(L._pi64))
         return out
 
     def __del__(self):
@@ -762,7 +799,7 @@ class MatrixOperator:
             f_order = f.order_total - self.space.order - over
             if f_order < 0:
                 raise L.wrong_input_given(1, "quadrature order lower than the functional's own order")
-            expected = self.grid.num_elements * self.grid.quadrature_points(f.order_total).shape[1]
+            expected = int(self.grid.quadrature_offsets(f.order_total)[-1])
             if f.values.numel() != expected:
                 raise L.shapes_do_not_match(2, "quadrature-point values do not match the grid's rule")
             self._keep.append(f.values)

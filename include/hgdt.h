@@ -14,9 +14,10 @@
  *     Status codes map the reference's exception families
  *     (dune/gdt/exceptions.hh:24-77).
  *   - all floating-point data is double (FP64).
- *   - geometry: HGDT_QUAD / HGDT_TRI (2d), HGDT_HEX / HGDT_TET (3d).  Reference
- *     cube vertex b has coordinates given by the bits of b (x = bit 0, y = bit 1,
- *     z = bit 2); reference simplex vertex 0 is the origin and vertex k is e_k.
+ *   - geometry: HGDT_QUAD / HGDT_TRI (2d), HGDT_HEX / HGDT_TET (3d), HGDT_MIXED
+ *     (2d grids of triangles and quadrilaterals).  Reference cube vertex b has
+ *     coordinates given by the bits of b (x = bit 0, y = bit 1, z = bit 2);
+ *     reference simplex vertex 0 is the origin and vertex k is e_k.
  *   - a handle must not be used from two host threads at the same time.
  *   - results are deterministic: no floating-point atomics anywhere; every sum
  *     is taken in a fixed order.
@@ -44,7 +45,7 @@ typedef enum hgdt_status {
   HGDT_ERR_OUT_OF_MEMORY = 10
 } hgdt_status;
 
-enum { HGDT_QUAD = 0, HGDT_TRI = 1, HGDT_HEX = 2, HGDT_TET = 3 };
+enum { HGDT_QUAD = 0, HGDT_TRI = 1, HGDT_HEX = 2, HGDT_TET = 3, HGDT_MIXED = 4 };
 /* space kinds */
 enum { HGDT_CONTINUOUS_LAGRANGE = 0, HGDT_DISCONTINUOUS_LAGRANGE = 1 };
 /* Stencil, dune/gdt/type_traits.hh:55 */
@@ -99,6 +100,19 @@ hgdt_status hgdt_grid_create_structured(int geometry, const int32_t* cells, cons
  * Cube elements must be affine (parallelograms / parallelepipeds). */
 hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* coords, int64_t num_elements,
                              const int32_t* connectivity, hgdt_grid* grid);
+/* grid of triangles and quadrilaterals, element by element as GridFactory<G>::insertElement(
+ * GeometryTypes::simplex(2) or cube(2), vertices) builds the mixed grids of
+ * dune/gdt/test/spaces/base.hh:355-376 (make_mixed_grid, a UGGrid): coords [num_vertices x 2]; element e has
+ * type geometry[e] (HGDT_TRI or HGDT_QUAD) and the vertices vertices[offsets[e] .. offsets[e + 1]) in the
+ * reference vertex order above.  Quadrilaterals must be parallelograms.  The grid's geometry
+ * (hgdt_grid_info) is HGDT_MIXED, with 4 vertices and 4 faces per element (its connectivity has 4 entries
+ * per element, -1 after a triangle's third), unless every element has one type: then it is a grid of that
+ * type as hgdt_grid_create makes it.  Spaces of order 1 and 2 on mixed grids take element forms, right-hand
+ * sides, Dirichlet constraints and error norms; intersection integrals, the adjoint kernel, matrix-free
+ * application and unions need one element type. */
+hgdt_status hgdt_grid_create_mixed(int64_t num_vertices, const double* coords, int64_t num_elements,
+                                   const int32_t* geometry, const int64_t* offsets, const int32_t* vertices,
+                                   hgdt_grid* grid);
 /* disjoint union of grids of one geometry type: the batch of independent problem
  * instances.  Vertices, elements and DoFs are numbered instance by instance. */
 hgdt_status hgdt_grid_create_union(const hgdt_grid* grids, int32_t count, hgdt_grid* grid);
@@ -112,8 +126,14 @@ hgdt_status hgdt_grid_copy_connectivity(hgdt_grid grid, int32_t* host_out);
 hgdt_status hgdt_grid_instance_elements(hgdt_grid grid, int64_t* host_out);
 /* physical coordinates of the element quadrature points of the given order,
  * host_out[num_elements x num_points x dim]; *num_points receives the rule size.
- * Pass host_out = NULL to query the size only. */
+ * Pass host_out = NULL to query the size only.  On a mixed grid the elements' rules differ
+ * in size: *num_points is 0 and element e's points are host_out[offsets[e] .. offsets[e + 1])
+ * (x dim) with the offsets of hgdt_grid_quadrature_offsets. */
 hgdt_status hgdt_grid_quadrature_points(hgdt_grid grid, int order, double* host_out, int32_t* num_points);
+/* where every element's quadrature points of the given order start in the layout of
+ * hgdt_grid_quadrature_points (and of per-quadrature-point data: sources, error norms):
+ * host_out[num_elements + 1] */
+hgdt_status hgdt_grid_quadrature_offsets(hgdt_grid grid, int order, int64_t* host_out);
 
 /* physical coordinates of the quadrature points (face rule of the given order: Gauss-Legendre on segments
  * in 2d, the triangle or tensor-Gauss square rule on the faces of tetrahedra / hexahedra) on the boundary
@@ -143,7 +163,8 @@ hgdt_status hgdt_space_destroy(hgdt_space space);
 /* mapper().size() and max_local_size(), spaces/mapper/continuous.hh:139-150 */
 hgdt_status hgdt_space_size(hgdt_space space, int64_t* size, int32_t* local_size);
 /* mapper().global_indices(element) for all elements, host_out[num_elements x local_size]
- * (spaces/mapper/continuous.hh:139-150, spaces/mapper/discontinuous.hh:102-125) */
+ * (spaces/mapper/continuous.hh:139-150, spaces/mapper/discontinuous.hh:102-125); on a mixed grid
+ * local_size is the largest element's (max_local_size()) and a triangle's row ends in -1 entries */
 hgdt_status hgdt_space_copy_global_indices(hgdt_space space, int32_t* host_out);
 /* DoF offsets of the instances of a union grid, host_out[num_instances + 1] */
 hgdt_status hgdt_space_instance_dofs(hgdt_space space, int64_t* host_out);

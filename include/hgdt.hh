@@ -224,6 +224,27 @@ public:
     check(hgdt_grid_create_structured(int(geometry), num_elements.data(), lower_left.data(), upper_right.data(), &g));
     return Grid(g);
   }
+  // triangles and quadrilaterals, element by element as GridFactory<G>::insertElement(type, vertices) in
+  // make_mixed_grid (dune/gdt/test/spaces/base.hh:355-376): coords [num_vertices][2], elements[e] the vertices of
+  // element e in reference order (3: a triangle, 4: a quadrilateral)
+  static Grid mixed(const std::vector<double>& coords, const std::vector<std::vector<int32_t>>& elements)
+  {
+    if (coords.size() % 2)
+      throw Exceptions::shapes_do_not_match(HGDT_ERR_SHAPES_DO_NOT_MATCH, "mixed grids are two-dimensional");
+    std::vector<int32_t> geometry, vertices;
+    std::vector<int64_t> offsets{0};
+    for (const auto& el : elements) {
+      if (el.size() != 3 && el.size() != 4)
+        throw Exceptions::wrong_input_given(HGDT_ERR_INVALID_ARGUMENT, "an element needs 3 or 4 vertices");
+      geometry.push_back(el.size() == 4 ? HGDT_QUAD : HGDT_TRI);
+      vertices.insert(vertices.end(), el.begin(), el.end());
+      offsets.push_back(int64_t(vertices.size()));
+    }
+    hgdt_grid g = nullptr;
+    check(hgdt_grid_create_mixed(int64_t(coords.size() / 2), coords.data(), int64_t(elements.size()),
+                                 geometry.data(), offsets.data(), vertices.data(), &g));
+    return Grid(g);
+  }
   // disjoint union of independent grids: the batch of problem instances
   static Grid union_of(const std::vector<Grid>& grids)
   {
@@ -255,13 +276,20 @@ public:
   {
     return ninst_;
   }
-  // physical coordinates of the element quadrature points of `order`, [num_elements][num_points][dim]
+  // physical coordinates of the element quadrature points of `order`, [num_elements][num_points][dim] (mixed
+  // grids: num_points 0, element e's points from quadrature_offsets(order)[e])
   std::vector<double> quadrature_points(int order, int32_t& num_points) const
   {
-    check(hgdt_grid_quadrature_points(handle(), order, nullptr, &num_points));
-    std::vector<double> out(size_t(ne_) * size_t(num_points) * size_t(dim_));
+    const std::vector<int64_t> off = quadrature_offsets(order);
+    std::vector<double> out(size_t(off.back()) * size_t(dim_));
     check(hgdt_grid_quadrature_points(handle(), order, out.data(), &num_points));
     return out;
+  }
+  std::vector<int64_t> quadrature_offsets(int order) const
+  {
+    std::vector<int64_t> off(size_t(ne_) + 1);
+    check(hgdt_grid_quadrature_offsets(handle(), order, off.data()));
+    return off;
   }
   int64_t num_boundary_intersections() const
   {
@@ -368,7 +396,7 @@ public:
   {
     int32_t nq = 0;
     const std::vector<double> pts = grid.quadrature_points(quadrature_order, nq);
-    const size_t np = size_t(grid.num_elements()) * size_t(nq), d = size_t(grid.dimension());
+    const size_t d = size_t(grid.dimension()), np = pts.size() / d;
     std::vector<double> out(np);
     for (size_t p = 0; p < np; ++p)
       out[p] = evaluate_(&pts[p * d]);
@@ -380,7 +408,7 @@ public:
       throw Exceptions::not_supported(HGDT_ERR_NOT_SUPPORTED, "GenericFunction '" + name_ + "' has no jacobian");
     int32_t nq = 0;
     const std::vector<double> pts = grid.quadrature_points(quadrature_order, nq);
-    const size_t np = size_t(grid.num_elements()) * size_t(nq), d = size_t(grid.dimension());
+    const size_t d = size_t(grid.dimension()), np = pts.size() / d;
     std::vector<double> out(np * d);
     for (size_t p = 0; p < np; ++p)
       jacobian_(&pts[p * d], &out[p * d]);

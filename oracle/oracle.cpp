@@ -434,7 +434,17 @@ struct Geometry {
 // grid
 // ---------------------------------------------------------------------------
 struct Mesh {
+  // mixed meshes (triangles and quadrilaterals): ref holds the strides of conn, nb and nbface (those of the
+  // quadrilateral: 4 vertices, 4 faces; -1 vertices and -2 neighbours past an element's own), eref(e) the
+  // element's own reference element
   RefElement ref;
+  bool mixed = false;
+  std::vector<int8_t> etype; // mixed: 0 quadrilateral, 1 triangle
+  RefElement refs[2];
+  const RefElement& eref(int64_t e) const
+  {
+    return mixed ? refs[etype[e]] : ref;
+  }
   int64_t nv = 0, ne = 0;
   std::vector<double> coords; // nv * dim
   std::vector<int32_t> conn;  // ne * nverts
@@ -450,7 +460,7 @@ struct Mesh {
     for (int a = 0; a < d; ++a)
       g.x0[a] = coords[int64_t(c[0]) * d + a];
     for (int k = 0; k < d; ++k) {
-      const int lv = ref.simplex ? (k + 1) : (1 << k);
+      const int lv = eref(e).simplex ? (k + 1) : (1 << k);
       for (int a = 0; a < d; ++a)
         g.J[a][k] = coords[int64_t(c[lv]) * d + a] - g.x0[a];
     }
@@ -491,6 +501,8 @@ struct Mesh {
       return;
     const int d = ref.dim;
     for (int64_t e = 0; e < ne; ++e) {
+      if (eref(e).simplex)
+        continue;
       const int32_t* c = &conn[e * ref.nverts];
       for (int b = 0; b < ref.nverts; ++b)
         for (int a = 0; a < d; ++a) {
@@ -513,8 +525,12 @@ struct Mesh {
     std::map<std::vector<int32_t>, std::pair<int64_t, int>> open;
     for (int64_t e = 0; e < ne; ++e)
       for (int f = 0; f < nf; ++f) {
+        if (f >= eref(e).nfaces) {
+          nb[e * nf + f] = -2;
+          continue;
+        }
         std::vector<int32_t> key;
-        for (int lv : ref.faces[f])
+        for (int lv : eref(e).faces[f])
           key.push_back(conn[e * ref.nverts + lv]);
         std::sort(key.begin(), key.end());
         auto it = open.find(key);
@@ -585,13 +601,22 @@ struct Space {
   const Mesh* mesh = nullptr;
   bool continuous = true;
   int order = 1;
-  LocalFE fe;
+  LocalFE fe;                   // mixed meshes: the quadrilateral's (the largest: fe.size is the stride)
+  LocalFE fes[2];               // mixed meshes: the local finite element of every element type
   int64_t size = 0;
-  std::vector<int32_t> indices; // ne * fe.size
+  std::vector<int32_t> indices; // ne * fe.size (mixed meshes: -1 past an element's own local size)
+  const LocalFE& fe_of(int64_t e) const
+  {
+    return mesh->mixed ? fes[mesh->etype[e]] : fe;
+  }
 
   void init(const Mesh* m, bool cont, int k)
   {
     mesh = m, continuous = cont, order = k;
+    if (m->mixed) {
+      init_mixed(m, cont, k);
+      return;
+    }
     fe.init(&m->ref, k);
     const int ls = fe.size;
     indices.resize(m->ne * ls);
@@ -627,6 +652,42 @@ struct Space {
         indices[e * ls + i] = it->second;
       }
     size = int64_t(ids.size());
+  }
+
+  // the mappers over a mixed mesh (spaces/mapper/continuous.hh:117-182, discontinuous.hh:102-125): every
+  // element with its own type's local finite element; continuous order 1: the vertices, order 2: one DoF per
+  // subentity in order of first appearance in the walk; discontinuous: offset[element] + local index
+  void init_mixed(const Mesh* m, bool cont, int k)
+  {
+    fe.init(&m->refs[0], k);
+    fes[0].init(&m->refs[0], k), fes[1].init(&m->refs[1], k);
+    const int ls = fe.size;
+    indices.assign(m->ne * ls, -1);
+    if (!cont) {
+      int64_t off = 0;
+      for (int64_t e = 0; e < m->ne; ++e)
+        for (int i = 0; i < fe_of(e).size; ++i)
+          indices[e * ls + i] = int32_t(off++);
+      size = off;
+      return;
+    }
+    std::map<std::vector<int32_t>, int32_t> ids;
+    for (int64_t e = 0; e < m->ne; ++e)
+      for (int i = 0; i < fe_of(e).size; ++i) {
+        std::vector<int32_t> key;
+        for (int lv : fe_of(e).node_subentity[i])
+          key.push_back(m->conn[e * m->ref.nverts + lv]);
+        if (k == 1) {
+          indices[e * ls + i] = key[0];
+          continue;
+        }
+        std::sort(key.begin(), key.end());
+        auto it = ids.find(key);
+        if (it == ids.end())
+          it = ids.emplace(key, int32_t(ids.size())).first;
+        indices[e * ls + i] = it->second;
+      }
+    size = k == 1 ? m->nv : int64_t(ids.size());
   }
 };
 
@@ -746,7 +807,7 @@ struct Operator {
     int64_t c = 0;
     for (int64_t e = 0; e < m.ne; ++e)
       for (int f = 0; f < nf; ++f)
-        if (m.nb[e * nf + f] < 0)
+        if (m.nb[e * nf + f] == -1)
           bface_index[e * nf + f] = c++;
   }
   bool dirichlet = false;
@@ -771,9 +832,10 @@ struct Operator {
     for (int64_t e = 0; e < m.ne; ++e) {
       const int32_t* gi = &space->indices[e * ls];
       const int32_t* si = &source->indices[e * ls];
+      const int le = space->fe_of(e).size;
       if (stencil != OR_STENCIL_INTERSECTION)
-        for (int i = 0; i < ls; ++i)
-          for (int j = 0; j < ls; ++j)
+        for (int i = 0; i < le; ++i)
+          for (int j = 0; j < le; ++j)
             pattern[gi[i]].insert(si[j]);
       if (stencil != OR_STENCIL_ELEMENT)
         for (int f = 0; f < nf; ++f) {
@@ -781,8 +843,8 @@ struct Operator {
           if (o < 0)
             continue;
           const int32_t* gj = &source->indices[int64_t(o) * ls];
-          for (int i = 0; i < ls; ++i)
-            for (int j = 0; j < ls; ++j)
+          for (int i = 0; i < le; ++i)
+            for (int j = 0; j < source->fe_of(o).size; ++j)
               pattern[gi[i]].insert(gj[j]);
         }
     }
@@ -820,13 +882,13 @@ struct Operator {
   void element_local(const ElementForm& F, int64_t e, const Geometry& geo, BoundBasis& basis,
                      std::vector<double>& result) const
   {
-    const int ls = space->fe.size, d = geo.dim;
+    const int ls = space->fe_of(e).size, d = geo.dim;
     result.assign(ls * ls, 0.);
     const int basis_order = space->fe.order;
     // LocalLaplaceIntegrand::order (local/integrands/laplace.hh:74-79) and
     // LocalElementProductIntegrand::order (local/integrands/product.hh:88-100): 0 + test + ansatz
     const int order = 0 + basis_order + basis_order + F.over_integrate;
-    const Quadrature q = element_rule(space->mesh->ref, order);
+    const Quadrature q = element_rule(space->mesh->eref(e), order);
     const double c = F.coeff.at(e);
     std::vector<double> iv(ls * ls);
     for (size_t qp = 0; qp < q.w.size(); ++qp) {
@@ -1128,10 +1190,10 @@ struct Operator {
   void source_local(const Source& S, int64_t e, const Geometry& geo, BoundBasis& basis,
                     std::vector<double>& result) const
   {
-    const int ls = space->fe.size;
+    const int ls = space->fe_of(e).size;
     result.assign(ls, 0.);
     const int order = 0 + space->fe.order + S.f_order + S.over_integrate;
-    const Quadrature q = element_rule(space->mesh->ref, order);
+    const Quadrature q = element_rule(space->mesh->eref(e), order);
     for (size_t qp = 0; qp < q.w.size(); ++qp) {
       const double* xh = q.x[qp].data();
       basis.evaluate(xh);
@@ -1165,18 +1227,19 @@ struct Operator {
       for (int64_t e = 0; e < m.ne; ++e) {
         try {
           const Geometry geo = m.geometry(e);
-          basis.bind(&space->fe, &geo);
+          basis.bind(&space->fe_of(e), &geo);
           const int32_t* gi = &space->indices[e * ls];  // rows: test functions of the range space
           const int32_t* si = &source->indices[e * ls]; // columns: ansatz functions of the source space
+          const int le = space->fe_of(e).size;
           for (const auto& F : eforms) {
             element_local(F, e, geo, basis, local);
-            for (int i = 0; i < ls; ++i)
-              for (int j = 0; j < ls; ++j)
-                add_to_entry(gi[i], si[j], F.scaling * local[i * ls + j], atomic);
+            for (int i = 0; i < le; ++i)
+              for (int j = 0; j < le; ++j)
+                add_to_entry(gi[i], si[j], F.scaling * local[i * le + j], atomic);
           }
           for (const auto& S : sources) {
             source_local(S, e, geo, basis, lvec);
-            for (int i = 0; i < ls; ++i) {
+            for (int i = 0; i < le; ++i) {
               if (atomic) {
 #pragma omp atomic
                 rhs[gi[i]] += lvec[i];
@@ -1266,12 +1329,12 @@ struct Operator {
     if (space->continuous) {
       for (int64_t e = 0; e < m.ne; ++e)
         for (int f = 0; f < nf; ++f) {
-          if (m.nb[e * nf + f] >= 0 || !on_boundary_subset(dirichlet_faces, e, f))
+          if (m.nb[e * nf + f] != -1 || !on_boundary_subset(dirichlet_faces, e, f))
             continue;
-          const auto& fv = m.ref.faces[f];
-          for (int i = 0; i < ls; ++i) {
+          const auto& fv = m.eref(e).faces[f];
+          for (int i = 0; i < space->fe_of(e).size; ++i) {
             bool on_face = true;
-            for (int lv : space->fe.node_subentity[i])
+            for (int lv : space->fe_of(e).node_subentity[i])
               if (std::find(fv.begin(), fv.end(), lv) == fv.end())
                 on_face = false;
             if (on_face)
@@ -1350,6 +1413,60 @@ OrMesh* or_mesh_create(int geom, int64_t nv, const double* coords, int64_t ne, c
     out = m;
   });
   return out;
+}
+
+// a mesh of triangles and quadrilaterals, element by element (GridFactory::insertElement with the element's
+// geometry type, as make_mixed_grid in test/spaces/base.hh:355-376): element e of type geometry[e] has the
+// vertices vertices[offsets[e] .. offsets[e + 1]); one type only gives the plain mesh of that type
+OrMesh* or_mesh_mixed(int64_t nv, const double* coords, int64_t ne, const int32_t* geometry, const int64_t* offsets,
+                      const int32_t* vertices)
+{
+  OrMesh* out = nullptr;
+  guarded([&] {
+    auto* m = new OrMesh;
+    try {
+      bool present[2] = {false, false};
+      for (int64_t e = 0; e < ne; ++e) {
+        if ((geometry[e] != OR_QUAD && geometry[e] != OR_TRI) ||
+            offsets[e + 1] - offsets[e] != (geometry[e] == OR_QUAD ? 4 : 3))
+          throw Error("mixed meshes hold triangles (3 vertices) and quadrilaterals (4 vertices)");
+        present[geometry[e]] = true;
+      }
+      m->nv = nv, m->ne = ne;
+      m->coords.assign(coords, coords + nv * 2);
+      if (!(present[0] && present[1])) {
+        m->ref = make_ref(present[0] ? OR_QUAD : OR_TRI);
+        m->conn.assign(vertices, vertices + offsets[ne]);
+      } else {
+        m->mixed = true;
+        m->ref = make_ref(OR_QUAD);
+        m->ref.geom = OR_MIXED;
+        m->refs[0] = make_ref(OR_QUAD), m->refs[1] = make_ref(OR_TRI);
+        m->etype.resize(ne);
+        m->conn.assign(ne * 4, -1);
+        for (int64_t e = 0; e < ne; ++e) {
+          m->etype[e] = int8_t(geometry[e] == OR_QUAD ? 0 : 1);
+          for (int64_t t = offsets[e]; t < offsets[e + 1]; ++t)
+            m->conn[e * 4 + (t - offsets[e])] = vertices[t];
+        }
+      }
+      for (int64_t t = 0; t < offsets[ne]; ++t)
+        if (vertices[t] < 0 || vertices[t] >= nv)
+          throw Error("connectivity references a vertex out of range");
+      m->check_affine();
+      m->build_neighbors();
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    out = m;
+  });
+  return out;
+}
+
+int or_mesh_is_mixed(const OrMesh* m)
+{
+  return m->mixed ? 1 : 0;
 }
 
 void or_mesh_destroy(OrMesh* m)
@@ -1487,6 +1604,8 @@ static int append_face(OrOperator* op, int kind, double param, const double* kap
                        double oc, int over_integrate, double scaling)
 {
   return guarded([&] {
+    if (op->space->mesh->mixed)
+      throw Error("intersection integrals need a mesh of one element type");
     FaceForm F{kind, param, {}, {}, over_integrate, scaling, {}};
     F.kappa.set(kappa, kc, op->space->mesh->ne);
     F.omega.set(omega, oc, op->space->mesh->ne);
@@ -1716,6 +1835,8 @@ int or_quadrature(int geom, int order, double* points, double* weights, int capa
 
 double or_esv2007_h1_semi_error(const OrSpace* s, const double* u, int quad_order)
 {
+  if (s->mesh->mixed)
+    return std::nan("");
   const Mesh& m = *s->mesh;
   const int ls = s->fe.size, d = m.ref.dim;
   const Quadrature q = element_rule(m.ref, quad_order);
@@ -1753,6 +1874,8 @@ int or_operator_append_boundary_source(OrOperator* op, int kind, double param, c
   return guarded([&] {
     if (kind < BS_DIRICHLET_COUPLING || kind > BS_PRODUCT)
       throw Error("unknown boundary functional kind");
+    if (op->space->mesh->mixed)
+      throw Error("boundary functionals need a mesh of one element type");
     BoundarySource S;
     S.kind = kind, S.param = param, S.g_const = g_const, S.g_order = g_order, S.over_integrate = over_integrate;
     const Mesh& m = *op->space->mesh;
@@ -1849,15 +1972,16 @@ int or_mesh_boundary_quadrature_points(const OrMesh* mesh, int order, double* ou
     int64_t bi = 0;
     for (int64_t e = 0; e < m.ne; ++e) {
       const Geometry geo = m.geometry(e);
+      const RefElement& r = m.eref(e);
       for (int f = 0; f < nf; ++f) {
-        if (m.nb[e * nf + f] >= 0)
+        if (m.nb[e * nf + f] != -1)
           continue;
-        const int a = m.ref.faces[f][0], b = m.ref.faces[f][1];
+        const int a = r.faces[f][0], b = r.faces[f][1];
         for (int qp = 0; qp < count; ++qp) {
           const double t = q.x[qp][0];
           double xh[3] = {0, 0, 0}, X[3];
           for (int c = 0; c < 2; ++c)
-            xh[c] = (1. - t) * m.ref.verts[a][c] + t * m.ref.verts[b][c];
+            xh[c] = (1. - t) * r.verts[a][c] + t * r.verts[b][c];
           geo.global(xh, X);
           out[(bi * count + qp) * 2 + 0] = X[0];
           out[(bi * count + qp) * 2 + 1] = X[1];
@@ -1875,63 +1999,81 @@ int or_error_norms(const OrSpace* s, const double* u, int order, const double* g
   return guarded([&] {
     const Mesh& m = *s->mesh;
     const int ls = s->fe.size, d = m.ref.dim;
-    const Quadrature q = element_rule(m.ref, order);
-    const int nq = int(q.w.size());
     double l2 = 0., h1 = 0.;
     BoundBasis basis;
+    int64_t q0 = 0; // the element's first point in gval / ggrad (every element with its own type's rule)
     for (int64_t e = 0; e < m.ne; ++e) {
+      const Quadrature q = element_rule(m.eref(e), order);
+      const int nq = int(q.w.size());
       const Geometry geo = m.geometry(e);
-      basis.bind(&s->fe, &geo);
+      basis.bind(&s->fe_of(e), &geo);
       const double k = kappa ? kappa[e] : kappa_const;
       double el2 = 0., eh1 = 0.;
       for (int qp = 0; qp < nq; ++qp) {
         basis.evaluate(q.x[qp].data());
         basis.jacobians(q.x[qp].data());
         double ev = 0., eg[3] = {0., 0., 0.};
-        for (int i = 0; i < ls; ++i) {
+        for (int i = 0; i < s->fe_of(e).size; ++i) {
           const double ui = u[s->indices[e * ls + i]];
           ev += ui * basis.vals[i];
           for (int a = 0; a < d; ++a)
             eg[a] += ui * basis.grads[3 * i + a];
         }
         if (gval)
-          ev -= gval[e * nq + qp];
+          ev -= gval[q0 + qp];
         if (ggrad)
           for (int a = 0; a < d; ++a)
-            eg[a] -= ggrad[(e * nq + qp) * d + a];
+            eg[a] -= ggrad[(q0 + qp) * d + a];
         const double f = geo.integration_element * q.w[qp];
         el2 += (ev * ev) * f;
         eh1 += kgrad_dot(k, eg, eg, d) * f;
       }
       l2 += el2, h1 += eh1;
+      q0 += nq;
     }
     out[0] = l2, out[1] = h1;
   });
 }
 
-// physical coordinates of the points of element_rule(order) on every element, out[ne][npts][dim]
+// physical coordinates of the points of element_rule(order) on every element, out[ne][npts][dim]; on a mixed
+// mesh (returns 0) every element's points follow the previous element's, with its own type's rule
 int or_mesh_quadrature_points(const OrMesh* mesh, int order, double* out)
 {
   int count = -1;
   guarded([&] {
     const Mesh& m = *mesh;
-    const Quadrature q = element_rule(m.ref, order);
-    count = int(q.w.size());
+    count = m.mixed ? 0 : int(element_rule(m.ref, order).w.size());
     if (!out)
       return;
+    int64_t q0 = 0;
     for (int64_t e = 0; e < m.ne; ++e) {
+      const Quadrature q = element_rule(m.eref(e), order);
       const Geometry geo = m.geometry(e);
-      for (int qp = 0; qp < count; ++qp)
-        geo.global(q.x[qp].data(), &out[(e * count + qp) * m.ref.dim]);
+      for (size_t qp = 0; qp < q.w.size(); ++qp)
+        geo.global(q.x[qp].data(), &out[(q0 + int64_t(qp)) * m.ref.dim]);
+      q0 += int64_t(q.w.size());
     }
   });
   return count;
+}
+
+// where every element's points start in that layout: out[ne + 1]
+int or_mesh_quadrature_offsets(const OrMesh* mesh, int order, int64_t* out)
+{
+  return guarded([&] {
+    const Mesh& m = *mesh;
+    out[0] = 0;
+    for (int64_t e = 0; e < m.ne; ++e)
+      out[e + 1] = out[e] + int64_t(element_rule(m.eref(e), order).w.size());
+  });
 }
 
 int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const double* u, const double* lambda,
                                   double* grad)
 {
   return guarded([&] {
+    if (s->mesh->mixed)
+      throw Error("the adjoint gradient needs a mesh of one element type");
     Operator op;
     op.space = const_cast<OrSpace*>(s);
     const Mesh& m = *s->mesh;

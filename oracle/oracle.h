@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 /* geometry types (match hgdt.h) */
-enum { OR_QUAD = 0, OR_TRI = 1, OR_HEX = 2, OR_TET = 3 };
+enum { OR_QUAD = 0, OR_TRI = 1, OR_HEX = 2, OR_TET = 3, OR_MIXED = 4 };
 /* sparsity stencils, dune/gdt/type_traits.hh:55 */
 enum { OR_STENCIL_ELEMENT = 0, OR_STENCIL_INTERSECTION = 1, OR_STENCIL_ELEMENT_AND_INTERSECTION = 2 };
 
@@ -32,6 +32,10 @@ const char* or_last_error(void);
 /* ---- grids ---- */
 OrMesh* or_mesh_structured(int geom, const int32_t* cells, const double* lo, const double* hi);
 OrMesh* or_mesh_create(int geom, int64_t nv, const double* coords, int64_t ne, const int32_t* conn);
+/* triangles and quadrilaterals: element e of type geometry[e] has vertices[offsets[e] .. offsets[e + 1]) */
+OrMesh* or_mesh_mixed(int64_t nv, const double* coords, int64_t ne, const int32_t* geometry, const int64_t* offsets,
+                      const int32_t* vertices);
+int or_mesh_is_mixed(const OrMesh* m);
 void or_mesh_destroy(OrMesh* m);
 int64_t or_mesh_num_vertices(const OrMesh* m);
 int64_t or_mesh_num_elements(const OrMesh* m);
@@ -118,6 +122,7 @@ int or_error_norms(const OrSpace* s, const double* u, int order, const double* g
                    const double* kappa, double kappa_const, double* out);
 /* physical coordinates of the element-rule points, out[ne][npts][dim] (NULL: size query); returns npts */
 int or_mesh_quadrature_points(const OrMesh* m, int order, double* out);
+int or_mesh_quadrature_offsets(const OrMesh* m, int order, int64_t* out /* ne + 1 */);
 
 /* right-hand side on the boundary intersections: VectorFunctional::append(LocalIntersectionIntegralFunctional(...))
  * kind 0: LocalLaplaceIPDGIntegrands::DirichletCoupling(param, coeff, g) as a unary integrand

@@ -38,6 +38,9 @@ def lib():
         "or_last_error": (ctypes.c_char_p, []),
         "or_mesh_structured": (_vp, [ctypes.c_int, _ip, _dp, _dp]),
         "or_mesh_create": (_vp, [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_int64, _ip]),
+        "or_mesh_mixed": (_vp, [ctypes.c_int64, _dp, ctypes.c_int64, _ip, _lp, _ip]),
+        "or_mesh_is_mixed": (ctypes.c_int, [_vp]),
+        "or_mesh_quadrature_offsets": (ctypes.c_int, [_vp, ctypes.c_int, _lp]),
         "or_mesh_destroy": (None, [_vp]),
         "or_mesh_num_vertices": (ctypes.c_int64, [_vp]),
         "or_mesh_num_elements": (ctypes.c_int64, [_vp]),
@@ -159,13 +162,38 @@ class Mesh:
         conn = np.ascontiguousarray(conn, np.int32)
         return cls(lib().or_mesh_create(geom, coords.shape[0], dptr(coords), conn.shape[0], iptr(conn)))
 
+    @classmethod
+    def mixed(cls, coords, elements):
+        """triangles (3 vertices) and quadrilaterals (4) in one mesh, element by element"""
+        coords = np.ascontiguousarray(coords, np.float64)
+        sizes = np.array([len(v) for v in elements], np.int64)
+        geometry = np.where(sizes == 4, QUAD, TRI).astype(np.int32)
+        offsets = np.zeros(len(elements) + 1, np.int64)
+        offsets[1:] = np.cumsum(sizes)
+        verts = np.ascontiguousarray(np.concatenate([np.asarray(v, np.int32) for v in elements]), np.int32)
+        return cls(lib().or_mesh_mixed(coords.shape[0], dptr(coords), len(elements), iptr(geometry), lptr(offsets),
+                                       iptr(verts)))
+
+    @property
+    def is_mixed(self):
+        return bool(lib().or_mesh_is_mixed(self.h))
+
     def quadrature_points(self, order):
-        """physical coordinates of the element-rule points, [ne, npts, dim]"""
+        """physical coordinates of the element-rule points, [ne, npts, dim] ([P, dim] on a mixed mesh)"""
         n = lib().or_mesh_quadrature_points(self.h, order, None)
         if n < 0:
             raise RuntimeError(_err())
-        out = np.empty((self.ne, n, self.dim))
+        if self.is_mixed:
+            out = np.empty((int(self.quadrature_offsets(order)[-1]), self.dim))
+        else:
+            out = np.empty((self.ne, n, self.dim))
         lib().or_mesh_quadrature_points(self.h, order, dptr(out))
+        return out
+
+    def quadrature_offsets(self, order):
+        out = np.empty(self.ne + 1, np.int64)
+        if lib().or_mesh_quadrature_offsets(self.h, order, lptr(out)) != 0:
+            raise RuntimeError(_err())
         return out
 
     def boundary_quadrature_points(self, order):
@@ -173,7 +201,7 @@ class Mesh:
         n = lib().or_mesh_boundary_quadrature_points(self.h, order, None)
         if n < 0:
             raise RuntimeError(_err())
-        nb = int(np.sum(self.neighbors() < 0))
+        nb = int(np.sum(self.neighbors() == -1))
         out = np.empty((nb, n, self.dim))
         lib().or_mesh_boundary_quadrature_points(self.h, order, dptr(out))
         return out

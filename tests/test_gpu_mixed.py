@@ -1,0 +1,171 @@
+"""Mixed triangle/quadrilateral grids on the HIP path (one kernel launch per element type).
+
+Parity with the oracle (same element list, same DoF numbering) on the reference's mixed grid
+(dune/gdt/test/spaces/base.hh:355-376, globally refined once) and on a strip grid with sheared cells:
+global indices, sparsity pattern, matrix and right-hand side (Laplace with a per-element coefficient,
+product, constant / per-element / quadrature-point sources), Dirichlet DoFs, the Jacobi-PCG solution and
+the error norms, for continuous and discontinuous Lagrange spaces of orders 1 and 2.  The device's mass
+matrix also sums to the grid's area (closed form), and what needs one element type is refused.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+from mixed_grids import grid_area, refined_reference_mixed_grid, strip_mixed_grid
+from problems import matrix_error
+
+pytestmark = pytest.mark.gpu
+
+ESV_FORCE = (lambda X: 0.5 * np.pi * np.pi * np.cos(0.5 * np.pi * X[0]) * np.cos(0.5 * np.pi * X[1]))
+
+
+def _grid(name):
+    if name == "reference":
+        return refined_reference_mixed_grid()
+    if name == "strip":
+        return strip_mixed_grid(24, 18, shear=0.35)
+    return strip_mixed_grid(200, 160)  # 48 000 elements
+
+
+def _pair(H, name, continuous, order, source):
+    V, E = _grid(name)
+    rng = np.random.default_rng(11)
+    kappa = np.exp(rng.normal(size=len(E)))
+    m = O.Mesh.mixed(V, E)
+    s = O.Space(m, continuous, order)
+    oop = O.Operator(s, O.STENCIL_ELEMENT)
+    oop.append_laplace(kappa)
+    oop.append_product(0.5)
+    if source == "element":
+        f = rng.normal(size=len(E))
+        oop.append_source(f)
+    elif source == "esv":
+        oop.append_source(esv2007=True, f_order=3)
+    else:
+        oop.append_source(1.0)
+    if continuous:
+        oop.set_dirichlet(True)
+    oop.assemble(1)
+    grid = H.Grid.mixed(V, E)
+    space = (H.ContinuousLagrangeSpace if continuous else H.DiscontinuousLagrangeSpace)(grid, order)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kappa).cuda())))
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(0.5)))
+    rhs = H.make_vector_functional(space)
+    if source == "element":
+        fn = torch.from_numpy(f).cuda()
+    elif source == "esv":
+        fn = H.GenericFunction(3, ESV_FORCE)
+    else:
+        fn = 1.0
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(fn)))
+    op.append(rhs)
+    if continuous:
+        op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.assemble()
+    return V, E, m, s, oop, grid, space, op
+
+
+@pytest.mark.parametrize("name", ["reference", "strip"])
+@pytest.mark.parametrize("continuous,order", [(True, 1), (True, 2), (False, 1), (False, 2)])
+@pytest.mark.parametrize("source", ["constant", "element", "esv"])
+def test_mixed_assembly_parity(hgdt_mod, name, continuous, order, source):
+    H = hgdt_mod
+    print(f"case mixed assembly {name} continuous={continuous} order={order} source={source}", flush=True)
+    V, E, m, s, oop, grid, space, op = _pair(H, name, continuous, order, source)
+    assert grid.is_mixed and grid.num_elements == len(E)
+    assert space.num_DoFs == s.size
+    assert np.array_equal(space.global_indices(), s.global_indices())
+    rp, cols = op.pattern()
+    orp, ocols = oop.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), orp)
+    b, ob = op.rhs.cpu().numpy(), oop.rhs()
+    berr = np.abs(b - ob).max() / np.abs(ob).max()
+    print(f"  {len(E)} elements, {s.size} DoFs: matrix max rel err {err:.2e}, rhs {berr:.2e}")
+    assert err <= 1e-12 and berr <= 1e-12
+    if continuous:
+        assert np.array_equal(op.dirichlet_dofs(), oop.dirichlet_dofs())
+
+
+@pytest.mark.parametrize("name", ["strip", "large"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_mixed_solve_parity(hgdt_mod, name, order):
+    H = hgdt_mod
+    print(f"case mixed solve {name} order={order}", flush=True)
+    V, E, m, s, oop, grid, space, op = _pair(H, name, True, order, "esv")
+    orp, ocols = oop.pattern()
+    xo, ito, relo, rc = O.pcg_jacobi(orp, ocols, oop.values(), oop.rhs(), rtol=1e-10)
+    x = op.apply_inverse("rhs", opts={"precision": 1e-10}).cpu().numpy()
+    rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    st = op.last_solve
+    print(f"  {s.size} DoFs: iterations {st['iterations']} (oracle {ito}), rel L2 {rel:.2e}, true relres "
+          f"{st['true_relres']:.2e}")
+    assert rel <= 1e-9 and abs(st["iterations"] - ito) <= 2 and st["true_relres"] <= 2e-10
+    # error norms against an analytic function at the grid's element-ordered quadrature points
+    g = H.GenericFunction(3, lambda X: np.sin(X[0]) * np.cos(X[1]),
+                          jacobian=lambda X: np.stack([np.cos(X[0]) * np.cos(X[1]), -np.sin(X[0]) * np.sin(X[1])]))
+    uh = H.make_discrete_function(space, torch.from_numpy(x))
+    f1 = H.make_bilinear_form(grid, uh - g)
+    f1 += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0))
+    f2 = H.make_bilinear_form(grid, uh - g)
+    f2 += H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand())
+    h1, l2 = f1.apply2(), f2.apply2()
+    P = m.quadrature_points(6)
+    assert np.abs(P - grid.quadrature_points(6)).max() <= 1e-14
+    ol2, oh1 = O.error_norms(s, x, 6, g.evaluate(P.T), np.ascontiguousarray(g.jacobian(P.T).T))
+    print(f"  error norms: H1 {h1:.6e} (oracle {oh1:.6e}), L2 {l2:.6e} (oracle {ol2:.6e})")
+    assert abs(h1 - oh1) <= 1e-12 * oh1 and abs(l2 - ol2) <= 1e-12 * ol2
+
+
+@pytest.mark.parametrize("continuous,order", [(True, 1), (True, 2), (False, 2)])
+def test_mixed_mass_sums_to_area_on_device(hgdt_mod, continuous, order):
+    H = hgdt_mod
+    print(f"case mixed mass closed form continuous={continuous} order={order}", flush=True)
+    V, E = refined_reference_mixed_grid()
+    grid = H.Grid.mixed(V, E)
+    space = (H.ContinuousLagrangeSpace if continuous else H.DiscontinuousLagrangeSpace)(grid, order)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(1.0)))
+    op.assemble()
+    total = float(op.matrix.values.sum())
+    print(f"  mass total {total!r}, area {grid_area(V, E)!r}")
+    assert abs(total - grid_area(V, E)) <= 1e-14
+
+
+def test_mixed_single_type_is_plain_grid(hgdt_mod):
+    """a 'mixed' element list of one type gives that type's grid: the same matrix as the explicit grid"""
+    H = hgdt_mod
+    print("case mixed grid of one element type", flush=True)
+    V, E = strip_mixed_grid(6, 4)
+    tris = [el for el in E if len(el) == 3]
+    g1 = H.Grid.mixed(V, tris)
+    g2 = H.Grid.from_arrays("tri", V, np.array(tris))
+    assert not g1.is_mixed and g1.geometry == g2.geometry
+    mats = []
+    for g in (g1, g2):
+        sp_ = H.ContinuousLagrangeSpace(g, 1)
+        op = H.make_matrix_operator(sp_, H.Stencil.element)
+        op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
+        op.assemble()
+        mats.append(op.matrix.values.cpu().numpy())
+    assert np.array_equal(mats[0], mats[1])
+
+
+def test_mixed_refusals(hgdt_mod):
+    H = hgdt_mod
+    print("case mixed refusals", flush=True)
+    V, E = refined_reference_mixed_grid()
+    grid = H.Grid.mixed(V, E)
+    space = H.DiscontinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(16.0, 1.0)), {},
+              H.ApplyOn.InnerIntersectionsOnce())
+    with pytest.raises(H.not_supported):
+        op.assemble()
+    with pytest.raises(H.not_supported):
+        H.Grid.union([grid, grid])
+    with pytest.raises(H.wrong_input_given):
+        H.Grid.mixed(V, [[0, 1]])
