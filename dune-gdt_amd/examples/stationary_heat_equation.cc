@@ -3,17 +3,41 @@
 // is written against Dune::GDT: Laplace operator, product functional with an analytic source, Dirichlet
 // constraints, solve, H1-semi and L2 errors against the exact solution u = cos(pi x / 2) cos(pi y / 2).
 //
-// usage: stationary_heat_equation [cells per direction = 128] [solution output file (raw float64)]
+// usage: stationary_heat_equation [cells per direction = 128] [solution output file (raw float64)] [grid]
+// grid: "cube" (default: quadrilaterals) or "mixed" (the cells of the left half quadrilaterals, those of
+// the right half split into two triangles, built element by element as GridFactory::insertElement does).
 // Prints the errors and one JSON line with the results.
 
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
+#include <string>
+#include <vector>
 
 #include "hgdt.hh"
 
 using namespace hgdt;
+
+// [-1, 1]^2 in n x n cells: columns left of n / 2 quadrilaterals, the others two triangles each
+// (v0, v0 + e_x, v0 + e_x + e_y) and (v0, v0 + e_y, v0 + e_x + e_y); vertices lexicographic, x fastest
+static Grid make_mixed_strip_grid(int n)
+{
+  std::vector<double> coords;
+  for (int j = 0; j <= n; ++j)
+    for (int i = 0; i <= n; ++i)
+      coords.push_back(-1. + 2. * i / n), coords.push_back(-1. + 2. * j / n);
+  std::vector<std::vector<int32_t>> elements;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const int32_t v0 = j * (n + 1) + i, v1 = v0 + 1, v2 = v0 + n + 1, v3 = v2 + 1;
+      if (i < n / 2)
+        elements.push_back({v0, v1, v2, v3});
+      else
+        elements.push_back({v0, v1, v3}), elements.push_back({v0, v2, v3});
+    }
+  return Grid::mixed(coords, elements);
+}
 
 int main(int argc, char* argv[])
 {
@@ -34,7 +58,9 @@ int main(int argc, char* argv[])
           g[1] = pre * std::cos(M_PI_2 * x[0]) * std::sin(M_PI_2 * x[1]);
         });
 
-    auto grid = make_cube_grid(Geometry::quad, /*lower_left=*/-1., /*upper_right=*/1., /*num_elements=*/n);
+    const bool mixed = argc > 3 && std::string(argv[3]) == "mixed";
+    auto grid = mixed ? make_mixed_strip_grid(n)
+                      : make_cube_grid(Geometry::quad, /*lower_left=*/-1., /*upper_right=*/1., /*num_elements=*/n);
     const auto& grid_view = grid.leaf_view();
     AllDirichletBoundaryInfo boundary_info;
 

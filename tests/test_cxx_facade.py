@@ -67,6 +67,44 @@ def test_example_matches_oracle():
     assert 0 < res["l2_error"] < res["h1_semi_error"]
 
 
+@pytest.mark.gpu
+def test_example_on_a_mixed_grid_matches_oracle():
+    """the same example on a grid of quadrilaterals (left half) and triangles (right half) built with
+    Grid::mixed: solution parity with the oracle on the same element list (1e-10) and the H1-semi and L2
+    errors equal the oracle's errors of that solution at its quadrature points (1e-12)"""
+    from mixed_grids import strip_mixed_grid
+
+    n = 32
+    print(f"case C++ example stationary_heat_equation n={n} on a mixed grid", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "u.bin")
+        r = subprocess.run([EXAMPLE, str(n), out, "mixed"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        u = np.fromfile(out, dtype=np.float64)
+    V, E = strip_mixed_grid(n, n)
+    m = O.Mesh.mixed(2.0 * V - 1.0, E)
+    s = O.Space(m, True, 1)
+    op = O.Operator(s, O.STENCIL_ELEMENT)
+    op.append_laplace(1.0)
+    op.append_source(esv2007=True, f_order=3)
+    op.set_dirichlet(True)
+    op.assemble(4)
+    rp, cols = op.pattern()
+    xo, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-10, nthreads=4)
+    assert rc == 0 and u.shape == xo.shape
+    diff = np.linalg.norm(u - xo) / np.linalg.norm(xo)
+    P = m.quadrature_points(6)
+    c = np.pi / 2
+    g = np.cos(c * P[:, 0]) * np.cos(c * P[:, 1])
+    gg = np.stack([-c * np.sin(c * P[:, 0]) * np.cos(c * P[:, 1]), -c * np.cos(c * P[:, 0]) * np.sin(c * P[:, 1])], 1)
+    l2o, h1o = np.sqrt(O.error_norms(s, u, 6, g, np.ascontiguousarray(gg)))
+    print(f"  iterations {res['iterations']} (oracle {it}), rel L2 {diff:.2e}, h1 error {res['h1_semi_error']:.6e} "
+          f"(oracle {h1o:.6e}), l2 error {res['l2_error']:.6e} (oracle {l2o:.6e})")
+    assert diff <= 1e-10 and abs(res["iterations"] - it) <= 2
+    assert abs(res["h1_semi_error"] - h1o) <= 1e-12 * h1o and abs(res["l2_error"] - l2o) <= 1e-12 * l2o
+
+
 SWIPDG = os.path.join(ROOT, "dune-gdt_amd", "bin", "elliptic_swipdg")
 
 
