@@ -86,6 +86,13 @@ struct FaceParams {
   // exactly symmetric (in_in, out_out and boundary blocks from their upper triangle, out_in as the
   // transpose of in_out), so the assembled matrix is symmetric bit for bit
   int symmetric;
+  // mixed grids: the part of every element (a launch takes the faces whose inside / outside elements are of
+  // parts part_in / part_out); conn holds cstride entries per element; the staged blocks have rows of ls
+  // entries (the largest local size).  One element type: etype NULL, cstride the type's vertices, ls its ND
+  const int8_t* etype;
+  int part_in, part_out;
+  int ls;
+  int64_t cstride;
 };
 
 // right-hand side terms on the boundary intersections
@@ -106,6 +113,10 @@ struct BoundaryRhsParams {
   const uint8_t* mask[MAX_BSOURCES]; // applied on the intersections with a nonzero byte (NULL: all)
   double* staging;
   int64_t base;
+  const int8_t* etype; // as FaceParams
+  int part_in;
+  int ls;
+  int64_t cstride;
 };
 
 template <int GEOM>
@@ -144,17 +155,18 @@ struct Geo {
   double det; // |det J|
 };
 
+// (stride: entries of conn per element, the type's vertex count, or 4 in a mixed grid's connectivity)
 template <int GEOM>
 __device__ __forceinline__ void load_geometry(const double* __restrict__ coords, const int32_t* __restrict__ conn,
-                                              int64_t e, Geo<Traits<GEOM>::D>& g)
+                                              int64_t e, Geo<Traits<GEOM>::D>& g, int64_t stride = Traits<GEOM>::NV)
 {
   using T = Traits<GEOM>;
   constexpr int D = T::D;
   int32_t v[D + 1];
-  v[0] = conn[e * T::NV];
+  v[0] = conn[e * stride];
 #pragma unroll
   for (int k = 0; k < D; ++k)
-    v[k + 1] = conn[e * T::NV + (T::SIMPLEX ? (k + 1) : (1 << k))];
+    v[k + 1] = conn[e * stride + (T::SIMPLEX ? (k + 1) : (1 << k))];
 #pragma unroll
   for (int a = 0; a < D; ++a)
     g.x0[a] = coords[int64_t(v[0]) * D + a];
@@ -609,14 +621,15 @@ struct FaceFrame {
 
 template <int GEOM>
 __device__ __forceinline__ void face_frame(const double* __restrict__ coords, const int32_t* __restrict__ conn,
-                                           int64_t e, int f, FaceFrame<Traits<GEOM>::D>& F)
+                                           int64_t e, int f, FaceFrame<Traits<GEOM>::D>& F,
+                                           int64_t stride = Traits<GEOM>::NV)
 {
   using T = Traits<GEOM>;
   constexpr int D = T::D, NC = (D == 2) ? 2 : (T::SIMPLEX ? 3 : 4);
   double P[NC][D], cen[D];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const int32_t v = conn[e * T::NV + face_vertex<GEOM>(f, k)];
+    const int32_t v = conn[e * stride + face_vertex<GEOM>(f, k)];
 #pragma unroll
     for (int a = 0; a < D; ++a)
       P[k][a] = coords[int64_t(v) * D + a];
@@ -625,7 +638,7 @@ __device__ __forceinline__ void face_frame(const double* __restrict__ coords, co
   for (int a = 0; a < D; ++a)
     cen[a] = 0.;
   for (int v = 0; v < T::NV; ++v) {
-    const int32_t vv = conn[e * T::NV + v];
+    const int32_t vv = conn[e * stride + v];
 #pragma unroll
     for (int a = 0; a < D; ++a)
       cen[a] += coords[int64_t(vv) * D + a] / T::NV;
@@ -828,6 +841,138 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
   }
 }
 
+// block K of the four blocks of an inner intersection between elements of types GIN (inside) and GOUT
+// (outside) on a mixed grid: 0 in_in [NI][NI], 1 in_out [NI][NO], 2 out_in [NO][NI], 3 out_out [NO][NO]; every
+// entry is the expression of k_inner_local for the same rows and columns, accumulated in the same order
+// (forms, then points), so faces between elements of one type give k_inner_local's bits.  Row chunks of at
+// most 16 entries per lane.
+template <int K, int GIN, int GOUT, int ORDER>
+__device__ __forceinline__ void mixed_face_block(const FaceParams& P, int64_t e, int64_t o, const Geo<Traits<GIN>::D>& gin,
+                                                 const Geo<Traits<GIN>::D>& gout, const FaceFrame<Traits<GIN>::D>& F,
+                                                 bool sym, double* __restrict__ out)
+{
+  constexpr int D = Traits<GIN>::D, NI = Basis<GIN, ORDER>::ND, NO = Basis<GOUT, ORDER>::ND;
+  constexpr int NR = (K < 2) ? NI : NO, NC = (K == 0 || K == 2) ? NI : NO;
+  constexpr int RB = NC <= 4 ? NR : (16 / NC > 0 ? 16 / NC : 1);
+  const double* n = F.n;
+  const int LS = P.ls;
+#pragma unroll
+  for (int i0 = 0; i0 < NR; i0 += RB) {
+    double B[RB][NC];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        B[r][j] = 0.;
+    for (int fm = 0; fm < P.nforms; ++fm) {
+      const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+      const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
+      const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+      const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+      const double sc = P.scaling[fm], prm = P.param[fm];
+      const QRule R = P.rule[fm];
+      for (int q = 0; q < R.npts; ++q) {
+        double s, t, wq, X[D];
+        face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+        F.point(s, t, X);
+        double pin[NI], pout[NO], Gin[NI][D], Gout[NO][D];
+        eval_at<GIN, ORDER>(gin, X, pin, Gin);
+        eval_at<GOUT, ORDER>(gout, X, pout, Gout);
+        const double delta_plus = n_w_n<D>(n, wout);
+        const double delta_minus = n_w_n<D>(n, win);
+        const double factor = F.ie * wq;
+        if (P.kind[fm] == FF_INNER_COUPLING) {
+          const double wm = delta_plus / (delta_plus + delta_minus);
+          const double wp = delta_minus / (delta_plus + delta_minus);
+          double fin[NI], fout[NO];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            fin[i] = flux<D>(kin, Gin[i], n);
+#pragma unroll
+          for (int i = 0; i < NO; ++i)
+            fout[i] = flux<D>(kout, Gout[i], n);
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+              const int i = i0 + r;
+              if (i >= NR)
+                continue;
+              const int a = (sym && i > j) ? j : i, b = (sym && i > j) ? i : j;
+              double v;
+              if constexpr (K == 0)
+                v = -1.0 * wm * fin[b] * pin[a] + -1.0 * prm * wm * pin[b] * fin[a];
+              else if constexpr (K == 1)
+                v = -1.0 * wp * fout[j] * pin[i] + prm * wm * pout[j] * fin[i];
+              else if constexpr (K == 2)
+                v = sym ? -1.0 * wp * fout[i] * pin[j] + prm * wm * pout[i] * fin[j]
+                        : wm * fin[j] * pout[i] + -1.0 * prm * wp * pin[j] * fout[i];
+              else
+                v = wp * fout[b] * pout[a] + prm * wp * pout[b] * fout[a];
+              B[r][j] += sc * (v * factor);
+            }
+        } else {
+          const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus);
+          const double pen = (prm * weight) / F.h;
+#pragma unroll
+          for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+              const int i = i0 + r;
+              if (i >= NR)
+                continue;
+              const int a = (sym && i > j) ? j : i, b = (sym && i > j) ? i : j;
+              double v;
+              if constexpr (K == 0)
+                v = pen * pin[b] * pin[a];
+              else if constexpr (K == 1)
+                v = -1.0 * pen * pout[j] * pin[i];
+              else if constexpr (K == 2)
+                v = sym ? -1.0 * pen * pout[i] * pin[j] : -1.0 * pen * pin[j] * pout[i];
+              else
+                v = pen * pout[b] * pout[a];
+              B[r][j] += sc * (v * factor);
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        if (i0 + r < NR)
+          out[int64_t(K) * LS * LS + (i0 + r) * LS + j] = B[r][j];
+  }
+}
+
+// inner intersections of a mixed grid between an inside element of part P.part_in (type GIN) and an
+// outside element of part P.part_out (type GOUT): one lane per intersection of the walk, the lanes of
+// other type pairs idle; blocks staged with rows of P.ls entries
+template <int GIN, int GOUT, int ORDER>
+__global__ void __launch_bounds__(256) k_inner_mixed(FaceParams P)
+{
+  constexpr int D = Traits<GIN>::D;
+  static_assert(Traits<GOUT>::D == D, "both sides of an intersection have the grid's dimension");
+  const int64_t fi = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (fi >= P.nfaces)
+    return;
+  const int64_t e = P.faces[4 * fi + 0], o = P.faces[4 * fi + 2];
+  const int f = P.faces[4 * fi + 1];
+  if (P.etype[e] != P.part_in || P.etype[o] != P.part_out)
+    return;
+  Geo<D> gin, gout;
+  load_geometry<GIN>(P.coords, P.conn, e, gin, P.cstride);
+  load_geometry<GOUT>(P.coords, P.conn, o, gout, P.cstride);
+  FaceFrame<D> F;
+  face_frame<GIN>(P.coords, P.conn, e, f, F, P.cstride);
+  const bool sym = P.symmetric != 0;
+  double* __restrict__ out = P.staging + P.base + fi * (4 * int64_t(P.ls) * P.ls);
+  mixed_face_block<0, GIN, GOUT, ORDER>(P, e, o, gin, gout, F, sym, out);
+  mixed_face_block<1, GIN, GOUT, ORDER>(P, e, o, gin, gout, F, sym, out);
+  mixed_face_block<2, GIN, GOUT, ORDER>(P, e, o, gin, gout, F, sym, out);
+  mixed_face_block<3, GIN, GOUT, ORDER>(P, e, o, gin, gout, F, sym, out);
+}
+
 template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
 {
@@ -838,10 +983,12 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
     return;
   const int64_t e = P.faces[2 * fi + 0];
   const int f = P.faces[2 * fi + 1];
+  if (P.etype && P.etype[e] != P.part_in)
+    return;
   Geo<D> gin;
-  load_geometry<GEOM>(P.coords, P.conn, e, gin);
+  load_geometry<GEOM>(P.coords, P.conn, e, gin, P.cstride);
   FaceFrame<D> F;
-  face_frame<GEOM>(P.coords, P.conn, e, f, F);
+  face_frame<GEOM>(P.coords, P.conn, e, f, F, P.cstride);
   const double* n = F.n;
   double B[ND][ND];
 #pragma unroll
@@ -883,12 +1030,13 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
       }
     }
   }
-  double* __restrict__ out = P.staging + P.base + fi * (ND * ND);
+  const int LS = P.ls;
+  double* __restrict__ out = P.staging + P.base + fi * (int64_t(LS) * LS);
 #pragma unroll
   for (int i = 0; i < ND; ++i)
 #pragma unroll
     for (int j = 0; j < ND; ++j)
-      out[i * ND + j] = (P.symmetric && i > j) ? B[j][i] : B[i][j];
+      out[i * LS + j] = (P.symmetric && i > j) ? B[j][i] : B[i][j];
 }
 
 // boundary functionals (LocalIntersectionIntegralFunctional::apply, local/functionals/integrals.hh:157-185),
@@ -906,10 +1054,12 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
     return;
   const int64_t e = P.faces[2 * fi + 0];
   const int f = P.faces[2 * fi + 1];
+  if (P.etype && P.etype[e] != P.part_in)
+    return;
   Geo<D> gin;
-  load_geometry<GEOM>(P.coords, P.conn, e, gin);
+  load_geometry<GEOM>(P.coords, P.conn, e, gin, P.cstride);
   FaceFrame<D> F;
-  face_frame<GEOM>(P.coords, P.conn, e, f, F);
+  face_frame<GEOM>(P.coords, P.conn, e, f, F, P.cstride);
   const double* n = F.n;
   double b[ND];
 #pragma unroll
@@ -945,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
       }
     }
   }
-  double* __restrict__ out = P.staging + P.base + fi * ND;
+  double* __restrict__ out = P.staging + P.base + fi * P.ls;
 #pragma unroll
   for (int i = 0; i < ND; ++i)
     out[i] = b[i];
@@ -1301,6 +1451,32 @@ void dispatch_faces(int geom, int order, F&& f)
 
 } // namespace
 
+// (inside type, outside type, order) of the faces of a mixed grid of triangles and quadrilaterals
+template <class F>
+void dispatch_mixed_faces(int gin, int gout, int order, F&& f)
+{
+  using Q = std::integral_constant<int, HGDT_QUAD>;
+  using T = std::integral_constant<int, HGDT_TRI>;
+  auto by_order = [&](auto GI, auto GO) {
+    if (order == 1)
+      f(GI, GO, std::integral_constant<int, 1>{});
+    else if (order == 2)
+      f(GI, GO, std::integral_constant<int, 2>{});
+    else
+      HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals on mixed grids need order 1 or 2");
+  };
+  if (gin == HGDT_QUAD && gout == HGDT_QUAD)
+    by_order(Q{}, Q{});
+  else if (gin == HGDT_QUAD && gout == HGDT_TRI)
+    by_order(Q{}, T{});
+  else if (gin == HGDT_TRI && gout == HGDT_QUAD)
+    by_order(T{}, Q{});
+  else if (gin == HGDT_TRI && gout == HGDT_TRI)
+    by_order(T{}, T{});
+  else
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "mixed grids hold triangles and quadrilaterals");
+}
+
 // quadrature tables of the appended forms: element rules and face rules (dim - 1 coordinates per point)
 // (one ElementParams per part of a mixed grid, each with the rules of its element type)
 static void build_tables(const Operator& op, QTable& etab, QTable& ltab, std::vector<ElementParams>& PE,
@@ -1359,8 +1535,6 @@ void Operator::prepare_assembly()
   if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->order == 2 &&
       space->grid->ref.geom == HGDT_HEX)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for Q2 on quadrilaterals, not hexahedra");
-  if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->grid->mixed)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals need a grid of one element type");
   if (bsources.size() > size_t(MAX_BSOURCES))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
 
@@ -1437,20 +1611,39 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     }
   }
   const int64_t nin = int64_t(M.inner_in.size()), nbd = int64_t(M.bnd_e.size());
+  // faces: one element type, or (mixed grids) one launch per pair of part types over every face of the
+  // walk, each lane keeping the faces of its pair; staged blocks with rows of the space's local size
+  const int nparts = g.mixed ? int(g.parts.size()) : 1;
+  auto part_geom = [&](int p) { return g.mixed ? g.parts[p].ref.geom : g.ref.geom; };
+  for (FaceParams* F : {&C, &B})
+    F->etype = g.mixed ? g.d_etype.ptr : nullptr, F->ls = op.space->local_size, F->cstride = g.ref.nverts;
   if (matrix && !op.cforms.empty() && nin > 0) {
     C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
     C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
-    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
-      hipLaunchKernelGGL((k_inner_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
-    });
+    if (!g.mixed)
+      dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
+        hipLaunchKernelGGL((k_inner_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+      });
+    else
+      for (int pi = 0; pi < nparts; ++pi)
+        for (int po = 0; po < nparts; ++po) {
+          C.part_in = pi, C.part_out = po;
+          dispatch_mixed_faces(part_geom(pi), part_geom(po), op.space->order, [&](auto GI, auto GO, auto O) {
+            hipLaunchKernelGGL((k_inner_mixed<decltype(GI)::value, decltype(GO)::value, decltype(O)::value>),
+                               dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
+          });
+        }
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && !op.bforms.empty() && nbd > 0) {
     B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
     B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
-    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
-      hipLaunchKernelGGL((k_boundary_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
-    });
+    for (int pi = 0; pi < nparts; ++pi) {
+      B.part_in = pi;
+      dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
+        hipLaunchKernelGGL((k_boundary_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+      });
+    }
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
@@ -1469,9 +1662,13 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
       BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
       BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
-    dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
-      hipLaunchKernelGGL((k_boundary_rhs<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
-    });
+    BR.etype = g.mixed ? g.d_etype.ptr : nullptr, BR.ls = op.space->local_size, BR.cstride = g.ref.nverts;
+    for (int pi = 0; pi < nparts; ++pi) {
+      BR.part_in = pi;
+      dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
+        hipLaunchKernelGGL((k_boundary_rhs<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+      });
+    }
     HGDT_HIP(hipGetLastError());
   }
   if (!op.sources.empty() || !op.bsources.empty()) {

@@ -180,6 +180,7 @@ struct Grid {
   int64_t n_inner = 0, n_boundary = 0;
   DeviceArray<double> d_coords;
   DeviceArray<int32_t> d_conn;
+  DeviceArray<int8_t> d_etype; // mixed: etype on the device (the face kernels select their type pair by it)
 
   int ninst() const
   {

@@ -378,6 +378,8 @@ void Grid::count_faces_and_upload()
   d_conn.upload(conn);
   for (GridPart& p : parts)
     p.d_eid.upload(p.eid), p.d_conn.upload(p.conn);
+  if (mixed)
+    d_etype.upload(etype);
 }
 
 // ----------------------------------------------------------------------------
@@ -601,27 +603,29 @@ void Operator::build_scatter_maps()
       for (int f = 0; f < nf; ++f) {
         const int64_t o = g.nb[e * nf + f];
         if (o >= 0 && has_c) {
+          // blocks of ls x ls entries (ls the largest local size: a mixed grid's elements use the first
+          // rows and columns of theirs)
           if (e < o) {
             const int64_t fi = inner_index[e * nf + f];
             const int32_t* dout = &src.dofs[o * ls];
             const int64_t key = e * (nf + 1) + 1 + f;
-            for (int lj = 0; lj < ls; ++lj) // in_in
+            for (int lj = 0; lj < src.elem_size(e); ++lj) // in_in
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 0 * k2 + li * ls + lj), de[lj]});
-            for (int lj = 0; lj < ls; ++lj) // in_out
+            for (int lj = 0; lj < src.elem_size(o); ++lj) // in_out
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 1 * k2 + li * ls + lj), dout[lj]});
           } else {
             const int fo = g.nbface[e * nf + f];
             const int64_t fi = inner_index[o * nf + fo];
             const int32_t* din = &src.dofs[o * ls];
             const int64_t key = o * (nf + 1) + 1 + fo;
-            for (int lj = 0; lj < ls; ++lj) // out_in
+            for (int lj = 0; lj < src.elem_size(o); ++lj) // out_in
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 2 * k2 + li * ls + lj), din[lj]});
-            for (int lj = 0; lj < ls; ++lj) // out_out
+            for (int lj = 0; lj < src.elem_size(e); ++lj) // out_out
               out.push_back({key, uint32_t(M.inner_base + fi * 4 * k2 + 3 * k2 + li * ls + lj), de[lj]});
           }
         } else if (o == -1 && has_b) {
           const int64_t bi = bnd_index[e * nf + f];
-          for (int lj = 0; lj < ls; ++lj)
+          for (int lj = 0; lj < src.elem_size(e); ++lj)
             out.push_back({e * (nf + 1) + 1 + f, uint32_t(M.bnd_base + bi * k2 + li * ls + lj), de[lj]});
         }
       }

@@ -108,8 +108,9 @@ hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* c
  * (hgdt_grid_info) is HGDT_MIXED, with 4 vertices and 4 faces per element (its connectivity has 4 entries
  * per element, -1 after a triangle's third), unless every element has one type: then it is a grid of that
  * type as hgdt_grid_create makes it.  Spaces of order 1 and 2 on mixed grids take element forms, right-hand
- * sides, Dirichlet constraints and error norms; intersection integrals, the adjoint kernel, matrix-free
- * application and unions need one element type. */
+ * sides, Dirichlet constraints, intersection integrals (faces between a triangle and a quadrilateral
+ * included), boundary functionals and error norms; the adjoint kernel, matrix-free application and unions
+ * need one element type. */
 hgdt_status hgdt_grid_create_mixed(int64_t num_vertices, const double* coords, int64_t num_elements,
                                    const int32_t* geometry, const int64_t* offsets, const int32_t* vertices,
                                    hgdt_grid* grid);

@@ -929,20 +929,21 @@ struct Operator {
     if (m.ref.dim != 2)
       throw Error("oracle face integrals are implemented for 2d and 3d grids");
     FaceGeo fg;
-    const int a = m.ref.faces[f][0], b = m.ref.faces[f][1];
+    const RefElement& r = m.eref(e); // the inside element's (mixed meshes: its own type)
+    const int a = r.faces[f][0], b = r.faces[f][1];
     double Pa[3], Pb[3];
-    gin.global(m.ref.verts[a].data(), Pa);
-    gin.global(m.ref.verts[b].data(), Pb);
+    gin.global(r.verts[a].data(), Pa);
+    gin.global(r.verts[b].data(), Pb);
     const double tx = Pb[0] - Pa[0], ty = Pb[1] - Pa[1];
     const double len = std::sqrt(tx * tx + ty * ty);
     fg.integration_element = len, fg.diameter = len;
     double nx = ty / len, ny = -tx / len;
     // orient away from the inside element's centroid
     double cen[2] = {0, 0};
-    for (int v = 0; v < m.ref.nverts; ++v) {
+    for (int v = 0; v < r.nverts; ++v) {
       double P[3];
-      gin.global(m.ref.verts[v].data(), P);
-      cen[0] += P[0] / m.ref.nverts, cen[1] += P[1] / m.ref.nverts;
+      gin.global(r.verts[v].data(), P);
+      cen[0] += P[0] / r.nverts, cen[1] += P[1] / r.nverts;
     }
     if (nx * (Pa[0] - cen[0]) + ny * (Pa[1] - cen[1]) < 0.)
       nx = -nx, ny = -ny;
@@ -952,7 +953,7 @@ struct Operator {
       const double t = q.x[qp][0];
       std::array<double, 3> xh = {0, 0, 0};
       for (int c = 0; c < 2; ++c)
-        xh[c] = (1. - t) * m.ref.verts[a][c] + t * m.ref.verts[b][c];
+        xh[c] = (1. - t) * r.verts[a][c] + t * r.verts[b][c];
       fg.xin.push_back(xh);
       if (gout) {
         double X[3], xo[3] = {0, 0, 0};
@@ -1034,12 +1035,15 @@ struct Operator {
   void coupling_local(const FaceForm& F, int64_t e, int64_t o, const FaceGeo& fg, BoundBasis& bin,
                       BoundBasis& bout, std::vector<double> blocks[4]) const
   {
-    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    // blocks in_in [li][li], in_out [li][lo], out_in [lo][li], out_out [lo][lo] (li, lo: the local sizes of
+    // the inside and outside elements, different between a triangle and a quadrilateral)
+    const int li = bin.fe->size, lo = bout.fe->size, d = space->mesh->ref.dim;
+    const int rows[4] = {li, li, lo, lo}, colsz[4] = {li, lo, li, lo};
     for (int k = 0; k < 4; ++k)
-      blocks[k].assign(ls * ls, 0.);
+      blocks[k].assign(rows[k] * colsz[k], 0.);
     std::vector<double> iv[4];
     for (int k = 0; k < 4; ++k)
-      iv[k].assign(ls * ls, 0.);
+      iv[k].assign(rows[k] * colsz[k], 0.);
     const double* n = fg.normal;
     const double kin = F.kappa.at(e), kout = F.kappa.at(o);
     const double win = F.omega.at(e), wout = F.omega.at(o);
@@ -1056,46 +1060,46 @@ struct Operator {
         const double s = F.param;
         const double weight_minus = delta_plus / (delta_plus + delta_minus);
         const double weight_plus = delta_minus / (delta_plus + delta_minus);
-        for (int i = 0; i < ls; ++i) {
+        for (int i = 0; i < li; ++i) {
           const double kgt_in = kgrad_dot(kin, &bin.grads[3 * i], n, d);
-          for (int j = 0; j < ls; ++j) {
-            iv[0][i * ls + j] += -1.0 * weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
-            iv[0][i * ls + j] += -1.0 * s * weight_minus * bin.vals[j] * kgt_in;
+          for (int j = 0; j < li; ++j) {
+            iv[0][i * li + j] += -1.0 * weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
+            iv[0][i * li + j] += -1.0 * s * weight_minus * bin.vals[j] * kgt_in;
           }
-          for (int j = 0; j < ls; ++j) {
-            iv[1][i * ls + j] += -1.0 * weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bin.vals[i];
-            iv[1][i * ls + j] += s * weight_minus * bout.vals[j] * kgt_in;
+          for (int j = 0; j < lo; ++j) {
+            iv[1][i * lo + j] += -1.0 * weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bin.vals[i];
+            iv[1][i * lo + j] += s * weight_minus * bout.vals[j] * kgt_in;
           }
         }
-        for (int i = 0; i < ls; ++i) {
+        for (int i = 0; i < lo; ++i) {
           const double kgt_out = kgrad_dot(kout, &bout.grads[3 * i], n, d);
-          for (int j = 0; j < ls; ++j) {
-            iv[2][i * ls + j] += weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bout.vals[i];
-            iv[2][i * ls + j] += -1.0 * s * weight_plus * bin.vals[j] * kgt_out;
+          for (int j = 0; j < li; ++j) {
+            iv[2][i * li + j] += weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bout.vals[i];
+            iv[2][i * li + j] += -1.0 * s * weight_plus * bin.vals[j] * kgt_out;
           }
-          for (int j = 0; j < ls; ++j) {
-            iv[3][i * ls + j] += weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bout.vals[i];
-            iv[3][i * ls + j] += s * weight_plus * bout.vals[j] * kgt_out;
+          for (int j = 0; j < lo; ++j) {
+            iv[3][i * lo + j] += weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bout.vals[i];
+            iv[3][i * lo + j] += s * weight_plus * bout.vals[j] * kgt_out;
           }
         }
       } else {
         const double weight = (delta_plus * delta_minus) / (delta_plus + delta_minus); // half harmonic average
         const double penalty = (F.param * weight) / fg.diameter;
-        for (int i = 0; i < ls; ++i) {
-          for (int j = 0; j < ls; ++j)
-            iv[0][i * ls + j] += penalty * bin.vals[j] * bin.vals[i];
-          for (int j = 0; j < ls; ++j)
-            iv[1][i * ls + j] += -1.0 * penalty * bout.vals[j] * bin.vals[i];
+        for (int i = 0; i < li; ++i) {
+          for (int j = 0; j < li; ++j)
+            iv[0][i * li + j] += penalty * bin.vals[j] * bin.vals[i];
+          for (int j = 0; j < lo; ++j)
+            iv[1][i * lo + j] += -1.0 * penalty * bout.vals[j] * bin.vals[i];
         }
-        for (int i = 0; i < ls; ++i) {
-          for (int j = 0; j < ls; ++j)
-            iv[2][i * ls + j] += -1.0 * penalty * bin.vals[j] * bout.vals[i];
-          for (int j = 0; j < ls; ++j)
-            iv[3][i * ls + j] += penalty * bout.vals[j] * bout.vals[i];
+        for (int i = 0; i < lo; ++i) {
+          for (int j = 0; j < li; ++j)
+            iv[2][i * li + j] += -1.0 * penalty * bin.vals[j] * bout.vals[i];
+          for (int j = 0; j < lo; ++j)
+            iv[3][i * lo + j] += penalty * bout.vals[j] * bout.vals[i];
         }
       }
       for (int k = 0; k < 4; ++k)
-        for (int t = 0; t < ls * ls; ++t)
+        for (size_t t = 0; t < iv[k].size(); ++t)
           blocks[k][t] += iv[k][t] * fg.integration_element * fg.w[qp];
     }
   }
@@ -1105,7 +1109,7 @@ struct Operator {
   void boundary_local(const FaceForm& F, int64_t e, const FaceGeo& fg, BoundBasis& bin,
                       std::vector<double>& result) const
   {
-    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    const int ls = bin.fe->size, d = space->mesh->ref.dim;
     result.assign(ls * ls, 0.);
     std::vector<double> iv(ls * ls);
     const double* n = fg.normal;
@@ -1154,7 +1158,7 @@ struct Operator {
   void boundary_source_local(const BoundarySource& S, int64_t e, int64_t bi, const FaceGeo& fg, BoundBasis& bin,
                              std::vector<double>& result) const
   {
-    const int ls = space->fe.size, d = space->mesh->ref.dim;
+    const int ls = bin.fe->size, d = space->mesh->ref.dim;
     result.assign(ls, 0.);
     const double* n = fg.normal;
     const double c = S.coeff.at(e);
@@ -1250,7 +1254,7 @@ struct Operator {
           }
           // boundary functionals, the element's boundary intersections in face order
           for (int f = 0; f < nf && !bsources.empty(); ++f) {
-            if (m.nb[e * nf + f] >= 0)
+            if (m.nb[e * nf + f] != -1)
               continue;
             const int64_t bi = bface_index[e * nf + f];
             for (const auto& S : bsources) {
@@ -1258,7 +1262,7 @@ struct Operator {
                 continue;
               const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, boundary_source_order(S));
               boundary_source_local(S, e, bi, fg, basis, lvec);
-              for (int i = 0; i < ls; ++i) {
+              for (int i = 0; i < le; ++i) {
                 if (atomic) {
 #pragma omp atomic
                   rhs[gi[i]] += lvec[i];
@@ -1273,7 +1277,8 @@ struct Operator {
             if (o >= 0 && e < o && !cforms.empty()) {
               // ApplyOn::InnerIntersectionsOnce: visited from the element with the smaller index
               const Geometry gout = m.geometry(o);
-              bout.bind(&space->fe, &gout);
+              bout.bind(&space->fe_of(o), &gout);
+              const int lo = space->fe_of(o).size;
               const int32_t* gj = &space->indices[o * ls];
               const int32_t* sj = &source->indices[o * ls];
               for (const auto& F : cforms) {
@@ -1281,29 +1286,29 @@ struct Operator {
                 const FaceGeo fg = face_geometry(e, f, o, geo, &gout, order);
                 coupling_local(F, e, o, fg, basis, bout, blocks);
                 // four scatter loops, bilinear-form-assemblers.hh:262-277
-                for (int i = 0; i < ls; ++i) {
-                  for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], si[j], F.scaling * blocks[0][i * ls + j], atomic);
-                  for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], sj[j], F.scaling * blocks[1][i * ls + j], atomic);
+                for (int i = 0; i < le; ++i) {
+                  for (int j = 0; j < le; ++j)
+                    add_to_entry(gi[i], si[j], F.scaling * blocks[0][i * le + j], atomic);
+                  for (int j = 0; j < lo; ++j)
+                    add_to_entry(gi[i], sj[j], F.scaling * blocks[1][i * lo + j], atomic);
                 }
-                for (int i = 0; i < ls; ++i) {
-                  for (int j = 0; j < ls; ++j)
-                    add_to_entry(gj[i], si[j], F.scaling * blocks[2][i * ls + j], atomic);
-                  for (int j = 0; j < ls; ++j)
-                    add_to_entry(gj[i], sj[j], F.scaling * blocks[3][i * ls + j], atomic);
+                for (int i = 0; i < lo; ++i) {
+                  for (int j = 0; j < le; ++j)
+                    add_to_entry(gj[i], si[j], F.scaling * blocks[2][i * le + j], atomic);
+                  for (int j = 0; j < lo; ++j)
+                    add_to_entry(gj[i], sj[j], F.scaling * blocks[3][i * lo + j], atomic);
                 }
               }
-            } else if (o < 0 && !bforms.empty()) {
+            } else if (o == -1 && !bforms.empty()) {
               for (const auto& F : bforms) {
                 if (!on_boundary_subset(F.mask, e, f))
                   continue;
                 const int order = 0 + space->fe.order + space->fe.order + F.over_integrate;
                 const FaceGeo fg = face_geometry(e, f, -1, geo, nullptr, order);
                 boundary_local(F, e, fg, basis, local);
-                for (int i = 0; i < ls; ++i)
-                  for (int j = 0; j < ls; ++j)
-                    add_to_entry(gi[i], si[j], F.scaling * local[i * ls + j], atomic);
+                for (int i = 0; i < le; ++i)
+                  for (int j = 0; j < le; ++j)
+                    add_to_entry(gi[i], si[j], F.scaling * local[i * le + j], atomic);
               }
             }
           }
@@ -1604,8 +1609,6 @@ static int append_face(OrOperator* op, int kind, double param, const double* kap
                        double oc, int over_integrate, double scaling)
 {
   return guarded([&] {
-    if (op->space->mesh->mixed)
-      throw Error("intersection integrals need a mesh of one element type");
     FaceForm F{kind, param, {}, {}, over_integrate, scaling, {}};
     F.kappa.set(kappa, kc, op->space->mesh->ne);
     F.omega.set(omega, oc, op->space->mesh->ne);
@@ -1874,8 +1877,6 @@ int or_operator_append_boundary_source(OrOperator* op, int kind, double param, c
   return guarded([&] {
     if (kind < BS_DIRICHLET_COUPLING || kind > BS_PRODUCT)
       throw Error("unknown boundary functional kind");
-    if (op->space->mesh->mixed)
-      throw Error("boundary functionals need a mesh of one element type");
     BoundarySource S;
     S.kind = kind, S.param = param, S.g_const = g_const, S.g_order = g_order, S.over_integrate = over_integrate;
     const Mesh& m = *op->space->mesh;

@@ -5,7 +5,9 @@ Parity with the oracle (same element list, same DoF numbering) on the reference'
 global indices, sparsity pattern, matrix and right-hand side (Laplace with a per-element coefficient,
 product, constant / per-element / quadrature-point sources), Dirichlet DoFs, the Jacobi-PCG solution and
 the error norms, for continuous and discontinuous Lagrange spaces of orders 1 and 2.  The device's mass
-matrix also sums to the grid's area (closed form), and what needs one element type is refused.
+matrix also sums to the grid's area (closed form), and what needs one element type is refused.  SWIPDG
+with weak Dirichlet data on the same grids (faces joining triangles and quadrilaterals), orders 1 and 2:
+matrix, right-hand side and solve against the oracle.
 """
 import numpy as np
 import pytest
@@ -153,19 +155,84 @@ def test_mixed_single_type_is_plain_grid(hgdt_mod):
     assert np.array_equal(mats[0], mats[1])
 
 
+def _g(X):
+    return 0.5 + np.sin(1.3 * X[0]) * np.cos(0.7 * X[1])
+
+
+@pytest.mark.parametrize("name", ["reference", "strip"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_mixed_swipdg_parity(hgdt_mod, name, order):
+    """SWIPDG (heterogeneous kappa, penalty weighted by kappa) with weak Dirichlet data on a mixed grid, whose
+    intersections join triangles and quadrilaterals: matrix and right-hand side against the oracle (1e-12),
+    the solve against the oracle's (1e-10, its iteration count)"""
+    H = hgdt_mod
+    print(f"case mixed SWIPDG {name} order={order}", flush=True)
+    V, E = _grid(name)
+    sigma = 16.0 if order == 1 else 40.0
+    kappa = np.exp(np.random.default_rng(5).normal(size=len(E)))
+    grid = H.Grid.mixed(V, E)
+    space = H.DiscontinuousLagrangeSpace(grid, order)
+    kd = torch.from_numpy(kappa).cuda()
+    g = H.GenericFunction(2, _g)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(kd))
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(
+        H.LocalLaplaceIPDGInnerCoupling(1.0, kd, kd) + H.LocalIPDGInnerPenalty(sigma, kd)), {},
+        H.ApplyOn.InnerIntersectionsOnce())
+    boundary = H.ApplyOn.CustomBoundaryIntersections(H.AllDirichletBoundaryInfo())
+    op.append(H.LocalIntersectionIntegralBilinearForm(
+        H.LocalIPDGBoundaryPenalty(sigma, kd) + H.LocalLaplaceIPDGDirichletCoupling(1.0, kd)), {}, boundary)
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalLaplaceIPDGDirichletCoupling(1.0, kd, g)), {}, boundary)
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalIPDGBoundaryPenalty(sigma, kd).with_ansatz(g)), {},
+               boundary)
+    op.append(rhs)
+    op.assemble()
+    m = O.Mesh.mixed(V, E)
+    s = O.Space(m, False, order)
+    oop = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    oop.append_laplace(kappa)
+    oop.append_inner_coupling(1.0, kappa, kappa)
+    oop.append_inner_penalty(sigma, kappa)
+    oop.append_dirichlet_coupling(1.0, kappa)
+    oop.append_boundary_penalty(sigma, kappa)
+    oop.append_source(1.0)
+    P1 = m.boundary_quadrature_points(2 + max(order - 1, 0))  # coupling: data order 2 + max(test - 1, 0)
+    P2 = m.boundary_quadrature_points(order + 2)  # penalty: test order + data order 2
+    assert np.abs(P1 - grid.boundary_quadrature_points(2 + max(order - 1, 0))).max() <= 1e-15
+    oop.append_boundary_source(0, 1.0, kappa, _g(P1.transpose(2, 0, 1)), g_order=2)
+    oop.append_boundary_source(1, sigma, kappa, _g(P2.transpose(2, 0, 1)), g_order=2)
+    oop.assemble(1)
+    assert np.array_equal(space.global_indices(), s.global_indices())
+    rp, cols = op.pattern()
+    orp, ocols = oop.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)
+    err = matrix_error(op.matrix.values.cpu().numpy(), oop.values(), orp)
+    b, ob = op.rhs.cpu().numpy(), oop.rhs()
+    berr = np.abs(b - ob).max() / np.abs(ob).max()
+    xo, ito, relo, rc = O.pcg_jacobi(orp, ocols, oop.values(), ob, rtol=1e-10, max_iter=100000)
+    x = op.apply_inverse("rhs", opts={"precision": 1e-10, "max_iter": 100000}).cpu().numpy()
+    rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"  {len(E)} elements, {s.size} DoFs: matrix {err:.2e}, rhs {berr:.2e}; iterations "
+          f"{op.last_solve['iterations']} (oracle {ito}), rel L2 {rel:.2e}")
+    assert err <= 1e-12 and berr <= 1e-12
+    assert rel <= 1e-10 and abs(op.last_solve["iterations"] - ito) <= 2
+
+
 def test_mixed_refusals(hgdt_mod):
     H = hgdt_mod
     print("case mixed refusals", flush=True)
     V, E = refined_reference_mixed_grid()
     grid = H.Grid.mixed(V, E)
-    space = H.DiscontinuousLagrangeSpace(grid, 1)
-    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
-    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
-    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(16.0, 1.0)), {},
-              H.ApplyOn.InnerIntersectionsOnce())
-    with pytest.raises(H.not_supported):
-        op.assemble()
     with pytest.raises(H.not_supported):
         H.Grid.union([grid, grid])
     with pytest.raises(H.wrong_input_given):
         H.Grid.mixed(V, [[0, 1]])
+    space = H.ContinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(1.0)))
+    op.set_format("matrix_free")
+    op.assemble()
+    with pytest.raises(H.not_supported):
+        op.apply(torch.zeros(space.num_DoFs, dtype=torch.float64, device="cuda"))

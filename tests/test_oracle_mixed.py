@@ -10,7 +10,10 @@ fixes, independently of the oracle's own formulas:
     stiffness matrix maps the interpolant of a linear function (orders 1, 2) and of a quadratic (order 2) to
     -Laplace(u) times the lumped mass in every interior row, and u^T K u = int |grad u|^2;
   * the element-ordered quadrature-point layout: the L2 error of the interpolant of a linear function
-    against its own values at the points is zero, and without them it equals u^T M u.
+    against its own values at the points is zero, and without them it equals u^T M u;
+  * intersections between a triangle and a quadrilateral: the penalty form of a two-element grid in closed
+    form (face mass of the linear traces), SWIPDG with weak Dirichlet data reproducing a linear (order 1) and
+    a quadratic (order 2) solution on a sheared mixed grid, and the SWIPDG matrix symmetric positive definite.
 """
 import numpy as np
 import pytest
@@ -177,10 +180,103 @@ def test_mixed_error_norm_layout():
     assert abs(norms[1] - (2.0 ** 2 + 0.5 ** 2) * grid_area(V, E)) < 1e-12
 
 
-def test_mixed_refusals():
+def element_nodes(V, E, order):
+    """Lagrange node positions of every element's local DoFs (cubes: tensor index, simplices: the vertices,
+    then the edge midpoints (a, b), a < b)"""
+    out = []
+    for el in E:
+        p = V[el]
+        a, b = p[1] - p[0], p[2] - p[0]
+        if len(el) == 4:
+            ref = [(i / order, j / order) for j in range(order + 1) for i in range(order + 1)]
+        else:
+            ref = [(0., 0.), (1., 0.), (0., 1.)] + ([(.5, 0.), (0., .5), (.5, .5)] if order == 2 else [])
+        out.append(np.array([p[0] + xi * a + eta * b for xi, eta in ref]))
+    return out
+
+
+def test_mixed_face_penalty_closed_form():
+    """one quadrilateral [0,1]^2 and one triangle (1,0), (2,0.5), (1,1) sharing the edge x = 1 (length and
+    diameter 1), inner penalty sigma with unit weight only (ipdg.hh:149-153: the half harmonic average of the
+    weights, 1/2): the face mass of the linear traces is (1/6)[[2, 1], [1, 2]], so the quadrilateral's trace
+    DoFs (local 1 at (1,0), 3 at (1,1)) get sigma/6 and sigma/12, the triangle's (local 0 at (1,0), 2 at (1,1))
+    the same, and the coupling block -sigma/6 between DoFs at the same vertex, -sigma/12 across; every other
+    entry 0"""
+    V = np.array([(0., 0.), (1., 0.), (0., 1.), (1., 1.), (2., 0.5)])
+    E = [[0, 1, 2, 3], [1, 4, 3]]
+    m = O.Mesh.mixed(V, E)
+    s = O.Space(m, False, 1)
+    sigma = 7.0
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_inner_penalty(sigma, 1.0)
+    op.assemble()
+    A = op.csr().toarray()
+    R = np.zeros((7, 7))
+    q, t = [1, 3], [4 + 0, 4 + 2]  # trace DoFs at (1, 0) and (1, 1)
+    M = 0.5 * np.array([[2., 1.], [1., 2.]]) / 6
+    for a in range(2):
+        for b in range(2):
+            R[q[a], q[b]] += sigma * M[a, b]
+            R[t[a], t[b]] += sigma * M[a, b]
+            R[q[a], t[b]] -= sigma * M[a, b]
+            R[t[a], q[b]] -= sigma * M[a, b]
+    assert A.shape == (7, 7) and np.abs(A - R).max() <= 1e-14
+
+
+def _swipdg_mixed(V, E, order, u, f, sigma):
+    m = O.Mesh.mixed(V, E)
+    s = O.Space(m, False, order)
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(1.0)
+    op.append_inner_coupling(1.0, 1.0, 1.0)
+    op.append_inner_penalty(sigma, 1.0)
+    op.append_dirichlet_coupling(1.0, 1.0)
+    op.append_boundary_penalty(sigma, 1.0)
+    if f != 0.0:
+        op.append_source(f)
+    # the quadrilaterals' coupling functional needs one order more than the reference's to be exact
+    op.append_boundary_source(0, 1.0, 1.0, u(m.boundary_quadrature_points(2 * order)), g_order=order,
+                              over_integrate=1)
+    op.append_boundary_source(1, sigma, 1.0, u(m.boundary_quadrature_points(2 * order)), g_order=order)
+    op.assemble(1)
+    return m, s, op
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mixed_swipdg_reproduces_polynomial_solutions(order):
+    """SWIPDG with weak Dirichlet data on a sheared mixed grid reproduces a solution in the discrete space:
+    u = 0.3 + 2x - 1.5y (order 1) and u = x^2 - xy + y^2/2 + 0.3x with -Laplace(u) = -3 (order 2); the faces
+    between triangles and quadrilaterals evaluate each side's basis at the face points"""
+    import scipy.sparse.linalg as spla
+
+    V, E = strip_mixed_grid(5, 4, shear=0.2)
+    if order == 1:
+        u = lambda P: 0.3 + 2.0 * P[..., 0] - 1.5 * P[..., 1]
+        f = 0.0
+    else:
+        u = lambda P: P[..., 0] ** 2 - P[..., 0] * P[..., 1] + 0.5 * P[..., 1] ** 2 + 0.3 * P[..., 0]
+        f = -3.0
+    m, s, op = _swipdg_mixed(V, E, order, u, f, 40.0)
+    x = spla.spsolve(op.csr().tocsc(), op.rhs())
+    gi = s.global_indices()
+    nodes = element_nodes(V, E, order)
+    err = max(np.abs(x[gi[e, :len(nodes[e])]] - u(nodes[e])).max() for e in range(len(E)))
+    assert err <= 1e-10, err
+
+
+def test_mixed_swipdg_is_spd():
+    """the SWIPDG matrix (symmetric prefactor 1, sigma = 16) on the reference's mixed grid is symmetric and
+    positive definite"""
     V, E = refined_reference_mixed_grid()
     m = O.Mesh.mixed(V, E)
     s = O.Space(m, False, 1)
     op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
-    with pytest.raises(RuntimeError):
-        op.append_inner_penalty(16.0)
+    op.append_laplace(1.0)
+    op.append_inner_coupling(1.0, 1.0, 1.0)
+    op.append_inner_penalty(16.0, 1.0)
+    op.append_dirichlet_coupling(1.0, 1.0)
+    op.append_boundary_penalty(16.0, 1.0)
+    op.assemble()
+    A = op.csr().toarray()
+    assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
+    assert np.linalg.eigvalsh(A).min() > 0
