@@ -1101,6 +1101,241 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
     out[i] = b[i];
 }
 
+// ---- intersections of Q2 hexahedra: 27 basis functions per side, one lane per (face, row) ----
+// The 1-d factors of the basis at the physical point X of an element (its reference coordinates by the
+// inverse map, as eval_at), and basis function j's value and physical gradient from them.
+__device__ __forceinline__ void q2hex_factors(const Geo<3>& g, const double* X, double (&l)[3][3], double (&dl)[3][3])
+{
+  double xh[3], dx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    dx[a] = X[a] - g.x0[a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double v = g.Ji[a][0] * dx[0];
+#pragma unroll
+    for (int b = 1; b < 3; ++b)
+      v += g.Ji[a][b] * dx[b];
+    xh[a] = v;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      l[a][m] = lag2(m, xh[a]), dl[a][m] = dlag2(m, xh[a]);
+}
+__device__ __forceinline__ void q2hex_basis(const Geo<3>& g, const double (&l)[3][3], const double (&dl)[3][3], int j,
+                                            double& phi, double (&G)[3])
+{
+  const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
+  const double lx = pick3(jx, l[0]), ly = pick3(jy, l[1]), lz = pick3(jz, l[2]);
+  phi = (lx * ly) * lz;
+  const double gh[3] = {(pick3(jx, dl[0]) * ly) * lz, (pick3(jy, dl[1]) * lx) * lz, (pick3(jz, dl[2]) * lx) * ly};
+  to_physical<3>(g, gh, G);
+}
+
+// inner intersections: lane (face, r), r < 27: row r of in_in and in_out; r >= 27: row r - 27 of out_in and
+// out_out.  Every entry is k_inner_local's expression (the diagonal blocks from their upper triangle and
+// out_in as the transpose of in_out for symmetric forms, so the entries of two lanes that mirror each
+// other are the same bits), accumulated over forms and points in the same order.
+__global__ void __launch_bounds__(256) k_inner_rows_q2hex(FaceParams P)
+{
+  constexpr int ND = 27, D = 3;
+  const int64_t lane = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= P.nfaces * 2 * ND)
+    return;
+  const int64_t fi = lane / (2 * ND);
+  const int rr = int(lane % (2 * ND));
+  const bool outside_rows = rr >= ND;
+  const int i = rr % ND;
+  const int64_t e = P.faces[4 * fi + 0], o = P.faces[4 * fi + 2];
+  const int f = P.faces[4 * fi + 1];
+  Geo<D> gin, gout;
+  load_geometry<HGDT_HEX>(P.coords, P.conn, e, gin);
+  load_geometry<HGDT_HEX>(P.coords, P.conn, o, gout);
+  FaceFrame<D> F;
+  face_frame<HGDT_HEX>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
+  const bool sym = P.symmetric != 0;
+  double A[ND], B[ND]; // inside rows: in_in, in_out; outside rows: out_in, out_out
+#pragma unroll
+  for (int j = 0; j < ND; ++j)
+    A[j] = B[j] = 0.;
+  for (int fm = 0; fm < P.nforms; ++fm) {
+    const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+    const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
+    const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+    const double sc = P.scaling[fm], prm = P.param[fm];
+    const QRule R = P.rule[fm];
+    for (int q = 0; q < R.npts; ++q) {
+      double s, t, wq, X[D];
+      face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+      F.point(s, t, X);
+      double li[3][3], dli[3][3], lo[3][3], dlo[3][3];
+      q2hex_factors(gin, X, li, dli);
+      q2hex_factors(gout, X, lo, dlo);
+      const double delta_plus = n_w_n<D>(n, wout);
+      const double delta_minus = n_w_n<D>(n, win);
+      const double factor = F.ie * wq;
+      const bool coupling = P.kind[fm] == FF_INNER_COUPLING;
+      const double wm = delta_plus / (delta_plus + delta_minus);
+      const double wp = delta_minus / (delta_plus + delta_minus);
+      const double pen = (prm * ((delta_plus * delta_minus) / (delta_plus + delta_minus))) / F.h;
+      // the lane's own row function (inside or outside) and its flux
+      double pr, Gr[3];
+      if (outside_rows)
+        q2hex_basis(gout, lo, dlo, i, pr, Gr);
+      else
+        q2hex_basis(gin, li, dli, i, pr, Gr);
+      const double fr = flux<D>(outside_rows ? kout : kin, Gr, n);
+      for (int j = 0; j < ND; ++j) {
+        double pin, Gin[3], pout, Gout[3];
+        q2hex_basis(gin, li, dli, j, pin, Gin);
+        q2hex_basis(gout, lo, dlo, j, pout, Gout);
+        const double fin = flux<D>(kin, Gin, n), fout = flux<D>(kout, Gout, n);
+        double va, vb;
+        if (!outside_rows) {
+          // in_in (i, j): from the upper triangle when symmetric; in_out (i, j)
+          const bool sw = sym && i > j;
+          const double pa = sw ? pin : pr, fa = sw ? fin : fr, pb = sw ? pr : pin, fb = sw ? fr : fin;
+          if (coupling) {
+            va = -1.0 * wm * fb * pa + -1.0 * prm * wm * pb * fa;
+            vb = -1.0 * wp * fout * pr + prm * wm * pout * fr;
+          } else {
+            va = pen * pb * pa;
+            vb = -1.0 * pen * pout * pr;
+          }
+        } else {
+          // out_in (i, j): the transpose of in_out when symmetric; out_out (i, j) from the upper triangle
+          const bool sw = sym && i > j;
+          const double pa = sw ? pout : pr, fa = sw ? fout : fr, pb = sw ? pr : pout, fb = sw ? fr : fout;
+          if (coupling) {
+            va = sym ? -1.0 * wp * fr * pin + prm * wm * pr * fin : wm * fin * pr + -1.0 * prm * wp * pin * fr;
+            vb = wp * fb * pa + prm * wp * pb * fa;
+          } else {
+            va = sym ? -1.0 * pen * pr * pin : -1.0 * pen * pin * pr;
+            vb = pen * pb * pa;
+          }
+        }
+        A[j] += sc * (va * factor);
+        B[j] += sc * (vb * factor);
+      }
+    }
+  }
+  constexpr int K2 = ND * ND;
+  double* __restrict__ out = P.staging + P.base + fi * (4 * K2);
+  const int ka = outside_rows ? 2 : 0, kb = outside_rows ? 3 : 1;
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    out[ka * K2 + i * ND + j] = A[j];
+    out[kb * K2 + i * ND + j] = B[j];
+  }
+}
+
+// boundary intersections of Q2 hexahedra: lane (face, row i); for symmetric forms an entry (i, j) with
+// i > j takes the value k_boundary_local stores there, B[j][i]
+__global__ void __launch_bounds__(256) k_boundary_rows_q2hex(FaceParams P)
+{
+  constexpr int ND = 27, D = 3;
+  const int64_t lane = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= P.nfaces * ND)
+    return;
+  const int64_t fi = lane / ND;
+  const int i = int(lane % ND);
+  const int64_t e = P.faces[2 * fi + 0];
+  const int f = P.faces[2 * fi + 1];
+  Geo<D> gin;
+  load_geometry<HGDT_HEX>(P.coords, P.conn, e, gin);
+  FaceFrame<D> F;
+  face_frame<HGDT_HEX>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
+  double row[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j)
+    row[j] = 0.;
+  for (int fm = 0; fm < P.nforms; ++fm) {
+    if (P.mask[fm] && !P.mask[fm][fi])
+      continue;
+    const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
+    const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double sc = P.scaling[fm], prm = P.param[fm];
+    const QRule R = P.rule[fm];
+    for (int q = 0; q < R.npts; ++q) {
+      double s, t, wq, X[D];
+      face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+      F.point(s, t, X);
+      double l[3][3], dl[3][3];
+      q2hex_factors(gin, X, l, dl);
+      double pr, Gr[3];
+      q2hex_basis(gin, l, dl, i, pr, Gr);
+      const double fr = flux<D>(kin, Gr, n);
+      const double factor = F.ie * wq;
+      const double pen = (prm * n_w_n<D>(n, win)) / F.h;
+      for (int j = 0; j < ND; ++j) {
+        double pj, Gj[3];
+        q2hex_basis(gin, l, dl, j, pj, Gj);
+        const double fj = flux<D>(kin, Gj, n);
+        // entry (r, c) of k_boundary_local: -fin[c] pin[r] - prm pin[c] fin[r] or pen pin[c] pin[r]; the stored
+        // (i, j) is (r, c) = (j, i) when symmetric and i > j
+        const bool sw = P.symmetric && i > j;
+        const double prr = sw ? pj : pr, frr = sw ? fj : fr, pc = sw ? pr : pj, fc = sw ? fr : fj;
+        const double v = (P.kind[fm] == FF_DIRICHLET_COUPLING) ? -1.0 * fc * prr + -1.0 * prm * pc * frr
+                                                               : pen * pc * prr;
+        row[j] += sc * (v * factor);
+      }
+    }
+  }
+  double* __restrict__ out = P.staging + P.base + fi * (ND * ND);
+#pragma unroll
+  for (int j = 0; j < ND; ++j)
+    out[i * ND + j] = row[j];
+}
+
+// boundary functionals on Q2 hexahedra: lane (face, row i), the terms of k_boundary_rhs for entry i
+__global__ void __launch_bounds__(256) k_boundary_rhs_rows_q2hex(BoundaryRhsParams P)
+{
+  constexpr int ND = 27, D = 3;
+  const int64_t lane = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (lane >= P.nfaces * ND)
+    return;
+  const int64_t fi = lane / ND;
+  const int i = int(lane % ND);
+  const int64_t e = P.faces[2 * fi + 0];
+  const int f = P.faces[2 * fi + 1];
+  Geo<D> gin;
+  load_geometry<HGDT_HEX>(P.coords, P.conn, e, gin);
+  FaceFrame<D> F;
+  face_frame<HGDT_HEX>(P.coords, P.conn, e, f, F);
+  const double* n = F.n;
+  double b = 0.;
+  for (int tm = 0; tm < P.nterms; ++tm) {
+    if (P.mask[tm] && !P.mask[tm][fi])
+      continue;
+    const double c = P.coeff[tm] ? P.coeff[tm][e] : P.coeff_value[tm];
+    const double prm = P.param[tm];
+    const QRule R = P.rule[tm];
+    for (int q = 0; q < R.npts; ++q) {
+      double s, t, wq, X[D];
+      face_qp<D - 1>(P.qtab, R, q, s, t, wq);
+      F.point(s, t, X);
+      double l[3][3], dl[3][3];
+      q2hex_factors(gin, X, l, dl);
+      double pr, Gr[3];
+      q2hex_basis(gin, l, dl, i, pr, Gr);
+      const double g = P.data[tm] ? P.data[tm][fi * R.npts + q] : P.value[tm];
+      const double factor = F.ie * wq;
+      if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING)
+        b += (-1.0 * prm * g * flux<D>(c, Gr, n)) * factor;
+      else if (P.kind[tm] == HGDT_BOUNDARY_PENALTY)
+        b += (((prm * n_w_n<D>(n, c)) / F.h) * g * pr) * factor;
+      else
+        b += ((c * g) * pr) * factor;
+    }
+  }
+  P.staging[P.base + fi * ND + i] = b;
+}
+
 // fixed-order gather of the staged local contributions into the CSR values
 // sum of staging[contrib[c]] for c = c0 .. c1 - 1, added in that order (the walk order); the loads of U
 // contributions are issued together so that a lane keeps U index / value pairs in flight
@@ -1532,9 +1767,6 @@ void Operator::prepare_assembly()
   if (eforms.size() > size_t(MAX_EFORMS) || sources.size() > size_t(MAX_SOURCES) || cforms.size() > size_t(MAX_FFORMS)
       || bforms.size() > size_t(MAX_FFORMS))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many local forms appended");
-  if ((!cforms.empty() || !bforms.empty() || !bsources.empty()) && space->order == 2 &&
-      space->grid->ref.geom == HGDT_HEX)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "intersection integrals are provided for Q2 on quadrilaterals, not hexahedra");
   if (bsources.size() > size_t(MAX_BSOURCES))
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "too many boundary functionals appended");
 
@@ -1614,13 +1846,16 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
   // faces: one element type, or (mixed grids) one launch per pair of part types over every face of the
   // walk, each lane keeping the faces of its pair; staged blocks with rows of the space's local size
   const int nparts = g.mixed ? int(g.parts.size()) : 1;
+  const bool q2hex = !g.mixed && g.ref.geom == HGDT_HEX && op.space->order == 2;
   auto part_geom = [&](int p) { return g.mixed ? g.parts[p].ref.geom : g.ref.geom; };
   for (FaceParams* F : {&C, &B})
     F->etype = g.mixed ? g.d_etype.ptr : nullptr, F->ls = op.space->local_size, F->cstride = g.ref.nverts;
   if (matrix && !op.cforms.empty() && nin > 0) {
     C.nfaces = nin, C.faces = M.d_inner.ptr, C.coords = g.d_coords.ptr, C.conn = g.d_conn.ptr;
     C.qtab = op.d_qtab_line.ptr, C.staging = op.d_staging.ptr, C.base = M.inner_base;
-    if (!g.mixed)
+    if (q2hex) // 27 basis functions per side: one lane per (face, row)
+      hipLaunchKernelGGL(k_inner_rows_q2hex, dim3(grid_for(nin * 54, fb)), dim3(fb), 0, stream, C);
+    else if (!g.mixed)
       dispatch_faces(g.ref.geom, op.space->order, [&](auto G, auto O) {
         hipLaunchKernelGGL((k_inner_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nin, fb)), dim3(fb), 0, stream, C);
       });
@@ -1638,12 +1873,15 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
   if (matrix && !op.bforms.empty() && nbd > 0) {
     B.nfaces = nbd, B.faces = M.d_bnd.ptr, B.coords = g.d_coords.ptr, B.conn = g.d_conn.ptr;
     B.qtab = op.d_qtab_line.ptr, B.staging = op.d_staging.ptr, B.base = M.bnd_base;
-    for (int pi = 0; pi < nparts; ++pi) {
-      B.part_in = pi;
-      dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
-        hipLaunchKernelGGL((k_boundary_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
-      });
-    }
+    if (q2hex)
+      hipLaunchKernelGGL(k_boundary_rows_q2hex, dim3(grid_for(nbd * 27, fb)), dim3(fb), 0, stream, B);
+    else
+      for (int pi = 0; pi < nparts; ++pi) {
+        B.part_in = pi;
+        dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
+          hipLaunchKernelGGL((k_boundary_local<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, B);
+        });
+      }
     HGDT_HIP(hipGetLastError());
   }
   if (matrix && (!op.eforms.empty() || !op.cforms.empty() || !op.bforms.empty())) {
@@ -1663,12 +1901,15 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
       BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
     BR.etype = g.mixed ? g.d_etype.ptr : nullptr, BR.ls = op.space->local_size, BR.cstride = g.ref.nverts;
-    for (int pi = 0; pi < nparts; ++pi) {
-      BR.part_in = pi;
-      dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
-        hipLaunchKernelGGL((k_boundary_rhs<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
-      });
-    }
+    if (q2hex)
+      hipLaunchKernelGGL(k_boundary_rhs_rows_q2hex, dim3(grid_for(nbd * 27, fb)), dim3(fb), 0, stream, BR);
+    else
+      for (int pi = 0; pi < nparts; ++pi) {
+        BR.part_in = pi;
+        dispatch_faces(part_geom(pi), op.space->order, [&](auto G, auto O) {
+          hipLaunchKernelGGL((k_boundary_rhs<decltype(G)::value, decltype(O)::value>), dim3(grid_for(nbd, fb)), dim3(fb), 0, stream, BR);
+        });
+      }
     HGDT_HIP(hipGetLastError());
   }
   if (!op.sources.empty() || !op.bsources.empty()) {

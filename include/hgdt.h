@@ -154,11 +154,11 @@ hgdt_status hgdt_grid_boundary_intersections(hgdt_grid grid, double* centers, do
 
 /* ContinuousLagrangeSpace<GV>(grid_view, order), dune/gdt/spaces/h1/continuous-lagrange.hh:68-79
  * DiscontinuousLagrangeSpace<GV>(grid_view, order), dune/gdt/spaces/l2/discontinuous-lagrange.hh:78-86
- * order 1 (P1 on simplices, Q1 on cubes) or 2 (P2 on triangles and tetrahedra, Q2 on quadrilaterals; the
- * local DoF order of the Lagrange nodes: cubes by tensor index, simplices vertices then edge midpoints
- * (a, b), a < b, lexicographic; continuous order 2 numbers one DoF per subentity in order of first
- * appearance in the element walk).  Order-2 spaces take element forms and right-hand sides; intersection
- * integrals and the adjoint kernel need order 1. */
+ * order 1 (P1 on simplices, Q1 on cubes) or 2 (P2 on triangles and tetrahedra, Q2 on quadrilaterals and
+ * hexahedra; the local DoF order of the Lagrange nodes: cubes by tensor index, simplices vertices then edge
+ * midpoints (a, b), a < b, lexicographic; continuous order 2 numbers one DoF per subentity in order of first
+ * appearance in the element walk).  Order-2 spaces take element forms, right-hand sides, intersection
+ * integrals and boundary functionals; the adjoint kernel needs order 1. */
 hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space);
 hgdt_status hgdt_space_destroy(hgdt_space space);
 /* mapper().size() and max_local_size(), spaces/mapper/continuous.hh:139-150 */

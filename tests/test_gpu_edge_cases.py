@@ -174,14 +174,15 @@ def test_invalid_input_raises(hgdt_mod):
     # order 3 spaces are not provided
     with pytest.raises(H.not_supported):
         H.ContinuousLagrangeSpace(grid, 3)
-    # intersection integrals on Q2 hexahedra (27 basis functions on each side of a face)
-    hex_grid = H.Grid.structured("hex", [0, 0, 0], [1, 1, 1], [2, 2, 2])
-    dg2 = H.DiscontinuousLagrangeSpace(hex_grid, 2)
-    op2 = H.make_matrix_operator(dg2, H.Stencil.element_and_intersection)
-    op2.append(H.LocalCouplingIntersectionIntegralBilinearForm(H.LocalIPDGInnerPenalty(8.0, 1.0)), {},
-               H.ApplyOn.InnerIntersectionsOnce())
+    # the adjoint kernel on order-2 spaces
+    dg2 = H.ContinuousLagrangeSpace(grid, 2)
+    op2 = H.make_matrix_operator(dg2, H.Stencil.element)
+    op2.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.ones(16, dtype=torch.float64,
+                                                                                      device="cuda"))))
+    op2.assemble()
+    u2 = torch.zeros(dg2.num_DoFs, dtype=torch.float64, device="cuda")
     with pytest.raises(H.not_supported):
-        op2.assemble()
+        op2.adjoint_log_diffusion_gradient(u2, u2)
     # an operator between spaces of different orders
     with pytest.raises(H.not_supported):
         H.make_matrix_operator(space, H.Stencil.element, range_space=H.ContinuousLagrangeSpace(grid, 2))

@@ -951,3 +951,44 @@ def test_swipdg_order2_reproduces_quadratic_solution(geom):
                                     lo[:, 1] + t[i // 3] * (hi[:, 1] - lo[:, 1])], -1) for i in range(9)], 1)
     err = np.abs(x[gi] - u(nodes)).max()
     assert err <= 1e-10, err
+
+@pytest.mark.parametrize("geom", [O.HEX, O.TET])
+def test_swipdg_order2_reproduces_quadratic_solution_3d(geom):
+    """the same on Q2 hexahedra (27 basis functions per side of a face) and P2 tetrahedra:
+    u = x^2 - xy + y^2/2 + 0.3x + z^2 - 0.2yz, -Laplace(u) = -5, weak Dirichlet data g = u, reproduced at every
+    Lagrange node (for hexahedra the Dirichlet coupling functional over-integrates by one)"""
+    import scipy.sparse.linalg as spla
+
+    n, sigma = 2, 60.0
+    m = O.Mesh.structured(geom, [n, n, n], [0, 0, 0], [1, 1, 1])
+    s = O.Space(m, False, 2)
+
+    def u(P):
+        x, y, z = P[..., 0], P[..., 1], P[..., 2]
+        return x * x - x * y + 0.5 * y * y + 0.3 * x + z * z - 0.2 * y * z
+
+    over = 1 if geom == O.HEX else 0
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace(1.0)
+    op.append_inner_coupling(1.0, 1.0, 1.0)
+    op.append_inner_penalty(sigma, 1.0)
+    op.append_dirichlet_coupling(1.0, 1.0)
+    op.append_boundary_penalty(sigma, 1.0)
+    op.append_source(-5.0)
+    op.append_boundary_source(0, 1.0, 1.0, u(m.boundary_quadrature_points(2 + 1 + over)), g_order=2,
+                              over_integrate=over)
+    op.append_boundary_source(1, sigma, 1.0, u(m.boundary_quadrature_points(2 + 2)), g_order=2)
+    op.assemble(1)
+    x = spla.spsolve(op.csr().tocsc(), op.rhs())
+    Xe = m.coords()[m.conn()]  # [ne, local vertex, 3]
+    gi = s.global_indices()
+    if geom == O.TET:
+        pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+        nodes = np.concatenate([Xe, np.stack([(Xe[:, a] + Xe[:, b]) / 2 for a, b in pairs], 1)], axis=1)
+    else:
+        x0 = Xe[:, 0]
+        E = np.stack([Xe[:, 1] - x0, Xe[:, 2] - x0, Xe[:, 4] - x0], 1)  # edge vectors along x, y, z
+        ref = np.array([[t % 3, (t // 3) % 3, t // 9] for t in range(27)]) / 2.0
+        nodes = x0[:, None, :] + np.einsum("ta,eab->etb", ref, E)
+    err = np.abs(x[gi] - u(nodes)).max()
+    assert err <= 1e-9, err
