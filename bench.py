@@ -345,6 +345,13 @@ def run_single(hgdt, torch, args, config, fmt, steps, warmup, spmv_sample):
     solve_ms = float(np.mean([s["solve_ms"] for s in stats]))
     mf = fi["format"] == "matrix_free"
     traffic, tsrc = traffic_record(config, mf)
+    # the whole step at its algorithmic bytes: the assembly's plus the iterations' (application, direction
+    # update, update), over the timed step
+    asm_bytes = op.assembly_bytes()
+    it_bytes = int(stats[-1]["iteration_bytes"])
+    mean_iters = float(np.mean([s["iterations"] for s in stats]))
+    total = asm_bytes + mean_iters * it_bytes
+    step_rate = total / (ms_per_step * 1e-3) / 1e9
     res = {
         "value": n / (ms_per_step * 1e-3), "ms_per_step": ms_per_step,
         "config": {"workload": workload, "dofs": n, "elements": grid.num_elements, "nnz": op.nnz,
@@ -361,6 +368,12 @@ def run_single(hgdt, torch, args, config, fmt, steps, warmup, spmv_sample):
                                "from the end of "
                                "the kernel before the application (the direction update; for a matrix-free application "
                                "fused with it, the previous iteration's update) to the application's end"},
+        "step_hbm": {"achieved": step_rate, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_rate / HBM_PEAK_GBS,
+                     "bytes_per_step": total, "assembly_bytes": asm_bytes, "iteration_bytes": it_bytes,
+                     "basis": "algorithmic bytes of the assembly and of the step's iterations (operator application, "
+                              "direction update, update; the formats' compulsory bytes, as the roofline's) over "
+                              "ms_per_step; the solve's set-up and exit passes (initial residual, the solver's scaled "
+                              "copy, true residual) are not counted"},
     }
     return res, (grid, space, op, kappa, x)
 
@@ -711,7 +724,7 @@ def main():
                                                   args.warmup, args.spmv_sample)
     line = dict(common, value=res["value"], steps=args.steps, warmup=args.warmup, ms_per_step=res["ms_per_step"],
                 data="synthetic (seeded splitmix64 / Box-Muller fields, structured grid)", config=res["config"],
-                roofline=res["roofline"])
+                roofline=res["roofline"], step_hbm=res["step_hbm"])
     iters = res["config"]["iterations"]
     x_np = x.cpu().numpy()
     kappa_np = kappa.cpu().numpy() if hasattr(kappa, "cpu") else kappa

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <unordered_map>
 
 #include "hgdt_internal.hpp"
@@ -120,15 +121,41 @@ static std::shared_ptr<Grid> make_structured(int geom, const int32_t* cells, con
   return g;
 }
 
+// per-element input arrays an assembly reads: the distinct coefficient fields of the forms and the per-element
+// sources
+static int64_t per_element_inputs(const Operator& op)
+{
+  std::set<const double*> fields;
+  for (const auto& f : op.eforms)
+    if (f.field)
+      fields.insert(f.field);
+  for (const auto* forms : {&op.cforms, &op.bforms})
+    for (const auto& f : *forms) {
+      if (f.kappa)
+        fields.insert(f.kappa);
+      if (f.omega)
+        fields.insert(f.omega);
+    }
+  int64_t k = int64_t(fields.size());
+  for (const auto& s : op.sources)
+    if (s.kind == HGDT_SOURCE_PER_ELEMENT)
+      ++k;
+  return k;
+}
+
 void Operator::assemble(hipStream_t stream)
 {
   prepare_assembly();
   // matrix-free mode: the right-hand side only, the operator is recomputed at every application
   matrix_free_assembly = (requested_format == HGDT_FORMAT_MATRIX_FREE);
+  const int64_t inputs = 8 * space->grid->ne * per_element_inputs(*this);
   if (!matrix_free_assembly && direct_applicable()) {
     if (dirichlet)
       build_dirichlet();
     assemble_direct(stream);
+    // one pass over the DoFs: the per-element inputs and the Dirichlet mask read; the diagonal and 7 upper
+    // diagonals, the right-hand side, D^{-1/2} and D^{1/2} written
+    assembly_bytes = inputs + (88 + (dirichlet ? 1 : 0)) * n;
     return;
   }
   build_scatter_maps();
@@ -138,6 +165,8 @@ void Operator::assemble(hipStream_t stream)
   fmt.valid = false;
   direct_storage = false;
   values_stale = false;
+  // the compulsory bytes: the per-element inputs read, the values and the right-hand side written
+  assembly_bytes = inputs + (matrix_free_assembly ? 0 : 8 * nnz) + 8 * n;
 }
 
 } // namespace hgdt
@@ -1037,6 +1066,14 @@ hgdt_status hgdt_operator_assembly_info(hgdt_operator op, int32_t* structured, i
       *structured = op->op->direct_storage ? 1 : 0;
     if (solver_upper_diagonals)
       *solver_upper_diagonals = op->op->solver_upper_diagonals();
+  });
+}
+
+hgdt_status hgdt_operator_assembly_bytes(hgdt_operator op, int64_t* bytes)
+{
+  return guarded([&] {
+    require(op && bytes, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *bytes = op->op->assembly_bytes;
   });
 }
 

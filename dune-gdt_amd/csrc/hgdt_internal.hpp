@@ -439,6 +439,8 @@ struct Operator {
   // p entries of the solver's symmetric DIA application staged in LDS: -1 automatic, 0 never, 1 wherever
   // the stencil allows it
   int column_staging = -1;
+  // algorithmic HBM bytes of the last assembly (hgdt_operator_assembly_bytes)
+  int64_t assembly_bytes = 0;
   // solver workspace
   DeviceArray<double> d_r, d_p, d_q;
   DeviceArray<double> d_p2; // second p buffer of the matrix-free solver with the fused direction update

@@ -2656,6 +2656,9 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     st.spmv_count = cnt;
   }
   st.step_bytes = fmt.step_bytes;
+  // one iteration: the application, the direction update (36 B per row on average over the two parities;
+  // inside the application for the matrix-free SWIPDG operator) and the update (32 B per row)
+  st.iteration_bytes = fmt.step_bytes + (fused_dir ? 0 : 36 * n) + 32 * n;
   st.compactions = compactions;
   st.column_staged = (kernel_format(*this) == FMT_DIAS && win_lds_bytes(*this) > 0 && C.win.ptr) ? 1 : 0;
   if (!st.converged) {

@@ -121,6 +121,7 @@ class SolverStats(ctypes.Structure):
         ("compactions", ctypes.c_int64),
         ("true_relres", ctypes.c_double),
         ("column_staged", ctypes.c_int32),
+        ("iteration_bytes", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -198,6 +199,7 @@ SIGNATURES = {
     "hgdt_space_error_norms_noalloc": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _dbl, _pdbl, _vp]),
     "hgdt_noalloc_scratch_bytes": (ctypes.c_int, [_pi64]),
     "hgdt_operator_assembly_info": (ctypes.c_int, [_vp, _pi32, _pi32]),
+    "hgdt_operator_assembly_bytes": (ctypes.c_int, [_vp, _pi64]),
     "hgdt_grid_boundary_intersections": (ctypes.c_int, [_vp, _pdbl, _pdbl]),
     "hgdt_operator_set_boundary_filter": (ctypes.c_int, [_vp, _vp, _i64]),
     "hgdt_operator_set_dirichlet_faces": (ctypes.c_int, [_vp, _vp, _i64]),

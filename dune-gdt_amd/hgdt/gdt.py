@@ -960,6 +960,13 @@ class MatrixOperator:
         L.call("hgdt_operator_assembly_info", self._h, ctypes.byref(st), ctypes.byref(su))
         return {"structured": bool(st.value), "solver_upper_diagonals": int(su.value)}
 
+    def assembly_bytes(self):
+        """algorithmic HBM bytes of the last assemble(): the structured assembly's one pass, otherwise the
+        compulsory bytes (per-element inputs read, values and right-hand side written)"""
+        b = ctypes.c_int64()
+        L.call("hgdt_operator_assembly_bytes", self._h, ctypes.byref(b))
+        return b.value
+
     def set_launch(self, kernel, block_size, items_per_thread=1):
         L.call("hgdt_operator_set_launch", self._h, int(kernel), int(block_size), int(items_per_thread))
 

@@ -328,6 +328,13 @@ hgdt_status hgdt_operator_copy_rhs(hgdt_operator op, double* host_out);
  * before the first apply / apply_inverse after an assembly.  Either pointer may be NULL. */
 hgdt_status hgdt_operator_assembly_info(hgdt_operator op, int32_t* structured, int32_t* solver_upper_diagonals);
 
+/* Algorithmic HBM bytes of the last assemble() (no reference counterpart; a measurement query like the one
+ * above): for the structured assembly its one pass (the per-element coefficient fields and sources and the
+ * Dirichlet mask read; 8 diagonals, the right-hand side, D^{-1/2} and D^{1/2} written: 88 B per DoF);
+ * otherwise the compulsory bytes, 8 per element and per-element input, 8 per nonzero (none in matrix-free
+ * mode) and 8 per DoF (right-hand side).  0 before the first assembly. */
+hgdt_status hgdt_operator_assembly_bytes(hgdt_operator op, int64_t* bytes);
+
 /* choose the storage used by apply / apply_inverse (default HGDT_FORMAT_AUTO: element blocks for a
  * discontinuous space with 4 DoFs per element, else DIA when every instance has at most 32 distinct
  * diagonals, else SELL-64).  DIA reads only the diagonals with
@@ -391,6 +398,10 @@ typedef struct hgdt_solver_stats {
                              (-1 when options.true_residual is 0) */
   int32_t column_staged;  /* 1: the solver's operator application ran with column staging (see
                              hgdt_operator_set_column_staging) */
+  int64_t iteration_bytes; /* algorithmic bytes of one PCG iteration over all instances: the application
+                              (step_bytes), the direction update (36 B per DoF on average over the two
+                              parities of the deferred iterate update, none when the application does it)
+                              and the update (32 B per DoF) */
 } hgdt_solver_stats;
 
 hgdt_status hgdt_solver_options_default(hgdt_solver_options* options);

@@ -149,3 +149,43 @@ def test_solver_copy_keeps_nonzero_diagonals(hgdt_mod):
     xd = op.apply_inverse("rhs").cpu().numpy()
     assert op.format_info()["format"] == "dia" and op.last_solve["iterations"] == it
     assert np.array_equal(x, xd)
+
+
+def test_reported_algorithmic_bytes(hgdt_mod):
+    """the algorithmic bytes the library reports for an assembly and a PCG iteration: the structured pass
+    reads the coefficient field (8 B per element) and the Dirichlet mask (1 B per DoF) and writes 88 B per
+    DoF; the generic path's compulsory bytes are 8 per element and field, 8 per nonzero and 8 per DoF; an
+    iteration moves the application's bytes plus 36 B per DoF (direction update) plus 32 B per DoF (update),
+    and the matrix-free SWIPDG application, which does the direction update itself, plus 32 B per DoF only"""
+    H = hgdt_mod
+    print("case reported algorithmic bytes", flush=True)
+    cells = [10, 9, 8]
+    grid = H.Grid.structured("tet", [0, 0, 0], [1, 1, 1], cells)
+    kappa = torch.from_numpy(H.fields.lognormal(3, grid.num_elements)).cuda()
+    space, op = _operator(H, grid, [("laplace", kappa, 1.0)], [1.0])
+    op.assemble()
+    n, ne = space.num_DoFs, grid.num_elements
+    assert op.assembly_info()["structured"]
+    assert op.assembly_bytes() == 8 * ne + 89 * n
+    op.apply_inverse("rhs")
+    st = op.last_solve
+    assert st["step_bytes"] == (3 * 8 + 16) * n
+    assert st["iteration_bytes"] == st["step_bytes"] + 68 * n
+    # the generic element walk (non-symmetric storage requested)
+    space2, op2 = _operator(H, grid, [("laplace", kappa, 1.0)], [1.0])
+    op2.set_format(H._lib.FORMAT_DIA_NONSYMMETRIC)
+    op2.assemble()
+    assert not op2.assembly_info()["structured"]
+    assert op2.assembly_bytes() == 8 * ne + 8 * op2.nnz + 8 * n
+    # matrix-free SWIPDG on quadrilaterals
+    from problems import hgdt_problem
+
+    cells2 = [12, 10]
+    k2 = torch.from_numpy(H.fields.lognormal(5, cells2[0] * cells2[1])).cuda()
+    _, s3, op3 = hgdt_problem(H, "swipdg", cells2, k2)
+    op3.set_format("matrix_free")
+    op3.assemble()
+    assert op3.assembly_bytes() == 8 * cells2[0] * cells2[1] + 8 * s3.num_DoFs
+    op3.apply_inverse("rhs")
+    st3 = op3.last_solve
+    assert st3["iteration_bytes"] == st3["step_bytes"] + 32 * s3.num_DoFs
