@@ -881,6 +881,28 @@ hgdt_status hgdt_operator_append_product(hgdt_operator op, const double* field, 
   });
 }
 
+static void append_element_points(hgdt_operator op, int kind, const double* values, int32_t coeff_order,
+                                  int32_t over_integrate, double scaling)
+{
+  require(op && values && coeff_order >= 0 && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+  ElementFormDesc F{kind, nullptr, 0., over_integrate, scaling};
+  F.points = values, F.coeff_order = coeff_order;
+  op->op->eforms.push_back(F);
+  forms_changed(*op->op);
+}
+
+hgdt_status hgdt_operator_append_laplace_points(hgdt_operator op, const double* values, int32_t coeff_order,
+                                                int32_t over_integrate, double scaling)
+{
+  return guarded([&] { append_element_points(op, EF_LAPLACE, values, coeff_order, over_integrate, scaling); });
+}
+
+hgdt_status hgdt_operator_append_product_points(hgdt_operator op, const double* values, int32_t coeff_order,
+                                                int32_t over_integrate, double scaling)
+{
+  return guarded([&] { append_element_points(op, EF_PRODUCT, values, coeff_order, over_integrate, scaling); });
+}
+
 static void append_face(hgdt_operator op, int kind, double param, const double* kappa, double kv, const double* omega,
                         double ov, int over_integrate, double scaling)
 {

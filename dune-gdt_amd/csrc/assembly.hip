@@ -46,6 +46,10 @@ struct ElementParams {
   // (NULL: element e, point q at e * npts + q)
   const int32_t* eid;
   const int64_t* src_qoff[MAX_SOURCES];
+  // element forms whose coefficient is given at the points of their rule (NULL: field or value), and on mixed
+  // grids where each element's points start
+  const double* points[MAX_EFORMS];
+  const int64_t* ef_qoff[MAX_EFORMS];
   const double* qtab; // points (dim each) then weights, per rule
   int nforms;
   int kind[MAX_EFORMS];
@@ -370,7 +374,10 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
     for (int f = 0; f < P.nforms; ++f) {
       const double c = P.field[f] ? P.field[f][ge] : P.value[f];
       const double s = P.scaling[f];
-      if (P.kind[f] == EF_LAPLACE && T::SIMPLEX && ORDER == 1) {
+      const double* __restrict__ cp = P.points[f]; // the coefficient at the rule's points of the element
+      if (cp)
+        cp += P.ef_qoff[f] ? P.ef_qoff[f][e] : e * P.rule[f].npts;
+      if (P.kind[f] == EF_LAPLACE && T::SIMPLEX && ORDER == 1 && !cp) {
         // P1 gradients are constant: the integrand is constant on the element and any
         // rule of the order chosen by integrals.hh:115 reduces to one evaluation
         double phi[ND], dphi[ND][D], G[ND][D];
@@ -405,6 +412,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
         double phi[ND], dphi[ND][D];
         basis<GEOM, ORDER>(xh, phi, dphi);
         const double factor = g.det * w;
+        const double cq = cp ? cp[q] : c;
         if (P.kind[f] == EF_LAPLACE) {
           double G[ND][D];
 #pragma unroll
@@ -417,7 +425,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
               double dotv = 0.;
 #pragma unroll
               for (int a = 0; a < D; ++a)
-                dotv += (c * G[j][a]) * G[i][a];
+                dotv += (cq * G[j][a]) * G[i][a];
               A[i][j] += s * (dotv * factor);
             }
         } else {
@@ -425,7 +433,7 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
           for (int i = 0; i < ND; ++i)
 #pragma unroll
             for (int j = i; j < ND; ++j)
-              A[i][j] += s * (((c * phi[i]) * phi[j]) * factor);
+              A[i][j] += s * (((cq * phi[i]) * phi[j]) * factor);
         }
       }
     }
@@ -517,6 +525,9 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
       const double c = P.field[f] ? P.field[f][ge] : P.value[f];
       const double s = P.scaling[f];
       const QRule R = P.rule[f];
+      const double* __restrict__ cp = P.points[f]; // the coefficient at the rule's points of the element
+      if (cp)
+        cp += P.ef_qoff[f] ? P.ef_qoff[f][e] : e * R.npts;
       for (int q = 0; q < R.npts; ++q) {
         double xh[D];
 #pragma unroll
@@ -527,6 +538,7 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
         factors(xh, l, dl);
         const double lx = pick3(ix, l[0]), ly = pick3(iy, l[1]), lz = pick3(iz, l[2]);
         const double factor = g.det * w;
+        const double cq = cp ? cp[q] : c;
         if (P.kind[f] == EF_LAPLACE) {
           const double gh[3] = {(pick3(ix, dl[0]) * ly) * lz, (pick3(iy, dl[1]) * lx) * lz,
                                 (pick3(iz, dl[2]) * lx) * ly};
@@ -542,7 +554,7 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
             double dotv = 0.;
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-              dotv += (c * Gj[a]) * Gi[a];
+              dotv += (cq * Gj[a]) * Gi[a];
             row[j] += s * (dotv * factor);
           }
         } else {
@@ -551,7 +563,7 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
           for (int j = 0; j < ND; ++j) {
             const int jx = j % 3, jy = (j / 3) % 3, jz = j / 9;
             const double phij = (l[0][jx] * l[1][jy]) * l[2][jz];
-            row[j] += s * (((c * phii) * phij) * factor);
+            row[j] += s * (((cq * phii) * phij) * factor);
           }
         }
       }
@@ -1727,8 +1739,10 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, std::ve
     for (int f = 0; f < P.nforms; ++f) {
       const auto& F = op.eforms[f];
       P.kind[f] = F.kind, P.field[f] = F.field, P.value[f] = F.value, P.scaling[f] = F.scaling;
-      // integrand order: 0 (piecewise constant coefficient) + test + ansatz (laplace.hh:74-79, product.hh:88-100)
-      P.rule[f] = etab.add(element_rule(r, 0 + k + k + F.over_integrate));
+      P.points[f] = F.points;
+      // integrand order: coefficient (0 when piecewise constant) + test + ansatz (laplace.hh:74-79,
+      // product.hh:88-100)
+      P.rule[f] = etab.add(element_rule(r, F.coeff_order + k + k + F.over_integrate));
     }
     P.nsrc = int(op.sources.size());
     for (int s = 0; s < P.nsrc; ++s) {
@@ -1797,6 +1811,22 @@ void Operator::prepare_assembly()
         part_qoff.back()->upload(po);
       }
     }
+  part_eqoff.clear();
+  if (g.mixed)
+    for (const auto& F : eforms) {
+      const std::vector<int64_t> off =
+          F.points ? element_point_offsets(g, F.coeff_order + 2 * space->order + F.over_integrate)
+                   : std::vector<int64_t>{};
+      for (const GridPart& P : g.parts) {
+        part_eqoff.push_back(std::make_unique<DeviceArray<int64_t>>());
+        if (off.empty())
+          continue;
+        std::vector<int64_t> po(P.eid.size());
+        for (size_t el = 0; el < po.size(); ++el)
+          po[el] = off[P.eid[el]];
+        part_eqoff.back()->upload(po);
+      }
+    }
   tables_ready = true;
 }
 
@@ -1824,6 +1854,8 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
       P.eid = g.mixed ? g.parts[part].d_eid.ptr : nullptr;
       for (int s = 0; s < P.nsrc && g.mixed; ++s)
         P.src_qoff[s] = op.part_qoff[s * g.parts.size() + part]->ptr;
+      for (int f = 0; f < P.nforms && g.mixed; ++f)
+        P.ef_qoff[f] = op.part_eqoff[f * g.parts.size() + part]->ptr;
       P.qtab = op.d_qtab_elem.ptr;
       P.staging = op.d_staging.ptr;
       P.elem_base = M.part_elem_base[part];
@@ -1933,6 +1965,9 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "form_index does not name an appended element form");
   if (F->kind != EF_LAPLACE)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "the adjoint gradient is defined for a Laplace form");
+  if (F->points)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED,
+               "the adjoint gradient is taken with respect to a per-element coefficient, not one given at points");
   if (space->order != 1)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel is provided for order 1 spaces");
   if (g.mixed)

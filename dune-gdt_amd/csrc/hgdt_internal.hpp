@@ -226,6 +226,10 @@ struct ElementFormDesc {
   double value;
   int over_integrate;
   double scaling;
+  // the coefficient at the points of the element rule of order coeff_order + 2 k + over_integrate (k the
+  // space order), element-ordered (NULL: field or value)
+  const double* points = nullptr;
+  int coeff_order = 0;
 };
 struct FaceFormDesc {
   int kind;
@@ -431,6 +435,8 @@ struct Operator {
   DeviceArray<double> d_qtab_elem, d_qtab_line; // quadrature tables of the appended forms
   // mixed grids: per (part, source given at quadrature points) where every element's points start
   std::vector<std::unique_ptr<DeviceArray<int64_t>>> part_qoff;
+  // the same per (part, element form with its coefficient at quadrature points)
+  std::vector<std::unique_ptr<DeviceArray<int64_t>>> part_eqoff;
   bool tables_ready = false;
   int requested_format = HGDT_FORMAT_AUTO;
   bool allow_symmetric = true;

@@ -2030,7 +2030,7 @@ bool Operator::direct_applicable()
       !bsources.empty() || sources.size() > size_t(DIRECT_MAX_SOURCES))
     return false;
   for (const auto& f : eforms)
-    if (f.kind != EF_LAPLACE)
+    if (f.kind != EF_LAPLACE || f.points)
       return false;
   for (const auto& src : sources)
     if (src.kind != HGDT_SOURCE_CONSTANT && src.kind != HGDT_SOURCE_PER_ELEMENT)

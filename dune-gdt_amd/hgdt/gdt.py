@@ -421,7 +421,16 @@ class GenericFunction:
 
 
 def _gf(v):
+    if isinstance(v, (GenericFunction, QuadraturePointFunction)):
+        raise L.not_supported(9, "this integrand takes constant or per-element coefficients (element integrands "
+                                 "also take coefficients varying inside the elements)")
     return v if isinstance(v, GridFunction) else GridFunction(v)
+
+
+def _element_coefficient(v):
+    """coefficient of an element integrand: a GridFunction (constant or per element), or one that varies inside
+    the elements (GenericFunction, QuadraturePointFunction), evaluated at the quadrature points"""
+    return v if isinstance(v, (GenericFunction, QuadraturePointFunction)) else _gf(v)
 
 
 # ---------------------------------------------------------------------------
@@ -446,14 +455,14 @@ class LocalLaplaceIntegrand(_Integrand):
     """(diffusion grad phi_j) . grad psi_i, local/integrands/laplace.hh:27-109"""
 
     def __init__(self, diffusion=1.0, logging_prefix=""):
-        self.diffusion = _gf(diffusion)
+        self.diffusion = _element_coefficient(diffusion)
 
 
 class LocalElementProductIntegrand(_Integrand):
     """(weight psi_i) phi_j, local/integrands/product.hh:43-137"""
 
     def __init__(self, weight=1.0, logging_prefix=""):
-        self.weight = _gf(weight)
+        self.weight = _element_coefficient(weight)
 
     def with_ansatz(self, function):
         """LocalBinaryToUnaryElementIntegrand (local/integrands/conversion.hh:42-124)"""
@@ -465,7 +474,8 @@ LocalProductIntegrand = LocalElementProductIntegrand
 
 class _ProductWithAnsatz(_Integrand):
     def __init__(self, product, function):
-        if product.weight.field is not None or product.weight.constant != 1.0:
+        if (not isinstance(product.weight, GridFunction) or product.weight.field is not None
+                or product.weight.constant != 1.0):
             raise L.not_supported(9, "right-hand side functionals use the unit product weight")
         self.function = function
 
@@ -727,6 +737,8 @@ class MatrixOperator:
                None if mask is None else ctypes.c_void_p(mask.ctypes.data), 0 if mask is None else len(mask))
 
     def _field(self, gf):
+        if not isinstance(gf, GridFunction):
+            raise L.not_supported(9, "intersection integrands take constant or per-element coefficients")
         ptr, val = gf.args(self.grid)
         if gf.field is not None:
             self._keep.append(gf.field)
@@ -734,13 +746,34 @@ class MatrixOperator:
 
     def _append_element(self, integrand, over, scaling):
         if isinstance(integrand, LocalLaplaceIntegrand):
-            p, v = self._field(integrand.diffusion)
-            L.call("hgdt_operator_append_laplace", self._h, p, v, over, scaling)
+            coeff, kind = integrand.diffusion, "laplace"
         elif isinstance(integrand, LocalElementProductIntegrand):
-            p, v = self._field(integrand.weight)
-            L.call("hgdt_operator_append_product", self._h, p, v, over, scaling)
+            coeff, kind = integrand.weight, "product"
         else:
             raise L.not_supported(9, f"{type(integrand).__name__} is not an element integrand of this path")
+        if isinstance(coeff, GridFunction):
+            p, v = self._field(coeff)
+            L.call(f"hgdt_operator_append_{kind}", self._h, p, v, over, scaling)
+        else:
+            # varying inside the elements: values at the points of the rule of order c + 2 k + over_integrate
+            # (laplace.hh:74-79, product.hh:88-100), c the coefficient's order
+            k = self.space.order
+            if isinstance(coeff, GenericFunction):
+                c = coeff.order()
+                vals = _dvec(coeff.values_at(self.grid, c + 2 * k + over), name="coefficient values")
+            else:
+                c = coeff.order_total - 2 * k - over
+                if c < 0:
+                    raise L.wrong_input_given(1, f"values at the points of order {coeff.order_total}: the rule of "
+                                                 f"this form has order >= {2 * k + over}")
+                vals = coeff.values
+            need = int(self.grid.quadrature_offsets(c + 2 * k + over)[-1])
+            if vals.numel() != need:
+                raise L.shapes_do_not_match(2, f"{vals.numel()} coefficient values, the rule of order "
+                                               f"{c + 2 * k + over} has {need} points on this grid")
+            self._keep.append(vals)
+            L.call(f"hgdt_operator_append_{kind}_points", self._h, ctypes.c_void_p(vals.data_ptr()), c, over,
+                   scaling)
         self._eforms += 1
 
     def _append_coupling(self, integrand, over, scaling):
@@ -1078,6 +1111,9 @@ class BilinearForm:
                 if isinstance(it, LocalLaplaceIntegrand):
                     gg = _dvec(g.gradients_at(self.grid, order), name="function gradients")
             out = np.empty(2 * ninst)
+            coeff = it.diffusion if isinstance(it, LocalLaplaceIntegrand) else it.weight
+            if not isinstance(coeff, GridFunction):
+                raise L.not_supported(9, "error norms take constant or per-element coefficients")
             if isinstance(it, LocalLaplaceIntegrand):
                 kp, kv = it.diffusion.args(self.grid)
                 comp, factor = 1, 1.0

@@ -246,6 +246,16 @@ hgdt_status hgdt_operator_append_laplace(hgdt_operator op, const double* diffusi
  * local/integrands/product.hh:104-130 */
 hgdt_status hgdt_operator_append_product(hgdt_operator op, const double* weight_field, double weight_value,
                                          int32_t over_integrate, double scaling);
+/* the same two forms with a coefficient that varies inside the elements, a GridFunction of polynomial order
+ * coeff_order (LocalLaplaceIntegrand(diffusion) / LocalElementProductIntegrand(weight) with
+ * local_weight_->order() = coeff_order: laplace.hh:74-79, product.hh:88-100): `values` holds its values at
+ * the points of hgdt_grid_quadrature_points(order), order = coeff_order + 2 k + over_integrate (k the space
+ * order), element-ordered (hgdt_grid_quadrature_offsets on mixed grids); a device array that must stay
+ * valid.  Not for the matrix-free format or the adjoint gradient (their coefficients are per element). */
+hgdt_status hgdt_operator_append_laplace_points(hgdt_operator op, const double* values, int32_t coeff_order,
+                                                int32_t over_integrate, double scaling);
+hgdt_status hgdt_operator_append_product_points(hgdt_operator op, const double* values, int32_t coeff_order,
+                                                int32_t over_integrate, double scaling);
 /* append(LocalCouplingIntersectionIntegralBilinearForm(LocalLaplaceIPDGIntegrands::InnerCoupling(
  *   symmetry_prefactor, diffusion, weight)), {}, ApplyOn::InnerIntersectionsOnce),
  * matrix-based.hh:371-393, local/integrands/laplace-ipdg.hh:47-186 */

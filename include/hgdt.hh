@@ -508,7 +508,12 @@ struct LocalLaplaceIntegrand
   explicit LocalLaplaceIntegrand(GridFunction diffusion_ = 1.)
     : diffusion(std::move(diffusion_))
   {}
+  // a diffusion varying inside the elements, evaluated at the quadrature points (rule order: its order + 2 k)
+  explicit LocalLaplaceIntegrand(const GenericFunction& diffusion_)
+    : generic(std::make_shared<GenericFunction>(diffusion_))
+  {}
   GridFunction diffusion;
+  std::shared_ptr<GenericFunction> generic;
 };
 
 // LocalBinaryToUnaryElementIntegrand(LocalElementProductIntegrand(1), f): the right-hand side integrand
@@ -528,6 +533,10 @@ struct LocalElementProductIntegrand
   explicit LocalElementProductIntegrand(GridFunction weight_ = 1.)
     : weight(std::move(weight_))
   {}
+  // a weight varying inside the elements, evaluated at the quadrature points (rule order: its order + 2 k)
+  explicit LocalElementProductIntegrand(const GenericFunction& weight_)
+    : generic(std::make_shared<GenericFunction>(weight_))
+  {}
   LocalProductWithAnsatz with_ansatz(const GridFunction& f) const
   {
     require_unit_weight();
@@ -544,11 +553,12 @@ struct LocalElementProductIntegrand
                                   std::make_shared<GenericFunction>(f)};
   }
   GridFunction weight;
+  std::shared_ptr<GenericFunction> generic;
 
 private:
   void require_unit_weight() const
   {
-    if (weight.field() || weight.value() != 1.)
+    if (generic || weight.field() || weight.value() != 1.)
       throw Exceptions::not_supported(HGDT_ERR_NOT_SUPPORTED, "right-hand side functionals use the unit weight");
   }
 };
@@ -673,15 +683,18 @@ struct LocalElementIntegralBilinearForm
   LocalElementIntegralBilinearForm(const LocalLaplaceIntegrand& integrand, int over_integrate_ = 0)
     : laplace(true)
     , coefficient(integrand.diffusion)
+    , generic(integrand.generic)
     , over_integrate(over_integrate_)
   {}
   LocalElementIntegralBilinearForm(const LocalElementProductIntegrand& integrand, int over_integrate_ = 0)
     : laplace(false)
     , coefficient(integrand.weight)
+    , generic(integrand.generic)
     , over_integrate(over_integrate_)
   {}
   bool laplace;
   GridFunction coefficient;
+  std::shared_ptr<GenericFunction> generic; // a coefficient varying inside the elements
   int over_integrate;
 };
 struct LocalCouplingIntersectionIntegralBilinearForm
@@ -1074,6 +1087,19 @@ public:
   {
     keep(form.coefficient);
     const double scaling = param.get("matrixoperator.scaling", 1.);
+    if (form.generic) {
+      // values at the points of the rule of order coefficient order + 2 k + over_integrate
+      const int order = form.generic->order() + 2 * space_.order() + form.over_integrate;
+      auto vals = std::make_shared<DeviceVector>(form.generic->values_at(space_.grid_view(), order));
+      kept_.push_back(vals);
+      if (form.laplace)
+        check(hgdt_operator_append_laplace_points(handle(), vals->data(), form.generic->order(), form.over_integrate,
+                                                  scaling));
+      else
+        check(hgdt_operator_append_product_points(handle(), vals->data(), form.generic->order(), form.over_integrate,
+                                                  scaling));
+      return *this;
+    }
     if (form.laplace)
       check(hgdt_operator_append_laplace(handle(), form.coefficient.field(), form.coefficient.value(),
                                          form.over_integrate, scaling));

@@ -717,6 +717,12 @@ struct ElementForm {
   Coefficient coeff;
   int over_integrate;
   double scaling;
+  // a coefficient varying inside the elements (an XT GridFunction of order coeff_order): its values at the
+  // points of each element's rule of order coeff_order + 2 k + over_integrate, element-ordered, element e's
+  // from points_off[e]; empty: coeff
+  std::vector<double> points;
+  std::vector<int64_t> points_off;
+  int coeff_order = 0;
 };
 struct FaceForm {
   int kind;
@@ -886,14 +892,16 @@ struct Operator {
     result.assign(ls * ls, 0.);
     const int basis_order = space->fe.order;
     // LocalLaplaceIntegrand::order (local/integrands/laplace.hh:74-79) and
-    // LocalElementProductIntegrand::order (local/integrands/product.hh:88-100): 0 + test + ansatz
-    const int order = 0 + basis_order + basis_order + F.over_integrate;
+    // LocalElementProductIntegrand::order (local/integrands/product.hh:88-100): weight order (0 for a
+    // piecewise constant coefficient) + test + ansatz
+    const int order = F.coeff_order + basis_order + basis_order + F.over_integrate;
     const Quadrature q = element_rule(space->mesh->eref(e), order);
-    const double c = F.coeff.at(e);
     std::vector<double> iv(ls * ls);
     for (size_t qp = 0; qp < q.w.size(); ++qp) {
       const double* xh = q.x[qp].data();
       const double factor = geo.integration_element * q.w[qp];
+      // local_weight_->evaluate(x) (laplace.hh:92, product.hh:118)
+      const double c = F.points.empty() ? F.coeff.at(e) : F.points[size_t(F.points_off[e]) + qp];
       if (F.kind == EF_LAPLACE) {
         // local/integrands/laplace.hh:94-101
         basis.jacobians(xh);
@@ -1591,7 +1599,7 @@ int or_operator_append_laplace(OrOperator* op, const double* kappa, double kappa
                                double scaling)
 {
   return guarded([&] {
-    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling};
+    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling, {}, {}, 0};
     F.coeff.set(kappa, kappa_const, op->space->mesh->ne);
     op->eforms.push_back(F);
   });
@@ -1600,10 +1608,41 @@ int or_operator_append_product(OrOperator* op, const double* weight, double weig
                                double scaling)
 {
   return guarded([&] {
-    ElementForm F{EF_PRODUCT, {}, over_integrate, scaling};
+    ElementForm F{EF_PRODUCT, {}, over_integrate, scaling, {}, {}, 0};
     F.coeff.set(weight, weight_const, op->space->mesh->ne);
     op->eforms.push_back(F);
   });
+}
+// element forms whose coefficient varies inside the elements: values at the points of the rules of order
+// coeff_order + 2 k + over_integrate (count: their total, checked)
+static int append_element_points(OrOperator* op, int kind, const double* values, int64_t count, int coeff_order,
+                                 int over_integrate, double scaling)
+{
+  return guarded([&] {
+    const Mesh& m = *op->space->mesh;
+    if (!values || coeff_order < 0 || over_integrate < 0)
+      throw Error("invalid coefficient values");
+    ElementForm F{kind, {}, over_integrate, scaling, {}, {}, 0};
+    F.coeff_order = coeff_order;
+    const int order = coeff_order + 2 * op->space->fe.order + over_integrate;
+    F.points_off.assign(size_t(m.ne) + 1, 0);
+    for (int64_t e = 0; e < m.ne; ++e)
+      F.points_off[e + 1] = F.points_off[e] + int64_t(element_rule(m.eref(e), order).w.size());
+    if (F.points_off[m.ne] != count)
+      throw Error("coefficient values: one per point of the element rules is expected");
+    F.points.assign(values, values + count);
+    op->eforms.push_back(F);
+  });
+}
+int or_operator_append_laplace_points(OrOperator* op, const double* values, int64_t count, int coeff_order,
+                                      int over_integrate, double scaling)
+{
+  return append_element_points(op, EF_LAPLACE, values, count, coeff_order, over_integrate, scaling);
+}
+int or_operator_append_product_points(OrOperator* op, const double* values, int64_t count, int coeff_order,
+                                      int over_integrate, double scaling)
+{
+  return append_element_points(op, EF_PRODUCT, values, count, coeff_order, over_integrate, scaling);
 }
 static int append_face(OrOperator* op, int kind, double param, const double* kappa, double kc, const double* omega,
                        double oc, int over_integrate, double scaling)
@@ -2079,7 +2118,7 @@ int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const d
     op.space = const_cast<OrSpace*>(s);
     const Mesh& m = *s->mesh;
     const int ls = s->fe.size;
-    ElementForm F{EF_LAPLACE, {}, 0, 1.};
+    ElementForm F{EF_LAPLACE, {}, 0, 1., {}, {}, 0};
     F.coeff.constant = 1.;
 #pragma omp parallel
     {

@@ -67,6 +67,12 @@ int or_operator_append_laplace(OrOperator* op, const double* kappa, double kappa
                                int over_integrate, double scaling);
 int or_operator_append_product(OrOperator* op, const double* weight, double weight_const,
                                int over_integrate, double scaling);
+/* the same forms with a coefficient varying inside the elements: `count` values at the points of the element
+ * rules of order coeff_order + 2 k + over_integrate (k the space order), element by element */
+int or_operator_append_laplace_points(OrOperator* op, const double* values, int64_t count, int coeff_order,
+                                      int over_integrate, double scaling);
+int or_operator_append_product_points(OrOperator* op, const double* values, int64_t count, int coeff_order,
+                                      int over_integrate, double scaling);
 int or_operator_append_inner_coupling(OrOperator* op, double symmetry_prefactor, const double* kappa,
                                       double kappa_const, const double* omega, double omega_const,
                                       int over_integrate, double scaling);

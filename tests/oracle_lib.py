@@ -102,6 +102,10 @@ def lib():
         "or_adjoint_log_kappa_gradient": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
         "or_error_norms": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]),
         "or_mesh_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "or_operator_append_laplace_points": (
+            ctypes.c_int, [_vp, _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+        "or_operator_append_product_points": (
+            ctypes.c_int, [_vp, _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
         "or_operator_append_boundary_source": (
             ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, _dp, ctypes.c_double,
                            ctypes.c_int, ctypes.c_int]),
@@ -285,6 +289,20 @@ class Operator:
     def append_product(self, weight=1.0, over_integrate=0, scaling=1.0):
         p, c = self._coef(weight)
         self._check(lib().or_operator_append_product(self.h, p, c, over_integrate, scaling))
+
+    def append_laplace_points(self, values, coeff_order, over_integrate=0, scaling=1.0):
+        """Laplace form with the diffusion at the points of the element rules of order
+        coeff_order + 2 k + over_integrate (Mesh.quadrature_points layout)"""
+        v = np.ascontiguousarray(values, np.float64).reshape(-1)
+        self._keep = getattr(self, "_keep", []) + [v]
+        self._check(lib().or_operator_append_laplace_points(self.h, dptr(v), v.size, coeff_order, over_integrate,
+                                                             scaling))
+
+    def append_product_points(self, values, coeff_order, over_integrate=0, scaling=1.0):
+        v = np.ascontiguousarray(values, np.float64).reshape(-1)
+        self._keep = getattr(self, "_keep", []) + [v]
+        self._check(lib().or_operator_append_product_points(self.h, dptr(v), v.size, coeff_order, over_integrate,
+                                                             scaling))
 
     def append_inner_coupling(self, symmetry_prefactor, kappa=1.0, omega=1.0, over_integrate=0, scaling=1.0):
         kp, kc = self._coef(kappa)
