@@ -1411,7 +1411,7 @@ __global__ void __launch_bounds__(256) k_gather_rhs(int64_t n, const uint32_t* _
 }
 
 // adjoint element kernel: g_e = -kappa_e lambda_e^T Khat_e u_e with Khat_e recomputed element-locally
-template <int GEOM>
+template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const double* __restrict__ coords,
                                                           const int32_t* __restrict__ conn,
                                                           const int32_t* __restrict__ dofs,
@@ -1422,7 +1422,7 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
                                                           double* __restrict__ grad)
 {
   using T = Traits<GEOM>;
-  constexpr int D = T::D, ND = T::ND;
+  constexpr int D = T::D, ND = Basis<GEOM, ORDER>::ND;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= ne)
     return;
@@ -1435,7 +1435,7 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
     ue[i] = u[d], le[i] = lambda[d];
   }
   double acc = 0.;
-  if constexpr (T::SIMPLEX) {
+  if constexpr (T::SIMPLEX && ORDER == 1) {
     double phi[ND], dphi[ND][D], G[ND][D], xc[D];
 #pragma unroll
     for (int a = 0; a < D; ++a)
@@ -1464,7 +1464,7 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
         xh[a] = qtab[R.offset * D + q * D + a];
       const double w = qtab[R.offset * D + R.npts * D + q];
       double phi[ND], dphi[ND][D];
-      shape<GEOM>(xh, phi, dphi);
+      basis<GEOM, ORDER>(xh, phi, dphi);
       double gu[D], gl[D];
 #pragma unroll
       for (int a = 0; a < D; ++a)
@@ -1968,8 +1968,6 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
   if (F->points)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED,
                "the adjoint gradient is taken with respect to a per-element coefficient, not one given at points");
-  if (space->order != 1)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel is provided for order 1 spaces");
   if (g.mixed)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel needs a grid of one element type");
   prepare_assembly();
@@ -1979,10 +1977,10 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
   build_tables(*this, etab, ltab, PE, C, B);
   const QRule R = PE[0].rule[form_index];
   const int block = launch[0].block;
-  dispatch_geom(g.ref.geom, [&](auto G) {
-    hipLaunchKernelGGL(k_adjoint_gradient<decltype(G)::value>, dim3(grid_for(g.ne, block)), dim3(block), 0, stream,
-                       g.ne, g.d_coords.ptr, g.d_conn.ptr, space->d_dofs.ptr, d_qtab_elem.ptr, R, F->field, F->value,
-                       u, lambda, grad);
+  dispatch_fe(g.ref.geom, space->order, [&](auto G, auto O) {
+    hipLaunchKernelGGL((k_adjoint_gradient<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, block)),
+                       dim3(block), 0, stream, g.ne, g.d_coords.ptr, g.d_conn.ptr, space->d_dofs.ptr, d_qtab_elem.ptr,
+                       R, F->field, F->value, u, lambda, grad);
   });
   HGDT_HIP(hipGetLastError());
 }

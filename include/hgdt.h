@@ -158,7 +158,7 @@ hgdt_status hgdt_grid_boundary_intersections(hgdt_grid grid, double* centers, do
  * hexahedra; the local DoF order of the Lagrange nodes: cubes by tensor index, simplices vertices then edge
  * midpoints (a, b), a < b, lexicographic; continuous order 2 numbers one DoF per subentity in order of first
  * appearance in the element walk).  Order-2 spaces take element forms, right-hand sides, intersection
- * integrals and boundary functionals; the adjoint kernel needs order 1. */
+ * integrals, boundary functionals and the adjoint kernel. */
 hgdt_status hgdt_space_create(hgdt_grid grid, int kind, int order, hgdt_space* space);
 hgdt_status hgdt_space_destroy(hgdt_space space);
 /* mapper().size() and max_local_size(), spaces/mapper/continuous.hh:139-150 */

@@ -174,14 +174,14 @@ def test_invalid_input_raises(hgdt_mod):
     # order 3 spaces are not provided
     with pytest.raises(H.not_supported):
         H.ContinuousLagrangeSpace(grid, 3)
-    # the adjoint kernel on order-2 spaces
+    # the adjoint kernel for a form that is not a Laplace form
     dg2 = H.ContinuousLagrangeSpace(grid, 2)
     op2 = H.make_matrix_operator(dg2, H.Stencil.element)
-    op2.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.ones(16, dtype=torch.float64,
-                                                                                      device="cuda"))))
+    op2.append(H.LocalElementIntegralBilinearForm(H.LocalElementProductIntegrand(torch.ones(16, dtype=torch.float64,
+                                                                                             device="cuda"))))
     op2.assemble()
     u2 = torch.zeros(dg2.num_DoFs, dtype=torch.float64, device="cuda")
-    with pytest.raises(H.not_supported):
+    with pytest.raises(H.wrong_input_given):
         op2.adjoint_log_diffusion_gradient(u2, u2)
     # an operator between spaces of different orders
     with pytest.raises(H.not_supported):

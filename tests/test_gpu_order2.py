@@ -165,3 +165,33 @@ def test_q2_hexahedron_element_matrices(hgdt_mod):
         err = np.abs(A - E).max() / np.abs(E).max()
         print(f"  max rel err {err:.2e}")
         assert A.shape == (27, 27) and err <= 1e-14
+
+
+@pytest.mark.parametrize("geometry,geom,cells", [("quad", O.QUAD, [5, 4]), ("tri", O.TRI, [4, 4]),
+                                                 ("hex", O.HEX, [2, 2, 2]), ("tet", O.TET, [2, 2, 2])])
+def test_adjoint_gradient_order2_matches_oracle(hgdt_mod, geometry, geom, cells):
+    """the adjoint element kernel on order-2 spaces (P2, Q2; Q2 hexahedra with 27 local DoFs): for the solve u
+    and the adjoint lambda (A lambda = u) of a lognormal diffusion problem, -kappa_e lambda_e^T Khat_e u_e
+    within 1e-12 (max-relative) of the oracle's, whose gradient is pinned by central differences"""
+    H = hgdt_mod
+    print(f"case adjoint gradient order 2 on {geometry} cells={cells}", flush=True)
+    d = len(cells)
+    grid = H.Grid.structured(geometry, [0] * d, [1] * d, cells)
+    space = H.ContinuousLagrangeSpace(grid, 2)
+    kap = np.exp(H.fields.normal(41, grid.num_elements))
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kap).cuda())))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.assemble()
+    u = op.apply_inverse("rhs", opts={"precision": 1e-13})
+    lam = op.apply_inverse(u.clone(), opts={"precision": 1e-13})
+    g = op.adjoint_log_diffusion_gradient(u, lam).cpu().numpy()
+    m = O.Mesh.structured(geom, cells, [0] * d, [1] * d)
+    s = O.Space(m, True, 2)
+    go = O.adjoint_log_kappa_gradient(s, kap, u.cpu().numpy(), lam.cpu().numpy())
+    rel = np.abs(g - go).max() / np.abs(go).max()
+    print(f"  {grid.num_elements} elements: gradient vs oracle {rel:.2e}")
+    assert rel <= 1e-12

@@ -992,3 +992,39 @@ def test_swipdg_order2_reproduces_quadratic_solution_3d(geom):
         nodes = x0[:, None, :] + np.einsum("ta,eab->etb", ref, E)
     err = np.abs(x[gi] - u(nodes)).max()
     assert err <= 1e-9, err
+
+
+@pytest.mark.parametrize("geom,cells", [(O.TRI, [4, 3]), (O.QUAD, [3, 3])])
+def test_adjoint_gradient_order2_central_differences(geom, cells):
+    """the adjoint gradient on P2 triangles and Q2 quadrilaterals: -kappa_e lambda_e^T Khat_e u_e with the
+    order-2 local Laplace matrix, against central differences of J = 1/2 |u - u_obs|^2"""
+    rng = np.random.default_rng(11)
+    m = O.Mesh.structured(geom, cells, [0, 0], [1, 1])
+    s = O.Space(m, True, 2)
+    ne = m.ne
+
+    def solve(logk):
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(np.exp(logk))
+        op.append_source(1.0)
+        op.set_dirichlet(True)
+        op.assemble(1)
+        rp, cols = op.pattern()
+        x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-15, max_iter=10000)
+        return x, (rp, cols, op.values())
+
+    g0 = rng.standard_normal(ne)
+    u_obs, _ = solve(rng.standard_normal(ne))
+    u, (rp, cols, vals) = solve(g0)
+    lam, _, _, _ = O.pcg_jacobi(rp, cols, vals, u - u_obs, rtol=1e-15, max_iter=10000)
+    grad = O.adjoint_log_kappa_gradient(s, np.exp(g0), u, lam)
+    h = 1e-6
+    gmax = np.abs(grad).max()
+    for e in rng.choice(ne, 6, replace=False):
+        gp, gm = g0.copy(), g0.copy()
+        gp[e] += h
+        gm[e] -= h
+        up, _ = solve(gp)
+        um, _ = solve(gm)
+        fd = (0.5 * np.sum((up - u_obs) ** 2) - 0.5 * np.sum((um - u_obs) ** 2)) / (2 * h)
+        assert abs(fd - grad[e]) <= 1e-6 * gmax, (e, fd, grad[e])
