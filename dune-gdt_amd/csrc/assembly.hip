@@ -1414,6 +1414,7 @@ __global__ void __launch_bounds__(256) k_gather_rhs(int64_t n, const uint32_t* _
 template <int GEOM, int ORDER>
 __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const double* __restrict__ coords,
                                                           const int32_t* __restrict__ conn,
+                                                          const int32_t* __restrict__ eid,
                                                           const int32_t* __restrict__ dofs,
                                                           const double* __restrict__ qtab, QRule R,
                                                           const double* __restrict__ kappa, double kappa_value,
@@ -1484,8 +1485,10 @@ __global__ void __launch_bounds__(256) k_adjoint_gradient(int64_t ne, const doub
       acc += dotv * (g.det * w);
     }
   }
-  const double k = kappa ? kappa[e] : kappa_value;
-  grad[e] = -k * acc;
+  // one part of a mixed grid: its elements' grid indices (coefficient and gradient)
+  const int64_t ge = eid ? eid[e] : e;
+  const double k = kappa ? kappa[ge] : kappa_value;
+  grad[ge] = -k * acc;
 }
 
 // BilinearForm(e, e).apply2() for e = u_h - g, element-local part: out[e] = int e^2 (product integrand,
@@ -1968,20 +1971,28 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
   if (F->points)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED,
                "the adjoint gradient is taken with respect to a per-element coefficient, not one given at points");
-  if (g.mixed)
-    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient kernel needs a grid of one element type");
   prepare_assembly();
   QTable etab, ltab;
   std::vector<ElementParams> PE;
   FaceParams C{}, B{};
   build_tables(*this, etab, ltab, PE, C, B);
-  const QRule R = PE[0].rule[form_index];
   const int block = launch[0].block;
-  dispatch_fe(g.ref.geom, space->order, [&](auto G, auto O) {
-    hipLaunchKernelGGL((k_adjoint_gradient<decltype(G)::value, decltype(O)::value>), dim3(grid_for(g.ne, block)),
-                       dim3(block), 0, stream, g.ne, g.d_coords.ptr, g.d_conn.ptr, space->d_dofs.ptr, d_qtab_elem.ptr,
-                       R, F->field, F->value, u, lambda, grad);
-  });
+  // one launch per element type (a mixed grid's parts, each over its own elements, DoFs and rule)
+  for (size_t part = 0; part < PE.size(); ++part) {
+    const int64_t ne = g.mixed ? g.parts[part].ne() : g.ne;
+    if (ne == 0)
+      continue;
+    const int geom = g.mixed ? g.parts[part].ref.geom : g.ref.geom;
+    const int32_t* conn = g.mixed ? g.parts[part].d_conn.ptr : g.d_conn.ptr;
+    const int32_t* eid = g.mixed ? g.parts[part].d_eid.ptr : nullptr;
+    const int32_t* dofs = g.mixed ? space->parts[part].d_dofs.ptr : space->d_dofs.ptr;
+    const QRule R = PE[part].rule[form_index];
+    dispatch_fe(geom, space->order, [&](auto G, auto O) {
+      hipLaunchKernelGGL((k_adjoint_gradient<decltype(G)::value, decltype(O)::value>), dim3(grid_for(ne, block)),
+                         dim3(block), 0, stream, ne, g.d_coords.ptr, conn, eid, dofs, d_qtab_elem.ptr, R, F->field,
+                         F->value, u, lambda, grad);
+    });
+  }
   HGDT_HIP(hipGetLastError());
 }
 

@@ -109,7 +109,7 @@ hgdt_status hgdt_grid_create(int geometry, int64_t num_vertices, const double* c
  * per element, -1 after a triangle's third), unless every element has one type: then it is a grid of that
  * type as hgdt_grid_create makes it.  Spaces of order 1 and 2 on mixed grids take element forms, right-hand
  * sides, Dirichlet constraints, intersection integrals (faces between a triangle and a quadrilateral
- * included), boundary functionals and error norms; the adjoint kernel, matrix-free application and unions
+ * included), boundary functionals, error norms and the adjoint kernel; matrix-free application and unions
  * need one element type. */
 hgdt_status hgdt_grid_create_mixed(int64_t num_vertices, const double* coords, int64_t num_elements,
                                    const int32_t* geometry, const int64_t* offsets, const int32_t* vertices,

@@ -2112,12 +2112,10 @@ int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const d
                                   double* grad)
 {
   return guarded([&] {
-    if (s->mesh->mixed)
-      throw Error("the adjoint gradient needs a mesh of one element type");
     Operator op;
     op.space = const_cast<OrSpace*>(s);
     const Mesh& m = *s->mesh;
-    const int ls = s->fe.size;
+    const int ls = s->fe.size; // the stride of the DoF indices (the largest local size on a mixed mesh)
     ElementForm F{EF_LAPLACE, {}, 0, 1., {}, {}, 0};
     F.coeff.constant = 1.;
 #pragma omp parallel
@@ -2127,14 +2125,15 @@ int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const d
 #pragma omp for schedule(static)
       for (int64_t e = 0; e < m.ne; ++e) {
         const Geometry geo = m.geometry(e);
-        basis.bind(&s->fe, &geo);
+        basis.bind(&s->fe_of(e), &geo);
         op.element_local(F, e, geo, basis, local);
         const int32_t* gi = &s->indices[e * ls];
+        const int le = s->fe_of(e).size;
         double acc = 0.;
-        for (int i = 0; i < ls; ++i) {
+        for (int i = 0; i < le; ++i) {
           double row = 0.;
-          for (int j = 0; j < ls; ++j)
-            row += local[i * ls + j] * u[gi[j]];
+          for (int j = 0; j < le; ++j)
+            row += local[i * le + j] * u[gi[j]];
           acc += lambda[gi[i]] * row;
         }
         grad[e] = -kappa[e] * acc;

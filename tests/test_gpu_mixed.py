@@ -236,3 +236,31 @@ def test_mixed_refusals(hgdt_mod):
     op.assemble()
     with pytest.raises(H.not_supported):
         op.apply(torch.zeros(space.num_DoFs, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_adjoint_gradient_on_mixed_grid(hgdt_mod, order):
+    """the adjoint element kernel on a mixed grid (one launch per element type, each element's gradient at its
+    grid index): for the solve u and the adjoint lambda (A lambda = u), within 1e-12 of the oracle's gradient
+    (pinned by central differences)"""
+    H = hgdt_mod
+    print(f"case adjoint gradient on a mixed grid, order {order}", flush=True)
+    V, E = strip_mixed_grid(10, 8, shear=0.25)
+    grid = H.Grid.mixed(V, E)
+    space = H.ContinuousLagrangeSpace(grid, order)
+    kap = np.exp(H.fields.normal(43, grid.num_elements))
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kap).cuda())))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    op.append(H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo()))
+    op.assemble()
+    u = op.apply_inverse("rhs", opts={"precision": 1e-13})
+    lam = op.apply_inverse(u.clone(), opts={"precision": 1e-13})
+    g = op.adjoint_log_diffusion_gradient(u, lam).cpu().numpy()
+    s = O.Space(O.Mesh.mixed(V, E), True, order)
+    go = O.adjoint_log_kappa_gradient(s, kap, u.cpu().numpy(), lam.cpu().numpy())
+    rel = np.abs(g - go).max() / np.abs(go).max()
+    print(f"  {grid.num_elements} elements: gradient vs oracle {rel:.2e}")
+    assert rel <= 1e-12

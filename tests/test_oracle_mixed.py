@@ -280,3 +280,44 @@ def test_mixed_swipdg_is_spd():
     A = op.csr().toarray()
     assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
     assert np.linalg.eigvalsh(A).min() > 0
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_adjoint_gradient_on_mixed_mesh_central_differences(order):
+    """the adjoint gradient on a mixed mesh (each element's own local Laplace matrix) against central
+    differences of J = 1/2 |u - u_obs|^2"""
+    import scipy.sparse  # noqa: F401  (the solves below are the oracle's PCG)
+
+    rng = np.random.default_rng(13)
+    V, E = strip_mixed_grid(4, 3, shear=0.3)
+    m = O.Mesh.mixed(V, E)
+    s = O.Space(m, True, order)
+    ne = m.ne
+
+    def solve(logk):
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(np.exp(logk))
+        op.append_source(1.0)
+        op.set_dirichlet(True)
+        op.assemble(1)
+        rp, cols = op.pattern()
+        x, it, rel, rc = O.pcg_jacobi(rp, cols, op.values(), op.rhs(), rtol=1e-15, max_iter=10000)
+        return x, (rp, cols, op.values())
+
+    g0 = rng.standard_normal(ne)
+    u_obs, _ = solve(rng.standard_normal(ne))
+    u, (rp, cols, vals) = solve(g0)
+    lam, _, _, _ = O.pcg_jacobi(rp, cols, vals, u - u_obs, rtol=1e-15, max_iter=10000)
+    grad = O.adjoint_log_kappa_gradient(s, np.exp(g0), u, lam)
+    h = 1e-6
+    gmax = np.abs(grad).max()
+    tri = [e for e in range(ne) if len(E[e]) == 3]
+    quad = [e for e in range(ne) if len(E[e]) == 4]
+    for e in list(rng.choice(tri, 3, replace=False)) + list(rng.choice(quad, 3, replace=False)):
+        gp, gm = g0.copy(), g0.copy()
+        gp[e] += h
+        gm[e] -= h
+        up, _ = solve(gp)
+        um, _ = solve(gm)
+        fd = (0.5 * np.sum((up - u_obs) ** 2) - 0.5 * np.sum((um - u_obs) ** 2)) / (2 * h)
+        assert abs(fd - grad[e]) <= 1e-6 * gmax, (e, fd, grad[e])
