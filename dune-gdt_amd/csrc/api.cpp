@@ -1022,6 +1022,19 @@ hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled)
   });
 }
 
+hgdt_status hgdt_operator_set_dirichlet_mode(hgdt_operator op, int32_t only_clear, int32_t ensure_symmetry)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    Operator& o = *op->op;
+    if (o.dirichlet_only_clear != (only_clear != 0) || o.dirichlet_ensure_symmetry != (ensure_symmetry != 0))
+      o.dirichlet_built = false; // the slot flags follow the mode
+    o.dirichlet_only_clear = only_clear != 0;
+    o.dirichlet_ensure_symmetry = ensure_symmetry != 0;
+    forms_changed(o);
+  });
+}
+
 hgdt_status hgdt_operator_dirichlet_dofs(hgdt_operator op, int64_t* count, int32_t* host_out)
 {
   return guarded([&] {

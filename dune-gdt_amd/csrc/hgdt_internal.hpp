@@ -419,6 +419,9 @@ struct Operator {
     return 0 + k + S.g_order + S.over_integrate;
   }
   bool dirichlet = false;
+  // DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry) (tools/dirichlet-constraints.hh:
+  // 152-184): clear instead of unit rows; the columns of the constrained DoFs too
+  bool dirichlet_only_clear = false, dirichlet_ensure_symmetry = true;
   // the Dirichlet intersections of the constraints (boundary_info.type == DirichletBoundary), one byte per
   // boundary intersection in walk order; empty: every boundary intersection
   std::vector<uint8_t> dirichlet_faces;

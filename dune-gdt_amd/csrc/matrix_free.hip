@@ -840,6 +840,8 @@ void mf_setup(Operator& op)
     unsupported("intersection forms are provided on discontinuous spaces");
   if (dg && op.dirichlet)
     unsupported("Dirichlet constraints are provided on continuous spaces");
+  if (op.dirichlet && (op.dirichlet_only_clear || !op.dirichlet_ensure_symmetry))
+    unsupported("Dirichlet constraints as unit rows and columns (the default of DirichletConstraints::apply)");
   if (op.eforms.size() > size_t(MF_MAXF) || op.cforms.size() > size_t(MF_MAXF) || op.bforms.size() > size_t(MF_MAXF))
     unsupported("too many forms appended");
   for (const auto& f : op.eforms)

@@ -760,12 +760,15 @@ void Operator::build_dirichlet()
   for (int64_t i = 0; i < n; ++i)
     if (mask[i])
       dirichlet_dofs.push_back(int32_t(i));
+  // slot flags: 1 the value 0, 2 the value 1; the rows of the constrained DoFs, and with ensure_symmetry
+  // their columns, unit (only_clear: cleared)
   std::vector<uint8_t> flag(std::max<int64_t>(nnz, 1), 0);
+  const bool clear = dirichlet_only_clear, cols_too = dirichlet_ensure_symmetry;
   parallel_for(n, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; ++i)
       for (int64_t t = row_ptr[i]; t < row_ptr[i + 1]; ++t)
-        if (mask[i] || mask[cols[t]])
-          flag[t] = (cols[t] == i) ? 2 : 1;
+        if (mask[i] || (cols_too && mask[cols[t]]))
+          flag[t] = (cols[t] == i && !clear) ? 2 : 1;
   });
   d_slot_flag.upload(flag);
   d_dof_mask.upload(mask);

@@ -2019,6 +2019,8 @@ bool Operator::direct_applicable()
   direct_state = 0;
   if (borrowed || !square()) // a caller's matrix or an operator between two spaces: the general path
     return false;
+  if (dirichlet && (dirichlet_only_clear || !dirichlet_ensure_symmetry)) // writes unit rows and columns
+    return false;
   const Space& s = *space;
   const Grid& g = *s.grid;
   // every instance a lattice (a structured grid, or a union of structured grids, which keeps their lattices)
@@ -2388,6 +2390,10 @@ void Operator::apply_inverse(const double* b, double* x, const hgdt_solver_optio
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "invalid solver options");
   if (!square())
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "apply_inverse needs an operator whose source and range spaces are the same");
+  if (dirichlet && !dirichlet_only_clear && !dirichlet_ensure_symmetry)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED,
+               "Jacobi-PCG needs a symmetric operator: unit rows without their columns (ensure_symmetry false) make "
+               "it non-symmetric");
   prepare_solver(user_stream);
   if (!fmt.diag_positive)
     HGDT_THROW(HGDT_ERR_SOLVER_FAILED, "Jacobi-PCG needs a positive diagonal: the matrix is not SPD");

@@ -182,6 +182,7 @@ SIGNATURES = {
     "hgdt_operator_append_source": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _dbl, _i32, _i32]),
     "hgdt_operator_append_boundary_source": (ctypes.c_int, [_vp, ctypes.c_int, _dbl, _vp, _dbl, _vp, _dbl, _i32, _i32]),
     "hgdt_operator_set_dirichlet": (ctypes.c_int, [_vp, _i32]),
+    "hgdt_operator_set_dirichlet_mode": (ctypes.c_int, [_vp, _i32, _i32]),
     "hgdt_operator_dirichlet_dofs": (ctypes.c_int, [_vp, _pi64, _pi32]),
     "hgdt_operator_assemble": (ctypes.c_int, [_vp, _vp]),
     "hgdt_operator_values_device": (ctypes.c_int, [_vp, _pvp]),

@@ -613,10 +613,22 @@ class DirichletConstraints:
             raise L.operator_error(3, "constraints not attached: append them to a MatrixOperator")
         return self._op.dirichlet_dofs()
 
-    def apply(self, *args, **kwargs):
-        """The unit-row/column update (ensure_symmetry) and b_i = 0 are fused into the assembly."""
+    def apply(self, *args, only_clear=None, ensure_symmetry=None):
+        """apply(matrix[, vector], only_clear=False, ensure_symmetry=True) (tools/dirichlet-constraints.hh:123-184):
+        fused into the operator's assembly (unit rows and columns, rhs entries 0, by default).  Another variant
+        (cleared rows; rows without their columns) is set for the attached operator, which is assembled again
+        so that its matrix holds it.  The flags may be given positionally after the matrix and vector, as in
+        the reference."""
         if self._op is None or not self._op._assembled:
             raise L.operator_error(3, "constraints are applied during MatrixOperator.assemble()")
+        flags = [a for a in args if isinstance(a, bool)]
+        oc = bool(flags[0]) if len(flags) > 0 else False if only_clear is None else bool(only_clear)
+        es = bool(flags[1]) if len(flags) > 1 else True if ensure_symmetry is None else bool(ensure_symmetry)
+        op = self._op
+        if (oc, es) != op._dirichlet_mode:
+            L.call("hgdt_operator_set_dirichlet_mode", op._h, int(oc), int(es))
+            op._dirichlet_mode = (oc, es)
+            op.assemble()
 
 
 def make_dirichlet_constraints(space, boundary_info=None):
@@ -683,6 +695,7 @@ class MatrixOperator:
         self.nnz = int(nnz.value)
         self._keep = []  # device fields referenced by the appended forms
         self._eforms = 0
+        self._dirichlet_mode = (False, True)  # DirichletConstraints::apply(..., only_clear, ensure_symmetry)
         self._assembled = False
         self.last_solve = None
         self._pattern = None

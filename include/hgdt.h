@@ -315,6 +315,12 @@ hgdt_status hgdt_operator_set_dirichlet_faces(hgdt_operator op, const uint8_t* m
 /* DirichletConstraints(space, AllDirichletBoundaryInfo) applied after assembly with
  * ensure_symmetry (tools/dirichlet-constraints.hh:85-184): unit rows and columns, rhs entry 0 */
 hgdt_status hgdt_operator_set_dirichlet(hgdt_operator op, int32_t enabled);
+/* the variant of DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry)
+ * (tools/dirichlet-constraints.hh:152-184) used by the next assemble(): the rows of the constrained DoFs
+ * unit (only_clear 0, the default) or cleared (1), and with ensure_symmetry (1, the default) their columns
+ * too; the rhs entries are 0 in every variant.  Jacobi-PCG takes the default (it refuses unit rows without
+ * their columns, and cleared rows have no positive diagonal); apply() takes every variant. */
+hgdt_status hgdt_operator_set_dirichlet_mode(hgdt_operator op, int32_t only_clear, int32_t ensure_symmetry);
 /* number of constrained DoFs and their (sorted) indices, host_out may be NULL */
 hgdt_status hgdt_operator_dirichlet_dofs(hgdt_operator op, int64_t* count, int32_t* host_out);
 

@@ -969,6 +969,9 @@ struct DirichletConstraints
   {}
   SpaceInterface space;
   ApplyOn::CustomBoundaryIntersections faces;
+  // the variant of apply(matrix, vector, only_clear, ensure_symmetry) (tools/dirichlet-constraints.hh:152-184)
+  // fused into the assembly: unit (or cleared) rows, and with ensure_symmetry their columns
+  bool only_clear = false, ensure_symmetry = true;
 };
 inline DirichletConstraints make_dirichlet_constraints(const SpaceInterface& space)
 {
@@ -1202,6 +1205,7 @@ public:
     const auto mask = constraints.faces.mask(space_.grid_view());
     check(hgdt_operator_set_dirichlet_faces(handle(), mask.empty() ? nullptr : mask.data(), int64_t(mask.size())));
     check(hgdt_operator_set_dirichlet(handle(), 1));
+    check(hgdt_operator_set_dirichlet_mode(handle(), constraints.only_clear ? 1 : 0, constraints.ensure_symmetry ? 1 : 0));
     return *this;
   }
   template <class T>

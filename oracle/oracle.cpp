@@ -817,6 +817,7 @@ struct Operator {
           bface_index[e * nf + f] = c++;
   }
   bool dirichlet = false;
+  bool only_clear = false, ensure_symmetry = true; // the variant of DirichletConstraints::apply
   std::vector<int32_t> dirichlet_dofs;
   // boundary_info_.type(intersection) == DirichletBoundary() (dirichlet-constraints.hh:93), by boundary
   // intersection in walk order; empty: every boundary intersection is a Dirichlet intersection
@@ -1332,8 +1333,10 @@ struct Operator {
       apply_dirichlet();
   }
 
-  // DirichletConstraints::apply_local (tools/dirichlet-constraints.hh:85-110) + apply(matrix, vector) with
-  // ensure_symmetry (:153-177): unit_col then unit_row for every DoF, vector entry cleared
+  // DirichletConstraints::apply_local (tools/dirichlet-constraints.hh:85-110) + apply(matrix, vector, only_clear,
+  // ensure_symmetry) (:152-184): for every DoF [unit|clear]_col (with ensure_symmetry) then [unit|clear]_row, the
+  // vector entry cleared.  Column d ends with 1 at (d, d) at most, which its row keeps unless cleared, so the
+  // sequence amounts to: rows of constrained DoFs unit (cleared), with ensure_symmetry their columns zero
   void apply_dirichlet()
   {
     const Mesh& m = *space->mesh;
@@ -1362,8 +1365,8 @@ struct Operator {
     for (int64_t i = 0; i < n; ++i)
       for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
         const int32_t j = cols[k];
-        if (mask[i] || mask[j])
-          values[k] = (i == j) ? 1. : 0.;
+        if (mask[i] || (ensure_symmetry && mask[j]))
+          values[k] = (i == j && !only_clear) ? 1. : 0.;
       }
     for (int32_t d : dirichlet_dofs)
       rhs[d] = 0.;
@@ -1689,6 +1692,12 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
     S.f.set(f, f_const, op->space->mesh->ne);
     op->sources.push_back(S);
   });
+}
+int or_operator_set_dirichlet_mode(OrOperator* op, int only_clear, int ensure_symmetry)
+{
+  op->only_clear = only_clear != 0;
+  op->ensure_symmetry = ensure_symmetry != 0;
+  return 0;
 }
 int or_operator_set_dirichlet(OrOperator* op, int enabled)
 {

@@ -88,6 +88,8 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
                               int over_integrate);
 /* constrain every boundary DoF (AllDirichletBoundaryInfo) -- only meaningful for CG spaces */
 int or_operator_set_dirichlet(OrOperator* op, int enabled);
+/* DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry), tools/dirichlet-constraints.hh:152-184 */
+int or_operator_set_dirichlet_mode(OrOperator* op, int only_clear, int ensure_symmetry);
 
 int or_operator_assemble(OrOperator* op, int nthreads);
 void or_operator_values(const OrOperator* op, double* out);

@@ -84,6 +84,7 @@ def lib():
             [_vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int],
         ),
         "or_operator_set_dirichlet": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "or_operator_set_dirichlet_mode": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "or_operator_assemble": (ctypes.c_int, [_vp, ctypes.c_int]),
         "or_operator_values": (None, [_vp, _dp]),
         "or_operator_rhs": (None, [_vp, _dp]),
@@ -357,6 +358,9 @@ class Operator:
 
     def set_dirichlet(self, enabled=True):
         self._check(lib().or_operator_set_dirichlet(self.h, int(enabled)))
+
+    def set_dirichlet_mode(self, only_clear=False, ensure_symmetry=True):
+        self._check(lib().or_operator_set_dirichlet_mode(self.h, int(only_clear), int(ensure_symmetry)))
 
     def assemble(self, nthreads=1):
         self._check(lib().or_operator_assemble(self.h, nthreads))

@@ -116,3 +116,55 @@ def test_swipdg_boundary_forms_on_dirichlet_faces(hgdt_mod, geometry):
     rep = np.abs(x[space.global_indices()] - u(np.moveaxis(Xe, -1, 0))).max()
     print(f"  matrix {err:.2e}, rhs {berr:.2e}, linear solution reproduced to {rep:.2e}")
     assert err <= 1e-12 and berr <= 1e-12 and rep <= 1e-9
+
+
+@pytest.mark.parametrize("only_clear,ensure_symmetry", [(False, False), (True, True), (True, False), (False, True)])
+def test_dirichlet_apply_variants_match_oracle(hgdt_mod, only_clear, ensure_symmetry):
+    """DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry) (dirichlet-constraints.hh:
+    152-184) through constraints.apply(...) after assembly (the operator is assembled again with the variant):
+    matrix and rhs bit-exact against the oracle's on the constrained slots, 1e-12 elsewhere; apply() of the
+    variant equals the oracle matrix's product; Jacobi-PCG refuses the non-symmetric and the cleared variants"""
+    H = hgdt_mod
+    print(f"case Dirichlet apply only_clear={only_clear} ensure_symmetry={ensure_symmetry}", flush=True)
+    cells = [9, 7]
+    kappa = H.fields.lognormal(5, cells[0] * cells[1])
+    grid = H.Grid.structured("quad", [0, 0], [1, 1], cells)
+    space = H.ContinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element)
+    op.append(H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(torch.from_numpy(kappa).cuda())))
+    rhs = H.make_vector_functional(space)
+    rhs.append(H.LocalElementIntegralFunctional(H.LocalProductIntegrand().with_ansatz(1.0)))
+    op.append(rhs)
+    constraints = H.make_dirichlet_constraints(space, H.AllDirichletBoundaryInfo())
+    op.append(constraints)
+    op.assemble()
+    constraints.apply(op.matrix, op.rhs, only_clear, ensure_symmetry)
+    m = O.Mesh.structured(O.QUAD, cells, [0, 0], [1, 1])
+    s = O.Space(m, True, 1)
+    oop = O.Operator(s, O.STENCIL_ELEMENT)
+    oop.append_laplace(kappa)
+    oop.append_source(1.0)
+    oop.set_dirichlet(True)
+    oop.set_dirichlet_mode(only_clear, ensure_symmetry)
+    oop.assemble(1)
+    rp, cols = oop.pattern()
+    assert np.array_equal(op.pattern()[0], rp) and np.array_equal(op.pattern()[1], cols)
+    v, ov = op.matrix.values.cpu().numpy(), oop.values()
+    dof = np.zeros(len(rp) - 1, bool)
+    dof[oop.dirichlet_dofs()] = True
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    constrained = dof[rows] | (ensure_symmetry & dof[cols])
+    assert np.array_equal(v[constrained], ov[constrained])
+    assert matrix_error(v, ov, rp) <= 1e-12
+    assert np.abs(op.rhs.cpu().numpy() - oop.rhs()).max() <= 1e-12 * np.abs(oop.rhs()).max()
+    x = torch.from_numpy(np.linspace(-1, 1, len(rp) - 1)).cuda()
+    y = op.apply(x).cpu().numpy()
+    import scipy.sparse as sp
+
+    yo = sp.csr_matrix((ov, cols, rp), shape=(len(rp) - 1,) * 2) @ x.cpu().numpy()
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+    if (only_clear, ensure_symmetry) == (False, True):
+        op.apply_inverse("rhs")
+    else:
+        with pytest.raises(H._lib.HgdtError):
+            op.apply_inverse("rhs")

@@ -1028,3 +1028,42 @@ def test_adjoint_gradient_order2_central_differences(geom, cells):
         um, _ = solve(gm)
         fd = (0.5 * np.sum((up - u_obs) ** 2) - 0.5 * np.sum((um - u_obs) ** 2)) / (2 * h)
         assert abs(fd - grad[e]) <= 1e-6 * gmax, (e, fd, grad[e])
+
+
+@pytest.mark.parametrize("only_clear,ensure_symmetry", [(False, True), (False, False), (True, True), (True, False)])
+def test_dirichlet_apply_variants(only_clear, ensure_symmetry):
+    """DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry) (dirichlet-constraints.hh:152-184)
+    replayed on the dense unconstrained matrix DoF by DoF as the reference does it: [clear|unit]_col (with
+    ensure_symmetry), then [clear|unit]_row, vector entry 0; the oracle's constrained matrix and rhs equal it"""
+    import scipy.sparse as sp
+
+    m = O.Mesh.structured(O.QUAD, [5, 4], [0, 0], [1, 1])
+    kap = 1.0 + 0.2 * np.arange(m.ne)
+
+    def build(dirichlet):
+        s = O.Space(m, True, 1)
+        op = O.Operator(s, O.STENCIL_ELEMENT)
+        op.append_laplace(kap)
+        op.append_source(1.0)
+        op.set_dirichlet(dirichlet)
+        op.set_dirichlet_mode(only_clear, ensure_symmetry)
+        op.assemble(1)
+        rp, cols = op.pattern()
+        return sp.csr_matrix((op.values(), cols, rp), shape=(len(rp) - 1,) * 2).toarray(), op.rhs(), op
+
+    A0, b0, _ = build(False)
+    A1, b1, op = build(True)
+    dofs = op.dirichlet_dofs()
+    assert len(dofs) == 2 * 6 + 2 * 3
+    A, b = A0.copy(), b0.copy()
+    pattern = A0 != 0
+    for d in dofs:
+        if ensure_symmetry:
+            A[:, d] = 0.0
+            if not only_clear:
+                A[d, d] = 1.0
+        A[d, :] = 0.0
+        if not only_clear:
+            A[d, d] = 1.0
+        b[d] = 0.0
+    assert np.array_equal(A1[pattern], A[pattern]) and np.array_equal(b1, b)
