@@ -891,6 +891,32 @@ static void append_element_points(hgdt_operator op, int kind, const double* valu
   forms_changed(*op->op);
 }
 
+hgdt_status hgdt_operator_append_laplace_tensor(hgdt_operator op, const double* tensor_field,
+                                                const double* tensor_value, int32_t over_integrate, double scaling)
+{
+  return guarded([&] {
+    require(op && (tensor_field || tensor_value) && over_integrate >= 0, HGDT_ERR_INVALID_ARGUMENT, "invalid argument");
+    const Grid& g = *op->op->space->grid;
+    const int d = g.ref.dim;
+    ElementFormDesc F{EF_LAPLACE, nullptr, 0., over_integrate, scaling};
+    if (tensor_field) {
+      require(tensor_field_symmetric(tensor_field, g.ne, d), HGDT_ERR_NOT_SUPPORTED,
+              "the diffusion tensor must be symmetric in every element (the local matrices are symmetric)");
+      F.tensor_kind = 1, F.tensor = tensor_field;
+    } else {
+      for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b)
+          require(tensor_value[a * d + b] == tensor_value[b * d + a], HGDT_ERR_NOT_SUPPORTED,
+                  "the diffusion tensor must be symmetric (the local matrices are symmetric)");
+      F.tensor_kind = 2;
+      for (int t = 0; t < d * d; ++t)
+        F.tensor_value[t] = tensor_value[t];
+    }
+    op->op->eforms.push_back(F);
+    forms_changed(*op->op);
+  });
+}
+
 hgdt_status hgdt_operator_append_laplace_points(hgdt_operator op, const double* values, int32_t coeff_order,
                                                 int32_t over_integrate, double scaling)
 {

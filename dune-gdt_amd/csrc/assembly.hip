@@ -50,6 +50,10 @@ struct ElementParams {
   // grids where each element's points start
   const double* points[MAX_EFORMS];
   const int64_t* ef_qoff[MAX_EFORMS];
+  // Laplace forms with a matrix-valued diffusion: 1 per element (tensor, d x d row-major), 2 constant (tvalue)
+  int tensor_kind[MAX_EFORMS];
+  const double* tensor[MAX_EFORMS];
+  double tvalue[MAX_EFORMS][9];
   const double* qtab; // points (dim each) then weights, per rule
   int nforms;
   int kind[MAX_EFORMS];
@@ -338,6 +342,35 @@ __device__ __forceinline__ void basis(const double* xh, double* phi, double (*dp
   }
 }
 
+// (K g) . h for a matrix-valued diffusion K (row-major D x D): (weight * ansatz_grad) * test_grad,
+// local/integrands/laplace.hh:99-101
+template <int D>
+__device__ __forceinline__ double kmat_dot(const double (&K)[D * D], const double* g, const double* h)
+{
+  double s = 0.;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double kg = 0.;
+#pragma unroll
+    for (int b = 0; b < D; ++b)
+      kg += K[a * D + b] * g[b];
+    s += kg * h[a];
+  }
+  return s;
+}
+
+// the form's matrix-valued diffusion at grid element ge, if it has one
+template <int D, class PP>
+__device__ __forceinline__ bool load_tensor(const PP& P, int f, int64_t ge, double (&K)[D * D])
+{
+  if (P.tensor_kind[f] == 0)
+    return false;
+#pragma unroll
+  for (int t = 0; t < D * D; ++t)
+    K[t] = P.tensor_kind[f] == 1 ? P.tensor[f][ge * (D * D) + t] : P.tvalue[f][t];
+  return true;
+}
+
 // physical gradient g = J^{-T} gh (spaces/basis/default.hh:152-175)
 template <int D>
 __device__ __forceinline__ void to_physical(const Geo<D>& g, const double* gh, double* out)
@@ -377,6 +410,8 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
       const double* __restrict__ cp = P.points[f]; // the coefficient at the rule's points of the element
       if (cp)
         cp += P.ef_qoff[f] ? P.ef_qoff[f][e] : e * P.rule[f].npts;
+      double K[D * D];
+      const bool tk = P.kind[f] == EF_LAPLACE && load_tensor<D>(P, f, ge, K);
       if (P.kind[f] == EF_LAPLACE && T::SIMPLEX && ORDER == 1 && !cp) {
         // P1 gradients are constant: the integrand is constant on the element and any
         // rule of the order chosen by integrals.hh:115 reduces to one evaluation
@@ -395,9 +430,13 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 #pragma unroll
           for (int j = i; j < ND; ++j) {
             double dotv = 0.;
+            if (tk) {
+              dotv = kmat_dot<D>(K, G[j], G[i]);
+            } else {
 #pragma unroll
-            for (int a = 0; a < D; ++a)
-              dotv += (c * G[j][a]) * G[i][a];
+              for (int a = 0; a < D; ++a)
+                dotv += (c * G[j][a]) * G[i][a];
+            }
             A[i][j] += s * (dotv * factor);
           }
         continue;
@@ -423,9 +462,13 @@ __global__ void __launch_bounds__(256) k_element_local(ElementParams P)
 #pragma unroll
             for (int j = i; j < ND; ++j) {
               double dotv = 0.;
+              if (tk) {
+                dotv = kmat_dot<D>(K, G[j], G[i]);
+              } else {
 #pragma unroll
-              for (int a = 0; a < D; ++a)
-                dotv += (cq * G[j][a]) * G[i][a];
+                for (int a = 0; a < D; ++a)
+                  dotv += (cq * G[j][a]) * G[i][a];
+              }
               A[i][j] += s * (dotv * factor);
             }
         } else {
@@ -528,6 +571,8 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
       const double* __restrict__ cp = P.points[f]; // the coefficient at the rule's points of the element
       if (cp)
         cp += P.ef_qoff[f] ? P.ef_qoff[f][e] : e * R.npts;
+      double K[9];
+      const bool tk = P.kind[f] == EF_LAPLACE && load_tensor<3>(P, f, ge, K);
       for (int q = 0; q < R.npts; ++q) {
         double xh[D];
 #pragma unroll
@@ -552,9 +597,13 @@ __global__ void __launch_bounds__(256) k_element_rows(ElementParams P)
             double Gj[3];
             to_physical<3>(g, ghj, Gj);
             double dotv = 0.;
+            if (tk) {
+              dotv = kmat_dot<3>(K, Gj, Gi);
+            } else {
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-              dotv += (cq * Gj[a]) * Gi[a];
+              for (int a = 0; a < 3; ++a)
+                dotv += (cq * Gj[a]) * Gi[a];
+            }
             row[j] += s * (dotv * factor);
           }
         } else {
@@ -1743,6 +1792,9 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, std::ve
       const auto& F = op.eforms[f];
       P.kind[f] = F.kind, P.field[f] = F.field, P.value[f] = F.value, P.scaling[f] = F.scaling;
       P.points[f] = F.points;
+      P.tensor_kind[f] = F.tensor_kind, P.tensor[f] = F.tensor;
+      for (int t = 0; t < 9; ++t)
+        P.tvalue[f][t] = F.tensor_value[t];
       // integrand order: coefficient (0 when piecewise constant) + test + ansatz (laplace.hh:74-79,
       // product.hh:88-100)
       P.rule[f] = etab.add(element_rule(r, F.coeff_order + k + k + F.over_integrate));
@@ -1968,6 +2020,8 @@ void Operator::adjoint_gradient(int form_index, const double* u, const double* l
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "form_index does not name an appended element form");
   if (F->kind != EF_LAPLACE)
     HGDT_THROW(HGDT_ERR_INVALID_ARGUMENT, "the adjoint gradient is defined for a Laplace form");
+  if (F->tensor_kind)
+    HGDT_THROW(HGDT_ERR_NOT_SUPPORTED, "the adjoint gradient is taken with respect to a scalar diffusion");
   if (F->points)
     HGDT_THROW(HGDT_ERR_NOT_SUPPORTED,
                "the adjoint gradient is taken with respect to a per-element coefficient, not one given at points");
@@ -2211,6 +2265,34 @@ void error_norms_noalloc(const Space& s, const double* dofs, int order, const do
             acc += part[2 * win0[w] + comp * (win0[w + 1] - win0[w]) + (c - win0[w])];
       host_out[2 * i + comp] = acc;
     }
+}
+
+namespace {
+__global__ void k_tensor_symmetric(int64_t ne, int d, const double* __restrict__ t, int* __restrict__ flag)
+{
+  const int64_t e = xcd_index(ne);
+  if (e >= ne)
+    return;
+  bool sym = true;
+  for (int a = 0; a < d; ++a)
+    for (int b = a + 1; b < d; ++b)
+      sym = sym && t[e * d * d + a * d + b] == t[e * d * d + b * d + a];
+  if (!sym)
+    *flag = 1; // every writer stores the same value
+}
+} // namespace
+
+bool tensor_field_symmetric(const double* tensor, int64_t ne, int d)
+{
+  DeviceArray<int> flag;
+  flag.allocate(1);
+  int h = 0;
+  HGDT_HIP(hipMemcpy(flag.ptr, &h, sizeof(int), hipMemcpyHostToDevice));
+  if (ne > 0)
+    hipLaunchKernelGGL(k_tensor_symmetric, dim3(xcd_grid(ne, 256)), dim3(256), 0, nullptr, ne, d, tensor, flag.ptr);
+  HGDT_HIP(hipGetLastError());
+  HGDT_HIP(hipMemcpy(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
+  return h == 0;
 }
 
 } // namespace hgdt

@@ -230,6 +230,11 @@ struct ElementFormDesc {
   // space order), element-ordered (NULL: field or value)
   const double* points = nullptr;
   int coeff_order = 0;
+  // Laplace forms with a matrix-valued diffusion (GridFunction<E, d, d>): 1 d x d per element (tensor, device,
+  // row-major), 2 one d x d for all (tensor_value)
+  int tensor_kind = 0;
+  const double* tensor = nullptr;
+  double tensor_value[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
 };
 struct FaceFormDesc {
   int kind;
@@ -532,6 +537,9 @@ void mf_launch(const Operator& op, int mode, const MfVectors& v, hipStream_t str
 // kernel launchers (assembly.hip)
 void launch_assembly(Operator& op, hipStream_t stream, bool matrix);
 // squared L2 and (kappa-weighted) H1-semi norms of u_h - g per instance (host_out[2 * ninst])
+// every element's d x d block of a device tensor field is symmetric (exactly)
+bool tensor_field_symmetric(const double* tensor, int64_t ne, int d);
+
 void error_norms(const Space& s, const double* dofs, int order, const double* gval, const double* ggrad,
                  const double* kappa, double kappa_value, double* host_out, hipStream_t stream);
 // the same results (bit for bit) through a fixed device array instead of allocated buffers

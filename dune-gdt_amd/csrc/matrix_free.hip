@@ -845,8 +845,9 @@ void mf_setup(Operator& op)
   if (op.eforms.size() > size_t(MF_MAXF) || op.cforms.size() > size_t(MF_MAXF) || op.bforms.size() > size_t(MF_MAXF))
     unsupported("too many forms appended");
   for (const auto& f : op.eforms)
-    if (f.points)
-      unsupported("element coefficients are per element or constant (not given at quadrature points)");
+    if (f.points || f.tensor_kind)
+      unsupported("element coefficients are scalar, per element or constant (not matrix-valued or given at "
+                  "quadrature points)");
   for (const auto& b : op.bforms)
     if (b.mask)
       unsupported("boundary forms are applied on every boundary intersection (no boundary-info filter)");

@@ -2032,7 +2032,7 @@ bool Operator::direct_applicable()
       !bsources.empty() || sources.size() > size_t(DIRECT_MAX_SOURCES))
     return false;
   for (const auto& f : eforms)
-    if (f.kind != EF_LAPLACE || f.points)
+    if (f.kind != EF_LAPLACE || f.points || f.tensor_kind)
       return false;
   for (const auto& src : sources)
     if (src.kind != HGDT_SOURCE_CONSTANT && src.kind != HGDT_SOURCE_PER_ELEMENT)

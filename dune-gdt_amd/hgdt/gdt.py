@@ -369,6 +369,8 @@ class GridFunction:
         if isinstance(value, (int, float)):
             self.constant, self.field = float(value), None
         else:
+            if isinstance(value, TensorGridFunction) or getattr(value, "ndim", 1) > 1:
+                raise L.not_supported(9, "matrix-valued coefficients are taken by LocalLaplaceIntegrand only")
             self.constant, self.field = 0.0, _dvec(value, name="element field")
 
     def args(self, grid=None):
@@ -381,6 +383,26 @@ class GridFunction:
 
     def order(self):
         return 0
+
+
+class TensorGridFunction:
+    """Matrix-valued GridFunction<E, d, d> for LocalLaplaceIntegrand (laplace.hh:40-48): one symmetric d x d
+    matrix for every element (shape (d, d)) or one per element (shape (num_elements, d, d), device tensor)."""
+
+    def __init__(self, value):
+        if isinstance(value, TensorGridFunction):
+            self.constant, self.field = value.constant, value.field
+            return
+        if isinstance(value, torch.Tensor) and value.dim() == 3:
+            self.constant, self.field = None, value.to(dtype=torch.float64, device=_device()).contiguous()
+        else:
+            a = np.asarray(value.cpu() if isinstance(value, torch.Tensor) else value, np.float64)
+            if a.ndim == 3:
+                self.constant, self.field = None, torch.from_numpy(np.ascontiguousarray(a)).to(_device())
+            elif a.ndim == 2 and a.shape[0] == a.shape[1]:
+                self.constant, self.field = np.ascontiguousarray(a), None
+            else:
+                raise L.shapes_do_not_match(2, f"a diffusion tensor has shape (d, d) or (elements, d, d), not {a.shape}")
 
 
 class QuadraturePointFunction:
@@ -430,7 +452,11 @@ def _gf(v):
 def _element_coefficient(v):
     """coefficient of an element integrand: a GridFunction (constant or per element), or one that varies inside
     the elements (GenericFunction, QuadraturePointFunction), evaluated at the quadrature points"""
-    return v if isinstance(v, (GenericFunction, QuadraturePointFunction)) else _gf(v)
+    if isinstance(v, (GenericFunction, QuadraturePointFunction, TensorGridFunction)):
+        return v
+    if getattr(v, "ndim", 1) > 1 or (isinstance(v, (list, tuple)) and np.ndim(v) > 1):
+        return TensorGridFunction(v)
+    return _gf(v)
 
 
 # ---------------------------------------------------------------------------
@@ -764,7 +790,23 @@ class MatrixOperator:
             coeff, kind = integrand.weight, "product"
         else:
             raise L.not_supported(9, f"{type(integrand).__name__} is not an element integrand of this path")
-        if isinstance(coeff, GridFunction):
+        if isinstance(coeff, TensorGridFunction):
+            if kind != "laplace":
+                raise L.not_supported(9, "matrix-valued coefficients are taken by LocalLaplaceIntegrand only")
+            d = self.grid.dimension
+            if coeff.field is not None:
+                if tuple(coeff.field.shape) != (self.grid.num_elements, d, d):
+                    raise L.shapes_do_not_match(2, f"diffusion tensor of shape {tuple(coeff.field.shape)}, expected "
+                                                   f"({self.grid.num_elements}, {d}, {d})")
+                self._keep.append(coeff.field)
+                L.call("hgdt_operator_append_laplace_tensor", self._h, ctypes.c_void_p(coeff.field.data_ptr()), None,
+                       over, scaling)
+            else:
+                if coeff.constant.shape != (d, d):
+                    raise L.shapes_do_not_match(2, f"diffusion tensor of shape {coeff.constant.shape}, expected ({d}, {d})")
+                L.call("hgdt_operator_append_laplace_tensor", self._h, None,
+                       coeff.constant.ctypes.data_as(ctypes.c_void_p), over, scaling)
+        elif isinstance(coeff, GridFunction):
             p, v = self._field(coeff)
             L.call(f"hgdt_operator_append_{kind}", self._h, p, v, over, scaling)
         else:

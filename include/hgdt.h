@@ -254,6 +254,13 @@ hgdt_status hgdt_operator_append_product(hgdt_operator op, const double* weight_
  * valid.  Not for the matrix-free format or the adjoint gradient (their coefficients are per element). */
 hgdt_status hgdt_operator_append_laplace_points(hgdt_operator op, const double* values, int32_t coeff_order,
                                                 int32_t over_integrate, double scaling);
+/* the Laplace form with a matrix-valued diffusion, LocalLaplaceIntegrand(GridFunction<E, d, d>)
+ * (laplace.hh:40-48,94-101: (K grad phi_j) . grad psi_i): tensor_field a device array of d x d values per
+ * element (row-major, num_elements * d * d), or NULL and tensor_value d x d host values for every element
+ * (copied).  K must be symmetric (checked; the local matrices are then symmetric and only their upper
+ * triangles are computed).  Not for the matrix-free format or the adjoint gradient. */
+hgdt_status hgdt_operator_append_laplace_tensor(hgdt_operator op, const double* tensor_field,
+                                                const double* tensor_value, int32_t over_integrate, double scaling);
 hgdt_status hgdt_operator_append_product_points(hgdt_operator op, const double* values, int32_t coeff_order,
                                                 int32_t over_integrate, double scaling);
 /* append(LocalCouplingIntersectionIntegralBilinearForm(LocalLaplaceIPDGIntegrands::InnerCoupling(

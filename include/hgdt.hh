@@ -371,6 +371,40 @@ private:
   std::shared_ptr<DeviceVector> field_;
 };
 
+// matrix-valued GridFunction<E, d, d> (the diffusion of LocalLaplaceIntegrand, laplace.hh:40-48): one symmetric
+// d x d matrix (row-major) for every element, or one per element (num_elements * d * d values)
+class TensorGridFunction
+{
+public:
+  // d x d values for every element
+  explicit TensorGridFunction(std::vector<double> value)
+    : value_(std::move(value))
+  {}
+  // d x d values per element, copied to the device (per_element must be true)
+  TensorGridFunction(const std::vector<double>& values, bool per_element)
+    : field_(std::make_shared<DeviceVector>(values))
+  {
+    if (!per_element)
+      value_ = values, field_.reset();
+  }
+  const double* field() const
+  {
+    return field_ ? field_->data() : nullptr;
+  }
+  const double* value() const
+  {
+    return value_.empty() ? nullptr : value_.data();
+  }
+  const std::shared_ptr<DeviceVector>& storage() const
+  {
+    return field_;
+  }
+
+private:
+  std::vector<double> value_;
+  std::shared_ptr<DeviceVector> field_;
+};
+
 // XT::Functions::GenericFunction<d>(order, evaluate, name, parameter_type, jacobian): analytic function of
 // the physical coordinates, evaluated on the host at quadrature points
 class GenericFunction
@@ -512,8 +546,13 @@ struct LocalLaplaceIntegrand
   explicit LocalLaplaceIntegrand(const GenericFunction& diffusion_)
     : generic(std::make_shared<GenericFunction>(diffusion_))
   {}
+  // a matrix-valued diffusion (symmetric)
+  explicit LocalLaplaceIntegrand(const TensorGridFunction& diffusion_)
+    : tensor(std::make_shared<TensorGridFunction>(diffusion_))
+  {}
   GridFunction diffusion;
   std::shared_ptr<GenericFunction> generic;
+  std::shared_ptr<TensorGridFunction> tensor;
 };
 
 // LocalBinaryToUnaryElementIntegrand(LocalElementProductIntegrand(1), f): the right-hand side integrand
@@ -684,6 +723,7 @@ struct LocalElementIntegralBilinearForm
     : laplace(true)
     , coefficient(integrand.diffusion)
     , generic(integrand.generic)
+    , tensor(integrand.tensor)
     , over_integrate(over_integrate_)
   {}
   LocalElementIntegralBilinearForm(const LocalElementProductIntegrand& integrand, int over_integrate_ = 0)
@@ -695,6 +735,7 @@ struct LocalElementIntegralBilinearForm
   bool laplace;
   GridFunction coefficient;
   std::shared_ptr<GenericFunction> generic; // a coefficient varying inside the elements
+  std::shared_ptr<TensorGridFunction> tensor; // a matrix-valued diffusion (Laplace forms)
   int over_integrate;
 };
 struct LocalCouplingIntersectionIntegralBilinearForm
@@ -1090,6 +1131,13 @@ public:
   {
     keep(form.coefficient);
     const double scaling = param.get("matrixoperator.scaling", 1.);
+    if (form.tensor) {
+      if (form.tensor->storage())
+        kept_.push_back(form.tensor->storage());
+      check(hgdt_operator_append_laplace_tensor(handle(), form.tensor->field(), form.tensor->value(),
+                                                form.over_integrate, scaling));
+      return *this;
+    }
     if (form.generic) {
       // values at the points of the rule of order coefficient order + 2 k + over_integrate
       const int order = form.generic->order() + 2 * space_.order() + form.over_integrate;

@@ -723,6 +723,8 @@ struct ElementForm {
   std::vector<double> points;
   std::vector<int64_t> points_off;
   int coeff_order = 0;
+  // a matrix-valued diffusion (GridFunction<E, d, d>): d x d per element, or d x d for all (Laplace forms)
+  std::vector<double> tensor;
 };
 struct FaceForm {
   int kind;
@@ -789,6 +791,19 @@ inline double kgrad_dot(double kappa, const double* g, const double* h, int d)
   double s = 0.;
   for (int a = 0; a < d; ++a)
     s += (kappa * g[a]) * h[a];
+  return s;
+}
+// (K g) . h for a matrix-valued weight K (d x d, row-major): (weight * ansatz_grad) * test_grad,
+// local/integrands/laplace.hh:99-101
+inline double kmat_grad_dot(const double* K, const double* g, const double* h, int d)
+{
+  double s = 0.;
+  for (int a = 0; a < d; ++a) {
+    double kg = 0.;
+    for (int b = 0; b < d; ++b)
+      kg += K[a * d + b] * g[b];
+    s += kg * h[a];
+  }
   return s;
 }
 
@@ -903,7 +918,13 @@ struct Operator {
       const double factor = geo.integration_element * q.w[qp];
       // local_weight_->evaluate(x) (laplace.hh:92, product.hh:118)
       const double c = F.points.empty() ? F.coeff.at(e) : F.points[size_t(F.points_off[e]) + qp];
-      if (F.kind == EF_LAPLACE) {
+      if (F.kind == EF_LAPLACE && !F.tensor.empty()) {
+        const double* K = &F.tensor[F.tensor.size() == size_t(d * d) ? 0 : size_t(e) * d * d];
+        basis.jacobians(xh);
+        for (int i = 0; i < ls; ++i)
+          for (int j = 0; j < ls; ++j)
+            iv[i * ls + j] = 0. + kmat_grad_dot(K, &basis.grads[3 * j], &basis.grads[3 * i], d);
+      } else if (F.kind == EF_LAPLACE) {
         // local/integrands/laplace.hh:94-101
         basis.jacobians(xh);
         for (int i = 0; i < ls; ++i)
@@ -1602,7 +1623,7 @@ int or_operator_append_laplace(OrOperator* op, const double* kappa, double kappa
                                double scaling)
 {
   return guarded([&] {
-    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling, {}, {}, 0};
+    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling, {}, {}, 0, {}};
     F.coeff.set(kappa, kappa_const, op->space->mesh->ne);
     op->eforms.push_back(F);
   });
@@ -1611,7 +1632,7 @@ int or_operator_append_product(OrOperator* op, const double* weight, double weig
                                double scaling)
 {
   return guarded([&] {
-    ElementForm F{EF_PRODUCT, {}, over_integrate, scaling, {}, {}, 0};
+    ElementForm F{EF_PRODUCT, {}, over_integrate, scaling, {}, {}, 0, {}};
     F.coeff.set(weight, weight_const, op->space->mesh->ne);
     op->eforms.push_back(F);
   });
@@ -1625,7 +1646,7 @@ static int append_element_points(OrOperator* op, int kind, const double* values,
     const Mesh& m = *op->space->mesh;
     if (!values || coeff_order < 0 || over_integrate < 0)
       throw Error("invalid coefficient values");
-    ElementForm F{kind, {}, over_integrate, scaling, {}, {}, 0};
+    ElementForm F{kind, {}, over_integrate, scaling, {}, {}, 0, {}};
     F.coeff_order = coeff_order;
     const int order = coeff_order + 2 * op->space->fe.order + over_integrate;
     F.points_off.assign(size_t(m.ne) + 1, 0);
@@ -1634,6 +1655,19 @@ static int append_element_points(OrOperator* op, int kind, const double* values,
     if (F.points_off[m.ne] != count)
       throw Error("coefficient values: one per point of the element rules is expected");
     F.points.assign(values, values + count);
+    op->eforms.push_back(F);
+  });
+}
+int or_operator_append_laplace_tensor(OrOperator* op, const double* tensor, int per_element, int over_integrate,
+                                      double scaling)
+{
+  return guarded([&] {
+    const Mesh& m = *op->space->mesh;
+    const int d = m.ref.dim;
+    if (!tensor || over_integrate < 0)
+      throw Error("invalid diffusion tensor");
+    ElementForm F{EF_LAPLACE, {}, over_integrate, scaling, {}, {}, 0, {}};
+    F.tensor.assign(tensor, tensor + (per_element ? size_t(m.ne) * d * d : size_t(d * d)));
     op->eforms.push_back(F);
   });
 }
@@ -2125,7 +2159,7 @@ int or_adjoint_log_kappa_gradient(const OrSpace* s, const double* kappa, const d
     op.space = const_cast<OrSpace*>(s);
     const Mesh& m = *s->mesh;
     const int ls = s->fe.size; // the stride of the DoF indices (the largest local size on a mixed mesh)
-    ElementForm F{EF_LAPLACE, {}, 0, 1., {}, {}, 0};
+    ElementForm F{EF_LAPLACE, {}, 0, 1., {}, {}, 0, {}};
     F.coeff.constant = 1.;
 #pragma omp parallel
     {

@@ -69,6 +69,10 @@ int or_operator_append_product(OrOperator* op, const double* weight, double weig
                                int over_integrate, double scaling);
 /* the same forms with a coefficient varying inside the elements: `count` values at the points of the element
  * rules of order coeff_order + 2 k + over_integrate (k the space order), element by element */
+/* the Laplace form with a matrix-valued diffusion: d x d values (row-major) per element (per_element 1) or for
+ * all elements (0) */
+int or_operator_append_laplace_tensor(OrOperator* op, const double* tensor, int per_element, int over_integrate,
+                                      double scaling);
 int or_operator_append_laplace_points(OrOperator* op, const double* values, int64_t count, int coeff_order,
                                       int over_integrate, double scaling);
 int or_operator_append_product_points(OrOperator* op, const double* values, int64_t count, int coeff_order,

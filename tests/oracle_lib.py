@@ -103,6 +103,7 @@ def lib():
         "or_adjoint_log_kappa_gradient": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
         "or_error_norms": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]),
         "or_mesh_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+        "or_operator_append_laplace_tensor": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
         "or_operator_append_laplace_points": (
             ctypes.c_int, [_vp, _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
         "or_operator_append_product_points": (
@@ -290,6 +291,13 @@ class Operator:
     def append_product(self, weight=1.0, over_integrate=0, scaling=1.0):
         p, c = self._coef(weight)
         self._check(lib().or_operator_append_product(self.h, p, c, over_integrate, scaling))
+
+    def append_laplace_tensor(self, tensor, over_integrate=0, scaling=1.0):
+        """Laplace form with a matrix-valued diffusion: [d, d] for all elements or [ne, d, d]"""
+        t = np.ascontiguousarray(tensor, np.float64)
+        self._keep = getattr(self, "_keep", []) + [t]
+        self._check(lib().or_operator_append_laplace_tensor(self.h, dptr(t), int(t.ndim == 3), over_integrate,
+                                                             scaling))
 
     def append_laplace_points(self, values, coeff_order, over_integrate=0, scaling=1.0):
         """Laplace form with the diffusion at the points of the element rules of order
