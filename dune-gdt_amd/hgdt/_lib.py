@@ -105,6 +105,7 @@ class SolverOptions(ctypes.Structure):
         ("time_spmv", ctypes.c_int32),
         ("zero_initial_guess", ctypes.c_int32),
         ("true_residual", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
     ]
 
 
@@ -121,6 +122,7 @@ class SolverStats(ctypes.Structure):
         ("compactions", ctypes.c_int64),
         ("true_relres", ctypes.c_double),
         ("column_staged", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
         ("iteration_bytes", ctypes.c_int64),
     ]
 

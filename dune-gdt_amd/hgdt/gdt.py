@@ -997,6 +997,7 @@ class MatrixOperator:
         o.use_graph = int(bool(use_graph))
         o.time_spmv = int(time_spmv)  # True -> every application, k -> every k-th
         o.zero_initial_guess = 0 if (has_source and opts.get("initial_guess", False)) else 1
+        o.restarts = int(opts.get("restarts", 0))  # restarts from the true residual (no reference counterpart)
         st = L.SolverStats()
         status = L.load().hgdt_operator_apply_inverse(self._h, b_ptr, ctypes.c_void_p(x.data_ptr()), ctypes.byref(o),
                                                       ctypes.byref(st), _stream_ptr(stream))

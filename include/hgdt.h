@@ -403,6 +403,10 @@ typedef struct hgdt_solver_options {
   int32_t zero_initial_guess; /* 1: start from x = 0, else from the given x */
   int32_t true_residual;  /* 1 (default): compute |b - A x|_2 / |b|_2 of every instance at exit with the
                              unscaled operator (one more application; reported in true_relres) */
+  int32_t restarts;       /* no reference counterpart; 0 (default): the reference's stopping rule on the
+                             recursive residual only.  k > 0 (with true_residual): while the true residual
+                             of an instance misses rtol, restart CG from the returned x (r = b - A x), at
+                             most k times; iterations add up */
 } hgdt_solver_options;
 
 typedef struct hgdt_solver_stats {
@@ -421,6 +425,7 @@ typedef struct hgdt_solver_stats {
                              (-1 when options.true_residual is 0) */
   int32_t column_staged;  /* 1: the solver's operator application ran with column staging (see
                              hgdt_operator_set_column_staging) */
+  int32_t restarts;       /* restarts from the true residual taken (options.restarts) */
   int64_t iteration_bytes; /* algorithmic bytes of one PCG iteration over all instances: the application
                               (step_bytes), the direction update (36 B per DoF on average over the two
                               parities of the deferred iterate update, none when the application does it)

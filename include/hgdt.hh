@@ -1033,6 +1033,7 @@ struct SolverOptions
   int64_t max_iter = 100000;
   int check_every = 16;
   bool use_graph = false;
+  int restarts = 0; // restarts from the true residual while it misses the precision (no reference counterpart)
 };
 
 struct SolverStats
@@ -1307,6 +1308,7 @@ public:
     check(hgdt_solver_options_default(&o));
     o.rtol = opts.precision, o.max_iter = opts.max_iter, o.check_every = opts.check_every;
     o.use_graph = opts.use_graph ? 1 : 0;
+    o.restarts = opts.restarts;
     hgdt_solver_stats st{};
     const hgdt_status s = hgdt_operator_apply_inverse(handle(), range_device, source.data(), &o, &st, stream);
     last_ = SolverStats{st.converged != 0, st.iterations, st.relres, st.solve_ms, st.true_relres};

@@ -86,3 +86,26 @@ def test_swipdg_512_full_size_assembly(hgdt_mod):
     print(f"  apply ({op.format_info()['format']}) max rel err {yerr:.2e}", flush=True)
     assert op.format_info()["format"] == "block4"
     assert yerr <= 1e-13
+
+
+def test_swipdg_512_restarts_meet_the_true_residual(hgdt_mod):
+    """C3's solve (Q1-DG SWIPDG 512^2, channel kappa, matrix-free) stops on the recursive residual after about
+    15 600 iterations, where the true residual |b - A x| / |b| has drifted above 1e-10; with restarts from the
+    true residual (solver option, default off) the returned x meets 1e-10 in the true residual"""
+    from problems import hgdt_problem
+
+    H = hgdt_mod
+    print("case full size: SWIPDG 512^2 matrix-free, restarts from the true residual", flush=True)
+    kappa = torch.from_numpy(H.fields.channel_field(3, 512, 512)).cuda()
+    _, space, op = hgdt_problem(H, "swipdg", [512, 512], kappa)
+    op.set_format("matrix_free")
+    op.assemble()
+    op.apply_inverse("rhs", opts={"precision": 1e-10})
+    st0 = dict(op.last_solve)
+    op.apply_inverse("rhs", opts={"precision": 1e-10, "restarts": 3})
+    st = dict(op.last_solve)
+    print(f"  without restarts: {st0['iterations']} iterations, true relres {st0['true_relres']:.2e}; with: "
+          f"{st['iterations']} iterations over {st['restarts']} restarts, true relres {st['true_relres']:.2e}")
+    assert st0["restarts"] == 0 and st["true_relres"] <= 1e-10
+    assert (st0["true_relres"] > 1e-10) == (st["restarts"] >= 1)
+    assert st["iterations"] >= st0["iterations"] and int(op.instance_stats()[0][0]) == st["iterations"]
