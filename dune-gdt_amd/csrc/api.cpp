@@ -945,6 +945,25 @@ static void append_face(hgdt_operator op, int kind, double param, const double* 
   forms_changed(*op->op);
 }
 
+hgdt_status hgdt_operator_set_face_tensors(hgdt_operator op, int32_t boundary, const double* diffusion_tensor,
+                                           const double* weight_tensor)
+{
+  return guarded([&] {
+    require(op, HGDT_ERR_INVALID_ARGUMENT, "NULL operator");
+    auto& forms = boundary ? op->op->bforms : op->op->cforms;
+    require(!forms.empty(), HGDT_ERR_INVALID_ARGUMENT, "no intersection form of that kind appended");
+    const Grid& g = *op->op->space->grid;
+    for (const double* t : {diffusion_tensor, weight_tensor})
+      require(!t || tensor_field_symmetric(t, g.ne, g.ref.dim), HGDT_ERR_NOT_SUPPORTED,
+              "the diffusion and weight tensors must be symmetric in every element");
+    if (diffusion_tensor)
+      forms.back().kappa_t = diffusion_tensor;
+    if (weight_tensor)
+      forms.back().omega_t = weight_tensor;
+    forms_changed(*op->op);
+  });
+}
+
 hgdt_status hgdt_operator_append_inner_coupling(hgdt_operator op, double s, const double* kappa, double kv,
                                                 const double* omega, double ov, int32_t over_integrate, double scaling)
 {

@@ -84,6 +84,9 @@ struct FaceParams {
   double kappa_value[MAX_FFORMS];
   const double* omega[MAX_FFORMS];
   double omega_value[MAX_FFORMS];
+  // matrix-valued diffusion / weight: d x d per element (NULL: the scalars)
+  const double* kappa_t[MAX_FFORMS];
+  const double* omega_t[MAX_FFORMS];
   double scaling[MAX_FFORMS];
   QRule rule[MAX_FFORMS];
   const uint8_t* mask[MAX_FFORMS]; // boundary forms: applied on the intersections with a nonzero byte (NULL: all)
@@ -779,9 +782,23 @@ __device__ __forceinline__ void eval_at(const Geo<Traits<GEOM>::D>& g, const dou
     to_physical<D>(g, dphi[i], G[i]);
 }
 
+// (k G) . n, or (K G) . n with a matrix-valued diffusion K (its d x d block of the element; laplace-ipdg.hh:
+// 158-186: (diffusion * grad) * normal)
 template <int D>
-__device__ __forceinline__ double flux(double k, const double* G, const double* n)
+__device__ __forceinline__ double flux(double k, const double* G, const double* n, const double* K = nullptr)
 {
+  if (K) {
+    double v = 0.;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      double kg = 0.;
+#pragma unroll
+      for (int b = 0; b < D; ++b)
+        kg += K[a * D + b] * G[b];
+      v += kg * n[a];
+    }
+    return v;
+  }
   double v = (k * G[0]) * n[0];
 #pragma unroll
   for (int a = 1; a < D; ++a)
@@ -789,14 +806,34 @@ __device__ __forceinline__ double flux(double k, const double* G, const double* 
   return v;
 }
 
+// n . (w n), or n . (W n) with a matrix-valued weight (laplace-ipdg.hh:149-150, ipdg.hh:149-153)
 template <int D>
-__device__ __forceinline__ double n_w_n(const double* n, double w)
+__device__ __forceinline__ double n_w_n(const double* n, double w, const double* W = nullptr)
 {
+  if (W) {
+    double v = 0.;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      double wn = 0.;
+#pragma unroll
+      for (int b = 0; b < D; ++b)
+        wn += W[a * D + b] * n[b];
+      v += n[a] * wn;
+    }
+    return v;
+  }
   double v = n[0] * (w * n[0]);
 #pragma unroll
   for (int a = 1; a < D; ++a)
     v += n[a] * (w * n[a]);
   return v;
+}
+
+// the d x d block of element e of a tensor field (NULL: none)
+template <int D>
+__device__ __forceinline__ const double* tblock(const double* t, int64_t e)
+{
+  return t ? t + e * (D * D) : nullptr;
 }
 
 // one inner intersection per lane: the four blocks (in_in, in_out, out_in, out_out) of every appended
@@ -838,6 +875,8 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
       const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
       const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
       const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+      const double *Kin = tblock<D>(P.kappa_t[fm], e), *Kout = tblock<D>(P.kappa_t[fm], o);
+      const double *Win = tblock<D>(P.omega_t[fm], e), *Wout = tblock<D>(P.omega_t[fm], o);
       const double sc = P.scaling[fm], prm = P.param[fm];
       const QRule R = P.rule[fm];
       for (int q = 0; q < R.npts; ++q) {
@@ -847,8 +886,8 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
         double pin[ND], pout[ND], Gin[ND][D], Gout[ND][D];
         eval_at<GEOM, ORDER>(gin, X, pin, Gin);
         eval_at<GEOM, ORDER>(gout, X, pout, Gout);
-        const double delta_plus = n_w_n<D>(n, wout);
-        const double delta_minus = n_w_n<D>(n, win);
+        const double delta_plus = n_w_n<D>(n, wout, Wout);
+        const double delta_minus = n_w_n<D>(n, win, Win);
         const double factor = F.ie * wq;
         if (P.kind[fm] == FF_INNER_COUPLING) {
           const double wm = delta_plus / (delta_plus + delta_minus);
@@ -856,8 +895,8 @@ __global__ void __launch_bounds__(256) k_inner_local(FaceParams P)
           double fin[ND], fout[ND]; // (kappa grad phi) . n
 #pragma unroll
           for (int i = 0; i < ND; ++i) {
-            fin[i] = flux<D>(kin, Gin[i], n);
-            fout[i] = flux<D>(kout, Gout[i], n);
+            fin[i] = flux<D>(kin, Gin[i], n, Kin);
+            fout[i] = flux<D>(kout, Gout[i], n, Kout);
           }
 #pragma unroll
           for (int r = 0; r < RB; ++r)
@@ -930,6 +969,8 @@ __device__ __forceinline__ void mixed_face_block(const FaceParams& P, int64_t e,
       const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
       const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
       const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+      const double *Kin = tblock<D>(P.kappa_t[fm], e), *Kout = tblock<D>(P.kappa_t[fm], o);
+      const double *Win = tblock<D>(P.omega_t[fm], e), *Wout = tblock<D>(P.omega_t[fm], o);
       const double sc = P.scaling[fm], prm = P.param[fm];
       const QRule R = P.rule[fm];
       for (int q = 0; q < R.npts; ++q) {
@@ -939,8 +980,8 @@ __device__ __forceinline__ void mixed_face_block(const FaceParams& P, int64_t e,
         double pin[NI], pout[NO], Gin[NI][D], Gout[NO][D];
         eval_at<GIN, ORDER>(gin, X, pin, Gin);
         eval_at<GOUT, ORDER>(gout, X, pout, Gout);
-        const double delta_plus = n_w_n<D>(n, wout);
-        const double delta_minus = n_w_n<D>(n, win);
+        const double delta_plus = n_w_n<D>(n, wout, Wout);
+        const double delta_minus = n_w_n<D>(n, win, Win);
         const double factor = F.ie * wq;
         if (P.kind[fm] == FF_INNER_COUPLING) {
           const double wm = delta_plus / (delta_plus + delta_minus);
@@ -948,10 +989,10 @@ __device__ __forceinline__ void mixed_face_block(const FaceParams& P, int64_t e,
           double fin[NI], fout[NO];
 #pragma unroll
           for (int i = 0; i < NI; ++i)
-            fin[i] = flux<D>(kin, Gin[i], n);
+            fin[i] = flux<D>(kin, Gin[i], n, Kin);
 #pragma unroll
           for (int i = 0; i < NO; ++i)
-            fout[i] = flux<D>(kout, Gout[i], n);
+            fout[i] = flux<D>(kout, Gout[i], n, Kout);
 #pragma unroll
           for (int r = 0; r < RB; ++r)
 #pragma unroll
@@ -1062,6 +1103,7 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
       continue;
     const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
     const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double *Kin = tblock<D>(P.kappa_t[fm], e), *Win = tblock<D>(P.omega_t[fm], e);
     const double sc = P.scaling[fm], prm = P.param[fm];
     const QRule R = P.rule[fm];
     for (int q = 0; q < R.npts; ++q) {
@@ -1075,14 +1117,14 @@ __global__ void __launch_bounds__(256) k_boundary_local(FaceParams P)
         double fin[ND];
 #pragma unroll
         for (int i = 0; i < ND; ++i)
-          fin[i] = flux<D>(kin, Gin[i], n);
+          fin[i] = flux<D>(kin, Gin[i], n, Kin);
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
           for (int j = 0; j < ND; ++j)
             B[i][j] += sc * ((-1.0 * fin[j] * pin[i] + -1.0 * prm * pin[j] * fin[i]) * factor);
       } else {
-        const double pen = (prm * n_w_n<D>(n, win)) / F.h;
+        const double pen = (prm * n_w_n<D>(n, win, Win)) / F.h;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
 #pragma unroll
@@ -1227,6 +1269,8 @@ __global__ void __launch_bounds__(256) k_inner_rows_q2hex(FaceParams P)
     const double kout = P.kappa[fm] ? P.kappa[fm][o] : P.kappa_value[fm];
     const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
     const double wout = P.omega[fm] ? P.omega[fm][o] : P.omega_value[fm];
+    const double *Kin = tblock<D>(P.kappa_t[fm], e), *Kout = tblock<D>(P.kappa_t[fm], o);
+    const double *Win = tblock<D>(P.omega_t[fm], e), *Wout = tblock<D>(P.omega_t[fm], o);
     const double sc = P.scaling[fm], prm = P.param[fm];
     const QRule R = P.rule[fm];
     for (int q = 0; q < R.npts; ++q) {
@@ -1236,8 +1280,8 @@ __global__ void __launch_bounds__(256) k_inner_rows_q2hex(FaceParams P)
       double li[3][3], dli[3][3], lo[3][3], dlo[3][3];
       q2hex_factors(gin, X, li, dli);
       q2hex_factors(gout, X, lo, dlo);
-      const double delta_plus = n_w_n<D>(n, wout);
-      const double delta_minus = n_w_n<D>(n, win);
+      const double delta_plus = n_w_n<D>(n, wout, Wout);
+      const double delta_minus = n_w_n<D>(n, win, Win);
       const double factor = F.ie * wq;
       const bool coupling = P.kind[fm] == FF_INNER_COUPLING;
       const double wm = delta_plus / (delta_plus + delta_minus);
@@ -1249,12 +1293,12 @@ __global__ void __launch_bounds__(256) k_inner_rows_q2hex(FaceParams P)
         q2hex_basis(gout, lo, dlo, i, pr, Gr);
       else
         q2hex_basis(gin, li, dli, i, pr, Gr);
-      const double fr = flux<D>(outside_rows ? kout : kin, Gr, n);
+      const double fr = flux<D>(outside_rows ? kout : kin, Gr, n, outside_rows ? Kout : Kin);
       for (int j = 0; j < ND; ++j) {
         double pin, Gin[3], pout, Gout[3];
         q2hex_basis(gin, li, dli, j, pin, Gin);
         q2hex_basis(gout, lo, dlo, j, pout, Gout);
-        const double fin = flux<D>(kin, Gin, n), fout = flux<D>(kout, Gout, n);
+        const double fin = flux<D>(kin, Gin, n, Kin), fout = flux<D>(kout, Gout, n, Kout);
         double va, vb;
         if (!outside_rows) {
           // in_in (i, j): from the upper triangle when symmetric; in_out (i, j)
@@ -1320,6 +1364,7 @@ __global__ void __launch_bounds__(256) k_boundary_rows_q2hex(FaceParams P)
       continue;
     const double kin = P.kappa[fm] ? P.kappa[fm][e] : P.kappa_value[fm];
     const double win = P.omega[fm] ? P.omega[fm][e] : P.omega_value[fm];
+    const double *Kin = tblock<D>(P.kappa_t[fm], e), *Win = tblock<D>(P.omega_t[fm], e);
     const double sc = P.scaling[fm], prm = P.param[fm];
     const QRule R = P.rule[fm];
     for (int q = 0; q < R.npts; ++q) {
@@ -1330,13 +1375,13 @@ __global__ void __launch_bounds__(256) k_boundary_rows_q2hex(FaceParams P)
       q2hex_factors(gin, X, l, dl);
       double pr, Gr[3];
       q2hex_basis(gin, l, dl, i, pr, Gr);
-      const double fr = flux<D>(kin, Gr, n);
+      const double fr = flux<D>(kin, Gr, n, Kin);
       const double factor = F.ie * wq;
-      const double pen = (prm * n_w_n<D>(n, win)) / F.h;
+      const double pen = (prm * n_w_n<D>(n, win, Win)) / F.h;
       for (int j = 0; j < ND; ++j) {
         double pj, Gj[3];
         q2hex_basis(gin, l, dl, j, pj, Gj);
-        const double fj = flux<D>(kin, Gj, n);
+        const double fj = flux<D>(kin, Gj, n, Kin);
         // entry (r, c) of k_boundary_local: -fin[c] pin[r] - prm pin[c] fin[r] or pen pin[c] pin[r]; the stored
         // (i, j) is (r, c) = (j, i) when symmetric and i > j
         const bool sw = P.symmetric && i > j;
@@ -1813,6 +1858,7 @@ static void build_tables(const Operator& op, QTable& etab, QTable& ltab, std::ve
       const auto& D = forms[f];
       F.kind[f] = D.kind, F.param[f] = D.param, F.kappa[f] = D.kappa, F.kappa_value[f] = D.kappa_value;
       F.omega[f] = D.omega, F.omega_value[f] = D.omega_value, F.scaling[f] = D.scaling, F.mask[f] = D.mask;
+      F.kappa_t[f] = D.kappa_t, F.omega_t[f] = D.omega_t;
       // coupling / penalty orders: 0 + 0 + test + ansatz (laplace-ipdg.hh:95-105, ipdg.hh:100-109,245-250)
       F.rule[f] = ltab.add(face_rule(g.ref, 0 + k + k + D.over_integrate));
     }

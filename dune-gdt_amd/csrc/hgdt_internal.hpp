@@ -248,6 +248,10 @@ struct FaceFormDesc {
   // boundary forms: the boundary intersections (walk order) the form is applied on, one byte each (device,
   // owned by the operator); NULL: every boundary intersection
   const uint8_t* mask;
+  // matrix-valued diffusion / weight (GridFunction<E, d, d>): d x d per element (device, row-major, symmetric);
+  // NULL: the scalars above
+  const double* kappa_t = nullptr;
+  const double* omega_t = nullptr;
 };
 struct SourceDesc {
   int kind;

@@ -844,6 +844,10 @@ void mf_setup(Operator& op)
     unsupported("Dirichlet constraints as unit rows and columns (the default of DirichletConstraints::apply)");
   if (op.eforms.size() > size_t(MF_MAXF) || op.cforms.size() > size_t(MF_MAXF) || op.bforms.size() > size_t(MF_MAXF))
     unsupported("too many forms appended");
+  for (const auto* forms : {&op.cforms, &op.bforms})
+    for (const auto& f : *forms)
+      if (f.kappa_t || f.omega_t)
+        unsupported("intersection coefficients are scalar (not matrix-valued)");
   for (const auto& f : op.eforms)
     if (f.points || f.tensor_kind)
       unsupported("element coefficients are scalar, per element or constant (not matrix-valued or given at "

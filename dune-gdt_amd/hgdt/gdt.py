@@ -449,6 +449,16 @@ def _gf(v):
     return v if isinstance(v, GridFunction) else GridFunction(v)
 
 
+def _face_coefficient(v):
+    """coefficient of an intersection integrand: a GridFunction, or a symmetric matrix-valued TensorGridFunction
+    (GridFunction<E, d, d>, laplace-ipdg.hh:47-61, ipdg.hh:59-72)"""
+    if isinstance(v, TensorGridFunction):
+        return v
+    if getattr(v, "ndim", 1) > 1 or (isinstance(v, (list, tuple)) and np.ndim(v) > 1):
+        return TensorGridFunction(v)
+    return _gf(v)
+
+
 def _element_coefficient(v):
     """coefficient of an element integrand: a GridFunction (constant or per element), or one that varies inside
     the elements (GenericFunction, QuadraturePointFunction), evaluated at the quadrature points"""
@@ -511,14 +521,14 @@ class LocalLaplaceIPDGInnerCoupling(_Integrand):
 
     def __init__(self, symmetry_prefactor, diffusion, weight=1.0, logging_prefix=""):
         self.symmetry_prefactor = float(symmetry_prefactor)
-        self.diffusion, self.weight = _gf(diffusion), _gf(weight)
+        self.diffusion, self.weight = _face_coefficient(diffusion), _face_coefficient(weight)
 
 
 class LocalIPDGInnerPenalty(_Integrand):
     """LocalIPDGIntegrands::InnerPenalty, local/integrands/ipdg.hh:44-183 (face diameter as h)"""
 
     def __init__(self, penalty, weight=1.0, logging_prefix=""):
-        self.penalty, self.weight = float(penalty), _gf(weight)
+        self.penalty, self.weight = float(penalty), _face_coefficient(weight)
 
 
 class LocalLaplaceIPDGDirichletCoupling(_Integrand):
@@ -527,7 +537,7 @@ class LocalLaplaceIPDGDirichletCoupling(_Integrand):
     unary one with the Dirichlet data (a constant or a GenericFunction)"""
 
     def __init__(self, symmetry_prefactor, diffusion, dirichlet_data=0.0, logging_prefix=""):
-        self.symmetry_prefactor, self.diffusion = float(symmetry_prefactor), _gf(diffusion)
+        self.symmetry_prefactor, self.diffusion = float(symmetry_prefactor), _face_coefficient(diffusion)
         self.dirichlet_data = dirichlet_data
 
 
@@ -535,7 +545,7 @@ class LocalIPDGBoundaryPenalty(_Integrand):
     """LocalIPDGIntegrands::BoundaryPenalty, local/integrands/ipdg.hh:186-291"""
 
     def __init__(self, penalty, weight=1.0, logging_prefix=""):
-        self.penalty, self.weight = float(penalty), _gf(weight)
+        self.penalty, self.weight = float(penalty), _face_coefficient(weight)
 
     def with_ansatz(self, function):
         """LocalBinaryToUnaryIntersectionIntegrand (local/integrands/conversion.hh:133-221)"""
@@ -775,7 +785,26 @@ class MatrixOperator:
         L.call("hgdt_operator_set_boundary_filter", self._h,
                None if mask is None else ctypes.c_void_p(mask.ctypes.data), 0 if mask is None else len(mask))
 
+    def _tensor_field(self, t):
+        """the per-element device tensor (elements, d, d) of a TensorGridFunction (a constant one expanded)"""
+        d, ne = self.grid.dimension, self.grid.num_elements
+        f = t.field if t.field is not None else torch.from_numpy(
+            np.ascontiguousarray(np.broadcast_to(t.constant, (ne, d, d)))).to(_device())
+        if tuple(f.shape) != (ne, d, d):
+            raise L.shapes_do_not_match(2, f"diffusion tensor of shape {tuple(f.shape)}, expected ({ne}, {d}, {d})")
+        self._keep.append(f)
+        return ctypes.c_void_p(f.data_ptr())
+
+    def _face_tensors(self, boundary, diffusion, weight):
+        """matrix-valued coefficients of the intersection form just appended"""
+        kt = self._tensor_field(diffusion) if isinstance(diffusion, TensorGridFunction) else None
+        wt = self._tensor_field(weight) if isinstance(weight, TensorGridFunction) else None
+        if kt is not None or wt is not None:
+            L.call("hgdt_operator_set_face_tensors", self._h, int(boundary), kt, wt)
+
     def _field(self, gf):
+        if isinstance(gf, TensorGridFunction):
+            return None, 1.0  # replaced by the tensor (_face_tensors)
         if not isinstance(gf, GridFunction):
             raise L.not_supported(9, "intersection integrands take constant or per-element coefficients")
         ptr, val = gf.args(self.grid)
@@ -837,9 +866,11 @@ class MatrixOperator:
             wp, wv = self._field(integrand.weight)
             L.call("hgdt_operator_append_inner_coupling", self._h, integrand.symmetry_prefactor, kp, kv, wp, wv,
                    over, scaling)
+            self._face_tensors(False, integrand.diffusion, integrand.weight)
         elif isinstance(integrand, LocalIPDGInnerPenalty):
             wp, wv = self._field(integrand.weight)
             L.call("hgdt_operator_append_inner_penalty", self._h, integrand.penalty, wp, wv, over, scaling)
+            self._face_tensors(False, None, integrand.weight)
         else:
             raise L.not_supported(9, f"{type(integrand).__name__} is not a coupling integrand of this path")
 
@@ -848,9 +879,11 @@ class MatrixOperator:
             kp, kv = self._field(integrand.diffusion)
             L.call("hgdt_operator_append_dirichlet_coupling", self._h, integrand.symmetry_prefactor, kp, kv, over,
                    scaling)
+            self._face_tensors(True, integrand.diffusion, None)
         elif isinstance(integrand, LocalIPDGBoundaryPenalty):
             wp, wv = self._field(integrand.weight)
             L.call("hgdt_operator_append_boundary_penalty", self._h, integrand.penalty, wp, wv, over, scaling)
+            self._face_tensors(True, None, integrand.weight)
         else:
             raise L.not_supported(9, f"{type(integrand).__name__} is not a boundary integrand of this path")
 
@@ -863,6 +896,8 @@ class MatrixOperator:
             kind, param, coeff, g = it.kind, it.param, it.coeff, it.function
         g_order = g.order() if isinstance(g, GenericFunction) else 0
         order = (g_order + max(k - 1, 0) + over) if kind == L.BOUNDARY_DIRICHLET_COUPLING else (k + g_order + over)
+        if isinstance(coeff, TensorGridFunction):
+            raise L.not_supported(9, "boundary functionals take scalar coefficients")
         cp, cv = self._field(coeff)
         if isinstance(g, GenericFunction):
             pts = self.grid.boundary_quadrature_points(order)

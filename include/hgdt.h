@@ -285,6 +285,13 @@ hgdt_status hgdt_operator_append_dirichlet_coupling(hgdt_operator op, double sym
  * local/integrands/ipdg.hh:201-282 */
 hgdt_status hgdt_operator_append_boundary_penalty(hgdt_operator op, double penalty, const double* weight_field,
                                                   double weight_value, int32_t over_integrate, double scaling);
+/* matrix-valued diffusion and weight (GridFunction<E, d, d>) of the last appended inner (boundary 0) or
+ * boundary (1) intersection form, replacing its scalar coefficients (laplace-ipdg.hh:47-61,145-186:
+ * delta = n . (weight n), fluxes (diffusion grad phi) . n; ipdg.hh:59-72,149-153): device arrays of d x d
+ * values per element (row-major, symmetric: checked), NULL to keep a scalar.  Not for the matrix-free format;
+ * boundary functionals keep scalar coefficients. */
+hgdt_status hgdt_operator_set_face_tensors(hgdt_operator op, int32_t boundary, const double* diffusion_tensor,
+                                           const double* weight_tensor);
 /* right-hand side functional assembled in the same walk:
  * VectorBasedFunctional::append(LocalElementIntegralFunctional(LocalProductIntegrand().with_ansatz(f)))
  * (functionals/vector-based.hh:132-308, local/functionals/integrals.hh:72-97,

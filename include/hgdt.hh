@@ -1249,6 +1249,18 @@ public:
     }
     return *this;
   }
+  // matrix-valued diffusion / weight (per-element TensorGridFunctions, symmetric) of the intersection form
+  // appended last (inner: boundary false), in place of its scalar coefficients (laplace-ipdg.hh:47-61)
+  MatrixOperator& set_face_tensors(bool boundary, const TensorGridFunction* diffusion,
+                                   const TensorGridFunction* weight = nullptr)
+  {
+    for (const TensorGridFunction* t : {diffusion, weight})
+      if (t && t->storage())
+        kept_.push_back(t->storage());
+    check(hgdt_operator_set_face_tensors(handle(), boundary ? 1 : 0, diffusion ? diffusion->field() : nullptr,
+                                         weight ? weight->field() : nullptr));
+    return *this;
+  }
   MatrixOperator& append(const DirichletConstraints& constraints)
   {
     const auto mask = constraints.faces.mask(space_.grid_view());

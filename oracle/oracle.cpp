@@ -735,6 +735,8 @@ struct FaceForm {
   // boundary forms: the intersections the form is applied on, by boundary intersection in walk order
   // (ApplyOn::CustomBoundaryIntersections(boundary_info, type)); empty: every boundary intersection
   std::vector<uint8_t> mask;
+  // matrix-valued diffusion and weight (GridFunction<E, d, d>): d x d per element; empty: the scalars above
+  std::vector<double> kappa_t, omega_t;
 };
 struct Source {
   int kind;
@@ -793,6 +795,23 @@ inline double kgrad_dot(double kappa, const double* g, const double* h, int d)
     s += (kappa * g[a]) * h[a];
   return s;
 }
+// (k g) . n, or (K g) . n with the element's d x d block K of a matrix-valued diffusion (laplace-ipdg.hh:158-186)
+inline double flux_n(double k, const double* K, const double* g, const double* n, int d);
+// n . (w n), or n . (W n) (laplace-ipdg.hh:149-150, ipdg.hh:149-153)
+inline double n_w_n(double w, const double* W, const double* n, int d)
+{
+  double v = 0.;
+  for (int a = 0; a < d; ++a) {
+    double wn = 0.;
+    if (W)
+      for (int b = 0; b < d; ++b)
+        wn += W[a * d + b] * n[b];
+    else
+      wn = w * n[a];
+    v += n[a] * wn;
+  }
+  return v;
+}
 // (K g) . h for a matrix-valued weight K (d x d, row-major): (weight * ansatz_grad) * test_grad,
 // local/integrands/laplace.hh:99-101
 inline double kmat_grad_dot(const double* K, const double* g, const double* h, int d)
@@ -805,6 +824,11 @@ inline double kmat_grad_dot(const double* K, const double* g, const double* h, i
     s += kg * h[a];
   }
   return s;
+}
+
+inline double flux_n(double k, const double* K, const double* g, const double* n, int d)
+{
+  return K ? kmat_grad_dot(K, g, n, d) : kgrad_dot(k, g, n, d);
 }
 
 struct Operator {
@@ -1077,38 +1101,41 @@ struct Operator {
     const double* n = fg.normal;
     const double kin = F.kappa.at(e), kout = F.kappa.at(o);
     const double win = F.omega.at(e), wout = F.omega.at(o);
+    const size_t dd = size_t(d) * d;
+    const double* Kin = F.kappa_t.empty() ? nullptr : &F.kappa_t[size_t(e) * dd];
+    const double* Kout = F.kappa_t.empty() ? nullptr : &F.kappa_t[size_t(o) * dd];
+    const double* Win = F.omega_t.empty() ? nullptr : &F.omega_t[size_t(e) * dd];
+    const double* Wout = F.omega_t.empty() ? nullptr : &F.omega_t[size_t(o) * dd];
     for (size_t qp = 0; qp < fg.w.size(); ++qp) {
       for (int k = 0; k < 4; ++k)
         std::fill(iv[k].begin(), iv[k].end(), 0.);
       bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
       bout.evaluate(fg.xout[qp].data()), bout.jacobians(fg.xout[qp].data());
       // delta = n . (omega n)
-      double delta_plus = 0., delta_minus = 0.;
-      for (int a = 0; a < d; ++a)
-        delta_plus += n[a] * (wout * n[a]), delta_minus += n[a] * (win * n[a]);
+      const double delta_plus = n_w_n(wout, Wout, n, d), delta_minus = n_w_n(win, Win, n, d);
       if (F.kind == FF_INNER_COUPLING) {
         const double s = F.param;
         const double weight_minus = delta_plus / (delta_plus + delta_minus);
         const double weight_plus = delta_minus / (delta_plus + delta_minus);
         for (int i = 0; i < li; ++i) {
-          const double kgt_in = kgrad_dot(kin, &bin.grads[3 * i], n, d);
+          const double kgt_in = flux_n(kin, Kin, &bin.grads[3 * i], n, d);
           for (int j = 0; j < li; ++j) {
-            iv[0][i * li + j] += -1.0 * weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
+            iv[0][i * li + j] += -1.0 * weight_minus * flux_n(kin, Kin, &bin.grads[3 * j], n, d) * bin.vals[i];
             iv[0][i * li + j] += -1.0 * s * weight_minus * bin.vals[j] * kgt_in;
           }
           for (int j = 0; j < lo; ++j) {
-            iv[1][i * lo + j] += -1.0 * weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bin.vals[i];
+            iv[1][i * lo + j] += -1.0 * weight_plus * flux_n(kout, Kout, &bout.grads[3 * j], n, d) * bin.vals[i];
             iv[1][i * lo + j] += s * weight_minus * bout.vals[j] * kgt_in;
           }
         }
         for (int i = 0; i < lo; ++i) {
-          const double kgt_out = kgrad_dot(kout, &bout.grads[3 * i], n, d);
+          const double kgt_out = flux_n(kout, Kout, &bout.grads[3 * i], n, d);
           for (int j = 0; j < li; ++j) {
-            iv[2][i * li + j] += weight_minus * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bout.vals[i];
+            iv[2][i * li + j] += weight_minus * flux_n(kin, Kin, &bin.grads[3 * j], n, d) * bout.vals[i];
             iv[2][i * li + j] += -1.0 * s * weight_plus * bin.vals[j] * kgt_out;
           }
           for (int j = 0; j < lo; ++j) {
-            iv[3][i * lo + j] += weight_plus * kgrad_dot(kout, &bout.grads[3 * j], n, d) * bout.vals[i];
+            iv[3][i * lo + j] += weight_plus * flux_n(kout, Kout, &bout.grads[3 * j], n, d) * bout.vals[i];
             iv[3][i * lo + j] += s * weight_plus * bout.vals[j] * kgt_out;
           }
         }
@@ -1144,22 +1171,23 @@ struct Operator {
     std::vector<double> iv(ls * ls);
     const double* n = fg.normal;
     const double kin = F.kappa.at(e), win = F.omega.at(e);
+    const size_t dd = size_t(d) * d;
+    const double* Kin = F.kappa_t.empty() ? nullptr : &F.kappa_t[size_t(e) * dd];
+    const double* Win = F.omega_t.empty() ? nullptr : &F.omega_t[size_t(e) * dd];
     for (size_t qp = 0; qp < fg.w.size(); ++qp) {
       std::fill(iv.begin(), iv.end(), 0.);
       bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
       if (F.kind == FF_DIRICHLET_COUPLING) {
         const double s = F.param;
         for (int i = 0; i < ls; ++i) {
-          const double kgt = kgrad_dot(kin, &bin.grads[3 * i], n, d);
+          const double kgt = flux_n(kin, Kin, &bin.grads[3 * i], n, d);
           for (int j = 0; j < ls; ++j) {
-            iv[i * ls + j] += -1.0 * kgrad_dot(kin, &bin.grads[3 * j], n, d) * bin.vals[i];
+            iv[i * ls + j] += -1.0 * flux_n(kin, Kin, &bin.grads[3 * j], n, d) * bin.vals[i];
             iv[i * ls + j] += -1.0 * s * bin.vals[j] * kgt;
           }
         }
       } else {
-        double nwn = 0.;
-        for (int a = 0; a < d; ++a)
-          nwn += n[a] * (win * n[a]);
+        const double nwn = n_w_n(win, Win, n, d);
         const double penalty = (F.param * nwn) / fg.diameter;
         for (int i = 0; i < ls; ++i)
           for (int j = 0; j < ls; ++j)
@@ -1685,7 +1713,7 @@ static int append_face(OrOperator* op, int kind, double param, const double* kap
                        double oc, int over_integrate, double scaling)
 {
   return guarded([&] {
-    FaceForm F{kind, param, {}, {}, over_integrate, scaling, {}};
+    FaceForm F{kind, param, {}, {}, over_integrate, scaling, {}, {}, {}};
     F.kappa.set(kappa, kc, op->space->mesh->ne);
     F.omega.set(omega, oc, op->space->mesh->ne);
     if (kind == FF_INNER_COUPLING || kind == FF_INNER_PENALTY)
@@ -1725,6 +1753,21 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
     Source S{kind, {}, f_order, over_integrate};
     S.f.set(f, f_const, op->space->mesh->ne);
     op->sources.push_back(S);
+  });
+}
+// matrix-valued diffusion / weight of the last appended inner (boundary = 0) or boundary (1) intersection form:
+// d x d per element (NULL: keep the scalar)
+int or_operator_set_face_tensors(OrOperator* op, int boundary, const double* kappa_t, const double* omega_t)
+{
+  return guarded([&] {
+    auto& forms = boundary ? op->bforms : op->cforms;
+    if (forms.empty())
+      throw Error("no intersection form appended");
+    const size_t n = size_t(op->space->mesh->ne) * op->space->mesh->ref.dim * op->space->mesh->ref.dim;
+    if (kappa_t)
+      forms.back().kappa_t.assign(kappa_t, kappa_t + n);
+    if (omega_t)
+      forms.back().omega_t.assign(omega_t, omega_t + n);
   });
 }
 int or_operator_set_dirichlet_mode(OrOperator* op, int only_clear, int ensure_symmetry)

@@ -93,6 +93,9 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
 /* constrain every boundary DoF (AllDirichletBoundaryInfo) -- only meaningful for CG spaces */
 int or_operator_set_dirichlet(OrOperator* op, int enabled);
 /* DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry), tools/dirichlet-constraints.hh:152-184 */
+/* matrix-valued diffusion / weight (d x d per element, NULL: scalar) of the last appended inner (boundary 0) or
+ * boundary (1) intersection form (laplace-ipdg.hh:47-61, ipdg.hh:59-72) */
+int or_operator_set_face_tensors(OrOperator* op, int boundary, const double* kappa_t, const double* omega_t);
 int or_operator_set_dirichlet_mode(OrOperator* op, int only_clear, int ensure_symmetry);
 
 int or_operator_assemble(OrOperator* op, int nthreads);

@@ -84,6 +84,7 @@ def lib():
             [_vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int],
         ),
         "or_operator_set_dirichlet": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "or_operator_set_face_tensors": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "or_operator_set_dirichlet_mode": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "or_operator_assemble": (ctypes.c_int, [_vp, ctypes.c_int]),
         "or_operator_values": (None, [_vp, _dp]),
@@ -366,6 +367,15 @@ class Operator:
 
     def set_dirichlet(self, enabled=True):
         self._check(lib().or_operator_set_dirichlet(self.h, int(enabled)))
+
+    def set_face_tensors(self, boundary, kappa=None, omega=None):
+        """matrix-valued diffusion / weight ([ne, d, d]) of the last appended inner (boundary False) or boundary
+        intersection form"""
+        kt = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        wt = None if omega is None else np.ascontiguousarray(omega, np.float64)
+        self._keep = getattr(self, "_keep", []) + [kt, wt]
+        self._check(lib().or_operator_set_face_tensors(self.h, int(bool(boundary)), None if kt is None else dptr(kt),
+                                                        None if wt is None else dptr(wt)))
 
     def set_dirichlet_mode(self, only_clear=False, ensure_symmetry=True):
         self._check(lib().or_operator_set_dirichlet_mode(self.h, int(only_clear), int(ensure_symmetry)))
