@@ -945,6 +945,22 @@ static void append_face(hgdt_operator op, int kind, double param, const double* 
   forms_changed(*op->op);
 }
 
+hgdt_status hgdt_operator_set_boundary_source_tensor(hgdt_operator op, const double* tensor)
+{
+  return guarded([&] {
+    require(op && tensor, HGDT_ERR_INVALID_ARGUMENT, "NULL argument");
+    auto& bs = op->op->bsources;
+    require(!bs.empty(), HGDT_ERR_INVALID_ARGUMENT, "no boundary functional appended");
+    require(bs.back().kind != HGDT_BOUNDARY_PRODUCT, HGDT_ERR_NOT_SUPPORTED,
+            "the boundary product term takes a scalar weight");
+    const Grid& g = *op->op->space->grid;
+    require(tensor_field_symmetric(tensor, g.ne, g.ref.dim), HGDT_ERR_NOT_SUPPORTED,
+            "the tensor must be symmetric in every element");
+    bs.back().coeff_t = tensor;
+    forms_changed(*op->op);
+  });
+}
+
 hgdt_status hgdt_operator_set_face_tensors(hgdt_operator op, int32_t boundary, const double* diffusion_tensor,
                                            const double* weight_tensor)
 {

@@ -122,6 +122,7 @@ struct BoundaryRhsParams {
   double value[MAX_BSOURCES];
   QRule rule[MAX_BSOURCES];
   const uint8_t* mask[MAX_BSOURCES]; // applied on the intersections with a nonzero byte (NULL: all)
+  const double* coeff_t[MAX_BSOURCES]; // matrix-valued coefficient, d x d per element (NULL: coeff)
   double* staging;
   int64_t base;
   const int8_t* etype; // as FaceParams
@@ -1172,6 +1173,7 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
     if (P.mask[tm] && !P.mask[tm][fi])
       continue;
     const double c = P.coeff[tm] ? P.coeff[tm][e] : P.coeff_value[tm];
+    const double* Ct = tblock<D>(P.coeff_t[tm], e);
     const double prm = P.param[tm];
     const QRule R = P.rule[tm];
     for (int q = 0; q < R.npts; ++q) {
@@ -1185,9 +1187,9 @@ __global__ void __launch_bounds__(256) k_boundary_rhs(BoundaryRhsParams P)
       if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING) {
 #pragma unroll
         for (int i = 0; i < ND; ++i)
-          b[i] += (-1.0 * prm * g * flux<D>(c, Gin[i], n)) * factor;
+          b[i] += (-1.0 * prm * g * flux<D>(c, Gin[i], n, Ct)) * factor;
       } else if (P.kind[tm] == HGDT_BOUNDARY_PENALTY) {
-        const double pen = (prm * n_w_n<D>(n, c)) / F.h;
+        const double pen = (prm * n_w_n<D>(n, c, Ct)) / F.h;
 #pragma unroll
         for (int i = 0; i < ND; ++i)
           b[i] += (pen * g * pin[i]) * factor;
@@ -1419,6 +1421,7 @@ __global__ void __launch_bounds__(256) k_boundary_rhs_rows_q2hex(BoundaryRhsPara
     if (P.mask[tm] && !P.mask[tm][fi])
       continue;
     const double c = P.coeff[tm] ? P.coeff[tm][e] : P.coeff_value[tm];
+    const double* Ct = tblock<D>(P.coeff_t[tm], e);
     const double prm = P.param[tm];
     const QRule R = P.rule[tm];
     for (int q = 0; q < R.npts; ++q) {
@@ -1432,9 +1435,9 @@ __global__ void __launch_bounds__(256) k_boundary_rhs_rows_q2hex(BoundaryRhsPara
       const double g = P.data[tm] ? P.data[tm][fi * R.npts + q] : P.value[tm];
       const double factor = F.ie * wq;
       if (P.kind[tm] == HGDT_BOUNDARY_DIRICHLET_COUPLING)
-        b += (-1.0 * prm * g * flux<D>(c, Gr, n)) * factor;
+        b += (-1.0 * prm * g * flux<D>(c, Gr, n, Ct)) * factor;
       else if (P.kind[tm] == HGDT_BOUNDARY_PENALTY)
-        b += (((prm * n_w_n<D>(n, c)) / F.h) * g * pr) * factor;
+        b += (((prm * n_w_n<D>(n, c, Ct)) / F.h) * g * pr) * factor;
       else
         b += ((c * g) * pr) * factor;
     }
@@ -2031,6 +2034,7 @@ void launch_assembly(Operator& op, hipStream_t stream, bool matrix)
     for (int t = 0; t < BR.nterms; ++t) {
       const auto& S = op.bsources[t];
       BR.kind[t] = S.kind, BR.param[t] = S.param, BR.coeff[t] = S.coeff, BR.coeff_value[t] = S.coeff_value;
+      BR.coeff_t[t] = S.coeff_t;
       BR.data[t] = S.data, BR.value[t] = S.value, BR.rule[t] = B.brule[t], BR.mask[t] = S.mask;
     }
     BR.etype = g.mixed ? g.d_etype.ptr : nullptr, BR.ls = op.space->local_size, BR.cstride = g.ref.nverts;

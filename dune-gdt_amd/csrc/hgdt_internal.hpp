@@ -272,6 +272,8 @@ struct BoundarySourceDesc {
   int g_order;
   int over_integrate;
   const uint8_t* mask; // as FaceFormDesc::mask
+  // matrix-valued diffusion (coupling) or weight (penalty): d x d per element (device, symmetric); NULL: coeff
+  const double* coeff_t = nullptr;
 };
 
 // launch configuration of one tunable kernel

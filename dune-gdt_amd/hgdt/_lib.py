@@ -176,6 +176,7 @@ SIGNATURES = {
     "hgdt_operator_append_laplace": (ctypes.c_int, [_vp, _vp, _dbl, _i32, _dbl]),
     "hgdt_operator_append_product": (ctypes.c_int, [_vp, _vp, _dbl, _i32, _dbl]),
     "hgdt_operator_append_laplace_points": (ctypes.c_int, [_vp, _vp, _i32, _i32, _dbl]),
+    "hgdt_operator_set_boundary_source_tensor": (ctypes.c_int, [_vp, _vp]),
     "hgdt_operator_set_face_tensors": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
     "hgdt_operator_append_laplace_tensor": (ctypes.c_int, [_vp, _vp, _vp, _i32, _dbl]),
     "hgdt_operator_append_product_points": (ctypes.c_int, [_vp, _vp, _i32, _i32, _dbl]),

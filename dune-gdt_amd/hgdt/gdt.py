@@ -896,8 +896,6 @@ class MatrixOperator:
             kind, param, coeff, g = it.kind, it.param, it.coeff, it.function
         g_order = g.order() if isinstance(g, GenericFunction) else 0
         order = (g_order + max(k - 1, 0) + over) if kind == L.BOUNDARY_DIRICHLET_COUPLING else (k + g_order + over)
-        if isinstance(coeff, TensorGridFunction):
-            raise L.not_supported(9, "boundary functionals take scalar coefficients")
         cp, cv = self._field(coeff)
         if isinstance(g, GenericFunction):
             pts = self.grid.boundary_quadrature_points(order)
@@ -908,6 +906,8 @@ class MatrixOperator:
                    ctypes.c_void_p(vals.data_ptr()), 0.0, g_order, over)
         else:
             L.call("hgdt_operator_append_boundary_source", self._h, kind, param, cp, cv, None, float(g), 0, over)
+        if isinstance(coeff, TensorGridFunction):
+            L.call("hgdt_operator_set_boundary_source_tensor", self._h, self._tensor_field(coeff))
 
     def _append_source(self, lf):
         if isinstance(lf, LocalIntersectionIntegralFunctional):

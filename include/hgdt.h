@@ -289,9 +289,13 @@ hgdt_status hgdt_operator_append_boundary_penalty(hgdt_operator op, double penal
  * boundary (1) intersection form, replacing its scalar coefficients (laplace-ipdg.hh:47-61,145-186:
  * delta = n . (weight n), fluxes (diffusion grad phi) . n; ipdg.hh:59-72,149-153): device arrays of d x d
  * values per element (row-major, symmetric: checked), NULL to keep a scalar.  Not for the matrix-free format;
- * boundary functionals keep scalar coefficients. */
+ * the boundary functionals take theirs through hgdt_operator_set_boundary_source_tensor. */
 hgdt_status hgdt_operator_set_face_tensors(hgdt_operator op, int32_t boundary, const double* diffusion_tensor,
                                            const double* weight_tensor);
+/* matrix-valued coefficient of the last appended boundary functional (the diffusion of the Dirichlet coupling
+ * term, (K grad psi_i) . n; the weight of the boundary penalty term, n . (W n)), replacing its scalar coefficient:
+ * a device array of d x d values per element (row-major, symmetric: checked).  Not for the product term. */
+hgdt_status hgdt_operator_set_boundary_source_tensor(hgdt_operator op, const double* tensor);
 /* right-hand side functional assembled in the same walk:
  * VectorBasedFunctional::append(LocalElementIntegralFunctional(LocalProductIntegrand().with_ansatz(f)))
  * (functionals/vector-based.hh:132-308, local/functionals/integrals.hh:72-97,

@@ -755,6 +755,7 @@ struct BoundarySource {
   int g_order = 0;
   int over_integrate = 0;
   std::vector<uint8_t> mask; // as FaceForm::mask
+  std::vector<double> coeff_t; // matrix-valued diffusion / weight, d x d per element; empty: coeff
 };
 
 // bound local basis: values and physical gradients at a reference point
@@ -1220,6 +1221,7 @@ struct Operator {
     result.assign(ls, 0.);
     const double* n = fg.normal;
     const double c = S.coeff.at(e);
+    const double* Ct = S.coeff_t.empty() ? nullptr : &S.coeff_t[size_t(e) * d * d];
     const size_t nq = fg.w.size();
     for (size_t qp = 0; qp < nq; ++qp) {
       bin.evaluate(fg.xin[qp].data()), bin.jacobians(fg.xin[qp].data());
@@ -1227,11 +1229,9 @@ struct Operator {
       for (int i = 0; i < ls; ++i) {
         double v;
         if (S.kind == BS_DIRICHLET_COUPLING) {
-          v = -1.0 * S.param * g * kgrad_dot(c, &bin.grads[3 * i], n, d);
+          v = -1.0 * S.param * g * flux_n(c, Ct, &bin.grads[3 * i], n, d);
         } else if (S.kind == BS_BOUNDARY_PENALTY) {
-          double nwn = 0.;
-          for (int a = 0; a < d; ++a)
-            nwn += n[a] * (c * n[a]);
+          const double nwn = n_w_n(c, Ct, n, d);
           const double penalty = (S.param * nwn) / fg.diameter;
           v = penalty * g * bin.vals[i];
         } else {
@@ -1757,6 +1757,16 @@ int or_operator_append_source(OrOperator* op, int kind, const double* f, double 
 }
 // matrix-valued diffusion / weight of the last appended inner (boundary = 0) or boundary (1) intersection form:
 // d x d per element (NULL: keep the scalar)
+// matrix-valued coefficient (d x d per element) of the last appended boundary functional
+int or_operator_set_boundary_source_tensor(OrOperator* op, const double* tensor)
+{
+  return guarded([&] {
+    if (op->bsources.empty() || !tensor)
+      throw Error("no boundary functional appended");
+    const size_t n = size_t(op->space->mesh->ne) * op->space->mesh->ref.dim * op->space->mesh->ref.dim;
+    op->bsources.back().coeff_t.assign(tensor, tensor + n);
+  });
+}
 int or_operator_set_face_tensors(OrOperator* op, int boundary, const double* kappa_t, const double* omega_t)
 {
   return guarded([&] {

@@ -95,6 +95,8 @@ int or_operator_set_dirichlet(OrOperator* op, int enabled);
 /* DirichletConstraints::apply(matrix, vector, only_clear, ensure_symmetry), tools/dirichlet-constraints.hh:152-184 */
 /* matrix-valued diffusion / weight (d x d per element, NULL: scalar) of the last appended inner (boundary 0) or
  * boundary (1) intersection form (laplace-ipdg.hh:47-61, ipdg.hh:59-72) */
+/* matrix-valued coefficient (d x d per element) of the last appended boundary functional */
+int or_operator_set_boundary_source_tensor(OrOperator* op, const double* tensor);
 int or_operator_set_face_tensors(OrOperator* op, int boundary, const double* kappa_t, const double* omega_t);
 int or_operator_set_dirichlet_mode(OrOperator* op, int only_clear, int ensure_symmetry);
 

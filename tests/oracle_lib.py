@@ -84,6 +84,7 @@ def lib():
             [_vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int],
         ),
         "or_operator_set_dirichlet": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "or_operator_set_boundary_source_tensor": (ctypes.c_int, [_vp, _dp]),
         "or_operator_set_face_tensors": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
         "or_operator_set_dirichlet_mode": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "or_operator_assemble": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -367,6 +368,11 @@ class Operator:
 
     def set_dirichlet(self, enabled=True):
         self._check(lib().or_operator_set_dirichlet(self.h, int(enabled)))
+
+    def set_boundary_source_tensor(self, tensor):
+        t = np.ascontiguousarray(tensor, np.float64)
+        self._keep = getattr(self, "_keep", []) + [t]
+        self._check(lib().or_operator_set_boundary_source_tensor(self.h, dptr(t)))
 
     def set_face_tensors(self, boundary, kappa=None, omega=None):
         """matrix-valued diffusion / weight ([ne, d, d]) of the last appended inner (boundary False) or boundary

@@ -108,9 +108,58 @@ def test_face_tensor_refusals(hgdt_mod):
         op.append(H.LocalCouplingIntersectionIntegralBilinearForm(
             H.LocalLaplaceIPDGInnerCoupling(1.0, torch.from_numpy(K).cuda(), 1.0)), {},
             H.ApplyOn.InnerIntersectionsOnce())
+    with pytest.raises(H._lib.HgdtError):  # the boundary product term takes a scalar weight
+        H.LocalIntersectionProductIntegrand([[1.0, 0.0], [0.0, 2.0]])
+
+
+@pytest.mark.parametrize("geometry,geom,cells", [("quad", O.QUAD, [4, 3]), ("tri", O.TRI, [4, 3]),
+                                                 ("hex", O.HEX, [2, 2, 2]), ("tet", O.TET, [2, 2, 1])])
+def test_swipdg_tensor_boundary_functionals_reproduce_linear_solution(hgdt_mod, geometry, geom, cells):
+    """a constant anisotropic K as diffusion and weight of every form and of the boundary functionals (weak
+    Dirichlet data g = u for a linear u, f = 0): rhs within 1e-12 of the oracle's and the solution equal to u at
+    every DoF to 1e-9"""
+    H = hgdt_mod
+    print(f"case SWIPDG tensor boundary functionals {geometry}", flush=True)
+    d = len(cells)
+    Kc = np.array([[2.0, 0.6, -0.3], [0.6, 1.0, 0.2], [-0.3, 0.2, 1.5]])[:d, :d]
+    cvec = np.array([0.7, -1.1, 0.4])[:d]
+
+    def u(X):
+        return 0.3 + sum(cvec[a] * X[a] for a in range(d))
+
+    sigma = 30.0
+    over = 1 if geometry in ("quad", "hex") else 0
+    grid = H.Grid.structured(geometry, [0] * d, [1] * d, cells)
+    space = H.DiscontinuousLagrangeSpace(grid, 1)
+    op = H.make_matrix_operator(space, H.Stencil.element_and_intersection)
+    op += H.LocalElementIntegralBilinearForm(H.LocalLaplaceIntegrand(Kc))
+    op.append(H.LocalCouplingIntersectionIntegralBilinearForm(
+        H.LocalLaplaceIPDGInnerCoupling(1.0, Kc, Kc) + H.LocalIPDGInnerPenalty(sigma, Kc)), {},
+        H.ApplyOn.InnerIntersectionsOnce())
+    op.append(H.LocalIntersectionIntegralBilinearForm(
+        H.LocalIPDGBoundaryPenalty(sigma, Kc) + H.LocalLaplaceIPDGDirichletCoupling(1.0, Kc)), {},
+        H.ApplyOn.BoundaryIntersections())
+    g = H.GenericFunction(1, u)
     rhs = H.make_vector_functional(space)
-    with pytest.raises(H._lib.HgdtError):  # boundary functionals take scalar coefficients
-        rhs.append(H.LocalIntersectionIntegralFunctional(
-            H.LocalLaplaceIPDGDirichletCoupling(1.0, [[1.0, 0.0], [0.0, 2.0]], 1.0)), {},
-            H.ApplyOn.BoundaryIntersections())
-        op.append(rhs)
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalLaplaceIPDGDirichletCoupling(1.0, Kc, g),
+                                                     over_integrate=over), {}, H.ApplyOn.BoundaryIntersections())
+    rhs.append(H.LocalIntersectionIntegralFunctional(H.LocalIPDGBoundaryPenalty(sigma, Kc).with_ansatz(g)), {},
+               H.ApplyOn.BoundaryIntersections())
+    op.append(rhs)
+    op.assemble()
+    m = O.Mesh.structured(geom, cells, [0] * d, [1] * d)
+    Kt = np.ascontiguousarray(np.broadcast_to(Kc, (m.ne, d, d)))
+    s = O.Space(m, False, 1)
+    oop = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    oop.append_boundary_source(0, 1.0, 1.0, u(np.moveaxis(m.boundary_quadrature_points(1 + over), -1, 0)),
+                               g_order=1, over_integrate=over)
+    oop.set_boundary_source_tensor(Kt)
+    oop.append_boundary_source(1, sigma, 1.0, u(np.moveaxis(m.boundary_quadrature_points(2), -1, 0)), g_order=1)
+    oop.set_boundary_source_tensor(Kt)
+    oop.assemble(1)
+    berr = np.abs(op.rhs.cpu().numpy() - oop.rhs()).max() / np.abs(oop.rhs()).max()
+    x = op.apply_inverse("rhs", opts={"precision": 1e-13}).cpu().numpy()
+    nodes = m.coords()[m.conn()]
+    err = np.abs(x[space.global_indices()] - u(np.moveaxis(nodes, -1, 0))).max()
+    print(f"  rhs vs oracle {berr:.2e}, solution vs u {err:.2e}")
+    assert berr <= 1e-12 and err <= 1e-9

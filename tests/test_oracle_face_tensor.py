@@ -76,3 +76,44 @@ def test_spd_with_tensors():
     A = _swipdg(m, _spd(3, m.ne, 2), _spd(4, m.ne, 2), sigma=60.0)
     assert np.abs(A - A.T).max() <= 1e-13 * np.abs(A).max()
     assert np.linalg.eigvalsh(A).min() > 0
+
+
+@pytest.mark.parametrize("geom,cells", [(O.QUAD, [3, 3]), (O.TRI, [3, 3]), (O.HEX, [2, 2, 2]), (O.TET, [2, 2, 1])])
+def test_linear_solution_with_anisotropic_tensor(geom, cells):
+    """SWIPDG with a constant anisotropic K as diffusion and weight, f = 0 and weak Dirichlet data g = u for a
+    linear u (then -div(K grad u) = 0): the discrete solution is u at every DoF (the forms and the boundary
+    functionals with tensors are consistent)"""
+    import scipy.sparse.linalg as spla
+
+    d = len(cells)
+    m = O.Mesh.structured(geom, cells, [0] * d, [1] * d)
+    Kc = np.array([[2.0, 0.6, -0.3], [0.6, 1.0, 0.2], [-0.3, 0.2, 1.5]])[:d, :d]
+    Kt = np.ascontiguousarray(np.broadcast_to(Kc, (m.ne, d, d)))
+    cvec = np.array([0.7, -1.1, 0.4])[:d]
+
+    def u(P):
+        return 0.3 + P @ cvec
+
+    sigma, one = 30.0, np.ones(m.ne)
+    s = O.Space(m, False, 1)
+    op = O.Operator(s, O.STENCIL_ELEMENT_AND_INTERSECTION)
+    op.append_laplace_tensor(Kt)
+    op.append_inner_coupling(1.0, one, one)
+    op.set_face_tensors(False, Kt, Kt)
+    op.append_inner_penalty(sigma, one)
+    op.set_face_tensors(False, None, Kt)
+    op.append_dirichlet_coupling(1.0, one)
+    op.set_face_tensors(True, Kt, None)
+    op.append_boundary_penalty(sigma, one)
+    op.set_face_tensors(True, None, Kt)
+    over = 1 if geom in (O.QUAD, O.HEX) else 0  # (K grad psi) . n g is of degree 2 along a cube's face
+    op.append_boundary_source(0, 1.0, 1.0, u(m.boundary_quadrature_points(1 + over)), g_order=1,
+                              over_integrate=over)
+    op.set_boundary_source_tensor(Kt)
+    op.append_boundary_source(1, sigma, 1.0, u(m.boundary_quadrature_points(2)), g_order=1)
+    op.set_boundary_source_tensor(Kt)
+    op.assemble(1)
+    x = spla.spsolve(op.csr().tocsc(), op.rhs())
+    nodes = m.coords()[m.conn()]
+    err = np.abs(x[s.global_indices()] - u(nodes)).max()
+    assert err <= 1e-10, err
